@@ -1,0 +1,398 @@
+"""Pin the oracle's integrator against closed forms, invariants, independent
+solvers and the reference's own known-answer tests (test/core.jl)."""
+import math
+
+import numpy as np
+import pytest
+from scipy.integrate import solve_ivp
+from scipy.linalg import expm
+
+LORENZ_P = [10.0, 28.0, 8.0 / 3.0]
+ROBER_P = [0.04, 3.0e7, 1.0e4]
+
+
+def lorenz_f(t, u, s, r, b):
+    return [s * (u[1] - u[0]), u[0] * (r - u[2]) - u[1], u[0] * u[1] - b * u[2]]
+
+
+def rober_f(t, y, k1, k2, k3):
+    return [-k1 * y[0] + k3 * y[1] * y[2], k1 * y[0] - k2 * y[1] ** 2 - k3 * y[1] * y[2],
+            k2 * y[1] ** 2]
+
+
+def rober_j(t, y, k1, k2, k3):
+    return [[-k1, k3 * y[2], k3 * y[1]], [k1, -2 * k2 * y[1] - k3 * y[2], -k3 * y[1]],
+            [0, 2 * k2 * y[1], 0]]
+
+
+# ----------------------------------------------------------- core.jl known answers
+def test_core_basic_ode_solve(oracle):
+    """test/core.jl:9-19: u'=1.01u, u0=0.5 on (0,1), Tsit5 at 1e-8."""
+    r = oracle.solve("exp", [0.5], [1.01], (0.0, 1.0), abstol=1e-8, reltol=1e-8)
+    assert r["retcode"] == 1
+    exact = 0.5 * math.exp(1.01)
+    assert abs(r["u"][0] - exact) / exact < 2e-8
+    assert r["t"] == 1.0
+
+
+def test_core_saveat_eleven_points(oracle):
+    """test/core.jl:90-96: saveat=0.1 on (0,1) gives 11 points, both ends."""
+    sa = np.array([i / 10 for i in range(11)])
+    r = oracle.solve("exp", [0.5], [1.01], (0.0, 1.0), saveat=sa)
+    s = r["saveat"][:, 0]
+    assert len(s) == 11 and np.all(np.isfinite(s))
+    assert s[0] == 0.5  # exact copy of u0 (core.jl:34 asserts ==)
+    assert s[-1] == r["u"][0]
+    exact = 0.5 * np.exp(1.01 * sa)
+    # default tolerances abstol=1e-6, reltol=1e-3
+    assert np.max(np.abs(s - exact) / exact) < 2e-3
+
+
+def test_core_inplace_lorenz_and_dense(oracle):
+    """test/core.jl:21-36 and 51-58."""
+    r = oracle.solve("lorenz", [1.0, 0.0, 0.0], LORENZ_P, (0.0, 1.0),
+                     saveat=np.array([0.0, 0.5, 1.0]))
+    assert r["retcode"] == 1 and len(r["u"]) == 3
+    np.testing.assert_array_equal(r["saveat"][0], [1.0, 0.0, 0.0])
+    r = oracle.solve("exp", [0.5], [1.01], (0.0, 1.0), saveat=np.array([0.5]))
+    assert r["saveat"][0, 0] > 0.5
+
+
+def test_core_robertson_success(oracle):
+    """test/core.jl:38-49 (Rodas5 stands in for Rodas5P)."""
+    r = oracle.solve("robertson", [1.0, 0.0, 0.0], ROBER_P, (0.0, 1e5), alg="rodas5")
+    assert r["retcode"] == 1
+
+
+# ----------------------------------------------------------- initdt
+def test_initdt_hand_derived(oracle):
+    """Hairer-Norsett-Wanner II.4 for u'=1.01u, u0=0.5, tol 1e-8, order 5."""
+    u0, lam, tol = 0.5, 1.01, 1e-8
+    sk = tol + u0 * tol
+    d0 = u0 / sk
+    f0 = lam * u0
+    d1 = f0 / sk
+    dt0 = (d0 / d1) / 100
+    u1 = u0 + dt0 * f0
+    d2 = abs(lam * u1 - f0) / sk / dt0
+    dt1 = 10 ** (-(2 + math.log10(max(d1, d2))) / 5)
+    expect = min(100 * dt0, dt1, 1.0)
+    got = oracle.initdt("exp", [u0], [lam], (0.0, 1.0), abstol=tol, reltol=tol)
+    assert got == pytest.approx(expect, rel=1e-10)
+    r = oracle.solve("exp", [u0], [lam], (0.0, 1.0), abstol=tol, reltol=tol)
+    assert r["dt0"] == got
+
+
+def test_initdt_small_norm_branch(oracle):
+    """d0 < 1e-5 (u0 = 0) takes the 1e-6 branch, then the f1==f0 early return."""
+    got = oracle.initdt("exp", [0.0], [1.0], (0.0, 1.0), abstol=1e-8, reltol=1e-8)
+    assert got == pytest.approx(100 * 1e-6)
+
+
+# ----------------------------------------------------------- controller
+def _pi_expect(E, qold, dt, qsteady_max):
+    b1, b2, gam, qmin, qmax = 7 / 50, 2 / 25, 0.9, 0.2, 10.0
+    if E == 0:
+        q, q11 = 1 / qmax, None
+    else:
+        q11 = E ** b1
+        q = max(1 / qmax, min(1 / qmin, q11 / qold ** b2 / gam))
+    if E <= 1:
+        if 1 <= q <= qsteady_max:
+            q = 1.0
+        return True, dt / q, max(E, 1e-4)
+    return False, dt / min(1 / qmin, q11 / gam), qold
+
+
+@pytest.mark.parametrize("E,qold,qs", [(0.0, 1e-4, 1.0), (0.3, 1e-4, 1.0), (0.3, 0.5, 1.0),
+                                       (0.99, 0.9, 1.2), (1.0, 0.2, 1.0), (1.5, 0.3, 1.0),
+                                       (1e6, 0.3, 1.0), (0.5, 0.5, 1.2), (2e-9, 1e-4, 1.0)])
+def test_pi_controller(oracle, E, qold, qs):
+    acc, dtn, qn = oracle.controller(E, qold, 0.1, qs)
+    eacc, edtn, eqn = _pi_expect(E, qold, 0.1, qs)
+    assert acc == eacc
+    assert dtn == pytest.approx(edtn, rel=1e-10)
+    assert qn == eqn
+
+
+def test_ctl_pow_accuracy(oracle):
+    rng = np.random.default_rng(1)
+    for x in np.concatenate([10 ** rng.uniform(-12, 8, 400), [1.0, 0.5, 2.0, 1e-4]]):
+        for y in (0.14, 0.08, -0.2, 1 / 5):
+            assert oracle.ctl_pow(x, y) == pytest.approx(x ** y, rel=5e-11)
+    assert oracle.ctl_exp2(3.0) == 8.0 and oracle.ctl_log2(1024.0) == 10.0
+
+
+# ----------------------------------------------------------- Tsit5 step vs linear theory
+def test_tsit5_step_linear_stability_polynomial(oracle):
+    """On u'=lam u one step is u1 = R(z) u0 and the embedded error is
+    u0 (Rhat(z) - R(z)); R, Rhat from the tableau by linear algebra."""
+    c, A, bt, R_ = oracle.tsit5_tableau()
+    b = A[6]
+    bhat = b + bt
+    one = np.ones(7)
+    for lam, dt in ((1.01, 0.1), (-3.0, 0.2), (0.5, 1.0)):
+        z = lam * dt
+        M = np.linalg.solve(np.eye(7) - z * A, one)
+        Rz = 1 + z * b @ M
+        Rh = 1 + z * bhat @ M
+        u0 = 0.7
+        u1, E, k = oracle.tsit5_step("exp", [u0], [lam], 0.0, dt, 1e-6, 1e-3)
+        assert u1[0] == pytest.approx(Rz * u0, rel=1e-14)
+        sc = 1e-6 + max(abs(u0), abs(u1[0])) * 1e-3
+        assert E == pytest.approx(abs(u0 * (Rh - Rz)) / sc, rel=1e-9)
+
+
+def test_tsit5_convergence_order(oracle):
+    """Fixed-step global error on a linear 3x3 system vs expm: ratio 2^5."""
+    M = np.array([[-0.5, 1.0, 0.2], [-1.0, -0.3, 0.0], [0.1, 0.4, -0.8]])
+    u0 = np.array([1.0, -0.5, 2.0])
+    T = 2.0
+    exact = expm(M * T) @ u0
+
+    def run(N):
+        u = u0.copy()
+        for i in range(N):
+            u, _, _ = oracle.tsit5_step("linear", u, M.reshape(-1), i * T / N, T / N, 1, 1)
+        return np.linalg.norm(u - exact)
+
+    e = [run(N) for N in (20, 40, 80)]
+    rates = [math.log2(e[i] / e[i + 1]) for i in range(2)]
+    assert all(4.6 < r < 5.5 for r in rates), rates
+
+
+def test_tsit5_interpolant_accuracy(oracle):
+    """Dense output on u'=lam u: theta=0 and 1 reproduce the endpoints, local
+    error of the 4th-order interpolant scales like dt^5."""
+    lam, u0 = -1.3, 1.0
+    errs = []
+    for dt in (0.2, 0.1, 0.05):
+        u1, _, k = oracle.tsit5_step("exp", [u0], [lam], 0.0, dt, 1, 1)
+        assert oracle.tsit5_interp([u0], k, dt, 0.0)[0] == u0
+        assert oracle.tsit5_interp([u0], k, dt, 1.0)[0] == pytest.approx(u1[0], abs=1e-15)
+        th = 0.4
+        errs.append(abs(oracle.tsit5_interp([u0], k, dt, th)[0] - u0 * math.exp(lam * th * dt)))
+    assert 4.5 < math.log2(errs[0] / errs[1]) < 5.6
+    assert 4.5 < math.log2(errs[1] / errs[2]) < 5.6
+
+
+# ----------------------------------------------------------- Rodas5
+def test_rodas5_convergence_order(oracle):
+    M = np.array([[-0.5, 1.0, 0.2], [-1.0, -0.3, 0.0], [0.1, 0.4, -0.8]])
+    u0 = np.array([1.0, -0.5, 2.0])
+    T = 2.0
+    exact = expm(M * T) @ u0
+
+    def run(N):
+        u = u0.copy()
+        for i in range(N):
+            rc, u, _ = oracle.rodas5_step("linear", u, M.reshape(-1), i * T / N, T / N, 1, 1)
+            assert rc == 0
+        return np.linalg.norm(u - exact)
+
+    e = [run(N) for N in (8, 16, 32)]
+    rates = [math.log2(e[i] / e[i + 1]) for i in range(2)]
+    assert all(4.6 < r < 5.6 for r in rates), rates
+
+
+def test_rodas5_nonlinear_order(oracle):
+    """Nonlinear (Lorenz, short horizon): fixed-step order 5 against a tight
+    DOP853 solution; exercises the multi-child (alpha) order conditions."""
+    u0 = np.array([1.0, 1.0, 1.0])
+    T = 0.2
+    ref = solve_ivp(lorenz_f, (0, T), u0, method="DOP853", rtol=1e-14, atol=1e-14,
+                    args=tuple(LORENZ_P)).y[:, -1]
+
+    def run(N):
+        u = u0.copy()
+        for i in range(N):
+            rc, u, _ = oracle.rodas5_step("lorenz", u, LORENZ_P, i * T / N, T / N, 1, 1)
+        return np.linalg.norm(u - ref)
+
+    e = [run(N) for N in (10, 20, 40)]
+    rates = [math.log2(e[i] / e[i + 1]) for i in range(2)]
+    assert all(4.5 < r < 5.8 for r in rates), rates
+
+
+def test_rodas5_prothero_robinson(oracle):
+    """y' = lam (y - sin t) + cos t, y(0)=0, exact sin t; stiff lam, large
+    steps: a stiffly accurate L-stable method stays accurate (the
+    non-autonomous d_i*dt*dT term and c_i are exercised here)."""
+    for lam in (-1e2, -1e6):
+        r = oracle.solve("prothero", [0.0], [lam], (0.0, 2.0), alg="rodas5", abstol=1e-9,
+                         reltol=1e-9)
+        assert r["retcode"] == 1
+        assert abs(r["u"][0] - math.sin(2.0)) < 5e-8, (lam, r)
+
+
+def test_rodas5_singular_W_rejected(oracle):
+    # W = J - I/(gamma dt) singular when lam = 1/(gamma dt): EEst := 2
+    g = 0.19
+    dt = 0.5
+    rc, u1, E = oracle.rodas5_step("exp", [1.0], [1.0 / (g * dt)], 0.0, dt, 1e-6, 1e-3)
+    assert rc == -1 and E == 2.0
+
+
+def test_robertson_invariant_and_radau(oracle):
+    """Sum y = 1 is preserved (linear invariant); final state vs Radau."""
+    r = oracle.solve("robertson", [1.0, 0.0, 0.0], ROBER_P, (0.0, 1e5), alg="rodas5",
+                     abstol=1e-8, reltol=1e-6)
+    assert r["retcode"] == 1
+    assert abs(r["u"].sum() - 1.0) < 1e-13
+    ref = solve_ivp(rober_f, (0, 1e5), [1, 0, 0], method="Radau", rtol=1e-12, atol=1e-14,
+                    jac=rober_j, args=tuple(ROBER_P)).y[:, -1]
+    w = np.abs(r["u"] - ref) / (1e-8 + 1e-6 * np.abs(ref))
+    assert w.max() < 10.0, (r["u"], ref)
+    # Rodas5 at rtol 1e-6 takes O(100) steps (Radau 166, BDF 288 in a scipy probe)
+    assert 50 < r["naccept"] < 400
+
+
+# ----------------------------------------------------------- independent solvers
+def test_lorenz_vs_dop853(oracle):
+    ref = solve_ivp(lorenz_f, (0, 10), [1, 0, 0], method="DOP853", rtol=1e-13, atol=1e-13,
+                    args=tuple(LORENZ_P)).y[:, -1]
+    r = oracle.solve("lorenz", [1, 0, 0], LORENZ_P, (0.0, 10.0), abstol=1e-12, reltol=1e-12)
+    assert np.linalg.norm(r["u"] - ref) / np.linalg.norm(ref) < 1e-9
+    r8 = oracle.solve("lorenz", [1, 0, 0], LORENZ_P, (0.0, 10.0), abstol=1e-8, reltol=1e-8)
+    err8 = np.linalg.norm(r8["u"] - ref) / np.linalg.norm(ref)
+    assert 1e-10 < err8 < 1e-5
+    assert 600 < r8["naccept"] < 1100
+
+
+def test_vdp_classic_vs_radau(oracle):
+    mu = 10.0
+
+    def f(t, u):
+        return [u[1], mu * (1 - u[0] ** 2) * u[1] - u[0]]
+
+    ref = solve_ivp(f, (0, 2 * mu), [2, 0], method="Radau", rtol=1e-12, atol=1e-12).y[:, -1]
+    for alg in ("tsit5", "rodas5", "auto"):
+        r = oracle.solve("vdp", [2.0, 0.0], [mu], (0.0, 2 * mu), alg=alg, abstol=1e-10,
+                         reltol=1e-10)
+        assert r["retcode"] == 1
+        assert np.abs(r["u"] - ref).max() < 1e-5 * (1 + np.abs(ref).max()), (alg, r["u"], ref)
+
+
+def test_brusselator_vs_bdf(oracle):
+    N = 8
+    A, B, alpha = 1.0, 3.0, 0.02
+    x = np.arange(N) / N
+    u0 = np.empty(2 * N)
+    u0[0::2] = 1 + np.sin(2 * np.pi * x)
+    u0[1::2] = 3.0
+    p = [A, B, alpha]
+
+    def f(t, y):
+        return oracle.rhs("brusselator", y, p)
+
+    def jac(t, y):
+        return oracle.jac("brusselator", y, p)
+
+    # Jacobian matches finite differences of the RHS
+    J = jac(0, u0)
+    eps = 1e-7
+    Jfd = np.stack([(f(0, u0 + eps * e) - f(0, u0 - eps * e)) / (2 * eps)
+                    for e in np.eye(2 * N)], axis=1)
+    np.testing.assert_allclose(J, Jfd, atol=1e-6)
+    ref = solve_ivp(f, (0, 10), u0, method="BDF", jac=jac, rtol=1e-11, atol=1e-11).y[:, -1]
+    for alg in ("tsit5", "rodas5"):
+        r = oracle.solve("brusselator", u0, p, (0.0, 10.0), alg=alg, abstol=1e-9, reltol=1e-9)
+        assert r["retcode"] == 1
+        assert np.abs(r["u"] - ref).max() < 1e-6, alg
+
+
+def test_model_jacobians_fd(oracle):
+    rng = np.random.default_rng(3)
+    cases = [("lorenz", 3, LORENZ_P), ("robertson", 3, [0.04, 3e7, 1e4]), ("vdp", 2, [7.0]),
+             ("exp", 1, [1.5])]
+    for model, n, p in cases:
+        u = rng.uniform(0.1, 1.0, n) * (1e-3 if model == "robertson" else 1)
+        J = oracle.jac(model, u, p)
+        h = 1e-6 * np.maximum(1e-6, np.abs(u))
+        Jfd = np.stack([(oracle.rhs(model, u + h[i] * e, p) - oracle.rhs(model, u - h[i] * e, p))
+                        / (2 * h[i]) for i, e in enumerate(np.eye(n))], axis=1)
+        np.testing.assert_allclose(J, Jfd, rtol=1e-5, atol=1e-6 * np.abs(J).max())
+
+
+# ----------------------------------------------------------- LU
+@pytest.mark.parametrize("n", [1, 2, 3, 7, 16, 64, 128])
+def test_lu_vs_numpy(oracle, n):
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, n))
+    b = rng.standard_normal(n)
+    rc, x = oracle.lu_solve(A, b)
+    assert rc == 0
+    np.testing.assert_allclose(x, np.linalg.solve(A, b), rtol=1e-9, atol=1e-9)
+
+
+def test_lu_pivoting_and_singular(oracle):
+    A = np.array([[0.0, 1.0], [1.0, 0.0]])  # needs a row swap
+    rc, x = oracle.lu_solve(A, np.array([2.0, 3.0]))
+    assert rc == 0 and list(x) == [3.0, 2.0]
+    rc, _ = oracle.lu_solve(np.array([[1.0, 2.0], [2.0, 4.0]]), np.array([1.0, 1.0]))
+    assert rc == -1
+
+
+# ----------------------------------------------------------- tstops, retcodes
+@pytest.mark.parametrize("tf", [0.3, 1.0 / 3.0, 10.0, 7.77])
+def test_final_time_is_exactly_tf(oracle, tf):
+    r = oracle.solve("lorenz", [1, 0, 0], LORENZ_P, (0.0, tf), abstol=1e-6, reltol=1e-6)
+    assert r["t"] == tf and r["retcode"] == 1
+
+
+def test_retcodes(oracle):
+    r = oracle.solve("lorenz", [1, 0, 0], LORENZ_P, (0.0, 10.0), maxiters=5)
+    assert r["retcode"] == 2 and r["naccept"] + r["nreject"] == 5
+    r = oracle.solve("lorenz", [1, 0, 0], LORENZ_P, (0.0, 10.0), dtmin=0.5)
+    assert r["retcode"] == 3
+    r = oracle.solve("lorenz", [1, 0, 0], [float("nan"), 28, 8 / 3], (0.0, 1.0))
+    assert r["retcode"] in (4, 5)
+    r = oracle.solve("exp", [1.0], [1.0], (0.0, 0.0))
+    assert r["retcode"] == 1 and r["naccept"] == 0
+
+
+# ----------------------------------------------------------- ensemble
+def test_ensemble_threads_matches_serial(oracle):
+    rng = np.random.default_rng(7)
+    N = 64
+    p = np.array(LORENZ_P)[:, None] * (1 + rng.uniform(-0.1, 0.1, (3, N)))
+    u0 = np.zeros((3, N))
+    u0[0] = 1.0
+    sa = np.linspace(0, 2, 7)
+    a = oracle.ensemble("lorenz", u0, p, 0.0, 2.0, nthreads=1, abstol=1e-8, reltol=1e-8,
+                        saveat=sa)
+    b = oracle.ensemble("lorenz", u0, p, 0.0, 2.0, nthreads=4, abstol=1e-8, reltol=1e-8,
+                        saveat=sa)
+    for k in ("u", "saveat", "naccept", "nreject", "retcode"):
+        np.testing.assert_array_equal(a[k], b[k])
+    for i in (0, 17, 63):
+        r = oracle.solve("lorenz", u0[:, i], p[:, i], (0.0, 2.0), abstol=1e-8, reltol=1e-8,
+                         saveat=sa)
+        np.testing.assert_array_equal(r["u"], a["u"][:, i])
+        np.testing.assert_array_equal(r["saveat"], a["saveat"][:, :, i])
+
+
+# ----------------------------------------------------------- mixed precision
+def test_mixed_precision_close_to_fp64(oracle):
+    r64 = oracle.solve("lorenz", [1, 0, 0], LORENZ_P, (0.0, 10.0), abstol=1e-6, reltol=1e-6)
+    r32 = oracle.solve("lorenz", [1, 0, 0], LORENZ_P, (0.0, 10.0), abstol=1e-6, reltol=1e-6,
+                       precision="mixed")
+    assert r32["retcode"] == 1 and r32["t"] == 10.0
+    rel = np.linalg.norm(r32["u"] - r64["u"]) / np.linalg.norm(r64["u"])
+    assert rel < 1e-3
+    rr = oracle.solve("robertson", [1, 0, 0], ROBER_P, (0.0, 1e5), alg="rodas5", abstol=1e-8,
+                      reltol=1e-6, precision="mixed")
+    # FP32 RHS rounding does not conserve sum y exactly; drift stays ~reltol
+    assert rr["retcode"] == 1 and abs(rr["u"].sum() - 1) < 1e-5
+
+
+# ----------------------------------------------------------- auto-switching
+def test_autoswitch(oracle):
+    r = oracle.solve("lorenz", [1, 0, 0], LORENZ_P, (0.0, 10.0), alg="auto", abstol=1e-8,
+                     reltol=1e-8)
+    assert r["nswitch"] == 0 and r["nstiff"] == 0
+    r = oracle.solve("robertson", [1, 0, 0], ROBER_P, (0.0, 1e5), alg="auto", abstol=1e-8,
+                     reltol=1e-6)
+    assert r["retcode"] == 1 and r["nswitch"] >= 1 and r["first_stiff_iter"] > 10
+    assert r["nstiff"] > 0.5 * r["naccept"]
+    assert abs(r["u"].sum() - 1) < 1e-12
