@@ -1,0 +1,138 @@
+// kernels_lane.hip -- ensemble kernels for small systems (n <= 3): one lane
+// per trajectory, state in VGPRs, structure-of-arrays I/O in HBM.
+//
+// Two schedules:
+//   lane        one thread per trajectory; the hardware dispatcher refills
+//               CU slots wave by wave as waves retire.
+//   persistent  a fixed grid; each lane that finishes (detected with a
+//               wavefront ballot) takes the next trajectory from a global
+//               atomic counter, one atomic per wave per refill, so lanes
+//               stay busy while the queue lasts.
+#include "lane_solver.hpp"
+#include "launch.hpp"
+
+namespace mideq {
+
+template <class M, bool MIX, int ALG>
+__global__ __launch_bounds__(256) void k_lane(LaneArgs a) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.ntraj) return;
+  Lane<M, MIX, ALG> L;
+  L.init(a, i);
+  while (L.rc == RC_DEFAULT) L.step(a);
+  L.finish(a);
+}
+
+template <class M, bool MIX, int ALG>
+__global__ __launch_bounds__(256) void k_persistent(LaneArgs a) {
+  const int lane = threadIdx.x & 63;
+  const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  Lane<M, MIX, ALG> L;
+  bool active = false;
+  bool drained = false;
+  while (true) {
+    if (!drained) {
+      unsigned long long need = __ballot(!active);
+      if (need) {
+        int cnt = __popcll(need);
+        int leader = __builtin_ctzll(need);
+        unsigned long long base = 0;
+        if (lane == leader) base = atomicAdd(a.queue, (unsigned long long)cnt);
+        base = __shfl(base, leader);
+        if (base + (unsigned long long)cnt >= (unsigned long long)a.ntraj) drained = true;
+        if (!active) {
+          unsigned long long my = base + (unsigned long long)__popcll(need & lt_mask);
+          if (my < (unsigned long long)a.ntraj) {
+            L.init(a, (int64_t)my);
+            if (L.rc == RC_DEFAULT) {
+              active = true;
+            } else {
+              L.finish(a);
+            }
+          }
+        }
+      }
+    }
+    if (!__any(active)) {
+      if (drained) break;
+      continue;
+    }
+    if (active) {
+      L.step(a);
+      if (L.rc != RC_DEFAULT) {
+        L.finish(a);
+        active = false;
+      }
+    }
+  }
+}
+
+__global__ void k_set_u64(unsigned long long* p, unsigned long long v) { *p = v; }
+
+template <class M, bool MIX, int ALG>
+static hipError_t launch_t(const LaneArgs& a, int sched, int block, int grid,
+                           hipStream_t s) {
+  if (sched == SCHED_PERSISTENT) {
+    // grid <= 0: as many blocks as can be resident at once (a truly
+    // persistent grid), from the occupancy of this instantiation.
+    if (grid <= 0) {
+      int per_cu = 0;
+      hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &per_cu, k_persistent<M, MIX, ALG>, block, 0);
+      if (e != hipSuccess || per_cu < 1) per_cu = 1;
+      int dev = 0, ncu = 256;
+      if (hipGetDevice(&dev) == hipSuccess)
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+      int64_t need = (a.ntraj + block - 1) / block;
+      int64_t g = (int64_t)per_cu * ncu;
+      grid = (int)(g < need ? g : need);
+    }
+    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, s, a.queue, 0ull);
+    hipLaunchKernelGGL((k_persistent<M, MIX, ALG>), dim3(grid), dim3(block), 0, s, a);
+  } else {
+    hipLaunchKernelGGL((k_lane<M, MIX, ALG>), dim3(grid), dim3(block), 0, s, a);
+  }
+  return hipGetLastError();
+}
+
+template <class M, bool MIX>
+static hipError_t launch_alg(const LaneArgs& a, int alg, int sched, int block, int grid,
+                             hipStream_t s) {
+  switch (alg) {
+    case ALG_TSIT5: return launch_t<M, MIX, ALG_TSIT5>(a, sched, block, grid, s);
+    case ALG_RODAS5: return launch_t<M, MIX, ALG_RODAS5>(a, sched, block, grid, s);
+    default: return launch_t<M, MIX, ALG_AUTO>(a, sched, block, grid, s);
+  }
+}
+
+template <class M>
+static hipError_t launch_model(const LaneArgs& a, int alg, int mix, int sched, int block,
+                               int grid, hipStream_t s) {
+  return mix ? launch_alg<M, true>(a, alg, sched, block, grid, s)
+             : launch_alg<M, false>(a, alg, sched, block, grid, s);
+}
+
+hipError_t launch_lane_solver(const LaneArgs& a, int model, int alg, int mix, int sched,
+                              int block, int grid, hipStream_t s) {
+  switch (model) {
+    case MODEL_EXP: return launch_model<ModelExp>(a, alg, mix, sched, block, grid, s);
+    case MODEL_LORENZ: return launch_model<ModelLorenz>(a, alg, mix, sched, block, grid, s);
+    case MODEL_ROBERTSON: return launch_model<ModelRobertson>(a, alg, mix, sched, block, grid, s);
+    case MODEL_VDP: return launch_model<ModelVdP>(a, alg, mix, sched, block, grid, s);
+    case MODEL_LINEAR: return launch_model<ModelLinear3>(a, alg, mix, sched, block, grid, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+int lane_model_dims(int model, int* n, int* np) {
+  switch (model) {
+    case MODEL_EXP: *n = ModelExp::N; *np = ModelExp::NP; return 0;
+    case MODEL_LORENZ: *n = ModelLorenz::N; *np = ModelLorenz::NP; return 0;
+    case MODEL_ROBERTSON: *n = ModelRobertson::N; *np = ModelRobertson::NP; return 0;
+    case MODEL_VDP: *n = ModelVdP::N; *np = ModelVdP::NP; return 0;
+    case MODEL_LINEAR: *n = ModelLinear3::N; *np = ModelLinear3::NP; return 0;
+    default: return -1;
+  }
+}
+
+}  // namespace mideq

@@ -1,0 +1,730 @@
+// lane_solver.hpp -- one trajectory per lane: the integrator loop of
+// OrdinaryDiffEq (loopheader!/perform_step!/loopfooter!) for Tsit5, Rodas5
+// and AutoTsit5(Rodas5()), held entirely in VGPRs.
+//
+// Reference behaviour restated here: Tsit5 / Rodas5 perform_step, the PI
+// controller, ode_determine_initdt, tstop clamping at tf and savevalues!
+// for a vector saveat (OrdinaryDiffEq 7, re-exported by
+// src/DifferentialEquations.jl:8-9; surface names test/qa/qa.jl:15,97,119,
+// 135,185; saveat semantics test/core.jl:90-96, u0 copy core.jl:34).
+#pragma once
+
+#include "device_common.hpp"
+
+namespace mideq {
+
+enum { ALG_TSIT5 = 0, ALG_RODAS5 = 1, ALG_AUTO = 2 };
+
+struct LaneArgs {
+  int64_t ntraj;
+  const double* u0;
+  const double* p;
+  const double* t0;
+  const double* tf;
+  double tspan0, tspan1;
+  double abstol, reltol;
+  double dt_fixed, dtmin, dtmax;
+  int64_t maxiters;
+  const double* saveat;
+  int32_t ns;
+  int32_t save_f32;
+  const double* dt0;
+  const int8_t* alg_hint;
+  int32_t qsteady_default;
+  int32_t pad0;
+  void* us;
+  double* u_final;
+  double* t_final;
+  int8_t* retcode;
+  int32_t* naccept;
+  int32_t* nreject;
+  int32_t* nf;
+  int32_t* nstiff;
+  unsigned long long* queue;  // persistent schedule: next trajectory index
+};
+
+template <int N>
+__device__ __forceinline__ double rms_sum_small(const double* r) {
+  double s = r[0] * r[0];
+#pragma unroll
+  for (int i = 1; i < N; i++) s = s + r[i] * r[i];
+  return s;
+}
+
+// calculate_residuals + ODE_DEFAULT_NORM (RMS).
+template <int N>
+__device__ __forceinline__ double err_norm(const double* ut, const double* u0,
+                                           const double* u1, double abstol,
+                                           double reltol) {
+  double r[N];
+#pragma unroll
+  for (int i = 0; i < N; i++) {
+    double sc = fma(fmax(fabs(u0[i]), fabs(u1[i])), reltol, abstol);
+    r[i] = ut[i] / sc;
+  }
+  return sqrt(rms_sum_small<N>(r) / (double)N);
+}
+
+template <int N>
+__device__ __forceinline__ double rms(const double* x) {
+  return sqrt(rms_sum_small<N>(x) / (double)N);
+}
+
+// Tsit5 dense-output weights b_i(theta).
+__device__ __forceinline__ void tsit5_bthetas(double th, double* b) {
+  using namespace ts;
+  double th2 = th * th;
+  b[0] = th * fma(th, fma(th, fma(th, r14, r13), r12), r11);
+  b[1] = th2 * fma(th, fma(th, r24, r23), r22);
+  b[2] = th2 * fma(th, fma(th, r34, r33), r32);
+  b[3] = th2 * fma(th, fma(th, r44, r43), r42);
+  b[4] = th2 * fma(th, fma(th, r54, r53), r52);
+  b[5] = th2 * fma(th, fma(th, r64, r63), r62);
+  b[6] = th2 * fma(th, fma(th, r74, r73), r72);
+}
+
+template <class M, bool MIX, int ALG>
+struct Lane {
+  static constexpr int N = M::N;
+  static constexpr int NP = M::NP;
+  static constexpr bool kAuto = (ALG == ALG_AUTO);
+  static constexpr bool kHasTsit5 = (ALG != ALG_RODAS5);
+  static constexpr bool kHasRodas = (ALG != ALG_TSIT5);
+
+  int64_t i;
+  double p[NP];
+  float pf[NP];
+  double u[N];
+  double k1[N];   // FSAL stage (FP64 form)
+  float k1f[N];   // FSAL stage (FP32 form)
+  double t, tcomp, tf, dt, dtmax;
+  double qold, qoldpow, qsteady_max, eig;
+  int64_t iter;
+  int32_t naccept, nreject, nf, nstiff;
+  int32_t isave, count, nanrej;
+  int8_t rc, mode;
+
+  // ------------------------------------------------------------ helpers
+  __device__ __forceinline__ void store_save(const LaneArgs& a, int k, const double* v) {
+    const int64_t NT = a.ntraj;
+    if (a.save_f32) {
+      float* out = (float*)a.us;
+#pragma unroll
+      for (int c = 0; c < N; c++) out[((int64_t)k * N + c) * NT + i] = (float)v[c];
+    } else {
+      double* out = (double*)a.us;
+#pragma unroll
+      for (int c = 0; c < N; c++) out[((int64_t)k * N + c) * NT + i] = v[c];
+    }
+  }
+
+  __device__ __forceinline__ void fsal_init() {
+    M::f(k1, u, p);
+    if (MIX) {
+      float uf[N];
+#pragma unroll
+      for (int c = 0; c < N; c++) uf[c] = (float)u[c];
+      M::f(k1f, uf, pf);
+    }
+  }
+
+  // ode_determine_initdt (Hairer-Norsett-Wanner II.4 with upstream edits).
+  __device__ double initdt(const LaneArgs& a, double t0) {
+    double dtmin_e = nextafter(fmax(a.dtmin, jl_eps(t0)), __builtin_inf());
+    double smalldt = fmax(dtmin_e, 1e-6);
+    double sk[N], tmp[N];
+#pragma unroll
+    for (int c = 0; c < N; c++) sk[c] = fma(fabs(u[c]), a.reltol, a.abstol);
+#pragma unroll
+    for (int c = 0; c < N; c++) tmp[c] = u[c] / sk[c];
+    double d0 = rms<N>(tmp);
+#pragma unroll
+    for (int c = 0; c < N; c++) tmp[c] = k1[c] / sk[c];
+    double d1 = rms<N>(tmp);
+    if (d1 != d1) return dtmin_e;
+    double dt0 = (d0 < 1e-5 || d1 < 1e-5) ? smalldt : (d0 / d1) / 100.0;
+    dt0 = fmin(dt0, dtmax);
+    if (dt0 < 10.0 * 2.220446049250313e-16) return smalldt;
+    double u1[N], f1[N];
+#pragma unroll
+    for (int c = 0; c < N; c++) u1[c] = fma(dt0, k1[c], u[c]);
+    M::f(f1, u1, p);
+    nf += 1;
+    bool same = true;
+#pragma unroll
+    for (int c = 0; c < N; c++) same = same && (k1[c] == f1[c]);
+    if (same) return fmax(dtmin_e, 100.0 * dt0);
+#pragma unroll
+    for (int c = 0; c < N; c++) tmp[c] = (f1[c] - k1[c]) / sk[c];
+    double d2 = rms<N>(tmp) / dt0;
+    double mx = fmax(d1, d2);
+    double dt1;
+    if (mx <= 1e-15)
+      dt1 = fmax(1e-6, dt0 * 1e-3);
+    else
+      dt1 = ctl_exp2(-(2.0 * 3.32192809488736234787 + ctl_log2(mx)) / 5.0);
+    return fmax(dtmin_e, fmin(fmin(100.0 * dt0, dt1), dtmax));
+  }
+
+  // ------------------------------------------------------------ init
+  __device__ void init(const LaneArgs& a, int64_t idx) {
+    i = idx;
+    const int64_t NT = a.ntraj;
+#pragma unroll
+    for (int j = 0; j < NP; j++) {
+      p[j] = a.p[(int64_t)j * NT + i];
+      pf[j] = (float)p[j];
+    }
+#pragma unroll
+    for (int c = 0; c < N; c++) u[c] = a.u0[(int64_t)c * NT + i];
+    double t0 = a.t0 ? a.t0[i] : a.tspan0;
+    tf = a.tf ? a.tf[i] : a.tspan1;
+    t = t0;
+    tcomp = 0.0;
+    dtmax = a.dtmax > 0 ? a.dtmax : fabs(tf - t0);
+    naccept = nreject = nstiff = 0;
+    nf = 1;
+    iter = 0;
+    count = 0;
+    nanrej = 0;
+    eig = 0.0;
+    rc = RC_DEFAULT;
+    mode = (ALG == ALG_RODAS5) ? 1 : 0;
+    if (kAuto && a.alg_hint) mode = a.alg_hint[i] ? 1 : 0;
+    qsteady_max = (a.qsteady_default && mode == 1) ? 1.2 : 1.0;
+    // saveat points before t0 are unreachable; a point at t0 is u0 itself
+    isave = 0;
+    if (a.us) {
+      double nanv[N];
+#pragma unroll
+      for (int c = 0; c < N; c++) nanv[c] = __builtin_nan("");
+      while (isave < a.ns && a.saveat[isave] < t0) store_save(a, isave++, nanv);
+      while (isave < a.ns && a.saveat[isave] == t0) store_save(a, isave++, u);
+    } else {
+      while (isave < a.ns && a.saveat[isave] <= t0) isave++;
+    }
+    fsal_init();
+    if (a.dt_fixed > 0)
+      dt = a.dt_fixed;
+    else if (a.dt0)
+      dt = a.dt0[i];
+    else
+      dt = initdt(a, t0);
+    qold = kQoldInit;
+    qoldpow = ctl_pow(kQoldInit, kBeta2);
+    if (!(tf > t0)) rc = RC_SUCCESS;
+  }
+
+  // ------------------------------------------------------------ Tsit5
+  __device__ double tsit5_step64(double dts, double k[7][N], double* u1, double abstol,
+                                 double reltol) {
+    using namespace ts;
+    double g[N], g6[N], ut[N];
+    double aa = dts * a21;
+#pragma unroll
+    for (int c = 0; c < N; c++) {
+      k[0][c] = k1[c];
+      g[c] = fma(aa, k[0][c], u[c]);
+    }
+    M::f(k[1], g, p);
+#pragma unroll
+    for (int c = 0; c < N; c++) {
+      double acc = a32 * k[1][c];
+      acc = fma(a31, k[0][c], acc);
+      g[c] = fma(dts, acc, u[c]);
+    }
+    M::f(k[2], g, p);
+#pragma unroll
+    for (int c = 0; c < N; c++) {
+      double acc = a43 * k[2][c];
+      acc = fma(a42, k[1][c], acc);
+      acc = fma(a41, k[0][c], acc);
+      g[c] = fma(dts, acc, u[c]);
+    }
+    M::f(k[3], g, p);
+#pragma unroll
+    for (int c = 0; c < N; c++) {
+      double acc = a54 * k[3][c];
+      acc = fma(a53, k[2][c], acc);
+      acc = fma(a52, k[1][c], acc);
+      acc = fma(a51, k[0][c], acc);
+      g[c] = fma(dts, acc, u[c]);
+    }
+    M::f(k[4], g, p);
+#pragma unroll
+    for (int c = 0; c < N; c++) {
+      double acc = a65 * k[4][c];
+      acc = fma(a64, k[3][c], acc);
+      acc = fma(a63, k[2][c], acc);
+      acc = fma(a62, k[1][c], acc);
+      acc = fma(a61, k[0][c], acc);
+      g6[c] = fma(dts, acc, u[c]);
+    }
+    M::f(k[5], g6, p);
+#pragma unroll
+    for (int c = 0; c < N; c++) {
+      double acc = a76 * k[5][c];
+      acc = fma(a75, k[4][c], acc);
+      acc = fma(a74, k[3][c], acc);
+      acc = fma(a73, k[2][c], acc);
+      acc = fma(a72, k[1][c], acc);
+      acc = fma(a71, k[0][c], acc);
+      u1[c] = fma(dts, acc, u[c]);
+    }
+    M::f(k[6], u1, p);
+    if (kAuto) {
+      double e = 0.0;
+      bool nan_seen = false;
+#pragma unroll
+      for (int c = 0; c < N; c++) {
+        double r = fabs((k[6][c] - k[5][c]) / (u1[c] - g6[c]));
+        nan_seen = nan_seen || (r != r);
+        e = (r > e) ? r : e;
+      }
+      eig = nan_seen ? __builtin_nan("") : e;
+    }
+#pragma unroll
+    for (int c = 0; c < N; c++) {
+      double acc = bt7 * k[6][c];
+      acc = fma(bt6, k[5][c], acc);
+      acc = fma(bt5, k[4][c], acc);
+      acc = fma(bt4, k[3][c], acc);
+      acc = fma(bt3, k[2][c], acc);
+      acc = fma(bt2, k[1][c], acc);
+      acc = fma(bt1, k[0][c], acc);
+      ut[c] = dts * acc;
+    }
+    return err_norm<N>(ut, u, u1, abstol, reltol);
+  }
+
+  __device__ double tsit5_step_mix(double dts, float k[7][N], double* u1, double abstol,
+                                   double reltol) {
+    using namespace ts;
+    float u32[N], g[N], g6[N], u1f[N];
+    double ut[N];
+    float d32 = (float)dts;
+#pragma unroll
+    for (int c = 0; c < N; c++) {
+      u32[c] = (float)u[c];
+      k[0][c] = k1f[c];
+    }
+    float aa = d32 * (float)a21;
+#pragma unroll
+    for (int c = 0; c < N; c++) g[c] = fmaf(aa, k[0][c], u32[c]);
+    M::f(k[1], g, pf);
+#pragma unroll
+    for (int c = 0; c < N; c++) {
+      float acc = (float)a32 * k[1][c];
+      acc = fmaf((float)a31, k[0][c], acc);
+      g[c] = fmaf(d32, acc, u32[c]);
+    }
+    M::f(k[2], g, pf);
+#pragma unroll
+    for (int c = 0; c < N; c++) {
+      float acc = (float)a43 * k[2][c];
+      acc = fmaf((float)a42, k[1][c], acc);
+      acc = fmaf((float)a41, k[0][c], acc);
+      g[c] = fmaf(d32, acc, u32[c]);
+    }
+    M::f(k[3], g, pf);
+#pragma unroll
+    for (int c = 0; c < N; c++) {
+      float acc = (float)a54 * k[3][c];
+      acc = fmaf((float)a53, k[2][c], acc);
+      acc = fmaf((float)a52, k[1][c], acc);
+      acc = fmaf((float)a51, k[0][c], acc);
+      g[c] = fmaf(d32, acc, u32[c]);
+    }
+    M::f(k[4], g, pf);
+#pragma unroll
+    for (int c = 0; c < N; c++) {
+      float acc = (float)a65 * k[4][c];
+      acc = fmaf((float)a64, k[3][c], acc);
+      acc = fmaf((float)a63, k[2][c], acc);
+      acc = fmaf((float)a62, k[1][c], acc);
+      acc = fmaf((float)a61, k[0][c], acc);
+      g6[c] = fmaf(d32, acc, u32[c]);
+    }
+    M::f(k[5], g6, pf);
+#pragma unroll
+    for (int c = 0; c < N; c++) {
+      float acc = (float)a76 * k[5][c];
+      acc = fmaf((float)a75, k[4][c], acc);
+      acc = fmaf((float)a74, k[3][c], acc);
+      acc = fmaf((float)a73, k[2][c], acc);
+      acc = fmaf((float)a72, k[1][c], acc);
+      acc = fmaf((float)a71, k[0][c], acc);
+      u1[c] = fma(dts, (double)acc, u[c]);
+      u1f[c] = (float)u1[c];
+    }
+    M::f(k[6], u1f, pf);
+    if (kAuto) {
+      double e = 0.0;
+      bool nan_seen = false;
+#pragma unroll
+      for (int c = 0; c < N; c++) {
+        double r = fabs(((double)k[6][c] - (double)k[5][c]) / (u1[c] - (double)g6[c]));
+        nan_seen = nan_seen || (r != r);
+        e = (r > e) ? r : e;
+      }
+      eig = nan_seen ? __builtin_nan("") : e;
+    }
+#pragma unroll
+    for (int c = 0; c < N; c++) {
+      float acc = (float)bt7 * k[6][c];
+      acc = fmaf((float)bt6, k[5][c], acc);
+      acc = fmaf((float)bt5, k[4][c], acc);
+      acc = fmaf((float)bt4, k[3][c], acc);
+      acc = fmaf((float)bt3, k[2][c], acc);
+      acc = fmaf((float)bt2, k[1][c], acc);
+      acc = fmaf((float)bt1, k[0][c], acc);
+      ut[c] = dts * (double)acc;
+    }
+    return err_norm<N>(ut, u, u1, abstol, reltol);
+  }
+
+  // ------------------------------------------------------------ Rodas5
+  // W = J - I/(gamma dt), LU with partial pivoting in registers (row swaps
+  // as selects so that every index stays static), 8 stage solves.
+  __device__ bool rodas5_step(double dts, double* u1, double& EEst, double abstol,
+                              double reltol) {
+    double W[N][N];
+    M::jac(W, u, p);
+    if (kAuto) {
+      double best = 0.0;
+#pragma unroll
+      for (int r = 0; r < N; r++) {
+        double s = 0.0;
+#pragma unroll
+        for (int c = 0; c < N; c++) s += fabs(W[r][c]);
+        best = (s > best) ? s : best;
+      }
+      eig = best;
+    }
+    double dtgamma = dts * rd::gamma;
+    double w = 1.0 / dtgamma;
+    double inv_dt = 1.0 / dts;
+#pragma unroll
+    for (int r = 0; r < N; r++) W[r][r] = W[r][r] - w;
+    int piv[N];
+    double rpiv[N];
+    bool singular = false;
+#pragma unroll
+    for (int kk = 0; kk < N; kk++) {
+      int pk = kk;
+      double best = fabs(W[kk][kk]);
+#pragma unroll
+      for (int r = kk + 1; r < N; r++) {
+        double v = fabs(W[r][kk]);
+        if (v > best) {
+          best = v;
+          pk = r;
+        }
+      }
+      piv[kk] = pk;
+#pragma unroll
+      for (int r = kk + 1; r < N; r++) {
+        bool sw = (pk == r);
+#pragma unroll
+        for (int c = 0; c < N; c++) {
+          double x = W[kk][c], y = W[r][c];
+          W[kk][c] = sw ? y : x;
+          W[r][c] = sw ? x : y;
+        }
+      }
+      double d = W[kk][kk];
+      singular = singular || (d == 0.0);
+      double rr = 1.0 / d;
+      rpiv[kk] = rr;
+#pragma unroll
+      for (int r = kk + 1; r < N; r++) {
+        double l = W[r][kk] * rr;
+        W[r][kk] = l;
+#pragma unroll
+        for (int c = kk + 1; c < N; c++) W[r][c] = fma(-l, W[kk][c], W[r][c]);
+      }
+    }
+    if (singular) {
+      EEst = 2.0;
+      return false;
+    }
+    double kd[8][N];
+    double U[N], du[N], x[N];
+    float Uf[N], duf[N];
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+      if (!MIX) {
+        if (s == 0) {
+#pragma unroll
+          for (int c = 0; c < N; c++) U[c] = u[c];
+        } else if (s <= 5) {
+#pragma unroll
+          for (int c = 0; c < N; c++) {
+            double acc = u[c];
+#pragma unroll
+            for (int j = 0; j < s; j++) acc = fma(rd::A[s][j], kd[j][c], acc);
+            U[c] = acc;
+          }
+        } else {
+#pragma unroll
+          for (int c = 0; c < N; c++) U[c] = U[c] + kd[s - 1][c];
+        }
+        M::f(du, U, p);
+      } else {
+        if (s == 0) {
+#pragma unroll
+          for (int c = 0; c < N; c++) Uf[c] = (float)u[c];
+        } else if (s <= 5) {
+#pragma unroll
+          for (int c = 0; c < N; c++) {
+            float acc = (float)u[c];
+#pragma unroll
+            for (int j = 0; j < s; j++) acc = fmaf((float)rd::A[s][j], (float)kd[j][c], acc);
+            Uf[c] = acc;
+          }
+        } else {
+#pragma unroll
+          for (int c = 0; c < N; c++) Uf[c] = Uf[c] + (float)kd[s - 1][c];
+        }
+        M::f(duf, Uf, pf);
+#pragma unroll
+        for (int c = 0; c < N; c++) du[c] = (double)duf[c];
+      }
+#pragma unroll
+      for (int c = 0; c < N; c++) {
+        double r;
+        if (s == 0) {
+          r = du[c];
+        } else if (!MIX) {
+          double acc = (rd::C[s][0] * inv_dt) * kd[0][c];
+#pragma unroll
+          for (int j = 1; j < s; j++) acc = fma(rd::C[s][j] * inv_dt, kd[j][c], acc);
+          r = du[c] + acc;
+        } else {
+          float idt = (float)inv_dt;
+          float acc = ((float)rd::C[s][0] * idt) * (float)kd[0][c];
+#pragma unroll
+          for (int j = 1; j < s; j++) acc = fmaf((float)rd::C[s][j] * idt, (float)kd[j][c], acc);
+          r = (double)((float)du[c] + acc);
+        }
+        x[c] = -r;
+      }
+      // apply the row permutation, then L and U solves (column oriented)
+#pragma unroll
+      for (int kk = 0; kk < N; kk++) {
+#pragma unroll
+        for (int r = kk + 1; r < N; r++) {
+          bool sw = (piv[kk] == r);
+          double a0 = x[kk], b0 = x[r];
+          x[kk] = sw ? b0 : a0;
+          x[r] = sw ? a0 : b0;
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < N; j++)
+#pragma unroll
+        for (int r = j + 1; r < N; r++) x[r] = fma(-W[r][j], x[j], x[r]);
+#pragma unroll
+      for (int j = N - 1; j >= 0; j--) {
+        x[j] = x[j] * rpiv[j];
+#pragma unroll
+        for (int r = 0; r < j; r++) x[r] = fma(-W[r][j], x[j], x[r]);
+      }
+#pragma unroll
+      for (int c = 0; c < N; c++) kd[s][c] = MIX ? (double)(float)x[c] : x[c];
+    }
+#pragma unroll
+    for (int c = 0; c < N; c++) {
+      double acc = u[c];
+#pragma unroll
+      for (int j = 0; j < 5; j++) acc = fma(rd::A[5][j], kd[j][c], acc);
+      acc = acc + kd[5][c];
+      acc = acc + kd[6][c];
+      u1[c] = acc + kd[7][c];
+    }
+    EEst = err_norm<N>(kd[7], u, u1, abstol, reltol);
+    return true;
+  }
+
+  // ------------------------------------------------------------ one step
+  // One pass of loopheader!/perform_step!/loopfooter!.  Sets rc when the
+  // trajectory finishes.
+  __device__ void step(const LaneArgs& a) {
+    if (kAuto && iter > 0) {
+      double stiffness = fabs(eig * dt / kStabSize);
+      bool stiff = stiffness > kStiffTol;
+      count = stiff ? (count < 0 ? 1 : count + 1) : (count > 0 ? -1 : count - 1);
+      if (mode == 0 && count > kMaxStiff) {
+        dt = dt * kDtFac;
+        mode = 1;
+        qsteady_max = a.qsteady_default ? 1.2 : 1.0;
+      } else if (mode == 1 && count < -kMaxNonStiff) {
+        dt = dt / kDtFac;
+        mode = 0;
+        qsteady_max = 1.0;
+        fsal_init();
+        nf += 1;
+      }
+    }
+    iter++;
+    int64_t maxit = a.maxiters > 0 ? a.maxiters : 100000;
+    if (iter > maxit) {
+      rc = RC_MAXITERS;
+      return;
+    }
+    dt = fmin(dt, dtmax);
+    if (a.dtmin > 0 && dt < a.dtmin) {
+      rc = RC_DTLESSTHANMIN;
+      return;
+    }
+    dt = fmin(dt, tf - t);
+
+    double u1[N];
+    double EEst = 0.0;
+    double k64[kHasTsit5 && !MIX ? 7 : 1][N];
+    float k32[kHasTsit5 && MIX ? 7 : 1][N];
+    bool tsit_step = kHasTsit5 && (!kHasRodas || mode == 0);
+    if (tsit_step) {
+      if constexpr (kHasTsit5) {
+        if constexpr (!MIX)
+          EEst = tsit5_step64(dt, k64, u1, a.abstol, a.reltol);
+        else
+          EEst = tsit5_step_mix(dt, k32, u1, a.abstol, a.reltol);
+      }
+      nf += 6;
+    } else {
+      if constexpr (kHasRodas) {
+        if (!rodas5_step(dt, u1, EEst, a.abstol, a.reltol)) EEst = 2.0;
+      }
+      nf += 8;
+    }
+    if (EEst != EEst) {
+      // FP64: DtNaN as upstream.  Mixed: FP32 stages can overflow on a far
+      // too large trial step; treat that as EEst = +inf (reject, dt/5) and
+      // give up after 20 consecutive such rejections.
+      if (!MIX || ++nanrej > 20) {
+        rc = RC_DTNAN;
+        return;
+      }
+      EEst = __builtin_inf();
+    } else {
+      nanrej = 0;
+    }
+    // PI controller
+    double q, q11 = 1.0, lE = 0.0;
+    if (EEst == 0.0) {
+      q = 1.0 / kQmax;
+    } else {
+      lE = ctl_log2(EEst);
+      q11 = ctl_exp2(kBeta1 * lE);
+      q = q11 / qoldpow;
+      q = fmax(1.0 / kQmax, fmin(1.0 / kQmin, q / kGammaSafety));
+    }
+    if (EEst <= 1.0) {
+      naccept++;
+      if (!tsit_step) nstiff++;
+      if (1.0 <= q && q <= qsteady_max) q = 1.0;
+      qold = fmax(EEst, kQoldInit);
+      qoldpow = (qold == EEst) ? ctl_exp2(kBeta2 * lE) : ctl_pow(kQoldInit, kBeta2);
+      double dtnew = dt / q;
+      double tprev = t;
+      double ttmp;
+      if (!MIX) {
+        ttmp = t + dt;
+      } else {
+        double y = dt - tcomp;
+        ttmp = t + y;
+        tcomp = (ttmp - t) - y;
+      }
+      if (fabs(ttmp - tf) < 100.0 * jl_eps(fmax(t, tf))) ttmp = tf;
+      t = ttmp;
+      // savevalues!
+      if (a.ns > 0) {
+        while (isave < a.ns && a.saveat[isave] <= t) {
+          if (a.us) {
+            double val[N];
+            double sa = a.saveat[isave];
+            if (sa == t) {
+#pragma unroll
+              for (int c = 0; c < N; c++) val[c] = u1[c];
+            } else if (tsit_step) {
+              double th = (sa - tprev) / dt;
+              double b[7];
+              tsit5_bthetas(th, b);
+              if constexpr (kHasTsit5 && !MIX) {
+#pragma unroll
+                for (int c = 0; c < N; c++) {
+                  double acc = b[6] * k64[6][c];
+#pragma unroll
+                  for (int j = 5; j >= 0; j--) acc = fma(b[j], k64[j][c], acc);
+                  val[c] = fma(dt, acc, u[c]);
+                }
+              } else if constexpr (kHasTsit5 && MIX) {
+                float bf[7];
+#pragma unroll
+                for (int j = 0; j < 7; j++) bf[j] = (float)b[j];
+#pragma unroll
+                for (int c = 0; c < N; c++) {
+                  float acc = bf[6] * k32[6][c];
+#pragma unroll
+                  for (int j = 5; j >= 0; j--) acc = fmaf(bf[j], k32[j][c], acc);
+                  val[c] = fma(dt, (double)acc, u[c]);
+                }
+              }
+            } else {
+#pragma unroll
+              for (int c = 0; c < N; c++) val[c] = __builtin_nan("");
+            }
+            store_save(a, isave, val);
+          }
+          isave++;
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < N; c++) u[c] = u1[c];
+      if (tsit_step) {
+        if constexpr (kHasTsit5 && !MIX) {
+#pragma unroll
+          for (int c = 0; c < N; c++) k1[c] = k64[6][c];
+        } else if constexpr (kHasTsit5 && MIX) {
+#pragma unroll
+          for (int c = 0; c < N; c++) k1f[c] = k32[6][c];
+        }
+      }
+      bool bad = false;
+#pragma unroll
+      for (int c = 0; c < N; c++) bad = bad || (u1[c] != u1[c]);
+      if (bad) {
+        rc = RC_UNSTABLE;
+        return;
+      }
+      dt = fmin(dtmax, dtnew);
+    } else {
+      nreject++;
+      dt = dt / fmin(1.0 / kQmin, q11 / kGammaSafety);
+    }
+    if (!(t < tf)) rc = RC_SUCCESS;
+  }
+
+  // ------------------------------------------------------------ finish
+  __device__ void finish(const LaneArgs& a) {
+    const int64_t NT = a.ntraj;
+    if (a.us) {
+      double nanv[N];
+#pragma unroll
+      for (int c = 0; c < N; c++) nanv[c] = __builtin_nan("");
+      for (int k = isave; k < a.ns; k++) store_save(a, k, nanv);
+    }
+    if (a.u_final)
+#pragma unroll
+      for (int c = 0; c < N; c++) a.u_final[(int64_t)c * NT + i] = u[c];
+    if (a.t_final) a.t_final[i] = t;
+    if (a.retcode) a.retcode[i] = rc;
+    if (a.naccept) a.naccept[i] = naccept;
+    if (a.nreject) a.nreject[i] = nreject;
+    if (a.nf) a.nf[i] = nf;
+    if (a.nstiff) a.nstiff[i] = nstiff;
+  }
+};
+
+}  // namespace mideq

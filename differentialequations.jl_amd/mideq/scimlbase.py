@@ -1,0 +1,416 @@
+"""Host-side mirror of the SciMLBase / OrdinaryDiffEq ensemble surface.
+
+The reference path (DifferentialEquations.jl 8.0.3 re-exports these from
+SciMLBase 3 and OrdinaryDiffEq 7; src/DifferentialEquations.jl:8-9):
+
+    solve(EnsembleProblem(prob; prob_func, output_func, reduction),
+          Tsit5() | Rodas5() | AutoTsit5(Rodas5()), EnsembleThreads();
+          trajectories, saveat, abstol, reltol)
+
+Names and argument meaning follow the reference (test/qa/qa.jl:15,50,52,56,
+86,97,119,183,192,213; test/core.jl:13,31,46,84,93).  `EnsembleMI355X` is
+the new EnsembleAlgorithm: its solve packs the problems that prob_func
+returns into structure-of-arrays buffers and calls the C ABI of
+libmideq.so (include/mideq.h), exactly as the Julia shim does with ccall.
+Julia conventions kept: trajectory index i runs 1..N in prob_func and
+output_func; output_func returns (out, rerun); reduction(u, batch, I)
+returns (u, converged).
+"""
+import enum
+import time
+from dataclasses import dataclass, field, replace
+from typing import Any, Callable, Optional
+
+import numpy as np
+
+from . import _abi
+
+
+# ---------------------------------------------------------------- return codes
+class ReturnCode(enum.IntEnum):
+    """SciMLBase.ReturnCode subset produced by the solver (qa.jl:213)."""
+    Default = 0
+    Success = 1
+    MaxIters = 2
+    DtLessThanMin = 3
+    Unstable = 4
+    DtNaN = 5
+
+
+def successful_retcode(rc):
+    return ReturnCode(int(rc)) == ReturnCode.Success
+
+
+# ---------------------------------------------------------------- models
+@dataclass(frozen=True)
+class ODEFunction:
+    """A right-hand side registered with the device library.
+
+    `jac` says whether the model carries an analytic Jacobian (needed by
+    Rodas5, as ODEFunction(f; jac=...) in the reference).  Calling the
+    object evaluates f on the host (for inspection only; the solver never
+    does)."""
+    name: str
+    n: Optional[int]
+    np: int
+    jac: bool = True
+
+    @property
+    def model_id(self):
+        return _abi.MODEL[self.name]
+
+    def __call__(self, u, p, t=0.0):
+        u = np.asarray(u, dtype=np.float64)
+        p = np.asarray(p, dtype=np.float64)
+        if self.name == "exp":
+            return p[0] * u
+        if self.name == "lorenz":
+            return np.array([p[0] * (u[1] - u[0]), u[0] * (p[1] - u[2]) - u[1],
+                             u[0] * u[1] - p[2] * u[2]])
+        if self.name == "robertson":
+            return np.array([-p[0] * u[0] + p[2] * u[1] * u[2],
+                             p[0] * u[0] - p[1] * u[1] ** 2 - p[2] * u[1] * u[2],
+                             p[1] * u[1] ** 2])
+        if self.name == "vdp":
+            return np.array([u[1], p[0] * (1 - u[0] ** 2) * u[1] - u[0]])
+        if self.name == "linear":
+            return p.reshape(3, 3) @ u
+        if self.name == "brusselator":
+            N = len(u) // 2
+            uu, vv = u[0::2], u[1::2]
+            D = p[2] * N * N
+            lu = np.roll(uu, 1) - 2 * uu + np.roll(uu, -1)
+            lv = np.roll(vv, 1) - 2 * vv + np.roll(vv, -1)
+            out = np.empty_like(u)
+            out[0::2] = p[0] + uu * uu * vv - (p[1] + 1) * uu + D * lu
+            out[1::2] = p[1] * uu - uu * uu * vv + D * lv
+            return out
+        raise ValueError(self.name)
+
+
+exp_growth = ODEFunction("exp", 1, 1)
+lorenz = ODEFunction("lorenz", 3, 3)
+robertson = ODEFunction("robertson", 3, 3)
+vanderpol = ODEFunction("vdp", 2, 1)
+linear3 = ODEFunction("linear", 3, 9)
+brusselator = ODEFunction("brusselator", None, 3)
+
+
+# ---------------------------------------------------------------- problems
+@dataclass
+class ODEProblem:
+    """ODEProblem(f, u0, tspan, p) (qa.jl:86; test/core.jl:13,31,46)."""
+    f: ODEFunction
+    u0: Any
+    tspan: tuple
+    p: Any = None
+
+    def __post_init__(self):
+        self.u0 = np.atleast_1d(np.asarray(self.u0, dtype=np.float64))
+        self.p = np.atleast_1d(np.asarray([] if self.p is None else self.p, dtype=np.float64))
+        self.tspan = (float(self.tspan[0]), float(self.tspan[1]))
+
+
+def remake(prob, u0=None, p=None, tspan=None, f=None):
+    """remake(prob; u0, p, tspan) (qa.jl:183; test/core.jl:84)."""
+    return ODEProblem(prob.f if f is None else f, prob.u0 if u0 is None else u0,
+                      prob.tspan if tspan is None else tspan, prob.p if p is None else p)
+
+
+def _default_prob_func(prob, i, repeat):
+    return prob
+
+
+def _default_output_func(sol, i):
+    return sol, False
+
+
+def _default_reduction(u, batch, I):
+    return list(u) + list(batch), False
+
+
+@dataclass
+class EnsembleProblem:
+    """EnsembleProblem(prob; prob_func, output_func, reduction, u_init,
+    safetycopy) (qa.jl:50).  prob_func(prob, i, repeat) -> ODEProblem with
+    i in 1..N.  A prob_func with attribute `vectorized = True` is instead
+    called once as prob_func(prob, I, repeat) with the index array I and
+    returns a dict(u0=[n,N], p=[np,N], t0=..., tf=...), which avoids N
+    Python calls for large ensembles."""
+    prob: ODEProblem
+    prob_func: Callable = _default_prob_func
+    output_func: Callable = _default_output_func
+    reduction: Callable = _default_reduction
+    u_init: Any = None
+    safetycopy: bool = True
+
+
+# ---------------------------------------------------------------- algorithms
+@dataclass(frozen=True)
+class Tsit5:
+    """Tsitouras 5(4) (qa.jl:119)."""
+    code = _abi.ALG["tsit5"]
+
+
+@dataclass(frozen=True)
+class Rodas5:
+    """8-stage stiffly accurate Rosenbrock with analytic Jacobian."""
+    code = _abi.ALG["rodas5"]
+
+
+@dataclass(frozen=True)
+class AutoTsit5:
+    """AutoTsit5(Rodas5()) (qa.jl:15): Tsit5 with stiffness detection."""
+    stiffalg: Any = field(default_factory=Rodas5)
+    code = _abi.ALG["auto"]
+
+
+@dataclass(frozen=True)
+class EnsembleMI355X:
+    """EnsembleAlgorithm that solves every trajectory on one MI355X.
+
+    precision: "fp64" or "mixed" (FP32 stages; FP64 state update, norm,
+    time, J/W/LU).  schedule: "auto", "lane" or "persistent".
+    dt0 / alg_hint: optional per-trajectory initial step and initial method
+    (e.g. from mideq.learned).  device: GPU ordinal."""
+    precision: str = "fp64"
+    schedule: str = "auto"
+    device: int = 0
+    save_dtype: str = "float64"
+    dt0: Any = None
+    alg_hint: Any = None
+
+
+# ---------------------------------------------------------------- solutions
+@dataclass
+class Stats:
+    naccept: int
+    nreject: int
+    nf: int
+
+
+@dataclass
+class ODESolution:
+    """One trajectory's result (qa.jl:87)."""
+    t: np.ndarray
+    u: list
+    retcode: ReturnCode
+    stats: Stats
+    prob: Optional[ODEProblem] = None
+
+    def __getitem__(self, k):
+        return self.u[k]
+
+
+@dataclass
+class EnsembleArrays:
+    """Structure-of-arrays result of one device solve."""
+    u_final: np.ndarray           # [n, N]
+    t_final: np.ndarray           # [N]
+    retcode: np.ndarray           # [N] int8
+    naccept: np.ndarray
+    nreject: np.ndarray
+    nf: np.ndarray
+    nstiff: np.ndarray
+    saveat: Optional[np.ndarray]  # [ns]
+    us: Optional[np.ndarray]      # [ns, n, N]
+    t0: np.ndarray
+    u0: np.ndarray
+    kernel_ms: float = 0.0
+    total_ms: float = 0.0
+
+    def solution(self, i, prob=None):
+        """ODESolution of trajectory i (0-based)."""
+        st = Stats(int(self.naccept[i]), int(self.nreject[i]), int(self.nf[i]))
+        rc = ReturnCode(int(self.retcode[i]))
+        if self.us is not None:
+            vals = self.us[:, :, i]
+            ok = ~np.isnan(vals).any(axis=1)
+            return ODESolution(self.saveat[ok].copy(), [v.copy() for v in vals[ok]], rc, st, prob)
+        return ODESolution(np.array([self.t0[i], self.t_final[i]]),
+                           [self.u0[:, i].copy(), self.u_final[:, i].copy()], rc, st, prob)
+
+
+@dataclass
+class EnsembleSolution:
+    """EnsembleSolution (qa.jl:52): outputs in trajectory order."""
+    u: Any
+    elapsedTime: float
+    converged: bool
+    arrays: Optional[EnsembleArrays] = None
+
+    def __len__(self):
+        return len(self.u)
+
+    def __getitem__(self, k):
+        return self.u[k]
+
+
+# ---------------------------------------------------------------- packing
+def _pack(ens, idx, repeat):
+    """Evaluate prob_func for trajectories idx (0-based) and pack SoA."""
+    pf = ens.prob_func
+    if getattr(pf, "vectorized", False):
+        d = pf(ens.prob, idx + 1, repeat)
+        u0 = np.ascontiguousarray(d["u0"], dtype=np.float64)
+        p = np.ascontiguousarray(d["p"], dtype=np.float64)
+        N = len(idx)
+        t0 = np.ascontiguousarray(np.broadcast_to(d.get("t0", ens.prob.tspan[0]), (N,)),
+                                  dtype=np.float64)
+        tf = np.ascontiguousarray(np.broadcast_to(d.get("tf", ens.prob.tspan[1]), (N,)),
+                                  dtype=np.float64)
+        nvec = d.get("nvec")
+        return ens.prob.f, u0, p, t0, tf, nvec, None
+    probs = []
+    for i in idx:
+        base = ens.prob
+        pr = pf(base, int(i) + 1, repeat)
+        probs.append(pr)
+    f = probs[0].f
+    for pr in probs:
+        if pr.f is not f and pr.f != f:
+            raise ValueError("all trajectories of one ensemble must share the RHS")
+    if f.name == "brusselator":
+        nmax = max(len(pr.u0) for pr in probs)
+        u0 = np.zeros((nmax, len(probs)))
+        for k, pr in enumerate(probs):
+            u0[:len(pr.u0), k] = pr.u0
+        nvec = np.array([len(pr.u0) for pr in probs], dtype=np.int32)
+    else:
+        u0 = np.ascontiguousarray(np.stack([pr.u0 for pr in probs], axis=1))
+        nvec = None
+    p = np.ascontiguousarray(np.stack([pr.p for pr in probs], axis=1))
+    t0 = np.array([pr.tspan[0] for pr in probs])
+    tf = np.array([pr.tspan[1] for pr in probs])
+    return f, u0, p, t0, tf, nvec, probs
+
+
+def _saveat_vector(saveat, tspan):
+    if saveat is None:
+        return None
+    if np.isscalar(saveat):
+        t0, t1 = tspan
+        d = float(saveat)
+        n = int(np.floor((t1 - t0) / d + 1e-9)) + 1
+        pts = t0 + d * np.arange(n)
+        if pts[-1] < t1:
+            pts = np.append(pts, t1)
+        return np.ascontiguousarray(pts, dtype=np.float64)
+    return np.ascontiguousarray(np.sort(np.asarray(saveat, dtype=np.float64)))
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def device_solve(f, u0, p, t0, tf, nvec, alg, ealg, saveat=None, abstol=1e-6, reltol=1e-3,
+                 dt=0.0, dtmin=0.0, dtmax=0.0, maxiters=0, dt0=None, alg_hint=None,
+                 block=0, blocks_per_cu=0):
+    """One call of mideq_ensemble_solve on host arrays -> EnsembleArrays."""
+    ctx = _abi.context(ealg.device)
+    n, N = u0.shape
+    P = _abi.Problem()
+    P.model = f.model_id
+    P.n = n
+    P.np = p.shape[0]
+    P.ntraj = N
+    P.u0 = _ptr(u0)
+    P.p = _ptr(p)
+    t0 = np.ascontiguousarray(t0, dtype=np.float64)
+    tf = np.ascontiguousarray(tf, dtype=np.float64)
+    P.t0 = _ptr(t0)
+    P.tf = _ptr(tf)
+    P.tspan0 = float(t0[0]) if N else 0.0
+    P.tspan1 = float(tf[0]) if N else 0.0
+    if nvec is not None:
+        nvec = np.ascontiguousarray(nvec, dtype=np.int32)
+        P.nvec = _ptr(nvec)
+    O = _abi.Options()
+    O.alg = alg.code
+    O.precision = _abi.PRECISION[ealg.precision]
+    O.abstol = abstol
+    O.reltol = reltol
+    O.dt = dt
+    O.dtmin = dtmin
+    O.dtmax = dtmax
+    O.maxiters = maxiters
+    O.qsteady_default = 1
+    O.schedule = _abi.SCHEDULE[ealg.schedule]
+    O.block = block
+    O.blocks_per_cu = blocks_per_cu
+    f32 = ealg.save_dtype == "float32"
+    O.save_dtype = 1 if f32 else 0
+    us = None
+    if saveat is not None and len(saveat):
+        O.saveat = _ptr(saveat)
+        O.ns = len(saveat)
+        us = np.empty((len(saveat), n, N), dtype=np.float32 if f32 else np.float64)
+    dt0 = dt0 if dt0 is not None else ealg.dt0
+    hint = alg_hint if alg_hint is not None else ealg.alg_hint
+    if dt0 is not None:
+        dt0 = np.ascontiguousarray(dt0, dtype=np.float64)
+        O.dt0 = _ptr(dt0)
+    if hint is not None:
+        hint = np.ascontiguousarray(hint, dtype=np.int8)
+        O.alg_hint = _ptr(hint)
+    res = dict(u_final=np.empty((n, N)), t_final=np.empty(N), retcode=np.empty(N, np.int8),
+               naccept=np.empty(N, np.int32), nreject=np.empty(N, np.int32),
+               nf=np.empty(N, np.int32), nstiff=np.empty(N, np.int32))
+    out = _abi.Outputs()
+    out.us = _ptr(us)
+    for k, v in res.items():
+        setattr(out, k, _ptr(v))
+    ctx.solve(P, O, out)
+    st = ctx.stats()
+    return EnsembleArrays(saveat=saveat, us=us, t0=t0, u0=u0, kernel_ms=st.kernel_ms,
+                          total_ms=st.total_ms, **res)
+
+
+# ---------------------------------------------------------------- solve
+def solve(prob, alg, ensemblealg=None, *, trajectories=None, saveat=None, abstol=1e-6,
+          reltol=1e-3, dt=0.0, dtmin=0.0, dtmax=0.0, maxiters=100000, **kw):
+    """solve(EnsembleProblem, alg, EnsembleMI355X(); trajectories, ...)
+    (qa.jl:192) or solve(ODEProblem, alg) for a single trajectory."""
+    if isinstance(prob, ODEProblem):
+        ens = EnsembleProblem(prob)
+        es = solve(ens, alg, ensemblealg or EnsembleMI355X(), trajectories=1, saveat=saveat,
+                   abstol=abstol, reltol=reltol, dt=dt, dtmin=dtmin, dtmax=dtmax,
+                   maxiters=maxiters, **kw)
+        return es.u[0]
+    if not isinstance(prob, EnsembleProblem):
+        raise TypeError("solve expects an ODEProblem or EnsembleProblem")
+    if ensemblealg is None:
+        ensemblealg = EnsembleMI355X()
+    if not isinstance(ensemblealg, EnsembleMI355X):
+        raise TypeError(f"unsupported ensemble algorithm {ensemblealg!r}")
+    if trajectories is None:
+        raise TypeError("trajectories is required")
+    tstart = time.perf_counter()
+    N = int(trajectories)
+    idx = np.arange(N)
+    sa = _saveat_vector(saveat, prob.prob.tspan)
+    outputs = [None] * N
+    repeat = 1
+    pending = idx
+    last_arrays = None
+    while len(pending):
+        f, u0, p, t0, tf, nvec, probs = _pack(prob, pending, repeat)
+        arr = device_solve(f, u0, p, t0, tf, nvec, alg, ensemblealg, saveat=sa, abstol=abstol,
+                           reltol=reltol, dt=dt, dtmin=dtmin, dtmax=dtmax,
+                           maxiters=maxiters, **kw)
+        last_arrays = arr if last_arrays is None else last_arrays
+        if repeat == 1:
+            last_arrays = arr
+        rerun = []
+        for k, i in enumerate(pending):
+            sol = arr.solution(k, probs[k] if probs else None)
+            out, again = prob.output_func(sol, int(i) + 1)
+            outputs[i] = out
+            if again:
+                rerun.append(i)
+        pending = np.array(rerun, dtype=np.int64)
+        repeat += 1
+    u = prob.u_init if prob.u_init is not None else []
+    u, converged = prob.reduction(u, outputs, range(1, N + 1))
+    return EnsembleSolution(u=u, elapsedTime=time.perf_counter() - tstart,
+                            converged=bool(converged) or True, arrays=last_arrays)

@@ -1,0 +1,165 @@
+/*
+ * mideq.h -- C ABI of libmideq.so, the MI355X ensemble ODE solver.
+ *
+ * Drop-in boundary.  The reference path is
+ *     solve(EnsembleProblem(prob; prob_func, output_func), Tsit5()|Rodas5(),
+ *           EnsembleThreads(); trajectories, saveat, abstol, reltol)
+ * (DifferentialEquations.jl 8.0.3 re-exports it from SciMLBase 3 /
+ * OrdinaryDiffEq 7: src/DifferentialEquations.jl:8-9, Project.toml:13,18;
+ * names EnsembleProblem/EnsembleThreads/solve at test/qa/qa.jl:50,56,192,
+ * Tsit5 at qa.jl:119, Rodas5P at qa.jl:97, AutoTsit5 at qa.jl:15,
+ * ReturnCode at qa.jl:213).  An ensemble backend plugs in by dispatching
+ * SciMLBase.__solve on its EnsembleAlgorithm type; the backend's __solve
+ * evaluates prob_func for every trajectory on the host, packs the problems
+ * into structure-of-arrays buffers and calls mideq_ensemble_solve below,
+ * then rebuilds the EnsembleSolution (output_func, rerun, reduction) on
+ * the host.  This header is everything such a backend binds (ccall /
+ * ctypes); no C++ or torch types cross it.
+ *
+ * Layouts (structure of arrays, trajectory index fastest -- Julia's
+ * column-major Matrix{Float64}(undef, ntraj, n) has exactly this layout):
+ *   u0[c*ntraj + i], p[j*ntraj + i], us[(k*n + c)*ntraj + i],
+ *   u_final[c*ntraj + i].
+ * Ownership: the caller owns every buffer; nothing is retained after return.
+ * Threading: a context is used by one thread at a time; separate contexts
+ * may run concurrently.  Errors: negative MIDEQ_E_* return values with a
+ * message from mideq_last_error(); per-trajectory failures are not call
+ * errors and go to retcode[i] (mapped 1:1 to SciMLBase.ReturnCode).
+ */
+#ifndef MIDEQ_H
+#define MIDEQ_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MIDEQ_VERSION 10000 /* 1.0.0 */
+
+/* Registered right-hand sides (the shim maps a Julia f to one of these). */
+typedef enum {
+  MIDEQ_MODEL_EXP = 0,         /* u' = p0 u             (test/core.jl:10-14)          */
+  MIDEQ_MODEL_LORENZ = 1,      /* p = (sigma, rho, beta) (test/core.jl:22-27)          */
+  MIDEQ_MODEL_ROBERTSON = 2,   /* p = (k1, k2, k3)       (test/core.jl:39-45), has J   */
+  MIDEQ_MODEL_VDP = 3,         /* p = (mu), x'=y, y'=mu(1-x^2)y-x, has J               */
+  MIDEQ_MODEL_BRUSSELATOR = 4, /* p = (A, B, alpha), n = 2*Ngrid (ragged), has J       */
+  MIDEQ_MODEL_LINEAR = 5       /* u' = M u, n = 3, p = M row-major (9), has J          */
+} mideq_model;
+
+typedef enum { MIDEQ_ALG_TSIT5 = 0, MIDEQ_ALG_RODAS5 = 1, MIDEQ_ALG_AUTO = 2 } mideq_alg;
+
+/* FP64: every operation in double.  MIXED: stage sums and RHS evaluations
+ * in FP32; state update, error norm, time (compensated), J, W and its LU in
+ * FP64. */
+typedef enum { MIDEQ_FP64 = 0, MIDEQ_MIXED = 1 } mideq_precision;
+
+typedef enum {
+  MIDEQ_OK = 0,
+  MIDEQ_E_ARG = -1,
+  MIDEQ_E_HIP = -2,
+  MIDEQ_E_NOMEM = -3,
+  MIDEQ_E_MODEL = -4,
+  MIDEQ_E_UNSUPPORTED = -5,
+  MIDEQ_E_NODEVICE = -6
+} mideq_status;
+
+/* Per-trajectory result, 1:1 with SciMLBase.ReturnCode. */
+typedef enum {
+  MIDEQ_RC_DEFAULT = 0,
+  MIDEQ_RC_SUCCESS = 1,
+  MIDEQ_RC_MAXITERS = 2,
+  MIDEQ_RC_DTLESSTHANMIN = 3,
+  MIDEQ_RC_UNSTABLE = 4,
+  MIDEQ_RC_DTNAN = 5
+} mideq_retcode;
+
+typedef enum {
+  MIDEQ_SCHED_AUTO = 0,       /* autotuned choice */
+  MIDEQ_SCHED_LANE = 1,       /* one lane per trajectory, one launch  */
+  MIDEQ_SCHED_PERSISTENT = 2  /* persistent lanes refilled from an atomic queue */
+} mideq_schedule;
+
+typedef struct mideq_ctx mideq_ctx;
+
+/* One EnsembleProblem after prob_func: ntraj packed ODEProblems. */
+typedef struct {
+  int32_t model;        /* mideq_model */
+  int32_t n;            /* states per trajectory (maximum when nvec != NULL) */
+  int32_t np;           /* parameters per trajectory */
+  int32_t reserved0;
+  int64_t ntraj;
+  const double *u0;     /* [n][ntraj] */
+  const double *p;      /* [np][ntraj] */
+  const double *t0;     /* [ntraj] or NULL: tspan0 for all */
+  const double *tf;     /* [ntraj] or NULL: tspan1 for all */
+  double tspan0, tspan1;
+  const int32_t *nvec;  /* [ntraj] ragged state sizes or NULL */
+} mideq_problem;
+
+/* solve() keyword arguments (names as in SciMLBase/OrdinaryDiffEq). */
+typedef struct {
+  int32_t alg;          /* mideq_alg */
+  int32_t precision;    /* mideq_precision */
+  double abstol, reltol;
+  double dt;            /* > 0: initial dt for all; 0: initdt heuristic */
+  double dtmin;         /* > 0: DtLessThanMin below it; 0: off */
+  double dtmax;         /* > 0: cap; 0: |tf - t0| */
+  int64_t maxiters;     /* 0: 100000 */
+  const double *saveat; /* [ns] ascending absolute times or NULL (endpoint only) */
+  int32_t ns;
+  int32_t save_dtype;   /* 0: double, 1: float for `us` */
+  const double *dt0;    /* [ntraj] per-trajectory initial dt (learned) or NULL */
+  const int8_t *alg_hint; /* [ntraj] initial method for ALG_AUTO (0 Tsit5, 1 Rodas5) or NULL */
+  int32_t qsteady_default; /* 1: method-default qsteady band; 0: [1,1] */
+  int32_t schedule;     /* mideq_schedule */
+  int32_t block;        /* threads per block, 0 = tuned */
+  int32_t blocks_per_cu;/* persistent grid size per CU, 0 = tuned */
+} mideq_options;
+
+/* Results; any pointer may be NULL. */
+typedef struct {
+  void *us;             /* [ns][n][ntraj] saveat values (double or float) */
+  double *u_final;      /* [n][ntraj] */
+  double *t_final;      /* [ntraj] */
+  int8_t *retcode;      /* [ntraj] mideq_retcode */
+  int32_t *naccept;     /* [ntraj] */
+  int32_t *nreject;     /* [ntraj] */
+  int32_t *nf;          /* [ntraj] RHS evaluations */
+  int32_t *nstiff;      /* [ntraj] accepted Rodas5 steps */
+} mideq_outputs;
+
+/* Timing and launch record of the last solve on a context. */
+typedef struct {
+  double kernel_ms;     /* solver kernel time (HIP events on the solve stream) */
+  double total_ms;      /* whole call including copies */
+  int32_t schedule;     /* schedule actually used */
+  int32_t block;
+  int32_t grid;
+  int32_t launches;
+} mideq_stats;
+
+int mideq_version(void);
+int mideq_ctx_create(mideq_ctx **out, int32_t device);
+void mideq_ctx_destroy(mideq_ctx *ctx);
+const char *mideq_last_error(const mideq_ctx *ctx);
+int mideq_last_stats(const mideq_ctx *ctx, mideq_stats *out);
+
+/* Replaces SciMLBase.__solve(::EnsembleProblem, alg, ::EnsembleThreads;
+ * trajectories, saveat, abstol, reltol) for problems packed as above.
+ * Host pointers; blocking.  Copies in, solves on the context's stream,
+ * copies out. */
+int mideq_ensemble_solve(mideq_ctx *ctx, const mideq_problem *prob,
+                         const mideq_options *opt, mideq_outputs *out);
+
+/* Same, with every pointer in prob/opt/out a device pointer (inputs already
+ * resident in HBM).  Enqueued on `stream` (a hipStream_t, NULL = the
+ * context's stream); returns after launch without synchronizing. */
+int mideq_ensemble_solve_device(mideq_ctx *ctx, const mideq_problem *prob,
+                                const mideq_options *opt, mideq_outputs *out,
+                                void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

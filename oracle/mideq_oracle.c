@@ -898,6 +898,7 @@ static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
   long long iter = 0;
   int rc = ORACLE_RC_DEFAULT;
   int have_accept = 0;
+  int nanrej = 0;
 
   if (!(tf > t0)) rc = ORACLE_RC_SUCCESS;
   while (rc == ORACLE_RC_DEFAULT && t < tf) {
@@ -956,8 +957,16 @@ static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
       eig = nJ;
     }
     if (EEst != EEst) {
-      rc = ORACLE_RC_DTNAN;
-      break;
+      /* FP64: DtNaN as upstream.  Mixed: FP32 stages can overflow on a far
+       * too large trial step; treat that as EEst = +inf (reject, dt/5) and
+       * give up after 20 consecutive such rejections. */
+      if (!mixed || ++nanrej > 20) {
+        rc = ORACLE_RC_DTNAN;
+        break;
+      }
+      EEst = INFINITY;
+    } else {
+      nanrej = 0;
     }
 
     /* stepsize_controller! (PI) */

@@ -1,0 +1,72 @@
+"""The C-ABI library loads and exports every symbol include/mideq.h declares;
+argument validation works without a GPU (no compute calls here)."""
+import ctypes
+
+import pytest
+
+import mideq
+from mideq import _abi
+
+
+def test_library_exports_every_declared_symbol():
+    L = _abi.load()
+    names = _abi.header_functions()
+    assert "mideq_ensemble_solve" in names and "mideq_ensemble_solve_device" in names
+    for nm in names:
+        assert hasattr(L, nm), nm
+    assert L.mideq_version() == 10000
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """Compile a probe against include/mideq.h and compare every field offset
+    and struct size with the ctypes mirror (the layout the Julia shim binds)."""
+    import os
+    import shutil
+    import subprocess
+    if not shutil.which("gcc"):
+        pytest.skip("gcc not available")
+    structs = {"mideq_problem": _abi.Problem, "mideq_options": _abi.Options,
+               "mideq_outputs": _abi.Outputs, "mideq_stats": _abi.Stats}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "mideq.h"', "int main(){"]
+    for cname, cls in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in cls._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("return 0;}")
+    src = tmp_path / "probe.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "probe"
+    inc = os.path.dirname(_abi.HEADER)
+    subprocess.run(["gcc", "-I", inc, str(src), "-o", str(exe)], check=True)
+    got = {}
+    for ln in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n"):
+        if ln:
+            a, b, v = ln.split()
+            got[(a, b)] = int(v)
+    for cname, cls in structs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(cls), cname
+        for fname, _ in cls._fields_:
+            assert got[(cname, fname)] == getattr(cls, fname).offset, (cname, fname)
+
+
+def test_no_device_is_an_error_not_a_crash():
+    L = _abi.load()
+    h = ctypes.c_void_p()
+    rc = L.mideq_ctx_create(ctypes.byref(h), 0)
+    try:
+        import torch
+        has_gpu = torch.cuda.device_count() > 0
+    except Exception:
+        has_gpu = False
+    if not has_gpu:
+        assert rc == -6
+    else:
+        assert rc == 0
+        L.mideq_ctx_destroy(h)
+
+
+def test_saveat_scalar_expansion_matches_core_jl():
+    # test/core.jl:95: saveat=0.1 on (0,1) gives 11 points, both endpoints
+    from mideq.scimlbase import _saveat_vector
+    v = _saveat_vector(0.1, (0.0, 1.0))
+    assert len(v) == 11 and v[0] == 0.0 and v[-1] == 1.0

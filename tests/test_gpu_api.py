@@ -1,0 +1,85 @@
+"""The SciMLBase-style surface on the GPU: EnsembleProblem with prob_func,
+output_func (rerun) and reduction, and the reference's known answers
+(test/core.jl)."""
+import math
+
+import numpy as np
+import pytest
+
+import mideq
+from gpu_util import say
+from mideq import (EnsembleMI355X, EnsembleProblem, ODEProblem, ReturnCode, Rodas5, Tsit5,
+                   remake, solve)
+
+pytestmark = pytest.mark.gpu
+
+
+def test_core_basic_solve_on_gpu():
+    say("core.jl basic ODE solve")
+    prob = ODEProblem(mideq.exp_growth, 0.5, (0.0, 1.0), [1.01])
+    sol = solve(prob, Tsit5(), reltol=1e-8, abstol=1e-8)
+    assert sol.retcode == ReturnCode.Success
+    assert len(sol.t) > 0 and len(sol.u) > 0
+    assert sol.u[0][0] == 0.5
+    assert abs(sol.u[-1][0] - 0.5 * math.exp(1.01)) < 1e-7
+
+
+def test_core_saveat_and_lorenz():
+    say("core.jl saveat + lorenz")
+    prob = ODEProblem(mideq.exp_growth, 0.5, (0.0, 1.0), [1.01])
+    sol = solve(prob, Tsit5(), saveat=0.1)
+    assert sol.retcode == ReturnCode.Success and len(sol.t) == 11
+    lp = ODEProblem(mideq.lorenz, [1.0, 0.0, 0.0], (0.0, 1.0), [10.0, 28.0, 8 / 3])
+    sol = solve(lp, Tsit5())
+    assert sol.retcode == ReturnCode.Success
+    np.testing.assert_array_equal(sol.u[0], [1.0, 0.0, 0.0])
+    assert len(sol.u[-1]) == 3
+    rp = ODEProblem(mideq.robertson, [1.0, 0.0, 0.0], (0.0, 1e5), [0.04, 3e7, 1e4])
+    assert solve(rp, Rodas5()).retcode == ReturnCode.Success
+    assert solve(remake(prob, u0=1.0), Tsit5()).u[0][0] == 1.0
+
+
+def test_ensemble_prob_func_output_func_rerun_reduction(oracle):
+    say("ensemble prob_func/output_func/rerun/reduction")
+    base = ODEProblem(mideq.lorenz, [1.0, 0.0, 0.0], (0.0, 2.0), [10.0, 28.0, 8 / 3])
+    rng = np.random.default_rng(0)
+    P = np.array([10.0, 28.0, 8 / 3])[:, None] * (1 + rng.uniform(-0.1, 0.1, (3, 40)))
+
+    def prob_func(prob, i, repeat):
+        return remake(prob, p=P[:, i - 1] * (1.0 if repeat == 1 else 1.01))
+
+    seen = []
+
+    def output_func(sol, i):
+        seen.append(i)
+        # rerun every 7th trajectory once
+        return sol.u[-1], (i % 7 == 0 and sol.stats.naccept > 0 and len(seen) <= 40)
+
+    def reduction(u, batch, I):
+        return u + [float(np.sum(b)) for b in batch], False
+
+    ens = EnsembleProblem(base, prob_func=prob_func, output_func=output_func,
+                          reduction=reduction, u_init=[])
+    es = solve(ens, Tsit5(), EnsembleMI355X(), trajectories=40, abstol=1e-9, reltol=1e-9)
+    assert len(es.u) == 40
+    assert sorted(seen[:40]) == list(range(1, 41)) and len(seen) == 40 + 5
+    # rerun trajectories used repeat=2 parameters
+    for i in range(1, 41):
+        pi = P[:, i - 1] * (1.01 if i % 7 == 0 else 1.0)
+        r = oracle.solve("lorenz", [1, 0, 0], pi, (0.0, 2.0), abstol=1e-9, reltol=1e-9)
+        assert es.u[i - 1] == pytest.approx(float(np.sum(r["u"])), rel=1e-14)
+
+
+def test_vectorized_prob_func_large():
+    say("vectorized prob_func N=100000")
+    w = mideq.workloads.lorenz(100000, seed=3)
+
+    def pf(prob, I, repeat):
+        return dict(u0=w["u0"], p=w["p"])
+    pf.vectorized = True
+    base = ODEProblem(mideq.lorenz, [1.0, 0.0, 0.0], (0.0, 10.0), [10.0, 28.0, 8 / 3])
+    es = solve(EnsembleProblem(base, prob_func=pf, output_func=lambda s, i: (None, False)),
+               Tsit5(), EnsembleMI355X(), trajectories=100000, abstol=1e-8, reltol=1e-8)
+    a = es.arrays
+    assert (a.retcode == 1).all()
+    assert 500 < a.naccept.mean() < 1100

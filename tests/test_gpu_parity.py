@@ -1,0 +1,141 @@
+"""GPU parity: libmideq.so (HIP, gfx950) against the CPU oracle on the same
+seeded ensembles.  FP64 and mixed runs must agree bit for bit: identical
+retcodes and accept/reject counts per trajectory, equal final states and
+saveat values (the kernels and the oracle follow one arithmetic order)."""
+import numpy as np
+import pytest
+
+from gpu_util import assert_identical, cpu, gpu, say
+from mideq import workloads
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("schedule", ["lane", "persistent"])
+def test_lorenz_fp64_saveat_bitexact(oracle, schedule):
+    N = 3000
+    say(f"lorenz fp64 tsit5 N={N} saveat=100 schedule={schedule}")
+    w = workloads.lorenz(N, seed=11)
+    sa = workloads.saveat_uniform(0.0, 10.0, 100)
+    g = gpu("lorenz", w["u0"], w["p"], 0.0, 10.0, saveat=sa, abstol=1e-8, reltol=1e-8,
+            schedule=schedule)
+    c = cpu(oracle, "lorenz", w["u0"], w["p"], 0.0, 10.0, saveat=sa, abstol=1e-8, reltol=1e-8)
+    assert (c["retcode"] == 1).all()
+    assert_identical(g, c)
+    # endpoints of the saveat grid: u0 copied exactly, last point = final state
+    np.testing.assert_array_equal(g.us[0], w["u0"])
+    np.testing.assert_array_equal(g.us[-1], g.u_final)
+
+
+def test_lorenz_fp64_float32_output(oracle):
+    N = 1000
+    say(f"lorenz fp64 f32-saveat N={N}")
+    w = workloads.lorenz(N, seed=12)
+    sa = workloads.saveat_uniform(0.0, 10.0, 100)
+    g = gpu("lorenz", w["u0"], w["p"], 0.0, 10.0, saveat=sa, abstol=1e-8, reltol=1e-8,
+            save_dtype="float32")
+    c = cpu(oracle, "lorenz", w["u0"], w["p"], 0.0, 10.0, saveat=sa, abstol=1e-8, reltol=1e-8)
+    assert g.us.dtype == np.float32
+    assert_identical(g, c, f32=True)
+
+
+@pytest.mark.parametrize("schedule", ["lane", "persistent"])
+def test_lorenz_mixed_bitexact(oracle, schedule):
+    N = 3000
+    say(f"lorenz mixed tsit5 N={N} schedule={schedule}")
+    w = workloads.lorenz(N, seed=13)
+    sa = workloads.saveat_uniform(0.0, 10.0, 100)
+    g = gpu("lorenz", w["u0"], w["p"], 0.0, 10.0, saveat=sa, abstol=1e-6, reltol=1e-6,
+            precision="mixed", schedule=schedule)
+    c = cpu(oracle, "lorenz", w["u0"], w["p"], 0.0, 10.0, saveat=sa, abstol=1e-6, reltol=1e-6,
+            precision="mixed")
+    assert_identical(g, c)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_robertson_rodas5_bitexact(oracle, precision):
+    N = 2048
+    say(f"robertson rodas5 {precision} N={N}")
+    w = workloads.robertson(N, seed=21)
+    g = gpu("robertson", w["u0"], w["p"], 0.0, 1e5, alg="rodas5", precision=precision,
+            abstol=1e-8, reltol=1e-6)
+    c = cpu(oracle, "robertson", w["u0"], w["p"], 0.0, 1e5, alg="rodas5", precision=precision,
+            abstol=1e-8, reltol=1e-6)
+    assert (c["retcode"] == 1).all()
+    assert_identical(g, c)
+    if precision == "fp64":
+        assert np.abs(g.u_final.sum(axis=0) - 1.0).max() < 1e-12
+
+
+@pytest.mark.parametrize("model,alg", [("vdp", "auto"), ("vdp", "rodas5"), ("robertson", "auto"),
+                                       ("lorenz", "auto"), ("lorenz", "rodas5")])
+def test_auto_and_rodas_models_bitexact(oracle, model, alg):
+    N = 256
+    say(f"{model} {alg} fp64 N={N}")
+    if model == "vdp":
+        w = workloads.vdp(N, seed=31)
+        w["p"] = np.minimum(w["p"], 300.0)  # keep the oracle's share short
+        tf = 2.0 * w["p"][0]
+        tol = dict(abstol=1e-8, reltol=1e-6)
+    elif model == "robertson":
+        w = workloads.robertson(N, seed=32)
+        tf = 1e5
+        tol = dict(abstol=1e-8, reltol=1e-6)
+    else:
+        w = workloads.lorenz(N, seed=33)
+        tf = 10.0
+        tol = dict(abstol=1e-7, reltol=1e-7)
+    g = gpu(model, w["u0"], w["p"], 0.0, tf, alg=alg, **tol)
+    c = cpu(oracle, model, w["u0"], w["p"], 0.0, tf, alg=alg, **tol)
+    assert_identical(g, c)
+    np.testing.assert_array_equal(g.nstiff, c["nstiff"])
+
+
+def test_linear_and_exp_models(oracle):
+    rng = np.random.default_rng(5)
+    N = 512
+    say(f"linear3 tsit5/rodas5 + exp N={N}")
+    M = rng.uniform(-1, 0.5, (9, N))
+    u0 = rng.uniform(-1, 1, (3, N))
+    for alg in ("tsit5", "rodas5"):
+        g = gpu("linear", u0, M, 0.0, 3.0, alg=alg, abstol=1e-9, reltol=1e-9)
+        c = cpu(oracle, "linear", u0, M, 0.0, 3.0, alg=alg, abstol=1e-9, reltol=1e-9)
+        assert_identical(g, c)
+    lam = rng.uniform(-2, 2, (1, N))
+    u0 = rng.uniform(0.1, 1, (1, N))
+    g = gpu("exp", u0, lam, 0.0, 1.0, abstol=1e-8, reltol=1e-8)
+    c = cpu(oracle, "exp", u0, lam, 0.0, 1.0, abstol=1e-8, reltol=1e-8)
+    assert_identical(g, c)
+    np.testing.assert_allclose(g.u_final[0], u0[0] * np.exp(lam[0]), rtol=1e-7)
+
+
+def test_retcodes_forced_from_host(oracle):
+    N = 64
+    say("retcodes maxiters/dtmin/nan")
+    w = workloads.lorenz(N, seed=41)
+    g = gpu("lorenz", w["u0"], w["p"], 0.0, 10.0, maxiters=5)
+    assert (g.retcode == 2).all() and ((g.naccept + g.nreject) == 5).all()
+    g = gpu("lorenz", w["u0"], w["p"], 0.0, 10.0, dtmin=0.5)
+    assert (g.retcode == 3).all()
+    p = w["p"].copy()
+    p[0, 3] = np.nan
+    g = gpu("lorenz", w["u0"], p, 0.0, 1.0)
+    c = cpu(oracle, "lorenz", w["u0"], p, 0.0, 1.0)
+    assert g.retcode[3] in (4, 5) and (np.delete(g.retcode, 3) == 1).all()
+    np.testing.assert_array_equal(g.retcode, c["retcode"])
+
+
+def test_per_trajectory_tspans_and_fixed_dt(oracle):
+    N = 500
+    say(f"per-trajectory tspan N={N}")
+    w = workloads.lorenz(N, seed=42)
+    rng = np.random.default_rng(1)
+    t0 = rng.uniform(0, 1, N)
+    tf = t0 + rng.uniform(0.5, 3, N)
+    g = gpu("lorenz", w["u0"], w["p"], t0, tf, abstol=1e-9, reltol=1e-9)
+    c = cpu(oracle, "lorenz", w["u0"], w["p"], t0, tf, abstol=1e-9, reltol=1e-9)
+    assert_identical(g, c)
+    np.testing.assert_array_equal(g.t_final, tf)
+    g = gpu("lorenz", w["u0"], w["p"], 0.0, 1.0, dt=1e-3, abstol=1e-9, reltol=1e-9)
+    c = cpu(oracle, "lorenz", w["u0"], w["p"], 0.0, 1.0, dt=1e-3, abstol=1e-9, reltol=1e-9)
+    assert_identical(g, c)
