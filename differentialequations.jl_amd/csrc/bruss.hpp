@@ -1,0 +1,60 @@
+// bruss.hpp -- uneven batch of 1-D periodic Brusselator instances (state
+// size n = 2*Ngrid, 16 <= n <= 128, per-instance tspan) solved by one
+// persistent kernel: a team of L lanes per instance pulls instance indices
+// from an atomic work queue.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mideq {
+
+constexpr int BRUSS_NMAX = 128;
+constexpr int BRUSS_NMIN = 16;
+
+struct BrussArgs {
+  int64_t ntraj;
+  int32_t nmax;
+  int32_t pad0;
+  const int32_t* nvec;   // [ntraj] state sizes (NULL: all nmax)
+  const double* u0;      // [nmax][ntraj]
+  const double* p;       // [3][ntraj]: A, B, alpha
+  const double* t0;
+  const double* tf;
+  double tspan0, tspan1;
+  double abstol, reltol;
+  double dt_fixed, dtmin, dtmax;
+  int64_t maxiters;
+  const double* dt0;
+  const int8_t* alg_hint;
+  int32_t qsteady_default;
+  int32_t pad1;
+  double* u_final;       // [nmax][ntraj]
+  double* t_final;
+  int8_t* retcode;
+  int32_t* naccept;
+  int32_t* nreject;
+  int32_t* nf;
+  int32_t* nstiff;
+  double* dt_first;
+  unsigned long long* queue;  // next position in `order`
+  const int64_t* order;       // work order (NULL: natural order)
+  int64_t nqueue;             // queue length (<= ntraj; a sample when autotuning)
+  double* scratch;            // Rodas5 workspace, scratch_stride doubles per team
+  int64_t scratch_stride;
+};
+
+// Per-team Rodas5 workspace: W (nmax*nmax, column-major), reciprocal pivots
+// (nmax) and pivot rows (nmax, stored as doubles).
+inline int64_t bruss_scratch_per_team(int nmax) {
+  return (int64_t)nmax * nmax + 2 * (int64_t)nmax;
+}
+
+// Resident teams of the persistent grid for (lanes, alg, precision):
+// grid blocks and teams per block.  Returns hipSuccess or an error.
+hipError_t bruss_plan(int lanes, int alg, int mix, int ncu, int64_t ntraj, int* grid,
+                      int* teams_per_block);
+hipError_t launch_bruss_solver(const BrussArgs& a, int lanes, int alg, int mix, int grid,
+                               hipStream_t s);
+
+}  // namespace mideq

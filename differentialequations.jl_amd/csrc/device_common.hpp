@@ -90,7 +90,7 @@ constexpr int kMaxStiff = 10, kMaxNonStiff = 3;
 constexpr double kStiffTol = 0.9, kDtFac = 2.0;
 
 enum : int8_t { RC_DEFAULT = 0, RC_SUCCESS = 1, RC_MAXITERS = 2, RC_DTLESSTHANMIN = 3,
-                RC_UNSTABLE = 4, RC_DTNAN = 5 };
+                RC_UNSTABLE = 4, RC_DTNAN = 5, RC_INITIALFAILURE = 6 };
 
 // ---------------------------------------------------------------------------
 // Controller power x^y = exp2(y log2 x): atanh series for ln of the mantissa

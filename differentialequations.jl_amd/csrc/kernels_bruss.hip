@@ -1,0 +1,974 @@
+// kernels_bruss.hip -- uneven Brusselator batch: one persistent kernel, a
+// team of L lanes per instance (C = 128/L state components per lane), work
+// pulled from an atomic queue in a host-chosen order.
+//
+// Per instance (state size n = 2*Ngrid, interleaved u_j, v_j):
+//   * RHS: each lane evaluates its components; neighbours come from the
+//     team's LDS slot (periodic indices).
+//   * Norms: per-lane partial sums of squares, then a butterfly tree across
+//     the team (shfl_xor), the order the CPU restatement reproduces.
+//   * Rodas5: W = J - I/(gamma dt) stored column-major in the team's
+//     scratch in HBM/L2, LU with partial pivoting parallel over rows (each
+//     matrix element sees the same fma sequence as a sequential LU),
+//     triangular solves column by column with the pivot element broadcast
+//     through LDS.
+// Stage arithmetic is FP64 or, in the mixed form, FP32 (RHS and stage
+// sums); state update, norm, time, J, W and LU are FP64.
+#include <type_traits>
+
+#include "bruss.hpp"
+#include "device_common.hpp"
+#include "lane_solver.hpp"
+
+namespace mideq {
+
+// Make this wave's earlier global/LDS writes visible to the other lanes of
+// the wave and keep the compiler from moving memory operations across.
+__device__ __forceinline__ void team_sync() {
+  __threadfence_block();
+  __builtin_amdgcn_wave_barrier();
+}
+
+template <int L>
+__device__ __forceinline__ double team_sum(double a) {
+#pragma unroll
+  for (int w = L / 2; w >= 1; w >>= 1) a = a + __shfl_xor(a, w, L);
+  return a;
+}
+
+template <int L>
+__device__ __forceinline__ double team_max(double a) {
+#pragma unroll
+  for (int w = L / 2; w >= 1; w >>= 1) {
+    double o = __shfl_xor(a, w, L);
+    a = (o > a) ? o : a;
+  }
+  return a;
+}
+
+template <int L>
+__device__ __forceinline__ int team_or(int v) {
+#pragma unroll
+  for (int w = L / 2; w >= 1; w >>= 1) v |= __shfl_xor(v, w, L);
+  return v;
+}
+
+template <int L>
+__device__ __forceinline__ int team_and(int v) {
+#pragma unroll
+  for (int w = L / 2; w >= 1; w >>= 1) v &= __shfl_xor(v, w, L);
+  return v;
+}
+
+template <int L, int C, bool MIX, int ALG>
+struct BTeam {
+  static constexpr bool kAuto = (ALG == ALG_AUTO);
+  static constexpr bool kHasTsit5 = (ALG != ALG_RODAS5);
+  static constexpr bool kHasRodas = (ALG != ALG_TSIT5);
+
+  int tl;
+  int64_t i;
+  int n, N;
+  double A, B, B1, D;
+  float Af, Bf, B1f, Df;
+  double u[C];
+  double k1[C];
+  float k1f[C];
+  double t, tcomp, tf, dt, dtmax, qold, qoldpow, qsteady_max, eig, dtfirst;
+  int64_t iter;
+  int32_t naccept, nreject, nf, nstiff, count, nanrej;
+  int8_t rc, mode;
+  double* ldsd;  // team LDS: exchange vector (BRUSS_NMAX doubles)
+  double* ldsr;  // team LDS: reciprocal pivots
+  float* ldsf;   // team LDS: FP32 exchange vector
+  int* ldsi;     // team LDS: pivot rows
+  double* W;     // team scratch: W column-major
+
+  __device__ __forceinline__ int comp(int m) const { return tl + L * m; }
+
+  // ---------------------------------------------------------------- RHS
+  __device__ void rhs64(double* out, const double* in) {
+#pragma unroll
+    for (int m = 0; m < C; m++) {
+      int c = comp(m);
+      if (c < n) ldsd[c] = in[m];
+    }
+    team_sync();
+#pragma unroll
+    for (int m = 0; m < C; m++) {
+      int c = comp(m);
+      double r = 0.0;
+      if (c < n) {
+        int j = c >> 1;
+        int jm = (j == 0) ? N - 1 : j - 1;
+        int jp = (j == N - 1) ? 0 : j + 1;
+        if ((c & 1) == 0) {
+          double uu = in[m], vv = ldsd[c + 1];
+          double uuv = uu * uu * vv;
+          double lu = ldsd[2 * jm] - 2.0 * uu + ldsd[2 * jp];
+          r = A + uuv - B1 * uu + D * lu;
+        } else {
+          double uu = ldsd[c - 1], vv = in[m];
+          double uuv = uu * uu * vv;
+          double lv = ldsd[2 * jm + 1] - 2.0 * vv + ldsd[2 * jp + 1];
+          r = B * uu - uuv + D * lv;
+        }
+      }
+      out[m] = r;
+    }
+    team_sync();
+  }
+
+  __device__ void rhs32(float* out, const float* in) {
+#pragma unroll
+    for (int m = 0; m < C; m++) {
+      int c = comp(m);
+      if (c < n) ldsf[c] = in[m];
+    }
+    team_sync();
+#pragma unroll
+    for (int m = 0; m < C; m++) {
+      int c = comp(m);
+      float r = 0.0f;
+      if (c < n) {
+        int j = c >> 1;
+        int jm = (j == 0) ? N - 1 : j - 1;
+        int jp = (j == N - 1) ? 0 : j + 1;
+        if ((c & 1) == 0) {
+          float uu = in[m], vv = ldsf[c + 1];
+          float uuv = uu * uu * vv;
+          float lu = ldsf[2 * jm] - 2.0f * uu + ldsf[2 * jp];
+          r = Af + uuv - B1f * uu + Df * lu;
+        } else {
+          float uu = ldsf[c - 1], vv = in[m];
+          float uuv = uu * uu * vv;
+          float lv = ldsf[2 * jm + 1] - 2.0f * vv + ldsf[2 * jp + 1];
+          r = Bf * uu - uuv + Df * lv;
+        }
+      }
+      out[m] = r;
+    }
+    team_sync();
+  }
+
+  __device__ void fsal_init() {
+    rhs64(k1, u);
+    if (MIX) {
+      float uf[C];
+#pragma unroll
+      for (int m = 0; m < C; m++) uf[m] = (float)u[m];
+      rhs32(k1f, uf);
+    }
+  }
+
+  // RMS of owned values x[m]/sc[m] style partial sums -> sqrt(sum/n)
+  __device__ __forceinline__ double rms_from_partial(double part) {
+    return sqrt(team_sum<L>(part) / (double)n);
+  }
+
+  // ---------------------------------------------------------------- initdt
+  __device__ double initdt(const BrussArgs& a, double t0) {
+    double dtmin_e = nextafter(fmax(a.dtmin, jl_eps(t0)), __builtin_inf());
+    double smalldt = fmax(dtmin_e, 1e-6);
+    double sk[C];
+    double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+    for (int m = 0; m < C; m++) {
+      sk[m] = 1.0;
+      if (comp(m) < n) {
+        sk[m] = fma(fabs(u[m]), a.reltol, a.abstol);
+        double x = u[m] / sk[m];
+        s0 = s0 + x * x;
+        double y = k1[m] / sk[m];
+        s1 = s1 + y * y;
+      }
+    }
+    double d0 = rms_from_partial(s0);
+    double d1 = rms_from_partial(s1);
+    if (d1 != d1) return dtmin_e;
+    double dt0 = (d0 < 1e-5 || d1 < 1e-5) ? smalldt : (d0 / d1) / 100.0;
+    dt0 = fmin(dt0, dtmax);
+    if (dt0 < 10.0 * 2.220446049250313e-16) return smalldt;
+    double u1[C], f1[C];
+#pragma unroll
+    for (int m = 0; m < C; m++) u1[m] = fma(dt0, k1[m], u[m]);
+    rhs64(f1, u1);
+    nf += 1;
+    int same = 1;
+#pragma unroll
+    for (int m = 0; m < C; m++)
+      if (comp(m) < n) same &= (k1[m] == f1[m]) ? 1 : 0;
+    if (team_and<L>(same)) return fmax(dtmin_e, 100.0 * dt0);
+    double s2 = 0.0;
+#pragma unroll
+    for (int m = 0; m < C; m++) {
+      if (comp(m) < n) {
+        double x = (f1[m] - k1[m]) / sk[m];
+        s2 = s2 + x * x;
+      }
+    }
+    double d2 = rms_from_partial(s2) / dt0;
+    double mx = fmax(d1, d2);
+    double dt1;
+    if (mx <= 1e-15)
+      dt1 = fmax(1e-6, dt0 * 1e-3);
+    else
+      dt1 = ctl_exp2(-(2.0 * 3.32192809488736234787 + ctl_log2(mx)) / 5.0);
+    return fmax(dtmin_e, fmin(fmin(100.0 * dt0, dt1), dtmax));
+  }
+
+  // ---------------------------------------------------------------- init
+  __device__ void init(const BrussArgs& a, int64_t idx) {
+    i = idx;
+    const int64_t NT = a.ntraj;
+    n = a.nvec ? a.nvec[i] : a.nmax;
+    N = n / 2;
+    A = a.p[i];
+    B = a.p[NT + i];
+    double al = a.p[2 * NT + i];
+    D = al * (double)(N * N);
+    B1 = B + 1.0;
+    Af = (float)A;
+    Bf = (float)B;
+    float alf = (float)al;
+    Df = alf * (float)(N * N);
+    B1f = Bf + 1.0f;
+    double t0 = a.t0 ? a.t0[i] : a.tspan0;
+    tf = a.tf ? a.tf[i] : a.tspan1;
+    t = t0;
+    tcomp = 0.0;
+    dtmax = a.dtmax > 0 ? a.dtmax : fabs(tf - t0);
+    naccept = nreject = nstiff = 0;
+    nf = 1;
+    iter = 0;
+    count = 0;
+    nanrej = 0;
+    eig = 0.0;
+    dtfirst = 0.0;
+    rc = RC_DEFAULT;
+    mode = (ALG == ALG_RODAS5) ? 1 : 0;
+    if (kAuto && a.alg_hint) mode = a.alg_hint[i] ? 1 : 0;
+    qsteady_max = (a.qsteady_default && mode == 1) ? 1.2 : 1.0;
+    if (n < BRUSS_NMIN || n > a.nmax || (n & 1)) {
+      rc = RC_INITIALFAILURE;
+      n = 0;
+      return;
+    }
+#pragma unroll
+    for (int m = 0; m < C; m++) {
+      int c = comp(m);
+      u[m] = (c < n) ? a.u0[(int64_t)c * NT + i] : 0.0;
+    }
+    fsal_init();
+    if (a.dt_fixed > 0)
+      dt = a.dt_fixed;
+    else if (a.dt0)
+      dt = a.dt0[i];
+    else
+      dt = initdt(a, t0);
+    qold = kQoldInit;
+    qoldpow = ctl_pow(kQoldInit, kBeta2);
+    if (!(tf > t0)) rc = RC_SUCCESS;
+  }
+
+  // ---------------------------------------------------------------- Tsit5
+  __device__ double err_norm(const double* ut, const double* u1, double abstol,
+                             double reltol) {
+    double s = 0.0;
+#pragma unroll
+    for (int m = 0; m < C; m++) {
+      if (comp(m) < n) {
+        double sc = fma(fmax(fabs(u[m]), fabs(u1[m])), reltol, abstol);
+        double r = ut[m] / sc;
+        s = s + r * r;
+      }
+    }
+    return rms_from_partial(s);
+  }
+
+  __device__ void eig_estimate(const double* k7, const double* k6, const double* u1,
+                               const double* g6) {
+    double e = 0.0;
+    int bad = 0;
+#pragma unroll
+    for (int m = 0; m < C; m++) {
+      if (comp(m) < n) {
+        double r = fabs((k7[m] - k6[m]) / (u1[m] - g6[m]));
+        bad |= (r != r) ? 1 : 0;
+        e = (r > e) ? r : e;
+      }
+    }
+    e = team_max<L>(e);
+    eig = team_or<L>(bad) ? __builtin_nan("") : e;
+  }
+
+  __device__ double tsit5_step64(double dts, double* u1, double abstol, double reltol) {
+    using namespace ts;
+    double k[7][C];
+    double g[C], g6[C], ut[C];
+    double aa = dts * a21;
+#pragma unroll
+    for (int m = 0; m < C; m++) {
+      k[0][m] = k1[m];
+      g[m] = fma(aa, k[0][m], u[m]);
+    }
+    rhs64(k[1], g);
+#pragma unroll
+    for (int m = 0; m < C; m++) {
+      double acc = a32 * k[1][m];
+      acc = fma(a31, k[0][m], acc);
+      g[m] = fma(dts, acc, u[m]);
+    }
+    rhs64(k[2], g);
+#pragma unroll
+    for (int m = 0; m < C; m++) {
+      double acc = a43 * k[2][m];
+      acc = fma(a42, k[1][m], acc);
+      acc = fma(a41, k[0][m], acc);
+      g[m] = fma(dts, acc, u[m]);
+    }
+    rhs64(k[3], g);
+#pragma unroll
+    for (int m = 0; m < C; m++) {
+      double acc = a54 * k[3][m];
+      acc = fma(a53, k[2][m], acc);
+      acc = fma(a52, k[1][m], acc);
+      acc = fma(a51, k[0][m], acc);
+      g[m] = fma(dts, acc, u[m]);
+    }
+    rhs64(k[4], g);
+#pragma unroll
+    for (int m = 0; m < C; m++) {
+      double acc = a65 * k[4][m];
+      acc = fma(a64, k[3][m], acc);
+      acc = fma(a63, k[2][m], acc);
+      acc = fma(a62, k[1][m], acc);
+      acc = fma(a61, k[0][m], acc);
+      g6[m] = fma(dts, acc, u[m]);
+    }
+    rhs64(k[5], g6);
+#pragma unroll
+    for (int m = 0; m < C; m++) {
+      double acc = a76 * k[5][m];
+      acc = fma(a75, k[4][m], acc);
+      acc = fma(a74, k[3][m], acc);
+      acc = fma(a73, k[2][m], acc);
+      acc = fma(a72, k[1][m], acc);
+      acc = fma(a71, k[0][m], acc);
+      u1[m] = fma(dts, acc, u[m]);
+    }
+    rhs64(k[6], u1);
+    if (kAuto) eig_estimate(k[6], k[5], u1, g6);
+#pragma unroll
+    for (int m = 0; m < C; m++) {
+      double acc = bt7 * k[6][m];
+      acc = fma(bt6, k[5][m], acc);
+      acc = fma(bt5, k[4][m], acc);
+      acc = fma(bt4, k[3][m], acc);
+      acc = fma(bt3, k[2][m], acc);
+      acc = fma(bt2, k[1][m], acc);
+      acc = fma(bt1, k[0][m], acc);
+      ut[m] = dts * acc;
+    }
+#pragma unroll
+    for (int m = 0; m < C; m++) k1[m] = k[6][m];  // FSAL candidate (restored on reject)
+    return err_norm(ut, u1, abstol, reltol);
+  }
+
+  __device__ double tsit5_step_mix(double dts, double* u1, float* k7out, double abstol,
+                                   double reltol) {
+    using namespace ts;
+    float k[7][C];
+    float u32[C], g[C], g6[C], u1f[C];
+    double ut[C];
+    float d32 = (float)dts;
+#pragma unroll
+    for (int m = 0; m < C; m++) {
+      u32[m] = (float)u[m];
+      k[0][m] = k1f[m];
+    }
+    float aa = d32 * (float)a21;
+#pragma unroll
+    for (int m = 0; m < C; m++) g[m] = fmaf(aa, k[0][m], u32[m]);
+    rhs32(k[1], g);
+#pragma unroll
+    for (int m = 0; m < C; m++) {
+      float acc = (float)a32 * k[1][m];
+      acc = fmaf((float)a31, k[0][m], acc);
+      g[m] = fmaf(d32, acc, u32[m]);
+    }
+    rhs32(k[2], g);
+#pragma unroll
+    for (int m = 0; m < C; m++) {
+      float acc = (float)a43 * k[2][m];
+      acc = fmaf((float)a42, k[1][m], acc);
+      acc = fmaf((float)a41, k[0][m], acc);
+      g[m] = fmaf(d32, acc, u32[m]);
+    }
+    rhs32(k[3], g);
+#pragma unroll
+    for (int m = 0; m < C; m++) {
+      float acc = (float)a54 * k[3][m];
+      acc = fmaf((float)a53, k[2][m], acc);
+      acc = fmaf((float)a52, k[1][m], acc);
+      acc = fmaf((float)a51, k[0][m], acc);
+      g[m] = fmaf(d32, acc, u32[m]);
+    }
+    rhs32(k[4], g);
+#pragma unroll
+    for (int m = 0; m < C; m++) {
+      float acc = (float)a65 * k[4][m];
+      acc = fmaf((float)a64, k[3][m], acc);
+      acc = fmaf((float)a63, k[2][m], acc);
+      acc = fmaf((float)a62, k[1][m], acc);
+      acc = fmaf((float)a61, k[0][m], acc);
+      g6[m] = fmaf(d32, acc, u32[m]);
+    }
+    rhs32(k[5], g6);
+#pragma unroll
+    for (int m = 0; m < C; m++) {
+      float acc = (float)a76 * k[5][m];
+      acc = fmaf((float)a75, k[4][m], acc);
+      acc = fmaf((float)a74, k[3][m], acc);
+      acc = fmaf((float)a73, k[2][m], acc);
+      acc = fmaf((float)a72, k[1][m], acc);
+      acc = fmaf((float)a71, k[0][m], acc);
+      u1[m] = fma(dts, (double)acc, u[m]);
+      u1f[m] = (float)u1[m];
+    }
+    rhs32(k[6], u1f);
+    if (kAuto) {
+      double k7d[C], k6d[C], g6d[C];
+#pragma unroll
+      for (int m = 0; m < C; m++) {
+        k7d[m] = (double)k[6][m];
+        k6d[m] = (double)k[5][m];
+        g6d[m] = (double)g6[m];
+      }
+      eig_estimate(k7d, k6d, u1, g6d);
+    }
+#pragma unroll
+    for (int m = 0; m < C; m++) {
+      float acc = (float)bt7 * k[6][m];
+      acc = fmaf((float)bt6, k[5][m], acc);
+      acc = fmaf((float)bt5, k[4][m], acc);
+      acc = fmaf((float)bt4, k[3][m], acc);
+      acc = fmaf((float)bt3, k[2][m], acc);
+      acc = fmaf((float)bt2, k[1][m], acc);
+      acc = fmaf((float)bt1, k[0][m], acc);
+      ut[m] = dts * (double)acc;
+      k7out[m] = k[6][m];
+    }
+    return err_norm(ut, u1, abstol, reltol);
+  }
+
+  // ---------------------------------------------------------------- Rodas5
+  // Column-major W in scratch: element (r, c) at W[c*n + r].
+  __device__ bool factor_W(double dts) {
+    // J at u: owned rows r; zero the row, then its four nonzeros
+#pragma unroll
+    for (int m = 0; m < C; m++) {
+      int c = comp(m);
+      if (c < n) ldsd[c] = u[m];
+    }
+    team_sync();
+#pragma unroll
+    for (int m = 0; m < C; m++) {
+      int r = comp(m);
+      if (r < n) {
+        for (int c = 0; c < n; c++) W[(int64_t)c * n + r] = 0.0;
+        int j = r >> 1;
+        int jm = (j == 0) ? N - 1 : j - 1;
+        int jp = (j == N - 1) ? 0 : j + 1;
+        double uu = ldsd[2 * j], vv = ldsd[2 * j + 1];
+        double uv2 = 2.0 * uu * vv, u2 = uu * uu;
+        int ru = 2 * j, rv = 2 * j + 1;
+        if ((r & 1) == 0) {
+          W[(int64_t)ru * n + ru] = 0.0 + (uv2 - B1 - 2.0 * D);
+          W[(int64_t)rv * n + ru] = 0.0 + u2;
+          W[(int64_t)(2 * jm) * n + ru] = 0.0 + D;
+          W[(int64_t)(2 * jp) * n + ru] = 0.0 + D;
+        } else {
+          W[(int64_t)ru * n + rv] = 0.0 + (B - uv2);
+          W[(int64_t)rv * n + rv] = 0.0 + (-u2 - 2.0 * D);
+          W[(int64_t)(2 * jm + 1) * n + rv] = 0.0 + D;
+          W[(int64_t)(2 * jp + 1) * n + rv] = 0.0 + D;
+        }
+      }
+    }
+    team_sync();
+    if (kAuto) {
+      double best = 0.0;
+#pragma unroll
+      for (int m = 0; m < C; m++) {
+        int r = comp(m);
+        if (r < n) {
+          double s = 0.0;
+          for (int c = 0; c < n; c++) s += fabs(W[(int64_t)c * n + r]);
+          best = (s > best) ? s : best;
+        }
+      }
+      eig = team_max<L>(best);
+    }
+    double dtgamma = dts * rd::gamma;
+    double w = 1.0 / dtgamma;
+#pragma unroll
+    for (int m = 0; m < C; m++) {
+      int r = comp(m);
+      if (r < n) W[(int64_t)r * n + r] = W[(int64_t)r * n + r] - w;
+    }
+    team_sync();
+    // LU with partial pivoting (first maximal |pivot|), rows in parallel
+    int singular = 0;
+    for (int k = 0; k < n; k++) {
+      double bv = -1.0;
+      int bi = n;
+#pragma unroll
+      for (int m = 0; m < C; m++) {
+        int r = comp(m);
+        if (r >= k && r < n) {
+          double v = fabs(W[(int64_t)k * n + r]);
+          if (v > bv) {
+            bv = v;
+            bi = r;
+          }
+        }
+      }
+#pragma unroll
+      for (int s = L / 2; s >= 1; s >>= 1) {
+        double ov = __shfl_xor(bv, s, L);
+        int oi = __shfl_xor(bi, s, L);
+        bool take = (ov > bv) || (ov == bv && oi < bi);
+        bv = take ? ov : bv;
+        bi = take ? oi : bi;
+      }
+      // no comparable pivot (NaN column): keep row k, flag singular
+      if (bi >= n) {
+        bi = k;
+        singular = 1;
+      }
+      int pk = bi;
+      if (pk != k) {
+#pragma unroll
+        for (int m = 0; m < C; m++) {
+          int c = comp(m);
+          if (c < n) {
+            double x = W[(int64_t)c * n + k], y = W[(int64_t)c * n + pk];
+            W[(int64_t)c * n + k] = y;
+            W[(int64_t)c * n + pk] = x;
+          }
+        }
+        team_sync();
+      }
+      double d = W[(int64_t)k * n + k];
+      singular |= (d == 0.0) ? 1 : 0;
+      double rr = 1.0 / d;
+      if (tl == 0) {
+        ldsr[k] = rr;
+        ldsi[k] = pk;
+      }
+#pragma unroll
+      for (int m = 0; m < C; m++) {
+        int r = comp(m);
+        if (r > k && r < n) {
+          double l = W[(int64_t)k * n + r] * rr;
+          W[(int64_t)k * n + r] = l;
+          for (int c = k + 1; c < n; c++)
+            W[(int64_t)c * n + r] = fma(-l, W[(int64_t)c * n + k], W[(int64_t)c * n + r]);
+        }
+      }
+      team_sync();
+    }
+    return singular == 0;
+  }
+
+  // Solve W x = b for owned components (x in/out); pivots applied as the
+  // composite permutation sigma (value at position r comes from sigma[r]).
+  __device__ void lu_solve(double* x, const int* sigma) {
+#pragma unroll
+    for (int m = 0; m < C; m++) {
+      int c = comp(m);
+      if (c < n) ldsd[c] = x[m];
+    }
+    team_sync();
+    double y[C];
+#pragma unroll
+    for (int m = 0; m < C; m++) y[m] = (comp(m) < n) ? ldsd[sigma[m]] : 0.0;
+    team_sync();
+    for (int j = 0; j < n; j++) {
+#pragma unroll
+      for (int m = 0; m < C; m++)
+        if (comp(m) == j) ldsd[j] = y[m];
+      team_sync();
+      double xj = ldsd[j];
+#pragma unroll
+      for (int m = 0; m < C; m++) {
+        int r = comp(m);
+        if (r > j && r < n) y[m] = fma(-W[(int64_t)j * n + r], xj, y[m]);
+      }
+    }
+    for (int j = n - 1; j >= 0; j--) {
+#pragma unroll
+      for (int m = 0; m < C; m++) {
+        if (comp(m) == j) {
+          y[m] = y[m] * ldsr[j];
+          ldsd[j] = y[m];
+        }
+      }
+      team_sync();
+      double xj = ldsd[j];
+#pragma unroll
+      for (int m = 0; m < C; m++) {
+        int r = comp(m);
+        if (r < j) y[m] = fma(-W[(int64_t)j * n + r], xj, y[m]);
+      }
+    }
+    team_sync();
+#pragma unroll
+    for (int m = 0; m < C; m++) x[m] = y[m];
+  }
+
+  __device__ bool rodas5_step(double dts, double* u1, double& EEst, double abstol,
+                              double reltol) {
+    if (!factor_W(dts)) {
+      EEst = 2.0;
+      return false;
+    }
+    int sigma[C];
+#pragma unroll
+    for (int m = 0; m < C; m++) {
+      int pos = comp(m);
+      for (int k = n - 1; k >= 0; k--) {
+        int pk = ldsi[k];
+        if (pos == k)
+          pos = pk;
+        else if (pos == pk)
+          pos = k;
+      }
+      sigma[m] = pos;
+    }
+    double inv_dt = 1.0 / dts;
+    double kd[8][C];
+    double U[C], du[C], x[C];
+    float Uf[C], duf[C];
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+      if (!MIX) {
+        if (s == 0) {
+#pragma unroll
+          for (int m = 0; m < C; m++) U[m] = u[m];
+        } else if (s <= 5) {
+#pragma unroll
+          for (int m = 0; m < C; m++) {
+            double acc = u[m];
+#pragma unroll
+            for (int j = 0; j < s; j++) acc = fma(rd::A[s][j], kd[j][m], acc);
+            U[m] = acc;
+          }
+        } else {
+#pragma unroll
+          for (int m = 0; m < C; m++) U[m] = U[m] + kd[s - 1][m];
+        }
+        rhs64(du, U);
+      } else {
+        if (s == 0) {
+#pragma unroll
+          for (int m = 0; m < C; m++) Uf[m] = (float)u[m];
+        } else if (s <= 5) {
+#pragma unroll
+          for (int m = 0; m < C; m++) {
+            float acc = (float)u[m];
+#pragma unroll
+            for (int j = 0; j < s; j++) acc = fmaf((float)rd::A[s][j], (float)kd[j][m], acc);
+            Uf[m] = acc;
+          }
+        } else {
+#pragma unroll
+          for (int m = 0; m < C; m++) Uf[m] = Uf[m] + (float)kd[s - 1][m];
+        }
+        rhs32(duf, Uf);
+#pragma unroll
+        for (int m = 0; m < C; m++) du[m] = (double)duf[m];
+      }
+#pragma unroll
+      for (int m = 0; m < C; m++) {
+        double r;
+        if (s == 0) {
+          r = du[m];
+        } else if (!MIX) {
+          double acc = (rd::C[s][0] * inv_dt) * kd[0][m];
+#pragma unroll
+          for (int j = 1; j < s; j++) acc = fma(rd::C[s][j] * inv_dt, kd[j][m], acc);
+          r = du[m] + acc;
+        } else {
+          float idt = (float)inv_dt;
+          float acc = ((float)rd::C[s][0] * idt) * (float)kd[0][m];
+#pragma unroll
+          for (int j = 1; j < s; j++) acc = fmaf((float)rd::C[s][j] * idt, (float)kd[j][m], acc);
+          r = (double)((float)du[m] + acc);
+        }
+        x[m] = -r;
+      }
+      lu_solve(x, sigma);
+#pragma unroll
+      for (int m = 0; m < C; m++) kd[s][m] = MIX ? (double)(float)x[m] : x[m];
+    }
+#pragma unroll
+    for (int m = 0; m < C; m++) {
+      double acc = u[m];
+#pragma unroll
+      for (int j = 0; j < 5; j++) acc = fma(rd::A[5][j], kd[j][m], acc);
+      acc = acc + kd[5][m];
+      acc = acc + kd[6][m];
+      u1[m] = acc + kd[7][m];
+    }
+    EEst = err_norm(kd[7], u1, abstol, reltol);
+    return true;
+  }
+
+  // ---------------------------------------------------------------- step
+  __device__ void step(const BrussArgs& a) {
+    if (kAuto && iter > 0) {
+      double stiffness = fabs(eig * dt / kStabSize);
+      bool stiff = stiffness > kStiffTol;
+      count = stiff ? (count < 0 ? 1 : count + 1) : (count > 0 ? -1 : count - 1);
+      if (mode == 0 && count > kMaxStiff) {
+        dt = dt * kDtFac;
+        mode = 1;
+        qsteady_max = a.qsteady_default ? 1.2 : 1.0;
+      } else if (mode == 1 && count < -kMaxNonStiff) {
+        dt = dt / kDtFac;
+        mode = 0;
+        qsteady_max = 1.0;
+        fsal_init();
+        nf += 1;
+      }
+    }
+    iter++;
+    int64_t maxit = a.maxiters > 0 ? a.maxiters : 100000;
+    if (iter > maxit) {
+      rc = RC_MAXITERS;
+      return;
+    }
+    dt = fmin(dt, dtmax);
+    if (a.dtmin > 0 && dt < a.dtmin) {
+      rc = RC_DTLESSTHANMIN;
+      return;
+    }
+    dt = fmin(dt, tf - t);
+
+    double u1[C];
+    double EEst = 0.0;
+    double k1save[C];
+    float k7f[C];
+    bool tsit_step = kHasTsit5 && (!kHasRodas || mode == 0);
+    if (tsit_step) {
+      if constexpr (kHasTsit5) {
+#pragma unroll
+        for (int m = 0; m < C; m++) k1save[m] = k1[m];
+        if constexpr (!MIX)
+          EEst = tsit5_step64(dt, u1, a.abstol, a.reltol);
+        else
+          EEst = tsit5_step_mix(dt, u1, k7f, a.abstol, a.reltol);
+      }
+      nf += 6;
+    } else {
+      if constexpr (kHasRodas) {
+        if (!rodas5_step(dt, u1, EEst, a.abstol, a.reltol)) EEst = 2.0;
+      }
+      nf += 8;
+    }
+    if (EEst != EEst) {
+      if (!MIX || ++nanrej > 20) {
+        rc = RC_DTNAN;
+        return;
+      }
+      EEst = __builtin_inf();
+    } else {
+      nanrej = 0;
+    }
+    double q, q11 = 1.0, lE = 0.0;
+    if (EEst == 0.0) {
+      q = 1.0 / kQmax;
+    } else {
+      lE = ctl_log2(EEst);
+      q11 = ctl_exp2(kBeta1 * lE);
+      q = q11 / qoldpow;
+      q = fmax(1.0 / kQmax, fmin(1.0 / kQmin, q / kGammaSafety));
+    }
+    if (EEst <= 1.0) {
+      if (naccept == 0) dtfirst = dt;
+      naccept++;
+      if (!tsit_step) nstiff++;
+      if (1.0 <= q && q <= qsteady_max) q = 1.0;
+      qold = fmax(EEst, kQoldInit);
+      qoldpow = (qold == EEst) ? ctl_exp2(kBeta2 * lE) : ctl_pow(kQoldInit, kBeta2);
+      double dtnew = dt / q;
+      double ttmp;
+      if (!MIX) {
+        ttmp = t + dt;
+      } else {
+        double y = dt - tcomp;
+        ttmp = t + y;
+        tcomp = (ttmp - t) - y;
+      }
+      if (fabs(ttmp - tf) < 100.0 * jl_eps(fmax(t, tf))) ttmp = tf;
+      t = ttmp;
+#pragma unroll
+      for (int m = 0; m < C; m++) u[m] = u1[m];
+      if (MIX && tsit_step) {
+#pragma unroll
+        for (int m = 0; m < C; m++) k1f[m] = k7f[m];
+      }
+      int bad = 0;
+#pragma unroll
+      for (int m = 0; m < C; m++)
+        if (comp(m) < n) bad |= (u1[m] != u1[m]) ? 1 : 0;
+      if (team_or<L>(bad)) {
+        rc = RC_UNSTABLE;
+        return;
+      }
+      dt = fmin(dtmax, dtnew);
+    } else {
+      nreject++;
+      if (tsit_step && !MIX) {
+#pragma unroll
+        for (int m = 0; m < C; m++) k1[m] = k1save[m];
+      }
+      dt = dt / fmin(1.0 / kQmin, q11 / kGammaSafety);
+    }
+    if (!(t < tf)) rc = RC_SUCCESS;
+  }
+
+  __device__ void finish(const BrussArgs& a) {
+    const int64_t NT = a.ntraj;
+    if (a.u_final)
+#pragma unroll
+      for (int m = 0; m < C; m++) {
+        int c = comp(m);
+        if (c < n) a.u_final[(int64_t)c * NT + i] = u[m];
+      }
+    if (tl == 0) {
+      if (a.t_final) a.t_final[i] = t;
+      if (a.retcode) a.retcode[i] = rc;
+      if (a.naccept) a.naccept[i] = naccept;
+      if (a.nreject) a.nreject[i] = nreject;
+      if (a.nf) a.nf[i] = nf;
+      if (a.nstiff) a.nstiff[i] = nstiff;
+      if (a.dt_first) a.dt_first[i] = dtfirst;
+    }
+  }
+};
+
+template <int L, bool MIX, int ALG>
+__global__ __launch_bounds__(256) void k_bruss(BrussArgs a) {
+  constexpr int C = BRUSS_NMAX / L;
+  constexpr int TPB = 256 / L;
+  __shared__ double s_d[TPB][BRUSS_NMAX];
+  __shared__ double s_r[TPB][BRUSS_NMAX];
+  __shared__ float s_f[TPB][BRUSS_NMAX];
+  __shared__ int s_i[TPB][BRUSS_NMAX];
+  const int team = threadIdx.x / L;
+  BTeam<L, C, MIX, ALG> T;
+  T.tl = threadIdx.x % L;
+  T.ldsd = s_d[team];
+  T.ldsr = s_r[team];
+  T.ldsf = s_f[team];
+  T.ldsi = s_i[team];
+  const int64_t gteam = (int64_t)blockIdx.x * TPB + team;
+  T.W = a.scratch ? a.scratch + gteam * a.scratch_stride : nullptr;
+  bool active = false, drained = false;
+  while (true) {
+    if (!active && !drained) {
+      unsigned long long q = 0;
+      if (T.tl == 0) q = atomicAdd(a.queue, 1ull);
+      q = __shfl(q, 0, L);
+      if (q >= (unsigned long long)a.nqueue) {
+        drained = true;
+      } else {
+        int64_t idx = a.order ? a.order[q] : (int64_t)q;
+        T.init(a, idx);
+        if (T.rc == RC_DEFAULT)
+          active = true;
+        else
+          T.finish(a);
+      }
+    }
+    if (!__any(active || !drained)) break;
+    if (active) {
+      T.step(a);
+      if (T.rc != RC_DEFAULT) {
+        T.finish(a);
+        active = false;
+      }
+    }
+  }
+}
+
+__global__ void k_bruss_reset(unsigned long long* q) { *q = 0ull; }
+
+template <int L, bool MIX, int ALG>
+static hipError_t occupancy(int* per_cu) {
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_bruss<L, MIX, ALG>, 256, 0);
+}
+
+template <int L>
+static hipError_t occ_dispatch(int alg, int mix, int* per_cu) {
+  if (mix) {
+    if (alg == ALG_TSIT5) return occupancy<L, true, ALG_TSIT5>(per_cu);
+    if (alg == ALG_RODAS5) return occupancy<L, true, ALG_RODAS5>(per_cu);
+    return occupancy<L, true, ALG_AUTO>(per_cu);
+  }
+  if (alg == ALG_TSIT5) return occupancy<L, false, ALG_TSIT5>(per_cu);
+  if (alg == ALG_RODAS5) return occupancy<L, false, ALG_RODAS5>(per_cu);
+  return occupancy<L, false, ALG_AUTO>(per_cu);
+}
+
+hipError_t bruss_plan(int lanes, int alg, int mix, int ncu, int64_t ntraj, int* grid,
+                      int* teams_per_block) {
+  int per_cu = 0;
+  hipError_t e;
+  if (lanes == 32)
+    e = occ_dispatch<32>(alg, mix, &per_cu);
+  else if (lanes == 64)
+    e = occ_dispatch<64>(alg, mix, &per_cu);
+  else
+    return hipErrorInvalidValue;
+  if (e != hipSuccess) return e;
+  if (per_cu < 1) per_cu = 1;
+  int tpb = 256 / lanes;
+  int64_t need = (ntraj + tpb - 1) / tpb;
+  int64_t g = (int64_t)per_cu * ncu;
+  *grid = (int)(g < need ? g : need);
+  if (*grid < 1) *grid = 1;
+  *teams_per_block = tpb;
+  return hipSuccess;
+}
+
+template <int L, bool MIX, int ALG>
+static hipError_t launch3(const BrussArgs& a, int grid, hipStream_t s) {
+  hipLaunchKernelGGL((k_bruss<L, MIX, ALG>), dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+template <int L>
+static hipError_t launch2(const BrussArgs& a, int alg, int mix, int grid, hipStream_t s) {
+  if (mix) {
+    if (alg == ALG_TSIT5) return launch3<L, true, ALG_TSIT5>(a, grid, s);
+    if (alg == ALG_RODAS5) return launch3<L, true, ALG_RODAS5>(a, grid, s);
+    return launch3<L, true, ALG_AUTO>(a, grid, s);
+  }
+  if (alg == ALG_TSIT5) return launch3<L, false, ALG_TSIT5>(a, grid, s);
+  if (alg == ALG_RODAS5) return launch3<L, false, ALG_RODAS5>(a, grid, s);
+  return launch3<L, false, ALG_AUTO>(a, grid, s);
+}
+
+hipError_t launch_bruss_solver(const BrussArgs& a, int lanes, int alg, int mix, int grid,
+                               hipStream_t s) {
+  hipLaunchKernelGGL(k_bruss_reset, dim3(1), dim3(1), 0, s, a.queue);
+  if (lanes == 32) return launch2<32>(a, alg, mix, grid, s);
+  if (lanes == 64) return launch2<64>(a, alg, mix, grid, s);
+  return hipErrorInvalidValue;
+}
+
+}  // namespace mideq

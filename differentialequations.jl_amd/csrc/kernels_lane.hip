@@ -82,7 +82,7 @@ static hipError_t launch_t(const LaneArgs& a, int sched, int block, int grid,
       if (e != hipSuccess || per_cu < 1) per_cu = 1;
       int dev = 0, ncu = 256;
       if (hipGetDevice(&dev) == hipSuccess)
-        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) ncu = 256;
       int64_t need = (a.ntraj + block - 1) / block;
       int64_t g = (int64_t)per_cu * ncu;
       grid = (int)(g < need ? g : need);

@@ -40,6 +40,7 @@ struct LaneArgs {
   int32_t* nreject;
   int32_t* nf;
   int32_t* nstiff;
+  double* dt_first;
   unsigned long long* queue;  // persistent schedule: next trajectory index
 };
 
@@ -98,7 +99,7 @@ struct Lane {
   double k1[N];   // FSAL stage (FP64 form)
   float k1f[N];   // FSAL stage (FP32 form)
   double t, tcomp, tf, dt, dtmax;
-  double qold, qoldpow, qsteady_max, eig;
+  double qold, qoldpow, qsteady_max, eig, dtfirst;
   int64_t iter;
   int32_t naccept, nreject, nf, nstiff;
   int32_t isave, count, nanrej;
@@ -188,6 +189,7 @@ struct Lane {
     count = 0;
     nanrej = 0;
     eig = 0.0;
+    dtfirst = 0.0;
     rc = RC_DEFAULT;
     mode = (ALG == ALG_RODAS5) ? 1 : 0;
     if (kAuto && a.alg_hint) mode = a.alg_hint[i] ? 1 : 0;
@@ -621,6 +623,7 @@ struct Lane {
       q = fmax(1.0 / kQmax, fmin(1.0 / kQmin, q / kGammaSafety));
     }
     if (EEst <= 1.0) {
+      if (naccept == 0) dtfirst = dt;
       naccept++;
       if (!tsit_step) nstiff++;
       if (1.0 <= q && q <= qsteady_max) q = 1.0;
@@ -724,6 +727,7 @@ struct Lane {
     if (a.nreject) a.nreject[i] = nreject;
     if (a.nf) a.nf[i] = nf;
     if (a.nstiff) a.nstiff[i] = nstiff;
+    if (a.dt_first) a.dt_first[i] = dtfirst;
   }
 };
 

@@ -1,19 +1,25 @@
 // runtime.hip -- the C ABI of libmideq.so (include/mideq.h): contexts,
-// argument validation, device buffers, schedule choice and launches.
+// argument validation, device buffers, schedule choice, the autotuner for
+// the large-system kernel, hipGraph replay and launches.
 //
 // Boundary replaced: SciMLBase.__solve(::EnsembleProblem, alg,
 // ::EnsembleThreads; trajectories, saveat, abstol, reltol) -- the ensemble
 // fan-out of test/qa/qa.jl:50,56 -- for problems packed into SoA buffers.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <numeric>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/mideq.h"
+#include "bruss.hpp"
 #include "launch.hpp"
 
 using namespace mideq;
@@ -21,6 +27,17 @@ using namespace mideq;
 struct DevBuf {
   void* ptr = nullptr;
   size_t size = 0;
+};
+
+enum {
+  BUF_U0 = 0, BUF_P, BUF_T0, BUF_TF, BUF_NVEC, BUF_SAVEAT, BUF_DT0, BUF_HINT, BUF_US,
+  BUF_UF, BUF_TFIN, BUF_RC, BUF_NA, BUF_NR, BUF_NF, BUF_NS, BUF_SCRATCH, BUF_ORDER,
+  BUF_SAMPLE, BUF_DTF, BUF_COUNT
+};
+
+struct GraphCache {
+  std::vector<unsigned char> key;
+  hipGraphExec_t exec = nullptr;
 };
 
 struct mideq_ctx {
@@ -32,7 +49,10 @@ struct mideq_ctx {
   bool ev_pending = false;
   int ncu = 256;
   unsigned long long* d_queue = nullptr;
-  DevBuf bufs[16];
+  DevBuf bufs[BUF_COUNT];
+  // autotuned lanes per instance, keyed by (alg, precision, nmax)
+  std::map<std::tuple<int, int, int>, int> tuned_lanes;
+  GraphCache graph;
 };
 
 static int set_err(mideq_ctx* c, int code, const char* fmt, ...) {
@@ -60,7 +80,9 @@ static int ensure_buf(mideq_ctx* c, int slot, size_t bytes, void** out) {
   }
   if (b.size < bytes) {
     if (b.ptr) {
-      hipError_t e = hipFree(b.ptr);
+      // the previous contents may still be in use by queued work
+      hipError_t e = hipDeviceSynchronize();
+      e = hipFree(b.ptr);
       (void)e;
       b.ptr = nullptr;
       b.size = 0;
@@ -72,6 +94,15 @@ static int ensure_buf(mideq_ctx* c, int slot, size_t bytes, void** out) {
   }
   *out = b.ptr;
   return 0;
+}
+
+static void drop_graph(mideq_ctx* c) {
+  if (c->graph.exec) {
+    hipError_t e = hipGraphExecDestroy(c->graph.exec);
+    (void)e;
+  }
+  c->graph.exec = nullptr;
+  c->graph.key.clear();
 }
 
 extern "C" int mideq_version(void) { return MIDEQ_VERSION; }
@@ -103,7 +134,8 @@ extern "C" int mideq_ctx_create(mideq_ctx** out, int32_t device) {
 extern "C" void mideq_ctx_destroy(mideq_ctx* c) {
   if (!c) return;
   hipError_t e = hipSetDevice(c->device);
-  if (c->stream) e = hipStreamSynchronize(c->stream);
+  e = hipDeviceSynchronize();
+  drop_graph(c);
   for (auto& b : c->bufs)
     if (b.ptr) e = hipFree(b.ptr);
   if (c->d_queue) e = hipFree(c->d_queue);
@@ -149,7 +181,12 @@ static int validate(mideq_ctx* c, const mideq_problem* P, const mideq_options* O
     return set_err(c, MIDEQ_E_MODEL, "unknown model id %d", P->model);
   if (P->ntraj < 0) return set_err(c, MIDEQ_E_ARG, "ntraj < 0");
   if (P->model == MODEL_BRUSSELATOR) {
-    return set_err(c, MIDEQ_E_UNSUPPORTED, "brusselator kernel not built");
+    if (P->n < BRUSS_NMIN || P->n > BRUSS_NMAX || (P->n & 1))
+      return set_err(c, MIDEQ_E_ARG, "brusselator n=%d must be even in [%d,%d]", P->n,
+                     BRUSS_NMIN, BRUSS_NMAX);
+    if (O->ns > 0) return set_err(c, MIDEQ_E_UNSUPPORTED, "saveat not provided for the brusselator");
+    if (O->lanes != 0 && O->lanes != 32 && O->lanes != 64)
+      return set_err(c, MIDEQ_E_ARG, "lanes must be 0, 32 or 64");
   } else if (P->n != n) {
     return set_err(c, MIDEQ_E_ARG, "model %d has n=%d, got n=%d", P->model, n, P->n);
   }
@@ -178,12 +215,168 @@ static int check_saveat_host(mideq_ctx* c, const double* sa, int ns) {
   return MIDEQ_OK;
 }
 
+// Enqueue a launch sequence, optionally through a cached hipGraph whose key
+// is the exact argument bytes of the sequence plus the stream.
+template <class F>
+static int enqueue(mideq_ctx* c, bool use_graph, const void* key, size_t keylen,
+                   hipStream_t s, F&& body) {
+  c->stats.graph = 0;
+  if (!use_graph) {
+    hipError_t e = body(s);
+    if (e != hipSuccess) return set_err(c, MIDEQ_E_HIP, "kernel launch: %s", hipGetErrorString(e));
+    return MIDEQ_OK;
+  }
+  const unsigned char* kb = (const unsigned char*)key;
+  std::vector<unsigned char> k(kb, kb + keylen);
+  const unsigned char* sb = (const unsigned char*)&s;
+  k.insert(k.end(), sb, sb + sizeof(s));
+  if (!c->graph.exec || c->graph.key != k) {
+    drop_graph(c);
+    hipGraph_t g = nullptr;
+    HIP_TRY(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    hipError_t e = body(s);
+    hipError_t e2 = hipStreamEndCapture(s, &g);
+    if (e != hipSuccess || e2 != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return set_err(c, MIDEQ_E_HIP, "graph capture: %s / %s", hipGetErrorString(e),
+                     hipGetErrorString(e2));
+    }
+    hipGraphExec_t ex = nullptr;
+    e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return set_err(c, MIDEQ_E_HIP, "graph instantiate: %s", hipGetErrorString(e));
+    c->graph.exec = ex;
+    c->graph.key = k;
+  }
+  HIP_TRY(c, hipGraphLaunch(c->graph.exec, s));
+  c->stats.graph = 1;
+  return MIDEQ_OK;
+}
+
+// ---------------------------------------------------------------- brusselator
+static int bruss_launch_with(mideq_ctx* c, BrussArgs b, int lanes, int alg, int mix,
+                             bool use_graph, hipStream_t s, int* grid_used) {
+  int grid = 0, tpb = 0;
+  hipError_t e = bruss_plan(lanes, alg, mix, c->ncu, b.nqueue, &grid, &tpb);
+  if (e != hipSuccess) return set_err(c, MIDEQ_E_HIP, "bruss_plan: %s", hipGetErrorString(e));
+  if (alg != MIDEQ_ALG_TSIT5) {
+    b.scratch_stride = bruss_scratch_per_team(b.nmax);
+    size_t bytes = sizeof(double) * (size_t)b.scratch_stride * (size_t)grid * (size_t)tpb;
+    void* d;
+    int rc = ensure_buf(c, BUF_SCRATCH, bytes, &d);
+    if (rc) return rc;
+    b.scratch = (double*)d;
+  } else {
+    b.scratch = nullptr;
+    b.scratch_stride = 0;
+  }
+  *grid_used = grid;
+  struct Key {
+    BrussArgs b;
+    int lanes, alg, mix, grid;
+  } key{b, lanes, alg, mix, grid};
+  return enqueue(c, use_graph, &key, sizeof key, s, [&](hipStream_t st) {
+    return launch_bruss_solver(b, lanes, alg, mix, grid, st);
+  });
+}
+
+static int run_bruss(mideq_ctx* c, const mideq_problem* P, const mideq_options* O,
+                     const mideq_outputs* out, const int64_t* d_order, hipStream_t s) {
+  BrussArgs b{};
+  b.ntraj = P->ntraj;
+  b.nqueue = P->ntraj;
+  b.nmax = P->n;
+  b.nvec = P->nvec;
+  b.u0 = P->u0;
+  b.p = P->p;
+  b.t0 = P->t0;
+  b.tf = P->tf;
+  b.tspan0 = P->tspan0;
+  b.tspan1 = P->tspan1;
+  b.abstol = O->abstol;
+  b.reltol = O->reltol;
+  b.dt_fixed = O->dt;
+  b.dtmin = O->dtmin;
+  b.dtmax = O->dtmax;
+  b.maxiters = O->maxiters;
+  b.dt0 = O->dt0;
+  b.alg_hint = O->alg_hint;
+  b.qsteady_default = O->qsteady_default;
+  b.u_final = out->u_final;
+  b.t_final = out->t_final;
+  b.retcode = out->retcode;
+  b.naccept = out->naccept;
+  b.nreject = out->nreject;
+  b.nf = out->nf;
+  b.nstiff = out->nstiff;
+  b.dt_first = out->dt_first;
+  b.queue = c->d_queue;
+  b.order = d_order;
+  int lanes = O->lanes;
+  int grid = 0;
+  if (lanes == 0) {
+    // autotuner: time each candidate on a strided sample of the batch and
+    // keep the fastest for this (alg, precision, nmax)
+    auto key = std::make_tuple((int)O->alg, (int)O->precision, (int)P->n);
+    auto it = c->tuned_lanes.find(key);
+    if (it != c->tuned_lanes.end()) {
+      lanes = it->second;
+    } else {
+      int64_t S = std::min<int64_t>(P->ntraj, 8192);
+      std::vector<int64_t> sample(S);
+      for (int64_t k = 0; k < S; k++) sample[k] = (k * P->ntraj) / S;
+      void* d;
+      int rc = ensure_buf(c, BUF_SAMPLE, sizeof(int64_t) * S, &d);
+      if (rc) return rc;
+      HIP_TRY(c, hipMemcpyAsync(d, sample.data(), sizeof(int64_t) * S, hipMemcpyHostToDevice, s));
+      BrussArgs bs = b;
+      bs.order = (const int64_t*)d;
+      bs.nqueue = S;
+      float best = 1e30f;
+      int best_l = 64;
+      hipEvent_t a0, a1;
+      HIP_TRY(c, hipEventCreate(&a0));
+      HIP_TRY(c, hipEventCreate(&a1));
+      for (int cand : {64, 32}) {
+        int g = 0;
+        if ((rc = bruss_launch_with(c, bs, cand, O->alg, O->precision, false, s, &g))) return rc;
+        HIP_TRY(c, hipEventRecord(a0, s));
+        if ((rc = bruss_launch_with(c, bs, cand, O->alg, O->precision, false, s, &g))) return rc;
+        HIP_TRY(c, hipEventRecord(a1, s));
+        HIP_TRY(c, hipEventSynchronize(a1));
+        float ms = 0.f;
+        HIP_TRY(c, hipEventElapsedTime(&ms, a0, a1));
+        if (ms < best) {
+          best = ms;
+          best_l = cand;
+        }
+      }
+      HIP_TRY(c, hipEventDestroy(a0));
+      HIP_TRY(c, hipEventDestroy(a1));
+      lanes = best_l;
+      c->tuned_lanes[key] = lanes;
+    }
+  }
+  HIP_TRY(c, hipEventRecord(c->ev0, s));
+  int rc = bruss_launch_with(c, b, lanes, O->alg, O->precision, O->graph != 0, s, &grid);
+  if (rc) return rc;
+  HIP_TRY(c, hipEventRecord(c->ev1, s));
+  c->ev_pending = true;
+  c->stats.schedule = MIDEQ_SCHED_PERSISTENT;
+  c->stats.block = 256;
+  c->stats.grid = grid;
+  c->stats.launches = 2;
+  c->stats.lanes = lanes;
+  return MIDEQ_OK;
+}
+
 // ---------------------------------------------------------------- launch
 static int run_solver(mideq_ctx* c, const mideq_problem* P, const mideq_options* O,
-                      const mideq_outputs* out, hipStream_t s) {
+                      const mideq_outputs* out, const int64_t* d_order, hipStream_t s) {
   c->stats = mideq_stats{};
   c->ev_pending = false;
   if (P->ntraj == 0) return MIDEQ_OK;
+  if (P->model == MODEL_BRUSSELATOR) return run_bruss(c, P, O, out, d_order, s);
   LaneArgs a{};
   a.ntraj = P->ntraj;
   a.u0 = P->u0;
@@ -212,6 +405,7 @@ static int run_solver(mideq_ctx* c, const mideq_problem* P, const mideq_options*
   a.nreject = out->nreject;
   a.nf = out->nf;
   a.nstiff = out->nstiff;
+  a.dt_first = out->dt_first;
   a.queue = c->d_queue;
   int sched = O->schedule == MIDEQ_SCHED_AUTO ? SCHED_LANE : O->schedule;
   int block = O->block > 0 ? O->block : 256;
@@ -225,14 +419,21 @@ static int run_solver(mideq_ctx* c, const mideq_problem* P, const mideq_options*
     grid = (int)((P->ntraj + block - 1) / block);
   }
   HIP_TRY(c, hipEventRecord(c->ev0, s));
-  hipError_t e = launch_lane_solver(a, P->model, O->alg, O->precision, sched, block, grid, s);
-  if (e != hipSuccess) return set_err(c, MIDEQ_E_HIP, "kernel launch: %s", hipGetErrorString(e));
+  struct Key {
+    LaneArgs a;
+    int model, alg, prec, sched, block, grid;
+  } key{a, P->model, O->alg, O->precision, sched, block, grid};
+  int rc = enqueue(c, O->graph != 0, &key, sizeof key, s, [&](hipStream_t st) {
+    return launch_lane_solver(a, P->model, O->alg, O->precision, sched, block, grid, st);
+  });
+  if (rc) return rc;
   HIP_TRY(c, hipEventRecord(c->ev1, s));
   c->ev_pending = true;
   c->stats.schedule = sched;
   c->stats.block = block;
   c->stats.grid = grid;
   c->stats.launches = sched == SCHED_PERSISTENT ? 2 : 1;
+  c->stats.lanes = 1;
   return MIDEQ_OK;
 }
 
@@ -245,7 +446,7 @@ extern "C" int mideq_ensemble_solve_device(mideq_ctx* c, const mideq_problem* P,
   if (!out) return set_err(c, MIDEQ_E_ARG, "null outputs");
   HIP_TRY(c, hipSetDevice(c->device));
   hipStream_t s = stream ? (hipStream_t)stream : c->stream;
-  return run_solver(c, P, O, out, s);
+  return run_solver(c, P, O, out, nullptr, s);
 }
 
 extern "C" int mideq_ensemble_solve(mideq_ctx* c, const mideq_problem* P,
@@ -260,6 +461,12 @@ extern "C" int mideq_ensemble_solve(mideq_ctx* c, const mideq_problem* P,
   const int64_t N = P->ntraj;
   const int n = P->n;
   if (N == 0) return MIDEQ_OK;
+  if (P->nvec) {
+    for (int64_t i = 0; i < N; i++)
+      if (P->nvec[i] < BRUSS_NMIN || P->nvec[i] > n || (P->nvec[i] & 1))
+        return set_err(c, MIDEQ_E_ARG, "nvec[%lld]=%d outside [%d,%d] or odd", (long long)i,
+                       P->nvec[i], BRUSS_NMIN, n);
+  }
   hipEvent_t t0e, t1e;
   HIP_TRY(c, hipEventCreate(&t0e));
   HIP_TRY(c, hipEventCreate(&t1e));
@@ -269,57 +476,81 @@ extern "C" int mideq_ensemble_solve(mideq_ctx* c, const mideq_problem* P,
   mideq_options dO = *O;
   mideq_outputs dOut{};
   void* d;
-  // inputs
-  if ((rc = ensure_buf(c, 0, sizeof(double) * (size_t)n * N, &d))) return rc;
+  if ((rc = ensure_buf(c, BUF_U0, sizeof(double) * (size_t)n * N, &d))) return rc;
   HIP_TRY(c, hipMemcpyAsync(d, P->u0, sizeof(double) * (size_t)n * N, hipMemcpyHostToDevice, s));
   dP.u0 = (const double*)d;
-  if ((rc = ensure_buf(c, 1, sizeof(double) * (size_t)P->np * N, &d))) return rc;
+  if ((rc = ensure_buf(c, BUF_P, sizeof(double) * (size_t)P->np * N, &d))) return rc;
   HIP_TRY(c, hipMemcpyAsync(d, P->p, sizeof(double) * (size_t)P->np * N, hipMemcpyHostToDevice, s));
   dP.p = (const double*)d;
   if (P->t0) {
-    if ((rc = ensure_buf(c, 2, sizeof(double) * N, &d))) return rc;
+    if ((rc = ensure_buf(c, BUF_T0, sizeof(double) * N, &d))) return rc;
     HIP_TRY(c, hipMemcpyAsync(d, P->t0, sizeof(double) * N, hipMemcpyHostToDevice, s));
     dP.t0 = (const double*)d;
   }
   if (P->tf) {
-    if ((rc = ensure_buf(c, 3, sizeof(double) * N, &d))) return rc;
+    if ((rc = ensure_buf(c, BUF_TF, sizeof(double) * N, &d))) return rc;
     HIP_TRY(c, hipMemcpyAsync(d, P->tf, sizeof(double) * N, hipMemcpyHostToDevice, s));
     dP.tf = (const double*)d;
   }
   if (P->nvec) {
-    if ((rc = ensure_buf(c, 4, sizeof(int32_t) * N, &d))) return rc;
+    if ((rc = ensure_buf(c, BUF_NVEC, sizeof(int32_t) * N, &d))) return rc;
     HIP_TRY(c, hipMemcpyAsync(d, P->nvec, sizeof(int32_t) * N, hipMemcpyHostToDevice, s));
     dP.nvec = (const int32_t*)d;
   }
   if (O->ns > 0) {
-    if ((rc = ensure_buf(c, 5, sizeof(double) * O->ns, &d))) return rc;
+    if ((rc = ensure_buf(c, BUF_SAVEAT, sizeof(double) * O->ns, &d))) return rc;
     HIP_TRY(c, hipMemcpyAsync(d, O->saveat, sizeof(double) * O->ns, hipMemcpyHostToDevice, s));
     dO.saveat = (const double*)d;
   }
   if (O->dt0) {
-    if ((rc = ensure_buf(c, 6, sizeof(double) * N, &d))) return rc;
+    if ((rc = ensure_buf(c, BUF_DT0, sizeof(double) * N, &d))) return rc;
     HIP_TRY(c, hipMemcpyAsync(d, O->dt0, sizeof(double) * N, hipMemcpyHostToDevice, s));
     dO.dt0 = (const double*)d;
   }
   if (O->alg_hint) {
-    if ((rc = ensure_buf(c, 7, N, &d))) return rc;
+    if ((rc = ensure_buf(c, BUF_HINT, N, &d))) return rc;
     HIP_TRY(c, hipMemcpyAsync(d, O->alg_hint, N, hipMemcpyHostToDevice, s));
     dO.alg_hint = (const int8_t*)d;
   }
-  // outputs
+  // work order for the large-system kernel: most expensive first (LPT),
+  // cost ~ n * Ngrid^2 * (tf - t0), the Tsit5 work when stability-bound
+  std::vector<int64_t> order;
+  const int64_t* d_order = nullptr;
+  if (P->model == MODEL_BRUSSELATOR && O->schedule != MIDEQ_SCHED_LANE) {
+    order.resize(N);
+    std::iota(order.begin(), order.end(), 0);
+    std::vector<double> cost(N);
+    for (int64_t i = 0; i < N; i++) {
+      double ni = P->nvec ? P->nvec[i] : n;
+      double span = (P->tf ? P->tf[i] : P->tspan1) - (P->t0 ? P->t0[i] : P->tspan0);
+      cost[i] = ni * (ni * 0.5) * (ni * 0.5) * std::fabs(span);
+    }
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int64_t x, int64_t y) { return cost[x] > cost[y]; });
+    if ((rc = ensure_buf(c, BUF_ORDER, sizeof(int64_t) * N, &d))) return rc;
+    HIP_TRY(c, hipMemcpyAsync(d, order.data(), sizeof(int64_t) * N, hipMemcpyHostToDevice, s));
+    d_order = (const int64_t*)d;
+  }
   size_t us_bytes = (size_t)O->ns * n * N * (O->save_dtype ? 4 : 8);
   if (out->us && O->ns > 0) {
-    if ((rc = ensure_buf(c, 8, us_bytes, &dOut.us))) return rc;
+    if ((rc = ensure_buf(c, BUF_US, us_bytes, &dOut.us))) return rc;
   }
-  if (out->u_final && (rc = ensure_buf(c, 9, sizeof(double) * (size_t)n * N, (void**)&dOut.u_final))) return rc;
-  if (out->t_final && (rc = ensure_buf(c, 10, sizeof(double) * N, (void**)&dOut.t_final))) return rc;
-  if (out->retcode && (rc = ensure_buf(c, 11, N, (void**)&dOut.retcode))) return rc;
-  if (out->naccept && (rc = ensure_buf(c, 12, 4 * N, (void**)&dOut.naccept))) return rc;
-  if (out->nreject && (rc = ensure_buf(c, 13, 4 * N, (void**)&dOut.nreject))) return rc;
-  if (out->nf && (rc = ensure_buf(c, 14, 4 * N, (void**)&dOut.nf))) return rc;
-  if (out->nstiff && (rc = ensure_buf(c, 15, 4 * N, (void**)&dOut.nstiff))) return rc;
+  if (out->u_final) {
+    if ((rc = ensure_buf(c, BUF_UF, sizeof(double) * (size_t)n * N, (void**)&dOut.u_final))) return rc;
+    // rows past a ragged instance's size are not written by the kernel
+    if (P->nvec)
+      HIP_TRY(c, hipMemcpyAsync(dOut.u_final, out->u_final, sizeof(double) * (size_t)n * N,
+                                hipMemcpyHostToDevice, s));
+  }
+  if (out->t_final && (rc = ensure_buf(c, BUF_TFIN, sizeof(double) * N, (void**)&dOut.t_final))) return rc;
+  if (out->retcode && (rc = ensure_buf(c, BUF_RC, N, (void**)&dOut.retcode))) return rc;
+  if (out->naccept && (rc = ensure_buf(c, BUF_NA, 4 * N, (void**)&dOut.naccept))) return rc;
+  if (out->nreject && (rc = ensure_buf(c, BUF_NR, 4 * N, (void**)&dOut.nreject))) return rc;
+  if (out->nf && (rc = ensure_buf(c, BUF_NF, 4 * N, (void**)&dOut.nf))) return rc;
+  if (out->nstiff && (rc = ensure_buf(c, BUF_NS, 4 * N, (void**)&dOut.nstiff))) return rc;
+  if (out->dt_first && (rc = ensure_buf(c, BUF_DTF, 8 * N, (void**)&dOut.dt_first))) return rc;
 
-  rc = run_solver(c, &dP, &dO, &dOut, s);
+  rc = run_solver(c, &dP, &dO, &dOut, d_order, s);
   if (rc) return rc;
   if (out->us && O->ns > 0)
     HIP_TRY(c, hipMemcpyAsync(out->us, dOut.us, us_bytes, hipMemcpyDeviceToHost, s));
@@ -332,6 +563,7 @@ extern "C" int mideq_ensemble_solve(mideq_ctx* c, const mideq_problem* P,
   if (out->nreject) HIP_TRY(c, hipMemcpyAsync(out->nreject, dOut.nreject, 4 * N, hipMemcpyDeviceToHost, s));
   if (out->nf) HIP_TRY(c, hipMemcpyAsync(out->nf, dOut.nf, 4 * N, hipMemcpyDeviceToHost, s));
   if (out->nstiff) HIP_TRY(c, hipMemcpyAsync(out->nstiff, dOut.nstiff, 4 * N, hipMemcpyDeviceToHost, s));
+  if (out->dt_first) HIP_TRY(c, hipMemcpyAsync(out->dt_first, dOut.dt_first, 8 * N, hipMemcpyDeviceToHost, s));
   HIP_TRY(c, hipEventRecord(t1e, s));
   HIP_TRY(c, hipStreamSynchronize(s));
   float ms = 0.f;

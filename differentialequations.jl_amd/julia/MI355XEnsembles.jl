@@ -1,0 +1,193 @@
+"""
+    MI355XEnsembles
+
+`EnsembleMI355X`: an `SciMLBase.EnsembleAlgorithm` that solves every
+trajectory of an `EnsembleProblem` on one AMD MI355X through libmideq.so
+(C ABI in include/mideq.h).  Drop-in for `EnsembleThreads()`:
+
+    using DifferentialEquations, MI355XEnsembles
+    sol = solve(EnsembleProblem(prob; prob_func, output_func), Tsit5(),
+                EnsembleMI355X(); trajectories = 10_000, saveat = 0.1,
+                abstol = 1e-8, reltol = 1e-8)
+
+Reference plug point: `SciMLBase.__solve(::AbstractEnsembleProblem, alg,
+::EnsembleAlgorithm; trajectories, ...)`, the dispatch EnsembleThreads uses
+(names re-exported by DifferentialEquations.jl, src/DifferentialEquations.jl:8-9;
+test/qa/qa.jl:50,52,56).  The right-hand side must be one of the registered
+device models (`MODEL_FOR` below); any other `f` falls back to
+`EnsembleThreads` with a warning, as the backend cannot run arbitrary Julia
+code on the device.
+
+This file is source only in this repository (no Julia toolchain is present
+where it is built and tested); tests/test_gpu_api.py drives the same C ABI
+through Python ctypes with the same packing, retcode mapping and
+prob_func / output_func / rerun / reduction order.
+"""
+module MI355XEnsembles
+
+using SciMLBase
+import SciMLBase: __solve, EnsembleSolution, ReturnCode
+
+export EnsembleMI355X, register_model!
+
+const LIB = Ref{String}(get(ENV, "MIDEQ_LIB",
+    joinpath(@__DIR__, "..", "_build", "libmideq.so")))
+
+Base.@kwdef struct EnsembleMI355X <: SciMLBase.EnsembleAlgorithm
+    precision::Symbol = :fp64          # :fp64 or :mixed
+    schedule::Symbol = :auto           # :auto, :lane, :persistent
+    device::Int = 0
+    save_dtype::Type = Float64
+end
+
+# ---- C structs (field order and types mirror include/mideq.h) -------------
+struct MideqProblem
+    model::Int32; n::Int32; np::Int32; reserved0::Int32
+    ntraj::Int64
+    u0::Ptr{Float64}; p::Ptr{Float64}; t0::Ptr{Float64}; tf::Ptr{Float64}
+    tspan0::Float64; tspan1::Float64
+    nvec::Ptr{Int32}
+end
+
+struct MideqOptions
+    alg::Int32; precision::Int32
+    abstol::Float64; reltol::Float64
+    dt::Float64; dtmin::Float64; dtmax::Float64
+    maxiters::Int64
+    saveat::Ptr{Float64}; ns::Int32; save_dtype::Int32
+    dt0::Ptr{Float64}; alg_hint::Ptr{Int8}
+    qsteady_default::Int32; schedule::Int32; block::Int32; blocks_per_cu::Int32
+    lanes::Int32; graph::Int32
+end
+
+struct MideqOutputs
+    us::Ptr{Cvoid}; u_final::Ptr{Float64}; t_final::Ptr{Float64}; retcode::Ptr{Int8}
+    naccept::Ptr{Int32}; nreject::Ptr{Int32}; nf::Ptr{Int32}; nstiff::Ptr{Int32}
+    dt_first::Ptr{Float64}
+end
+
+# ---- registered right-hand sides ------------------------------------------
+# f (the user's Julia function object) => device model id.  Users register
+# their in-place f! once: register_model!(lorenz!, :lorenz).
+const MODEL_IDS = Dict(:exp => 0, :lorenz => 1, :robertson => 2, :vdp => 3,
+                       :brusselator => 4, :linear => 5)
+const MODEL_FOR = IdDict{Any, Int32}()
+register_model!(f, name::Symbol) = (MODEL_FOR[f] = MODEL_IDS[name]; f)
+
+alg_code(::Tsit5) = Int32(0)
+alg_code(::Rodas5) = Int32(1)
+alg_code(::CompositeAlgorithm) = Int32(2)       # AutoTsit5(Rodas5())
+alg_code(alg) = error("EnsembleMI355X supports Tsit5, Rodas5 and AutoTsit5(Rodas5())")
+
+const RETCODES = (ReturnCode.Default, ReturnCode.Success, ReturnCode.MaxIters,
+                  ReturnCode.DtLessThanMin, ReturnCode.Unstable, ReturnCode.DtNaN,
+                  ReturnCode.InitialFailure)
+
+mutable struct Ctx
+    h::Ptr{Cvoid}
+end
+const CTX = Dict{Int, Ctx}()
+
+function context(dev::Int)
+    get!(CTX, dev) do
+        h = Ref{Ptr{Cvoid}}(C_NULL)
+        rc = ccall((:mideq_ctx_create, LIB[]), Cint, (Ref{Ptr{Cvoid}}, Int32), h, dev)
+        rc == 0 || error("mideq_ctx_create failed ($rc)")
+        c = Ctx(h[])
+        finalizer(x -> ccall((:mideq_ctx_destroy, LIB[]), Cvoid, (Ptr{Cvoid},), x.h), c)
+        c
+    end
+end
+
+function check(ctx, rc, what)
+    rc == 0 && return
+    msg = unsafe_string(ccall((:mideq_last_error, LIB[]), Cstring, (Ptr{Cvoid},), ctx.h))
+    error("$what failed ($rc): $msg")
+end
+
+function _saveat_vector(saveat, tspan)
+    saveat === nothing && return Float64[]
+    if saveat isa Number
+        pts = collect(tspan[1]:saveat:tspan[2])
+        last(pts) < tspan[2] && push!(pts, tspan[2])
+        return Float64.(pts)
+    end
+    return sort(Float64.(collect(saveat)))
+end
+
+function __solve(ens::SciMLBase.AbstractEnsembleProblem, alg, ealg::EnsembleMI355X;
+                 trajectories, saveat = nothing, abstol = 1e-6, reltol = 1e-3,
+                 dt = 0.0, dtmin = 0.0, dtmax = 0.0, maxiters = 100_000, kwargs...)
+    f = ens.prob.f.f
+    if !haskey(MODEL_FOR, f)
+        @warn "EnsembleMI355X: unregistered right-hand side, falling back to EnsembleThreads"
+        return __solve(ens, alg, EnsembleThreads(); trajectories, saveat, abstol,
+                       reltol, kwargs...)
+    end
+    t_start = time()
+    N = trajectories
+    outputs = Vector{Any}(undef, N)
+    pending = collect(1:N)
+    repeat = 1
+    sa = _saveat_vector(saveat, ens.prob.tspan)
+    while !isempty(pending)
+        probs = [ens.prob_func(deepcopy(ens.prob), i, repeat) for i in pending]
+        M = length(probs)
+        n = maximum(length(pr.u0) for pr in probs)
+        np = length(probs[1].p)
+        # column-major (M, n): trajectory index fastest, the SoA layout of the ABI
+        U0 = zeros(Float64, M, n); P = zeros(Float64, M, np)
+        T0 = Float64[pr.tspan[1] for pr in probs]; TF = Float64[pr.tspan[2] for pr in probs]
+        nvec = Int32[length(pr.u0) for pr in probs]
+        for (k, pr) in enumerate(probs)
+            U0[k, 1:length(pr.u0)] .= pr.u0
+            P[k, :] .= collect(pr.p)
+        end
+        ns = length(sa)
+        US = ealg.save_dtype === Float32 ? zeros(Float32, M, n, ns) : zeros(Float64, M, n, ns)
+        UF = zeros(Float64, M, n); TFo = zeros(Float64, M)
+        RC = zeros(Int8, M); NA = zeros(Int32, M); NR = zeros(Int32, M)
+        NF = zeros(Int32, M); NS = zeros(Int32, M)
+        ctx = context(ealg.device)
+        GC.@preserve U0 P T0 TF nvec sa US UF TFo RC NA NR NF NS begin
+            prob_c = MideqProblem(MODEL_FOR[f], n, np, 0, M, pointer(U0), pointer(P),
+                                  pointer(T0), pointer(TF), T0[1], TF[1],
+                                  MODEL_FOR[f] == 4 ? pointer(nvec) : C_NULL)
+            opt_c = MideqOptions(alg_code(alg), ealg.precision === :mixed ? 1 : 0,
+                                 abstol, reltol, dt, dtmin, dtmax, maxiters,
+                                 ns > 0 ? pointer(sa) : C_NULL, ns,
+                                 ealg.save_dtype === Float32 ? 1 : 0, C_NULL, C_NULL, 1,
+                                 ealg.schedule === :lane ? 1 : ealg.schedule === :persistent ? 2 : 0,
+                                 0, 0, 0, 0)
+            out_c = MideqOutputs(ns > 0 ? pointer(US) : C_NULL, pointer(UF), pointer(TFo),
+                                 pointer(RC), pointer(NA), pointer(NR), pointer(NF), pointer(NS),
+                                 C_NULL)
+            rc = ccall((:mideq_ensemble_solve, LIB[]), Cint,
+                       (Ptr{Cvoid}, Ref{MideqProblem}, Ref{MideqOptions}, Ref{MideqOutputs}),
+                       ctx.h, prob_c, opt_c, out_c)
+            check(ctx, rc, "mideq_ensemble_solve")
+        end
+        rerun = Int[]
+        for (k, i) in enumerate(pending)
+            pr = probs[k]
+            nk = length(pr.u0)
+            if ns > 0
+                keep = [j for j in 1:ns if !isnan(US[k, 1, j])]
+                ts = sa[keep]; us = [Vector{Float64}(US[k, 1:nk, j]) for j in keep]
+            else
+                ts = [pr.tspan[1], TFo[k]]; us = [copy(pr.u0), UF[k, 1:nk]]
+            end
+            sol = SciMLBase.build_solution(pr, alg, ts, us;
+                                           retcode = RETCODES[RC[k] + 1])
+            out, again = ens.output_func(sol, i)
+            outputs[i] = out
+            again && push!(rerun, i)
+        end
+        pending = rerun
+        repeat += 1
+    end
+    u, converged = ens.reduction(ens.u_init === nothing ? [] : ens.u_init, outputs, 1:N)
+    return EnsembleSolution(u, time() - t_start, converged)
+end
+
+end # module

@@ -39,18 +39,20 @@ class Options(C.Structure):
                 ("dtmax", C.c_double), ("maxiters", C.c_int64), ("saveat", C.c_void_p),
                 ("ns", C.c_int32), ("save_dtype", C.c_int32), ("dt0", C.c_void_p),
                 ("alg_hint", C.c_void_p), ("qsteady_default", C.c_int32),
-                ("schedule", C.c_int32), ("block", C.c_int32), ("blocks_per_cu", C.c_int32)]
+                ("schedule", C.c_int32), ("block", C.c_int32), ("blocks_per_cu", C.c_int32),
+                ("lanes", C.c_int32), ("graph", C.c_int32)]
 
 
 class Outputs(C.Structure):
     _fields_ = [("us", C.c_void_p), ("u_final", C.c_void_p), ("t_final", C.c_void_p),
                 ("retcode", C.c_void_p), ("naccept", C.c_void_p), ("nreject", C.c_void_p),
-                ("nf", C.c_void_p), ("nstiff", C.c_void_p)]
+                ("nf", C.c_void_p), ("nstiff", C.c_void_p), ("dt_first", C.c_void_p)]
 
 
 class Stats(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("total_ms", C.c_double), ("schedule", C.c_int32),
-                ("block", C.c_int32), ("grid", C.c_int32), ("launches", C.c_int32)]
+                ("block", C.c_int32), ("grid", C.c_int32), ("launches", C.c_int32),
+                ("lanes", C.c_int32), ("graph", C.c_int32)]
 
 
 class MideqError(RuntimeError):

@@ -35,6 +35,7 @@ class ReturnCode(enum.IntEnum):
     DtLessThanMin = 3
     Unstable = 4
     DtNaN = 5
+    InitialFailure = 6
 
 
 def successful_retcode(rc):
@@ -179,6 +180,8 @@ class EnsembleMI355X:
     save_dtype: str = "float64"
     dt0: Any = None
     alg_hint: Any = None
+    lanes: int = 0
+    graph: bool = False
 
 
 # ---------------------------------------------------------------- solutions
@@ -212,12 +215,15 @@ class EnsembleArrays:
     nreject: np.ndarray
     nf: np.ndarray
     nstiff: np.ndarray
+    dt_first: np.ndarray          # [N] first accepted step
     saveat: Optional[np.ndarray]  # [ns]
     us: Optional[np.ndarray]      # [ns, n, N]
     t0: np.ndarray
     u0: np.ndarray
     kernel_ms: float = 0.0
     total_ms: float = 0.0
+    lanes: int = 0
+    graph: bool = False
 
     def solution(self, i, prob=None):
         """ODESolution of trajectory i (0-based)."""
@@ -338,6 +344,8 @@ def device_solve(f, u0, p, t0, tf, nvec, alg, ealg, saveat=None, abstol=1e-6, re
     O.schedule = _abi.SCHEDULE[ealg.schedule]
     O.block = block
     O.blocks_per_cu = blocks_per_cu
+    O.lanes = ealg.lanes
+    O.graph = 1 if ealg.graph else 0
     f32 = ealg.save_dtype == "float32"
     O.save_dtype = 1 if f32 else 0
     us = None
@@ -353,9 +361,9 @@ def device_solve(f, u0, p, t0, tf, nvec, alg, ealg, saveat=None, abstol=1e-6, re
     if hint is not None:
         hint = np.ascontiguousarray(hint, dtype=np.int8)
         O.alg_hint = _ptr(hint)
-    res = dict(u_final=np.empty((n, N)), t_final=np.empty(N), retcode=np.empty(N, np.int8),
+    res = dict(u_final=np.zeros((n, N)), t_final=np.empty(N), retcode=np.empty(N, np.int8),
                naccept=np.empty(N, np.int32), nreject=np.empty(N, np.int32),
-               nf=np.empty(N, np.int32), nstiff=np.empty(N, np.int32))
+               nf=np.empty(N, np.int32), nstiff=np.empty(N, np.int32), dt_first=np.empty(N))
     out = _abi.Outputs()
     out.us = _ptr(us)
     for k, v in res.items():
@@ -363,7 +371,7 @@ def device_solve(f, u0, p, t0, tf, nvec, alg, ealg, saveat=None, abstol=1e-6, re
     ctx.solve(P, O, out)
     st = ctx.stats()
     return EnsembleArrays(saveat=saveat, us=us, t0=t0, u0=u0, kernel_ms=st.kernel_ms,
-                          total_ms=st.total_ms, **res)
+                          total_ms=st.total_ms, lanes=st.lanes, graph=bool(st.graph), **res)
 
 
 # ---------------------------------------------------------------- solve

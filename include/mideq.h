@@ -43,7 +43,7 @@ typedef enum {
   MIDEQ_MODEL_LORENZ = 1,      /* p = (sigma, rho, beta) (test/core.jl:22-27)          */
   MIDEQ_MODEL_ROBERTSON = 2,   /* p = (k1, k2, k3)       (test/core.jl:39-45), has J   */
   MIDEQ_MODEL_VDP = 3,         /* p = (mu), x'=y, y'=mu(1-x^2)y-x, has J               */
-  MIDEQ_MODEL_BRUSSELATOR = 4, /* p = (A, B, alpha), n = 2*Ngrid (ragged), has J       */
+  MIDEQ_MODEL_BRUSSELATOR = 4, /* p = (A, B, alpha), n = 2*Ngrid in [16,128] (ragged), J */
   MIDEQ_MODEL_LINEAR = 5       /* u' = M u, n = 3, p = M row-major (9), has J          */
 } mideq_model;
 
@@ -71,7 +71,8 @@ typedef enum {
   MIDEQ_RC_MAXITERS = 2,
   MIDEQ_RC_DTLESSTHANMIN = 3,
   MIDEQ_RC_UNSTABLE = 4,
-  MIDEQ_RC_DTNAN = 5
+  MIDEQ_RC_DTNAN = 5,
+  MIDEQ_RC_INITIALFAILURE = 6  /* invalid per-trajectory input (e.g. ragged size out of range) */
 } mideq_retcode;
 
 typedef enum {
@@ -115,6 +116,8 @@ typedef struct {
   int32_t schedule;     /* mideq_schedule */
   int32_t block;        /* threads per block, 0 = tuned */
   int32_t blocks_per_cu;/* persistent grid size per CU, 0 = tuned */
+  int32_t lanes;        /* lanes per trajectory for large systems (32, 64), 0 = autotuned */
+  int32_t graph;        /* 1: replay the launch sequence as a cached hipGraph */
 } mideq_options;
 
 /* Results; any pointer may be NULL. */
@@ -127,6 +130,7 @@ typedef struct {
   int32_t *nreject;     /* [ntraj] */
   int32_t *nf;          /* [ntraj] RHS evaluations */
   int32_t *nstiff;      /* [ntraj] accepted Rodas5 steps */
+  double *dt_first;     /* [ntraj] size of the first accepted step */
 } mideq_outputs;
 
 /* Timing and launch record of the last solve on a context. */
@@ -137,6 +141,8 @@ typedef struct {
   int32_t block;
   int32_t grid;
   int32_t launches;
+  int32_t lanes;        /* lanes per trajectory used */
+  int32_t graph;        /* 1 if the launch was a graph replay */
 } mideq_stats;
 
 int mideq_version(void);
@@ -158,6 +164,33 @@ int mideq_ensemble_solve(mideq_ctx *ctx, const mideq_problem *prob,
 int mideq_ensemble_solve_device(mideq_ctx *ctx, const mideq_problem *prob,
                                 const mideq_options *opt, mideq_outputs *out,
                                 void *stream);
+
+/* ---- learned dt0 / method-class predictor (no counterpart in the
+ * reference: it feeds dt0 and alg_hint above; the initdt heuristic and
+ * AutoTsit5 switching remain the fallback).  16 -> 64 -> 64 -> 64 -> 2 GELU
+ * MLP, BF16 weights/activations, FP32 accumulation on the matrix cores.
+ * Parameters (float32, W[in][out] row-major): W1[16x64] b1[64] W2[64x64]
+ * b2[64] W3[64x64] b3[64] W4[64x2] b4[2].  Outputs per sample: (log10 dt0,
+ * stiffness logit).  X, Y, target, label are device pointers. */
+#define MIDEQ_MLP_NIN 16
+#define MIDEQ_MLP_NOUT 2
+#define MIDEQ_MLP_NPARAM 9538
+typedef struct mideq_mlp mideq_mlp;
+int mideq_mlp_create(mideq_mlp **out, int32_t device);
+void mideq_mlp_destroy(mideq_mlp *m);
+const char *mideq_mlp_last_error(const mideq_mlp *m);
+int mideq_mlp_set_params(mideq_mlp *m, const float *params);   /* host [NPARAM] */
+int mideq_mlp_get_params(mideq_mlp *m, float *params);         /* host [NPARAM] */
+int mideq_mlp_set_norm(mideq_mlp *m, const float *mean, const float *std); /* host [16] */
+/* Y[B][2] = net(X[B][16]) */
+int mideq_mlp_infer(mideq_mlp *m, const float *X, int64_t B, float *Y, void *stream);
+/* gradient of w_dt*mean((y0-target)^2) + w_cls*mean(BCE(y1,label)) into the
+ * internal buffer; loss_host (may be NULL) receives the loss (synchronizes) */
+int mideq_mlp_grad(mideq_mlp *m, const float *X, const float *target, const float *label,
+                   int64_t B, float w_dt, float w_cls, float *loss_host, void *stream);
+int mideq_mlp_get_grads(mideq_mlp *m, float *grads);           /* host [NPARAM] */
+int mideq_mlp_adam_step(mideq_mlp *m, float lr, float beta1, float beta2, float eps,
+                        void *stream);
 
 #ifdef __cplusplus
 }

@@ -422,17 +422,18 @@ static void mdl_init(mdl_t *m, int model, int n, const double *p, int np) {
 /* Norms                                                                     */
 /* ------------------------------------------------------------------------ */
 /* ODE_DEFAULT_NORM for arrays: sqrt(sum(abs2, x) / length(x)); summed left  */
-/* to right for n <= 8, by a fixed pairwise tree over 8 strided partial sums */
-/* for larger n (the order the wavefront kernels use).                       */
-static double rms_norm(const double *x, int n) {
+/* to right for n <= 8; for larger n as L strided partial sums (partial l    */
+/* adds x_l^2, x_{l+L}^2, ... in order) combined by a pairwise tree          */
+/* (part[l] += part[l + w] for w = L/2, ..., 1): the order in which a team   */
+/* of L GPU lanes reduces.                                                   */
+static double rms_norm(const double *x, int n, int L) {
   double s;
   if (n <= 8) {
     s = x[0] * x[0];
     for (int i = 1; i < n; i++) s = s + x[i] * x[i];
   } else {
-    s = 0.0;
     double part[64];
-    int L = 64;
+    if (L <= 0 || L > 64) L = 64;
     for (int l = 0; l < L; l++) {
       double a = 0.0;
       for (int i = l; i < n; i += L) a = a + x[i] * x[i];
@@ -447,13 +448,13 @@ static double rms_norm(const double *x, int n) {
 
 /* calculate_residuals + internalnorm: EEst of an error vector. */
 static double err_norm(const double *ut, const double *u0, const double *u1,
-                       int n, double abstol, double reltol) {
+                       int n, double abstol, double reltol, int L) {
   double r[NMAX];
   for (int i = 0; i < n; i++) {
     double sc = fma(fmax(fabs(u0[i]), fabs(u1[i])), reltol, abstol);
     r[i] = ut[i] / sc;
   }
-  return rms_norm(r, n);
+  return rms_norm(r, n, L);
 }
 
 /* Julia eps(x) for finite positive x: distance to the next float. */
@@ -474,9 +475,9 @@ static double initdt_impl(const mdl_t *m, const double *u0, const double *f0,
   double sk[NMAX], tmp[NMAX];
   for (int i = 0; i < n; i++) sk[i] = fma(fabs(u0[i]), o->reltol, o->abstol);
   for (int i = 0; i < n; i++) tmp[i] = u0[i] / sk[i];
-  double d0 = rms_norm(tmp, n);
+  double d0 = rms_norm(tmp, n, o->norm_lanes);
   for (int i = 0; i < n; i++) tmp[i] = f0[i] / sk[i];
-  double d1 = rms_norm(tmp, n);
+  double d1 = rms_norm(tmp, n, o->norm_lanes);
   if (d1 != d1) return dtmin;
   double dt0 = (d0 < 1e-5 || d1 < 1e-5) ? smalldt : (d0 / d1) / 100.0;
   dt0 = fmin(dt0, dtmax);
@@ -489,7 +490,7 @@ static double initdt_impl(const mdl_t *m, const double *u0, const double *f0,
   for (int i = 0; i < n; i++) same &= (f0[i] == f1[i]);
   if (same) return fmax(dtmin, 100.0 * dt0);
   for (int i = 0; i < n; i++) tmp[i] = (f1[i] - f0[i]) / sk[i];
-  double d2 = rms_norm(tmp, n) / dt0;
+  double d2 = rms_norm(tmp, n, o->norm_lanes) / dt0;
   double mx = fmax(d1, d2);
   double dt1;
   if (mx <= 1e-15) {
@@ -580,7 +581,7 @@ int oracle_lu_solve(int32_t n, double *a, double *b) {
 /* ------------------------------------------------------------------------ */
 static double tsit5_step64(const mdl_t *m, const double *uprev, double t,
                            double dt, double abstol, double reltol,
-                           double k[7][NMAX], double *u1, double *eig) {
+                           double k[7][NMAX], double *u1, double *eig, int L) {
   int n = m->n;
   double g[NMAX], g6[NMAX], ut[NMAX];
   double a = dt * TS_A[1][0];
@@ -615,14 +616,14 @@ static double tsit5_step64(const mdl_t *m, const double *uprev, double t,
     for (int j = 5; j >= 0; j--) acc = fma(TS_BT[j], k[j][c], acc);
     ut[c] = dt * acc;
   }
-  return err_norm(ut, uprev, u1, n, abstol, reltol);
+  return err_norm(ut, uprev, u1, n, abstol, reltol, L);
 }
 
 /* Mixed Tsit5 step: stages, stage sums and the RHS in FP32; the state      */
 /* update u1 = uprev + dt*(sum b_j k_j), the error norm and t in FP64.       */
 static double tsit5_step_mix(const mdl_t *m, const double *uprev, double t,
                              double dt, double abstol, double reltol,
-                             float k[7][NMAX], double *u1, double *eig) {
+                             float k[7][NMAX], double *u1, double *eig, int L) {
   int n = m->n;
   float u32[NMAX], g[NMAX], g6[NMAX], u1f[NMAX];
   double ut[NMAX];
@@ -661,7 +662,7 @@ static double tsit5_step_mix(const mdl_t *m, const double *uprev, double t,
     for (int j = 5; j >= 0; j--) acc = fmaf((float)TS_BT[j], k[j][c], acc);
     ut[c] = dt * (double)acc;
   }
-  return err_norm(ut, uprev, u1, n, abstol, reltol);
+  return err_norm(ut, uprev, u1, n, abstol, reltol, L);
 }
 
 void oracle_tsit5_step(int32_t model, int32_t n, const double *u, const double *p,
@@ -672,7 +673,7 @@ void oracle_tsit5_step(int32_t model, int32_t n, const double *u, const double *
   static __thread double k[7][NMAX];
   double eig;
   rhs64(&m, u, t, k[0]);
-  *eest = tsit5_step64(&m, u, t, dt, abstol, reltol, k, u1, &eig);
+  *eest = tsit5_step64(&m, u, t, dt, abstol, reltol, k, u1, &eig, 64);
   if (kout)
     for (int s = 0; s < 7; s++)
       for (int c = 0; c < n; c++) kout[s * n + c] = k[s][c];
@@ -736,7 +737,7 @@ typedef struct {
 
 static int rodas5_step(const mdl_t *m, int mixed, const double *uprev, double t,
                        double dt, double abstol, double reltol, double *u1,
-                       double *eest, rodas_ws *ws, double *normJ) {
+                       double *eest, rodas_ws *ws, double *normJ, int L) {
   int n = m->n;
   double kd[8][NMAX];  /* k in FP64 (values exactly representable in FP32 when mixed) */
   double dT[NMAX], x[NMAX], U[NMAX];
@@ -826,7 +827,7 @@ static int rodas5_step(const mdl_t *m, int mixed, const double *uprev, double t,
     acc = acc + kd[6][c];
     u1[c] = acc + kd[7][c];
   }
-  *eest = err_norm(kd[7], uprev, u1, n, abstol, reltol);
+  *eest = err_norm(kd[7], uprev, u1, n, abstol, reltol, L);
   return 0;
 }
 
@@ -836,7 +837,7 @@ int oracle_rodas5_step(int32_t model, int32_t n, const double *u, const double *
   mdl_t m;
   mdl_init(&m, model, n, p, model_np(model, n));
   rodas_ws *ws = (rodas_ws *)malloc(sizeof(rodas_ws));
-  int rc = rodas5_step(&m, 0, u, t, dt, abstol, reltol, u1, eest, ws, NULL);
+  int rc = rodas5_step(&m, 0, u, t, dt, abstol, reltol, u1, eest, ws, NULL, 64);
   free(ws);
   return rc;
 }
@@ -944,13 +945,13 @@ static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
     double EEst;
     if (mode == 0) {
       if (!mixed)
-        EEst = tsit5_step64(m, uprev, t, dt, abstol, reltol, k64, u1, &eig);
+        EEst = tsit5_step64(m, uprev, t, dt, abstol, reltol, k64, u1, &eig, o->norm_lanes);
       else
-        EEst = tsit5_step_mix(m, uprev, t, dt, abstol, reltol, k32, u1, &eig);
+        EEst = tsit5_step_mix(m, uprev, t, dt, abstol, reltol, k32, u1, &eig, o->norm_lanes);
       res->nf += 6;
     } else {
       double nJ;
-      if (rodas5_step(m, mixed, uprev, t, dt, abstol, reltol, u1, &EEst, ws, &nJ) != 0)
+      if (rodas5_step(m, mixed, uprev, t, dt, abstol, reltol, u1, &EEst, ws, &nJ, o->norm_lanes) != 0)
         EEst = 2.0;
       res->nf += 8;
       res->njac += 1;
@@ -1070,6 +1071,7 @@ typedef struct {
   double *out_save, *u_final, *t_final, *dt0;
   int8_t *retcode;
   int32_t *naccept, *nreject, *nf, *nstiff;
+  double *dt_first;
   atomic_llong next;
 } ens_job;
 
@@ -1109,6 +1111,7 @@ static void *ens_worker(void *arg) {
       if (J->nf) J->nf[i] = res.nf;
       if (J->dt0) J->dt0[i] = res.dt0;
       if (J->nstiff) J->nstiff[i] = res.nstiff_steps;
+      if (J->dt_first) J->dt_first[i] = res.dt_first_accept;
     }
   }
   free(pl);
@@ -1122,7 +1125,7 @@ int oracle_ensemble_solve(int32_t model, int32_t n, int32_t np, int64_t ntraj,
                           const oracle_options *opt, int32_t nthreads,
                           double *out_save, double *u_final, double *t_final,
                           int8_t *retcode, int32_t *naccept, int32_t *nreject,
-                          int32_t *nf, double *dt0, int32_t *nstiff) {
+                          int32_t *nf, double *dt0, int32_t *nstiff, double *dt_first) {
   if (n < 1 || n > NMAX || ntraj < 0) return -1;
   ens_job J;
   memset(&J, 0, sizeof(J));
@@ -1145,6 +1148,7 @@ int oracle_ensemble_solve(int32_t model, int32_t n, int32_t np, int64_t ntraj,
   J.nf = nf;
   J.dt0 = dt0;
   J.nstiff = nstiff;
+  J.dt_first = dt_first;
   atomic_init(&J.next, 0);
   if (nthreads < 1) nthreads = 1;
   if (nthreads == 1 || ntraj <= 1) {

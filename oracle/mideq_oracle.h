@@ -76,6 +76,8 @@ typedef struct {
   const double *saveat;   /* [ns] ascending absolute times, may be NULL */
   int32_t ns;
   int32_t qsteady_default;/* 1: method default qsteady band (Tsit5 [1,1], Rodas5 [1,6/5]) */
+  int32_t norm_lanes;     /* n > 8: RMS sum as norm_lanes strided partials + pairwise tree
+                             (the order of a team of that many GPU lanes); 0 = 64 */
 } oracle_options;
 
 typedef struct {
@@ -106,7 +108,7 @@ int oracle_ensemble_solve(int32_t model, int32_t n, int32_t np, int64_t ntraj,
                           const oracle_options *opt, int32_t nthreads,
                           double *out_save, double *u_final, double *t_final,
                           int8_t *retcode, int32_t *naccept, int32_t *nreject,
-                          int32_t *nf, double *dt0, int32_t *nstiff);
+                          int32_t *nf, double *dt0, int32_t *nstiff, double *dt_first);
 
 /* ---- hooks that expose internals to the tests ---- */
 int oracle_tsit5_tableau(double *c7, double *a49, double *btilde7, double *r28);

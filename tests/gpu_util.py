@@ -21,30 +21,37 @@ def say(msg):
 
 
 def gpu(model, u0, p, t0, tf, alg="tsit5", precision="fp64", schedule="lane", saveat=None,
-        save_dtype="float64", nvec=None, **kw):
+        save_dtype="float64", nvec=None, lanes=0, graph=False, **kw):
     N = u0.shape[1]
     t0 = np.broadcast_to(np.asarray(t0, dtype=np.float64), (N,)).copy()
     tf = np.broadcast_to(np.asarray(tf, dtype=np.float64), (N,)).copy()
-    ealg = EnsembleMI355X(precision=precision, schedule=schedule, save_dtype=save_dtype)
+    ealg = EnsembleMI355X(precision=precision, schedule=schedule, save_dtype=save_dtype,
+                          lanes=lanes, graph=graph)
     return device_solve(FUNCS[model], np.ascontiguousarray(u0), np.ascontiguousarray(p), t0, tf,
                         nvec, ALGS[alg], ealg, saveat=saveat, **kw)
 
 
 def cpu(oracle, model, u0, p, t0, tf, alg="tsit5", precision="fp64", saveat=None, nvec=None,
-        **kw):
+        norm_lanes=64, **kw):
     return oracle.ensemble(model, u0, p, t0, tf, nthreads=NTHREADS, alg=alg,
-                           precision=precision, saveat=saveat, nvec=nvec, **kw)
+                           precision=precision, saveat=saveat, nvec=nvec,
+                           norm_lanes=norm_lanes, **kw)
 
 
-def assert_identical(g, c, save=True, f32=False):
+def assert_identical(g, c, save=True, f32=False, nvec=None):
     """FP64 parity: same retcodes, same accept/reject/f-eval counts, bit-equal
     final states and saveat values."""
     np.testing.assert_array_equal(g.retcode, c["retcode"])
     np.testing.assert_array_equal(g.naccept, c["naccept"])
     np.testing.assert_array_equal(g.nreject, c["nreject"])
     np.testing.assert_array_equal(g.nf, c["nf"])
-    np.testing.assert_array_equal(g.u_final, c["u"])
+    if nvec is not None:
+        mask = np.arange(g.u_final.shape[0])[:, None] < np.asarray(nvec)[None, :]
+        np.testing.assert_array_equal(np.where(mask, g.u_final, 0.0), np.where(mask, c["u"], 0.0))
+    else:
+        np.testing.assert_array_equal(g.u_final, c["u"])
     np.testing.assert_array_equal(g.t_final, c["t"])
+    np.testing.assert_array_equal(g.dt_first, c["dt_first"])
     if save and g.us is not None:
         ref = c["saveat"].astype(np.float32) if f32 else c["saveat"]
         np.testing.assert_array_equal(g.us, ref)

@@ -33,6 +33,7 @@ class Options(C.Structure):
         ("saveat", _dp),
         ("ns", C.c_int32),
         ("qsteady_default", C.c_int32),
+        ("norm_lanes", C.c_int32),
     ]
 
 
@@ -69,7 +70,7 @@ class Oracle:
         L.oracle_ensemble_solve.argtypes = [
             C.c_int32, C.c_int32, C.c_int32, C.c_int64, _dp, _dp, _dp, _dp, _ip,
             C.POINTER(Options), C.c_int32, _dp, _dp, _dp, C.POINTER(C.c_int8), _ip, _ip, _ip,
-            _dp, _ip]
+            _dp, _ip, _dp]
         L.oracle_tsit5_tableau.argtypes = [_dp, _dp, _dp, _dp]
         L.oracle_rodas5_tableau.argtypes = [_dp, _dp, _dp, _dp, _dp]
         for nm in ("oracle_ctl_log2", "oracle_ctl_exp2"):
@@ -93,7 +94,7 @@ class Oracle:
     # ---------------------------------------------------------------- options
     @staticmethod
     def options(alg="tsit5", precision="fp64", abstol=1e-6, reltol=1e-3, dt=0.0, dtmin=0.0,
-                dtmax=0.0, maxiters=0, saveat=None, qsteady_default=1):
+                dtmax=0.0, maxiters=0, saveat=None, qsteady_default=1, norm_lanes=64):
         o = Options()
         o.alg = ALG[alg]
         o.precision = PREC[precision]
@@ -104,6 +105,7 @@ class Oracle:
         o.dtmax = dtmax
         o.maxiters = maxiters
         o.qsteady_default = qsteady_default
+        o.norm_lanes = norm_lanes
         if saveat is not None:
             sa = np.ascontiguousarray(saveat, dtype=np.float64)
             o._saveat_keep = sa
@@ -153,15 +155,16 @@ class Oracle:
         nf = np.zeros(N, dtype=np.int32)
         dt0 = np.zeros(N)
         nst = np.zeros(N, dtype=np.int32)
+        dtf = np.zeros(N)
         nv = None if nvec is None else np.ascontiguousarray(nvec, dtype=np.int32)
         r = self.lib.oracle_ensemble_solve(
             MODEL[model], n, p.shape[0], N, _ptr(u0), _ptr(p), _ptr(t0), _ptr(tf),
             _ptr(nv, C.c_int32), C.byref(o), nthreads, _ptr(save), _ptr(uf), _ptr(tfin),
             rc.ctypes.data_as(C.POINTER(C.c_int8)), _ptr(na, C.c_int32), _ptr(nr, C.c_int32),
-            _ptr(nf, C.c_int32), _ptr(dt0), _ptr(nst, C.c_int32))
+            _ptr(nf, C.c_int32), _ptr(dt0), _ptr(nst, C.c_int32), _ptr(dtf))
         assert r == 0
         out = dict(u=uf, t=tfin, retcode=rc, naccept=na, nreject=nr, nf=nf, dt0=dt0,
-                   nstiff=nst)
+                   nstiff=nst, dt_first=dtf)
         if save is not None:
             out["saveat"] = save
         return out

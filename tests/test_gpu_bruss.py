@@ -1,0 +1,88 @@
+"""Uneven Brusselator batch (ragged state sizes 16..128, per-instance
+tspans) in the persistent team kernel, against the oracle run with the
+same team-size reduction order: identical retcodes and step counts, equal
+final states."""
+import numpy as np
+import pytest
+
+from gpu_util import assert_identical, cpu, gpu, say
+from mideq import workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def batch(ntraj, seed, nmax_grid=64, tf_max=None):
+    w = workloads.brusselator(ntraj, seed=seed)
+    nvec = w["nvec"].copy()
+    if nmax_grid < 64:
+        ng = 8 + (nvec // 2 - 8) % (nmax_grid - 7)
+        nvec = (2 * ng).astype(np.int32)
+        # rebuild initial states for the smaller grids
+        u0 = np.zeros_like(w["u0"])
+        for k, n in enumerate(nvec):
+            x = np.arange(n // 2) / (n // 2)
+            u0[0:n:2, k] = 1 + np.sin(2 * np.pi * x)
+            u0[1:n:2, k] = 3.0
+        w["u0"] = u0
+        w["nvec"] = nvec
+    if tf_max is not None:
+        w["tf"] = np.minimum(w["tf"], tf_max)
+    return w
+
+
+@pytest.mark.parametrize("lanes,precision", [(64, "fp64"), (32, "fp64"), (64, "mixed")])
+def test_bruss_tsit5_bitexact(oracle, lanes, precision):
+    N = 256
+    say(f"brusselator tsit5 {precision} lanes={lanes} N={N}")
+    w = batch(N, seed=51, tf_max=5.0)
+    tol = dict(abstol=1e-8, reltol=1e-6)
+    g = gpu("brusselator", w["u0"], w["p"], 0.0, w["tf"], nvec=w["nvec"], lanes=lanes,
+            precision=precision, schedule="persistent", **tol)
+    c = cpu(oracle, "brusselator", w["u0"], w["p"], 0.0, w["tf"], nvec=w["nvec"],
+            norm_lanes=lanes, precision=precision, **tol)
+    assert (c["retcode"] == 1).all()
+    assert g.lanes == lanes
+    assert_identical(g, c, nvec=w["nvec"])
+
+
+@pytest.mark.parametrize("alg,lanes", [("rodas5", 64), ("rodas5", 32), ("auto", 64)])
+def test_bruss_rodas5_and_auto_bitexact(oracle, alg, lanes):
+    N = 48
+    say(f"brusselator {alg} fp64 lanes={lanes} N={N}")
+    w = batch(N, seed=52, nmax_grid=20, tf_max=3.0)
+    tol = dict(abstol=1e-8, reltol=1e-6)
+    g = gpu("brusselator", w["u0"], w["p"], 0.0, w["tf"], nvec=w["nvec"], lanes=lanes,
+            alg=alg, schedule="persistent", **tol)
+    c = cpu(oracle, "brusselator", w["u0"], w["p"], 0.0, w["tf"], nvec=w["nvec"],
+            norm_lanes=lanes, alg=alg, **tol)
+    assert (c["retcode"] == 1).all()
+    assert_identical(g, c, nvec=w["nvec"])
+    np.testing.assert_array_equal(g.nstiff, c["nstiff"])
+
+
+def test_bruss_autotuned_graph_and_order(oracle):
+    N = 300
+    say(f"brusselator tsit5 autotuned+graph N={N}")
+    w = batch(N, seed=53, tf_max=4.0)
+    tol = dict(abstol=1e-8, reltol=1e-6)
+    g1 = gpu("brusselator", w["u0"], w["p"], 0.0, w["tf"], nvec=w["nvec"], lanes=0,
+             graph=True, schedule="persistent", **tol)
+    assert g1.lanes in (32, 64)
+    g2 = gpu("brusselator", w["u0"], w["p"], 0.0, w["tf"], nvec=w["nvec"], lanes=0,
+             graph=True, schedule="persistent", **tol)
+    np.testing.assert_array_equal(g1.u_final, g2.u_final)
+    c = cpu(oracle, "brusselator", w["u0"], w["p"], 0.0, w["tf"], nvec=w["nvec"],
+            norm_lanes=g1.lanes, **tol)
+    assert_identical(g1, c, nvec=w["nvec"])
+    # natural (unsorted) queue order gives the same per-instance results
+    g3 = gpu("brusselator", w["u0"], w["p"], 0.0, w["tf"], nvec=w["nvec"], lanes=g1.lanes,
+             schedule="lane", **tol)
+    assert_identical(g3, c, nvec=w["nvec"])
+
+
+def test_bruss_invalid_sizes_rejected():
+    say("brusselator invalid nvec")
+    w = batch(8, seed=54)
+    w["nvec"][3] = 15
+    with pytest.raises(Exception):
+        gpu("brusselator", w["u0"], w["p"], 0.0, w["tf"], nvec=w["nvec"], lanes=64)
