@@ -29,6 +29,9 @@ LORENZ_F = 8
 N_STATE = 3
 FLOPS_PER_STEP = 6 * LORENZ_F + N_STATE * (48 + 14 + 6) + 10        # 262
 FLOPS_PER_SAVE = 56 + 14 * N_STATE                                 # 98
+# Rodas5 attempted step on Robertson: 8 RHS (9) + J (8) + W (9) + LU (18) +
+# 8 solves (120) + stage sums (279) + norm (18) + controller (10)
+ROBERTSON_FLOPS_PER_STEP = 534
 PEAK_FP64 = 78.6   # TFLOP/s, MI355X FP64 vector
 PEAK_HBM = 8000.0  # GB/s
 # HBM bytes per launch of the solver kernel, from rocprofv3 FETCH_SIZE and
@@ -39,10 +42,11 @@ MEASURED_TRAFFIC = {"plain": 11549978186}
 ALGORITHMIC_BYTES_PER_TRAJ = 48 + 100 * 3 * 8 + 24 + 8 + 1 + 12
 
 FORMS = {
-    # name: (precision, abstol=reltol, schedule, save dtype)
-    "plain": ("fp64", 1e-8, "lane", "float64"),
-    "mixed": ("mixed", 1e-6, "lane", "float32"),
-    "tuned": ("fp64", 1e-8, "persistent", "float64"),
+    # name: (precision, abstol=reltol, schedule, save dtype, controller)
+    "plain": ("fp64", 1e-8, "lane", "float64", 0),
+    "mixed": ("mixed", 1e-6, "lane", "float32", 0),
+    "tuned": ("fp64", 1e-8, "lane", "float64", 1),
+    "tuned_persistent": ("fp64", 1e-8, "persistent", "float64", 1),
 }
 
 
@@ -52,7 +56,9 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--ntraj", type=int, default=1 << 20)
-    ap.add_argument("--forms", default="plain,mixed,tuned")
+    ap.add_argument("--forms", default="plain,mixed,tuned,tuned_persistent")
+    ap.add_argument("--other", default="robertson,brusselator,mix",
+                    help="other configurations to time (comma list, '' for none)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=20000)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -101,7 +107,7 @@ class DeviceEnsemble:
             self.us[dtype] = self.torch.empty((self.ns, 3, self.N), dtype=tdt, device="cuda:0")
         return self.us[dtype]
 
-    def structs(self, precision, tol, schedule, save_dtype, with_save=True):
+    def structs(self, precision, tol, schedule, save_dtype, with_save=True, controller=0):
         from mideq import _abi
         P = _abi.Problem()
         P.model = _abi.MODEL["lorenz"]
@@ -119,6 +125,7 @@ class DeviceEnsemble:
         O.reltol = tol
         O.qsteady_default = 1
         O.schedule = _abi.SCHEDULE[schedule]
+        O.controller = controller
         if with_save:
             O.saveat = self.saveat.data_ptr()
             O.ns = self.ns
@@ -135,8 +142,8 @@ class DeviceEnsemble:
 
 
 def run_form(torch, ctx, ens, form, steps, warmup):
-    precision, tol, schedule, save_dtype = FORMS[form]
-    P, O, out = ens.structs(precision, tol, schedule, save_dtype)
+    precision, tol, schedule, save_dtype, controller = FORMS[form]
+    P, O, out = ens.structs(precision, tol, schedule, save_dtype, controller=controller)
     # a dedicated stream: the kernels and the timing events share it
     stream = ens.stream
     sp = C.c_void_p(stream.cuda_stream)
@@ -160,7 +167,8 @@ def run_form(torch, ctx, ens, form, steps, warmup):
     uf = ens.u_final.cpu().numpy().copy()
     attempted = int(na.sum()) + int(nr.sum())
     return dict(form=form, precision=precision, tol=tol, schedule=schedule,
-                save_dtype=save_dtype, wall_s=wall, kernel_ms=kms,
+                save_dtype=save_dtype, controller=["reference", "fast"][controller],
+                wall_s=wall, kernel_ms=kms,
                 kernel_ms_mean=statistics.fmean(kms), attempted_steps=attempted,
                 naccept_mean=float(na.mean()), nreject_mean=float(nr.mean()),
                 all_success=bool((rc == 1).all()), u_final=uf)
@@ -175,6 +183,247 @@ def reference_solution(torch, ctx, ens):
     rc = ens.retcode.cpu().numpy()
     assert (rc == 1).all(), "reference solve failed"
     return ens.u_final.cpu().numpy().copy()
+
+
+def timed_solve(torch, fn, steps, warmup, stream):
+    """Run fn() warmup + steps times on `stream`; wall and event times."""
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(steps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0), e0.elapsed_time(e1) / steps
+
+
+def other_robertson(torch, ctx, steps, warmup, N=262144):
+    """Robertson 262,144 trajectories, Rodas5, abstol 1e-8 reltol 1e-6,
+    endpoint only; plain FP64 vs fast controller vs mixed; weighted error
+    against a Rodas5 FP64 solve at reltol 1e-10."""
+    from mideq import _abi, workloads
+    w = workloads.robertson(N, seed=1234)
+    dev = torch.device("cuda:0")
+    u0 = torch.from_numpy(w["u0"]).to(dev)
+    p = torch.from_numpy(w["p"]).to(dev)
+    uf = torch.empty((3, N), dtype=torch.float64, device=dev)
+    rc = torch.empty(N, dtype=torch.int8, device=dev)
+    na = torch.empty(N, dtype=torch.int32, device=dev)
+    nr = torch.empty(N, dtype=torch.int32, device=dev)
+    stream = torch.cuda.Stream()
+    sp = C.c_void_p(stream.cuda_stream)
+
+    def structs(prec, ctl, reltol=1e-6, abstol=1e-8):
+        P = _abi.Problem()
+        P.model, P.n, P.np, P.ntraj = _abi.MODEL["robertson"], 3, 3, N
+        P.u0, P.p, P.tspan0, P.tspan1 = u0.data_ptr(), p.data_ptr(), 0.0, 1e5
+        O = _abi.Options()
+        O.alg, O.precision, O.abstol, O.reltol = _abi.ALG["rodas5"], _abi.PRECISION[prec], abstol, reltol
+        O.qsteady_default, O.controller = 1, ctl
+        out = _abi.Outputs()
+        out.u_final, out.retcode, out.naccept, out.nreject = (uf.data_ptr(), rc.data_ptr(),
+                                                              na.data_ptr(), nr.data_ptr())
+        return P, O, out
+
+    P, O, out = structs("fp64", 0, reltol=1e-10, abstol=1e-14)
+    ctx.solve_device(P, O, out, sp)
+    torch.cuda.synchronize()
+    ref = uf.cpu().numpy().copy()
+    res = {}
+    for name, prec, ctl in (("plain", "fp64", 0), ("tuned", "fp64", 1), ("mixed", "mixed", 1)):
+        P, O, out = structs(prec, ctl)
+        wall, kms = timed_solve(torch, lambda: ctx.solve_device(P, O, out, sp), steps, warmup,
+                                stream)
+        u = uf.cpu().numpy()
+        werr = (np.abs(u - ref) / (1e-8 + 1e-6 * np.abs(ref))).max(axis=0)
+        attempted = int(na.cpu().numpy().sum() + nr.cpu().numpy().sum())
+        res[name] = dict(traj_per_s=N * steps / wall, kernel_ms=kms,
+                         ok=float((rc.cpu().numpy() == 1).mean()),
+                         steps_mean=attempted / N,
+                         weighted_err_median=float(np.median(werr)),
+                         weighted_err_max=float(werr.max()),
+                         tflops=attempted * ROBERTSON_FLOPS_PER_STEP / (kms * 1e-3) / 1e12)
+    return dict(workload="Robertson 262,144 trajectories, k log-uniform +-0.5 decades, "
+                         "tspan (0,1e5), Rodas5 analytic J, abstol 1e-8 reltol 1e-6, endpoint",
+                error_reference="Rodas5 FP64 reltol 1e-10 abstol 1e-14 (device)",
+                flops_per_attempted_step=ROBERTSON_FLOPS_PER_STEP, forms=res)
+
+
+def other_brusselator(torch, steps, warmup, N=131072):
+    """Uneven Brusselator batch (n = 16..128, tspan ends 1..20), one
+    persistent kernel, autotuned lanes per instance."""
+    from mideq import EnsembleMI355X, brusselator, device_solve, workloads
+    from mideq.scimlbase import AutoTsit5, Tsit5
+    w = workloads.brusselator(N, seed=1234)
+    t0 = np.zeros(N)
+    res = {}
+    for name, prec in (("plain", "fp64"), ("mixed", "mixed")):
+        ealg = EnsembleMI355X(precision=prec, schedule="persistent")
+        r = None
+        for _ in range(max(1, warmup)):
+            r = device_solve(brusselator, w["u0"], w["p"], t0, w["tf"], w["nvec"], Tsit5(), ealg,
+                             abstol=1e-8, reltol=1e-6)
+        kms = []
+        for _ in range(steps):
+            r = device_solve(brusselator, w["u0"], w["p"], t0, w["tf"], w["nvec"], Tsit5(), ealg,
+                             abstol=1e-8, reltol=1e-6)
+            kms.append(r.kernel_ms)
+        res[name] = dict(kernel_ms=statistics.fmean(kms), instances_per_s=N / (statistics.fmean(kms) * 1e-3),
+                         lanes=r.lanes, ok=float((r.retcode == 1).mean()),
+                         steps_mean=float((r.naccept + r.nreject).mean()), u=r.u_final)
+    d = np.abs(res["mixed"].pop("u") - res["plain"]["u"])
+    d = d / (1e-8 + 1e-6 * np.abs(res["plain"].pop("u")))
+    res["mixed"]["weighted_diff_vs_fp64_median"] = float(np.median(d.max(axis=0)))
+    # AutoTsit5(Rodas5()) on a sample: dense-LU Rosenbrock at n up to 128
+    S = 4096
+    ws = workloads.brusselator(S, seed=1234)
+    r = device_solve(brusselator, ws["u0"], ws["p"], np.zeros(S), ws["tf"], ws["nvec"],
+                     AutoTsit5(), EnsembleMI355X(schedule="persistent"), abstol=1e-8, reltol=1e-6)
+    r = device_solve(brusselator, ws["u0"], ws["p"], np.zeros(S), ws["tf"], ws["nvec"],
+                     AutoTsit5(), EnsembleMI355X(schedule="persistent"), abstol=1e-8, reltol=1e-6)
+    res["auto_sample"] = dict(instances=S, kernel_ms=r.kernel_ms,
+                              instances_per_s=S / (r.kernel_ms * 1e-3),
+                              stiff_step_fraction=float(r.nstiff.sum() / max(1, r.naccept.sum())))
+    return dict(workload="Brusselator uneven batch 131,072 instances, Ngrid U{8..64}, "
+                         "B U[2,4], tf U[1,20], Tsit5, abstol 1e-8 reltol 1e-6",
+                timing="kernel only (inputs copied in beforehand by the host entry point)",
+                forms=res)
+
+
+def _mix_sets(n_each, seed):
+    from mideq import workloads
+    s = {"lorenz": workloads.lorenz(n_each, seed=seed),
+         "robertson": workloads.robertson(n_each, seed=seed),
+         "vdp": workloads.vdp(n_each, seed=seed)}
+    for w in s.values():
+        w["tf"] = np.broadcast_to(np.asarray(w["tf"], dtype=np.float64),
+                                  (w["u0"].shape[1],)).copy()
+    return s
+
+
+MIX_TOL = {"lorenz": (1e-8, 1e-8), "robertson": (1e-8, 1e-6), "vdp": (1e-8, 1e-6)}
+
+
+def other_mix(torch, n_total=1_000_000, n_train=1_000_000):
+    """Learned dt0 / method class on the 1e6 Lorenz + Robertson + Van der Pol
+    mix: plain = AutoTsit5(Rodas5()) FP64 with the initdt heuristic;
+    learned+batched = MLP inference, trajectories of the predicted stiff
+    class solved by Rodas5 and the rest by AutoTsit5 (each group its own
+    launch), predicted dt0, in FP64 and in the FP32-stage form.  The net is
+    trained on the plain solver's logs of a separately seeded mix."""
+    import mideq
+    from mideq import EnsembleMI355X, device_solve
+    from mideq.learned import MLP, features, training_targets
+    from mideq.scimlbase import AutoTsit5, Rodas5
+    F = {"lorenz": mideq.lorenz, "robertson": mideq.robertson, "vdp": mideq.vanderpol}
+
+    def run(m, w, idx=None, alg=None, prec="fp64", dt0=None, reltol=None, abstol=None):
+        at, rt = MIX_TOL[m]
+        at = abstol or at
+        rt = reltol or rt
+        u0, p, tf = w["u0"], w["p"], w["tf"]
+        if idx is not None:
+            u0, p, tf = u0[:, idx].copy(), p[:, idx].copy(), tf[idx].copy()
+            dt0 = None if dt0 is None else dt0[idx].copy()
+        N = u0.shape[1]
+        return device_solve(F[m], u0, p, np.zeros(N), tf, None, alg or AutoTsit5(),
+                            EnsembleMI355X(precision=prec), abstol=at, reltol=rt, dt0=dt0,
+                            maxiters=1_000_000)
+
+    t_logs = time.perf_counter()
+    train = _mix_sets(n_train // 3, seed=777)
+    Xs, Ts, Ls = [], [], []
+    for m, w in train.items():
+        r = run(m, w)
+        t, l = training_targets(r.dt_first, r.naccept, r.nstiff)
+        ok = r.retcode == 1
+        at, rt = MIX_TOL[m]
+        Xs.append(features(m, w["p"], w["u0"], 0.0, w["tf"], at, rt)[ok])
+        Ts.append(t[ok])
+        Ls.append(l[ok])
+    X = np.concatenate(Xs)
+    T = np.concatenate(Ts)
+    Lb = np.concatenate(Ls)
+    t_logs = time.perf_counter() - t_logs
+    net = MLP(seed=0)
+    net.set_norm(X.mean(0), X.std(0) + 1e-6)
+    Xd, Td, Ld = (torch.from_numpy(a).cuda() for a in (X, T, Lb))
+    torch.cuda.synchronize()
+    t_train = time.perf_counter()
+    losses = net.fit(Xd, Td, Ld, epochs=4, batch=8192, lr=3e-3)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t_train
+
+    evals = _mix_sets(n_total // 3, seed=1234)
+    res = {"plain": dict(kernel_ms=0.0), "learned": dict(kernel_ms=0.0, infer_ms=0.0),
+           "learned_mixed": dict(kernel_ms=0.0, infer_ms=0.0)}
+    agree = 0
+    cnt = dict(plain_rej=0, learned_rej=0, plain_steps=0, learned_steps=0)
+    errs = {k: [] for k in res}
+    for m, w in evals.items():
+        N = w["u0"].shape[1]
+        ref = run(m, w, reltol=1e-10, abstol=1e-12)
+        plain = run(m, w)
+        res["plain"]["kernel_ms"] += plain.kernel_ms
+        at, rt = MIX_TOL[m]
+        Xe = torch.from_numpy(features(m, w["p"], w["u0"], 0.0, w["tf"], at, rt)).cuda()
+        for form, prec in (("learned", "fp64"), ("learned_mixed", "mixed")):
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            Y = net.infer(Xe)
+            e1.record()
+            torch.cuda.synchronize()
+            res[form]["infer_ms"] += e0.elapsed_time(e1)
+            Y = Y.cpu().numpy()
+            dt0 = 10.0 ** Y[:, 0].astype(np.float64)
+            stiff = Y[:, 1] > 0
+            u = np.empty((w["u0"].shape[0], N))
+            rej = steps = 0
+            okall = True
+            for sel, alg in ((np.where(stiff)[0], Rodas5()), (np.where(~stiff)[0], AutoTsit5())):
+                if len(sel) == 0:
+                    continue
+                r = run(m, w, idx=sel, alg=alg, prec=prec, dt0=dt0)
+                res[form]["kernel_ms"] += r.kernel_ms
+                u[:, sel] = r.u_final
+                rej += int(r.nreject.sum())
+                steps += int((r.naccept + r.nreject).sum())
+                okall &= bool((r.retcode == 1).all())
+            res[form].setdefault("all_success", True)
+            res[form]["all_success"] &= okall
+            if form == "learned":
+                _, lab = training_targets(plain.dt_first, plain.naccept, plain.nstiff)
+                agree += int(((lab > 0.5) == stiff).sum())
+                cnt["learned_rej"] += rej
+                cnt["learned_steps"] += steps
+            errs[form].append((np.abs(u - ref.u_final) /
+                               (at + rt * np.abs(ref.u_final))).max(axis=0))
+        cnt["plain_rej"] += int(plain.nreject.sum())
+        cnt["plain_steps"] += int((plain.naccept + plain.nreject).sum())
+        errs["plain"].append((np.abs(plain.u_final - ref.u_final) /
+                              (at + rt * np.abs(ref.u_final))).max(axis=0))
+    for form in res:
+        e = np.concatenate(errs[form])
+        tot = res[form]["kernel_ms"] + res[form].get("infer_ms", 0.0)
+        res[form].update(total_ms=tot, traj_per_s=n_total / (tot * 1e-3),
+                         weighted_err_median=float(np.median(e)),
+                         weighted_err_p99=float(np.quantile(e, 0.99)))
+    res["learned"]["speedup_vs_plain"] = res["plain"]["total_ms"] / res["learned"]["total_ms"]
+    res["learned_mixed"]["speedup_vs_plain"] = (res["plain"]["total_ms"] /
+                                               res["learned_mixed"]["total_ms"])
+    return dict(workload="1e6 mix: 1/3 Lorenz (1e-8), 1/3 Robertson (1e-8/1e-6), 1/3 classic "
+                         "Van der Pol mu log-uniform 1..1e4 tspan (0, 2mu) (1e-8/1e-6)",
+                training=dict(samples=int(len(X)), log_solve_s=t_logs, train_s=t_train,
+                              epochs=4, batch=8192, final_loss=losses[-1] if losses else None),
+                class_agreement_with_autoswitch=agree / n_total,
+                counts=cnt,
+                error_metric="max_c |u - u*| / (abstol + reltol |u*|), u* = FP64 reltol 1e-10",
+                forms=res)
 
 
 def errors(u, ref):
@@ -249,6 +498,15 @@ def main():
                 attempted_steps=h["attempted_steps"], kernel_ms=h["kernel_ms_mean"],
                 output_gbps=out_bytes / (h["kernel_ms_mean"] * 1e-3) / 1e9)
     cpu = None if a.no_cpu else cpu_baseline(a.cpu_sample)
+    other = {}
+    for name in [x for x in a.other.split(",") if x]:
+        print(f"[bench] other config {name}", file=sys.stderr, flush=True)
+        if name == "robertson":
+            other[name] = other_robertson(torch, ctx, max(2, a.steps // 2), 1)
+        elif name == "brusselator":
+            other[name] = other_brusselator(torch, max(2, a.steps // 2), 1)
+        elif name == "mix":
+            other[name] = other_mix(torch)
     line = {
         "metric": "trajectories/s at <=1e-6 final-state err vs FP64 oracle, plain vs tuned form",
         "value": h["traj_per_s"], "unit": "trajectories/s", "n_gpus": 1, "steps": a.steps,
@@ -266,6 +524,10 @@ def main():
     }
     if cpu:
         line["gpu_vs_cpu"] = h["traj_per_s"] / cpu["value"]
+    if "plain" in detail and head != "plain":
+        line["tuned_vs_plain"] = h["traj_per_s"] / detail["plain"]["traj_per_s"]
+    if other:
+        line["other_configs"] = other
     print(json.dumps(line))
 
 

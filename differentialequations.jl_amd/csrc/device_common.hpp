@@ -141,6 +141,38 @@ __device__ __forceinline__ double ctl_pow(double x, double y) {
   return ctl_exp2(y * ctl_log2(x));
 }
 
+// Division-free log2 for the fast controller: log2(1 + t) = t * q(t) on
+// t = m - 1, m in [sqrt(1/2), sqrt(2)), q a degree-8 least-squares fit at
+// Chebyshev nodes (|error| < 1.4e-8).
+__device__ __forceinline__ double fast_log2(double x) {
+  if (!(x > 0.0)) return x == 0.0 ? -__builtin_inf() : __builtin_nan("");
+  if (x == __builtin_inf()) return x;
+  int e;
+  double m = frexp(x, &e);
+  if (m < 0.70710678118654752440) {
+    m = m * 2.0;
+    e = e - 1;
+  }
+  double t = m - 1.0;
+  double q = 0.12614846624505;
+  q = fma(q, t, -0.20742103505566906);
+  q = fma(q, t, 0.2156698589589577);
+  q = fma(q, t, -0.23892033981765542);
+  q = fma(q, t, 0.28791832447651855);
+  q = fma(q, t, -0.36070482809673765);
+  q = fma(q, t, 0.4809106099449815);
+  q = fma(q, t, -0.7213473333598942);
+  q = fma(q, t, 1.442695003764732);
+  return fma(t, q, (double)e);
+}
+
+// log2 of the controller constants (fast form)
+constexpr double kL2QmaxInv = -3.321928094887362;   // log2(1/10)
+constexpr double kL2QminInv = 2.321928094887362;    // log2(5)
+constexpr double kL2Gamma = -0.15200309344504997;   // log2(0.9)
+constexpr double kL2QoldInit = -13.287712379549449; // log2(1e-4)
+constexpr double kL2QsteadyRodas = 0.2630344058337938;  // log2(6/5)
+
 // Julia eps(x): spacing to the next larger float.
 __device__ __forceinline__ double jl_eps(double x) {
   x = fabs(x);

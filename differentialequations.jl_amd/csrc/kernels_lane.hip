@@ -13,21 +13,21 @@
 
 namespace mideq {
 
-template <class M, bool MIX, int ALG>
+template <class M, bool MIX, int ALG, bool FAST>
 __global__ __launch_bounds__(256) void k_lane(LaneArgs a) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.ntraj) return;
-  Lane<M, MIX, ALG> L;
+  Lane<M, MIX, ALG, FAST> L;
   L.init(a, i);
   while (L.rc == RC_DEFAULT) L.step(a);
   L.finish(a);
 }
 
-template <class M, bool MIX, int ALG>
+template <class M, bool MIX, int ALG, bool FAST>
 __global__ __launch_bounds__(256) void k_persistent(LaneArgs a) {
   const int lane = threadIdx.x & 63;
   const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  Lane<M, MIX, ALG> L;
+  Lane<M, MIX, ALG, FAST> L;
   bool active = false;
   bool drained = false;
   while (true) {
@@ -69,7 +69,7 @@ __global__ __launch_bounds__(256) void k_persistent(LaneArgs a) {
 
 __global__ void k_set_u64(unsigned long long* p, unsigned long long v) { *p = v; }
 
-template <class M, bool MIX, int ALG>
+template <class M, bool MIX, int ALG, bool FAST>
 static hipError_t launch_t(const LaneArgs& a, int sched, int block, int grid,
                            hipStream_t s) {
   if (sched == SCHED_PERSISTENT) {
@@ -78,7 +78,7 @@ static hipError_t launch_t(const LaneArgs& a, int sched, int block, int grid,
     if (grid <= 0) {
       int per_cu = 0;
       hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-          &per_cu, k_persistent<M, MIX, ALG>, block, 0);
+          &per_cu, k_persistent<M, MIX, ALG, FAST>, block, 0);
       if (e != hipSuccess || per_cu < 1) per_cu = 1;
       int dev = 0, ncu = 256;
       if (hipGetDevice(&dev) == hipSuccess)
@@ -88,38 +88,42 @@ static hipError_t launch_t(const LaneArgs& a, int sched, int block, int grid,
       grid = (int)(g < need ? g : need);
     }
     hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, s, a.queue, 0ull);
-    hipLaunchKernelGGL((k_persistent<M, MIX, ALG>), dim3(grid), dim3(block), 0, s, a);
+    hipLaunchKernelGGL((k_persistent<M, MIX, ALG, FAST>), dim3(grid), dim3(block), 0, s, a);
   } else {
-    hipLaunchKernelGGL((k_lane<M, MIX, ALG>), dim3(grid), dim3(block), 0, s, a);
+    hipLaunchKernelGGL((k_lane<M, MIX, ALG, FAST>), dim3(grid), dim3(block), 0, s, a);
   }
   return hipGetLastError();
 }
 
-template <class M, bool MIX>
+template <class M, bool MIX, bool FAST>
 static hipError_t launch_alg(const LaneArgs& a, int alg, int sched, int block, int grid,
                              hipStream_t s) {
   switch (alg) {
-    case ALG_TSIT5: return launch_t<M, MIX, ALG_TSIT5>(a, sched, block, grid, s);
-    case ALG_RODAS5: return launch_t<M, MIX, ALG_RODAS5>(a, sched, block, grid, s);
-    default: return launch_t<M, MIX, ALG_AUTO>(a, sched, block, grid, s);
+    case ALG_TSIT5: return launch_t<M, MIX, ALG_TSIT5, FAST>(a, sched, block, grid, s);
+    case ALG_RODAS5: return launch_t<M, MIX, ALG_RODAS5, FAST>(a, sched, block, grid, s);
+    default: return launch_t<M, MIX, ALG_AUTO, FAST>(a, sched, block, grid, s);
   }
 }
 
 template <class M>
-static hipError_t launch_model(const LaneArgs& a, int alg, int mix, int sched, int block,
-                               int grid, hipStream_t s) {
-  return mix ? launch_alg<M, true>(a, alg, sched, block, grid, s)
-             : launch_alg<M, false>(a, alg, sched, block, grid, s);
+static hipError_t launch_model(const LaneArgs& a, int alg, int mix, int fast, int sched,
+                               int block, int grid, hipStream_t s) {
+  if (fast)
+    return mix ? launch_alg<M, true, true>(a, alg, sched, block, grid, s)
+               : launch_alg<M, false, true>(a, alg, sched, block, grid, s);
+  return mix ? launch_alg<M, true, false>(a, alg, sched, block, grid, s)
+             : launch_alg<M, false, false>(a, alg, sched, block, grid, s);
 }
 
-hipError_t launch_lane_solver(const LaneArgs& a, int model, int alg, int mix, int sched,
-                              int block, int grid, hipStream_t s) {
+hipError_t launch_lane_solver(const LaneArgs& a, int model, int alg, int mix, int fast,
+                              int sched, int block, int grid, hipStream_t s) {
   switch (model) {
-    case MODEL_EXP: return launch_model<ModelExp>(a, alg, mix, sched, block, grid, s);
-    case MODEL_LORENZ: return launch_model<ModelLorenz>(a, alg, mix, sched, block, grid, s);
-    case MODEL_ROBERTSON: return launch_model<ModelRobertson>(a, alg, mix, sched, block, grid, s);
-    case MODEL_VDP: return launch_model<ModelVdP>(a, alg, mix, sched, block, grid, s);
-    case MODEL_LINEAR: return launch_model<ModelLinear3>(a, alg, mix, sched, block, grid, s);
+    case MODEL_EXP: return launch_model<ModelExp>(a, alg, mix, fast, sched, block, grid, s);
+    case MODEL_LORENZ: return launch_model<ModelLorenz>(a, alg, mix, fast, sched, block, grid, s);
+    case MODEL_ROBERTSON:
+      return launch_model<ModelRobertson>(a, alg, mix, fast, sched, block, grid, s);
+    case MODEL_VDP: return launch_model<ModelVdP>(a, alg, mix, fast, sched, block, grid, s);
+    case MODEL_LINEAR: return launch_model<ModelLinear3>(a, alg, mix, fast, sched, block, grid, s);
     default: return hipErrorInvalidValue;
   }
 }

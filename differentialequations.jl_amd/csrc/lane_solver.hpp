@@ -66,6 +66,47 @@ __device__ __forceinline__ double err_norm(const double* ut, const double* u0,
   return sqrt(rms_sum_small<N>(r) / (double)N);
 }
 
+// Fast-controller error estimate: EEst^2 from one division over a common
+// denominator (n <= 3); an overflowing numerator stays +inf (rejected step).
+template <int N>
+__device__ __forceinline__ double err_sq_fast(const double* ut, const double* u0,
+                                              const double* u1, double abstol, double reltol) {
+  double sc[N];
+#pragma unroll
+  for (int i = 0; i < N; i++) sc[i] = fma(fmax(fabs(u0[i]), fabs(u1[i])), reltol, abstol);
+  double num, den;
+  if constexpr (N == 1) {
+    num = ut[0] * ut[0];
+    den = sc[0] * sc[0];
+  } else if constexpr (N == 2) {
+    double a0 = ut[0] * sc[1], a1 = ut[1] * sc[0];
+    num = a0 * a0;
+    num = fma(a1, a1, num);
+    double P = sc[0] * sc[1];
+    den = (P * P) * 2.0;
+  } else {
+    static_assert(N == 3, "fast controller supports n <= 3");
+    double p01 = sc[0] * sc[1], p02 = sc[0] * sc[2], p12 = sc[1] * sc[2];
+    double a0 = ut[0] * p12, a1 = ut[1] * p02, a2 = ut[2] * p01;
+    num = a0 * a0;
+    num = fma(a1, a1, num);
+    num = fma(a2, a2, num);
+    double P = p01 * sc[2];
+    den = (P * P) * 3.0;
+  }
+  if (!(num < __builtin_inf())) return num;
+  return num / den;
+}
+
+template <int N, bool FAST>
+__device__ __forceinline__ double err_est(const double* ut, const double* u0, const double* u1,
+                                          double abstol, double reltol) {
+  if constexpr (FAST)
+    return err_sq_fast<N>(ut, u0, u1, abstol, reltol);
+  else
+    return err_norm<N>(ut, u0, u1, abstol, reltol);
+}
+
 template <int N>
 __device__ __forceinline__ double rms(const double* x) {
   return sqrt(rms_sum_small<N>(x) / (double)N);
@@ -84,7 +125,7 @@ __device__ __forceinline__ void tsit5_bthetas(double th, double* b) {
   b[6] = th2 * fma(th, fma(th, r74, r73), r72);
 }
 
-template <class M, bool MIX, int ALG>
+template <class M, bool MIX, int ALG, bool FAST>
 struct Lane {
   static constexpr int N = M::N;
   static constexpr int NP = M::NP;
@@ -99,7 +140,7 @@ struct Lane {
   double k1[N];   // FSAL stage (FP64 form)
   float k1f[N];   // FSAL stage (FP32 form)
   double t, tcomp, tf, dt, dtmax;
-  double qold, qoldpow, qsteady_max, eig, dtfirst;
+  double qold, qoldpow, qsteady_max, eig, dtfirst, lqold;
   int64_t iter;
   int32_t naccept, nreject, nf, nstiff;
   int32_t isave, count, nanrej;
@@ -214,6 +255,7 @@ struct Lane {
       dt = initdt(a, t0);
     qold = kQoldInit;
     qoldpow = ctl_pow(kQoldInit, kBeta2);
+    lqold = kL2QoldInit;
     if (!(tf > t0)) rc = RC_SUCCESS;
   }
 
@@ -296,7 +338,7 @@ struct Lane {
       acc = fma(bt1, k[0][c], acc);
       ut[c] = dts * acc;
     }
-    return err_norm<N>(ut, u, u1, abstol, reltol);
+    return err_est<N, FAST>(ut, u, u1, abstol, reltol);
   }
 
   __device__ double tsit5_step_mix(double dts, float k[7][N], double* u1, double abstol,
@@ -382,7 +424,7 @@ struct Lane {
       acc = fmaf((float)bt1, k[0][c], acc);
       ut[c] = dts * (double)acc;
     }
-    return err_norm<N>(ut, u, u1, abstol, reltol);
+    return err_est<N, FAST>(ut, u, u1, abstol, reltol);
   }
 
   // ------------------------------------------------------------ Rodas5
@@ -544,7 +586,7 @@ struct Lane {
       acc = acc + kd[6][c];
       u1[c] = acc + kd[7][c];
     }
-    EEst = err_norm<N>(kd[7], u, u1, abstol, reltol);
+    EEst = err_est<N, FAST>(kd[7], u, u1, abstol, reltol);
     return true;
   }
 
@@ -612,24 +654,43 @@ struct Lane {
     } else {
       nanrej = 0;
     }
-    // PI controller
-    double q, q11 = 1.0, lE = 0.0;
-    if (EEst == 0.0) {
-      q = 1.0 / kQmax;
+    // PI controller (the fast form works on EEst^2 in the log2 domain)
+    double q = 1.0, q11 = 1.0, lE = 0.0, lq = 0.0, l11 = 0.0, lE2 = 0.0;
+    if constexpr (!FAST) {
+      if (EEst == 0.0) {
+        q = 1.0 / kQmax;
+      } else {
+        lE = ctl_log2(EEst);
+        q11 = ctl_exp2(kBeta1 * lE);
+        q = q11 / qoldpow;
+        q = fmax(1.0 / kQmax, fmin(1.0 / kQmin, q / kGammaSafety));
+      }
     } else {
-      lE = ctl_log2(EEst);
-      q11 = ctl_exp2(kBeta1 * lE);
-      q = q11 / qoldpow;
-      q = fmax(1.0 / kQmax, fmin(1.0 / kQmin, q / kGammaSafety));
+      if (EEst == 0.0) {
+        lq = kL2QmaxInv;
+      } else {
+        lE2 = fast_log2(EEst);
+        l11 = (0.5 * kBeta1) * lE2;
+        lq = l11 - kBeta2 * lqold - kL2Gamma;
+        lq = fmax(kL2QmaxInv, fmin(kL2QminInv, lq));
+      }
     }
     if (EEst <= 1.0) {
       if (naccept == 0) dtfirst = dt;
       naccept++;
       if (!tsit_step) nstiff++;
-      if (1.0 <= q && q <= qsteady_max) q = 1.0;
-      qold = fmax(EEst, kQoldInit);
-      qoldpow = (qold == EEst) ? ctl_exp2(kBeta2 * lE) : ctl_pow(kQoldInit, kBeta2);
-      double dtnew = dt / q;
+      double dtnew;
+      if constexpr (!FAST) {
+        if (1.0 <= q && q <= qsteady_max) q = 1.0;
+        qold = fmax(EEst, kQoldInit);
+        qoldpow = (qold == EEst) ? ctl_exp2(kBeta2 * lE) : ctl_pow(kQoldInit, kBeta2);
+        dtnew = dt / q;
+      } else {
+        double lqs = qsteady_max > 1.0 ? kL2QsteadyRodas : 0.0;
+        if (0.0 <= lq && lq <= lqs) lq = 0.0;
+        lqold = (EEst == 0.0) ? kL2QoldInit : fmax(0.5 * lE2, kL2QoldInit);
+        dtnew = dt * ctl_exp2(-lq);
+      }
       double tprev = t;
       double ttmp;
       if (!MIX) {
@@ -704,7 +765,10 @@ struct Lane {
       dt = fmin(dtmax, dtnew);
     } else {
       nreject++;
-      dt = dt / fmin(1.0 / kQmin, q11 / kGammaSafety);
+      if constexpr (!FAST)
+        dt = dt / fmin(1.0 / kQmin, q11 / kGammaSafety);
+      else
+        dt = dt * ctl_exp2(-fmin(kL2QminInv, l11 - kL2Gamma));
     }
     if (!(t < tf)) rc = RC_SUCCESS;
   }

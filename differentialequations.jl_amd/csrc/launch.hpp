@@ -13,8 +13,8 @@ enum { MODEL_EXP = 0, MODEL_LORENZ = 1, MODEL_ROBERTSON = 2, MODEL_VDP = 3,
 enum { SCHED_AUTO = 0, SCHED_LANE = 1, SCHED_PERSISTENT = 2 };
 
 // One lane per trajectory (n <= 3 models).
-hipError_t launch_lane_solver(const LaneArgs& a, int model, int alg, int mix, int sched,
-                              int block, int grid, hipStream_t s);
+hipError_t launch_lane_solver(const LaneArgs& a, int model, int alg, int mix, int fast,
+                              int sched, int block, int grid, hipStream_t s);
 int lane_model_dims(int model, int* n, int* np);
 
 }  // namespace mideq

@@ -187,6 +187,8 @@ static int validate(mideq_ctx* c, const mideq_problem* P, const mideq_options* O
     if (O->ns > 0) return set_err(c, MIDEQ_E_UNSUPPORTED, "saveat not provided for the brusselator");
     if (O->lanes != 0 && O->lanes != 32 && O->lanes != 64)
       return set_err(c, MIDEQ_E_ARG, "lanes must be 0, 32 or 64");
+    if (O->controller != 0)
+      return set_err(c, MIDEQ_E_UNSUPPORTED, "the fast controller needs n <= 3");
   } else if (P->n != n) {
     return set_err(c, MIDEQ_E_ARG, "model %d has n=%d, got n=%d", P->model, n, P->n);
   }
@@ -204,6 +206,7 @@ static int validate(mideq_ctx* c, const mideq_problem* P, const mideq_options* O
     return set_err(c, MIDEQ_E_UNSUPPORTED, "saveat needs Tsit5 (Rodas5 dense output not provided)");
   if (O->save_dtype < 0 || O->save_dtype > 1) return set_err(c, MIDEQ_E_ARG, "bad save_dtype");
   if (O->schedule < 0 || O->schedule > 2) return set_err(c, MIDEQ_E_ARG, "bad schedule");
+  if (O->controller < 0 || O->controller > 1) return set_err(c, MIDEQ_E_ARG, "bad controller");
   if (O->block != 0 && (O->block < 64 || O->block > 256 || (O->block & 63)))
     return set_err(c, MIDEQ_E_ARG, "block must be 64..256, multiple of 64");
   return MIDEQ_OK;
@@ -421,10 +424,11 @@ static int run_solver(mideq_ctx* c, const mideq_problem* P, const mideq_options*
   HIP_TRY(c, hipEventRecord(c->ev0, s));
   struct Key {
     LaneArgs a;
-    int model, alg, prec, sched, block, grid;
-  } key{a, P->model, O->alg, O->precision, sched, block, grid};
+    int model, alg, prec, ctl, sched, block, grid;
+  } key{a, P->model, O->alg, O->precision, O->controller, sched, block, grid};
   int rc = enqueue(c, O->graph != 0, &key, sizeof key, s, [&](hipStream_t st) {
-    return launch_lane_solver(a, P->model, O->alg, O->precision, sched, block, grid, st);
+    return launch_lane_solver(a, P->model, O->alg, O->precision, O->controller, sched, block,
+                              grid, st);
   });
   if (rc) return rc;
   HIP_TRY(c, hipEventRecord(c->ev1, s));

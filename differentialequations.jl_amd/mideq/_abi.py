@@ -40,7 +40,8 @@ class Options(C.Structure):
                 ("ns", C.c_int32), ("save_dtype", C.c_int32), ("dt0", C.c_void_p),
                 ("alg_hint", C.c_void_p), ("qsteady_default", C.c_int32),
                 ("schedule", C.c_int32), ("block", C.c_int32), ("blocks_per_cu", C.c_int32),
-                ("lanes", C.c_int32), ("graph", C.c_int32)]
+                ("lanes", C.c_int32), ("graph", C.c_int32), ("controller", C.c_int32),
+                ("reserved1", C.c_int32)]
 
 
 class Outputs(C.Structure):
@@ -92,11 +93,28 @@ def load(path=LIB_PATH):
     return L
 
 
+def _init_torch_gpu_first():
+    """PyTorch-ROCm bundles its own HIP runtime; it fails to start ("No HIP
+    GPUs are available") if another HIP runtime in the process opened the
+    GPU first.  When torch is installed, let it initialise before the
+    library does, so that both can share device memory in one process."""
+    try:
+        import torch
+    except ImportError:
+        return
+    try:
+        if torch.cuda.is_available():
+            torch.cuda.init()
+    except Exception:
+        pass
+
+
 class Context:
     """One mideq_ctx: a HIP stream and cached device buffers on one GPU."""
 
     def __init__(self, device=0):
         self.lib = load()
+        _init_torch_gpu_first()
         h = C.c_void_p()
         rc = self.lib.mideq_ctx_create(C.byref(h), device)
         if rc != 0:

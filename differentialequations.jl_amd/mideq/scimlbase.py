@@ -182,6 +182,7 @@ class EnsembleMI355X:
     alg_hint: Any = None
     lanes: int = 0
     graph: bool = False
+    controller: str = "reference"   # or "fast" (n <= 3 models)
 
 
 # ---------------------------------------------------------------- solutions
@@ -346,6 +347,7 @@ def device_solve(f, u0, p, t0, tf, nvec, alg, ealg, saveat=None, abstol=1e-6, re
     O.blocks_per_cu = blocks_per_cu
     O.lanes = ealg.lanes
     O.graph = 1 if ealg.graph else 0
+    O.controller = dict(reference=0, fast=1)[ealg.controller]
     f32 = ealg.save_dtype == "float32"
     O.save_dtype = 1 if f32 else 0
     us = None

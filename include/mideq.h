@@ -118,6 +118,10 @@ typedef struct {
   int32_t blocks_per_cu;/* persistent grid size per CU, 0 = tuned */
   int32_t lanes;        /* lanes per trajectory for large systems (32, 64), 0 = autotuned */
   int32_t graph;        /* 1: replay the launch sequence as a cached hipGraph */
+  int32_t controller;   /* 0: PI controller as OrdinaryDiffEq states it; 1: same PI law
+                           evaluated on EEst^2 in the log2 domain without divisions
+                           (n <= 3 models) */
+  int32_t reserved1;
 } mideq_options;
 
 /* Results; any pointer may be NULL. */

@@ -177,6 +177,34 @@ double oracle_ctl_pow(double x, double y) {
   return oracle_ctl_exp2(y * oracle_ctl_log2(x));
 }
 
+/* Division-free log2 for the fast controller: log2(1 + t) = t * q(t) on
+ * t = m - 1, m in [sqrt(1/2), sqrt(2)), q a degree-8 least-squares fit at
+ * Chebyshev nodes (|error| < 1.4e-8). */
+static const double FL2[9] = {1.442695003764732, -0.7213473333598942, 0.4809106099449815,
+                              -0.36070482809673765, 0.28791832447651855, -0.23892033981765542,
+                              0.2156698589589577, -0.20742103505566906, 0.12614846624505};
+
+double oracle_fast_log2(double x) {
+  if (!(x > 0.0)) return x == 0.0 ? -INFINITY : NAN;
+  if (x == INFINITY) return INFINITY;
+  int e;
+  double m = frexp(x, &e);
+  if (m < SQRT_HALF_D) {
+    m = m * 2.0;
+    e = e - 1;
+  }
+  double t = m - 1.0;
+  double q = FL2[8];
+  for (int i = 7; i >= 0; i--) q = fma(q, t, FL2[i]);
+  return fma(t, q, (double)e);
+}
+
+/* log2 of the controller constants */
+#define L2_QMAXINV (-3.321928094887362)   /* log2(1/10) */
+#define L2_QMININV 2.321928094887362      /* log2(5) */
+#define L2_GAMMA (-0.15200309344504997)   /* log2(0.9) */
+#define L2_QOLDINIT (-13.287712379549449) /* log2(1e-4) */
+
 /* PI controller (stepsize_controller! for PIController):                    */
 /*   q11 = EEst^beta1, q = clamp(q11 / qold^beta2 / gamma, 1/qmax, 1/qmin),   */
 /*   EEst == 0 gives q = 1/qmax.                                              */
@@ -446,9 +474,44 @@ static double rms_norm(const double *x, int n, int L) {
   return sqrt(s / (double)n);
 }
 
-/* calculate_residuals + internalnorm: EEst of an error vector. */
+/* Set per solve: 1 selects the fast controller, whose error estimate is
+ * EEst^2 from one division over a common denominator (n <= 3). */
+static __thread int g_fast_ctl = 0;
+
+static double err_sq_fast(const double *ut, const double *u0, const double *u1, int n,
+                          double abstol, double reltol) {
+  double sc[3];
+  for (int i = 0; i < n; i++) sc[i] = fma(fmax(fabs(u0[i]), fabs(u1[i])), reltol, abstol);
+  double num, den;
+  if (n == 1) {
+    num = ut[0] * ut[0];
+    den = sc[0] * sc[0];
+  } else if (n == 2) {
+    double a0 = ut[0] * sc[1], a1 = ut[1] * sc[0];
+    num = a0 * a0;
+    num = fma(a1, a1, num);
+    double P = sc[0] * sc[1];
+    den = (P * P) * 2.0;
+  } else {
+    double p01 = sc[0] * sc[1], p02 = sc[0] * sc[2], p12 = sc[1] * sc[2];
+    double a0 = ut[0] * p12, a1 = ut[1] * p02, a2 = ut[2] * p01;
+    num = a0 * a0;
+    num = fma(a1, a1, num);
+    num = fma(a2, a2, num);
+    double P = p01 * sc[2];
+    den = (P * P) * 3.0;
+  }
+  /* a far too large trial step can overflow the squared products: keep +inf
+   * (a rejected step) instead of inf/inf; NaN stays NaN */
+  if (!(num < INFINITY)) return num;
+  return num / den;
+}
+
+/* calculate_residuals + internalnorm: EEst of an error vector (EEst^2 in
+ * the fast-controller mode). */
 static double err_norm(const double *ut, const double *u0, const double *u1,
                        int n, double abstol, double reltol, int L) {
+  if (g_fast_ctl && n <= 3) return err_sq_fast(ut, u0, u1, n, abstol, reltol);
   double r[NMAX];
   for (int i = 0; i < n; i++) {
     double sc = fma(fmax(fabs(u0[i]), fabs(u1[i])), reltol, abstol);
@@ -900,6 +963,9 @@ static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
   int rc = ORACLE_RC_DEFAULT;
   int have_accept = 0;
   int nanrej = 0;
+  g_fast_ctl = (o->controller == 1 && n <= 3);
+  double lqold = L2_QOLDINIT;
+  double lqs_cur = 0.0; /* log2 of qsteady_max: 0 or log2(1.2) */
 
   if (!(tf > t0)) rc = ORACLE_RC_SUCCESS;
   while (rc == ORACLE_RC_DEFAULT && t < tf) {
@@ -970,8 +1036,19 @@ static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
       nanrej = 0;
     }
 
-    /* stepsize_controller! (PI) */
-    double q = pi_stepsize(EEst, qold, &q11);
+    /* stepsize_controller! (PI); the fast form works on EEst^2 in log2 */
+    double q = 1.0, lq = 0.0, l11 = 0.0, lE2 = 0.0;
+    lqs_cur = qsteady_max > 1.0 ? 0.2630344058337938 : 0.0;
+    if (!g_fast_ctl) {
+      q = pi_stepsize(EEst, qold, &q11);
+    } else if (EEst == 0.0) {
+      lq = L2_QMAXINV;
+    } else {
+      lE2 = oracle_fast_log2(EEst);
+      l11 = (0.5 * CTL_BETA1) * lE2;
+      lq = l11 - CTL_BETA2 * lqold - L2_GAMMA;
+      lq = fmax(L2_QMAXINV, fmin(L2_QMININV, lq));
+    }
 
     if (EEst <= 1.0) {
       /* step_accept_controller! */
@@ -981,9 +1058,16 @@ static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
         res->dt_first_accept = dt;
         have_accept = 1;
       }
-      if (1.0 <= q && q <= qsteady_max) q = 1.0;
-      qold = fmax(EEst, CTL_QOLDINIT);
-      double dtnew = dt / q;
+      double dtnew;
+      if (!g_fast_ctl) {
+        if (1.0 <= q && q <= qsteady_max) q = 1.0;
+        qold = fmax(EEst, CTL_QOLDINIT);
+        dtnew = dt / q;
+      } else {
+        if (0.0 <= lq && lq <= lqs_cur) lq = 0.0;
+        lqold = (EEst == 0.0) ? L2_QOLDINIT : fmax(0.5 * lE2, L2_QOLDINIT);
+        dtnew = dt * oracle_ctl_exp2(-lq);
+      }
       double tprev = t;
       double ttmp;
       if (!mixed) {
@@ -1032,7 +1116,10 @@ static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
     } else {
       /* step_reject_controller! */
       res->nreject++;
-      dt = dt / fmin(1.0 / CTL_QMIN, q11 / CTL_GAMMA);
+      if (!g_fast_ctl)
+        dt = dt / fmin(1.0 / CTL_QMIN, q11 / CTL_GAMMA);
+      else
+        dt = dt * oracle_ctl_exp2(-fmin(L2_QMININV, l11 - L2_GAMMA));
     }
   }
   if (rc == ORACLE_RC_DEFAULT) rc = ORACLE_RC_SUCCESS;
@@ -1042,6 +1129,7 @@ static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
   for (int c = 0; c < n; c++) res->u[c] = uprev[c];
   res->t = t;
   res->retcode = rc;
+  g_fast_ctl = 0;
   return rc;
 }
 

@@ -76,6 +76,7 @@ typedef struct {
   const double *saveat;   /* [ns] ascending absolute times, may be NULL */
   int32_t ns;
   int32_t qsteady_default;/* 1: method default qsteady band (Tsit5 [1,1], Rodas5 [1,6/5]) */
+  int32_t controller;     /* 0: PI controller as restated; 1: fast form (EEst^2, log2 domain, n <= 3) */
   int32_t norm_lanes;     /* n > 8: RMS sum as norm_lanes strided partials + pairwise tree
                              (the order of a team of that many GPU lanes); 0 = 64 */
 } oracle_options;
@@ -117,6 +118,7 @@ int oracle_rodas5_tableau(double *gamma, double *a64, double *C64, double *c8,
 double oracle_ctl_log2(double x);
 double oracle_ctl_exp2(double x);
 double oracle_ctl_pow(double x, double y);
+double oracle_fast_log2(double x);
 /* One PI-controller decision: returns 1 (accept) or 0 (reject) and the next
  * dt / qold, exactly as the integrator loop applies them. */
 int oracle_controller(double EEst, double qold, double dt, double qsteady_max,

@@ -33,6 +33,7 @@ class Options(C.Structure):
         ("saveat", _dp),
         ("ns", C.c_int32),
         ("qsteady_default", C.c_int32),
+        ("controller", C.c_int32),
         ("norm_lanes", C.c_int32),
     ]
 
@@ -76,6 +77,8 @@ class Oracle:
         for nm in ("oracle_ctl_log2", "oracle_ctl_exp2"):
             getattr(L, nm).argtypes = [C.c_double]
             getattr(L, nm).restype = C.c_double
+        L.oracle_fast_log2.argtypes = [C.c_double]
+        L.oracle_fast_log2.restype = C.c_double
         L.oracle_ctl_pow.argtypes = [C.c_double, C.c_double]
         L.oracle_ctl_pow.restype = C.c_double
         L.oracle_controller.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, _dp, _dp]
@@ -94,7 +97,8 @@ class Oracle:
     # ---------------------------------------------------------------- options
     @staticmethod
     def options(alg="tsit5", precision="fp64", abstol=1e-6, reltol=1e-3, dt=0.0, dtmin=0.0,
-                dtmax=0.0, maxiters=0, saveat=None, qsteady_default=1, norm_lanes=64):
+                dtmax=0.0, maxiters=0, saveat=None, qsteady_default=1, norm_lanes=64,
+                controller=0):
         o = Options()
         o.alg = ALG[alg]
         o.precision = PREC[precision]
@@ -106,6 +110,7 @@ class Oracle:
         o.maxiters = maxiters
         o.qsteady_default = qsteady_default
         o.norm_lanes = norm_lanes
+        o.controller = controller
         if saveat is not None:
             sa = np.ascontiguousarray(saveat, dtype=np.float64)
             o._saveat_keep = sa

@@ -139,3 +139,29 @@ def test_per_trajectory_tspans_and_fixed_dt(oracle):
     g = gpu("lorenz", w["u0"], w["p"], 0.0, 1.0, dt=1e-3, abstol=1e-9, reltol=1e-9)
     c = cpu(oracle, "lorenz", w["u0"], w["p"], 0.0, 1.0, dt=1e-3, abstol=1e-9, reltol=1e-9)
     assert_identical(g, c)
+
+
+@pytest.mark.parametrize("model,alg,precision", [("lorenz", "tsit5", "fp64"), ("lorenz", "tsit5", "mixed"),
+                                                 ("robertson", "rodas5", "fp64"), ("vdp", "auto", "fp64"),
+                                                 ("robertson", "auto", "mixed")])
+def test_fast_controller_bitexact(oracle, model, alg, precision):
+    """The division-free controller (EEst^2 in log2) against the oracle's
+    restatement of the same form."""
+    N = 1024 if model != "vdp" else 256
+    say(f"fast controller {model} {alg} {precision} N={N}")
+    if model == "lorenz":
+        w = workloads.lorenz(N, seed=61)
+        tf, tol, sa = 10.0, dict(abstol=1e-8, reltol=1e-8), workloads.saveat_uniform(0.0, 10.0, 100)
+    elif model == "robertson":
+        w = workloads.robertson(N, seed=62)
+        tf, tol, sa = 1e5, dict(abstol=1e-8, reltol=1e-6), None
+    else:
+        w = workloads.vdp(N, seed=63)
+        w["p"] = np.minimum(w["p"], 300.0)
+        tf, tol, sa = 2.0 * w["p"][0], dict(abstol=1e-8, reltol=1e-6), None
+    g = gpu(model, w["u0"], w["p"], 0.0, tf, alg=alg, precision=precision, saveat=sa,
+            controller="fast", **tol)
+    c = cpu(oracle, model, w["u0"], w["p"], 0.0, tf, alg=alg, precision=precision, saveat=sa,
+            controller=1, **tol)
+    assert (c["retcode"] == 1).all()
+    assert_identical(g, c)
