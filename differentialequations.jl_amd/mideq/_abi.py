@@ -77,6 +77,15 @@ def load(path=LIB_PATH):
         return _LIB
     if not os.path.exists(path):
         raise MideqError(f"libmideq.so not built: {path} (run __graft_entry__.build())")
+    # PyTorch-ROCm ships its own HIP runtime under the same SONAME
+    # (libamdhip64.so.7).  Loaded first, it is the one libmideq binds to, so
+    # the process has a single HIP runtime and torch tensors and library
+    # kernels share one device context.  Loaded after libmideq, torch would
+    # pull in a second runtime that cannot open the GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(path)
     L.mideq_version.restype = C.c_int
     L.mideq_ctx_create.argtypes = [C.POINTER(C.c_void_p), C.c_int32]
@@ -93,28 +102,11 @@ def load(path=LIB_PATH):
     return L
 
 
-def _init_torch_gpu_first():
-    """PyTorch-ROCm bundles its own HIP runtime; it fails to start ("No HIP
-    GPUs are available") if another HIP runtime in the process opened the
-    GPU first.  When torch is installed, let it initialise before the
-    library does, so that both can share device memory in one process."""
-    try:
-        import torch
-    except ImportError:
-        return
-    try:
-        if torch.cuda.is_available():
-            torch.cuda.init()
-    except Exception:
-        pass
-
-
 class Context:
     """One mideq_ctx: a HIP stream and cached device buffers on one GPU."""
 
     def __init__(self, device=0):
         self.lib = load()
-        _init_torch_gpu_first()
         h = C.c_void_p()
         rc = self.lib.mideq_ctx_create(C.byref(h), device)
         if rc != 0:

@@ -70,7 +70,6 @@ class MLP:
 
     def __init__(self, device=0, params=None, seed=0):
         self.L = _lib()
-        _abi._init_torch_gpu_first()
         h = C.c_void_p()
         rc = self.L.mideq_mlp_create(C.byref(h), device)
         if rc != 0:
