@@ -15,6 +15,7 @@ namespace mideq {
 
 template <class M, bool MIX, int ALG, bool FAST>
 __global__ __launch_bounds__(256) void k_lane(LaneArgs a) {
+  lane_stage_saveat(a);
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.ntraj) return;
   Lane<M, MIX, ALG, FAST> L;
@@ -27,6 +28,7 @@ template <class M, bool MIX, int ALG, bool FAST>
 __global__ __launch_bounds__(256) void k_persistent(LaneArgs a) {
   const int lane = threadIdx.x & 63;
   const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  lane_stage_saveat(a);
   Lane<M, MIX, ALG, FAST> L;
   bool active = false;
   bool drained = false;
@@ -72,13 +74,14 @@ __global__ void k_set_u64(unsigned long long* p, unsigned long long v) { *p = v;
 template <class M, bool MIX, int ALG, bool FAST>
 static hipError_t launch_t(const LaneArgs& a, int sched, int block, int grid,
                            hipStream_t s) {
+  const size_t lds = lane_lds_bytes(a.ns);
   if (sched == SCHED_PERSISTENT) {
     // grid <= 0: as many blocks as can be resident at once (a truly
     // persistent grid), from the occupancy of this instantiation.
     if (grid <= 0) {
       int per_cu = 0;
       hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-          &per_cu, k_persistent<M, MIX, ALG, FAST>, block, 0);
+          &per_cu, k_persistent<M, MIX, ALG, FAST>, block, lds);
       if (e != hipSuccess || per_cu < 1) per_cu = 1;
       int dev = 0, ncu = 256;
       if (hipGetDevice(&dev) == hipSuccess)
@@ -88,9 +91,9 @@ static hipError_t launch_t(const LaneArgs& a, int sched, int block, int grid,
       grid = (int)(g < need ? g : need);
     }
     hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, s, a.queue, 0ull);
-    hipLaunchKernelGGL((k_persistent<M, MIX, ALG, FAST>), dim3(grid), dim3(block), 0, s, a);
+    hipLaunchKernelGGL((k_persistent<M, MIX, ALG, FAST>), dim3(grid), dim3(block), lds, s, a);
   } else {
-    hipLaunchKernelGGL((k_lane<M, MIX, ALG, FAST>), dim3(grid), dim3(block), 0, s, a);
+    hipLaunchKernelGGL((k_lane<M, MIX, ALG, FAST>), dim3(grid), dim3(block), lds, s, a);
   }
   return hipGetLastError();
 }

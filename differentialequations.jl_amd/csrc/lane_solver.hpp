@@ -44,6 +44,26 @@ struct LaneArgs {
   unsigned long long* queue;  // persistent schedule: next trajectory index
 };
 
+// saveat times are staged in LDS by the lane kernels when they fit, so the
+// per-step "next save point reached?" test reads a register and a save reads
+// LDS: no global load (and no vmcnt wait, which on this ISA also waits for
+// the previous step's saveat stores) sits in the step loop.
+constexpr int kSaveLds = 2048;
+extern __shared__ double lane_saveat_lds[];
+
+// Dynamic LDS bytes a lane kernel launch needs for ns save times.
+__host__ __device__ inline size_t lane_lds_bytes(int32_t ns) {
+  return (ns > 0 && ns <= kSaveLds) ? (size_t)ns * sizeof(double) : 0;
+}
+
+// Whole workgroup: copy the save times into LDS (call before any lane exits).
+__device__ __forceinline__ void lane_stage_saveat(const LaneArgs& a) {
+  if (a.ns > 0 && a.ns <= kSaveLds) {
+    for (int k = threadIdx.x; k < a.ns; k += blockDim.x) lane_saveat_lds[k] = a.saveat[k];
+    __syncthreads();
+  }
+}
+
 template <int N>
 __device__ __forceinline__ double rms_sum_small(const double* r) {
   double s = r[0] * r[0];
@@ -145,18 +165,29 @@ struct Lane {
   int32_t naccept, nreject, nf, nstiff;
   int32_t isave, count, nanrej;
   int8_t rc, mode;
+  double tnext;  // saveat[isave], +inf once every point is saved
+
+  // ------------------------------------------------------------ save times
+  __device__ __forceinline__ double save_time(const LaneArgs& a, int k) const {
+    if (a.ns <= kSaveLds) return lane_saveat_lds[k];
+    return ((const __attribute__((address_space(1))) double*)a.saveat)[k];
+  }
+  __device__ __forceinline__ void set_tnext(const LaneArgs& a) {
+    tnext = isave < a.ns ? save_time(a, isave) : __builtin_inf();
+  }
 
   // ------------------------------------------------------------ helpers
   __device__ __forceinline__ void store_save(const LaneArgs& a, int k, const double* v) {
     const int64_t NT = a.ntraj;
+    const int64_t o = (int64_t)k * (N * NT) + i;
     if (a.save_f32) {
-      float* out = (float*)a.us;
+      float* out = (float*)a.us + o;
 #pragma unroll
-      for (int c = 0; c < N; c++) out[((int64_t)k * N + c) * NT + i] = (float)v[c];
+      for (int c = 0; c < N; c++) out[c * NT] = (float)v[c];
     } else {
-      double* out = (double*)a.us;
+      double* out = (double*)a.us + o;
 #pragma unroll
-      for (int c = 0; c < N; c++) out[((int64_t)k * N + c) * NT + i] = v[c];
+      for (int c = 0; c < N; c++) out[c * NT] = v[c];
     }
   }
 
@@ -241,11 +272,12 @@ struct Lane {
       double nanv[N];
 #pragma unroll
       for (int c = 0; c < N; c++) nanv[c] = __builtin_nan("");
-      while (isave < a.ns && a.saveat[isave] < t0) store_save(a, isave++, nanv);
-      while (isave < a.ns && a.saveat[isave] == t0) store_save(a, isave++, u);
+      while (isave < a.ns && save_time(a, isave) < t0) store_save(a, isave++, nanv);
+      while (isave < a.ns && save_time(a, isave) == t0) store_save(a, isave++, u);
     } else {
-      while (isave < a.ns && a.saveat[isave] <= t0) isave++;
+      while (isave < a.ns && save_time(a, isave) <= t0) isave++;
     }
+    set_tnext(a);
     fsal_init();
     if (a.dt_fixed > 0)
       dt = a.dt_fixed;
@@ -703,11 +735,11 @@ struct Lane {
       if (fabs(ttmp - tf) < 100.0 * jl_eps(fmax(t, tf))) ttmp = tf;
       t = ttmp;
       // savevalues!
-      if (a.ns > 0) {
-        while (isave < a.ns && a.saveat[isave] <= t) {
+      {
+        while (tnext <= t) {
           if (a.us) {
             double val[N];
-            double sa = a.saveat[isave];
+            double sa = tnext;
             if (sa == t) {
 #pragma unroll
               for (int c = 0; c < N; c++) val[c] = u1[c];
@@ -742,6 +774,7 @@ struct Lane {
             store_save(a, isave, val);
           }
           isave++;
+          set_tnext(a);
         }
       }
 #pragma unroll
