@@ -309,58 +309,59 @@ MIX_TOL = {"lorenz": (1e-8, 1e-8), "robertson": (1e-8, 1e-6), "vdp": (1e-8, 1e-6
 
 
 def other_mix(torch, n_total=1_000_000, n_train=1_000_000):
-    """Learned dt0 / method class on the 1e6 Lorenz + Robertson + Van der Pol
-    mix: plain = AutoTsit5(Rodas5()) FP64 with the initdt heuristic;
-    learned+batched = MLP inference, trajectories of the predicted stiff
-    class solved by Rodas5 and the rest by AutoTsit5 (each group its own
-    launch), predicted dt0, in FP64 and in the FP32-stage form.  The net is
-    trained on the plain solver's logs of a separately seeded mix."""
+    """Learned dt0 / method class / cost on the 1e6 Lorenz + Robertson + Van
+    der Pol mix: plain = AutoTsit5(Rodas5()) FP64 with the initdt heuristic,
+    natural order; learned+batched = MLP inference, then one AutoTsit5
+    launch per model with the predicted dt0, the predicted class as the
+    initial method and a launch order that groups trajectories by predicted
+    class and cost (longest first), in FP64 and in the FP32-stage form.  The
+    net is trained on the plain solver's logs of a separately seeded mix."""
     import mideq
     from mideq import EnsembleMI355X, device_solve
-    from mideq.learned import MLP, features, training_targets
-    from mideq.scimlbase import AutoTsit5, Rodas5
+    from mideq.learned import MLP, features, launch_order, training_targets
+    from mideq.scimlbase import AutoTsit5
     F = {"lorenz": mideq.lorenz, "robertson": mideq.robertson, "vdp": mideq.vanderpol}
 
-    def run(m, w, idx=None, alg=None, prec="fp64", dt0=None, reltol=None, abstol=None):
+    def run(m, w, prec="fp64", dt0=None, hint=None, order=None, reltol=None, abstol=None):
         at, rt = MIX_TOL[m]
         at = abstol or at
         rt = reltol or rt
         u0, p, tf = w["u0"], w["p"], w["tf"]
-        if idx is not None:
-            u0, p, tf = u0[:, idx].copy(), p[:, idx].copy(), tf[idx].copy()
-            dt0 = None if dt0 is None else dt0[idx].copy()
         N = u0.shape[1]
-        return device_solve(F[m], u0, p, np.zeros(N), tf, None, alg or AutoTsit5(),
+        return device_solve(F[m], u0, p, np.zeros(N), tf, None, AutoTsit5(),
                             EnsembleMI355X(precision=prec), abstol=at, reltol=rt, dt0=dt0,
-                            maxiters=1_000_000)
+                            alg_hint=hint, order=order, maxiters=1_000_000)
 
     t_logs = time.perf_counter()
     train = _mix_sets(n_train // 3, seed=777)
-    Xs, Ts, Ls = [], [], []
+    Xs, Ts, Ls, Cs = [], [], [], []
     for m, w in train.items():
         r = run(m, w)
-        t, l = training_targets(r.dt_first, r.naccept, r.nstiff)
+        t, l, c = training_targets(r.dt_first, r.naccept, r.nstiff, r.nreject)
         ok = r.retcode == 1
         at, rt = MIX_TOL[m]
         Xs.append(features(m, w["p"], w["u0"], 0.0, w["tf"], at, rt)[ok])
         Ts.append(t[ok])
         Ls.append(l[ok])
+        Cs.append(c[ok])
     X = np.concatenate(Xs)
     T = np.concatenate(Ts)
     Lb = np.concatenate(Ls)
+    Cb = np.concatenate(Cs)
     t_logs = time.perf_counter() - t_logs
     net = MLP(seed=0)
     net.set_norm(X.mean(0), X.std(0) + 1e-6)
-    Xd, Td, Ld = (torch.from_numpy(a).cuda() for a in (X, T, Lb))
+    Xd, Td, Ld, Cd = (torch.from_numpy(a).cuda() for a in (X, T, Lb, Cb))
     torch.cuda.synchronize()
     t_train = time.perf_counter()
-    losses = net.fit(Xd, Td, Ld, epochs=4, batch=8192, lr=3e-3)
+    losses = net.fit(Xd, Td, Ld, Cd, epochs=4, batch=8192, lr=3e-3)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t_train
 
     evals = _mix_sets(n_total // 3, seed=1234)
-    res = {"plain": dict(kernel_ms=0.0), "learned": dict(kernel_ms=0.0, infer_ms=0.0),
-           "learned_mixed": dict(kernel_ms=0.0, infer_ms=0.0)}
+    res = {"plain": dict(kernel_ms=0.0, per_model_ms={}),
+           "learned": dict(kernel_ms=0.0, infer_ms=0.0, per_model_ms={}),
+           "learned_mixed": dict(kernel_ms=0.0, infer_ms=0.0, per_model_ms={})}
     agree = 0
     cnt = dict(plain_rej=0, learned_rej=0, plain_steps=0, learned_steps=0)
     errs = {k: [] for k in res}
@@ -369,35 +370,32 @@ def other_mix(torch, n_total=1_000_000, n_train=1_000_000):
         ref = run(m, w, reltol=1e-10, abstol=1e-12)
         plain = run(m, w)
         res["plain"]["kernel_ms"] += plain.kernel_ms
+        res["plain"]["per_model_ms"][m] = plain.kernel_ms
         at, rt = MIX_TOL[m]
         Xe = torch.from_numpy(features(m, w["p"], w["u0"], 0.0, w["tf"], at, rt)).cuda()
         for form, prec in (("learned", "fp64"), ("learned_mixed", "mixed")):
             e0 = torch.cuda.Event(enable_timing=True)
             e1 = torch.cuda.Event(enable_timing=True)
             e0.record()
-            Y = net.infer(Xe)
+            Yd = net.infer(Xe)
+            od = launch_order(Yd[:, 1], Yd[:, 2])
             e1.record()
             torch.cuda.synchronize()
             res[form]["infer_ms"] += e0.elapsed_time(e1)
-            Y = Y.cpu().numpy()
+            Y = Yd.cpu().numpy()
+            order = od.cpu().numpy()
             dt0 = 10.0 ** Y[:, 0].astype(np.float64)
             stiff = Y[:, 1] > 0
-            u = np.empty((w["u0"].shape[0], N))
-            rej = steps = 0
-            okall = True
-            for sel, alg in ((np.where(stiff)[0], Rodas5()), (np.where(~stiff)[0], AutoTsit5())):
-                if len(sel) == 0:
-                    continue
-                r = run(m, w, idx=sel, alg=alg, prec=prec, dt0=dt0)
-                res[form]["kernel_ms"] += r.kernel_ms
-                u[:, sel] = r.u_final
-                rej += int(r.nreject.sum())
-                steps += int((r.naccept + r.nreject).sum())
-                okall &= bool((r.retcode == 1).all())
+            r = run(m, w, prec=prec, dt0=dt0, hint=stiff.astype(np.int8), order=order)
+            res[form]["kernel_ms"] += r.kernel_ms
+            res[form]["per_model_ms"][m] = r.kernel_ms
+            u = r.u_final
+            rej = int(r.nreject.sum())
+            steps = int((r.naccept + r.nreject).sum())
             res[form].setdefault("all_success", True)
-            res[form]["all_success"] &= okall
+            res[form]["all_success"] &= bool((r.retcode == 1).all())
             if form == "learned":
-                _, lab = training_targets(plain.dt_first, plain.naccept, plain.nstiff)
+                _, lab, _ = training_targets(plain.dt_first, plain.naccept, plain.nstiff)
                 agree += int(((lab > 0.5) == stiff).sum())
                 cnt["learned_rej"] += rej
                 cnt["learned_steps"] += steps

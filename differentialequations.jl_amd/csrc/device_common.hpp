@@ -92,6 +92,17 @@ constexpr double kStiffTol = 0.9, kDtFac = 2.0;
 enum : int8_t { RC_DEFAULT = 0, RC_SUCCESS = 1, RC_MAXITERS = 2, RC_DTLESSTHANMIN = 3,
                 RC_UNSTABLE = 4, RC_DTNAN = 5, RC_INITIALFAILURE = 6 };
 
+// FP64 constant pinned to an SGPR pair at its point of use.  gfx9 VOP3
+// instructions take one scalar operand and no literal; an unpinned constant
+// addend of a Horner step is first copied into a VGPR pair (two v_mov_b32 on
+// the vector ALU) so that the fma can be shrunk to v_fmac.  Pinned, it costs
+// two scalar moves and the fma stays one VOP3 instruction.  Same value, same
+// operation: results are unchanged.
+__device__ __forceinline__ double kpin(double c) {
+  asm volatile("" : "+s"(c));
+  return c;
+}
+
 // ---------------------------------------------------------------------------
 // Controller power x^y = exp2(y log2 x): atanh series for ln of the mantissa
 // in [sqrt(1/2), sqrt(2)), Taylor series of 2^f for |f| <= 1/2.  Relative
@@ -109,10 +120,10 @@ __device__ __forceinline__ double ctl_log2(double x) {
   double s = (m - 1.0) / (m + 1.0);
   double s2 = s * s;
   double p = fma(s2, 1.0 / 13.0, 1.0 / 11.0);
-  p = fma(s2, p, 1.0 / 9.0);
-  p = fma(s2, p, 1.0 / 7.0);
-  p = fma(s2, p, 1.0 / 5.0);
-  p = fma(s2, p, 1.0 / 3.0);
+  p = fma(s2, p, kpin(1.0 / 9.0));
+  p = fma(s2, p, kpin(1.0 / 7.0));
+  p = fma(s2, p, kpin(1.0 / 5.0));
+  p = fma(s2, p, kpin(1.0 / 3.0));
   p = fma(s2, p, 1.0);
   double lnm = (2.0 * s) * p;
   return fma(lnm, 1.4426950408889634074, (double)e);
@@ -126,11 +137,11 @@ __device__ __forceinline__ double ctl_exp2(double z) {
   double f = z - nn;
   double x = f * 0.69314718055994530942;
   double p = fma(x, 1.0 / 362880.0, 1.0 / 40320.0);
-  p = fma(x, p, 1.0 / 5040.0);
-  p = fma(x, p, 1.0 / 720.0);
-  p = fma(x, p, 1.0 / 120.0);
-  p = fma(x, p, 1.0 / 24.0);
-  p = fma(x, p, 1.0 / 6.0);
+  p = fma(x, p, kpin(1.0 / 5040.0));
+  p = fma(x, p, kpin(1.0 / 720.0));
+  p = fma(x, p, kpin(1.0 / 120.0));
+  p = fma(x, p, kpin(1.0 / 24.0));
+  p = fma(x, p, kpin(1.0 / 6.0));
   p = fma(x, p, 0.5);
   p = fma(x, p, 1.0);
   p = fma(x, p, 1.0);
@@ -156,13 +167,13 @@ __device__ __forceinline__ double fast_log2(double x) {
   double t = m - 1.0;
   double q = 0.12614846624505;
   q = fma(q, t, -0.20742103505566906);
-  q = fma(q, t, 0.2156698589589577);
-  q = fma(q, t, -0.23892033981765542);
-  q = fma(q, t, 0.28791832447651855);
-  q = fma(q, t, -0.36070482809673765);
-  q = fma(q, t, 0.4809106099449815);
-  q = fma(q, t, -0.7213473333598942);
-  q = fma(q, t, 1.442695003764732);
+  q = fma(q, t, kpin(0.2156698589589577));
+  q = fma(q, t, kpin(-0.23892033981765542));
+  q = fma(q, t, kpin(0.28791832447651855));
+  q = fma(q, t, kpin(-0.36070482809673765));
+  q = fma(q, t, kpin(0.4809106099449815));
+  q = fma(q, t, kpin(-0.7213473333598942));
+  q = fma(q, t, kpin(1.442695003764732));
   return fma(t, q, (double)e);
 }
 

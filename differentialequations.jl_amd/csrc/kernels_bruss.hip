@@ -887,11 +887,14 @@ __global__ __launch_bounds__(256) void k_bruss(BrussArgs a) {
         drained = true;
       } else {
         int64_t idx = a.order ? a.order[q] : (int64_t)q;
-        T.init(a, idx);
-        if (T.rc == RC_DEFAULT)
-          active = true;
-        else
-          T.finish(a);
+        // an order entry outside [0, ntraj) is skipped
+        if ((uint64_t)idx < (uint64_t)a.ntraj) {
+          T.init(a, idx);
+          if (T.rc == RC_DEFAULT)
+            active = true;
+          else
+            T.finish(a);
+        }
       }
     }
     if (!__any(active || !drained)) break;

@@ -4,6 +4,8 @@
 // Two schedules:
 //   lane        one thread per trajectory; the hardware dispatcher refills
 //               CU slots wave by wave as waves retire.
+// Both walk the trajectories in the caller's launch order when one is given
+// (consecutive entries share a wavefront); results land at natural indices.
 //   persistent  a fixed grid; each lane that finishes (detected with a
 //               wavefront ballot) takes the next trajectory from a global
 //               atomic counter, one atomic per wave per refill, so lanes
@@ -16,8 +18,10 @@ namespace mideq {
 template <class M, bool MIX, int ALG, bool FAST>
 __global__ __launch_bounds__(256) void k_lane(LaneArgs a) {
   lane_stage_saveat(a);
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.ntraj) return;
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= a.ntraj) return;
+  int64_t i = lane_traj(a, k);
+  if (i < 0) return;
   Lane<M, MIX, ALG, FAST> L;
   L.init(a, i);
   while (L.rc == RC_DEFAULT) L.step(a);
@@ -44,8 +48,9 @@ __global__ __launch_bounds__(256) void k_persistent(LaneArgs a) {
         if (base + (unsigned long long)cnt >= (unsigned long long)a.ntraj) drained = true;
         if (!active) {
           unsigned long long my = base + (unsigned long long)__popcll(need & lt_mask);
-          if (my < (unsigned long long)a.ntraj) {
-            L.init(a, (int64_t)my);
+          int64_t i = my < (unsigned long long)a.ntraj ? lane_traj(a, (int64_t)my) : -1;
+          if (i >= 0) {
+            L.init(a, i);
             if (L.rc == RC_DEFAULT) {
               active = true;
             } else {

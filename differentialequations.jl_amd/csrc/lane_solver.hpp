@@ -41,8 +41,16 @@ struct LaneArgs {
   int32_t* nf;
   int32_t* nstiff;
   double* dt_first;
-  unsigned long long* queue;  // persistent schedule: next trajectory index
+  unsigned long long* queue;  // persistent schedule: next position in the order
+  const int64_t* order;       // launch order (NULL: natural order)
 };
+
+// Trajectory scheduled at position k of the launch order; -1 for an entry
+// outside [0, ntraj) (skipped, so a bad order cannot address out of bounds).
+__device__ __forceinline__ int64_t lane_traj(const LaneArgs& a, int64_t k) {
+  int64_t i = a.order ? a.order[k] : k;
+  return (uint64_t)i < (uint64_t)a.ntraj ? i : -1;
+}
 
 // saveat times are staged in LDS by the lane kernels when they fit, so the
 // per-step "next save point reached?" test reads a register and a save reads
@@ -160,6 +168,7 @@ struct Lane {
   double k1[N];   // FSAL stage (FP64 form)
   float k1f[N];   // FSAL stage (FP32 form)
   double t, tcomp, tf, dt, dtmax;
+  double tsnap;  // 100 eps(tf): a step ending this close to tf ends at tf
   double qold, qoldpow, qsteady_max, eig, dtfirst, lqold;
   int64_t iter;
   int32_t naccept, nreject, nf, nstiff;
@@ -255,6 +264,8 @@ struct Lane {
     t = t0;
     tcomp = 0.0;
     dtmax = a.dtmax > 0 ? a.dtmax : fabs(tf - t0);
+    // the step loop runs while t < tf, so max(t, tf) there is tf
+    tsnap = 100.0 * jl_eps(tf);
     naccept = nreject = nstiff = 0;
     nf = 1;
     iter = 0;
@@ -732,7 +743,7 @@ struct Lane {
         ttmp = t + y;
         tcomp = (ttmp - t) - y;
       }
-      if (fabs(ttmp - tf) < 100.0 * jl_eps(fmax(t, tf))) ttmp = tf;
+      if (fabs(ttmp - tf) < tsnap) ttmp = tf;
       t = ttmp;
       // savevalues!
       {

@@ -410,6 +410,7 @@ static int run_solver(mideq_ctx* c, const mideq_problem* P, const mideq_options*
   a.nstiff = out->nstiff;
   a.dt_first = out->dt_first;
   a.queue = c->d_queue;
+  a.order = d_order;
   int sched = O->schedule == MIDEQ_SCHED_AUTO ? SCHED_LANE : O->schedule;
   int block = O->block > 0 ? O->block : 256;
   int grid;
@@ -450,7 +451,20 @@ extern "C" int mideq_ensemble_solve_device(mideq_ctx* c, const mideq_problem* P,
   if (!out) return set_err(c, MIDEQ_E_ARG, "null outputs");
   HIP_TRY(c, hipSetDevice(c->device));
   hipStream_t s = stream ? (hipStream_t)stream : c->stream;
-  return run_solver(c, P, O, out, nullptr, s);
+  return run_solver(c, P, O, out, O->order, s);
+}
+
+// A launch order must be a permutation of 0..N-1.
+static int check_order_host(mideq_ctx* c, const int64_t* order, int64_t N) {
+  std::vector<unsigned char> seen((size_t)N, 0);
+  for (int64_t k = 0; k < N; k++) {
+    int64_t i = order[k];
+    if (i < 0 || i >= N || seen[(size_t)i])
+      return set_err(c, MIDEQ_E_ARG, "order is not a permutation of 0..ntraj-1 (entry %lld = %lld)",
+                     (long long)k, (long long)i);
+    seen[(size_t)i] = 1;
+  }
+  return MIDEQ_OK;
 }
 
 extern "C" int mideq_ensemble_solve(mideq_ctx* c, const mideq_problem* P,
@@ -516,11 +530,17 @@ extern "C" int mideq_ensemble_solve(mideq_ctx* c, const mideq_problem* P,
     HIP_TRY(c, hipMemcpyAsync(d, O->alg_hint, N, hipMemcpyHostToDevice, s));
     dO.alg_hint = (const int8_t*)d;
   }
-  // work order for the large-system kernel: most expensive first (LPT),
-  // cost ~ n * Ngrid^2 * (tf - t0), the Tsit5 work when stability-bound
+  // launch order: the caller's, else for the large-system kernel most
+  // expensive first (LPT), cost ~ n * Ngrid^2 * (tf - t0), the Tsit5 work
+  // when stability-bound
   std::vector<int64_t> order;
   const int64_t* d_order = nullptr;
-  if (P->model == MODEL_BRUSSELATOR && O->schedule != MIDEQ_SCHED_LANE) {
+  if (O->order) {
+    if ((rc = check_order_host(c, O->order, N))) return rc;
+    if ((rc = ensure_buf(c, BUF_ORDER, sizeof(int64_t) * N, &d))) return rc;
+    HIP_TRY(c, hipMemcpyAsync(d, O->order, sizeof(int64_t) * N, hipMemcpyHostToDevice, s));
+    d_order = (const int64_t*)d;
+  } else if (P->model == MODEL_BRUSSELATOR && O->schedule != MIDEQ_SCHED_LANE) {
     order.resize(N);
     std::iota(order.begin(), order.end(), 0);
     std::vector<double> cost(N);

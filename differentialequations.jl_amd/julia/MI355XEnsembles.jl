@@ -58,6 +58,7 @@ struct MideqOptions
     dt0::Ptr{Float64}; alg_hint::Ptr{Int8}
     qsteady_default::Int32; schedule::Int32; block::Int32; blocks_per_cu::Int32
     lanes::Int32; graph::Int32; controller::Int32; reserved1::Int32
+    order::Ptr{Int64}
 end
 
 struct MideqOutputs
@@ -158,7 +159,7 @@ function __solve(ens::SciMLBase.AbstractEnsembleProblem, alg, ealg::EnsembleMI35
                                  ns > 0 ? pointer(sa) : C_NULL, ns,
                                  ealg.save_dtype === Float32 ? 1 : 0, C_NULL, C_NULL, 1,
                                  ealg.schedule === :lane ? 1 : ealg.schedule === :persistent ? 2 : 0,
-                                 0, 0, 0, 0, 0, 0)
+                                 0, 0, 0, 0, 0, 0, C_NULL)
             out_c = MideqOutputs(ns > 0 ? pointer(US) : C_NULL, pointer(UF), pointer(TFo),
                                  pointer(RC), pointer(NA), pointer(NR), pointer(NF), pointer(NS),
                                  C_NULL)

@@ -1,18 +1,21 @@
-"""Learned initial step and method class for ensemble solves.
+"""Learned initial step, method class and cost for ensemble solves.
 
-A 16 -> 3x64 (GELU) -> 2 MLP, run by the BF16 matrix-core kernels of
+A 16 -> 3x64 (GELU) -> 3 MLP, run by the BF16 matrix-core kernels of
 libmideq.so (`mideq_mlp_*`), predicts for every trajectory
   * log10 of a step size the controller accepts first time (the plain
     solver's first accepted step), used as `dt0`;
   * a stiffness logit, used as the initial method of AutoTsit5(Rodas5())
-    (`alg_hint`) and to batch trajectories of one class together so that
-    wavefronts do not mix Tsit5 and Rodas5 lanes.
+    (`alg_hint`);
+  * log2 of the solve's cost (attempted steps, a Rodas5 step counted twice),
+    which with the class sets the launch `order`: trajectories of one class
+    and similar cost share a wavefront, so lanes neither wait for a much
+    longer neighbour nor mix Tsit5 and Rodas5 steps.
 The plain initdt heuristic and AutoTsit5 switching stay in place: the
-prediction only sets where each trajectory starts, and the solver's own
-controller and stiffness test still decide every later step.
+prediction only sets where each trajectory starts and when it is scheduled,
+and the solver's own controller and stiffness test still decide every step.
 
-Training data are the plain solver's own logs (first accepted step and
-accepted Rodas5 steps per trajectory).  Training runs on the GPU with the
+Training data are the plain solver's own logs (first accepted step, accepted
+Rodas5 steps and attempted steps per trajectory).  Training runs on the GPU with the
 library's forward/backward kernels and Adam; tensors live in torch device
 memory, passed to the C ABI as raw pointers.
 """
@@ -22,8 +25,9 @@ import numpy as np
 
 from . import _abi
 
-NIN, NOUT = 16, 2
-NPARAM = 9538
+NIN, NOUT = 16, 3
+NPARAM = 9603
+RODAS_STEP_COST = 2.0  # a Rodas5 step in Tsit5 steps, for the cost target
 MODEL_SLOT = {"lorenz": 0, "robertson": 1, "vdp": 2}
 
 
@@ -40,7 +44,8 @@ def _lib():
         L.mideq_mlp_get_grads.argtypes = [vp, vp]
         L.mideq_mlp_set_norm.argtypes = [vp, vp, vp]
         L.mideq_mlp_infer.argtypes = [vp, vp, C.c_int64, vp, vp]
-        L.mideq_mlp_grad.argtypes = [vp, vp, vp, vp, C.c_int64, C.c_float, C.c_float, vp, vp]
+        L.mideq_mlp_grad.argtypes = [vp, vp, vp, vp, vp, C.c_int64, C.c_float, C.c_float,
+                                     C.c_float, vp, vp]
         L.mideq_mlp_adam_step.argtypes = [vp, C.c_float, C.c_float, C.c_float, C.c_float, vp]
         L._mlp_declared = True
     return L
@@ -127,11 +132,13 @@ class MLP:
         self._check(self.L.mideq_mlp_infer(self.h, X.data_ptr(), B, Y.data_ptr(), s), "infer")
         return Y
 
-    def grad(self, X, target, label, w_dt=1.0, w_cls=1.0, want_loss=True, stream=None):
+    def grad(self, X, target, label, cost=None, w_dt=1.0, w_cls=1.0, w_cost=1.0,
+             want_loss=True, stream=None):
         loss = C.c_float(0.0)
         s = None if stream is None else stream.cuda_stream
+        cp = None if cost is None else cost.data_ptr()
         self._check(self.L.mideq_mlp_grad(self.h, X.data_ptr(), target.data_ptr(), label.data_ptr(),
-                                          X.shape[0], w_dt, w_cls,
+                                          cp, X.shape[0], w_dt, w_cls, w_cost,
                                           C.byref(loss) if want_loss else None, s), "grad")
         return loss.value if want_loss else None
 
@@ -139,11 +146,22 @@ class MLP:
         s = None if stream is None else stream.cuda_stream
         self._check(self.L.mideq_mlp_adam_step(self.h, lr, beta1, beta2, eps, s), "adam")
 
-    def fit(self, X, target, label, epochs=4, batch=8192, lr=2e-3, seed=0, w_cls=1.0,
-            log=None):
-        """Minibatch Adam over device tensors X [N,16], target [N], label [N]."""
+    def fit(self, X, target, label, cost=None, epochs=4, batch=8192, lr=2e-3, seed=0,
+            w_cls=1.0, w_cost=0.25, init_bias=True, log=None):
+        """Minibatch Adam over device tensors X [N,16], target [N], label [N]
+        and (optional) cost [N].  init_bias: start the output biases at the
+        target means and the label log-odds, so that no head's offset
+        dominates the loss at the start (pass False to continue training)."""
         import torch
         N = X.shape[0]
+        if init_bias:
+            theta = self.get_params()
+            pl = float(label.float().mean().clamp(1e-3, 1 - 1e-3))
+            theta[NPARAM - 3] = float(target.float().mean())
+            theta[NPARAM - 2] = float(np.log(pl / (1.0 - pl)))
+            if cost is not None:
+                theta[NPARAM - 1] = float(cost.float().mean())
+            self.set_params(theta)
         g = torch.Generator(device="cpu").manual_seed(seed)
         losses = []
         for ep in range(epochs):
@@ -154,8 +172,9 @@ class MLP:
                 xb = X.index_select(0, idx).contiguous()
                 tb = target.index_select(0, idx).contiguous()
                 lb = label.index_select(0, idx).contiguous()
+                cb = None if cost is None else cost.index_select(0, idx).contiguous()
                 want = (nb % 16 == 0)
-                lv = self.grad(xb, tb, lb, 1.0, w_cls, want_loss=want)
+                lv = self.grad(xb, tb, lb, cb, 1.0, w_cls, w_cost, want_loss=want)
                 self.adam_step(lr * (0.5 ** ep))
                 if want:
                     tot += lv
@@ -178,8 +197,25 @@ def init_params(seed=0):
     return np.concatenate(parts).astype(np.float32)
 
 
-def training_targets(dt_first, naccept, nstiff):
-    """(log10 first accepted dt, stiff label) from plain-solver logs."""
+def training_targets(dt_first, naccept, nstiff, nreject=None):
+    """(log10 first accepted dt, stiff label, log2 cost) from plain-solver
+    logs; cost = attempted steps with an accepted Rodas5 step counted
+    RODAS_STEP_COST times."""
     tgt = np.log10(np.maximum(dt_first, 1e-300)).astype(np.float32)
     lab = (nstiff > 0.5 * np.maximum(naccept, 1)).astype(np.float32)
-    return tgt, lab
+    rej = 0 if nreject is None else nreject
+    cost = naccept + rej + (RODAS_STEP_COST - 1.0) * nstiff
+    return tgt, lab, np.log2(np.maximum(cost, 1.0)).astype(np.float32)
+
+
+def launch_order(stiff_logit, log2_cost):
+    """Launch order from predictions: predicted non-stiff trajectories first,
+    each class by decreasing predicted cost (longest first, so the last
+    wavefronts to start are the short ones).  Works on numpy arrays or torch
+    tensors (device sort)."""
+    if hasattr(stiff_logit, "device"):
+        import torch
+        key = torch.where(stiff_logit > 0, 1.0e6, 0.0).double() - log2_cost.double()
+        return torch.argsort(key, stable=True)
+    key = np.where(np.asarray(stiff_logit) > 0, 1.0e6, 0.0) - np.asarray(log2_cost, np.float64)
+    return np.argsort(key, kind="stable")

@@ -173,13 +173,17 @@ class EnsembleMI355X:
     precision: "fp64" or "mixed" (FP32 stages; FP64 state update, norm,
     time, J/W/LU).  schedule: "auto", "lane" or "persistent".
     dt0 / alg_hint: optional per-trajectory initial step and initial method
-    (e.g. from mideq.learned).  device: GPU ordinal."""
+    (e.g. from mideq.learned).  order: optional launch order, a permutation
+    of the trajectory indices (0-based) -- consecutive entries share a
+    wavefront, so grouping trajectories of similar cost cuts lane
+    divergence; results keep their natural positions.  device: GPU ordinal."""
     precision: str = "fp64"
     schedule: str = "auto"
     device: int = 0
     save_dtype: str = "float64"
     dt0: Any = None
     alg_hint: Any = None
+    order: Any = None
     lanes: int = 0
     graph: bool = False
     controller: str = "reference"   # or "fast" (n <= 3 models)
@@ -312,7 +316,7 @@ def _ptr(a):
 
 def device_solve(f, u0, p, t0, tf, nvec, alg, ealg, saveat=None, abstol=1e-6, reltol=1e-3,
                  dt=0.0, dtmin=0.0, dtmax=0.0, maxiters=0, dt0=None, alg_hint=None,
-                 block=0, blocks_per_cu=0):
+                 order=None, block=0, blocks_per_cu=0):
     """One call of mideq_ensemble_solve on host arrays -> EnsembleArrays."""
     ctx = _abi.context(ealg.device)
     n, N = u0.shape
@@ -363,6 +367,12 @@ def device_solve(f, u0, p, t0, tf, nvec, alg, ealg, saveat=None, abstol=1e-6, re
     if hint is not None:
         hint = np.ascontiguousarray(hint, dtype=np.int8)
         O.alg_hint = _ptr(hint)
+    order = order if order is not None else ealg.order
+    if order is not None:
+        order = np.ascontiguousarray(order, dtype=np.int64)
+        if order.shape != (N,):
+            raise ValueError(f"order has shape {order.shape}, expected ({N},)")
+        O.order = _ptr(order)
     res = dict(u_final=np.zeros((n, N)), t_final=np.empty(N), retcode=np.empty(N, np.int8),
                naccept=np.empty(N, np.int32), nreject=np.empty(N, np.int32),
                nf=np.empty(N, np.int32), nstiff=np.empty(N, np.int32), dt_first=np.empty(N))
@@ -374,6 +384,21 @@ def device_solve(f, u0, p, t0, tf, nvec, alg, ealg, saveat=None, abstol=1e-6, re
     st = ctx.stats()
     return EnsembleArrays(saveat=saveat, us=us, t0=t0, u0=u0, kernel_ms=st.kernel_ms,
                           total_ms=st.total_ms, lanes=st.lanes, graph=bool(st.graph), **res)
+
+
+def _subset_ealg(ealg, pending):
+    """The per-trajectory settings of `ealg` restricted to the trajectories
+    `pending` (a rerun pass), the launch order kept as their relative order."""
+    kw = {}
+    if ealg.dt0 is not None:
+        kw["dt0"] = np.asarray(ealg.dt0)[pending]
+    if ealg.alg_hint is not None:
+        kw["alg_hint"] = np.asarray(ealg.alg_hint)[pending]
+    if ealg.order is not None:
+        rank = np.empty(len(ealg.order), dtype=np.int64)
+        rank[np.asarray(ealg.order, dtype=np.int64)] = np.arange(len(ealg.order))
+        kw["order"] = np.argsort(rank[pending], kind="stable")
+    return replace(ealg, **kw) if kw else ealg
 
 
 # ---------------------------------------------------------------- solve
@@ -405,7 +430,8 @@ def solve(prob, alg, ensemblealg=None, *, trajectories=None, saveat=None, abstol
     last_arrays = None
     while len(pending):
         f, u0, p, t0, tf, nvec, probs = _pack(prob, pending, repeat)
-        arr = device_solve(f, u0, p, t0, tf, nvec, alg, ensemblealg, saveat=sa, abstol=abstol,
+        ealg = ensemblealg if repeat == 1 else _subset_ealg(ensemblealg, pending)
+        arr = device_solve(f, u0, p, t0, tf, nvec, alg, ealg, saveat=sa, abstol=abstol,
                            reltol=reltol, dt=dt, dtmin=dtmin, dtmax=dtmax,
                            maxiters=maxiters, **kw)
         last_arrays = arr if last_arrays is None else last_arrays

@@ -122,6 +122,13 @@ typedef struct {
                            evaluated on EEst^2 in the log2 domain without divisions
                            (n <= 3 models) */
   int32_t reserved1;
+  const int64_t *order; /* [ntraj] launch order or NULL (natural order).  A permutation
+                           of 0..ntraj-1: entry k is the k-th trajectory scheduled, so
+                           consecutive entries share a wavefront -- grouping trajectories
+                           of similar cost and method cuts lane divergence.  Results stay
+                           at natural positions.  The host entry rejects a non-permutation;
+                           on the device entry, entries outside [0, ntraj) are skipped.
+                           Overrides the built-in cost order of the Brusselator queue. */
 } mideq_options;
 
 /* Results; any pointer may be NULL. */
@@ -169,16 +176,17 @@ int mideq_ensemble_solve_device(mideq_ctx *ctx, const mideq_problem *prob,
                                 const mideq_options *opt, mideq_outputs *out,
                                 void *stream);
 
-/* ---- learned dt0 / method-class predictor (no counterpart in the
- * reference: it feeds dt0 and alg_hint above; the initdt heuristic and
- * AutoTsit5 switching remain the fallback).  16 -> 64 -> 64 -> 64 -> 2 GELU
- * MLP, BF16 weights/activations, FP32 accumulation on the matrix cores.
+/* ---- learned dt0 / method-class / cost predictor (no counterpart in the
+ * reference: it feeds dt0, alg_hint and order above; the initdt heuristic
+ * and AutoTsit5 switching remain the fallback).  16 -> 64 -> 64 -> 64 -> 3
+ * GELU MLP, BF16 weights/activations, FP32 accumulation on the matrix cores.
  * Parameters (float32, W[in][out] row-major): W1[16x64] b1[64] W2[64x64]
- * b2[64] W3[64x64] b3[64] W4[64x2] b4[2].  Outputs per sample: (log10 dt0,
- * stiffness logit).  X, Y, target, label are device pointers. */
+ * b2[64] W3[64x64] b3[64] W4[64x3] b4[3].  Outputs per sample: (log10 dt0,
+ * stiffness logit, log2 cost).  X, Y, target, label, cost are device
+ * pointers. */
 #define MIDEQ_MLP_NIN 16
-#define MIDEQ_MLP_NOUT 2
-#define MIDEQ_MLP_NPARAM 9538
+#define MIDEQ_MLP_NOUT 3
+#define MIDEQ_MLP_NPARAM 9603
 typedef struct mideq_mlp mideq_mlp;
 int mideq_mlp_create(mideq_mlp **out, int32_t device);
 void mideq_mlp_destroy(mideq_mlp *m);
@@ -186,12 +194,15 @@ const char *mideq_mlp_last_error(const mideq_mlp *m);
 int mideq_mlp_set_params(mideq_mlp *m, const float *params);   /* host [NPARAM] */
 int mideq_mlp_get_params(mideq_mlp *m, float *params);         /* host [NPARAM] */
 int mideq_mlp_set_norm(mideq_mlp *m, const float *mean, const float *std); /* host [16] */
-/* Y[B][2] = net(X[B][16]) */
+/* Y[B][3] = net(X[B][16]) */
 int mideq_mlp_infer(mideq_mlp *m, const float *X, int64_t B, float *Y, void *stream);
-/* gradient of w_dt*mean((y0-target)^2) + w_cls*mean(BCE(y1,label)) into the
- * internal buffer; loss_host (may be NULL) receives the loss (synchronizes) */
+/* gradient of w_dt*mean((y0-target)^2) + w_cls*mean(BCE(y1,label))
+ * + w_cost*mean((y2-cost)^2) into the internal buffer (cost may be NULL:
+ * that term is dropped); loss_host (may be NULL) receives the loss
+ * (synchronizes) */
 int mideq_mlp_grad(mideq_mlp *m, const float *X, const float *target, const float *label,
-                   int64_t B, float w_dt, float w_cls, float *loss_host, void *stream);
+                   const float *cost, int64_t B, float w_dt, float w_cls, float w_cost,
+                   float *loss_host, void *stream);
 int mideq_mlp_get_grads(mideq_mlp *m, float *grads);           /* host [NPARAM] */
 int mideq_mlp_adam_step(mideq_mlp *m, float lr, float beta1, float beta2, float eps,
                         void *stream);

@@ -52,18 +52,27 @@ def test_mlp_gradients_match_reference():
     net, theta, X, Xd, mean, std, rng = _setup(B, 4)
     target = rng.standard_normal(B).astype(np.float32)
     label = (rng.uniform(size=B) > 0.4).astype(np.float32)
-    loss = net.grad(Xd, torch.from_numpy(target).cuda(), torch.from_numpy(label).cuda(), 1.0, 0.5)
-    g = net.get_grads()
-    Lr, gr, _ = mlp_ref.loss_and_grad(theta, X, target.astype(np.float64),
-                                      label.astype(np.float64), mean, std, 1.0, 0.5)
-    assert abs(loss - Lr) < 1e-4 * abs(Lr) + 1e-6
-    k = 0
-    for name, shp in mlp_ref.SHAPES:
-        n = int(np.prod(shp))
-        a, b = g[k:k + n], gr[k:k + n]
-        rel = np.linalg.norm(a - b) / (np.linalg.norm(b) + 1e-30)
-        assert rel < 1e-2, (name, rel)
-        k += n
+    cost = (8.0 + 2.0 * rng.standard_normal(B)).astype(np.float32)
+    for with_cost in (True, False):
+        cd = torch.from_numpy(cost).cuda() if with_cost else None
+        loss = net.grad(Xd, torch.from_numpy(target).cuda(), torch.from_numpy(label).cuda(), cd,
+                        1.0, 0.5, 0.3)
+        g = net.get_grads()
+        Lr, gr, _ = mlp_ref.loss_and_grad(theta, X, target.astype(np.float64),
+                                          label.astype(np.float64), mean, std, 1.0, 0.5,
+                                          cost=cost.astype(np.float64) if with_cost else None,
+                                          w_cost=0.3)
+        assert abs(loss - Lr) < 1e-4 * abs(Lr) + 1e-6, (with_cost, loss, Lr)
+        k = 0
+        for name, shp in mlp_ref.SHAPES:
+            n = int(np.prod(shp))
+            a, b = g[k:k + n], gr[k:k + n]
+            rel = np.linalg.norm(a - b) / (np.linalg.norm(b) + 1e-30)
+            assert rel < 1e-2, (with_cost, name, rel)
+            k += n
+        if not with_cost:
+            # no cost target: the cost column of W4 and its bias get no gradient
+            assert g[-1] == 0.0 and np.all(g[-3 - 3 * 64 + 2:-3:3] == 0.0)
 
 
 def test_mlp_training_lowers_loss():
@@ -75,16 +84,22 @@ def test_mlp_training_lowers_loss():
     X = rng.standard_normal((N, 16)).astype(np.float32)
     target = (np.sin(X[:, 0]) + 0.5 * X[:, 1] * X[:, 2]).astype(np.float32)
     label = (X[:, 3] + X[:, 4] > 0).astype(np.float32)
+    cost = (6.0 + X[:, 5] - 0.5 * X[:, 6] ** 2).astype(np.float32)
     net = MLP(seed=1)
     net.set_norm(np.zeros(16), np.ones(16))
     Xd = torch.from_numpy(X).cuda()
     td = torch.from_numpy(target).cuda()
     ld = torch.from_numpy(label).cuda()
-    l0 = net.grad(Xd, td, ld)
-    losses = net.fit(Xd, td, ld, epochs=6, batch=4096, lr=3e-3)
-    l1 = net.grad(Xd, td, ld)
+    cd = torch.from_numpy(cost).cuda()
+    l0 = net.grad(Xd, td, ld, cd)
+    losses = net.fit(Xd, td, ld, cd, epochs=6, batch=4096, lr=3e-3)
+    l1 = net.grad(Xd, td, ld, cd)
     assert l1 < 0.2 * l0, (l0, l1, losses[-5:])
     # classification accuracy on the training data
     Y = net.infer(Xd).cpu().numpy()
     acc = np.mean((Y[:, 1] > 0) == (label > 0.5))
     assert acc > 0.95
+    # the cost head ranks trajectories: rank correlation with the target (the
+    # same training emulated with oracle/mlp_ref.py reaches 0.961 / 0.90)
+    rk = lambda v: np.argsort(np.argsort(v))
+    assert np.corrcoef(rk(Y[:, 2]), rk(cost))[0, 1] > 0.85

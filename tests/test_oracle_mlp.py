@@ -37,18 +37,25 @@ def test_analytic_gradient_matches_finite_differences():
     std = np.ones(16, np.float32)
     target = rng.standard_normal(B)
     label = (rng.uniform(size=B) > 0.5).astype(np.float64)
+    cost = 8.0 + rng.standard_normal(B)
     theta = mlp_ref.init_params(1).astype(np.float64)
     theta += 0.01 * rng.standard_normal(theta.size)  # nonzero biases
-    L0, g, _ = mlp_ref.loss_and_grad(theta, X, target, label, mean, std, 1.0, 0.7, exact=True)
+    kw = dict(exact=True, cost=cost, w_cost=0.4)
+    L0, g, _ = mlp_ref.loss_and_grad(theta, X, target, label, mean, std, 1.0, 0.7, **kw)
     idx = rng.choice(theta.size, 60, replace=False)
-    idx = np.concatenate([idx, [mlp_ref.NPARAM - 1, mlp_ref.NPARAM - 3, 1024, 1100]])
+    # the three output biases, the last output column of W4, and W1/b1 entries
+    P = mlp_ref.NPARAM
+    idx = np.concatenate([idx, [P - 1, P - 2, P - 3, P - 4, P - 7, 1024, 1100]])
     h = 1e-6
     for i in idx:
         tp = theta.copy()
         tm = theta.copy()
         tp[i] += h
         tm[i] -= h
-        fd = (mlp_ref.loss(tp, X, target, label, mean, std, 1.0, 0.7, exact=True)
-              - mlp_ref.loss(tm, X, target, label, mean, std, 1.0, 0.7, exact=True)) / (2 * h)
+        fd = (mlp_ref.loss(tp, X, target, label, mean, std, 1.0, 0.7, **kw)
+              - mlp_ref.loss(tm, X, target, label, mean, std, 1.0, 0.7, **kw)) / (2 * h)
         assert abs(fd - g[i]) <= 1e-6 + 1e-5 * abs(fd), (i, fd, g[i])
     assert np.isfinite(L0)
+    # without a cost target the cost head gets no gradient
+    _, g0, _ = mlp_ref.loss_and_grad(theta, X, target, label, mean, std, 1.0, 0.7, exact=True)
+    assert g0[P - 1] == 0.0 and g0[P - 4] == 0.0
