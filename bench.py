@@ -38,7 +38,7 @@ PEAK_HBM = 8000.0  # GB/s
 # WRITE_SIZE passes scaled by a 1 GiB copy kernel (factors 2.0 and 1.0);
 # profiles/tsit5_lorenz_1M_saveat100_hbm_counters.md.  Re-measured when
 # the kernel changes; None where no counter pass exists for that form.
-MEASURED_TRAFFIC = {"plain": 11549978186, "tuned": 11581290592}
+MEASURED_TRAFFIC = {"plain": 11549978186, "tuned": 12420613856}
 ALGORITHMIC_BYTES_PER_TRAJ = 48 + 100 * 3 * 8 + 24 + 8 + 1 + 12
 
 FORMS = {
@@ -311,26 +311,29 @@ MIX_TOL = {"lorenz": (1e-8, 1e-8), "robertson": (1e-8, 1e-6), "vdp": (1e-8, 1e-6
 def other_mix(torch, n_total=1_000_000, n_train=1_000_000):
     """Learned dt0 / method class / cost on the 1e6 Lorenz + Robertson + Van
     der Pol mix: plain = AutoTsit5(Rodas5()) FP64 with the initdt heuristic,
-    natural order; learned+batched = MLP inference, then one AutoTsit5
-    launch per model with the predicted dt0, the predicted class as the
-    initial method and a launch order that groups trajectories by predicted
-    class and cost (longest first), in FP64 and in the FP32-stage form.  The
-    net is trained on the plain solver's logs of a separately seeded mix."""
+    natural order; ordered = MLP inference, then one AutoTsit5 launch per
+    model with the predicted dt0, the predicted class as the initial method
+    and a launch order that groups trajectories by predicted class and cost
+    (longest first); learned = ordered with the division-free log2 PI
+    controller; learned_mixed = learned with FP32 stages.  The net is
+    trained on the plain solver's logs of a separately seeded mix."""
     import mideq
     from mideq import EnsembleMI355X, device_solve
     from mideq.learned import MLP, features, launch_order, training_targets
     from mideq.scimlbase import AutoTsit5
     F = {"lorenz": mideq.lorenz, "robertson": mideq.robertson, "vdp": mideq.vanderpol}
 
-    def run(m, w, prec="fp64", dt0=None, hint=None, order=None, reltol=None, abstol=None):
+    def run(m, w, prec="fp64", dt0=None, hint=None, order=None, reltol=None, abstol=None,
+            ctl="reference"):
         at, rt = MIX_TOL[m]
         at = abstol or at
         rt = reltol or rt
         u0, p, tf = w["u0"], w["p"], w["tf"]
         N = u0.shape[1]
         return device_solve(F[m], u0, p, np.zeros(N), tf, None, AutoTsit5(),
-                            EnsembleMI355X(precision=prec), abstol=at, reltol=rt, dt0=dt0,
-                            alg_hint=hint, order=order, maxiters=1_000_000)
+                            EnsembleMI355X(precision=prec, controller=ctl), abstol=at,
+                            reltol=rt, dt0=dt0, alg_hint=hint, order=order,
+                            maxiters=1_000_000)
 
     t_logs = time.perf_counter()
     train = _mix_sets(n_train // 3, seed=777)
@@ -359,9 +362,11 @@ def other_mix(torch, n_total=1_000_000, n_train=1_000_000):
     t_train = time.perf_counter() - t_train
 
     evals = _mix_sets(n_total // 3, seed=1234)
-    res = {"plain": dict(kernel_ms=0.0, per_model_ms={}),
-           "learned": dict(kernel_ms=0.0, infer_ms=0.0, per_model_ms={}),
-           "learned_mixed": dict(kernel_ms=0.0, infer_ms=0.0, per_model_ms={})}
+    forms = (("ordered", "fp64", "reference"), ("learned", "fp64", "fast"),
+             ("learned_mixed", "mixed", "fast"))
+    res = {"plain": dict(kernel_ms=0.0, per_model_ms={})}
+    for form, _, _ in forms:
+        res[form] = dict(kernel_ms=0.0, infer_ms=0.0, per_model_ms={})
     agree = 0
     cnt = dict(plain_rej=0, learned_rej=0, plain_steps=0, learned_steps=0)
     errs = {k: [] for k in res}
@@ -373,7 +378,9 @@ def other_mix(torch, n_total=1_000_000, n_train=1_000_000):
         res["plain"]["per_model_ms"][m] = plain.kernel_ms
         at, rt = MIX_TOL[m]
         Xe = torch.from_numpy(features(m, w["p"], w["u0"], 0.0, w["tf"], at, rt)).cuda()
-        for form, prec in (("learned", "fp64"), ("learned_mixed", "mixed")):
+        launch_order(*net.infer(Xe)[:, 1:].T)  # warm-up (first-call set-up untimed)
+        torch.cuda.synchronize()
+        for form, prec, ctl in forms:
             e0 = torch.cuda.Event(enable_timing=True)
             e1 = torch.cuda.Event(enable_timing=True)
             e0.record()
@@ -386,7 +393,7 @@ def other_mix(torch, n_total=1_000_000, n_train=1_000_000):
             order = od.cpu().numpy()
             dt0 = 10.0 ** Y[:, 0].astype(np.float64)
             stiff = Y[:, 1] > 0
-            r = run(m, w, prec=prec, dt0=dt0, hint=stiff.astype(np.int8), order=order)
+            r = run(m, w, prec=prec, dt0=dt0, hint=stiff.astype(np.int8), order=order, ctl=ctl)
             res[form]["kernel_ms"] += r.kernel_ms
             res[form]["per_model_ms"][m] = r.kernel_ms
             u = r.u_final
@@ -411,9 +418,8 @@ def other_mix(torch, n_total=1_000_000, n_train=1_000_000):
         res[form].update(total_ms=tot, traj_per_s=n_total / (tot * 1e-3),
                          weighted_err_median=float(np.median(e)),
                          weighted_err_p99=float(np.quantile(e, 0.99)))
-    res["learned"]["speedup_vs_plain"] = res["plain"]["total_ms"] / res["learned"]["total_ms"]
-    res["learned_mixed"]["speedup_vs_plain"] = (res["plain"]["total_ms"] /
-                                               res["learned_mixed"]["total_ms"])
+    for form, _, _ in forms:
+        res[form]["speedup_vs_plain"] = res["plain"]["total_ms"] / res[form]["total_ms"]
     return dict(workload="1e6 mix: 1/3 Lorenz (1e-8), 1/3 Robertson (1e-8/1e-6), 1/3 classic "
                          "Van der Pol mu log-uniform 1..1e4 tspan (0, 2mu) (1e-8/1e-6)",
                 training=dict(samples=int(len(X)), log_solve_s=t_logs, train_s=t_train,
