@@ -451,6 +451,82 @@ __global__ void k_mlp_tobf16(const float* w, short* wb, int n) {
   if (i < n) wb[i] = f2bf(w[i]);
 }
 
+// ------------------------------------------------------------ launch order
+// Counting sort of the predictions into a launch order: predicted
+// non-stiff class first, each class by decreasing predicted log2 cost in
+// 1/16-octave buckets (log2 cost clamped to [0, 32)).  Chunks of ORD_CHUNK
+// samples per block rank their samples in LDS, so the only global atomics
+// are one per (block, non-empty bucket).  Order inside a bucket is arbitrary.
+constexpr int ORD_NB = 512;
+constexpr int ORD_BINS = 2 * ORD_NB;
+constexpr int ORD_CHUNK = 4096;
+
+__device__ __forceinline__ int ord_key(const float* Y, int64_t s) {
+  float cls = Y[s * NOUT + 1], c = Y[s * NOUT + 2];
+  int b;
+  if (!(c >= 0.0f))
+    b = 0;
+  else if (c >= 32.0f)
+    b = ORD_NB - 1;
+  else
+    b = (int)(c * 16.0f);
+  return (cls > 0.0f ? ORD_NB : 0) + (ORD_NB - 1 - b);
+}
+
+__global__ __launch_bounds__(256) void k_ord_hist(const float* Y, int64_t B, int* hist) {
+  __shared__ int lh[ORD_BINS];
+  for (int b = threadIdx.x; b < ORD_BINS; b += blockDim.x) lh[b] = 0;
+  __syncthreads();
+  const int64_t s0 = (int64_t)blockIdx.x * ORD_CHUNK;
+  const int64_t s1 = s0 + ORD_CHUNK < B ? s0 + ORD_CHUNK : B;
+  for (int64_t s = s0 + threadIdx.x; s < s1; s += blockDim.x) atomicAdd(&lh[ord_key(Y, s)], 1);
+  __syncthreads();
+  for (int b = threadIdx.x; b < ORD_BINS; b += blockDim.x)
+    if (lh[b]) atomicAdd(&hist[b], lh[b]);
+}
+
+// exclusive scan of the ORD_BINS counts, one block of ORD_BINS threads
+__global__ __launch_bounds__(ORD_BINS) void k_ord_scan(const int* hist, int* offs) {
+  __shared__ int v[ORD_BINS];
+  const int t = threadIdx.x;
+  v[t] = hist[t];
+  __syncthreads();
+  for (int d = 1; d < ORD_BINS; d <<= 1) {
+    int x = t >= d ? v[t - d] : 0;
+    __syncthreads();
+    v[t] += x;
+    __syncthreads();
+  }
+  offs[t] = v[t] - hist[t];
+}
+
+__global__ __launch_bounds__(256) void k_ord_scatter(const float* Y, int64_t B, int* offs,
+                                                     int64_t* order) {
+  __shared__ int lh[ORD_BINS];
+  __shared__ int base[ORD_BINS];
+  for (int b = threadIdx.x; b < ORD_BINS; b += blockDim.x) lh[b] = 0;
+  __syncthreads();
+  const int64_t s0 = (int64_t)blockIdx.x * ORD_CHUNK;
+  const int64_t s1 = s0 + ORD_CHUNK < B ? s0 + ORD_CHUNK : B;
+  for (int64_t s = s0 + threadIdx.x; s < s1; s += blockDim.x) atomicAdd(&lh[ord_key(Y, s)], 1);
+  __syncthreads();
+  for (int b = threadIdx.x; b < ORD_BINS; b += blockDim.x) {
+    base[b] = lh[b] ? atomicAdd(&offs[b], lh[b]) : 0;
+    lh[b] = 0;
+  }
+  __syncthreads();
+  for (int64_t s = s0 + threadIdx.x; s < s1; s += blockDim.x) {
+    int k = ord_key(Y, s);
+    int r = atomicAdd(&lh[k], 1);
+    order[base[k] + r] = s;
+  }
+}
+
+__global__ void k_ord_zero(int* p, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
 }  // namespace mlp
 }  // namespace mideq
 
@@ -468,6 +544,8 @@ struct mideq_mlp {
   float* loss = nullptr;
   short* scratch = nullptr;
   size_t scratch_bytes = 0;
+  int* ord = nullptr;  // launch-order histogram and offsets [2][ORD_BINS]
+  int ncu = 256;
   int64_t step = 0;
   std::string err;
 };
@@ -492,7 +570,10 @@ extern "C" int mideq_mlp_create(mideq_mlp** out, int32_t device) {
             hipMalloc((void**)&m->v, sizeof(float) * NPARAM) == hipSuccess &&
             hipMalloc((void**)&m->grad, sizeof(float) * NPARAM) == hipSuccess &&
             hipMalloc((void**)&m->norm, sizeof(float) * 2 * NIN) == hipSuccess &&
-            hipMalloc((void**)&m->loss, sizeof(float) * 4) == hipSuccess;
+            hipMalloc((void**)&m->loss, sizeof(float) * 4) == hipSuccess &&
+            hipMalloc((void**)&m->ord, sizeof(int) * 2 * ORD_BINS) == hipSuccess;
+  hipDeviceProp_t prop;
+  if (ok && hipGetDeviceProperties(&prop, device) == hipSuccess) m->ncu = prop.multiProcessorCount;
   if (!ok) {
     mideq_mlp_destroy(m);
     return MIDEQ_E_NOMEM;
@@ -523,7 +604,7 @@ extern "C" void mideq_mlp_destroy(mideq_mlp* m) {
   hipError_t e = hipSetDevice(m->device);
   e = hipDeviceSynchronize();
   for (void* p : {(void*)m->w, (void*)m->wb, (void*)m->m, (void*)m->v, (void*)m->grad,
-                  (void*)m->norm, (void*)m->loss, (void*)m->scratch})
+                  (void*)m->norm, (void*)m->loss, (void*)m->scratch, (void*)m->ord})
     if (p) e = hipFree(p);
   (void)e;
   delete m;
@@ -583,8 +664,11 @@ extern "C" int mideq_mlp_infer(mideq_mlp* m, const float* X, int64_t B, float* Y
   if (B == 0) return MIDEQ_OK;
   hipError_t e = hipSetDevice(m->device);
   if (e != hipSuccess) return mlp_err(m, MIDEQ_E_HIP, "infer", e);
+  // every wave first loads the whole net into registers (~19 KB of
+  // fragments), so keep the grid to 8 waves per CU and loop over tiles
   int64_t tiles = (B + 15) / 16;
-  int64_t waves = tiles < 16384 ? tiles : 16384;
+  int64_t cap = (int64_t)m->ncu * 8;
+  int64_t waves = tiles < cap ? tiles : cap;
   int blocks = (int)((waves + 3) / 4);
   hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(256), 0, (hipStream_t)stream, m->wb, m->w,
                      m->norm, m->norm + NIN, X, B, Y);
@@ -657,4 +741,23 @@ extern "C" int mideq_mlp_adam_step(mideq_mlp* m, float lr, float beta1, float be
   hipLaunchKernelGGL(k_mlp_adam, dim3((NPARAM + 255) / 256), dim3(256), 0, (hipStream_t)stream, a);
   e = hipGetLastError();
   return e == hipSuccess ? MIDEQ_OK : mlp_err(m, MIDEQ_E_HIP, "adam launch", e);
+}
+
+extern "C" int mideq_mlp_launch_order(mideq_mlp* m, const float* Y, int64_t B, int64_t* order,
+                                      void* stream) {
+  if (!m || B < 0 || (B > 0 && (!Y || !order))) return MIDEQ_E_ARG;
+  if (B > (int64_t)0x7fffffff) return mlp_err(m, MIDEQ_E_ARG, "launch_order: B >= 2^31");
+  if (B == 0) return MIDEQ_OK;
+  hipError_t e = hipSetDevice(m->device);
+  if (e != hipSuccess) return mlp_err(m, MIDEQ_E_HIP, "launch_order", e);
+  hipStream_t s = (hipStream_t)stream;
+  int* hist = m->ord;
+  int* offs = m->ord + ORD_BINS;
+  int nblk = (int)((B + ORD_CHUNK - 1) / ORD_CHUNK);
+  hipLaunchKernelGGL(k_ord_zero, dim3((ORD_BINS + 255) / 256), dim3(256), 0, s, hist, ORD_BINS);
+  hipLaunchKernelGGL(k_ord_hist, dim3(nblk), dim3(256), 0, s, Y, B, hist);
+  hipLaunchKernelGGL(k_ord_scan, dim3(1), dim3(ORD_BINS), 0, s, (const int*)hist, offs);
+  hipLaunchKernelGGL(k_ord_scatter, dim3(nblk), dim3(256), 0, s, Y, B, offs, order);
+  e = hipGetLastError();
+  return e == hipSuccess ? MIDEQ_OK : mlp_err(m, MIDEQ_E_HIP, "launch_order launch", e);
 }

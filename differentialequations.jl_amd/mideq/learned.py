@@ -47,6 +47,7 @@ def _lib():
         L.mideq_mlp_grad.argtypes = [vp, vp, vp, vp, vp, C.c_int64, C.c_float, C.c_float,
                                      C.c_float, vp, vp]
         L.mideq_mlp_adam_step.argtypes = [vp, C.c_float, C.c_float, C.c_float, C.c_float, vp]
+        L.mideq_mlp_launch_order.argtypes = [vp, vp, C.c_int64, vp, vp]
         L._mlp_declared = True
     return L
 
@@ -132,6 +133,19 @@ class MLP:
         self._check(self.L.mideq_mlp_infer(self.h, X.data_ptr(), B, Y.data_ptr(), s), "infer")
         return Y
 
+    def launch_order(self, Y, order=None, stream=None):
+        """Launch order (device int64 [B]) from infer()'s output Y [B, 3]: the
+        library's counting sort by (predicted class, decreasing predicted
+        cost in 1/16-octave buckets); see order_keys()."""
+        import torch
+        B = Y.shape[0]
+        if order is None:
+            order = torch.empty(B, dtype=torch.int64, device=Y.device)
+        s = None if stream is None else stream.cuda_stream
+        self._check(self.L.mideq_mlp_launch_order(self.h, Y.data_ptr(), B, order.data_ptr(), s),
+                    "launch_order")
+        return order
+
     def grad(self, X, target, label, cost=None, w_dt=1.0, w_cls=1.0, w_cost=1.0,
              want_loss=True, stream=None):
         loss = C.c_float(0.0)
@@ -208,14 +222,22 @@ def training_targets(dt_first, naccept, nstiff, nreject=None):
     return tgt, lab, np.log2(np.maximum(cost, 1.0)).astype(np.float32)
 
 
+ORDER_BUCKETS = 512  # log2-cost buckets per class, 1/16 octave each over [0, 32)
+
+
+def order_keys(stiff_logit, log2_cost):
+    """Sort keys of the launch order (numpy): predicted non-stiff class
+    first, each class by decreasing predicted log2 cost in 1/16-octave
+    buckets -- the keys mideq_mlp_launch_order sorts by."""
+    c = np.asarray(log2_cost, dtype=np.float32)
+    b = np.where(c >= 0, np.minimum(np.floor(np.where(c >= 0, c, 0) * np.float32(16)),
+                                    ORDER_BUCKETS - 1), 0).astype(np.int64)
+    cls = (np.asarray(stiff_logit) > 0).astype(np.int64)
+    return cls * ORDER_BUCKETS + (ORDER_BUCKETS - 1 - b)
+
+
 def launch_order(stiff_logit, log2_cost):
-    """Launch order from predictions: predicted non-stiff trajectories first,
-    each class by decreasing predicted cost (longest first, so the last
-    wavefronts to start are the short ones).  Works on numpy arrays or torch
-    tensors (device sort)."""
-    if hasattr(stiff_logit, "device"):
-        import torch
-        key = torch.where(stiff_logit > 0, 1.0e6, 0.0).double() - log2_cost.double()
-        return torch.argsort(key, stable=True)
-    key = np.where(np.asarray(stiff_logit) > 0, 1.0e6, 0.0) - np.asarray(log2_cost, np.float64)
-    return np.argsort(key, kind="stable")
+    """Launch order from predictions on the host (numpy): a stable sort by
+    order_keys (longest first inside each class, so the last wavefronts to
+    start are the short ones).  MLP.launch_order does the same on the GPU."""
+    return np.argsort(order_keys(stiff_logit, log2_cost), kind="stable")

@@ -63,7 +63,7 @@ def feats(model, w):
 
 def test_learned_dt0_and_method_beat_plain():
     import torch
-    from mideq.learned import MLP, launch_order, training_targets
+    from mideq.learned import MLP, training_targets
     say("learned: plain logs -> train -> evaluate")
     train = mix(6000, seed=101)
     Xs, ts, ls, cs = [], [], [], []
@@ -89,7 +89,7 @@ def test_learned_dt0_and_method_beat_plain():
     for m, w in evals.items():
         plain = solve(m, w)
         Yd = net.infer(torch.from_numpy(feats(m, w)).cuda())
-        order = launch_order(Yd[:, 1], Yd[:, 2]).cpu().numpy()
+        order = net.launch_order(Yd).cpu().numpy()
         Y = Yd.cpu().numpy()
         dt0 = 10.0 ** Y[:, 0].astype(np.float64)
         hint = (Y[:, 1] > 0).astype(np.int8)

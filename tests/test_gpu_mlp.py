@@ -103,3 +103,25 @@ def test_mlp_training_lowers_loss():
     # same training emulated with oracle/mlp_ref.py reaches 0.961 / 0.90)
     rk = lambda v: np.argsort(np.argsort(v))
     assert np.corrcoef(rk(Y[:, 2]), rk(cost))[0, 1] > 0.85
+
+
+@pytest.mark.parametrize("B", [1, 4095, 300001])
+def test_launch_order_kernel_is_sorted_permutation(B):
+    """mideq_mlp_launch_order: a permutation of 0..B-1 whose keys (class,
+    then decreasing log2-cost bucket, as learned.order_keys states them) are
+    non-decreasing; NaN, negative and huge costs land in the end buckets."""
+    import torch
+    from mideq.learned import MLP, order_keys
+    say(f"launch order B={B}")
+    rng = np.random.default_rng(B)
+    Y = np.zeros((B, 3), np.float32)
+    Y[:, 1] = rng.standard_normal(B)
+    Y[:, 2] = rng.uniform(-2.0, 40.0, B)
+    Y[::97, 2] = np.nan
+    net = MLP(seed=0)
+    order = net.launch_order(torch.from_numpy(Y).cuda()).cpu().numpy()
+    assert order.dtype == np.int64
+    np.testing.assert_array_equal(np.sort(order), np.arange(B))
+    keys = order_keys(Y[:, 1], Y[:, 2])
+    assert np.all(np.diff(keys[order]) >= 0)
+

@@ -165,3 +165,21 @@ def test_fast_controller_bitexact(oracle, model, alg, precision):
             controller=1, **tol)
     assert (c["retcode"] == 1).all()
     assert_identical(g, c)
+
+
+@pytest.mark.parametrize("controller", ["reference", "fast"])
+def test_cfg1_lorenz_10000_endpoint_bitexact(oracle, controller):
+    """The reference's own CPU-sized case: 10,000 seeded Lorenz trajectories
+    (sigma, rho, beta +-10 %), u0 = [1, 0, 0], tspan (0, 10), Tsit5,
+    abstol = reltol = 1e-8, endpoint only -- identical to the oracle's
+    EnsembleThreads restatement trajectory by trajectory."""
+    N = 10_000
+    say(f"cfg1 lorenz N={N} endpoint controller={controller}")
+    w = workloads.lorenz(N, seed=1234)
+    g = gpu("lorenz", w["u0"], w["p"], 0.0, 10.0, abstol=1e-8, reltol=1e-8,
+            controller=controller)
+    c = cpu(oracle, "lorenz", w["u0"], w["p"], 0.0, 10.0, abstol=1e-8, reltol=1e-8,
+            controller=1 if controller == "fast" else 0)
+    assert (c["retcode"] == 1).all()
+    assert_identical(g, c)
+    assert (g.t_final == 10.0).all()
