@@ -217,12 +217,12 @@ def other_robertson(torch, ctx, steps, warmup, N=262144):
     stream = torch.cuda.Stream()
     sp = C.c_void_p(stream.cuda_stream)
 
-    def structs(prec, ctl, reltol=1e-6, abstol=1e-8):
+    def structs(prec, ctl, reltol=1e-6, abstol=1e-8, alg="rodas5"):
         P = _abi.Problem()
         P.model, P.n, P.np, P.ntraj = _abi.MODEL["robertson"], 3, 3, N
         P.u0, P.p, P.tspan0, P.tspan1 = u0.data_ptr(), p.data_ptr(), 0.0, 1e5
         O = _abi.Options()
-        O.alg, O.precision, O.abstol, O.reltol = _abi.ALG["rodas5"], _abi.PRECISION[prec], abstol, reltol
+        O.alg, O.precision, O.abstol, O.reltol = _abi.ALG[alg], _abi.PRECISION[prec], abstol, reltol
         O.qsteady_default, O.controller = 1, ctl
         out = _abi.Outputs()
         out.u_final, out.retcode, out.naccept, out.nreject = (uf.data_ptr(), rc.data_ptr(),
@@ -234,8 +234,11 @@ def other_robertson(torch, ctx, steps, warmup, N=262144):
     torch.cuda.synchronize()
     ref = uf.cpu().numpy().copy()
     res = {}
-    for name, prec, ctl in (("plain", "fp64", 0), ("tuned", "fp64", 1), ("mixed", "mixed", 1)):
-        P, O, out = structs(prec, ctl)
+    for name, prec, ctl, alg in (("plain", "fp64", 0, "rodas5"), ("tuned", "fp64", 1, "rodas5"),
+                                 ("mixed", "mixed", 1, "rodas5"),
+                                 ("rodas5p", "fp64", 0, "rodas5p"),
+                                 ("rodas5p_tuned", "fp64", 1, "rodas5p")):
+        P, O, out = structs(prec, ctl, alg=alg)
         wall, kms = timed_solve(torch, lambda: ctx.solve_device(P, O, out, sp), steps, warmup,
                                 stream)
         u = uf.cpu().numpy()

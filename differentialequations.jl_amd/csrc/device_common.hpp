@@ -80,6 +80,46 @@ constexpr double C[8][7] = {
 }  // namespace rd
 
 // ---------------------------------------------------------------------------
+// Rodas5P (Steinebach, BIT 63 (2023) 27): same W-form, 8 stages and stiffly
+// accurate structure as Rodas5, new coefficients.
+// ---------------------------------------------------------------------------
+namespace rd5p {
+constexpr double gamma = 0.21193756319429014;
+constexpr double A[6][5] = {
+    {0, 0, 0, 0, 0},
+    {3.0, 0, 0, 0, 0},
+    {2.849394379747939, 0.45842242204463923, 0, 0, 0},
+    {-6.954028509809101, 2.489845061869568, -10.358996098473584, 0, 0},
+    {2.8029986275628964, 0.5072464736228206, -0.3988312541770524, -0.04721187230404641, 0},
+    {-7.502846399306121, 2.561846144803919, -11.627539656261098, -0.18268767659942256,
+     0.030198172008377946}};
+constexpr double C[8][7] = {
+    {0, 0, 0, 0, 0, 0, 0},
+    {-14.155112264123755, 0, 0, 0, 0, 0, 0},
+    {-17.97296035885952, -2.859693295451294, 0, 0, 0, 0, 0},
+    {147.12150275711716, -1.41221402718213, 71.68940251302358, 0, 0, 0, 0},
+    {165.43517024871676, -0.4592823456491126, 42.90938336958603, -5.961986721573306, 0, 0, 0},
+    {24.854864614690072, -3.0009227002832186, 47.4931110020768, 5.5814197821558125,
+     -0.6610691825249471, 0, 0},
+    {30.91273214028599, -3.1208243349937974, 77.79954646070892, 34.28646028294783,
+     -19.097331116725623, -28.087943162872662, 0},
+    {37.80277123390563, -3.2571969029072276, 112.26918849496327, 66.9347231244047,
+     -40.06618937091002, -54.66780262877968, -9.48861652309627}};
+}  // namespace rd5p
+
+// Rosenbrock tableau selected at compile time (P: Rodas5P, else Rodas5).
+template <bool P>
+__device__ __forceinline__ constexpr double rd_gamma() { return P ? rd5p::gamma : rd::gamma; }
+template <bool P>
+__device__ __forceinline__ constexpr double rd_A(int s, int j) {
+  return P ? rd5p::A[s][j] : rd::A[s][j];
+}
+template <bool P>
+__device__ __forceinline__ constexpr double rd_C(int s, int j) {
+  return P ? rd5p::C[s][j] : rd::C[s][j];
+}
+
+// ---------------------------------------------------------------------------
 // Controller constants (PI controller defaults for an order-5 method) and
 // the auto-switching thresholds of AutoTsit5(Rodas5()).
 // ---------------------------------------------------------------------------

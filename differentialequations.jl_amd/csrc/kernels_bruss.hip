@@ -62,9 +62,10 @@ __device__ __forceinline__ int team_and(int v) {
 
 template <int L, int C, bool MIX, int ALG>
 struct BTeam {
-  static constexpr bool kAuto = (ALG == ALG_AUTO);
-  static constexpr bool kHasTsit5 = (ALG != ALG_RODAS5);
+  static constexpr bool kAuto = (ALG == ALG_AUTO || ALG == ALG_AUTO5P);
+  static constexpr bool kHasTsit5 = (ALG != ALG_RODAS5 && ALG != ALG_RODAS5P);
   static constexpr bool kHasRodas = (ALG != ALG_TSIT5);
+  static constexpr bool kP = (ALG == ALG_RODAS5P || ALG == ALG_AUTO5P);  // Rodas5P tableau
 
   int tl;
   int64_t i;
@@ -246,7 +247,7 @@ struct BTeam {
     eig = 0.0;
     dtfirst = 0.0;
     rc = RC_DEFAULT;
-    mode = (ALG == ALG_RODAS5) ? 1 : 0;
+    mode = (ALG == ALG_RODAS5 || ALG == ALG_RODAS5P) ? 1 : 0;
     if (kAuto && a.alg_hint) mode = a.alg_hint[i] ? 1 : 0;
     qsteady_max = (a.qsteady_default && mode == 1) ? 1.2 : 1.0;
     if (n < BRUSS_NMIN || n > a.nmax || (n & 1)) {
@@ -510,7 +511,7 @@ struct BTeam {
       }
       eig = team_max<L>(best);
     }
-    double dtgamma = dts * rd::gamma;
+    double dtgamma = dts * rd_gamma<kP>();
     double w = 1.0 / dtgamma;
 #pragma unroll
     for (int m = 0; m < C; m++) {
@@ -662,7 +663,7 @@ struct BTeam {
           for (int m = 0; m < C; m++) {
             double acc = u[m];
 #pragma unroll
-            for (int j = 0; j < s; j++) acc = fma(rd::A[s][j], kd[j][m], acc);
+            for (int j = 0; j < s; j++) acc = fma(rd_A<kP>(s, j), kd[j][m], acc);
             U[m] = acc;
           }
         } else {
@@ -679,7 +680,7 @@ struct BTeam {
           for (int m = 0; m < C; m++) {
             float acc = (float)u[m];
 #pragma unroll
-            for (int j = 0; j < s; j++) acc = fmaf((float)rd::A[s][j], (float)kd[j][m], acc);
+            for (int j = 0; j < s; j++) acc = fmaf((float)rd_A<kP>(s, j), (float)kd[j][m], acc);
             Uf[m] = acc;
           }
         } else {
@@ -696,15 +697,15 @@ struct BTeam {
         if (s == 0) {
           r = du[m];
         } else if (!MIX) {
-          double acc = (rd::C[s][0] * inv_dt) * kd[0][m];
+          double acc = (rd_C<kP>(s, 0) * inv_dt) * kd[0][m];
 #pragma unroll
-          for (int j = 1; j < s; j++) acc = fma(rd::C[s][j] * inv_dt, kd[j][m], acc);
+          for (int j = 1; j < s; j++) acc = fma(rd_C<kP>(s, j) * inv_dt, kd[j][m], acc);
           r = du[m] + acc;
         } else {
           float idt = (float)inv_dt;
-          float acc = ((float)rd::C[s][0] * idt) * (float)kd[0][m];
+          float acc = ((float)rd_C<kP>(s, 0) * idt) * (float)kd[0][m];
 #pragma unroll
-          for (int j = 1; j < s; j++) acc = fmaf((float)rd::C[s][j] * idt, (float)kd[j][m], acc);
+          for (int j = 1; j < s; j++) acc = fmaf((float)rd_C<kP>(s, j) * idt, (float)kd[j][m], acc);
           r = (double)((float)du[m] + acc);
         }
         x[m] = -r;
@@ -717,7 +718,7 @@ struct BTeam {
     for (int m = 0; m < C; m++) {
       double acc = u[m];
 #pragma unroll
-      for (int j = 0; j < 5; j++) acc = fma(rd::A[5][j], kd[j][m], acc);
+      for (int j = 0; j < 5; j++) acc = fma(rd_A<kP>(5, j), kd[j][m], acc);
       acc = acc + kd[5][m];
       acc = acc + kd[6][m];
       u1[m] = acc + kd[7][m];
@@ -920,10 +921,14 @@ static hipError_t occ_dispatch(int alg, int mix, int* per_cu) {
   if (mix) {
     if (alg == ALG_TSIT5) return occupancy<L, true, ALG_TSIT5>(per_cu);
     if (alg == ALG_RODAS5) return occupancy<L, true, ALG_RODAS5>(per_cu);
+    if (alg == ALG_RODAS5P) return occupancy<L, true, ALG_RODAS5P>(per_cu);
+    if (alg == ALG_AUTO5P) return occupancy<L, true, ALG_AUTO5P>(per_cu);
     return occupancy<L, true, ALG_AUTO>(per_cu);
   }
   if (alg == ALG_TSIT5) return occupancy<L, false, ALG_TSIT5>(per_cu);
   if (alg == ALG_RODAS5) return occupancy<L, false, ALG_RODAS5>(per_cu);
+  if (alg == ALG_RODAS5P) return occupancy<L, false, ALG_RODAS5P>(per_cu);
+  if (alg == ALG_AUTO5P) return occupancy<L, false, ALG_AUTO5P>(per_cu);
   return occupancy<L, false, ALG_AUTO>(per_cu);
 }
 
@@ -959,10 +964,14 @@ static hipError_t launch2(const BrussArgs& a, int alg, int mix, int grid, hipStr
   if (mix) {
     if (alg == ALG_TSIT5) return launch3<L, true, ALG_TSIT5>(a, grid, s);
     if (alg == ALG_RODAS5) return launch3<L, true, ALG_RODAS5>(a, grid, s);
+    if (alg == ALG_RODAS5P) return launch3<L, true, ALG_RODAS5P>(a, grid, s);
+    if (alg == ALG_AUTO5P) return launch3<L, true, ALG_AUTO5P>(a, grid, s);
     return launch3<L, true, ALG_AUTO>(a, grid, s);
   }
   if (alg == ALG_TSIT5) return launch3<L, false, ALG_TSIT5>(a, grid, s);
   if (alg == ALG_RODAS5) return launch3<L, false, ALG_RODAS5>(a, grid, s);
+  if (alg == ALG_RODAS5P) return launch3<L, false, ALG_RODAS5P>(a, grid, s);
+  if (alg == ALG_AUTO5P) return launch3<L, false, ALG_AUTO5P>(a, grid, s);
   return launch3<L, false, ALG_AUTO>(a, grid, s);
 }
 

@@ -109,6 +109,8 @@ static hipError_t launch_alg(const LaneArgs& a, int alg, int sched, int block, i
   switch (alg) {
     case ALG_TSIT5: return launch_t<M, MIX, ALG_TSIT5, FAST>(a, sched, block, grid, s);
     case ALG_RODAS5: return launch_t<M, MIX, ALG_RODAS5, FAST>(a, sched, block, grid, s);
+    case ALG_RODAS5P: return launch_t<M, MIX, ALG_RODAS5P, FAST>(a, sched, block, grid, s);
+    case ALG_AUTO5P: return launch_t<M, MIX, ALG_AUTO5P, FAST>(a, sched, block, grid, s);
     default: return launch_t<M, MIX, ALG_AUTO, FAST>(a, sched, block, grid, s);
   }
 }

@@ -13,7 +13,8 @@
 
 namespace mideq {
 
-enum { ALG_TSIT5 = 0, ALG_RODAS5 = 1, ALG_AUTO = 2 };
+// AUTO: AutoTsit5(Rodas5()); AUTO5P: AutoTsit5(Rodas5P())
+enum { ALG_TSIT5 = 0, ALG_RODAS5 = 1, ALG_AUTO = 2, ALG_RODAS5P = 3, ALG_AUTO5P = 4 };
 
 struct LaneArgs {
   int64_t ntraj;
@@ -157,9 +158,10 @@ template <class M, bool MIX, int ALG, bool FAST>
 struct Lane {
   static constexpr int N = M::N;
   static constexpr int NP = M::NP;
-  static constexpr bool kAuto = (ALG == ALG_AUTO);
-  static constexpr bool kHasTsit5 = (ALG != ALG_RODAS5);
+  static constexpr bool kAuto = (ALG == ALG_AUTO || ALG == ALG_AUTO5P);
+  static constexpr bool kHasTsit5 = (ALG != ALG_RODAS5 && ALG != ALG_RODAS5P);
   static constexpr bool kHasRodas = (ALG != ALG_TSIT5);
+  static constexpr bool kP = (ALG == ALG_RODAS5P || ALG == ALG_AUTO5P);  // Rodas5P tableau
 
   int64_t i;
   double p[NP];
@@ -274,7 +276,7 @@ struct Lane {
     eig = 0.0;
     dtfirst = 0.0;
     rc = RC_DEFAULT;
-    mode = (ALG == ALG_RODAS5) ? 1 : 0;
+    mode = (ALG == ALG_RODAS5 || ALG == ALG_RODAS5P) ? 1 : 0;
     if (kAuto && a.alg_hint) mode = a.alg_hint[i] ? 1 : 0;
     qsteady_max = (a.qsteady_default && mode == 1) ? 1.2 : 1.0;
     // saveat points before t0 are unreachable; a point at t0 is u0 itself
@@ -488,7 +490,7 @@ struct Lane {
       }
       eig = best;
     }
-    double dtgamma = dts * rd::gamma;
+    double dtgamma = dts * rd_gamma<kP>();
     double w = 1.0 / dtgamma;
     double inv_dt = 1.0 / dts;
 #pragma unroll
@@ -549,7 +551,7 @@ struct Lane {
           for (int c = 0; c < N; c++) {
             double acc = u[c];
 #pragma unroll
-            for (int j = 0; j < s; j++) acc = fma(rd::A[s][j], kd[j][c], acc);
+            for (int j = 0; j < s; j++) acc = fma(rd_A<kP>(s, j), kd[j][c], acc);
             U[c] = acc;
           }
         } else {
@@ -566,7 +568,7 @@ struct Lane {
           for (int c = 0; c < N; c++) {
             float acc = (float)u[c];
 #pragma unroll
-            for (int j = 0; j < s; j++) acc = fmaf((float)rd::A[s][j], (float)kd[j][c], acc);
+            for (int j = 0; j < s; j++) acc = fmaf((float)rd_A<kP>(s, j), (float)kd[j][c], acc);
             Uf[c] = acc;
           }
         } else {
@@ -583,15 +585,15 @@ struct Lane {
         if (s == 0) {
           r = du[c];
         } else if (!MIX) {
-          double acc = (rd::C[s][0] * inv_dt) * kd[0][c];
+          double acc = (rd_C<kP>(s, 0) * inv_dt) * kd[0][c];
 #pragma unroll
-          for (int j = 1; j < s; j++) acc = fma(rd::C[s][j] * inv_dt, kd[j][c], acc);
+          for (int j = 1; j < s; j++) acc = fma(rd_C<kP>(s, j) * inv_dt, kd[j][c], acc);
           r = du[c] + acc;
         } else {
           float idt = (float)inv_dt;
-          float acc = ((float)rd::C[s][0] * idt) * (float)kd[0][c];
+          float acc = ((float)rd_C<kP>(s, 0) * idt) * (float)kd[0][c];
 #pragma unroll
-          for (int j = 1; j < s; j++) acc = fmaf((float)rd::C[s][j] * idt, (float)kd[j][c], acc);
+          for (int j = 1; j < s; j++) acc = fmaf((float)rd_C<kP>(s, j) * idt, (float)kd[j][c], acc);
           r = (double)((float)du[c] + acc);
         }
         x[c] = -r;
@@ -624,7 +626,7 @@ struct Lane {
     for (int c = 0; c < N; c++) {
       double acc = u[c];
 #pragma unroll
-      for (int j = 0; j < 5; j++) acc = fma(rd::A[5][j], kd[j][c], acc);
+      for (int j = 0; j < 5; j++) acc = fma(rd_A<kP>(5, j), kd[j][c], acc);
       acc = acc + kd[5][c];
       acc = acc + kd[6][c];
       u1[c] = acc + kd[7][c];

@@ -454,10 +454,10 @@ __global__ void k_mlp_tobf16(const float* w, short* wb, int n) {
 // ------------------------------------------------------------ launch order
 // Counting sort of the predictions into a launch order: predicted
 // non-stiff class first, each class by decreasing predicted log2 cost in
-// 1/16-octave buckets (log2 cost clamped to [0, 32)).  Chunks of ORD_CHUNK
+// 1/64-octave buckets (log2 cost clamped to [0, 32)).  Chunks of ORD_CHUNK
 // samples per block rank their samples in LDS, so the only global atomics
 // are one per (block, non-empty bucket).  Order inside a bucket is arbitrary.
-constexpr int ORD_NB = 512;
+constexpr int ORD_NB = 2048;
 constexpr int ORD_BINS = 2 * ORD_NB;
 constexpr int ORD_CHUNK = 4096;
 
@@ -469,7 +469,7 @@ __device__ __forceinline__ int ord_key(const float* Y, int64_t s) {
   else if (c >= 32.0f)
     b = ORD_NB - 1;
   else
-    b = (int)(c * 16.0f);
+    b = (int)(c * 64.0f);
   return (cls > 0.0f ? ORD_NB : 0) + (ORD_NB - 1 - b);
 }
 
@@ -485,19 +485,36 @@ __global__ __launch_bounds__(256) void k_ord_hist(const float* Y, int64_t B, int
     if (lh[b]) atomicAdd(&hist[b], lh[b]);
 }
 
-// exclusive scan of the ORD_BINS counts, one block of ORD_BINS threads
-__global__ __launch_bounds__(ORD_BINS) void k_ord_scan(const int* hist, int* offs) {
-  __shared__ int v[ORD_BINS];
+// exclusive scan of the ORD_BINS counts: one block of ORD_SCAN threads,
+// ORD_BINS / ORD_SCAN consecutive bins per thread
+constexpr int ORD_SCAN = 1024;
+constexpr int ORD_PER = ORD_BINS / ORD_SCAN;
+static_assert(ORD_BINS % ORD_SCAN == 0, "scan layout");
+
+__global__ __launch_bounds__(ORD_SCAN) void k_ord_scan(const int* hist, int* offs) {
+  __shared__ int v[ORD_SCAN];
   const int t = threadIdx.x;
-  v[t] = hist[t];
+  int loc[ORD_PER];
+  int sum = 0;
+#pragma unroll
+  for (int k = 0; k < ORD_PER; k++) {
+    loc[k] = hist[t * ORD_PER + k];
+    sum += loc[k];
+  }
+  v[t] = sum;
   __syncthreads();
-  for (int d = 1; d < ORD_BINS; d <<= 1) {
+  for (int d = 1; d < ORD_SCAN; d <<= 1) {
     int x = t >= d ? v[t - d] : 0;
     __syncthreads();
     v[t] += x;
     __syncthreads();
   }
-  offs[t] = v[t] - hist[t];
+  int run = v[t] - sum;  // exclusive prefix of this thread's bins
+#pragma unroll
+  for (int k = 0; k < ORD_PER; k++) {
+    offs[t * ORD_PER + k] = run;
+    run += loc[k];
+  }
 }
 
 __global__ __launch_bounds__(256) void k_ord_scatter(const float* Y, int64_t B, int* offs,
@@ -756,7 +773,7 @@ extern "C" int mideq_mlp_launch_order(mideq_mlp* m, const float* Y, int64_t B, i
   int nblk = (int)((B + ORD_CHUNK - 1) / ORD_CHUNK);
   hipLaunchKernelGGL(k_ord_zero, dim3((ORD_BINS + 255) / 256), dim3(256), 0, s, hist, ORD_BINS);
   hipLaunchKernelGGL(k_ord_hist, dim3(nblk), dim3(256), 0, s, Y, B, hist);
-  hipLaunchKernelGGL(k_ord_scan, dim3(1), dim3(ORD_BINS), 0, s, (const int*)hist, offs);
+  hipLaunchKernelGGL(k_ord_scan, dim3(1), dim3(ORD_SCAN), 0, s, (const int*)hist, offs);
   hipLaunchKernelGGL(k_ord_scatter, dim3(nblk), dim3(256), 0, s, Y, B, offs, order);
   e = hipGetLastError();
   return e == hipSuccess ? MIDEQ_OK : mlp_err(m, MIDEQ_E_HIP, "launch_order launch", e);

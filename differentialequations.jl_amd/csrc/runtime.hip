@@ -196,7 +196,7 @@ static int validate(mideq_ctx* c, const mideq_problem* P, const mideq_options* O
     return set_err(c, MIDEQ_E_UNSUPPORTED, "ragged nvec only for the brusselator model");
   if (P->np < np) return set_err(c, MIDEQ_E_ARG, "model %d needs np>=%d, got %d", P->model, np, P->np);
   if (P->ntraj > 0 && (!P->u0 || !P->p)) return set_err(c, MIDEQ_E_ARG, "null u0 or p");
-  if (O->alg < 0 || O->alg > 2) return set_err(c, MIDEQ_E_ARG, "bad alg %d", O->alg);
+  if (O->alg < 0 || O->alg > 4) return set_err(c, MIDEQ_E_ARG, "bad alg %d", O->alg);
   if (O->precision < 0 || O->precision > 1)
     return set_err(c, MIDEQ_E_ARG, "bad precision %d", O->precision);
   if (!(O->abstol > 0) || !(O->reltol > 0))

@@ -27,6 +27,8 @@ module MI355XEnsembles
 
 using SciMLBase
 import SciMLBase: __solve, EnsembleSolution, ReturnCode
+# method types re-exported by DifferentialEquations (test/qa/qa.jl:15,97,119)
+using OrdinaryDiffEq: Tsit5, Rodas5, Rodas5P, CompositeAlgorithm
 
 export EnsembleMI355X, register_model!
 
@@ -77,8 +79,12 @@ register_model!(f, name::Symbol) = (MODEL_FOR[f] = MODEL_IDS[name]; f)
 
 alg_code(::Tsit5) = Int32(0)
 alg_code(::Rodas5) = Int32(1)
-alg_code(::CompositeAlgorithm) = Int32(2)       # AutoTsit5(Rodas5())
-alg_code(alg) = error("EnsembleMI355X supports Tsit5, Rodas5 and AutoTsit5(Rodas5())")
+alg_code(::Rodas5P) = Int32(3)
+# AutoTsit5(Rodas5()) / AutoTsit5(Rodas5P())
+alg_code(alg::CompositeAlgorithm) = alg.algs[2] isa Rodas5P ? Int32(4) :
+                                    alg.algs[2] isa Rodas5 ? Int32(2) :
+                                    error("EnsembleMI355X: AutoTsit5 needs Rodas5() or Rodas5P()")
+alg_code(alg) = error("EnsembleMI355X supports Tsit5, Rodas5, Rodas5P and AutoTsit5 of those")
 
 const RETCODES = (ReturnCode.Default, ReturnCode.Success, ReturnCode.MaxIters,
                   ReturnCode.DtLessThanMin, ReturnCode.Unstable, ReturnCode.DtNaN,

@@ -1,5 +1,6 @@
-"""mideq -- MI355X-native ensemble ODE solver (Tsit5 / Rodas5 /
-AutoTsit5(Rodas5()) over many independent trajectories), exposed with the
+"""mideq -- MI355X-native ensemble ODE solver (Tsit5 / Rodas5 / Rodas5P /
+AutoTsit5(Rodas5()) / AutoTsit5(Rodas5P()) over many independent
+trajectories), exposed with the
 SciMLBase surface of the reference:
 
     from mideq import *
@@ -15,11 +16,12 @@ from . import workloads
 from ._abi import MideqError, header_functions
 from .scimlbase import (AutoTsit5, EnsembleArrays, EnsembleMI355X, EnsembleProblem,
                         EnsembleSolution, ODEFunction, ODEProblem, ODESolution, ReturnCode,
-                        Rodas5, Stats, Tsit5, brusselator, device_solve, exp_growth, linear3,
-                        lorenz, remake, robertson, solve, successful_retcode, vanderpol)
+                        Rodas5, Rodas5P, Stats, Tsit5, brusselator, device_solve, exp_growth,
+                        linear3, lorenz, remake, robertson, solve, successful_retcode,
+                        vanderpol)
 
 __all__ = ["AutoTsit5", "EnsembleArrays", "EnsembleMI355X", "EnsembleProblem",
            "EnsembleSolution", "MideqError", "ODEFunction", "ODEProblem", "ODESolution",
-           "ReturnCode", "Rodas5", "Stats", "Tsit5", "brusselator", "device_solve",
+           "ReturnCode", "Rodas5", "Rodas5P", "Stats", "Tsit5", "brusselator", "device_solve",
            "exp_growth", "header_functions", "linear3", "lorenz", "remake", "robertson",
            "solve", "successful_retcode", "vanderpol", "workloads"]

@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(PKG_ROOT, "_build", "libmideq.so")
 HEADER = os.path.join(os.path.dirname(PKG_ROOT), "include", "mideq.h")
 
 MODEL = dict(exp=0, lorenz=1, robertson=2, vdp=3, brusselator=4, linear=5)
-ALG = dict(tsit5=0, rodas5=1, auto=2)
+ALG = dict(tsit5=0, rodas5=1, auto=2, rodas5p=3, auto5p=4)
 PRECISION = dict(fp64=0, mixed=1)
 SCHEDULE = dict(auto=0, lane=1, persistent=2)
 STATUS = {0: "OK", -1: "E_ARG", -2: "E_HIP", -3: "E_NOMEM", -4: "E_MODEL",

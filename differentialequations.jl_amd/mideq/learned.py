@@ -136,7 +136,7 @@ class MLP:
     def launch_order(self, Y, order=None, stream=None):
         """Launch order (device int64 [B]) from infer()'s output Y [B, 3]: the
         library's counting sort by (predicted class, decreasing predicted
-        cost in 1/16-octave buckets); see order_keys()."""
+        cost in 1/64-octave buckets); see order_keys()."""
         import torch
         B = Y.shape[0]
         if order is None:
@@ -222,15 +222,15 @@ def training_targets(dt_first, naccept, nstiff, nreject=None):
     return tgt, lab, np.log2(np.maximum(cost, 1.0)).astype(np.float32)
 
 
-ORDER_BUCKETS = 512  # log2-cost buckets per class, 1/16 octave each over [0, 32)
+ORDER_BUCKETS = 2048  # log2-cost buckets per class, 1/64 octave each over [0, 32)
 
 
 def order_keys(stiff_logit, log2_cost):
     """Sort keys of the launch order (numpy): predicted non-stiff class
-    first, each class by decreasing predicted log2 cost in 1/16-octave
+    first, each class by decreasing predicted log2 cost in 1/64-octave
     buckets -- the keys mideq_mlp_launch_order sorts by."""
     c = np.asarray(log2_cost, dtype=np.float32)
-    b = np.where(c >= 0, np.minimum(np.floor(np.where(c >= 0, c, 0) * np.float32(16)),
+    b = np.where(c >= 0, np.minimum(np.floor(np.where(c >= 0, c, 0) * np.float32(64)),
                                     ORDER_BUCKETS - 1), 0).astype(np.int64)
     cls = (np.asarray(stiff_logit) > 0).astype(np.int64)
     return cls * ORDER_BUCKETS + (ORDER_BUCKETS - 1 - b)

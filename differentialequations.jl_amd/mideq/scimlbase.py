@@ -160,10 +160,25 @@ class Rodas5:
 
 
 @dataclass(frozen=True)
+class Rodas5P:
+    """Rodas5P (qa.jl:97): Rodas5's stage structure with Steinebach's 2023
+    coefficients, the Rosenbrock that DifferentialEquations re-exports."""
+    code = _abi.ALG["rodas5p"]
+
+
+@dataclass(frozen=True)
 class AutoTsit5:
-    """AutoTsit5(Rodas5()) (qa.jl:15): Tsit5 with stiffness detection."""
+    """AutoTsit5(Rodas5()) / AutoTsit5(Rodas5P()) (qa.jl:15): Tsit5 with
+    stiffness detection, switching to the given Rosenbrock method."""
     stiffalg: Any = field(default_factory=Rodas5)
-    code = _abi.ALG["auto"]
+
+    def __post_init__(self):
+        if not isinstance(self.stiffalg, (Rodas5, Rodas5P)):
+            raise TypeError(f"AutoTsit5 supports Rodas5() and Rodas5P(), got {self.stiffalg!r}")
+
+    @property
+    def code(self):
+        return _abi.ALG["auto5p"] if isinstance(self.stiffalg, Rodas5P) else _abi.ALG["auto"]
 
 
 @dataclass(frozen=True)

@@ -47,7 +47,13 @@ typedef enum {
   MIDEQ_MODEL_LINEAR = 5       /* u' = M u, n = 3, p = M row-major (9), has J          */
 } mideq_model;
 
-typedef enum { MIDEQ_ALG_TSIT5 = 0, MIDEQ_ALG_RODAS5 = 1, MIDEQ_ALG_AUTO = 2 } mideq_alg;
+typedef enum {
+  MIDEQ_ALG_TSIT5 = 0,       /* Tsit5()                                     */
+  MIDEQ_ALG_RODAS5 = 1,      /* Rodas5()                                    */
+  MIDEQ_ALG_AUTO = 2,        /* AutoTsit5(Rodas5())                         */
+  MIDEQ_ALG_RODAS5P = 3,     /* Rodas5P() (test/qa/qa.jl:97)                */
+  MIDEQ_ALG_AUTO_RODAS5P = 4 /* AutoTsit5(Rodas5P())                        */
+} mideq_alg;
 
 /* FP64: every operation in double.  MIXED: stage sums and RHS evaluations
  * in FP32; state update, error norm, time (compensated), J, W and its LU in
@@ -204,6 +210,12 @@ int mideq_mlp_grad(mideq_mlp *m, const float *X, const float *target, const floa
                    const float *cost, int64_t B, float w_dt, float w_cls, float w_cost,
                    float *loss_host, void *stream);
 int mideq_mlp_get_grads(mideq_mlp *m, float *grads);           /* host [NPARAM] */
+/* order[B] (device) = a launch order for mideq_options.order from Y[B][3]
+ * (device, the output of mideq_mlp_infer): predicted non-stiff trajectories
+ * first, each class by decreasing predicted cost in 1/64-octave buckets
+ * (order inside a bucket arbitrary).  Enqueued on `stream`. */
+int mideq_mlp_launch_order(mideq_mlp *m, const float *Y, int64_t B, int64_t *order,
+                           void *stream);
 int mideq_mlp_adam_step(mideq_mlp *m, float lr, float beta1, float beta2, float eps,
                         void *stream);
 

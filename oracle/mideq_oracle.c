@@ -77,36 +77,67 @@ static const double TS_R[7][4] = {
 /* C = diag(1/gamma) - Gamma^-1.  Stages 7 and 8 reuse row 6 of a plus       */
 /* unit entries (stiffly accurate), so only rows 2..6 are stored.            */
 /* ------------------------------------------------------------------------ */
-static const double RD_GAMMA = 0.19;
-static const double RD_A[6][5] = {
-    {0, 0, 0, 0, 0},
-    {2.0, 0, 0, 0, 0},
-    {3.040894194418781, 1.041747909077569, 0, 0, 0},
-    {2.576417536461461, 1.622083060776640, -0.9089668560264532, 0, 0},
-    {2.760842080225597, 1.446624659844071, -0.3036980084553738,
-     0.2877498600325443, 0},
-    {-14.09640773051259, 6.925207756232704, -41.47510893210728,
-     2.343771018586405, 24.13215229196062}};
-static const double RD_C[8][7] = {
-    {0, 0, 0, 0, 0, 0, 0},
-    {-10.31323885133993, 0, 0, 0, 0, 0, 0},
-    {-21.04823117650003, -7.234992135176716, 0, 0, 0, 0, 0},
-    {32.22751541853323, -4.943732386540191, 19.44922031041879, 0, 0, 0, 0},
-    {-20.69865579590063, -8.816374604402768, 1.260436877740897,
-     -0.7495647613787146, 0, 0, 0},
-    {-46.22004352711257, -17.49534862857472, -289.6389582892057,
-     93.60855400400906, 318.3822534212147, 0, 0},
-    {34.20013733472935, -14.15535402717690, 57.82335640988400,
-     25.83362985412365, 1.408950972071624, -6.551835421242162, 0},
-    {42.57076742291101, -13.80770672017997, 93.98938432427124,
-     18.77919633714503, -31.58359187223370, -6.685968952921985,
-     -5.810979938412932}};
-static const double RD_CC[8] = {0.0, 0.38, 0.3878509998321533,
-                                0.4839718937873840, 0.4570477008819580,
-                                1.0, 1.0, 1.0};
-static const double RD_D[8] = {0.19, -0.1823079225333714636,
-                               -0.319231832186874912, 0.3449828624712811,
-                               -0.377417564392089818, 0.0, 0.0, 0.0};
+typedef struct {
+  double gamma;
+  double A[6][5];
+  double C[8][7];
+  double c[8];
+  double d[8];
+} rdtab_t;
+
+static const rdtab_t RD5 = {
+    0.19,
+    {{0, 0, 0, 0, 0},
+     {2.0, 0, 0, 0, 0},
+     {3.040894194418781, 1.041747909077569, 0, 0, 0},
+     {2.576417536461461, 1.622083060776640, -0.9089668560264532, 0, 0},
+     {2.760842080225597, 1.446624659844071, -0.3036980084553738, 0.2877498600325443, 0},
+     {-14.09640773051259, 6.925207756232704, -41.47510893210728, 2.343771018586405,
+      24.13215229196062}},
+    {{0, 0, 0, 0, 0, 0, 0},
+     {-10.31323885133993, 0, 0, 0, 0, 0, 0},
+     {-21.04823117650003, -7.234992135176716, 0, 0, 0, 0, 0},
+     {32.22751541853323, -4.943732386540191, 19.44922031041879, 0, 0, 0, 0},
+     {-20.69865579590063, -8.816374604402768, 1.260436877740897, -0.7495647613787146, 0, 0, 0},
+     {-46.22004352711257, -17.49534862857472, -289.6389582892057, 93.60855400400906,
+      318.3822534212147, 0, 0},
+     {34.20013733472935, -14.15535402717690, 57.82335640988400, 25.83362985412365,
+      1.408950972071624, -6.551835421242162, 0},
+     {42.57076742291101, -13.80770672017997, 93.98938432427124, 18.77919633714503,
+      -31.58359187223370, -6.685968952921985, -5.810979938412932}},
+    {0.0, 0.38, 0.3878509998321533, 0.4839718937873840, 0.4570477008819580, 1.0, 1.0, 1.0},
+    {0.19, -0.1823079225333714636, -0.319231832186874912, 0.3449828624712811,
+     -0.377417564392089818, 0.0, 0.0, 0.0}};
+
+/* Rodas5P (Steinebach, "Construction of Rosenbrock-Wanner method Rodas5P
+ * and numerical benchmarks within the Julia Differential Equations
+ * package", BIT 63 (2023) 27), same W-form and stage structure as Rodas5:
+ * the Rosenbrock that DifferentialEquations re-exports (test/qa/qa.jl:97,
+ * src/DifferentialEquations.jl:28). */
+static const rdtab_t RD5P = {
+    0.21193756319429014,
+    {{0, 0, 0, 0, 0},
+     {3.0, 0, 0, 0, 0},
+     {2.849394379747939, 0.45842242204463923, 0, 0, 0},
+     {-6.954028509809101, 2.489845061869568, -10.358996098473584, 0, 0},
+     {2.8029986275628964, 0.5072464736228206, -0.3988312541770524, -0.04721187230404641, 0},
+     {-7.502846399306121, 2.561846144803919, -11.627539656261098, -0.18268767659942256,
+      0.030198172008377946}},
+    {{0, 0, 0, 0, 0, 0, 0},
+     {-14.155112264123755, 0, 0, 0, 0, 0, 0},
+     {-17.97296035885952, -2.859693295451294, 0, 0, 0, 0, 0},
+     {147.12150275711716, -1.41221402718213, 71.68940251302358, 0, 0, 0, 0},
+     {165.43517024871676, -0.4592823456491126, 42.90938336958603, -5.961986721573306, 0, 0, 0},
+     {24.854864614690072, -3.0009227002832186, 47.4931110020768, 5.5814197821558125,
+      -0.6610691825249471, 0, 0},
+     {30.91273214028599, -3.1208243349937974, 77.79954646070892, 34.28646028294783,
+      -19.097331116725623, -28.087943162872662, 0},
+     {37.80277123390563, -3.2571969029072276, 112.26918849496327, 66.9347231244047,
+      -40.06618937091002, -54.66780262877968, -9.48861652309627}},
+    {0.0, 0.6358126895828704, 0.4095798393397535, 0.9769306725060716, 0.4288403609558664, 1.0,
+     1.0, 1.0},
+    {0.21193756319429014, -0.42387512638858027, -0.3384627126235924, 1.8046452872882734,
+     2.325825639765069, 0.0, 0.0, 0.0}};
 
 /* ------------------------------------------------------------------------ */
 /* Controller constants (OrdinaryDiffEq defaults for an order-5 method).     */
@@ -798,15 +829,15 @@ typedef struct {
   double rpiv[NMAX];
 } rodas_ws;
 
-static int rodas5_step(const mdl_t *m, int mixed, const double *uprev, double t,
-                       double dt, double abstol, double reltol, double *u1,
+static int rodas5_step(const mdl_t *m, const rdtab_t *T, int mixed, const double *uprev,
+                       double t, double dt, double abstol, double reltol, double *u1,
                        double *eest, rodas_ws *ws, double *normJ, int L) {
   int n = m->n;
   double kd[8][NMAX];  /* k in FP64 (values exactly representable in FP32 when mixed) */
   double dT[NMAX], x[NMAX], U[NMAX];
   float Uf[NMAX], duf[NMAX];
   double du[NMAX];
-  double dtgamma = dt * RD_GAMMA;
+  double dtgamma = dt * T->gamma;
   double w = 1.0 / dtgamma;
   double inv_dt = 1.0 / dt;
   int autonomous = model_autonomous(m->model);
@@ -830,14 +861,14 @@ static int rodas5_step(const mdl_t *m, int mixed, const double *uprev, double t,
 
   for (int s = 0; s < 8; s++) {
     /* stage input */
-    double ts = t + RD_CC[s] * dt;
+    double ts = t + T->c[s] * dt;
     if (!mixed) {
       if (s == 0) {
         for (int c = 0; c < n; c++) U[c] = uprev[c];
       } else if (s <= 5) {
         for (int c = 0; c < n; c++) {
           double acc = uprev[c];
-          for (int j = 0; j < s; j++) acc = fma(RD_A[s][j], kd[j][c], acc);
+          for (int j = 0; j < s; j++) acc = fma(T->A[s][j], kd[j][c], acc);
           U[c] = acc;
         }
       } else {
@@ -851,7 +882,7 @@ static int rodas5_step(const mdl_t *m, int mixed, const double *uprev, double t,
       } else if (s <= 5) {
         for (int c = 0; c < n; c++) {
           float acc = (float)uprev[c];
-          for (int j = 0; j < s; j++) acc = fmaf((float)RD_A[s][j], (float)kd[j][c], acc);
+          for (int j = 0; j < s; j++) acc = fmaf((float)T->A[s][j], (float)kd[j][c], acc);
           Uf[c] = acc;
         }
       } else {
@@ -863,18 +894,18 @@ static int rodas5_step(const mdl_t *m, int mixed, const double *uprev, double t,
     /* right-hand side */
     for (int c = 0; c < n; c++) {
       double base = du[c];
-      if (!autonomous) base = base + (dt * RD_D[s]) * dT[c];
+      if (!autonomous) base = base + (dt * T->d[s]) * dT[c];
       double r;
       if (s == 0) {
         r = base;
       } else if (!mixed) {
-        double acc = (RD_C[s][0] * inv_dt) * kd[0][c];
-        for (int j = 1; j < s; j++) acc = fma(RD_C[s][j] * inv_dt, kd[j][c], acc);
+        double acc = (T->C[s][0] * inv_dt) * kd[0][c];
+        for (int j = 1; j < s; j++) acc = fma(T->C[s][j] * inv_dt, kd[j][c], acc);
         r = base + acc;
       } else {
         float idt = (float)inv_dt;
-        float acc = ((float)RD_C[s][0] * idt) * (float)kd[0][c];
-        for (int j = 1; j < s; j++) acc = fmaf((float)RD_C[s][j] * idt, (float)kd[j][c], acc);
+        float acc = ((float)T->C[s][0] * idt) * (float)kd[0][c];
+        for (int j = 1; j < s; j++) acc = fmaf((float)T->C[s][j] * idt, (float)kd[j][c], acc);
         r = (double)((float)base + acc);
       }
       x[c] = -r;
@@ -885,7 +916,7 @@ static int rodas5_step(const mdl_t *m, int mixed, const double *uprev, double t,
   /* u = U_6 + k6 + k7 + k8, accumulated in FP64 in both modes */
   for (int c = 0; c < n; c++) {
     double acc = uprev[c];
-    for (int j = 0; j < 5; j++) acc = fma(RD_A[5][j], kd[j][c], acc);
+    for (int j = 0; j < 5; j++) acc = fma(T->A[5][j], kd[j][c], acc);
     acc = acc + kd[5][c];
     acc = acc + kd[6][c];
     u1[c] = acc + kd[7][c];
@@ -894,15 +925,22 @@ static int rodas5_step(const mdl_t *m, int mixed, const double *uprev, double t,
   return 0;
 }
 
-int oracle_rodas5_step(int32_t model, int32_t n, const double *u, const double *p,
-                       double t, double dt, double abstol, double reltol,
-                       double *u1, double *eest) {
+int oracle_rosenbrock_step(int32_t which, int32_t model, int32_t n, const double *u,
+                           const double *p, double t, double dt, double abstol, double reltol,
+                           double *u1, double *eest) {
   mdl_t m;
   mdl_init(&m, model, n, p, model_np(model, n));
   rodas_ws *ws = (rodas_ws *)malloc(sizeof(rodas_ws));
-  int rc = rodas5_step(&m, 0, u, t, dt, abstol, reltol, u1, eest, ws, NULL, 64);
+  int rc = rodas5_step(&m, which == 1 ? &RD5P : &RD5, 0, u, t, dt, abstol, reltol, u1, eest,
+                       ws, NULL, 64);
   free(ws);
   return rc;
+}
+
+int oracle_rodas5_step(int32_t model, int32_t n, const double *u, const double *p,
+                       double t, double dt, double abstol, double reltol,
+                       double *u1, double *eest) {
+  return oracle_rosenbrock_step(0, model, n, u, p, t, dt, abstol, reltol, u1, eest);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -919,8 +957,11 @@ static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
   double abstol = o->abstol, reltol = o->reltol;
   long long maxiters = o->maxiters > 0 ? o->maxiters : 100000;
   double dtmax = o->dtmax > 0 ? o->dtmax : fabs(tf - t0);
-  int mode = (o->alg == ORACLE_ALG_RODAS5) ? 1 : 0; /* 0 Tsit5, 1 Rodas5 */
-  int autosw = (o->alg == ORACLE_ALG_AUTO);
+  /* 0 Tsit5, 1 Rosenbrock (Rodas5 or Rodas5P) */
+  int mode = (o->alg == ORACLE_ALG_RODAS5 || o->alg == ORACLE_ALG_RODAS5P) ? 1 : 0;
+  int autosw = (o->alg == ORACLE_ALG_AUTO || o->alg == ORACLE_ALG_AUTO5P);
+  const rdtab_t *rtab =
+      (o->alg == ORACLE_ALG_RODAS5P || o->alg == ORACLE_ALG_AUTO5P) ? &RD5P : &RD5;
   double qsteady_max = (o->qsteady_default && mode == 1) ? 1.2 : 1.0;
   int count = 0;
   double eig = 0.0;
@@ -1017,7 +1058,8 @@ static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
       res->nf += 6;
     } else {
       double nJ;
-      if (rodas5_step(m, mixed, uprev, t, dt, abstol, reltol, u1, &EEst, ws, &nJ, o->norm_lanes) != 0)
+      if (rodas5_step(m, rtab, mixed, uprev, t, dt, abstol, reltol, u1, &EEst, ws, &nJ,
+                      o->norm_lanes) != 0)
         EEst = 2.0;
       res->nf += 8;
       res->njac += 1;
@@ -1272,28 +1314,33 @@ int oracle_tsit5_tableau(double *c7, double *a49, double *btilde7, double *r28) 
   return 0;
 }
 
-int oracle_rodas5_tableau(double *gamma, double *a64, double *C64, double *c8,
-                          double *d8) {
+int oracle_rosenbrock_tableau(int32_t which, double *gamma, double *a64, double *C64,
+                              double *c8, double *d8) {
   /* full 8x8 a (stages 7, 8 expanded: a7j = a6j, a76 = 1; a8j = a7j, a87 = 1) */
-  *gamma = RD_GAMMA;
+  const rdtab_t *T = which == 1 ? &RD5P : &RD5;
+  *gamma = T->gamma;
   memset(a64, 0, sizeof(double) * 64);
   memset(C64, 0, sizeof(double) * 64);
   for (int i = 1; i < 6; i++)
-    for (int j = 0; j < i; j++) a64[i * 8 + j] = RD_A[i][j];
+    for (int j = 0; j < i; j++) a64[i * 8 + j] = T->A[i][j];
   for (int j = 0; j < 5; j++) {
-    a64[6 * 8 + j] = RD_A[5][j];
-    a64[7 * 8 + j] = RD_A[5][j];
+    a64[6 * 8 + j] = T->A[5][j];
+    a64[7 * 8 + j] = T->A[5][j];
   }
   a64[6 * 8 + 5] = 1.0;
   a64[7 * 8 + 5] = 1.0;
   a64[7 * 8 + 6] = 1.0;
   for (int i = 1; i < 8; i++)
-    for (int j = 0; j < i; j++) C64[i * 8 + j] = RD_C[i][j];
+    for (int j = 0; j < i; j++) C64[i * 8 + j] = T->C[i][j];
   for (int i = 0; i < 8; i++) {
-    c8[i] = RD_CC[i];
-    d8[i] = RD_D[i];
+    c8[i] = T->c[i];
+    d8[i] = T->d[i];
   }
   return 0;
+}
+
+int oracle_rodas5_tableau(double *gamma, double *a64, double *C64, double *c8, double *d8) {
+  return oracle_rosenbrock_tableau(0, gamma, a64, C64, c8, d8);
 }
 
 void oracle_rhs(int32_t model, int32_t n, const double *u, const double *p,

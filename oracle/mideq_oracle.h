@@ -52,7 +52,13 @@ enum {
   ORACLE_MODEL_PROTHERO = 6     /* y' = lambda (y - sin t) + cos t (non-autonomous)*/
 };
 
-enum { ORACLE_ALG_TSIT5 = 0, ORACLE_ALG_RODAS5 = 1, ORACLE_ALG_AUTO = 2 };
+enum {
+  ORACLE_ALG_TSIT5 = 0,
+  ORACLE_ALG_RODAS5 = 1,
+  ORACLE_ALG_AUTO = 2,     /* AutoTsit5(Rodas5()) */
+  ORACLE_ALG_RODAS5P = 3,
+  ORACLE_ALG_AUTO5P = 4    /* AutoTsit5(Rodas5P()) */
+};
 enum { ORACLE_PREC_FP64 = 0, ORACLE_PREC_MIXED = 1 };
 
 /* return codes, mapped 1:1 to SciMLBase.ReturnCode (qa.jl:213) */
@@ -115,6 +121,12 @@ int oracle_ensemble_solve(int32_t model, int32_t n, int32_t np, int64_t ntraj,
 int oracle_tsit5_tableau(double *c7, double *a49, double *btilde7, double *r28);
 int oracle_rodas5_tableau(double *gamma, double *a64, double *C64, double *c8,
                           double *d8);
+/* which: 0 Rodas5, 1 Rodas5P (tableau; one fixed step) */
+int oracle_rosenbrock_step(int32_t which, int32_t model, int32_t n, const double *u,
+                           const double *p, double t, double dt, double abstol, double reltol,
+                           double *u1, double *eest);
+int oracle_rosenbrock_tableau(int32_t which, double *gamma, double *a64, double *C64,
+                              double *c8, double *d8);
 double oracle_ctl_log2(double x);
 double oracle_ctl_exp2(double x);
 double oracle_ctl_pow(double x, double y);

@@ -7,9 +7,10 @@ import numpy as np
 
 import mideq
 from mideq import EnsembleMI355X, device_solve
-from mideq.scimlbase import AutoTsit5, Rodas5, Tsit5
+from mideq.scimlbase import AutoTsit5, Rodas5, Rodas5P, Tsit5
 
-ALGS = dict(tsit5=Tsit5(), rodas5=Rodas5(), auto=AutoTsit5())
+ALGS = dict(tsit5=Tsit5(), rodas5=Rodas5(), auto=AutoTsit5(), rodas5p=Rodas5P(),
+            auto5p=AutoTsit5(Rodas5P()))
 FUNCS = dict(exp=mideq.exp_growth, lorenz=mideq.lorenz, robertson=mideq.robertson,
              vdp=mideq.vanderpol, linear=mideq.linear3, brusselator=mideq.brusselator)
 NTHREADS = min(16, os.cpu_count() or 1)

@@ -11,7 +11,7 @@ REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB = os.path.join(REPO, "oracle", "_build", "libmideq_oracle.so")
 
 MODEL = dict(exp=0, lorenz=1, robertson=2, vdp=3, brusselator=4, linear=5, prothero=6)
-ALG = dict(tsit5=0, rodas5=1, auto=2)
+ALG = dict(tsit5=0, rodas5=1, auto=2, rodas5p=3, auto5p=4)
 PREC = dict(fp64=0, mixed=1)
 RC = dict(default=0, success=1, maxiters=2, dtlessthanmin=3, unstable=4, dtnan=5)
 NMAX = 128
@@ -74,6 +74,7 @@ class Oracle:
             _dp, _ip, _dp]
         L.oracle_tsit5_tableau.argtypes = [_dp, _dp, _dp, _dp]
         L.oracle_rodas5_tableau.argtypes = [_dp, _dp, _dp, _dp, _dp]
+        L.oracle_rosenbrock_tableau.argtypes = [C.c_int32, _dp, _dp, _dp, _dp, _dp]
         for nm in ("oracle_ctl_log2", "oracle_ctl_exp2"):
             getattr(L, nm).argtypes = [C.c_double]
             getattr(L, nm).restype = C.c_double
@@ -92,6 +93,7 @@ class Oracle:
                                         C.c_double, C.c_double, _dp, _dp, _dp]
         L.oracle_rodas5_step.argtypes = [C.c_int32, C.c_int32, _dp, _dp, C.c_double,
                                          C.c_double, C.c_double, C.c_double, _dp, _dp]
+        L.oracle_rosenbrock_step.argtypes = [C.c_int32] + L.oracle_rodas5_step.argtypes
         L.oracle_tsit5_interp.argtypes = [C.c_int32, _dp, _dp, C.c_double, C.c_double, _dp]
 
     # ---------------------------------------------------------------- options
@@ -183,13 +185,14 @@ class Oracle:
         self.lib.oracle_tsit5_tableau(_ptr(c), _ptr(a), _ptr(bt), _ptr(r))
         return c, a.reshape(7, 7), bt, r.reshape(7, 4)
 
-    def rodas5_tableau(self):
+    def rodas5_tableau(self, which=0):
+        """which: 0 Rodas5, 1 Rodas5P."""
         g = C.c_double()
         a = np.zeros(64)
         Cm = np.zeros(64)
         c = np.zeros(8)
         d = np.zeros(8)
-        self.lib.oracle_rodas5_tableau(C.byref(g), _ptr(a), _ptr(Cm), _ptr(c), _ptr(d))
+        self.lib.oracle_rosenbrock_tableau(which, C.byref(g), _ptr(a), _ptr(Cm), _ptr(c), _ptr(d))
         return g.value, a.reshape(8, 8), Cm.reshape(8, 8), c, d
 
     def lu_solve(self, A, b):
@@ -231,14 +234,15 @@ class Oracle:
                                    _ptr(u1), C.byref(e), _ptr(k))
         return u1, e.value, k.reshape(7, n)
 
-    def rodas5_step(self, model, u, p, t, dt, abstol, reltol):
+    def rodas5_step(self, model, u, p, t, dt, abstol, reltol, which=0):
+        """One fixed Rosenbrock step; which: 0 Rodas5, 1 Rodas5P."""
         u = np.ascontiguousarray(u, dtype=np.float64)
         p = np.ascontiguousarray(p, dtype=np.float64)
         n = len(u)
         u1 = np.zeros(n)
         e = C.c_double()
-        rc = self.lib.oracle_rodas5_step(MODEL[model], n, _ptr(u), _ptr(p), t, dt, abstol,
-                                         reltol, _ptr(u1), C.byref(e))
+        rc = self.lib.oracle_rosenbrock_step(which, MODEL[model], n, _ptr(u), _ptr(p), t, dt,
+                                             abstol, reltol, _ptr(u1), C.byref(e))
         return rc, u1, e.value
 
     def tsit5_interp(self, uprev, k, dt, theta):

@@ -45,7 +45,8 @@ def test_bruss_tsit5_bitexact(oracle, lanes, precision):
     assert_identical(g, c, nvec=w["nvec"])
 
 
-@pytest.mark.parametrize("alg,lanes", [("rodas5", 64), ("rodas5", 32), ("auto", 64)])
+@pytest.mark.parametrize("alg,lanes", [("rodas5", 64), ("rodas5", 32), ("auto", 64),
+                                       ("rodas5p", 64), ("auto5p", 32)])
 def test_bruss_rodas5_and_auto_bitexact(oracle, alg, lanes):
     N = 48
     say(f"brusselator {alg} fp64 lanes={lanes} N={N}")

@@ -52,14 +52,15 @@ def test_lorenz_mixed_bitexact(oracle, schedule):
     assert_identical(g, c)
 
 
+@pytest.mark.parametrize("alg", ["rodas5", "rodas5p"])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_robertson_rodas5_bitexact(oracle, precision):
+def test_robertson_rodas5_bitexact(oracle, precision, alg):
     N = 2048
-    say(f"robertson rodas5 {precision} N={N}")
+    say(f"robertson {alg} {precision} N={N}")
     w = workloads.robertson(N, seed=21)
-    g = gpu("robertson", w["u0"], w["p"], 0.0, 1e5, alg="rodas5", precision=precision,
+    g = gpu("robertson", w["u0"], w["p"], 0.0, 1e5, alg=alg, precision=precision,
             abstol=1e-8, reltol=1e-6)
-    c = cpu(oracle, "robertson", w["u0"], w["p"], 0.0, 1e5, alg="rodas5", precision=precision,
+    c = cpu(oracle, "robertson", w["u0"], w["p"], 0.0, 1e5, alg=alg, precision=precision,
             abstol=1e-8, reltol=1e-6)
     assert (c["retcode"] == 1).all()
     assert_identical(g, c)
@@ -68,7 +69,9 @@ def test_robertson_rodas5_bitexact(oracle, precision):
 
 
 @pytest.mark.parametrize("model,alg", [("vdp", "auto"), ("vdp", "rodas5"), ("robertson", "auto"),
-                                       ("lorenz", "auto"), ("lorenz", "rodas5")])
+                                       ("lorenz", "auto"), ("lorenz", "rodas5"),
+                                       ("vdp", "auto5p"), ("vdp", "rodas5p"),
+                                       ("robertson", "auto5p"), ("lorenz", "rodas5p")])
 def test_auto_and_rodas_models_bitexact(oracle, model, alg):
     N = 256
     say(f"{model} {alg} fp64 N={N}")
@@ -143,7 +146,9 @@ def test_per_trajectory_tspans_and_fixed_dt(oracle):
 
 @pytest.mark.parametrize("model,alg,precision", [("lorenz", "tsit5", "fp64"), ("lorenz", "tsit5", "mixed"),
                                                  ("robertson", "rodas5", "fp64"), ("vdp", "auto", "fp64"),
-                                                 ("robertson", "auto", "mixed")])
+                                                 ("robertson", "auto", "mixed"),
+                                                 ("robertson", "rodas5p", "fp64"),
+                                                 ("vdp", "auto5p", "fp64")])
 def test_fast_controller_bitexact(oracle, model, alg, precision):
     """The division-free controller (EEst^2 in log2) against the oracle's
     restatement of the same form."""

@@ -79,8 +79,11 @@ def test_tsit5_interpolant(oracle):
         assert worst5 > 1e-6
 
 
-def rodas5(oracle):
-    g, a, Cm, c, d = oracle.rodas5_tableau()
+ROSENBROCK = pytest.mark.parametrize("which", [0, 1], ids=["rodas5", "rodas5p"])
+
+
+def rodas5(oracle, which=0):
+    g, a, Cm, c, d = oracle.rodas5_tableau(which)
     Ginv = np.diag(np.full(8, 1.0 / g)) - Cm
     Gam = np.linalg.inv(Ginv)
     alpha = a @ Gam
@@ -92,34 +95,39 @@ def rodas5(oracle):
     return g, a, Cm, c, d, Gam, alpha, b, bhat
 
 
-def test_rodas5_gamma_diagonal(oracle):
-    g, a, Cm, c, d, Gam, *_ = rodas5(oracle)
+@ROSENBROCK
+def test_rodas5_gamma_diagonal(oracle, which):
+    g, a, Cm, c, d, Gam, *_ = rodas5(oracle, which)
     np.testing.assert_allclose(np.diag(Gam), g, rtol=1e-14)
     assert np.allclose(np.triu(Gam, 1), 0.0)
 
 
-def test_rodas5_c_and_d_are_row_sums(oracle):
-    g, a, Cm, c, d, Gam, alpha, b, bhat = rodas5(oracle)
+@ROSENBROCK
+def test_rodas5_c_and_d_are_row_sums(oracle, which):
+    g, a, Cm, c, d, Gam, alpha, b, bhat = rodas5(oracle, which)
     np.testing.assert_allclose(alpha.sum(axis=1), c, atol=2e-13)
     np.testing.assert_allclose(Gam.sum(axis=1), d, atol=2e-13)
 
 
-def test_rodas5_order5(oracle):
-    g, a, Cm, c, d, Gam, alpha, b, bhat = rodas5(oracle)
+@ROSENBROCK
+def test_rodas5_order5(oracle, which):
+    g, a, Cm, c, d, Gam, alpha, b, bhat = rodas5(oracle, which)
     res = row_residuals(alpha, Gam, b, 5)
     bad = [r for r in res if abs(r[2]) > 1e-11]
     assert not bad, bad
 
 
-def test_rodas5_embedded_order4(oracle):
-    g, a, Cm, c, d, Gam, alpha, b, bhat = rodas5(oracle)
+@ROSENBROCK
+def test_rodas5_embedded_order4(oracle, which):
+    g, a, Cm, c, d, Gam, alpha, b, bhat = rodas5(oracle, which)
     res = row_residuals(alpha, Gam, bhat, 5)
     assert max(abs(r) for p, t, r in res if p <= 4) < 1e-11
     assert max(abs(r) for p, t, r in res if p == 5) > 1e-6
 
 
-def test_rodas5_stiffly_accurate_and_Rinf(oracle):
-    g, a, Cm, c, d, Gam, alpha, b, bhat = rodas5(oracle)
+@ROSENBROCK
+def test_rodas5_stiffly_accurate_and_Rinf(oracle, which):
+    g, a, Cm, c, d, Gam, alpha, b, bhat = rodas5(oracle, which)
     beta = alpha + Gam
     # stiff accuracy: b equals the last row of beta
     np.testing.assert_allclose(b, beta[7], atol=1e-12)
@@ -159,3 +167,33 @@ def test_planted_mistake_is_caught(oracle):
     m[7] = 1.0
     res = row_residuals(a @ Gam2, Gam2, m @ Gam2, 5)
     assert max(abs(r) for *_, r in res) > 1e-10
+
+
+def test_rodas5p_tableau_values(oracle):
+    """Rodas5P's gamma and node/derivative vectors (Steinebach 2023) and that
+    it is a different method from Rodas5."""
+    g, a, Cm, c, d = oracle.rodas5_tableau(1)
+    assert g == 0.21193756319429014
+    np.testing.assert_allclose(c[:5], [0.0, 0.6358126895828704, 0.4095798393397535,
+                                       0.9769306725060716, 0.4288403609558664], rtol=1e-15)
+    np.testing.assert_allclose(d[:5], [0.21193756319429014, -0.42387512638858027,
+                                       -0.3384627126235924, 1.8046452872882734,
+                                       2.325825639765069], rtol=1e-15)
+    g5, a5, *_ = oracle.rodas5_tableau(0)
+    assert g5 == 0.19 and not np.allclose(a, a5)
+
+
+def test_rosenbrock_linear_order(oracle):
+    """R(z) = 1 + z b (I - z beta)^-1 1 has Taylor coefficients b beta^k 1:
+    both methods match exp(z) through z^5; Rodas5P also through z^6 (order 6
+    on linear problems), Rodas5 not."""
+    import math
+    one = np.ones(8)
+    for which, k_exact in ((0, 5), (1, 6)):
+        g, a, Cm, c, d, Gam, alpha, b, bhat = rodas5(oracle, which)
+        beta = alpha + Gam
+        dev = [abs(b @ np.linalg.matrix_power(beta, k) @ one - 1 / math.factorial(k + 1))
+               for k in range(8)]
+        assert max(dev[:k_exact]) < 1e-13, (which, dev)
+        assert dev[k_exact] > 1e-6, (which, dev)
+
