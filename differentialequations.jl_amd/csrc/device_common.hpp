@@ -128,6 +128,7 @@ constexpr double kQmin = 0.2, kQmax = 10.0, kQoldInit = 1e-4;
 constexpr double kStabSize = 3.5068;
 constexpr int kMaxStiff = 10, kMaxNonStiff = 3;
 constexpr double kStiffTol = 0.9, kDtFac = 2.0;
+constexpr double kStiffTolS = kStiffTol * kStabSize;  // fast form of the stiffness test
 
 enum : int8_t { RC_DEFAULT = 0, RC_SUCCESS = 1, RC_MAXITERS = 2, RC_DTLESSTHANMIN = 3,
                 RC_UNSTABLE = 4, RC_DTNAN = 5, RC_INITIALFAILURE = 6 };

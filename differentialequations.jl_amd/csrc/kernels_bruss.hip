@@ -269,7 +269,11 @@ struct BTeam {
       dt = initdt(a, t0);
     qold = kQoldInit;
     qoldpow = ctl_pow(kQoldInit, kBeta2);
-    if (!(tf > t0)) rc = RC_SUCCESS;
+    // a zero-length span is done; a reversed (or NaN) span is not integrated
+    if (tf == t0)
+      rc = RC_SUCCESS;
+    else if (!(tf > t0))
+      rc = RC_INITIALFAILURE;
   }
 
   // ---------------------------------------------------------------- Tsit5

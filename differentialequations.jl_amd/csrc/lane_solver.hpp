@@ -154,6 +154,42 @@ __device__ __forceinline__ void tsit5_bthetas(double th, double* b) {
   b[6] = th2 * fma(th, fma(th, r74, r73), r72);
 }
 
+// AutoSwitch stiffness estimate max_c |dk_c / dg_c| over the components.
+// Reference form: one division per component.  Fast form: the running
+// maximum kept as a fraction (num, den) and compared by cross-multiplying,
+// so no division; a component whose quotient would be NaN (0/0, inf/inf,
+// NaN) makes the estimate NaN in both forms.
+template <int N, bool FAST>
+__device__ __forceinline__ void stiff_estimate(const double* dk, const double* dg, double& num,
+                                               double& den) {
+  if constexpr (!FAST) {
+    double e = 0.0;
+    bool nan_seen = false;
+#pragma unroll
+    for (int c = 0; c < N; c++) {
+      double r = fabs(dk[c] / dg[c]);
+      nan_seen = nan_seen || (r != r);
+      e = (r > e) ? r : e;
+    }
+    num = nan_seen ? __builtin_nan("") : e;
+    den = 1.0;
+  } else {
+    double en = 0.0, ed = 1.0;
+    bool nan_seen = false;
+#pragma unroll
+    for (int c = 0; c < N; c++) {
+      double nn = fabs(dk[c]), dd = fabs(dg[c]);
+      nan_seen = nan_seen || (nn != nn) || (dd != dd) || (nn == 0.0 && dd == 0.0) ||
+                 (nn == __builtin_inf() && dd == __builtin_inf());
+      bool take = nn * ed > en * dd;
+      en = take ? nn : en;
+      ed = take ? dd : ed;
+    }
+    num = nan_seen ? __builtin_nan("") : en;
+    den = ed;
+  }
+}
+
 template <class M, bool MIX, int ALG, bool FAST>
 struct Lane {
   static constexpr int N = M::N;
@@ -172,6 +208,7 @@ struct Lane {
   double t, tcomp, tf, dt, dtmax;
   double tsnap;  // 100 eps(tf): a step ending this close to tf ends at tf
   double qold, qoldpow, qsteady_max, eig, dtfirst, lqold;
+  double eigd;  // fast form: the stiffness estimate is eig / eigd
   int64_t iter;
   int32_t naccept, nreject, nf, nstiff;
   int32_t isave, count, nanrej;
@@ -274,6 +311,7 @@ struct Lane {
     count = 0;
     nanrej = 0;
     eig = 0.0;
+    eigd = 1.0;
     dtfirst = 0.0;
     rc = RC_DEFAULT;
     mode = (ALG == ALG_RODAS5 || ALG == ALG_RODAS5P) ? 1 : 0;
@@ -301,7 +339,11 @@ struct Lane {
     qold = kQoldInit;
     qoldpow = ctl_pow(kQoldInit, kBeta2);
     lqold = kL2QoldInit;
-    if (!(tf > t0)) rc = RC_SUCCESS;
+    // a zero-length span is done; a reversed (or NaN) span is not integrated
+    if (tf == t0)
+      rc = RC_SUCCESS;
+    else if (!(tf > t0))
+      rc = RC_INITIALFAILURE;
   }
 
   // ------------------------------------------------------------ Tsit5
@@ -362,15 +404,13 @@ struct Lane {
     }
     M::f(k[6], u1, p);
     if (kAuto) {
-      double e = 0.0;
-      bool nan_seen = false;
+      double dk[N], dg[N];
 #pragma unroll
       for (int c = 0; c < N; c++) {
-        double r = fabs((k[6][c] - k[5][c]) / (u1[c] - g6[c]));
-        nan_seen = nan_seen || (r != r);
-        e = (r > e) ? r : e;
+        dk[c] = k[6][c] - k[5][c];
+        dg[c] = u1[c] - g6[c];
       }
-      eig = nan_seen ? __builtin_nan("") : e;
+      stiff_estimate<N, FAST>(dk, dg, eig, eigd);
     }
 #pragma unroll
     for (int c = 0; c < N; c++) {
@@ -448,15 +488,13 @@ struct Lane {
     }
     M::f(k[6], u1f, pf);
     if (kAuto) {
-      double e = 0.0;
-      bool nan_seen = false;
+      double dk[N], dg[N];
 #pragma unroll
       for (int c = 0; c < N; c++) {
-        double r = fabs(((double)k[6][c] - (double)k[5][c]) / (u1[c] - (double)g6[c]));
-        nan_seen = nan_seen || (r != r);
-        e = (r > e) ? r : e;
+        dk[c] = (double)k[6][c] - (double)k[5][c];
+        dg[c] = u1[c] - (double)g6[c];
       }
-      eig = nan_seen ? __builtin_nan("") : e;
+      stiff_estimate<N, FAST>(dk, dg, eig, eigd);
     }
 #pragma unroll
     for (int c = 0; c < N; c++) {
@@ -489,6 +527,7 @@ struct Lane {
         best = (s > best) ? s : best;
       }
       eig = best;
+      eigd = 1.0;
     }
     double dtgamma = dts * rd_gamma<kP>();
     double w = 1.0 / dtgamma;
@@ -640,8 +679,14 @@ struct Lane {
   // trajectory finishes.
   __device__ void step(const LaneArgs& a) {
     if (kAuto && iter > 0) {
-      double stiffness = fabs(eig * dt / kStabSize);
-      bool stiff = stiffness > kStiffTol;
+      bool stiff;
+      if constexpr (!FAST) {
+        double stiffness = fabs(eig * dt / kStabSize);
+        stiff = stiffness > kStiffTol;
+      } else {
+        // |eig dt / S| > tol  <=>  num * dt > (tol S) * den   (num, dt >= 0, den > 0)
+        stiff = eig * dt > kStiffTolS * eigd;
+      }
       count = stiff ? (count < 0 ? 1 : count + 1) : (count > 0 ? -1 : count - 1);
       if (mode == 0 && count > kMaxStiff) {
         dt = dt * kDtFac;

@@ -152,6 +152,7 @@ static const rdtab_t RD5P = {
 #define AS_MAXSTIFF 10
 #define AS_MAXNONSTIFF 3
 #define AS_TOL 0.9
+#define AS_TOL_S (AS_TOL * AS_STAB_SIZE) /* fast form of the stiffness test */
 #define AS_DTFAC 2.0
 
 /* ------------------------------------------------------------------------ */
@@ -669,13 +670,47 @@ int oracle_lu_solve(int32_t n, double *a, double *b) {
 }
 
 /* ------------------------------------------------------------------------ */
+/* AutoSwitch stiffness estimate max_c |dk_c / dg_c| as a fraction num/den.  */
+/* Reference form: one division per component (den = 1).  Fast-controller    */
+/* form: the running maximum kept as (num, den) and compared by cross-       */
+/* multiplying; a component whose quotient would be NaN makes num NaN.       */
+/* ------------------------------------------------------------------------ */
+static void stiff_estimate(const double *dk, const double *dg, int n, double *num,
+                           double *den) {
+  int nan_seen = 0;
+  if (!g_fast_ctl) {
+    double e = 0.0;
+    for (int c = 0; c < n; c++) {
+      double r = fabs(dk[c] / dg[c]);
+      if (r != r) nan_seen = 1;
+      if (r > e) e = r;
+    }
+    *num = nan_seen ? NAN : e;
+    *den = 1.0;
+    return;
+  }
+  double en = 0.0, ed = 1.0;
+  for (int c = 0; c < n; c++) {
+    double nn = fabs(dk[c]), dd = fabs(dg[c]);
+    if (nn != nn || dd != dd || (nn == 0.0 && dd == 0.0) || (nn == INFINITY && dd == INFINITY))
+      nan_seen = 1;
+    if (nn * ed > en * dd) {
+      en = nn;
+      ed = dd;
+    }
+  }
+  *num = nan_seen ? NAN : en;
+  *den = ed;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Tsit5 step, FP64.  k[0] = f(uprev) on entry (FSAL); on exit k[0..6]       */
 /* hold the 7 stages and u1 the new state.  Returns EEst; *eig receives the  */
 /* composite-algorithm eigenvalue estimate max|(k7-k6)/(g7-g6)|.             */
 /* ------------------------------------------------------------------------ */
 static double tsit5_step64(const mdl_t *m, const double *uprev, double t,
                            double dt, double abstol, double reltol,
-                           double k[7][NMAX], double *u1, double *eig, int L) {
+                           double k[7][NMAX], double *u1, double *eig, double *eigd, int L) {
   int n = m->n;
   double g[NMAX], g6[NMAX], ut[NMAX];
   double a = dt * TS_A[1][0];
@@ -697,14 +732,14 @@ static double tsit5_step64(const mdl_t *m, const double *uprev, double t,
   }
   /* stage s=5 input is g6, its RHS is k6 (index 5); s=6 row gives u1 */
   rhs64(m, u1, t + dt, k[6]);
-  double e = 0.0;
-  int nan_seen = 0;
-  for (int c = 0; c < n; c++) {
-    double r = fabs((k[6][c] - k[5][c]) / (u1[c] - g6[c]));
-    if (r != r) nan_seen = 1;
-    if (r > e) e = r;
+  {
+    double dk[NMAX], dg[NMAX];
+    for (int c = 0; c < n; c++) {
+      dk[c] = k[6][c] - k[5][c];
+      dg[c] = u1[c] - g6[c];
+    }
+    stiff_estimate(dk, dg, n, eig, eigd);
   }
-  *eig = nan_seen ? NAN : e;
   for (int c = 0; c < n; c++) {
     double acc = TS_BT[6] * k[6][c];
     for (int j = 5; j >= 0; j--) acc = fma(TS_BT[j], k[j][c], acc);
@@ -717,7 +752,7 @@ static double tsit5_step64(const mdl_t *m, const double *uprev, double t,
 /* update u1 = uprev + dt*(sum b_j k_j), the error norm and t in FP64.       */
 static double tsit5_step_mix(const mdl_t *m, const double *uprev, double t,
                              double dt, double abstol, double reltol,
-                             float k[7][NMAX], double *u1, double *eig, int L) {
+                             float k[7][NMAX], double *u1, double *eig, double *eigd, int L) {
   int n = m->n;
   float u32[NMAX], g[NMAX], g6[NMAX], u1f[NMAX];
   double ut[NMAX];
@@ -743,14 +778,14 @@ static double tsit5_step_mix(const mdl_t *m, const double *uprev, double t,
     u1f[c] = (float)u1[c];
   }
   rhs32(m, u1f, t + dt, k[6]);
-  double e = 0.0;
-  int nan_seen = 0;
-  for (int c = 0; c < n; c++) {
-    double r = fabs(((double)k[6][c] - (double)k[5][c]) / (u1[c] - (double)g6[c]));
-    if (r != r) nan_seen = 1;
-    if (r > e) e = r;
+  {
+    double dk[NMAX], dg[NMAX];
+    for (int c = 0; c < n; c++) {
+      dk[c] = (double)k[6][c] - (double)k[5][c];
+      dg[c] = u1[c] - (double)g6[c];
+    }
+    stiff_estimate(dk, dg, n, eig, eigd);
   }
-  *eig = nan_seen ? NAN : e;
   for (int c = 0; c < n; c++) {
     float acc = (float)TS_BT[6] * k[6][c];
     for (int j = 5; j >= 0; j--) acc = fmaf((float)TS_BT[j], k[j][c], acc);
@@ -765,9 +800,9 @@ void oracle_tsit5_step(int32_t model, int32_t n, const double *u, const double *
   mdl_t m;
   mdl_init(&m, model, n, p, model_np(model, n));
   static __thread double k[7][NMAX];
-  double eig;
+  double eig, eigd;
   rhs64(&m, u, t, k[0]);
-  *eest = tsit5_step64(&m, u, t, dt, abstol, reltol, k, u1, &eig, 64);
+  *eest = tsit5_step64(&m, u, t, dt, abstol, reltol, k, u1, &eig, &eigd, 64);
   if (kout)
     for (int s = 0; s < 7; s++)
       for (int c = 0; c < n; c++) kout[s * n + c] = k[s][c];
@@ -964,7 +999,7 @@ static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
       (o->alg == ORACLE_ALG_RODAS5P || o->alg == ORACLE_ALG_AUTO5P) ? &RD5P : &RD5;
   double qsteady_max = (o->qsteady_default && mode == 1) ? 1.2 : 1.0;
   int count = 0;
-  double eig = 0.0;
+  double eig = 0.0, eigd = 1.0; /* stiffness estimate eig / eigd */
 
   memset(res, 0, sizeof(*res));
   res->first_stiff_iter = -1;
@@ -1008,12 +1043,22 @@ static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
   double lqold = L2_QOLDINIT;
   double lqs_cur = 0.0; /* log2 of qsteady_max: 0 or log2(1.2) */
 
-  if (!(tf > t0)) rc = ORACLE_RC_SUCCESS;
+  /* a zero-length span is done; a reversed (or NaN) span is not integrated */
+  if (tf == t0)
+    rc = ORACLE_RC_SUCCESS;
+  else if (!(tf > t0))
+    rc = ORACLE_RC_INITIALFAILURE;
   while (rc == ORACLE_RC_DEFAULT && t < tf) {
     /* loopheader!: choose algorithm, fix dt at bounds, clamp to tstop */
     if (autosw && iter > 0) {
-      double stiffness = fabs(eig * dt / AS_STAB_SIZE);
-      int stiff = stiffness > AS_TOL;
+      int stiff;
+      if (!g_fast_ctl) {
+        double stiffness = fabs(eig * dt / AS_STAB_SIZE);
+        stiff = stiffness > AS_TOL;
+      } else {
+        /* |eig dt / S| > tol  <=>  num * dt > (tol S) * den */
+        stiff = eig * dt > AS_TOL_S * eigd;
+      }
       count = stiff ? (count < 0 ? 1 : count + 1) : (count > 0 ? -1 : count - 1);
       if (mode == 0 && count > AS_MAXSTIFF) {
         dt = dt * AS_DTFAC;
@@ -1052,9 +1097,11 @@ static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
     double EEst;
     if (mode == 0) {
       if (!mixed)
-        EEst = tsit5_step64(m, uprev, t, dt, abstol, reltol, k64, u1, &eig, o->norm_lanes);
+        EEst = tsit5_step64(m, uprev, t, dt, abstol, reltol, k64, u1, &eig, &eigd,
+                            o->norm_lanes);
       else
-        EEst = tsit5_step_mix(m, uprev, t, dt, abstol, reltol, k32, u1, &eig, o->norm_lanes);
+        EEst = tsit5_step_mix(m, uprev, t, dt, abstol, reltol, k32, u1, &eig, &eigd,
+                              o->norm_lanes);
       res->nf += 6;
     } else {
       double nJ;
@@ -1064,6 +1111,7 @@ static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
       res->nf += 8;
       res->njac += 1;
       eig = nJ;
+      eigd = 1.0;
     }
     if (EEst != EEst) {
       /* FP64: DtNaN as upstream.  Mixed: FP32 stages can overflow on a far

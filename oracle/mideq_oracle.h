@@ -68,7 +68,8 @@ enum {
   ORACLE_RC_MAXITERS = 2,
   ORACLE_RC_DTLESSTHANMIN = 3,
   ORACLE_RC_UNSTABLE = 4,
-  ORACLE_RC_DTNAN = 5
+  ORACLE_RC_DTNAN = 5,
+  ORACLE_RC_INITIALFAILURE = 6
 };
 
 typedef struct {

@@ -126,6 +126,14 @@ def test_retcodes_forced_from_host(oracle):
     c = cpu(oracle, "lorenz", w["u0"], p, 0.0, 1.0)
     assert g.retcode[3] in (4, 5) and (np.delete(g.retcode, 3) == 1).all()
     np.testing.assert_array_equal(g.retcode, c["retcode"])
+    # per-trajectory spans: zero length -> Success at u0; reversed -> InitialFailure
+    tf = np.full(N, 1.0)
+    tf[5], tf[9] = 0.0, -1.0
+    g = gpu("lorenz", w["u0"], w["p"], 0.0, tf)
+    c = cpu(oracle, "lorenz", w["u0"], w["p"], 0.0, tf)
+    assert g.retcode[5] == 1 and g.retcode[9] == 6
+    np.testing.assert_array_equal(g.u_final[:, [5, 9]], w["u0"][:, [5, 9]])
+    assert_identical(g, c)
 
 
 def test_per_trajectory_tspans_and_fixed_dt(oracle):

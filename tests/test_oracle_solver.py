@@ -358,6 +358,9 @@ def test_retcodes(oracle):
     assert r["retcode"] in (4, 5)
     r = oracle.solve("exp", [1.0], [1.0], (0.0, 0.0))
     assert r["retcode"] == 1 and r["naccept"] == 0
+    # a reversed span is not integrated (InitialFailure), never reported as Success
+    r = oracle.solve("exp", [1.0], [1.0], (1.0, 0.0))
+    assert r["retcode"] == 6 and r["naccept"] == 0 and r["u"][0] == 1.0
 
 
 # ----------------------------------------------------------- ensemble
