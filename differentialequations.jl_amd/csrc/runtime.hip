@@ -325,7 +325,8 @@ static int run_bruss(mideq_ctx* c, const mideq_problem* P, const mideq_options* 
     if (it != c->tuned_lanes.end()) {
       lanes = it->second;
     } else {
-      int64_t S = std::min<int64_t>(P->ntraj, 8192);
+      // Rosenbrock steps on n up to 128 cost ~500 Tsit5 steps: a smaller sample
+      int64_t S = std::min<int64_t>(P->ntraj, O->alg == MIDEQ_ALG_TSIT5 ? 8192 : 1024);
       std::vector<int64_t> sample(S);
       for (int64_t k = 0; k < S; k++) sample[k] = (k * P->ntraj) / S;
       void* d;

@@ -196,3 +196,34 @@ def test_cfg1_lorenz_10000_endpoint_bitexact(oracle, controller):
     assert (c["retcode"] == 1).all()
     assert_identical(g, c)
     assert (g.t_final == 10.0).all()
+
+
+@pytest.mark.parametrize("ns", [2048, 2500])
+def test_many_saveat_points_bitexact(oracle, ns):
+    """Save times staged in LDS (ns <= 2048) and read from global memory
+    (ns > 2048) give the same values as the oracle."""
+    N = 300
+    say(f"lorenz saveat ns={ns} N={N}")
+    w = workloads.lorenz(N, seed=81)
+    sa = workloads.saveat_uniform(0.0, 10.0, ns)
+    g = gpu("lorenz", w["u0"], w["p"], 0.0, 10.0, saveat=sa, abstol=1e-8, reltol=1e-8,
+            controller="fast")
+    c = cpu(oracle, "lorenz", w["u0"], w["p"], 0.0, 10.0, saveat=sa, abstol=1e-8, reltol=1e-8,
+            controller=1)
+    assert_identical(g, c)
+
+
+def test_saveat_outside_span_and_duplicates(oracle):
+    """Points before t0 and after tf are NaN, a point at t0 is u0 exactly,
+    duplicated points repeat the same value."""
+    N = 200
+    say(f"lorenz saveat edges N={N}")
+    w = workloads.lorenz(N, seed=82)
+    sa = np.array([-1.0, 0.0, 0.5, 0.5, 1.25, 2.0, 3.5])
+    g = gpu("lorenz", w["u0"], w["p"], 0.0, 2.0, saveat=sa, abstol=1e-8, reltol=1e-8)
+    c = cpu(oracle, "lorenz", w["u0"], w["p"], 0.0, 2.0, saveat=sa, abstol=1e-8, reltol=1e-8)
+    assert_identical(g, c)
+    assert np.isnan(g.us[0]).all() and np.isnan(g.us[6]).all()
+    np.testing.assert_array_equal(g.us[1], w["u0"])
+    np.testing.assert_array_equal(g.us[2], g.us[3])
+    np.testing.assert_array_equal(g.us[5], g.u_final)
