@@ -85,7 +85,14 @@ struct BTeam {
   int* ldsi;     // team LDS: pivot rows
   double* W;     // team scratch: W column-major
 
-  __device__ __forceinline__ int comp(int m) const { return tl + L * m; }
+  // Lane tl owns grid points j = tl + L*g (g < C/2), each as the pair of
+  // components (u_j, v_j) = (2j, 2j + 1) in slots (2g, 2g + 1), so both rows
+  // of a grid point's RHS are evaluated by one lane with no exchange.
+  static_assert(C % 2 == 0, "slots hold (u, v) pairs");
+  __device__ __forceinline__ int comp(int m) const { return 2 * (tl + L * (m >> 1)) + (m & 1); }
+  // Rows of W (and of the triangular solves) are owned lane-strided instead,
+  // so that a lane's row updates of a column-major W are coalesced.
+  __device__ __forceinline__ int rowc(int m) const { return tl + L * m; }
 
   // ---------------------------------------------------------------- RHS
   __device__ void rhs64(double* out, const double* in) {
@@ -96,26 +103,22 @@ struct BTeam {
     }
     team_sync();
 #pragma unroll
-    for (int m = 0; m < C; m++) {
-      int c = comp(m);
-      double r = 0.0;
-      if (c < n) {
-        int j = c >> 1;
+    for (int g = 0; g < C / 2; g++) {
+      int cu = comp(2 * g);  // u of grid point j = cu / 2; v is cu + 1 (n is even)
+      double du = 0.0, dv = 0.0;
+      if (cu < n) {
+        int j = cu >> 1;
         int jm = (j == 0) ? N - 1 : j - 1;
         int jp = (j == N - 1) ? 0 : j + 1;
-        if ((c & 1) == 0) {
-          double uu = in[m], vv = ldsd[c + 1];
-          double uuv = uu * uu * vv;
-          double lu = ldsd[2 * jm] - 2.0 * uu + ldsd[2 * jp];
-          r = A + uuv - B1 * uu + D * lu;
-        } else {
-          double uu = ldsd[c - 1], vv = in[m];
-          double uuv = uu * uu * vv;
-          double lv = ldsd[2 * jm + 1] - 2.0 * vv + ldsd[2 * jp + 1];
-          r = B * uu - uuv + D * lv;
-        }
+        double uu = in[2 * g], vv = in[2 * g + 1];
+        double uuv = uu * uu * vv;
+        double lu = ldsd[2 * jm] - 2.0 * uu + ldsd[2 * jp];
+        double lv = ldsd[2 * jm + 1] - 2.0 * vv + ldsd[2 * jp + 1];
+        du = A + uuv - B1 * uu + D * lu;
+        dv = B * uu - uuv + D * lv;
       }
-      out[m] = r;
+      out[2 * g] = du;
+      out[2 * g + 1] = dv;
     }
     team_sync();
   }
@@ -128,26 +131,22 @@ struct BTeam {
     }
     team_sync();
 #pragma unroll
-    for (int m = 0; m < C; m++) {
-      int c = comp(m);
-      float r = 0.0f;
-      if (c < n) {
-        int j = c >> 1;
+    for (int g = 0; g < C / 2; g++) {
+      int cu = comp(2 * g);
+      float du = 0.0f, dv = 0.0f;
+      if (cu < n) {
+        int j = cu >> 1;
         int jm = (j == 0) ? N - 1 : j - 1;
         int jp = (j == N - 1) ? 0 : j + 1;
-        if ((c & 1) == 0) {
-          float uu = in[m], vv = ldsf[c + 1];
-          float uuv = uu * uu * vv;
-          float lu = ldsf[2 * jm] - 2.0f * uu + ldsf[2 * jp];
-          r = Af + uuv - B1f * uu + Df * lu;
-        } else {
-          float uu = ldsf[c - 1], vv = in[m];
-          float uuv = uu * uu * vv;
-          float lv = ldsf[2 * jm + 1] - 2.0f * vv + ldsf[2 * jp + 1];
-          r = Bf * uu - uuv + Df * lv;
-        }
+        float uu = in[2 * g], vv = in[2 * g + 1];
+        float uuv = uu * uu * vv;
+        float lu = ldsf[2 * jm] - 2.0f * uu + ldsf[2 * jp];
+        float lv = ldsf[2 * jm + 1] - 2.0f * vv + ldsf[2 * jp + 1];
+        du = Af + uuv - B1f * uu + Df * lu;
+        dv = Bf * uu - uuv + Df * lv;
       }
-      out[m] = r;
+      out[2 * g] = du;
+      out[2 * g + 1] = dv;
     }
     team_sync();
   }
@@ -479,7 +478,7 @@ struct BTeam {
     team_sync();
 #pragma unroll
     for (int m = 0; m < C; m++) {
-      int r = comp(m);
+      int r = rowc(m);
       if (r < n) {
         for (int c = 0; c < n; c++) W[(int64_t)c * n + r] = 0.0;
         int j = r >> 1;
@@ -506,7 +505,7 @@ struct BTeam {
       double best = 0.0;
 #pragma unroll
       for (int m = 0; m < C; m++) {
-        int r = comp(m);
+        int r = rowc(m);
         if (r < n) {
           double s = 0.0;
           for (int c = 0; c < n; c++) s += fabs(W[(int64_t)c * n + r]);
@@ -519,7 +518,7 @@ struct BTeam {
     double w = 1.0 / dtgamma;
 #pragma unroll
     for (int m = 0; m < C; m++) {
-      int r = comp(m);
+      int r = rowc(m);
       if (r < n) W[(int64_t)r * n + r] = W[(int64_t)r * n + r] - w;
     }
     team_sync();
@@ -530,7 +529,7 @@ struct BTeam {
       int bi = n;
 #pragma unroll
       for (int m = 0; m < C; m++) {
-        int r = comp(m);
+        int r = rowc(m);
         if (r >= k && r < n) {
           double v = fabs(W[(int64_t)k * n + r]);
           if (v > bv) {
@@ -556,7 +555,7 @@ struct BTeam {
       if (pk != k) {
 #pragma unroll
         for (int m = 0; m < C; m++) {
-          int c = comp(m);
+          int c = rowc(m);
           if (c < n) {
             double x = W[(int64_t)c * n + k], y = W[(int64_t)c * n + pk];
             W[(int64_t)c * n + k] = y;
@@ -574,7 +573,7 @@ struct BTeam {
       }
 #pragma unroll
       for (int m = 0; m < C; m++) {
-        int r = comp(m);
+        int r = rowc(m);
         if (r > k && r < n) {
           double l = W[(int64_t)k * n + r] * rr;
           W[(int64_t)k * n + r] = l;
@@ -587,8 +586,9 @@ struct BTeam {
     return singular == 0;
   }
 
-  // Solve W x = b for owned components (x in/out); pivots applied as the
-  // composite permutation sigma (value at position r comes from sigma[r]).
+  // Solve W x = b for owned components (x in/out, component layout); rows
+  // are processed in row ownership, pivots applied as the composite
+  // permutation sigma (value at row position r comes from sigma[r]).
   __device__ void lu_solve(double* x, const int* sigma) {
 #pragma unroll
     for (int m = 0; m < C; m++) {
@@ -598,24 +598,24 @@ struct BTeam {
     team_sync();
     double y[C];
 #pragma unroll
-    for (int m = 0; m < C; m++) y[m] = (comp(m) < n) ? ldsd[sigma[m]] : 0.0;
+    for (int m = 0; m < C; m++) y[m] = (rowc(m) < n) ? ldsd[sigma[m]] : 0.0;
     team_sync();
     for (int j = 0; j < n; j++) {
 #pragma unroll
       for (int m = 0; m < C; m++)
-        if (comp(m) == j) ldsd[j] = y[m];
+        if (rowc(m) == j) ldsd[j] = y[m];
       team_sync();
       double xj = ldsd[j];
 #pragma unroll
       for (int m = 0; m < C; m++) {
-        int r = comp(m);
+        int r = rowc(m);
         if (r > j && r < n) y[m] = fma(-W[(int64_t)j * n + r], xj, y[m]);
       }
     }
     for (int j = n - 1; j >= 0; j--) {
 #pragma unroll
       for (int m = 0; m < C; m++) {
-        if (comp(m) == j) {
+        if (rowc(m) == j) {
           y[m] = y[m] * ldsr[j];
           ldsd[j] = y[m];
         }
@@ -624,13 +624,19 @@ struct BTeam {
       double xj = ldsd[j];
 #pragma unroll
       for (int m = 0; m < C; m++) {
-        int r = comp(m);
+        int r = rowc(m);
         if (r < j) y[m] = fma(-W[(int64_t)j * n + r], xj, y[m]);
       }
     }
     team_sync();
+    // back from row ownership to the (u, v) component layout
 #pragma unroll
-    for (int m = 0; m < C; m++) x[m] = y[m];
+    for (int m = 0; m < C; m++)
+      if (rowc(m) < n) ldsd[rowc(m)] = y[m];
+    team_sync();
+#pragma unroll
+    for (int m = 0; m < C; m++) x[m] = (comp(m) < n) ? ldsd[comp(m)] : 0.0;
+    team_sync();
   }
 
   __device__ bool rodas5_step(double dts, double* u1, double& EEst, double abstol,
@@ -642,7 +648,7 @@ struct BTeam {
     int sigma[C];
 #pragma unroll
     for (int m = 0; m < C; m++) {
-      int pos = comp(m);
+      int pos = rowc(m);
       for (int k = n - 1; k >= 0; k--) {
         int pk = ldsi[k];
         if (pos == k)

@@ -482,10 +482,11 @@ static void mdl_init(mdl_t *m, int model, int n, const double *p, int np) {
 /* Norms                                                                     */
 /* ------------------------------------------------------------------------ */
 /* ODE_DEFAULT_NORM for arrays: sqrt(sum(abs2, x) / length(x)); summed left  */
-/* to right for n <= 8; for larger n as L strided partial sums (partial l    */
-/* adds x_l^2, x_{l+L}^2, ... in order) combined by a pairwise tree          */
-/* (part[l] += part[l + w] for w = L/2, ..., 1): the order in which a team   */
-/* of L GPU lanes reduces.                                                   */
+/* to right for n <= 8; for larger n (the interleaved (u, v) Brusselator) as */
+/* L partial sums combined by a pairwise tree (part[l] += part[l + w] for    */
+/* w = L/2, ..., 1): the order in which a team of L GPU lanes reduces.       */
+/* Partial l adds, in order, the components its lane owns: the (u, v) pairs  */
+/* of grid points l, l + L, ... , i.e. x_{2l}, x_{2l+1}, x_{2(l+L)}, ...     */
 static double rms_norm(const double *x, int n, int L) {
   double s;
   if (n <= 8) {
@@ -494,9 +495,13 @@ static double rms_norm(const double *x, int n, int L) {
   } else {
     double part[64];
     if (L <= 0 || L > 64) L = 64;
+    int slots = NMAX / L; /* components per lane */
     for (int l = 0; l < L; l++) {
       double a = 0.0;
-      for (int i = l; i < n; i += L) a = a + x[i] * x[i];
+      for (int m = 0; m < slots; m++) {
+        int i = 2 * (l + L * (m / 2)) + (m % 2);
+        if (i < n) a = a + x[i] * x[i];
+      }
       part[l] = a;
     }
     for (int w = L / 2; w >= 1; w /= 2)
