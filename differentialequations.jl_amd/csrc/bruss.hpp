@@ -50,11 +50,12 @@ inline int64_t bruss_scratch_per_team(int nmax) {
   return (int64_t)nmax * nmax + 2 * (int64_t)nmax;
 }
 
-// Resident teams of the persistent grid for (lanes, alg, precision):
-// grid blocks and teams per block.  Returns hipSuccess or an error.
-hipError_t bruss_plan(int lanes, int alg, int mix, int ncu, int64_t ntraj, int* grid,
+// Resident teams of the persistent grid for (lanes, alg, precision,
+// controller): grid blocks and teams per block.  Returns hipSuccess or an
+// error.  fast: the division-free log2-domain PI controller.
+hipError_t bruss_plan(int lanes, int alg, int mix, int fast, int ncu, int64_t ntraj, int* grid,
                       int* teams_per_block);
-hipError_t launch_bruss_solver(const BrussArgs& a, int lanes, int alg, int mix, int grid,
-                               hipStream_t s);
+hipError_t launch_bruss_solver(const BrussArgs& a, int lanes, int alg, int mix, int fast,
+                               int grid, hipStream_t s);
 
 }  // namespace mideq

@@ -60,7 +60,7 @@ __device__ __forceinline__ int team_and(int v) {
   return v;
 }
 
-template <int L, int C, bool MIX, int ALG>
+template <int L, int C, bool MIX, int ALG, bool FAST>
 struct BTeam {
   static constexpr bool kAuto = (ALG == ALG_AUTO || ALG == ALG_AUTO5P);
   static constexpr bool kHasTsit5 = (ALG != ALG_RODAS5 && ALG != ALG_RODAS5P);
@@ -76,6 +76,8 @@ struct BTeam {
   double k1[C];
   float k1f[C];
   double t, tcomp, tf, dt, dtmax, qold, qoldpow, qsteady_max, eig, dtfirst;
+  double lqold;  // fast controller: log2 of the previous accepted EEst (>= log2 1e-4)
+  double tsnap;  // 100 eps(tf): a step ending this close to tf ends at tf
   int64_t iter;
   int32_t naccept, nreject, nf, nstiff, count, nanrej;
   int8_t rc, mode;
@@ -268,6 +270,9 @@ struct BTeam {
       dt = initdt(a, t0);
     qold = kQoldInit;
     qoldpow = ctl_pow(kQoldInit, kBeta2);
+    lqold = kL2QoldInit;
+    // the step loop runs while t < tf, so max(t, tf) there is tf
+    tsnap = 100.0 * jl_eps(tf);
     // a zero-length span is done; a reversed (or NaN) span is not integrated
     if (tf == t0)
       rc = RC_SUCCESS;
@@ -276,6 +281,8 @@ struct BTeam {
   }
 
   // ---------------------------------------------------------------- Tsit5
+  // EEst (reference controller) or EEst^2 = mean(r^2) without the square
+  // root (fast controller, which works on log2 EEst^2).
   __device__ double err_norm(const double* ut, const double* u1, double abstol,
                              double reltol) {
     double s = 0.0;
@@ -287,6 +294,7 @@ struct BTeam {
         s = s + r * r;
       }
     }
+    if constexpr (FAST) return team_sum<L>(s) / (double)n;
     return rms_from_partial(s);
   }
 
@@ -798,23 +806,43 @@ struct BTeam {
     } else {
       nanrej = 0;
     }
-    double q, q11 = 1.0, lE = 0.0;
-    if (EEst == 0.0) {
-      q = 1.0 / kQmax;
+    // PI controller (the fast form works on EEst^2 in the log2 domain)
+    double q = 1.0, q11 = 1.0, lE = 0.0, lq = 0.0, l11 = 0.0, lE2 = 0.0;
+    if constexpr (!FAST) {
+      if (EEst == 0.0) {
+        q = 1.0 / kQmax;
+      } else {
+        lE = ctl_log2(EEst);
+        q11 = ctl_exp2(kBeta1 * lE);
+        q = q11 / qoldpow;
+        q = fmax(1.0 / kQmax, fmin(1.0 / kQmin, q / kGammaSafety));
+      }
     } else {
-      lE = ctl_log2(EEst);
-      q11 = ctl_exp2(kBeta1 * lE);
-      q = q11 / qoldpow;
-      q = fmax(1.0 / kQmax, fmin(1.0 / kQmin, q / kGammaSafety));
+      if (EEst == 0.0) {
+        lq = kL2QmaxInv;
+      } else {
+        lE2 = fast_log2(EEst);
+        l11 = (0.5 * kBeta1) * lE2;
+        lq = l11 - kBeta2 * lqold - kL2Gamma;
+        lq = fmax(kL2QmaxInv, fmin(kL2QminInv, lq));
+      }
     }
     if (EEst <= 1.0) {
       if (naccept == 0) dtfirst = dt;
       naccept++;
       if (!tsit_step) nstiff++;
-      if (1.0 <= q && q <= qsteady_max) q = 1.0;
-      qold = fmax(EEst, kQoldInit);
-      qoldpow = (qold == EEst) ? ctl_exp2(kBeta2 * lE) : ctl_pow(kQoldInit, kBeta2);
-      double dtnew = dt / q;
+      double dtnew;
+      if constexpr (!FAST) {
+        if (1.0 <= q && q <= qsteady_max) q = 1.0;
+        qold = fmax(EEst, kQoldInit);
+        qoldpow = (qold == EEst) ? ctl_exp2(kBeta2 * lE) : ctl_pow(kQoldInit, kBeta2);
+        dtnew = dt / q;
+      } else {
+        double lqs = qsteady_max > 1.0 ? kL2QsteadyRodas : 0.0;
+        if (0.0 <= lq && lq <= lqs) lq = 0.0;
+        lqold = (EEst == 0.0) ? kL2QoldInit : fmax(0.5 * lE2, kL2QoldInit);
+        dtnew = dt * ctl_exp2(-lq);
+      }
       double ttmp;
       if (!MIX) {
         ttmp = t + dt;
@@ -823,7 +851,7 @@ struct BTeam {
         ttmp = t + y;
         tcomp = (ttmp - t) - y;
       }
-      if (fabs(ttmp - tf) < 100.0 * jl_eps(fmax(t, tf))) ttmp = tf;
+      if (fabs(ttmp - tf) < tsnap) ttmp = tf;
       t = ttmp;
 #pragma unroll
       for (int m = 0; m < C; m++) u[m] = u1[m];
@@ -846,7 +874,10 @@ struct BTeam {
 #pragma unroll
         for (int m = 0; m < C; m++) k1[m] = k1save[m];
       }
-      dt = dt / fmin(1.0 / kQmin, q11 / kGammaSafety);
+      if constexpr (!FAST)
+        dt = dt / fmin(1.0 / kQmin, q11 / kGammaSafety);
+      else
+        dt = dt * ctl_exp2(-fmin(kL2QminInv, l11 - kL2Gamma));
     }
     if (!(t < tf)) rc = RC_SUCCESS;
   }
@@ -871,7 +902,7 @@ struct BTeam {
   }
 };
 
-template <int L, bool MIX, int ALG>
+template <int L, bool MIX, int ALG, bool FAST>
 __global__ __launch_bounds__(256) void k_bruss(BrussArgs a) {
   constexpr int C = BRUSS_NMAX / L;
   constexpr int TPB = 256 / L;
@@ -880,7 +911,7 @@ __global__ __launch_bounds__(256) void k_bruss(BrussArgs a) {
   __shared__ float s_f[TPB][BRUSS_NMAX];
   __shared__ int s_i[TPB][BRUSS_NMAX];
   const int team = threadIdx.x / L;
-  BTeam<L, C, MIX, ALG> T;
+  BTeam<L, C, MIX, ALG, FAST> T;
   T.tl = threadIdx.x % L;
   T.ldsd = s_d[team];
   T.ldsr = s_r[team];
@@ -921,37 +952,55 @@ __global__ __launch_bounds__(256) void k_bruss(BrussArgs a) {
 
 __global__ void k_bruss_reset(unsigned long long* q) { *q = 0ull; }
 
-template <int L, bool MIX, int ALG>
-static hipError_t occupancy(int* per_cu) {
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_bruss<L, MIX, ALG>, 256, 0);
-}
-
-template <int L>
-static hipError_t occ_dispatch(int alg, int mix, int* per_cu) {
-  if (mix) {
-    if (alg == ALG_TSIT5) return occupancy<L, true, ALG_TSIT5>(per_cu);
-    if (alg == ALG_RODAS5) return occupancy<L, true, ALG_RODAS5>(per_cu);
-    if (alg == ALG_RODAS5P) return occupancy<L, true, ALG_RODAS5P>(per_cu);
-    if (alg == ALG_AUTO5P) return occupancy<L, true, ALG_AUTO5P>(per_cu);
-    return occupancy<L, true, ALG_AUTO>(per_cu);
+// One instantiation per (lanes, precision, algorithm, controller).
+struct OccupancyOp {
+  int* per_cu;
+  template <int L, bool MIX, int ALG, bool FAST>
+  hipError_t run() const {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_bruss<L, MIX, ALG, FAST>, 256,
+                                                        0);
   }
-  if (alg == ALG_TSIT5) return occupancy<L, false, ALG_TSIT5>(per_cu);
-  if (alg == ALG_RODAS5) return occupancy<L, false, ALG_RODAS5>(per_cu);
-  if (alg == ALG_RODAS5P) return occupancy<L, false, ALG_RODAS5P>(per_cu);
-  if (alg == ALG_AUTO5P) return occupancy<L, false, ALG_AUTO5P>(per_cu);
-  return occupancy<L, false, ALG_AUTO>(per_cu);
+};
+
+struct LaunchOp {
+  const BrussArgs* a;
+  int grid;
+  hipStream_t s;
+  template <int L, bool MIX, int ALG, bool FAST>
+  hipError_t run() const {
+    hipLaunchKernelGGL((k_bruss<L, MIX, ALG, FAST>), dim3(grid), dim3(256), 0, s, *a);
+    return hipGetLastError();
+  }
+};
+
+template <int L, bool MIX, bool FAST, class Op>
+static hipError_t by_alg(int alg, const Op& op) {
+  switch (alg) {
+    case ALG_TSIT5: return op.template run<L, MIX, ALG_TSIT5, FAST>();
+    case ALG_RODAS5: return op.template run<L, MIX, ALG_RODAS5, FAST>();
+    case ALG_RODAS5P: return op.template run<L, MIX, ALG_RODAS5P, FAST>();
+    case ALG_AUTO5P: return op.template run<L, MIX, ALG_AUTO5P, FAST>();
+    default: return op.template run<L, MIX, ALG_AUTO, FAST>();
+  }
 }
 
-hipError_t bruss_plan(int lanes, int alg, int mix, int ncu, int64_t ntraj, int* grid,
+template <class Op>
+static hipError_t dispatch(int lanes, int alg, int mix, int fast, const Op& op) {
+  if (lanes == 64) {
+    if (mix) return fast ? by_alg<64, true, true>(alg, op) : by_alg<64, true, false>(alg, op);
+    return fast ? by_alg<64, false, true>(alg, op) : by_alg<64, false, false>(alg, op);
+  }
+  if (lanes == 32) {
+    if (mix) return fast ? by_alg<32, true, true>(alg, op) : by_alg<32, true, false>(alg, op);
+    return fast ? by_alg<32, false, true>(alg, op) : by_alg<32, false, false>(alg, op);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t bruss_plan(int lanes, int alg, int mix, int fast, int ncu, int64_t ntraj, int* grid,
                       int* teams_per_block) {
   int per_cu = 0;
-  hipError_t e;
-  if (lanes == 32)
-    e = occ_dispatch<32>(alg, mix, &per_cu);
-  else if (lanes == 64)
-    e = occ_dispatch<64>(alg, mix, &per_cu);
-  else
-    return hipErrorInvalidValue;
+  hipError_t e = dispatch(lanes, alg, mix, fast, OccupancyOp{&per_cu});
   if (e != hipSuccess) return e;
   if (per_cu < 1) per_cu = 1;
   int tpb = 256 / lanes;
@@ -963,34 +1012,11 @@ hipError_t bruss_plan(int lanes, int alg, int mix, int ncu, int64_t ntraj, int* 
   return hipSuccess;
 }
 
-template <int L, bool MIX, int ALG>
-static hipError_t launch3(const BrussArgs& a, int grid, hipStream_t s) {
-  hipLaunchKernelGGL((k_bruss<L, MIX, ALG>), dim3(grid), dim3(256), 0, s, a);
-  return hipGetLastError();
-}
-
-template <int L>
-static hipError_t launch2(const BrussArgs& a, int alg, int mix, int grid, hipStream_t s) {
-  if (mix) {
-    if (alg == ALG_TSIT5) return launch3<L, true, ALG_TSIT5>(a, grid, s);
-    if (alg == ALG_RODAS5) return launch3<L, true, ALG_RODAS5>(a, grid, s);
-    if (alg == ALG_RODAS5P) return launch3<L, true, ALG_RODAS5P>(a, grid, s);
-    if (alg == ALG_AUTO5P) return launch3<L, true, ALG_AUTO5P>(a, grid, s);
-    return launch3<L, true, ALG_AUTO>(a, grid, s);
-  }
-  if (alg == ALG_TSIT5) return launch3<L, false, ALG_TSIT5>(a, grid, s);
-  if (alg == ALG_RODAS5) return launch3<L, false, ALG_RODAS5>(a, grid, s);
-  if (alg == ALG_RODAS5P) return launch3<L, false, ALG_RODAS5P>(a, grid, s);
-  if (alg == ALG_AUTO5P) return launch3<L, false, ALG_AUTO5P>(a, grid, s);
-  return launch3<L, false, ALG_AUTO>(a, grid, s);
-}
-
-hipError_t launch_bruss_solver(const BrussArgs& a, int lanes, int alg, int mix, int grid,
-                               hipStream_t s) {
+hipError_t launch_bruss_solver(const BrussArgs& a, int lanes, int alg, int mix, int fast,
+                               int grid, hipStream_t s) {
+  if (lanes != 32 && lanes != 64) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_bruss_reset, dim3(1), dim3(1), 0, s, a.queue);
-  if (lanes == 32) return launch2<32>(a, alg, mix, grid, s);
-  if (lanes == 64) return launch2<64>(a, alg, mix, grid, s);
-  return hipErrorInvalidValue;
+  return dispatch(lanes, alg, mix, fast, LaunchOp{&a, grid, s});
 }
 
 }  // namespace mideq

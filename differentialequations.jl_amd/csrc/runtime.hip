@@ -50,8 +50,8 @@ struct mideq_ctx {
   int ncu = 256;
   unsigned long long* d_queue = nullptr;
   DevBuf bufs[BUF_COUNT];
-  // autotuned lanes per instance, keyed by (alg, precision, nmax)
-  std::map<std::tuple<int, int, int>, int> tuned_lanes;
+  // autotuned lanes per instance, keyed by (alg, precision, controller, nmax)
+  std::map<std::tuple<int, int, int, int>, int> tuned_lanes;
   GraphCache graph;
 };
 
@@ -187,8 +187,6 @@ static int validate(mideq_ctx* c, const mideq_problem* P, const mideq_options* O
     if (O->ns > 0) return set_err(c, MIDEQ_E_UNSUPPORTED, "saveat not provided for the brusselator");
     if (O->lanes != 0 && O->lanes != 32 && O->lanes != 64)
       return set_err(c, MIDEQ_E_ARG, "lanes must be 0, 32 or 64");
-    if (O->controller != 0)
-      return set_err(c, MIDEQ_E_UNSUPPORTED, "the fast controller needs n <= 3");
   } else if (P->n != n) {
     return set_err(c, MIDEQ_E_ARG, "model %d has n=%d, got n=%d", P->model, n, P->n);
   }
@@ -257,10 +255,10 @@ static int enqueue(mideq_ctx* c, bool use_graph, const void* key, size_t keylen,
 }
 
 // ---------------------------------------------------------------- brusselator
-static int bruss_launch_with(mideq_ctx* c, BrussArgs b, int lanes, int alg, int mix,
+static int bruss_launch_with(mideq_ctx* c, BrussArgs b, int lanes, int alg, int mix, int fast,
                              bool use_graph, hipStream_t s, int* grid_used) {
   int grid = 0, tpb = 0;
-  hipError_t e = bruss_plan(lanes, alg, mix, c->ncu, b.nqueue, &grid, &tpb);
+  hipError_t e = bruss_plan(lanes, alg, mix, fast, c->ncu, b.nqueue, &grid, &tpb);
   if (e != hipSuccess) return set_err(c, MIDEQ_E_HIP, "bruss_plan: %s", hipGetErrorString(e));
   if (alg != MIDEQ_ALG_TSIT5) {
     b.scratch_stride = bruss_scratch_per_team(b.nmax);
@@ -276,10 +274,10 @@ static int bruss_launch_with(mideq_ctx* c, BrussArgs b, int lanes, int alg, int 
   *grid_used = grid;
   struct Key {
     BrussArgs b;
-    int lanes, alg, mix, grid;
-  } key{b, lanes, alg, mix, grid};
+    int lanes, alg, mix, fast, grid;
+  } key{b, lanes, alg, mix, fast, grid};
   return enqueue(c, use_graph, &key, sizeof key, s, [&](hipStream_t st) {
-    return launch_bruss_solver(b, lanes, alg, mix, grid, st);
+    return launch_bruss_solver(b, lanes, alg, mix, fast, grid, st);
   });
 }
 
@@ -320,7 +318,7 @@ static int run_bruss(mideq_ctx* c, const mideq_problem* P, const mideq_options* 
   if (lanes == 0) {
     // autotuner: time each candidate on a strided sample of the batch and
     // keep the fastest for this (alg, precision, nmax)
-    auto key = std::make_tuple((int)O->alg, (int)O->precision, (int)P->n);
+    auto key = std::make_tuple((int)O->alg, (int)O->precision, (int)O->controller, (int)P->n);
     auto it = c->tuned_lanes.find(key);
     if (it != c->tuned_lanes.end()) {
       lanes = it->second;
@@ -343,9 +341,13 @@ static int run_bruss(mideq_ctx* c, const mideq_problem* P, const mideq_options* 
       HIP_TRY(c, hipEventCreate(&a1));
       for (int cand : {64, 32}) {
         int g = 0;
-        if ((rc = bruss_launch_with(c, bs, cand, O->alg, O->precision, false, s, &g))) return rc;
+        if ((rc = bruss_launch_with(c, bs, cand, O->alg, O->precision, O->controller, false, s,
+                                    &g)))
+          return rc;
         HIP_TRY(c, hipEventRecord(a0, s));
-        if ((rc = bruss_launch_with(c, bs, cand, O->alg, O->precision, false, s, &g))) return rc;
+        if ((rc = bruss_launch_with(c, bs, cand, O->alg, O->precision, O->controller, false, s,
+                                    &g)))
+          return rc;
         HIP_TRY(c, hipEventRecord(a1, s));
         HIP_TRY(c, hipEventSynchronize(a1));
         float ms = 0.f;
@@ -362,7 +364,8 @@ static int run_bruss(mideq_ctx* c, const mideq_problem* P, const mideq_options* 
     }
   }
   HIP_TRY(c, hipEventRecord(c->ev0, s));
-  int rc = bruss_launch_with(c, b, lanes, O->alg, O->precision, O->graph != 0, s, &grid);
+  int rc = bruss_launch_with(c, b, lanes, O->alg, O->precision, O->controller, O->graph != 0, s,
+                             &grid);
   if (rc) return rc;
   HIP_TRY(c, hipEventRecord(c->ev1, s));
   c->ev_pending = true;

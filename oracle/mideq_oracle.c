@@ -487,7 +487,7 @@ static void mdl_init(mdl_t *m, int model, int n, const double *p, int np) {
 /* w = L/2, ..., 1): the order in which a team of L GPU lanes reduces.       */
 /* Partial l adds, in order, the components its lane owns: the (u, v) pairs  */
 /* of grid points l, l + L, ... , i.e. x_{2l}, x_{2l+1}, x_{2(l+L)}, ...     */
-static double rms_norm(const double *x, int n, int L) {
+static double sum_sq(const double *x, int n, int L) {
   double s;
   if (n <= 8) {
     s = x[0] * x[0];
@@ -508,11 +508,14 @@ static double rms_norm(const double *x, int n, int L) {
       for (int l = 0; l < w; l++) part[l] = part[l] + part[l + w];
     s = part[0];
   }
-  return sqrt(s / (double)n);
+  return s;
 }
 
-/* Set per solve: 1 selects the fast controller, whose error estimate is
- * EEst^2 from one division over a common denominator (n <= 3). */
+static double rms_norm(const double *x, int n, int L) { return sqrt(sum_sq(x, n, L) / (double)n); }
+
+/* Set per solve: 1 selects the fast controller, which works on EEst^2: from
+ * one division over a common denominator for n <= 3, else mean(r^2) without
+ * the square root.  Its division-free stiffness test applies for n <= 3. */
 static __thread int g_fast_ctl = 0;
 
 static double err_sq_fast(const double *ut, const double *u0, const double *u1, int n,
@@ -554,6 +557,7 @@ static double err_norm(const double *ut, const double *u0, const double *u1,
     double sc = fma(fmax(fabs(u0[i]), fabs(u1[i])), reltol, abstol);
     r[i] = ut[i] / sc;
   }
+  if (g_fast_ctl) return sum_sq(r, n, L) / (double)n; /* EEst^2 for the fast controller */
   return rms_norm(r, n, L);
 }
 
@@ -683,7 +687,7 @@ int oracle_lu_solve(int32_t n, double *a, double *b) {
 static void stiff_estimate(const double *dk, const double *dg, int n, double *num,
                            double *den) {
   int nan_seen = 0;
-  if (!g_fast_ctl) {
+  if (!g_fast_ctl || n > 3) {
     double e = 0.0;
     for (int c = 0; c < n; c++) {
       double r = fabs(dk[c] / dg[c]);
@@ -1044,7 +1048,7 @@ static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
   int rc = ORACLE_RC_DEFAULT;
   int have_accept = 0;
   int nanrej = 0;
-  g_fast_ctl = (o->controller == 1 && n <= 3);
+  g_fast_ctl = (o->controller == 1);
   double lqold = L2_QOLDINIT;
   double lqs_cur = 0.0; /* log2 of qsteady_max: 0 or log2(1.2) */
 
@@ -1057,7 +1061,7 @@ static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
     /* loopheader!: choose algorithm, fix dt at bounds, clamp to tstop */
     if (autosw && iter > 0) {
       int stiff;
-      if (!g_fast_ctl) {
+      if (!g_fast_ctl || n > 3) {
         double stiffness = fabs(eig * dt / AS_STAB_SIZE);
         stiff = stiffness > AS_TOL;
       } else {

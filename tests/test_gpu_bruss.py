@@ -87,3 +87,21 @@ def test_bruss_invalid_sizes_rejected():
     w["nvec"][3] = 15
     with pytest.raises(Exception):
         gpu("brusselator", w["u0"], w["p"], 0.0, w["tf"], nvec=w["nvec"], lanes=64)
+
+
+@pytest.mark.parametrize("alg,lanes,precision", [("tsit5", 64, "fp64"), ("tsit5", 32, "mixed"),
+                                                 ("rodas5", 64, "fp64"), ("auto", 32, "fp64")])
+def test_bruss_fast_controller_bitexact(oracle, alg, lanes, precision):
+    """The team kernels with the division-free controller (EEst^2 = mean r^2,
+    log2-domain PI law) against the oracle's restatement."""
+    N = 48 if alg != "tsit5" else 200
+    say(f"brusselator fast controller {alg} {precision} lanes={lanes} N={N}")
+    w = batch(N, seed=55, nmax_grid=20 if alg != "tsit5" else 64, tf_max=3.0)
+    tol = dict(abstol=1e-8, reltol=1e-6)
+    g = gpu("brusselator", w["u0"], w["p"], 0.0, w["tf"], nvec=w["nvec"], lanes=lanes,
+            alg=alg, precision=precision, schedule="persistent", controller="fast", **tol)
+    c = cpu(oracle, "brusselator", w["u0"], w["p"], 0.0, w["tf"], nvec=w["nvec"],
+            norm_lanes=lanes, alg=alg, precision=precision, controller=1, **tol)
+    assert (c["retcode"] == 1).all()
+    assert_identical(g, c, nvec=w["nvec"])
+    np.testing.assert_array_equal(g.nstiff, c["nstiff"])
