@@ -991,7 +991,7 @@ int oracle_rodas5_step(int32_t model, int32_t n, const double *u, const double *
 /* The integrator loop (solve! with loopheader!/perform_step!/loopfooter!). */
 /* ------------------------------------------------------------------------ */
 static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
-                     const oracle_options *o, double *save, long save_stride,
+                     const oracle_options *o, int64_t idx, double *save, long save_stride,
                      oracle_result *res, rodas_ws *ws) {
   int n = m->n;
   int mixed = o->precision == ORACLE_PREC_MIXED;
@@ -1004,6 +1004,7 @@ static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
   /* 0 Tsit5, 1 Rosenbrock (Rodas5 or Rodas5P) */
   int mode = (o->alg == ORACLE_ALG_RODAS5 || o->alg == ORACLE_ALG_RODAS5P) ? 1 : 0;
   int autosw = (o->alg == ORACLE_ALG_AUTO || o->alg == ORACLE_ALG_AUTO5P);
+  if (autosw && o->alg_hint) mode = o->alg_hint[idx] ? 1 : 0;
   const rdtab_t *rtab =
       (o->alg == ORACLE_ALG_RODAS5P || o->alg == ORACLE_ALG_AUTO5P) ? &RD5P : &RD5;
   double qsteady_max = (o->qsteady_default && mode == 1) ? 1.2 : 1.0;
@@ -1040,7 +1041,9 @@ static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
   }
   for (int c = 0; c < n; c++) k64[0][c] = f0[c];
 
-  double dt = o->dt > 0 ? o->dt : initdt_impl(m, uprev, f0, t0, dtmax, o, 5, &res->nf);
+  double dt = o->dt > 0 ? o->dt
+               : o->dt0_in ? o->dt0_in[idx]
+                           : initdt_impl(m, uprev, f0, t0, dtmax, o, 5, &res->nf);
   res->dt0 = dt;
   double qold = CTL_QOLDINIT;
   double q11 = 1.0;
@@ -1240,7 +1243,7 @@ int oracle_solve(int32_t model, int32_t n, const double *u0, const double *p,
   mdl_init(&m, model, n, p, np);
   rodas_ws *ws = (rodas_ws *)malloc(sizeof(rodas_ws));
   if (!ws) return -2;
-  solve_one(&m, u0, t0, tf, opt, out_save, 1, res, ws);
+  solve_one(&m, u0, t0, tf, opt, 0, out_save, 1, res, ws);
   free(ws);
   return 0;
 }
@@ -1287,7 +1290,7 @@ static void *ens_worker(void *arg) {
         for (int j = 0; j < J->np && j < 16; j++) p[j] = J->p[(long long)j * N + i];
         mdl_init(&m, J->model, n, p, J->np);
       }
-      solve_one(&m, u0, J->t0[i], J->tf[i], J->opt,
+      solve_one(&m, u0, J->t0[i], J->tf[i], J->opt, (int64_t)i,
                 J->out_save ? J->out_save + i : NULL, N, &res, ws);
       if (J->u_final)
         for (int c = 0; c < n; c++) J->u_final[(long long)c * N + i] = res.u[c];

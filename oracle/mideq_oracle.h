@@ -86,6 +86,10 @@ typedef struct {
   int32_t controller;     /* 0: PI controller as restated; 1: fast form (EEst^2, log2 domain, n <= 3) */
   int32_t norm_lanes;     /* n > 8: RMS sum as norm_lanes strided partials + pairwise tree
                              (the order of a team of that many GPU lanes); 0 = 64 */
+  const double *dt0_in;   /* [ntraj] per-trajectory initial dt instead of initdt, or NULL
+                             (ignored when dt > 0) */
+  const int8_t *alg_hint; /* [ntraj] initial method of AUTO/AUTO5P (0 Tsit5, 1 Rosenbrock),
+                             or NULL */
 } oracle_options;
 
 typedef struct {

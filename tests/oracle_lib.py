@@ -36,6 +36,8 @@ class Options(C.Structure):
         ("qsteady_default", C.c_int32),
         ("controller", C.c_int32),
         ("norm_lanes", C.c_int32),
+        ("dt0_in", _dp),
+        ("alg_hint", C.POINTER(C.c_int8)),
     ]
 
 
@@ -101,7 +103,7 @@ class Oracle:
     @staticmethod
     def options(alg="tsit5", precision="fp64", abstol=1e-6, reltol=1e-3, dt=0.0, dtmin=0.0,
                 dtmax=0.0, maxiters=0, saveat=None, qsteady_default=1, norm_lanes=64,
-                controller=0):
+                controller=0, dt0=None, alg_hint=None):
         o = Options()
         o.alg = ALG[alg]
         o.precision = PREC[precision]
@@ -114,6 +116,12 @@ class Oracle:
         o.qsteady_default = qsteady_default
         o.norm_lanes = norm_lanes
         o.controller = controller
+        if dt0 is not None:
+            o._dt0_keep = np.ascontiguousarray(dt0, dtype=np.float64)
+            o.dt0_in = _ptr(o._dt0_keep)
+        if alg_hint is not None:
+            o._hint_keep = np.ascontiguousarray(alg_hint, dtype=np.int8)
+            o.alg_hint = o._hint_keep.ctypes.data_as(C.POINTER(C.c_int8))
         if saveat is not None:
             sa = np.ascontiguousarray(saveat, dtype=np.float64)
             o._saveat_keep = sa

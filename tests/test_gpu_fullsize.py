@@ -72,3 +72,30 @@ def test_brusselator_131072_sampled_bitexact(oracle):
     # rows past each instance's size are left as the host supplied them (zero)
     mask = np.arange(g.u_final.shape[0])[:, None] >= w["nvec"][None, :]
     assert (g.u_final[mask] == 0.0).all()
+
+
+def test_vdp_mix_third_learned_inputs_sampled_bitexact(oracle):
+    """The Van der Pol third of the 1e6 mix (333,333 trajectories, mu up to
+    1e4, tspan (0, 2 mu)) solved the way the learned form runs it --
+    AutoTsit5 with per-trajectory dt0 and method hints, a launch order and
+    the fast controller -- with a strided sample checked against the oracle
+    given the same dt0 and hints."""
+    N = 333_333
+    say(f"full size vdp N={N} auto, dt0 + hint + order, fast controller")
+    w = workloads.vdp(N, seed=1234)
+    tf = np.asarray(w["tf"], dtype=np.float64)
+    rng = np.random.default_rng(10)
+    mu = w["p"][0]
+    dt0 = 10.0 ** (rng.uniform(-5, -3, N) - 0.5 * np.log10(mu))
+    hint = (mu > 100.0).astype(np.int8)
+    order = np.argsort(-np.log(mu), kind="stable")
+    kw = dict(abstol=1e-8, reltol=1e-6, dt0=dt0, alg_hint=hint, maxiters=1_000_000)
+    g = gpu("vdp", w["u0"], w["p"], 0.0, tf, alg="auto", controller="fast", order=order, **kw)
+    assert (g.retcode == 1).all()
+    np.testing.assert_array_equal(g.t_final, tf)
+    idx = _sample(N, 300)
+    kws = dict(abstol=1e-8, reltol=1e-6, dt0=dt0[idx], alg_hint=hint[idx], maxiters=1_000_000)
+    c = cpu(oracle, "vdp", w["u0"][:, idx], w["p"][:, idx], 0.0, tf[idx], alg="auto",
+            controller=1, **kws)
+    assert_identical(_Sub(g, idx), c)
+

@@ -26,6 +26,29 @@ def test_lorenz_saveat_with_order_bitexact(oracle, schedule):
     assert_identical(g, c)
 
 
+@pytest.mark.parametrize("alg,controller", [("auto", "reference"), ("auto", "fast"),
+                                            ("auto5p", "fast"), ("tsit5", "fast")])
+def test_dt0_and_hint_match_oracle(oracle, alg, controller):
+    """Per-trajectory dt0 and initial-method hints (what the learned predictor
+    feeds) against the oracle given the same arrays."""
+    N = 600
+    say(f"vdp {alg} {controller} with dt0 + hint N={N}")
+    w = workloads.vdp(N, seed=76)
+    w["p"] = np.minimum(w["p"], 300.0)
+    tf = 2.0 * w["p"][0]
+    rng = np.random.default_rng(8)
+    dt0 = 10.0 ** rng.uniform(-6, -2, N)
+    hint = (rng.uniform(size=N) > 0.5).astype(np.int8)
+    kw = dict(abstol=1e-8, reltol=1e-6, dt0=dt0, alg_hint=hint)
+    g = gpu("vdp", w["u0"], w["p"], 0.0, tf, alg=alg, controller=controller,
+            order=np.random.default_rng(9).permutation(N), **kw)
+    c = cpu(oracle, "vdp", w["u0"], w["p"], 0.0, tf, alg=alg,
+            controller=1 if controller == "fast" else 0, **kw)
+    assert (c["retcode"] == 1).all()
+    assert_identical(g, c)
+    np.testing.assert_array_equal(g.nstiff, c["nstiff"])
+
+
 def test_auto_hint_dt0_order_bitexact():
     N = 2500
     say(f"vdp auto with dt0, hint, order N={N}")
