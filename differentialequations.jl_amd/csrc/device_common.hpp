@@ -218,6 +218,23 @@ __device__ __forceinline__ double fast_log2(double x) {
   return fma(t, q, (double)e);
 }
 
+// 2^z for the fast controller's step factors, whose exponent is clamped to
+// |z| <= log2(10): z = n + f, n = rint(z), |f| <= 1/2, degree-6 Taylor
+// polynomial of e^x at x = f ln 2 (|x| <= 0.347, relative error < 2e-7,
+// exact at integer z).  No range checks: the domain is bounded.
+__device__ __forceinline__ double fast_exp2(double z) {
+  double nn = rint(z);
+  double f = z - nn;
+  double x = f * 0.69314718055994530942;
+  double p = fma(x, 1.0 / 720.0, 1.0 / 120.0);
+  p = fma(x, p, kpin(1.0 / 24.0));
+  p = fma(x, p, kpin(1.0 / 6.0));
+  p = fma(x, p, 0.5);
+  p = fma(x, p, 1.0);
+  p = fma(x, p, 1.0);
+  return ldexp(p, (int)nn);
+}
+
 // log2 of the controller constants (fast form)
 constexpr double kL2QmaxInv = -3.321928094887362;   // log2(1/10)
 constexpr double kL2QminInv = 2.321928094887362;    // log2(5)

@@ -779,7 +779,7 @@ struct Lane {
         double lqs = qsteady_max > 1.0 ? kL2QsteadyRodas : 0.0;
         if (0.0 <= lq && lq <= lqs) lq = 0.0;
         lqold = (EEst == 0.0) ? kL2QoldInit : fmax(0.5 * lE2, kL2QoldInit);
-        dtnew = dt * ctl_exp2(-lq);
+        dtnew = dt * fast_exp2(-lq);
       }
       double tprev = t;
       double ttmp;
@@ -859,7 +859,7 @@ struct Lane {
       if constexpr (!FAST)
         dt = dt / fmin(1.0 / kQmin, q11 / kGammaSafety);
       else
-        dt = dt * ctl_exp2(-fmin(kL2QminInv, l11 - kL2Gamma));
+        dt = dt * fast_exp2(-fmin(kL2QminInv, l11 - kL2Gamma));
     }
     if (!(t < tf)) rc = RC_SUCCESS;
   }

@@ -231,6 +231,23 @@ double oracle_fast_log2(double x) {
   return fma(t, q, (double)e);
 }
 
+/* 2^z for the fast controller's step factors, whose exponent is clamped to
+ * |z| <= log2(10): z = n + f with n = rint(z), |f| <= 1/2, and a degree-6
+ * Taylor polynomial of e^x at x = f ln 2 (|x| <= 0.347, relative error
+ * < 2e-7, exact at integer z).  No range checks: the domain is bounded. */
+double oracle_fast_exp2(double z) {
+  double nn = rint(z);
+  double f = z - nn;
+  double x = f * LN2_D;
+  double p = fma(x, 1.0 / 720.0, 1.0 / 120.0);
+  p = fma(x, p, 1.0 / 24.0);
+  p = fma(x, p, 1.0 / 6.0);
+  p = fma(x, p, 0.5);
+  p = fma(x, p, 1.0);
+  p = fma(x, p, 1.0);
+  return ldexp(p, (int)nn);
+}
+
 /* log2 of the controller constants */
 #define L2_QMAXINV (-3.321928094887362)   /* log2(1/10) */
 #define L2_QMININV 2.321928094887362      /* log2(5) */
@@ -1168,7 +1185,7 @@ static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
       } else {
         if (0.0 <= lq && lq <= lqs_cur) lq = 0.0;
         lqold = (EEst == 0.0) ? L2_QOLDINIT : fmax(0.5 * lE2, L2_QOLDINIT);
-        dtnew = dt * oracle_ctl_exp2(-lq);
+        dtnew = dt * oracle_fast_exp2(-lq);
       }
       double tprev = t;
       double ttmp;
@@ -1221,7 +1238,7 @@ static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
       if (!g_fast_ctl)
         dt = dt / fmin(1.0 / CTL_QMIN, q11 / CTL_GAMMA);
       else
-        dt = dt * oracle_ctl_exp2(-fmin(L2_QMININV, l11 - L2_GAMMA));
+        dt = dt * oracle_fast_exp2(-fmin(L2_QMININV, l11 - L2_GAMMA));
     }
   }
   if (rc == ORACLE_RC_DEFAULT) rc = ORACLE_RC_SUCCESS;

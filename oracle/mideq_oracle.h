@@ -136,6 +136,8 @@ double oracle_ctl_log2(double x);
 double oracle_ctl_exp2(double x);
 double oracle_ctl_pow(double x, double y);
 double oracle_fast_log2(double x);
+/* 2^z for |z| <= log2(10) (the fast controller's clamped step factors) */
+double oracle_fast_exp2(double z);
 /* One PI-controller decision: returns 1 (accept) or 0 (reject) and the next
  * dt / qold, exactly as the integrator loop applies them. */
 int oracle_controller(double EEst, double qold, double dt, double qsteady_max,

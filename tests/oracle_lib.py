@@ -83,6 +83,8 @@ class Oracle:
             getattr(L, nm).restype = C.c_double
         L.oracle_fast_log2.argtypes = [C.c_double]
         L.oracle_fast_log2.restype = C.c_double
+        L.oracle_fast_exp2.argtypes = [C.c_double]
+        L.oracle_fast_exp2.restype = C.c_double
         L.oracle_ctl_pow.argtypes = [C.c_double, C.c_double]
         L.oracle_ctl_pow.restype = C.c_double
         L.oracle_controller.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, _dp, _dp]
@@ -275,6 +277,12 @@ class Oracle:
 
     def ctl_exp2(self, x):
         return self.lib.oracle_ctl_exp2(x)
+
+    def fast_log2(self, x):
+        return self.lib.oracle_fast_log2(x)
+
+    def fast_exp2(self, x):
+        return self.lib.oracle_fast_exp2(x)
 
 
 _INST = None

@@ -123,6 +123,52 @@ def test_ctl_pow_accuracy(oracle):
     assert oracle.ctl_exp2(3.0) == 8.0 and oracle.ctl_log2(1024.0) == 10.0
 
 
+def test_fast_log2_accuracy(oracle):
+    """Division-free log2 of the fast controller: |error| < 1.4e-8 from its
+    degree-8 fit, exact at powers of two, IEEE special values."""
+    rng = np.random.default_rng(2)
+    xs = np.concatenate([10 ** rng.uniform(-300, 300, 3000), rng.uniform(0.5, 2.0, 3000)])
+    err = max(abs(oracle.fast_log2(float(x)) - math.log2(x)) for x in xs)
+    assert err < 1.4e-8, err
+    for k in (-1000, -3, 0, 1, 7, 1000):
+        assert oracle.fast_log2(2.0 ** k) == float(k)
+    assert oracle.fast_log2(0.0) == -math.inf and oracle.fast_log2(math.inf) == math.inf
+    assert math.isnan(oracle.fast_log2(-1.0))
+
+
+def test_fast_exp2_accuracy(oracle):
+    """2^z on the fast controller's clamped domain |z| <= log2(10): relative
+    error < 2e-7 (degree-6 series), exact at integers."""
+    zs = np.linspace(-math.log2(10.0), math.log2(10.0), 4001)
+    err = max(abs(oracle.fast_exp2(float(z)) / 2.0 ** z - 1.0) for z in zs)
+    assert err < 2e-7, err
+    for k in range(-3, 4):
+        assert oracle.fast_exp2(float(k)) == 2.0 ** k
+
+
+@pytest.mark.parametrize("model,alg,tol", [("lorenz", "tsit5", (1e-8, 1e-8)),
+                                           ("robertson", "rodas5", (1e-8, 1e-6)),
+                                           ("vdp", "auto", (1e-8, 1e-6))])
+def test_fast_controller_tracks_reference_controller(oracle, model, alg, tol):
+    """The fast controller is the same PI law evaluated differently: step
+    counts within 2 % and final-state errors of the same size as the
+    reference controller's, against a tight solve."""
+    w = {"lorenz": ([1.0, 0.0, 0.0], LORENZ_P, (0.0, 10.0)),
+         "robertson": ([1.0, 0.0, 0.0], ROBER_P, (0.0, 1e5)),
+         "vdp": ([2.0, 0.0], [300.0], (0.0, 600.0))}[model]
+    u0, p, span = w
+    at, rt = tol
+    ref = oracle.solve(model, u0, p, span, alg=alg, abstol=at * 1e-4, reltol=rt * 1e-4)
+    a = oracle.solve(model, u0, p, span, alg=alg, abstol=at, reltol=rt, controller=0)
+    b = oracle.solve(model, u0, p, span, alg=alg, abstol=at, reltol=rt, controller=1)
+    assert a["retcode"] == 1 and b["retcode"] == 1
+    na, nb = a["naccept"] + a["nreject"], b["naccept"] + b["nreject"]
+    assert abs(na - nb) <= 0.02 * na + 2, (na, nb)
+    ea = np.max(np.abs(a["u"] - ref["u"]) / (at + rt * np.abs(ref["u"])))
+    eb = np.max(np.abs(b["u"] - ref["u"]) / (at + rt * np.abs(ref["u"])))
+    assert eb < 3.0 * ea + 1.0, (ea, eb)
+
+
 # ----------------------------------------------------------- Tsit5 step vs linear theory
 def test_tsit5_step_linear_stability_polynomial(oracle):
     """On u'=lam u one step is u1 = R(z) u0 and the embedded error is
