@@ -131,7 +131,10 @@ constexpr double kStiffTol = 0.9, kDtFac = 2.0;
 constexpr double kStiffTolS = kStiffTol * kStabSize;  // fast form of the stiffness test
 
 enum : int8_t { RC_DEFAULT = 0, RC_SUCCESS = 1, RC_MAXITERS = 2, RC_DTLESSTHANMIN = 3,
-                RC_UNSTABLE = 4, RC_DTNAN = 5, RC_INITIALFAILURE = 6 };
+                RC_UNSTABLE = 4, RC_DTNAN = 5, RC_INITIALFAILURE = 6,
+                // internal: the first pass of a two-phase AutoSwitch solve hands
+                // the trajectory to the full AutoSwitch pass (never returned)
+                RC_HANDOFF = 100 };
 
 // FP64 constant pinned to an SGPR pair at its point of use.  gfx9 VOP3
 // instructions take one scalar operand and no literal; an unpinned constant

@@ -17,9 +17,11 @@ namespace mideq {
 
 template <class M, bool MIX, int ALG, bool FAST>
 __global__ __launch_bounds__(256) void k_lane(LaneArgs a) {
+  const int64_t nq = lane_queue_len(a);
+  if ((int64_t)blockIdx.x * blockDim.x >= nq) return;  // whole block idle
   lane_stage_saveat(a);
   int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= a.ntraj) return;
+  if (k >= nq) return;
   int64_t i = lane_traj(a, k);
   if (i < 0) return;
   Lane<M, MIX, ALG, FAST> L;
@@ -33,6 +35,7 @@ __global__ __launch_bounds__(256) void k_persistent(LaneArgs a) {
   const int lane = threadIdx.x & 63;
   const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   lane_stage_saveat(a);
+  const unsigned long long nq = (unsigned long long)lane_queue_len(a);
   Lane<M, MIX, ALG, FAST> L;
   bool active = false;
   bool drained = false;
@@ -45,10 +48,10 @@ __global__ __launch_bounds__(256) void k_persistent(LaneArgs a) {
         unsigned long long base = 0;
         if (lane == leader) base = atomicAdd(a.queue, (unsigned long long)cnt);
         base = __shfl(base, leader);
-        if (base + (unsigned long long)cnt >= (unsigned long long)a.ntraj) drained = true;
+        if (base + (unsigned long long)cnt >= nq) drained = true;
         if (!active) {
           unsigned long long my = base + (unsigned long long)__popcll(need & lt_mask);
-          int64_t i = my < (unsigned long long)a.ntraj ? lane_traj(a, (int64_t)my) : -1;
+          int64_t i = my < nq ? lane_traj(a, (int64_t)my) : -1;
           if (i >= 0) {
             L.init(a, i);
             if (L.rc == RC_DEFAULT) {
@@ -111,6 +114,7 @@ static hipError_t launch_alg(const LaneArgs& a, int alg, int sched, int block, i
     case ALG_RODAS5: return launch_t<M, MIX, ALG_RODAS5, FAST>(a, sched, block, grid, s);
     case ALG_RODAS5P: return launch_t<M, MIX, ALG_RODAS5P, FAST>(a, sched, block, grid, s);
     case ALG_AUTO5P: return launch_t<M, MIX, ALG_AUTO5P, FAST>(a, sched, block, grid, s);
+    case ALG_AUTO_T: return launch_t<M, MIX, ALG_AUTO_T, FAST>(a, sched, block, grid, s);
     default: return launch_t<M, MIX, ALG_AUTO, FAST>(a, sched, block, grid, s);
   }
 }
@@ -123,6 +127,11 @@ static hipError_t launch_model(const LaneArgs& a, int alg, int mix, int fast, in
                : launch_alg<M, false, true>(a, alg, sched, block, grid, s);
   return mix ? launch_alg<M, true, false>(a, alg, sched, block, grid, s)
              : launch_alg<M, false, false>(a, alg, sched, block, grid, s);
+}
+
+hipError_t launch_set_u64(unsigned long long* p, unsigned long long v, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, s, p, v);
+  return hipGetLastError();
 }
 
 hipError_t launch_lane_solver(const LaneArgs& a, int model, int alg, int mix, int fast,

@@ -13,8 +13,13 @@
 
 namespace mideq {
 
-// AUTO: AutoTsit5(Rodas5()); AUTO5P: AutoTsit5(Rodas5P())
-enum { ALG_TSIT5 = 0, ALG_RODAS5 = 1, ALG_AUTO = 2, ALG_RODAS5P = 3, ALG_AUTO5P = 4 };
+// AUTO: AutoTsit5(Rodas5()); AUTO5P: AutoTsit5(Rodas5P()).  AUTO_T (internal):
+// first pass of a two-pass AutoSwitch solve -- Tsit5 with the stiffness
+// test, handing every trajectory that would switch (or starts stiff) to a
+// second, full AutoSwitch pass; a trajectory that never switches takes the
+// same steps either way, so results equal the single-pass solve.
+enum { ALG_TSIT5 = 0, ALG_RODAS5 = 1, ALG_AUTO = 2, ALG_RODAS5P = 3, ALG_AUTO5P = 4,
+       ALG_AUTO_T = 5 };
 
 struct LaneArgs {
   int64_t ntraj;
@@ -44,13 +49,39 @@ struct LaneArgs {
   double* dt_first;
   unsigned long long* queue;  // persistent schedule: next position in the order
   const int64_t* order;       // launch order (NULL: natural order)
+  const unsigned long long* order_len;  // device length of `order` (NULL: ntraj)
+  int64_t* handoff;           // two-phase AutoSwitch: trajectories for the second pass
+  unsigned long long* handoff_count;
 };
 
 // Trajectory scheduled at position k of the launch order; -1 for an entry
-// outside [0, ntraj) (skipped, so a bad order cannot address out of bounds).
+// outside [0, ntraj) (skipped, so a bad order cannot address out of bounds)
+// or past a device-side order length.
 __device__ __forceinline__ int64_t lane_traj(const LaneArgs& a, int64_t k) {
+  if (a.order_len && (uint64_t)k >= *a.order_len) return -1;
   int64_t i = a.order ? a.order[k] : k;
   return (uint64_t)i < (uint64_t)a.ntraj ? i : -1;
+}
+
+// Number of launch-order positions to walk: ntraj, or the device-side order
+// length when one is given (the second pass of a two-phase AutoSwitch solve).
+__device__ __forceinline__ int64_t lane_queue_len(const LaneArgs& a) {
+  if (!a.order_len) return a.ntraj;
+  unsigned long long n = *a.order_len;
+  return n < (unsigned long long)a.ntraj ? (int64_t)n : a.ntraj;
+}
+
+// Append trajectory i to the hand-off list when `mine`: one atomic per
+// wavefront (over the lanes active here), lanes placed by ballot prefix.
+__device__ __forceinline__ void handoff_append(const LaneArgs& a, int64_t i, bool mine) {
+  unsigned long long m = __ballot(mine);
+  if (m == 0) return;
+  const int lane = threadIdx.x & 63;
+  const int leader = __builtin_ctzll(m);
+  unsigned long long base = 0;
+  if (lane == leader) base = atomicAdd(a.handoff_count, (unsigned long long)__popcll(m));
+  base = __shfl(base, leader);
+  if (mine) a.handoff[base + __popcll(m & ((1ull << lane) - 1ull))] = i;
 }
 
 // saveat times are staged in LDS by the lane kernels when they fit, so the
@@ -194,9 +225,10 @@ template <class M, bool MIX, int ALG, bool FAST>
 struct Lane {
   static constexpr int N = M::N;
   static constexpr int NP = M::NP;
-  static constexpr bool kAuto = (ALG == ALG_AUTO || ALG == ALG_AUTO5P);
+  static constexpr bool kAuto = (ALG == ALG_AUTO || ALG == ALG_AUTO5P || ALG == ALG_AUTO_T);
   static constexpr bool kHasTsit5 = (ALG != ALG_RODAS5 && ALG != ALG_RODAS5P);
-  static constexpr bool kHasRodas = (ALG != ALG_TSIT5);
+  static constexpr bool kHasRodas = (ALG != ALG_TSIT5 && ALG != ALG_AUTO_T);
+  static constexpr bool kHandoff = (ALG == ALG_AUTO_T);
   static constexpr bool kP = (ALG == ALG_RODAS5P || ALG == ALG_AUTO5P);  // Rodas5P tableau
 
   int64_t i;
@@ -316,6 +348,10 @@ struct Lane {
     rc = RC_DEFAULT;
     mode = (ALG == ALG_RODAS5 || ALG == ALG_RODAS5P) ? 1 : 0;
     if (kAuto && a.alg_hint) mode = a.alg_hint[i] ? 1 : 0;
+    if (kHandoff && mode == 1) {
+      rc = RC_HANDOFF;  // starts stiff: the full AutoSwitch pass solves it
+      return;
+    }
     qsteady_max = (a.qsteady_default && mode == 1) ? 1.2 : 1.0;
     // saveat points before t0 are unreachable; a point at t0 is u0 itself
     isave = 0;
@@ -689,6 +725,10 @@ struct Lane {
       }
       count = stiff ? (count < 0 ? 1 : count + 1) : (count > 0 ? -1 : count - 1);
       if (mode == 0 && count > kMaxStiff) {
+        if constexpr (kHandoff) {
+          rc = RC_HANDOFF;  // would switch to the Rosenbrock method
+          return;
+        }
         dt = dt * kDtFac;
         mode = 1;
         qsteady_max = a.qsteady_default ? 1.2 : 1.0;
@@ -866,6 +906,10 @@ struct Lane {
 
   // ------------------------------------------------------------ finish
   __device__ void finish(const LaneArgs& a) {
+    if constexpr (kHandoff) {
+      handoff_append(a, i, rc == RC_HANDOFF);
+      if (rc == RC_HANDOFF) return;  // the second pass writes every output
+    }
     const int64_t NT = a.ntraj;
     if (a.us) {
       double nanv[N];

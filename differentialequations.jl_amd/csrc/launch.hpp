@@ -16,5 +16,7 @@ enum { SCHED_AUTO = 0, SCHED_LANE = 1, SCHED_PERSISTENT = 2 };
 hipError_t launch_lane_solver(const LaneArgs& a, int model, int alg, int mix, int fast,
                               int sched, int block, int grid, hipStream_t s);
 int lane_model_dims(int model, int* n, int* np);
+// *p = v by a one-thread kernel (no memset node inside a captured graph).
+hipError_t launch_set_u64(unsigned long long* p, unsigned long long v, hipStream_t s);
 
 }  // namespace mideq

@@ -32,7 +32,7 @@ struct DevBuf {
 enum {
   BUF_U0 = 0, BUF_P, BUF_T0, BUF_TF, BUF_NVEC, BUF_SAVEAT, BUF_DT0, BUF_HINT, BUF_US,
   BUF_UF, BUF_TFIN, BUF_RC, BUF_NA, BUF_NR, BUF_NF, BUF_NS, BUF_SCRATCH, BUF_ORDER,
-  BUF_SAMPLE, BUF_DTF, BUF_COUNT
+  BUF_SAMPLE, BUF_DTF, BUF_HANDOFF, BUF_COUNT
 };
 
 struct GraphCache {
@@ -205,6 +205,7 @@ static int validate(mideq_ctx* c, const mideq_problem* P, const mideq_options* O
   if (O->save_dtype < 0 || O->save_dtype > 1) return set_err(c, MIDEQ_E_ARG, "bad save_dtype");
   if (O->schedule < 0 || O->schedule > 2) return set_err(c, MIDEQ_E_ARG, "bad schedule");
   if (O->controller < 0 || O->controller > 1) return set_err(c, MIDEQ_E_ARG, "bad controller");
+  if (O->auto_passes < 0 || O->auto_passes > 2) return set_err(c, MIDEQ_E_ARG, "bad auto_passes");
   if (O->block != 0 && (O->block < 64 || O->block > 256 || (O->block & 63)))
     return set_err(c, MIDEQ_E_ARG, "block must be 64..256, multiple of 64");
   return MIDEQ_OK;
@@ -426,13 +427,46 @@ static int run_solver(mideq_ctx* c, const mideq_problem* P, const mideq_options*
   } else {
     grid = (int)((P->ntraj + block - 1) / block);
   }
+  // AutoSwitch in two passes: a Tsit5 pass with the stiffness test solves
+  // every trajectory that never switches and lists the rest, which the full
+  // AutoSwitch kernel then solves from the start.  The Tsit5-only kernel is
+  // leaner (no Rosenbrock code or registers), and the second pass runs only
+  // the switching trajectories side by side; results equal the single pass.
+  // Opt-in: the Tsit5 pass re-solves the non-stiff start of every switching
+  // trajectory, so it pays only when few trajectories switch.
+  const bool two_pass = (O->alg == MIDEQ_ALG_AUTO || O->alg == MIDEQ_ALG_AUTO_RODAS5P) &&
+                        O->auto_passes == 2;
+  if (two_pass) {
+    void* d;
+    int rc0 = ensure_buf(c, BUF_HANDOFF, sizeof(int64_t) * (size_t)P->ntraj, &d);
+    if (rc0) return rc0;
+    a.handoff = (int64_t*)d;
+    a.handoff_count = c->d_queue + 1;  // d_queue[0] is the persistent queue
+  }
   HIP_TRY(c, hipEventRecord(c->ev0, s));
   struct Key {
     LaneArgs a;
-    int model, alg, prec, ctl, sched, block, grid;
-  } key{a, P->model, O->alg, O->precision, O->controller, sched, block, grid};
+    int model, alg, prec, ctl, sched, block, grid, two_pass;
+  } key{a, P->model, O->alg, O->precision, O->controller, sched, block, grid, two_pass};
   int rc = enqueue(c, O->graph != 0, &key, sizeof key, s, [&](hipStream_t st) {
-    return launch_lane_solver(a, P->model, O->alg, O->precision, O->controller, sched, block,
+    if (!two_pass) {
+      LaneArgs a1 = a;
+      a1.handoff = nullptr;
+      a1.handoff_count = nullptr;
+      return launch_lane_solver(a1, P->model, O->alg, O->precision, O->controller, sched,
+                                block, grid, st);
+    }
+    hipError_t e = launch_set_u64(a.handoff_count, 0ull, st);
+    if (e != hipSuccess) return e;
+    e = launch_lane_solver(a, P->model, ALG_AUTO_T, O->precision, O->controller, sched, block,
+                           grid, st);
+    if (e != hipSuccess) return e;
+    LaneArgs a2 = a;
+    a2.order = a.handoff;
+    a2.order_len = a.handoff_count;
+    a2.handoff = nullptr;
+    a2.handoff_count = nullptr;
+    return launch_lane_solver(a2, P->model, O->alg, O->precision, O->controller, sched, block,
                               grid, st);
   });
   if (rc) return rc;
@@ -441,7 +475,7 @@ static int run_solver(mideq_ctx* c, const mideq_problem* P, const mideq_options*
   c->stats.schedule = sched;
   c->stats.block = block;
   c->stats.grid = grid;
-  c->stats.launches = sched == SCHED_PERSISTENT ? 2 : 1;
+  c->stats.launches = (sched == SCHED_PERSISTENT ? 2 : 1) * (two_pass ? 2 : 1) + (two_pass ? 1 : 0);
   c->stats.lanes = 1;
   return MIDEQ_OK;
 }

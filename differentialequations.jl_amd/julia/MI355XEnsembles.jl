@@ -59,7 +59,7 @@ struct MideqOptions
     saveat::Ptr{Float64}; ns::Int32; save_dtype::Int32
     dt0::Ptr{Float64}; alg_hint::Ptr{Int8}
     qsteady_default::Int32; schedule::Int32; block::Int32; blocks_per_cu::Int32
-    lanes::Int32; graph::Int32; controller::Int32; reserved1::Int32
+    lanes::Int32; graph::Int32; controller::Int32; auto_passes::Int32
     order::Ptr{Int64}
 end
 

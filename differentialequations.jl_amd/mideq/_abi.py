@@ -41,7 +41,7 @@ class Options(C.Structure):
                 ("alg_hint", C.c_void_p), ("qsteady_default", C.c_int32),
                 ("schedule", C.c_int32), ("block", C.c_int32), ("blocks_per_cu", C.c_int32),
                 ("lanes", C.c_int32), ("graph", C.c_int32), ("controller", C.c_int32),
-                ("reserved1", C.c_int32), ("order", C.c_void_p)]
+                ("auto_passes", C.c_int32), ("order", C.c_void_p)]
 
 
 class Outputs(C.Structure):

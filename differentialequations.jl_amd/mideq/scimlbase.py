@@ -191,7 +191,10 @@ class EnsembleMI355X:
     (e.g. from mideq.learned).  order: optional launch order, a permutation
     of the trajectory indices (0-based) -- consecutive entries share a
     wavefront, so grouping trajectories of similar cost cuts lane
-    divergence; results keep their natural positions.  device: GPU ordinal."""
+    divergence; results keep their natural positions.  device: GPU ordinal.
+    auto_passes (AutoTsit5 on n <= 3 models): 0/1 = one AutoSwitch pass;
+    2 = a Tsit5 pass, then the full AutoSwitch kernel over the trajectories
+    that switch (faster when few switch).  Both give identical results."""
     precision: str = "fp64"
     schedule: str = "auto"
     device: int = 0
@@ -202,6 +205,7 @@ class EnsembleMI355X:
     lanes: int = 0
     graph: bool = False
     controller: str = "reference"   # or "fast" (n <= 3 models)
+    auto_passes: int = 0
 
 
 # ---------------------------------------------------------------- solutions
@@ -367,6 +371,7 @@ def device_solve(f, u0, p, t0, tf, nvec, alg, ealg, saveat=None, abstol=1e-6, re
     O.lanes = ealg.lanes
     O.graph = 1 if ealg.graph else 0
     O.controller = dict(reference=0, fast=1)[ealg.controller]
+    O.auto_passes = int(ealg.auto_passes)
     f32 = ealg.save_dtype == "float32"
     O.save_dtype = 1 if f32 else 0
     us = None

@@ -127,7 +127,14 @@ typedef struct {
   int32_t controller;   /* 0: PI controller as OrdinaryDiffEq states it; 1: same PI law
                            evaluated on EEst^2 in the log2 domain without divisions
                            (n <= 3 models) */
-  int32_t reserved1;
+  int32_t auto_passes;  /* MIDEQ_ALG_AUTO / AUTO_RODAS5P on n <= 3 models: 0 or 1 = one
+                           AutoSwitch pass; 2 = two passes (a Tsit5 pass with the
+                           stiffness test solves every trajectory that never switches,
+                           then the full AutoSwitch kernel solves the rest from t0).
+                           Results are identical.  Two passes pay when a minority of
+                           the batch switches and the schedule is lane (Van der Pol,
+                           mu in [1, 1e4], 78% switching: 5.7 -> 5.0 ms); they cost
+                           when all switch.  Ignored elsewhere. */
   const int64_t *order; /* [ntraj] launch order or NULL (natural order).  A permutation
                            of 0..ntraj-1: entry k is the k-th trajectory scheduled, so
                            consecutive entries share a wavefront -- grouping trajectories
