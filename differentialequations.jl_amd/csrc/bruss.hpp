@@ -50,12 +50,20 @@ inline int64_t bruss_scratch_per_team(int nmax) {
   return (int64_t)nmax * nmax + 2 * (int64_t)nmax;
 }
 
-// Resident teams of the persistent grid for (lanes, alg, precision,
+// Resident teams of the persistent grid for (team shape, alg, precision,
 // controller): grid blocks and teams per block.  Returns hipSuccess or an
-// error.  fast: the division-free log2-domain PI controller.
-hipError_t bruss_plan(int lanes, int alg, int mix, int fast, int ncu, int64_t ntraj, int* grid,
-                      int* teams_per_block);
-hipError_t launch_bruss_solver(const BrussArgs& a, int lanes, int alg, int mix, int fast,
+// error.  Team shapes (lanes, nk): (64, 128), (32, 128) and (32, 64), nk the
+// largest state size a team holds.  fast: the division-free log2-domain PI
+// controller.
+hipError_t bruss_plan(int lanes, int nk, int alg, int mix, int fast, int ncu, int64_t ntraj,
+                      int* grid, int* teams_per_block);
+// out[k] = order[(k * ntraj) / S], k < S (natural order for a NULL order):
+// the autotuner's sample, strided over the launch order.
+hipError_t bruss_sample_order(int64_t* out, const int64_t* order, int64_t ntraj, int64_t S,
+                              hipStream_t s);
+// Launches the solver over the queue of `a`.  Every queued instance must
+// have n <= nk (a larger one finishes with RC_INITIALFAILURE).
+hipError_t launch_bruss_solver(const BrussArgs& a, int lanes, int nk, int alg, int mix, int fast,
                                int grid, hipStream_t s);
 
 }  // namespace mideq

@@ -251,7 +251,9 @@ struct BTeam {
     mode = (ALG == ALG_RODAS5 || ALG == ALG_RODAS5P) ? 1 : 0;
     if (kAuto && a.alg_hint) mode = a.alg_hint[i] ? 1 : 0;
     qsteady_max = (a.qsteady_default && mode == 1) ? 1.2 : 1.0;
-    if (n < BRUSS_NMIN || n > a.nmax || (n & 1)) {
+    // n > L * C: more components than the team holds (the host routes only
+    // instances that fit; this keeps a wrong routing in bounds)
+    if (n < BRUSS_NMIN || n > a.nmax || n > L * C || (n & 1)) {
       rc = RC_INITIALFAILURE;
       n = 0;
       return;
@@ -902,14 +904,16 @@ struct BTeam {
   }
 };
 
-template <int L, bool MIX, int ALG, bool FAST>
+// NK: largest state size the instantiation holds (C = NK / L slots per
+// lane); every instance it is given must have n <= NK.
+template <int L, int NK, bool MIX, int ALG, bool FAST>
 __global__ __launch_bounds__(256) void k_bruss(BrussArgs a) {
-  constexpr int C = BRUSS_NMAX / L;
+  constexpr int C = NK / L;
   constexpr int TPB = 256 / L;
-  __shared__ double s_d[TPB][BRUSS_NMAX];
-  __shared__ double s_r[TPB][BRUSS_NMAX];
-  __shared__ float s_f[TPB][BRUSS_NMAX];
-  __shared__ int s_i[TPB][BRUSS_NMAX];
+  __shared__ double s_d[TPB][NK];
+  __shared__ double s_r[TPB][NK];
+  __shared__ float s_f[TPB][NK];
+  __shared__ int s_i[TPB][NK];
   const int team = threadIdx.x / L;
   BTeam<L, C, MIX, ALG, FAST> T;
   T.tl = threadIdx.x % L;
@@ -952,13 +956,24 @@ __global__ __launch_bounds__(256) void k_bruss(BrussArgs a) {
 
 __global__ void k_bruss_reset(unsigned long long* q) { *q = 0ull; }
 
-// One instantiation per (lanes, precision, algorithm, controller).
+// out[k] = order[(k * ntraj) / S] (natural order when order is NULL): a
+// strided sample of the launch order, for the autotuner.
+__global__ void k_sample_order(int64_t* out, const int64_t* order, int64_t ntraj, int64_t S) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= S) return;
+  // floor(k ntraj / S) = k a + floor(k r / S) with ntraj = a S + r (no overflow)
+  const int64_t a = ntraj / S, r = ntraj % S;
+  int64_t q = k * a + (k * r) / S;
+  out[k] = order ? order[q] : q;
+}
+
+// One instantiation per (team shape, precision, algorithm, controller).
 struct OccupancyOp {
   int* per_cu;
-  template <int L, bool MIX, int ALG, bool FAST>
+  template <int L, int NK, bool MIX, int ALG, bool FAST>
   hipError_t run() const {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_bruss<L, MIX, ALG, FAST>, 256,
-                                                        0);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_bruss<L, NK, MIX, ALG, FAST>,
+                                                        256, 0);
   }
 };
 
@@ -966,41 +981,45 @@ struct LaunchOp {
   const BrussArgs* a;
   int grid;
   hipStream_t s;
-  template <int L, bool MIX, int ALG, bool FAST>
+  template <int L, int NK, bool MIX, int ALG, bool FAST>
   hipError_t run() const {
-    hipLaunchKernelGGL((k_bruss<L, MIX, ALG, FAST>), dim3(grid), dim3(256), 0, s, *a);
+    hipLaunchKernelGGL((k_bruss<L, NK, MIX, ALG, FAST>), dim3(grid), dim3(256), 0, s, *a);
     return hipGetLastError();
   }
 };
 
-template <int L, bool MIX, bool FAST, class Op>
+template <int L, int NK, bool MIX, bool FAST, class Op>
 static hipError_t by_alg(int alg, const Op& op) {
   switch (alg) {
-    case ALG_TSIT5: return op.template run<L, MIX, ALG_TSIT5, FAST>();
-    case ALG_RODAS5: return op.template run<L, MIX, ALG_RODAS5, FAST>();
-    case ALG_RODAS5P: return op.template run<L, MIX, ALG_RODAS5P, FAST>();
-    case ALG_AUTO5P: return op.template run<L, MIX, ALG_AUTO5P, FAST>();
-    default: return op.template run<L, MIX, ALG_AUTO, FAST>();
+    case ALG_TSIT5: return op.template run<L, NK, MIX, ALG_TSIT5, FAST>();
+    case ALG_RODAS5: return op.template run<L, NK, MIX, ALG_RODAS5, FAST>();
+    case ALG_RODAS5P: return op.template run<L, NK, MIX, ALG_RODAS5P, FAST>();
+    case ALG_AUTO5P: return op.template run<L, NK, MIX, ALG_AUTO5P, FAST>();
+    default: return op.template run<L, NK, MIX, ALG_AUTO, FAST>();
   }
 }
 
+template <int L, int NK, class Op>
+static hipError_t by_prec(int alg, int mix, int fast, const Op& op) {
+  if (mix) return fast ? by_alg<L, NK, true, true>(alg, op) : by_alg<L, NK, true, false>(alg, op);
+  return fast ? by_alg<L, NK, false, true>(alg, op) : by_alg<L, NK, false, false>(alg, op);
+}
+
+// Team shapes: 64 or 32 lanes holding up to 128 components, or 32 lanes
+// holding up to 64 (two instances per wavefront, one (u, v) pair per lane:
+// the registers of the 64-lane shape for batches with n <= 64).
 template <class Op>
-static hipError_t dispatch(int lanes, int alg, int mix, int fast, const Op& op) {
-  if (lanes == 64) {
-    if (mix) return fast ? by_alg<64, true, true>(alg, op) : by_alg<64, true, false>(alg, op);
-    return fast ? by_alg<64, false, true>(alg, op) : by_alg<64, false, false>(alg, op);
-  }
-  if (lanes == 32) {
-    if (mix) return fast ? by_alg<32, true, true>(alg, op) : by_alg<32, true, false>(alg, op);
-    return fast ? by_alg<32, false, true>(alg, op) : by_alg<32, false, false>(alg, op);
-  }
+static hipError_t dispatch(int lanes, int nk, int alg, int mix, int fast, const Op& op) {
+  if (lanes == 64 && nk == BRUSS_NMAX) return by_prec<64, BRUSS_NMAX>(alg, mix, fast, op);
+  if (lanes == 32 && nk == BRUSS_NMAX) return by_prec<32, BRUSS_NMAX>(alg, mix, fast, op);
+  if (lanes == 32 && nk == BRUSS_NMAX / 2) return by_prec<32, BRUSS_NMAX / 2>(alg, mix, fast, op);
   return hipErrorInvalidValue;
 }
 
-hipError_t bruss_plan(int lanes, int alg, int mix, int fast, int ncu, int64_t ntraj, int* grid,
-                      int* teams_per_block) {
+hipError_t bruss_plan(int lanes, int nk, int alg, int mix, int fast, int ncu, int64_t ntraj,
+                      int* grid, int* teams_per_block) {
   int per_cu = 0;
-  hipError_t e = dispatch(lanes, alg, mix, fast, OccupancyOp{&per_cu});
+  hipError_t e = dispatch(lanes, nk, alg, mix, fast, OccupancyOp{&per_cu});
   if (e != hipSuccess) return e;
   if (per_cu < 1) per_cu = 1;
   int tpb = 256 / lanes;
@@ -1012,11 +1031,19 @@ hipError_t bruss_plan(int lanes, int alg, int mix, int fast, int ncu, int64_t nt
   return hipSuccess;
 }
 
-hipError_t launch_bruss_solver(const BrussArgs& a, int lanes, int alg, int mix, int fast,
+hipError_t bruss_sample_order(int64_t* out, const int64_t* order, int64_t ntraj, int64_t S,
+                              hipStream_t s) {
+  if (S <= 0 || S > ntraj || S > (1 << 24)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_sample_order, dim3((unsigned)((S + 255) / 256)), dim3(256), 0, s, out,
+                     order, ntraj, S);
+  return hipGetLastError();
+}
+
+hipError_t launch_bruss_solver(const BrussArgs& a, int lanes, int nk, int alg, int mix, int fast,
                                int grid, hipStream_t s) {
   if (lanes != 32 && lanes != 64) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_bruss_reset, dim3(1), dim3(1), 0, s, a.queue);
-  return dispatch(lanes, alg, mix, fast, LaunchOp{&a, grid, s});
+  return dispatch(lanes, nk, alg, mix, fast, LaunchOp{&a, grid, s});
 }
 
 }  // namespace mideq

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -256,10 +257,17 @@ static int enqueue(mideq_ctx* c, bool use_graph, const void* key, size_t keylen,
 }
 
 // ---------------------------------------------------------------- brusselator
+// Team shape for a lanes choice: 32-lane teams hold (u, v) pairs for n <= 64
+// in the registers of the 64-lane shape when the batch allows.
+static int bruss_nk(int lanes, int nmax) {
+  return (lanes == 32 && nmax <= BRUSS_NMAX / 2) ? BRUSS_NMAX / 2 : BRUSS_NMAX;
+}
+
 static int bruss_launch_with(mideq_ctx* c, BrussArgs b, int lanes, int alg, int mix, int fast,
                              bool use_graph, hipStream_t s, int* grid_used) {
   int grid = 0, tpb = 0;
-  hipError_t e = bruss_plan(lanes, alg, mix, fast, c->ncu, b.nqueue, &grid, &tpb);
+  const int nk = bruss_nk(lanes, b.nmax);
+  hipError_t e = bruss_plan(lanes, nk, alg, mix, fast, c->ncu, b.nqueue, &grid, &tpb);
   if (e != hipSuccess) return set_err(c, MIDEQ_E_HIP, "bruss_plan: %s", hipGetErrorString(e));
   if (alg != MIDEQ_ALG_TSIT5) {
     b.scratch_stride = bruss_scratch_per_team(b.nmax);
@@ -278,7 +286,7 @@ static int bruss_launch_with(mideq_ctx* c, BrussArgs b, int lanes, int alg, int 
     int lanes, alg, mix, fast, grid;
   } key{b, lanes, alg, mix, fast, grid};
   return enqueue(c, use_graph, &key, sizeof key, s, [&](hipStream_t st) {
-    return launch_bruss_solver(b, lanes, alg, mix, fast, grid, st);
+    return launch_bruss_solver(b, lanes, nk, alg, mix, fast, grid, st);
   });
 }
 
@@ -324,14 +332,31 @@ static int run_bruss(mideq_ctx* c, const mideq_problem* P, const mideq_options* 
     if (it != c->tuned_lanes.end()) {
       lanes = it->second;
     } else {
-      // Rosenbrock steps on n up to 128 cost ~500 Tsit5 steps: a smaller sample
-      int64_t S = std::min<int64_t>(P->ntraj, O->alg == MIDEQ_ALG_TSIT5 ? 8192 : 1024);
-      std::vector<int64_t> sample(S);
-      for (int64_t k = 0; k < S; k++) sample[k] = (k * P->ntraj) / S;
+      // The sample is strided over the launch order and, for Tsit5, holds
+      // ~8 instances per resident team: with fewer, the run is its tail
+      // (the last instances' latency), which favours the shape with one
+      // instance per wavefront over the one with the higher throughput.
+      // Rosenbrock steps on n up to 128 cost ~500 Tsit5 steps: 1,024.
+      int64_t S = 1024;
+      if (O->alg == MIDEQ_ALG_TSIT5) {
+        int64_t resident = 0;
+        for (int cand : {64, 32}) {
+          int g = 0, tpb = 0;
+          if (bruss_plan(cand, bruss_nk(cand, P->n), O->alg, O->precision, O->controller,
+                         c->ncu, INT64_MAX / 2, &g, &tpb) == hipSuccess)
+            resident = std::max<int64_t>(resident, (int64_t)g * tpb);
+        }
+        S = std::max<int64_t>(8192, 8 * resident);
+      }
+      S = std::min<int64_t>(S, P->ntraj);
       void* d;
       int rc = ensure_buf(c, BUF_SAMPLE, sizeof(int64_t) * S, &d);
       if (rc) return rc;
-      HIP_TRY(c, hipMemcpyAsync(d, sample.data(), sizeof(int64_t) * S, hipMemcpyHostToDevice, s));
+      {
+        hipError_t e = bruss_sample_order((int64_t*)d, d_order, P->ntraj, S, s);
+        if (e != hipSuccess)
+          return set_err(c, MIDEQ_E_HIP, "sample order: %s", hipGetErrorString(e));
+      }
       BrussArgs bs = b;
       bs.order = (const int64_t*)d;
       bs.nqueue = S;

@@ -105,3 +105,25 @@ def test_bruss_fast_controller_bitexact(oracle, alg, lanes, precision):
     assert (c["retcode"] == 1).all()
     assert_identical(g, c, nvec=w["nvec"])
     np.testing.assert_array_equal(g.nstiff, c["nstiff"])
+
+
+@pytest.mark.parametrize("alg,precision,controller", [("tsit5", "fp64", "reference"),
+                                                      ("tsit5", "mixed", "fast"),
+                                                      ("auto", "fp64", "reference")])
+def test_bruss_narrow_batch_32_lane_shape(oracle, alg, precision, controller):
+    """A batch whose layout size is n <= 64 runs the 32-lane teams with one
+    (u, v) pair per lane (two instances per wavefront); same results as the
+    oracle with the 32-lane reduction order."""
+    N = 200 if alg == "tsit5" else 40
+    say(f"brusselator n<=64 layout, 32-lane shape, {alg} {precision} {controller} N={N}")
+    w = batch(N, seed=60, nmax_grid=32, tf_max=3.0 if alg == "tsit5" else 1.0)
+    w["u0"] = np.ascontiguousarray(w["u0"][:64])
+    assert w["nvec"].max() <= 64
+    tol = dict(abstol=1e-8, reltol=1e-6)
+    g = gpu("brusselator", w["u0"], w["p"], 0.0, w["tf"], nvec=w["nvec"], lanes=32, alg=alg,
+            precision=precision, controller=controller, schedule="persistent", **tol)
+    c = cpu(oracle, "brusselator", w["u0"], w["p"], 0.0, w["tf"], nvec=w["nvec"], norm_lanes=32,
+            alg=alg, precision=precision, controller=1 if controller == "fast" else 0, **tol)
+    assert (c["retcode"] == 1).all()
+    assert_identical(g, c, nvec=w["nvec"])
+    np.testing.assert_array_equal(g.nstiff, c["nstiff"])
