@@ -77,6 +77,14 @@ constexpr double C[8][7] = {
     {42.57076742291101, -13.80770672017997, 93.98938432427124,
      18.77919633714503, -31.58359187223370, -6.685968952921985,
      -5.810979938412932}};
+// Continuous extension (this library's derivation; see rd_H below).
+constexpr double H[3][7] = {
+    {46.335061333323615, -6.136914790748748, 108.40712618588684, -15.432459674306823,
+     -79.5099979954734, -1.9050165705652515, -2.7727981784318065},
+    {-39.14106358101113, -3.4608988577808986, -157.10746284880244, 34.66137042565768,
+     146.53644774500066, 4.746961424327032, 8.03303264503124},
+    {35.36683737689737, 3.300351318514294, 162.0764418691076, -9.977463628946106,
+     -129.9992347337098, -8.753639789000294, -10.993114152187328}};
 }  // namespace rd
 
 // ---------------------------------------------------------------------------
@@ -105,6 +113,13 @@ constexpr double C[8][7] = {
      -19.097331116725623, -28.087943162872662, 0},
     {37.80277123390563, -3.2571969029072276, 112.26918849496327, 66.9347231244047,
      -40.06618937091002, -54.66780262877968, -9.48861652309627}};
+constexpr double H[3][7] = {
+    {18.16834780539935, -2.4039764856508183, -28.49033949791586, -39.2353239248467,
+     24.203612973337464, 31.138884328789313, 5.813992390809251},
+    {20.204639179440537, -1.56515746765803, 153.73045429293606, 118.22912892648397,
+     -71.49081351328653, -100.21520495678435, -27.642530934779916},
+    {-15.086356891234155, 3.4125953268012186, -59.68473410632594, -45.47407592791441,
+     28.595178703267447, 42.46386356049424, 18.195502534440195}};
 }  // namespace rd5p
 
 // Rosenbrock tableau selected at compile time (P: Rodas5P, else Rodas5).
@@ -117,6 +132,19 @@ __device__ __forceinline__ constexpr double rd_A(int s, int j) {
 template <bool P>
 __device__ __forceinline__ constexpr double rd_C(int s, int j) {
   return P ? rd5p::C[s][j] : rd::C[s][j];
+}
+// Dense output of a Rosenbrock step (saveat inside a Rodas5/Rodas5P step):
+//   u(th) = (1-th) uprev + th (u1 + (1-th)(K1 + th (K2 + th K3))),
+//   K_m = sum_{j<7} H[m][j] k_j
+// with k_j the stage increments.  H is the unique solution, for each m, of
+// (a) the eight Rosenbrock order conditions up to order 4 holding for every
+// th (so the interpolant is locally 4th order), (b) the stiff limit
+// u(th)/uprev -> (1-th)^4 for y' = lambda y as lambda dt -> -inf, and (c) no
+// weight on the error stage k_8.  (a) has rank 6 for these tableaus;
+// (a)+(b)+(c) is a consistent 9 x 7 system of rank 7.
+template <bool P>
+__device__ __forceinline__ constexpr double rd_H(int m, int j) {
+  return P ? rd5p::H[m][j] : rd::H[m][j];
 }
 
 // ---------------------------------------------------------------------------

@@ -109,13 +109,23 @@ static hipError_t launch_t(const LaneArgs& a, int sched, int block, int grid,
 template <class M, bool MIX, bool FAST>
 static hipError_t launch_alg(const LaneArgs& a, int alg, int sched, int block, int grid,
                              hipStream_t s) {
+  // saveat with a Rosenbrock method: the build with its dense output
+  const bool dense = a.ns > 0 && a.us;
   switch (alg) {
     case ALG_TSIT5: return launch_t<M, MIX, ALG_TSIT5, FAST>(a, sched, block, grid, s);
-    case ALG_RODAS5: return launch_t<M, MIX, ALG_RODAS5, FAST>(a, sched, block, grid, s);
-    case ALG_RODAS5P: return launch_t<M, MIX, ALG_RODAS5P, FAST>(a, sched, block, grid, s);
-    case ALG_AUTO5P: return launch_t<M, MIX, ALG_AUTO5P, FAST>(a, sched, block, grid, s);
+    case ALG_RODAS5:
+      return dense ? launch_t<M, MIX, ALG_RODAS5 | ALG_DENSE, FAST>(a, sched, block, grid, s)
+                   : launch_t<M, MIX, ALG_RODAS5, FAST>(a, sched, block, grid, s);
+    case ALG_RODAS5P:
+      return dense ? launch_t<M, MIX, ALG_RODAS5P | ALG_DENSE, FAST>(a, sched, block, grid, s)
+                   : launch_t<M, MIX, ALG_RODAS5P, FAST>(a, sched, block, grid, s);
+    case ALG_AUTO5P:
+      return dense ? launch_t<M, MIX, ALG_AUTO5P | ALG_DENSE, FAST>(a, sched, block, grid, s)
+                   : launch_t<M, MIX, ALG_AUTO5P, FAST>(a, sched, block, grid, s);
     case ALG_AUTO_T: return launch_t<M, MIX, ALG_AUTO_T, FAST>(a, sched, block, grid, s);
-    default: return launch_t<M, MIX, ALG_AUTO, FAST>(a, sched, block, grid, s);
+    default:
+      return dense ? launch_t<M, MIX, ALG_AUTO | ALG_DENSE, FAST>(a, sched, block, grid, s)
+                   : launch_t<M, MIX, ALG_AUTO, FAST>(a, sched, block, grid, s);
   }
 }
 

@@ -20,6 +20,10 @@ namespace mideq {
 // same steps either way, so results equal the single-pass solve.
 enum { ALG_TSIT5 = 0, ALG_RODAS5 = 1, ALG_AUTO = 2, ALG_RODAS5P = 3, ALG_AUTO5P = 4,
        ALG_AUTO_T = 5 };
+// Template flag on a Rosenbrock-capable ALG: build the Rosenbrock dense
+// output for saveat.  It keeps the 8 stage increments live past the step,
+// so kernels without saveat are built without it.
+enum { ALG_DENSE = 16 };
 
 struct LaneArgs {
   int64_t ntraj;
@@ -225,11 +229,13 @@ template <class M, bool MIX, int ALG, bool FAST>
 struct Lane {
   static constexpr int N = M::N;
   static constexpr int NP = M::NP;
-  static constexpr bool kAuto = (ALG == ALG_AUTO || ALG == ALG_AUTO5P || ALG == ALG_AUTO_T);
-  static constexpr bool kHasTsit5 = (ALG != ALG_RODAS5 && ALG != ALG_RODAS5P);
-  static constexpr bool kHasRodas = (ALG != ALG_TSIT5 && ALG != ALG_AUTO_T);
-  static constexpr bool kHandoff = (ALG == ALG_AUTO_T);
-  static constexpr bool kP = (ALG == ALG_RODAS5P || ALG == ALG_AUTO5P);  // Rodas5P tableau
+  static constexpr int kAlg = ALG & (ALG_DENSE - 1);
+  static constexpr bool kDense = (ALG & ALG_DENSE) != 0;
+  static constexpr bool kAuto = (kAlg == ALG_AUTO || kAlg == ALG_AUTO5P || kAlg == ALG_AUTO_T);
+  static constexpr bool kHasTsit5 = (kAlg != ALG_RODAS5 && kAlg != ALG_RODAS5P);
+  static constexpr bool kHasRodas = (kAlg != ALG_TSIT5 && kAlg != ALG_AUTO_T);
+  static constexpr bool kHandoff = (kAlg == ALG_AUTO_T);
+  static constexpr bool kP = (kAlg == ALG_RODAS5P || kAlg == ALG_AUTO5P);  // Rodas5P tableau
 
   int64_t i;
   double p[NP];
@@ -346,7 +352,7 @@ struct Lane {
     eigd = 1.0;
     dtfirst = 0.0;
     rc = RC_DEFAULT;
-    mode = (ALG == ALG_RODAS5 || ALG == ALG_RODAS5P) ? 1 : 0;
+    mode = (kAlg == ALG_RODAS5 || kAlg == ALG_RODAS5P) ? 1 : 0;
     if (kAuto && a.alg_hint) mode = a.alg_hint[i] ? 1 : 0;
     if (kHandoff && mode == 1) {
       rc = RC_HANDOFF;  // starts stiff: the full AutoSwitch pass solves it
@@ -550,7 +556,7 @@ struct Lane {
   // W = J - I/(gamma dt), LU with partial pivoting in registers (row swaps
   // as selects so that every index stays static), 8 stage solves.
   __device__ bool rodas5_step(double dts, double* u1, double& EEst, double abstol,
-                              double reltol) {
+                              double reltol, double (&kd)[8][N]) {
     double W[N][N];
     M::jac(W, u, p);
     if (kAuto) {
@@ -612,7 +618,6 @@ struct Lane {
       EEst = 2.0;
       return false;
     }
-    double kd[8][N];
     double U[N], du[N], x[N];
     float Uf[N], duf[N];
 #pragma unroll
@@ -757,6 +762,7 @@ struct Lane {
     double EEst = 0.0;
     double k64[kHasTsit5 && !MIX ? 7 : 1][N];
     float k32[kHasTsit5 && MIX ? 7 : 1][N];
+    double kd[8][N];  // Rosenbrock stage increments (dense output)
     bool tsit_step = kHasTsit5 && (!kHasRodas || mode == 0);
     if (tsit_step) {
       if constexpr (kHasTsit5) {
@@ -768,7 +774,7 @@ struct Lane {
       nf += 6;
     } else {
       if constexpr (kHasRodas) {
-        if (!rodas5_step(dt, u1, EEst, a.abstol, a.reltol)) EEst = 2.0;
+        if (!rodas5_step(dt, u1, EEst, a.abstol, a.reltol, kd)) EEst = 2.0;
       }
       nf += 8;
     }
@@ -865,9 +871,29 @@ struct Lane {
                   val[c] = fma(dt, (double)acc, u[c]);
                 }
               }
-            } else {
+            } else if constexpr (kHasRodas && !kDense) {
+              // not reached: saveat runs the kDense build
 #pragma unroll
               for (int c = 0; c < N; c++) val[c] = __builtin_nan("");
+            } else if constexpr (kHasRodas) {
+              // Rosenbrock dense output (rd_H), FP64 in both precisions
+              double th = (sa - tprev) / dt;
+              double th1 = 1.0 - th;
+#pragma unroll
+              for (int c = 0; c < N; c++) {
+                double K[3];
+#pragma unroll
+                for (int m = 0; m < 3; m++) {
+                  double acc = rd_H<kP>(m, 0) * kd[0][c];
+#pragma unroll
+                  for (int j = 1; j < 7; j++) acc = fma(rd_H<kP>(m, j), kd[j][c], acc);
+                  K[m] = acc;
+                }
+                double inner = fma(th, K[2], K[1]);
+                inner = fma(th, inner, K[0]);
+                double tt = fma(th1, inner, u1[c]);
+                val[c] = fma(th, tt, th1 * u[c]);
+              }
             }
             store_save(a, isave, val);
           }

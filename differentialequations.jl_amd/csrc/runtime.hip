@@ -201,8 +201,6 @@ static int validate(mideq_ctx* c, const mideq_problem* P, const mideq_options* O
   if (!(O->abstol > 0) || !(O->reltol > 0))
     return set_err(c, MIDEQ_E_ARG, "abstol and reltol must be > 0");
   if (O->ns < 0 || (O->ns > 0 && !O->saveat)) return set_err(c, MIDEQ_E_ARG, "bad saveat");
-  if (O->ns > 0 && O->alg != MIDEQ_ALG_TSIT5)
-    return set_err(c, MIDEQ_E_UNSUPPORTED, "saveat needs Tsit5 (Rodas5 dense output not provided)");
   if (O->save_dtype < 0 || O->save_dtype > 1) return set_err(c, MIDEQ_E_ARG, "bad save_dtype");
   if (O->schedule < 0 || O->schedule > 2) return set_err(c, MIDEQ_E_ARG, "bad schedule");
   if (O->controller < 0 || O->controller > 1) return set_err(c, MIDEQ_E_ARG, "bad controller");

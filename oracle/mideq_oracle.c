@@ -20,6 +20,12 @@
  *   ctl_log2/exp2/pow .................... pinned: math.log2 / 2**x bounds.
  *   initdt ............................... pinned: hand-derived d0,d1,d2
  *        for u'=1.01u (HNW II.4 algorithm).
+ *   Rosenbrock dense output (saveat) ..... pinned: its defining conditions
+ *        recomputed from the tableau (order 4 for every theta, stiff
+ *        limit), measured local order 4 vs DOP853, exact end points; its
+ *        values vs upstream's own Rodas interpolant: parity unpinned (the
+ *        upstream H table is not vendored; this is the library's own
+ *        derivation).
  *   PI controller, tstop snap, saveat .... pinned: hand-computed single
  *        steps, 11-point saveat count and exact u0 copy (core.jl:34,95).
  *   AutoSwitch thresholds ................ parity unpinned (upstream
@@ -83,6 +89,13 @@ typedef struct {
   double C[8][7];
   double c[8];
   double d[8];
+  /* continuous extension (this library's derivation, not an upstream table):
+   * u(th) = (1-th) uprev + th (u1 + (1-th)(K1 + th (K2 + th K3))),
+   * K_m = sum_{j<7} H[m][j] k_j.  For each m, H solves the 8 Rosenbrock
+   * order conditions up to order 4 for every th (rank 6), the stiff limit
+   * u(th)/uprev -> (1-th)^4 for y' = lambda y, lambda dt -> -inf, and puts
+   * no weight on the error stage k_8: a consistent 9x7 system of rank 7. */
+  double H[3][7];
 } rdtab_t;
 
 static const rdtab_t RD5 = {
@@ -107,7 +120,13 @@ static const rdtab_t RD5 = {
       -31.58359187223370, -6.685968952921985, -5.810979938412932}},
     {0.0, 0.38, 0.3878509998321533, 0.4839718937873840, 0.4570477008819580, 1.0, 1.0, 1.0},
     {0.19, -0.1823079225333714636, -0.319231832186874912, 0.3449828624712811,
-     -0.377417564392089818, 0.0, 0.0, 0.0}};
+     -0.377417564392089818, 0.0, 0.0, 0.0},
+    {{46.335061333323615, -6.136914790748748, 108.40712618588684, -15.432459674306823,
+      -79.5099979954734, -1.9050165705652515, -2.7727981784318065},
+     {-39.14106358101113, -3.4608988577808986, -157.10746284880244, 34.66137042565768,
+      146.53644774500066, 4.746961424327032, 8.03303264503124},
+     {35.36683737689737, 3.300351318514294, 162.0764418691076, -9.977463628946106,
+      -129.9992347337098, -8.753639789000294, -10.993114152187328}}};
 
 /* Rodas5P (Steinebach, "Construction of Rosenbrock-Wanner method Rodas5P
  * and numerical benchmarks within the Julia Differential Equations
@@ -137,7 +156,13 @@ static const rdtab_t RD5P = {
     {0.0, 0.6358126895828704, 0.4095798393397535, 0.9769306725060716, 0.4288403609558664, 1.0,
      1.0, 1.0},
     {0.21193756319429014, -0.42387512638858027, -0.3384627126235924, 1.8046452872882734,
-     2.325825639765069, 0.0, 0.0, 0.0}};
+     2.325825639765069, 0.0, 0.0, 0.0},
+    {{18.16834780539935, -2.4039764856508183, -28.49033949791586, -39.2353239248467,
+      24.203612973337464, 31.138884328789313, 5.813992390809251},
+     {20.204639179440537, -1.56515746765803, 153.73045429293606, 118.22912892648397,
+      -71.49081351328653, -100.21520495678435, -27.642530934779916},
+     {-15.086356891234155, 3.4125953268012186, -59.68473410632594, -45.47407592791441,
+      28.595178703267447, 42.46386356049424, 18.195502534440195}}};
 
 /* ------------------------------------------------------------------------ */
 /* Controller constants (OrdinaryDiffEq defaults for an order-5 method).     */
@@ -890,11 +915,13 @@ typedef struct {
   double rpiv[NMAX];
 } rodas_ws;
 
+/* kd: the 8 stage increments (FP64; values exactly representable in FP32  */
+/* when mixed), kept by the caller for the dense output.                    */
 static int rodas5_step(const mdl_t *m, const rdtab_t *T, int mixed, const double *uprev,
                        double t, double dt, double abstol, double reltol, double *u1,
-                       double *eest, rodas_ws *ws, double *normJ, int L) {
+                       double *eest, rodas_ws *ws, double *normJ, int L,
+                       double kd[8][NMAX]) {
   int n = m->n;
-  double kd[8][NMAX];  /* k in FP64 (values exactly representable in FP32 when mixed) */
   double dT[NMAX], x[NMAX], U[NMAX];
   float Uf[NMAX], duf[NMAX];
   double du[NMAX];
@@ -986,16 +1013,61 @@ static int rodas5_step(const mdl_t *m, const rdtab_t *T, int mixed, const double
   return 0;
 }
 
+/* Rosenbrock dense output at th in (0, 1) from uprev, u1 and the stage     */
+/* increments (FP64 in both precisions):                                    */
+/*   K_m = sum_{j<7} H[m][j] k_j   (fma ascending in j)                     */
+/*   u(th) = fma(th, fma(1-th, fma(th, fma(th, K3, K2), K1), u1), (1-th) uprev) */
+static void rosenbrock_interp(int n, const rdtab_t *T, const double *uprev, const double *u1,
+                              double kd[8][NMAX], double th, double *out) {
+  double th1 = 1.0 - th;
+  for (int c = 0; c < n; c++) {
+    double K[3];
+    for (int mm = 0; mm < 3; mm++) {
+      double acc = T->H[mm][0] * kd[0][c];
+      for (int j = 1; j < 7; j++) acc = fma(T->H[mm][j], kd[j][c], acc);
+      K[mm] = acc;
+    }
+    double inner = fma(th, K[2], K[1]);
+    inner = fma(th, inner, K[0]);
+    double tt = fma(th1, inner, u1[c]);
+    out[c] = fma(th, tt, th1 * uprev[c]);
+  }
+}
+
 int oracle_rosenbrock_step(int32_t which, int32_t model, int32_t n, const double *u,
                            const double *p, double t, double dt, double abstol, double reltol,
                            double *u1, double *eest) {
   mdl_t m;
   mdl_init(&m, model, n, p, model_np(model, n));
   rodas_ws *ws = (rodas_ws *)malloc(sizeof(rodas_ws));
+  double kd[8][NMAX];
   int rc = rodas5_step(&m, which == 1 ? &RD5P : &RD5, 0, u, t, dt, abstol, reltol, u1, eest,
-                       ws, NULL, 64);
+                       ws, NULL, 64, kd);
   free(ws);
   return rc;
+}
+
+int oracle_rosenbrock_interp(int32_t which, int32_t model, int32_t n, const double *u,
+                             const double *p, double t, double dt, int32_t ntheta,
+                             const double *theta, double *u1, double *out) {
+  mdl_t m;
+  mdl_init(&m, model, n, p, model_np(model, n));
+  rodas_ws *ws = (rodas_ws *)malloc(sizeof(rodas_ws));
+  double kd[8][NMAX];
+  double eest;
+  const rdtab_t *T = which == 1 ? &RD5P : &RD5;
+  int rc = rodas5_step(&m, T, 0, u, t, dt, 1.0, 1.0, u1, &eest, ws, NULL, 64, kd);
+  free(ws);
+  if (rc != 0) return rc;
+  for (int k = 0; k < ntheta; k++) rosenbrock_interp(n, T, u, u1, kd, theta[k], out + (long)k * n);
+  return 0;
+}
+
+int oracle_rosenbrock_dense(int32_t which, double *H21) {
+  const rdtab_t *T = which == 1 ? &RD5P : &RD5;
+  for (int mm = 0; mm < 3; mm++)
+    for (int j = 0; j < 7; j++) H21[mm * 7 + j] = T->H[mm][j];
+  return 0;
 }
 
 int oracle_rodas5_step(int32_t model, int32_t n, const double *u, const double *p,
@@ -1015,6 +1087,7 @@ static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
   double uprev[NMAX], u1[NMAX];
   double k64[7][NMAX];
   float k32[7][NMAX];
+  double kdsave[8][NMAX]; /* Rosenbrock stage increments of the current step */
   double abstol = o->abstol, reltol = o->reltol;
   long long maxiters = o->maxiters > 0 ? o->maxiters : 100000;
   double dtmax = o->dtmax > 0 ? o->dtmax : fabs(tf - t0);
@@ -1135,7 +1208,7 @@ static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
     } else {
       double nJ;
       if (rodas5_step(m, rtab, mixed, uprev, t, dt, abstol, reltol, u1, &EEst, ws, &nJ,
-                      o->norm_lanes) != 0)
+                      o->norm_lanes, kdsave) != 0)
         EEst = 2.0;
       res->nf += 8;
       res->njac += 1;
@@ -1212,7 +1285,8 @@ static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
             else
               tsit5_interp_mix(n, uprev, k32, dt, th, val);
           } else {
-            for (int c = 0; c < n; c++) val[c] = NAN; /* Rodas5 dense output: not provided */
+            double th = (sa[isave] - tprev) / dt;
+            rosenbrock_interp(n, rtab, uprev, u1, kdsave, th, val);
           }
           for (int c = 0; c < n; c++) save[((long)isave * n + c) * save_stride] = val[c];
         }

@@ -15,7 +15,8 @@
  *     partially pivoted LU (Rosenbrock family; Rodas5P at qa.jl:97)
  *   - ode_determine_initdt (behind auto_dt_reset!, qa.jl:135)
  *   - PI step controller, tstop clamping and snapping at tf
- *   - savevalues! for a vector saveat (behaviour pinned by core.jl:90-96)
+ *   - savevalues! for a vector saveat (behaviour pinned by core.jl:90-96),
+ *     with the Tsit5 interpolant or a Rosenbrock dense output
  *   - AutoTsit5(Rodas5()) stiffness detection and switching (qa.jl:15)
  *   - EnsembleThreads fan-out over trajectories (qa.jl:56)
  * The upstream packages are not vendored in the reference; their published
@@ -132,6 +133,13 @@ int oracle_rosenbrock_step(int32_t which, int32_t model, int32_t n, const double
                            double *u1, double *eest);
 int oracle_rosenbrock_tableau(int32_t which, double *gamma, double *a64, double *C64,
                               double *c8, double *d8);
+/* Dense-output coefficients H[3][7] (row-major) of Rodas5 (0) / Rodas5P (1). */
+int oracle_rosenbrock_dense(int32_t which, double *H21);
+/* One FP64 Rosenbrock step from (t, u) with step dt, then the dense output
+ * at ntheta values theta[k] into out[k*n + c]; u1 gets the step's end. */
+int oracle_rosenbrock_interp(int32_t which, int32_t model, int32_t n, const double *u,
+                             const double *p, double t, double dt, int32_t ntheta,
+                             const double *theta, double *u1, double *out);
 double oracle_ctl_log2(double x);
 double oracle_ctl_exp2(double x);
 double oracle_ctl_pow(double x, double y);

@@ -78,6 +78,9 @@ class Oracle:
         L.oracle_tsit5_tableau.argtypes = [_dp, _dp, _dp, _dp]
         L.oracle_rodas5_tableau.argtypes = [_dp, _dp, _dp, _dp, _dp]
         L.oracle_rosenbrock_tableau.argtypes = [C.c_int32, _dp, _dp, _dp, _dp, _dp]
+        L.oracle_rosenbrock_dense.argtypes = [C.c_int32, _dp]
+        L.oracle_rosenbrock_interp.argtypes = [C.c_int32, C.c_int32, C.c_int32, _dp, _dp,
+                                               C.c_double, C.c_double, C.c_int32, _dp, _dp, _dp]
         for nm in ("oracle_ctl_log2", "oracle_ctl_exp2"):
             getattr(L, nm).argtypes = [C.c_double]
             getattr(L, nm).restype = C.c_double
@@ -205,6 +208,25 @@ class Oracle:
         d = np.zeros(8)
         self.lib.oracle_rosenbrock_tableau(which, C.byref(g), _ptr(a), _ptr(Cm), _ptr(c), _ptr(d))
         return g.value, a.reshape(8, 8), Cm.reshape(8, 8), c, d
+
+    def rosenbrock_dense(self, which=0):
+        """Dense-output coefficients H [3, 8] (the error stage column is 0)."""
+        h = np.zeros(21)
+        self.lib.oracle_rosenbrock_dense(which, _ptr(h))
+        return np.concatenate([h.reshape(3, 7), np.zeros((3, 1))], axis=1)
+
+    def rosenbrock_interp(self, which, model, u, p, t, dt, thetas):
+        """One FP64 Rosenbrock step, then its dense output at each theta:
+        (u1, values [len(thetas), n])."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        th = np.ascontiguousarray(thetas, dtype=np.float64)
+        u1 = np.zeros_like(u)
+        out = np.zeros((len(th), len(u)))
+        rc = self.lib.oracle_rosenbrock_interp(which, MODEL[model], len(u), _ptr(u), _ptr(p), t,
+                                               dt, len(th), _ptr(th), _ptr(u1), _ptr(out))
+        assert rc == 0
+        return u1, out
 
     def lu_solve(self, A, b):
         A = np.ascontiguousarray(A, dtype=np.float64).copy()
