@@ -907,7 +907,7 @@ struct BTeam {
 // NK: largest state size the instantiation holds (C = NK / L slots per
 // lane); every instance it is given must have n <= NK.
 template <int L, int NK, bool MIX, int ALG, bool FAST>
-__global__ __launch_bounds__(256) void k_bruss(BrussArgs a) {
+__device__ __forceinline__ void bruss_run(const BrussArgs& a) {
   constexpr int C = NK / L;
   constexpr int TPB = 256 / L;
   __shared__ double s_d[TPB][NK];
@@ -954,6 +954,29 @@ __global__ __launch_bounds__(256) void k_bruss(BrussArgs a) {
   }
 }
 
+template <int L, int NK, bool MIX, int ALG, bool FAST>
+__global__ __launch_bounds__(256) void k_bruss(BrussArgs a) {
+  bruss_run<L, NK, MIX, ALG, FAST>(a);
+}
+
+// The 32-lane FP64 Tsit5 team holding two (u, v) pairs per lane: left alone
+// the compiler takes 135 VGPRs (3 waves per SIMD, VALU busy ~84 %); held to
+// 4 waves it spills 5 dwords and runs 5 % faster (131,072-instance batch:
+// 34.3 -> 32.7 ms).  The FP32-stage build fits 4 waves by itself.
+template <int L, int NK, bool MIX, bool FAST>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_bruss_w4(
+    BrussArgs a) {
+  bruss_run<L, NK, MIX, ALG_TSIT5, FAST>(a);
+}
+
+template <int L, int NK, bool MIX, int ALG, bool FAST>
+static constexpr auto bruss_kernel() {
+  if constexpr (ALG == ALG_TSIT5 && L == 32 && NK == BRUSS_NMAX && !MIX)
+    return k_bruss_w4<L, NK, MIX, FAST>;
+  else
+    return k_bruss<L, NK, MIX, ALG, FAST>;
+}
+
 __global__ void k_bruss_reset(unsigned long long* q) { *q = 0ull; }
 
 // out[k] = order[(k * ntraj) / S] (natural order when order is NULL): a
@@ -972,8 +995,8 @@ struct OccupancyOp {
   int* per_cu;
   template <int L, int NK, bool MIX, int ALG, bool FAST>
   hipError_t run() const {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_bruss<L, NK, MIX, ALG, FAST>,
-                                                        256, 0);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        per_cu, bruss_kernel<L, NK, MIX, ALG, FAST>(), 256, 0);
   }
 };
 
@@ -983,7 +1006,8 @@ struct LaunchOp {
   hipStream_t s;
   template <int L, int NK, bool MIX, int ALG, bool FAST>
   hipError_t run() const {
-    hipLaunchKernelGGL((k_bruss<L, NK, MIX, ALG, FAST>), dim3(grid), dim3(256), 0, s, *a);
+    hipLaunchKernelGGL((bruss_kernel<L, NK, MIX, ALG, FAST>()), dim3(grid), dim3(256), 0, s,
+                       *a);
     return hipGetLastError();
   }
 };
