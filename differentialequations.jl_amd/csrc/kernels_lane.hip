@@ -17,64 +17,12 @@ namespace mideq {
 
 template <class M, bool MIX, int ALG, bool FAST>
 __global__ __launch_bounds__(256) void k_lane(LaneArgs a) {
-  const int64_t nq = lane_queue_len(a);
-  if ((int64_t)blockIdx.x * blockDim.x >= nq) return;  // whole block idle
-  lane_stage_saveat(a);
-  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nq) return;
-  int64_t i = lane_traj(a, k);
-  if (i < 0) return;
-  Lane<M, MIX, ALG, FAST> L;
-  L.init(a, i);
-  while (L.rc == RC_DEFAULT) L.step(a);
-  L.finish(a);
+  lane_kernel_run<M, MIX, ALG, FAST>(a);
 }
 
 template <class M, bool MIX, int ALG, bool FAST>
 __global__ __launch_bounds__(256) void k_persistent(LaneArgs a) {
-  const int lane = threadIdx.x & 63;
-  const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  lane_stage_saveat(a);
-  const unsigned long long nq = (unsigned long long)lane_queue_len(a);
-  Lane<M, MIX, ALG, FAST> L;
-  bool active = false;
-  bool drained = false;
-  while (true) {
-    if (!drained) {
-      unsigned long long need = __ballot(!active);
-      if (need) {
-        int cnt = __popcll(need);
-        int leader = __builtin_ctzll(need);
-        unsigned long long base = 0;
-        if (lane == leader) base = atomicAdd(a.queue, (unsigned long long)cnt);
-        base = __shfl(base, leader);
-        if (base + (unsigned long long)cnt >= nq) drained = true;
-        if (!active) {
-          unsigned long long my = base + (unsigned long long)__popcll(need & lt_mask);
-          int64_t i = my < nq ? lane_traj(a, (int64_t)my) : -1;
-          if (i >= 0) {
-            L.init(a, i);
-            if (L.rc == RC_DEFAULT) {
-              active = true;
-            } else {
-              L.finish(a);
-            }
-          }
-        }
-      }
-    }
-    if (!__any(active)) {
-      if (drained) break;
-      continue;
-    }
-    if (active) {
-      L.step(a);
-      if (L.rc != RC_DEFAULT) {
-        L.finish(a);
-        active = false;
-      }
-    }
-  }
+  persistent_kernel_run<M, MIX, ALG, FAST>(a);
 }
 
 __global__ void k_set_u64(unsigned long long* p, unsigned long long v) { *p = v; }

@@ -131,13 +131,20 @@ __device__ __forceinline__ double err_norm(const double* ut, const double* u0,
 }
 
 // Fast-controller error estimate: EEst^2 from one division over a common
-// denominator (n <= 3); an overflowing numerator stays +inf (rejected step).
+// denominator (n <= 3; an overflowing numerator stays +inf, a rejected
+// step), else mean(r^2) without the square root.
 template <int N>
 __device__ __forceinline__ double err_sq_fast(const double* ut, const double* u0,
                                               const double* u1, double abstol, double reltol) {
   double sc[N];
 #pragma unroll
   for (int i = 0; i < N; i++) sc[i] = fma(fmax(fabs(u0[i]), fabs(u1[i])), reltol, abstol);
+  if constexpr (N > 3) {
+    double r[N];
+#pragma unroll
+    for (int i = 0; i < N; i++) r[i] = ut[i] / sc[i];
+    return rms_sum_small<N>(r) / (double)N;
+  }
   double num, den;
   if constexpr (N == 1) {
     num = ut[0] * ut[0];
@@ -149,7 +156,7 @@ __device__ __forceinline__ double err_sq_fast(const double* ut, const double* u0
     double P = sc[0] * sc[1];
     den = (P * P) * 2.0;
   } else {
-    static_assert(N == 3, "fast controller supports n <= 3");
+    static_assert(N == 3 || N > 3, "n >= 1");
     double p01 = sc[0] * sc[1], p02 = sc[0] * sc[2], p12 = sc[1] * sc[2];
     double a0 = ut[0] * p12, a1 = ut[1] * p02, a2 = ut[2] * p01;
     num = a0 * a0;
@@ -236,6 +243,8 @@ struct Lane {
   static constexpr bool kHasRodas = (kAlg != ALG_TSIT5 && kAlg != ALG_AUTO_T);
   static constexpr bool kHandoff = (kAlg == ALG_AUTO_T);
   static constexpr bool kP = (kAlg == ALG_RODAS5P || kAlg == ALG_AUTO5P);  // Rodas5P tableau
+  // the division-free stiffness test of the fast controller (n <= 3)
+  static constexpr bool kFastStiff = FAST && N <= 3;
 
   int64_t i;
   double p[NP];
@@ -452,7 +461,7 @@ struct Lane {
         dk[c] = k[6][c] - k[5][c];
         dg[c] = u1[c] - g6[c];
       }
-      stiff_estimate<N, FAST>(dk, dg, eig, eigd);
+      stiff_estimate<N, kFastStiff>(dk, dg, eig, eigd);
     }
 #pragma unroll
     for (int c = 0; c < N; c++) {
@@ -536,7 +545,7 @@ struct Lane {
         dk[c] = (double)k[6][c] - (double)k[5][c];
         dg[c] = u1[c] - (double)g6[c];
       }
-      stiff_estimate<N, FAST>(dk, dg, eig, eigd);
+      stiff_estimate<N, kFastStiff>(dk, dg, eig, eigd);
     }
 #pragma unroll
     for (int c = 0; c < N; c++) {
@@ -721,7 +730,7 @@ struct Lane {
   __device__ void step(const LaneArgs& a) {
     if (kAuto && iter > 0) {
       bool stiff;
-      if constexpr (!FAST) {
+      if constexpr (!kFastStiff) {
         double stiffness = fabs(eig * dt / kStabSize);
         stiff = stiffness > kStiffTol;
       } else {
@@ -955,5 +964,70 @@ struct Lane {
     if (a.dt_first) a.dt_first[i] = dtfirst;
   }
 };
+
+// Kernel bodies, shared by the built-in kernels and the run-time compiled
+// ones.  lane: one thread per trajectory of the launch order.
+template <class M, bool MIX, int ALG, bool FAST>
+__device__ __forceinline__ void lane_kernel_run(const LaneArgs& a) {
+  const int64_t nq = lane_queue_len(a);
+  if ((int64_t)blockIdx.x * blockDim.x >= nq) return;  // whole block idle
+  lane_stage_saveat(a);
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nq) return;
+  int64_t i = lane_traj(a, k);
+  if (i < 0) return;
+  Lane<M, MIX, ALG, FAST> L;
+  L.init(a, i);
+  while (L.rc == RC_DEFAULT) L.step(a);
+  L.finish(a);
+}
+
+// persistent: lanes refill from the atomic queue, one atomic per wave.
+template <class M, bool MIX, int ALG, bool FAST>
+__device__ __forceinline__ void persistent_kernel_run(const LaneArgs& a) {
+  const int lane = threadIdx.x & 63;
+  const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  lane_stage_saveat(a);
+  const unsigned long long nq = (unsigned long long)lane_queue_len(a);
+  Lane<M, MIX, ALG, FAST> L;
+  bool active = false;
+  bool drained = false;
+  while (true) {
+    if (!drained) {
+      unsigned long long need = __ballot(!active);
+      if (need) {
+        int cnt = __popcll(need);
+        int leader = __builtin_ctzll(need);
+        unsigned long long base = 0;
+        if (lane == leader) base = atomicAdd(a.queue, (unsigned long long)cnt);
+        base = __shfl(base, leader);
+        if (base + (unsigned long long)cnt >= nq) drained = true;
+        if (!active) {
+          unsigned long long my = base + (unsigned long long)__popcll(need & lt_mask);
+          int64_t i = my < nq ? lane_traj(a, (int64_t)my) : -1;
+          if (i >= 0) {
+            L.init(a, i);
+            if (L.rc == RC_DEFAULT) {
+              active = true;
+            } else {
+              L.finish(a);
+            }
+          }
+        }
+      }
+    }
+    if (!__any(active)) {
+      if (drained) break;
+      continue;
+    }
+    if (active) {
+      L.step(a);
+      if (L.rc != RC_DEFAULT) {
+        L.finish(a);
+        active = false;
+      }
+    }
+  }
+}
 
 }  // namespace mideq

@@ -6,6 +6,7 @@
 // ::EnsembleThreads; trajectories, saveat, abstol, reltol) -- the ensemble
 // fan-out of test/qa/qa.jl:50,56 -- for problems packed into SoA buffers.
 #include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
 
 #include <algorithm>
 #include <cmath>
@@ -36,6 +37,19 @@ enum {
   BUF_SAMPLE, BUF_DTF, BUF_HANDOFF, BUF_COUNT
 };
 
+// A run-time compiled right-hand side: its generated model source and the
+// kernels built from it so far, one code object per (algorithm, precision,
+// controller) configuration.
+struct UserKernels {
+  hipModule_t mod = nullptr;
+  hipFunction_t lane = nullptr, persistent = nullptr;
+};
+struct UserModel {
+  int n = 0, np = 0;
+  std::string model_src;
+  std::map<int, UserKernels> kernels;
+};
+
 struct GraphCache {
   std::vector<unsigned char> key;
   hipGraphExec_t exec = nullptr;
@@ -54,6 +68,8 @@ struct mideq_ctx {
   // autotuned lanes per instance, keyed by (alg, precision, controller, nmax)
   std::map<std::tuple<int, int, int, int>, int> tuned_lanes;
   GraphCache graph;
+  std::vector<UserModel> user_models;  // model id MIDEQ_USER_MODEL_BASE + index
+  std::string arch;                    // device code target for hipRTC
 };
 
 static int set_err(mideq_ctx* c, int code, const char* fmt, ...) {
@@ -121,7 +137,13 @@ extern "C" int mideq_ctx_create(mideq_ctx** out, int32_t device) {
     return MIDEQ_E_HIP;
   }
   hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->ncu = prop.multiProcessorCount;
+  c->arch = "gfx950";
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+    c->ncu = prop.multiProcessorCount;
+    // "gfx950:sramecc+:xnack-" -> the processor name
+    std::string a = prop.gcnArchName;
+    if (!a.empty()) c->arch = a.substr(0, a.find(':'));
+  }
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
       hipMalloc((void**)&c->d_queue, 256) != hipSuccess) {
@@ -139,6 +161,9 @@ extern "C" void mideq_ctx_destroy(mideq_ctx* c) {
   drop_graph(c);
   for (auto& b : c->bufs)
     if (b.ptr) e = hipFree(b.ptr);
+  for (auto& um : c->user_models)
+    for (auto& kv : um.kernels)
+      if (kv.second.mod) e = hipModuleUnload(kv.second.mod);
   if (c->d_queue) e = hipFree(c->d_queue);
   if (c->ev0) e = hipEventDestroy(c->ev0);
   if (c->ev1) e = hipEventDestroy(c->ev1);
@@ -165,11 +190,216 @@ extern "C" int mideq_last_stats(const mideq_ctx* cc, mideq_stats* out) {
   return MIDEQ_OK;
 }
 
+// ---------------------------------------------------------------- user models
+// The solver headers as text (generated from the sources at build time).
+static const char* const kRtcSources[] = {
+#include "rtc_sources.inc"
+};
+
+static UserModel* user_model(mideq_ctx* c, int model) {
+  int k = model - MIDEQ_USER_MODEL_BASE;
+  if (!c || k < 0 || k >= (int)c->user_models.size()) return nullptr;
+  return &c->user_models[k];
+}
+
+static int check_user_spec(const mideq_user_model* m, std::string* why) {
+  if (!m || !m->rhs) {
+    *why = "null model or rhs";
+    return MIDEQ_E_ARG;
+  }
+  if (m->n < 1 || m->n > MIDEQ_USER_NMAX || m->np < 0 || m->np > MIDEQ_USER_NPMAX) {
+    char b[128];
+    snprintf(b, sizeof b, "user model n=%d np=%d outside 1..%d / 0..%d", m->n, m->np,
+             MIDEQ_USER_NMAX, MIDEQ_USER_NPMAX);
+    *why = b;
+    return MIDEQ_E_ARG;
+  }
+  return MIDEQ_OK;
+}
+
+// struct ModelUser: the user's statements as the templated RHS (T = double,
+// float, or Dual<N> for the Jacobian when none is given).
+static std::string user_model_source(const mideq_user_model* m) {
+  const int np = m->np > 0 ? m->np : 1;
+  std::string s = "namespace mideq {\nstruct ModelUser {\n  static constexpr int N = " +
+                  std::to_string(m->n) + ", NP = " + std::to_string(np) + ";\n";
+  s += "  template <class T>\n  __device__ static void f(T* du, const T* u, const T* p) {\n";
+  s += "#line 1 \"rhs\"\n";
+  s += m->rhs;
+  s += "\n  }\n  __device__ static void jac(double (*J)[N], const double* u, const double* p) {\n";
+  if (m->jac) {
+    s += "#line 1 \"jac\"\n";
+    s += m->jac;
+    s += "\n";
+  } else {
+    s += "    Dual<N> ud[N], pd[NP], fd[N];\n"
+         "    for (int i = 0; i < N; i++) { ud[i].v = u[i]; ud[i].d[i] = 1.0; }\n"
+         "    for (int j = 0; j < NP; j++) pd[j].v = p[j];\n"
+         "    f(fd, ud, pd);\n"
+         "    for (int i = 0; i < N; i++)\n"
+         "      for (int j = 0; j < N; j++) J[i][j] = fd[i].d[j];\n";
+  }
+  s += "  }\n};\n}  // namespace mideq\n";
+  return s;
+}
+
+// Compiles the solver for one configuration of a user model: kernels
+// mideq_user_lane / mideq_user_persistent.  Returns the code object.
+static int rtc_compile(const std::string& model_src, const std::string& arch, int alg, int mix,
+                       int fast, std::vector<char>* code, std::string* log) {
+  std::string src =
+      "typedef signed char int8_t; typedef short int16_t; typedef int int32_t;\n"
+      "typedef long long int64_t; typedef unsigned char uint8_t;\n"
+      "typedef unsigned int uint32_t; typedef unsigned long long uint64_t;\n";
+  for (const char* part : kRtcSources) src += part;
+  src += model_src;
+  char k[512];
+  snprintf(k, sizeof k,
+           "extern \"C\" __global__ __launch_bounds__(256) void mideq_user_lane(mideq::LaneArgs a) "
+           "{ mideq::lane_kernel_run<mideq::ModelUser, %s, %d, %s>(a); }\n"
+           "extern \"C\" __global__ __launch_bounds__(256) void "
+           "mideq_user_persistent(mideq::LaneArgs a) "
+           "{ mideq::persistent_kernel_run<mideq::ModelUser, %s, %d, %s>(a); }\n",
+           mix ? "true" : "false", alg, fast ? "true" : "false", mix ? "true" : "false", alg,
+           fast ? "true" : "false");
+  src += k;
+  hiprtcProgram prog;
+  if (hiprtcCreateProgram(&prog, src.c_str(), "mideq_user.hip", 0, nullptr, nullptr) !=
+      HIPRTC_SUCCESS) {
+    *log = "hiprtcCreateProgram failed";
+    return MIDEQ_E_HIP;
+  }
+  // the library's own flags: FP64 results reproducible (only explicit fma
+  // fuses), no packed FP32, no machine-level constant hoisting
+  std::string arch_opt = "--offload-arch=" + arch;
+  const char* opts[] = {arch_opt.c_str(), "-O3", "-std=c++17", "-ffp-contract=off",
+                        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+                        "-mllvm", "-disable-machine-licm"};
+  hiprtcResult r = hiprtcCompileProgram(prog, (int)(sizeof opts / sizeof opts[0]), opts);
+  size_t ls = 0;
+  if (hiprtcGetProgramLogSize(prog, &ls) == HIPRTC_SUCCESS && ls > 1) {
+    log->assign(ls, '\0');
+    if (hiprtcGetProgramLog(prog, &(*log)[0]) != HIPRTC_SUCCESS) log->clear();
+  }
+  int rc = MIDEQ_OK;
+  if (r != HIPRTC_SUCCESS) {
+    rc = MIDEQ_E_ARG;
+  } else {
+    size_t cs = 0;
+    if (hiprtcGetCodeSize(prog, &cs) != HIPRTC_SUCCESS || cs == 0) {
+      rc = MIDEQ_E_HIP;
+    } else {
+      code->resize(cs);
+      if (hiprtcGetCode(prog, code->data()) != HIPRTC_SUCCESS) rc = MIDEQ_E_HIP;
+    }
+  }
+  (void)hiprtcDestroyProgram(&prog);
+  return rc;
+}
+
+static int user_kernels(mideq_ctx* c, UserModel* um, int alg, int mix, int fast,
+                        UserKernels** out) {
+  int key = (alg << 2) | (mix << 1) | fast;
+  auto it = um->kernels.find(key);
+  if (it == um->kernels.end()) {
+    std::vector<char> code;
+    std::string log;
+    int rc = rtc_compile(um->model_src, c->arch, alg, mix, fast, &code, &log);
+    if (rc) return set_err(c, rc, "user model compile: %.400s", log.c_str());
+    UserKernels uk;
+    HIP_TRY(c, hipModuleLoadData(&uk.mod, code.data()));
+    HIP_TRY(c, hipModuleGetFunction(&uk.lane, uk.mod, "mideq_user_lane"));
+    HIP_TRY(c, hipModuleGetFunction(&uk.persistent, uk.mod, "mideq_user_persistent"));
+    it = um->kernels.emplace(key, uk).first;
+  }
+  *out = &it->second;
+  return MIDEQ_OK;
+}
+
+// Template ALG of a lane launch: Rosenbrock methods with saveat use the
+// build with the dense output.
+static int lane_alg_code(const LaneArgs& a, int alg) {
+  return (a.ns > 0 && a.us && alg != ALG_TSIT5 && alg != ALG_AUTO_T) ? (alg | ALG_DENSE) : alg;
+}
+
+// The lane-kernel launch of launch_lane_solver for a user model whose
+// kernels for this configuration are built (user_kernels).
+static hipError_t launch_user(const UserKernels* uk, const LaneArgs& a, int sched, int block,
+                              int grid, int ncu, hipStream_t s) {
+  const unsigned lds = (unsigned)lane_lds_bytes(a.ns);
+  LaneArgs arg = a;
+  void* params[] = {&arg};
+  if (sched == SCHED_PERSISTENT) {
+    if (grid <= 0) {
+      int per_cu = 0;
+      if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, uk->persistent, block,
+                                                             lds) != hipSuccess ||
+          per_cu < 1)
+        per_cu = 1;
+      int64_t need = (a.ntraj + block - 1) / block;
+      int64_t g = (int64_t)per_cu * ncu;
+      grid = (int)(g < need ? g : need);
+    }
+    hipError_t e = launch_set_u64(a.queue, 0ull, s);
+    if (e != hipSuccess) return e;
+    return hipModuleLaunchKernel(uk->persistent, grid, 1, 1, block, 1, 1, lds, s, params,
+                                 nullptr);
+  }
+  return hipModuleLaunchKernel(uk->lane, grid, 1, 1, block, 1, 1, lds, s, params, nullptr);
+}
+
+extern "C" int mideq_model_check(const mideq_user_model* m, char* log, size_t log_size) {
+  std::string why;
+  int rc = check_user_spec(m, &why);
+  if (rc == MIDEQ_OK) {
+    std::vector<char> code;
+    // AutoTsit5 with FP32 stages instantiates f for double, float and (for
+    // the Jacobian) dual numbers
+    rc = rtc_compile(user_model_source(m), "gfx950", ALG_AUTO, 1, 0, &code, &why);
+  }
+  if (log && log_size) {
+    snprintf(log, log_size, "%s", why.c_str());
+  }
+  return rc;
+}
+
+extern "C" int mideq_model_register(mideq_ctx* c, const mideq_user_model* m,
+                                    int32_t* model_id) {
+  if (!c || !model_id) return MIDEQ_E_ARG;
+  std::string why;
+  int rc = check_user_spec(m, &why);
+  if (rc) return set_err(c, rc, "%s", why.c_str());
+  HIP_TRY(c, hipSetDevice(c->device));
+  UserModel um;
+  um.n = m->n;
+  um.np = m->np > 0 ? m->np : 1;  // the kernel reads at least one parameter row
+  um.model_src = user_model_source(m);
+  c->user_models.push_back(um);
+  UserModel* p = &c->user_models.back();
+  // compile one configuration now, so that errors surface here: AutoTsit5
+  // with FP32 stages instantiates f for double, float and dual numbers
+  UserKernels* uk = nullptr;
+  rc = user_kernels(c, p, ALG_AUTO, 1, 0, &uk);
+  if (rc) {
+    for (auto& kv : p->kernels)
+      if (kv.second.mod) (void)hipModuleUnload(kv.second.mod);
+    c->user_models.pop_back();
+    return rc;
+  }
+  *model_id = MIDEQ_USER_MODEL_BASE + (int32_t)(c->user_models.size() - 1);
+  return MIDEQ_OK;
+}
+
 // ---------------------------------------------------------------- validation
-static int model_dims(int model, int* n, int* np) {
+static int model_dims(mideq_ctx* c, int model, int* n, int* np) {
   if (model == MODEL_BRUSSELATOR) {
     *n = -1;
     *np = 3;
+    return 0;
+  }
+  if (UserModel* um = user_model(c, model)) {
+    *n = um->n;
+    *np = um->np;
     return 0;
   }
   return lane_model_dims(model, n, np);
@@ -178,7 +408,7 @@ static int model_dims(int model, int* n, int* np) {
 static int validate(mideq_ctx* c, const mideq_problem* P, const mideq_options* O) {
   if (!P || !O) return set_err(c, MIDEQ_E_ARG, "null problem or options");
   int n, np;
-  if (model_dims(P->model, &n, &np) != 0)
+  if (model_dims(c, P->model, &n, &np) != 0)
     return set_err(c, MIDEQ_E_MODEL, "unknown model id %d", P->model);
   if (P->ntraj < 0) return set_err(c, MIDEQ_E_ARG, "ntraj < 0");
   if (P->model == MODEL_BRUSSELATOR) {
@@ -471,26 +701,41 @@ static int run_solver(mideq_ctx* c, const mideq_problem* P, const mideq_options*
     LaneArgs a;
     int model, alg, prec, ctl, sched, block, grid, two_pass;
   } key{a, P->model, O->alg, O->precision, O->controller, sched, block, grid, two_pass};
+  // user model: build its kernels for this configuration before any capture
+  UserModel* um = user_model(c, P->model);
+  UserKernels* uk_main = nullptr;
+  UserKernels* uk_first = nullptr;
+  if (um) {
+    int rc0 = user_kernels(c, um, lane_alg_code(a, O->alg), O->precision, O->controller,
+                           &uk_main);
+    if (rc0 == MIDEQ_OK && two_pass)
+      rc0 = user_kernels(c, um, ALG_AUTO_T, O->precision, O->controller, &uk_first);
+    if (rc0) return rc0;
+  }
+  auto launch = [&](const LaneArgs& la, int alg, hipStream_t st) -> hipError_t {
+    if (um)
+      return launch_user(alg == ALG_AUTO_T ? uk_first : uk_main, la, sched, block, grid, c->ncu,
+                         st);
+    return launch_lane_solver(la, P->model, alg, O->precision, O->controller, sched, block, grid,
+                              st);
+  };
   int rc = enqueue(c, O->graph != 0, &key, sizeof key, s, [&](hipStream_t st) {
     if (!two_pass) {
       LaneArgs a1 = a;
       a1.handoff = nullptr;
       a1.handoff_count = nullptr;
-      return launch_lane_solver(a1, P->model, O->alg, O->precision, O->controller, sched,
-                                block, grid, st);
+      return launch(a1, O->alg, st);
     }
     hipError_t e = launch_set_u64(a.handoff_count, 0ull, st);
     if (e != hipSuccess) return e;
-    e = launch_lane_solver(a, P->model, ALG_AUTO_T, O->precision, O->controller, sched, block,
-                           grid, st);
+    e = launch(a, ALG_AUTO_T, st);
     if (e != hipSuccess) return e;
     LaneArgs a2 = a;
     a2.order = a.handoff;
     a2.order_len = a.handoff_count;
     a2.handoff = nullptr;
     a2.handoff_count = nullptr;
-    return launch_lane_solver(a2, P->model, O->alg, O->precision, O->controller, sched, block,
-                              grid, st);
+    return launch(a2, O->alg, st);
   });
   if (rc) return rc;
   HIP_TRY(c, hipEventRecord(c->ev1, s));

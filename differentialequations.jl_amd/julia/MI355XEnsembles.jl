@@ -13,8 +13,10 @@ trajectory of an `EnsembleProblem` on one AMD MI355X through libmideq.so
 Reference plug point: `SciMLBase.__solve(::AbstractEnsembleProblem, alg,
 ::EnsembleAlgorithm; trajectories, ...)`, the dispatch EnsembleThreads uses
 (names re-exported by DifferentialEquations.jl, src/DifferentialEquations.jl:8-9;
-test/qa/qa.jl:50,52,56).  The right-hand side must be one of the registered
-device models (`MODEL_FOR` below); any other `f` falls back to
+test/qa/qa.jl:50,52,56).  The right-hand side must be registered
+(`MODEL_FOR` below): as a built-in device model, or with its body as C++
+statements that the library compiles for the GPU at run time
+(`register_model!(f, n, np, rhs)`); any other `f` falls back to
 `EnsembleThreads` with a warning, as the backend cannot run arbitrary Julia
 code on the device.
 
@@ -76,6 +78,41 @@ const MODEL_IDS = Dict(:exp => 0, :lorenz => 1, :robertson => 2, :vdp => 3,
                        :brusselator => 4, :linear => 5)
 const MODEL_FOR = IdDict{Any, Int32}()
 register_model!(f, name::Symbol) = (MODEL_FOR[f] = MODEL_IDS[name]; f)
+
+struct MideqUserModel
+    n::Int32; np::Int32; rhs::Ptr{UInt8}; jac::Ptr{UInt8}
+end
+
+"""
+    register_model!(f!, n, np, rhs; jac = nothing, device = 0)
+
+Registers `f!` with a device right-hand side written as C++ statements for
+an autonomous u' = f(u, p): assign `du[0..n-1]` from `u[i]`, `p[j]`
+(0-based); the number type is `T`, constants `2.0` or `T(2)`, functions
+`sqrt exp log sin cos tanh fabs pow(x, c)`.  `jac` assigns `J[i][j]`;
+without it the Rosenbrock methods differentiate `rhs` in forward mode.
+The model id belongs to `device`'s context.  Example (test/core.jl:22-27):
+
+    register_model!(lorenz!, 3, 3, """
+        du[0] = p[0] * (u[1] - u[0]);
+        du[1] = u[0] * (p[1] - u[2]) - u[1];
+        du[2] = u[0] * u[1] - p[2] * u[2];""")
+"""
+function register_model!(f, n::Integer, np::Integer, rhs::AbstractString;
+                         jac::Union{Nothing, AbstractString} = nothing, device::Int = 0)
+    ctx = context(device)
+    id = Ref{Int32}(0)
+    r = String(rhs)
+    j = jac === nothing ? nothing : String(jac)
+    rc = GC.@preserve r j begin
+        m = MideqUserModel(n, np, pointer(r), j === nothing ? C_NULL : pointer(j))
+        ccall((:mideq_model_register, LIB[]), Cint,
+              (Ptr{Cvoid}, Ref{MideqUserModel}, Ref{Int32}), ctx.h, m, id)
+    end
+    check(ctx, rc, "mideq_model_register")
+    MODEL_FOR[f] = id[]
+    return f
+end
 
 alg_code(::Tsit5) = Int32(0)
 alg_code(::Rodas5) = Int32(1)
@@ -141,14 +178,14 @@ function __solve(ens::SciMLBase.AbstractEnsembleProblem, alg, ealg::EnsembleMI35
         probs = [ens.prob_func(deepcopy(ens.prob), i, repeat) for i in pending]
         M = length(probs)
         n = maximum(length(pr.u0) for pr in probs)
-        np = length(probs[1].p)
+        np = max(length(probs[1].p), 1)   # a user model without parameters reads one row
         # column-major (M, n): trajectory index fastest, the SoA layout of the ABI
         U0 = zeros(Float64, M, n); P = zeros(Float64, M, np)
         T0 = Float64[pr.tspan[1] for pr in probs]; TF = Float64[pr.tspan[2] for pr in probs]
         nvec = Int32[length(pr.u0) for pr in probs]
         for (k, pr) in enumerate(probs)
             U0[k, 1:length(pr.u0)] .= pr.u0
-            P[k, :] .= collect(pr.p)
+            P[k, 1:length(pr.p)] .= collect(pr.p)
         end
         ns = length(sa)
         US = ealg.save_dtype === Float32 ? zeros(Float32, M, n, ns) : zeros(Float64, M, n, ns)

@@ -16,12 +16,13 @@ from . import workloads
 from ._abi import MideqError, header_functions
 from .scimlbase import (AutoTsit5, EnsembleArrays, EnsembleMI355X, EnsembleProblem,
                         EnsembleSolution, ODEFunction, ODEProblem, ODESolution, ReturnCode,
-                        Rodas5, Rodas5P, Stats, Tsit5, brusselator, device_solve, exp_growth,
-                        linear3, lorenz, remake, robertson, solve, successful_retcode,
-                        vanderpol)
+                        Rodas5, Rodas5P, Stats, Tsit5, UserModel, brusselator, device_solve,
+                        exp_growth, linear3, lorenz, remake, robertson, solve,
+                        successful_retcode, vanderpol)
 
 __all__ = ["AutoTsit5", "EnsembleArrays", "EnsembleMI355X", "EnsembleProblem",
            "EnsembleSolution", "MideqError", "ODEFunction", "ODEProblem", "ODESolution",
-           "ReturnCode", "Rodas5", "Rodas5P", "Stats", "Tsit5", "brusselator", "device_solve",
+           "ReturnCode", "Rodas5", "Rodas5P", "Stats", "Tsit5", "UserModel", "brusselator",
+           "device_solve",
            "exp_growth", "header_functions", "linear3", "lorenz", "remake", "robertson",
            "solve", "successful_retcode", "vanderpol", "workloads"]

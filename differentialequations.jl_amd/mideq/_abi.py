@@ -33,6 +33,14 @@ class Problem(C.Structure):
                 ("tspan0", C.c_double), ("tspan1", C.c_double), ("nvec", C.c_void_p)]
 
 
+class UserModelSpec(C.Structure):
+    _fields_ = [("n", C.c_int32), ("np", C.c_int32), ("rhs", C.c_char_p), ("jac", C.c_char_p)]
+
+
+USER_MODEL_BASE = 1000
+USER_NMAX, USER_NPMAX = 8, 16
+
+
 class Options(C.Structure):
     _fields_ = [("alg", C.c_int32), ("precision", C.c_int32), ("abstol", C.c_double),
                 ("reltol", C.c_double), ("dt", C.c_double), ("dtmin", C.c_double),
@@ -95,6 +103,9 @@ def load(path=LIB_PATH):
     L.mideq_last_stats.argtypes = [C.c_void_p, C.POINTER(Stats)]
     L.mideq_ensemble_solve.argtypes = [C.c_void_p, C.POINTER(Problem), C.POINTER(Options),
                                        C.POINTER(Outputs)]
+    L.mideq_model_register.argtypes = [C.c_void_p, C.POINTER(UserModelSpec),
+                                       C.POINTER(C.c_int32)]
+    L.mideq_model_check.argtypes = [C.POINTER(UserModelSpec), C.c_char_p, C.c_size_t]
     L.mideq_ensemble_solve_device.argtypes = [C.c_void_p, C.POINTER(Problem),
                                               C.POINTER(Options), C.POINTER(Outputs),
                                               C.c_void_p]
@@ -137,6 +148,12 @@ class Context:
         self.check(self.lib.mideq_ensemble_solve_device(self.h, C.byref(prob), C.byref(opt),
                                                         C.byref(out), stream),
                    "mideq_ensemble_solve_device")
+
+    def register_model(self, spec):
+        mid = C.c_int32()
+        self.check(self.lib.mideq_model_register(self.h, C.byref(spec), C.byref(mid)),
+                   "mideq_model_register")
+        return mid.value
 
     def stats(self):
         s = Stats()

@@ -21,6 +21,8 @@ import time
 from dataclasses import dataclass, field, replace
 from typing import Any, Callable, Optional
 
+import ctypes as C
+
 import numpy as np
 
 from . import _abi
@@ -87,6 +89,54 @@ class ODEFunction:
             out[1::2] = p[1] * uu - uu * uu * vv + D * lv
             return out
         raise ValueError(self.name)
+
+
+class UserModel:
+    """A right-hand side given as C++ statements, compiled for the GPU at run
+    time (hipRTC) -- the counterpart of passing an in-place f!(du, u, p, t)
+    to ODEProblem (test/core.jl:22-27).  `rhs` assigns du[0..n-1] from u[i]
+    and p[j] for an autonomous system; its number type is T (double, float
+    for FP32 stages, or a dual number when differentiated), so constants are
+    written 2.0 or T(2), and sqrt exp log sin cos tanh fabs pow(x, c) are
+    available.  `jac` assigns J[i][j] = df_i/du_j (double); without it the
+    Rosenbrock methods differentiate rhs in forward mode.  A model with no
+    parameters still takes one (unused) parameter row."""
+
+    def __init__(self, n, np_, rhs, jac=None, name="user"):
+        if not (1 <= n <= _abi.USER_NMAX) or not (0 <= np_ <= _abi.USER_NPMAX):
+            raise ValueError(f"user model n={n} np={np_} outside 1..{_abi.USER_NMAX} / "
+                             f"0..{_abi.USER_NPMAX}")
+        self.n = int(n)
+        self.np = max(int(np_), 1)
+        self.rhs = str(rhs)
+        self.jac_src = None if jac is None else str(jac)
+        self.jac = True
+        self.name = name
+        self._ids = {}
+
+    def spec(self):
+        return _abi.UserModelSpec(self.n, self.np, self.rhs.encode(),
+                                  None if self.jac_src is None else self.jac_src.encode())
+
+    def check(self):
+        """Compile without a GPU; returns the compiler log (raises on error)."""
+        L = _abi.load()
+        buf = C.create_string_buffer(16384)
+        sp = self.spec()
+        rc = L.mideq_model_check(C.byref(sp), buf, len(buf))
+        if rc != 0:
+            raise _abi.MideqError(f"user model does not compile:\n{buf.value.decode()}")
+        return buf.value.decode()
+
+    def model_id_in(self, ctx):
+        """The model id registered with `ctx` (registers on first use)."""
+        key = id(ctx)
+        if key not in self._ids:
+            self._ids[key] = ctx.register_model(self.spec())
+        return self._ids[key]
+
+    def __call__(self, u, p, t=0.0):
+        raise NotImplementedError("a user model runs on the GPU only")
 
 
 exp_growth = ODEFunction("exp", 1, 1)
@@ -340,7 +390,7 @@ def device_solve(f, u0, p, t0, tf, nvec, alg, ealg, saveat=None, abstol=1e-6, re
     ctx = _abi.context(ealg.device)
     n, N = u0.shape
     P = _abi.Problem()
-    P.model = f.model_id
+    P.model = f.model_id_in(ctx) if isinstance(f, UserModel) else f.model_id
     P.n = n
     P.np = p.shape[0]
     P.ntraj = N

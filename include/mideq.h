@@ -29,6 +29,7 @@
 #ifndef MIDEQ_H
 #define MIDEQ_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -188,6 +189,37 @@ int mideq_ensemble_solve(mideq_ctx *ctx, const mideq_problem *prob,
 int mideq_ensemble_solve_device(mideq_ctx *ctx, const mideq_problem *prob,
                                 const mideq_options *opt, mideq_outputs *out,
                                 void *stream);
+
+/* ---- run-time compiled right-hand sides (hipRTC).  Replaces the user's
+ * in-place f!(du, u, p, t) of ODEProblem(f!, u0, tspan, p) (test/core.jl:22-27,
+ * test/qa/qa.jl:86) and ODEFunction(f!; jac) (qa.jl:84) for the lane kernels:
+ * the shim passes f's body as C++ statements instead of a registered id. */
+#define MIDEQ_USER_MODEL_BASE 1000 /* first id of a registered user model */
+#define MIDEQ_USER_NMAX 8          /* states (one lane per trajectory)     */
+#define MIDEQ_USER_NPMAX 16        /* parameters                           */
+typedef struct {
+  int32_t n;        /* states, 1..MIDEQ_USER_NMAX */
+  int32_t np;       /* parameters, 0..MIDEQ_USER_NPMAX */
+  const char *rhs;  /* statements of an autonomous u' = f(u, p): assign du[0..n-1] from
+                       u[i] and p[j].  The number type is T (double; float for FP32
+                       stages; a dual number when differentiated): write T(2) or 2.0 for
+                       constants and use sqrt exp log sin cos tanh fabs pow(x, c). */
+  const char *jac;  /* statements assigning double J[i][j] = df_i/du_j from u[], p[];
+                       NULL: forward-mode automatic differentiation of rhs (the default
+                       of OrdinaryDiffEq's Rosenbrock methods without a Jacobian) */
+} mideq_user_model;
+
+/* Registers and compiles a user model on the context's device; *model_id
+ * (>= MIDEQ_USER_MODEL_BASE) then goes in mideq_problem.model, with n and np
+ * as registered.  A compile error returns MIDEQ_E_ARG with the compiler log
+ * in mideq_last_error.  Kernels for further configurations (algorithm,
+ * precision, controller, dense output) are compiled on their first use. */
+int mideq_model_register(mideq_ctx *ctx, const mideq_user_model *m, int32_t *model_id);
+
+/* Compiles a user model for gfx950 without a context or device (a syntax
+ * check of f for double, float and dual numbers and of jac); `log` (may be
+ * NULL) receives the compiler output.  Returns 0 or MIDEQ_E_ARG. */
+int mideq_model_check(const mideq_user_model *m, char *log, size_t log_size);
 
 /* ---- learned dt0 / method-class / cost predictor (no counterpart in the
  * reference: it feeds dt0, alg_hint and order above; the initdt heuristic

@@ -316,9 +316,30 @@ typedef struct {
   float pf[16];
 } mdl_t;
 
+static struct {
+  int n, np;
+  oracle_user_f64 f64;
+  oracle_user_f32 f32;
+  oracle_user_jac jac;
+} g_user;
+
+int oracle_set_user_model(int32_t n, int32_t np, oracle_user_f64 f64, oracle_user_f32 f32,
+                          oracle_user_jac jac) {
+  if (n < 1 || n > NMAX || np < 0 || np > 16 || !f64) return -1;
+  g_user.n = n;
+  g_user.np = np;
+  g_user.f64 = f64;
+  g_user.f32 = f32;
+  g_user.jac = jac;
+  return 0;
+}
+
 static void rhs64(const mdl_t *m, const double *u, double t, double *du) {
   const double *p = m->p;
   switch (m->model) {
+    case ORACLE_MODEL_USER:
+      g_user.f64(du, u, p);
+      break;
     case ORACLE_MODEL_EXP:
       du[0] = p[0] * u[0];
       break;
@@ -374,6 +395,9 @@ static void rhs64(const mdl_t *m, const double *u, double t, double *du) {
 static void rhs32(const mdl_t *m, const float *u, double t, float *du) {
   const float *p = m->pf;
   switch (m->model) {
+    case ORACLE_MODEL_USER:
+      g_user.f32(du, u, p);
+      break;
     case ORACLE_MODEL_EXP:
       du[0] = p[0] * u[0];
       break;
@@ -432,6 +456,9 @@ static void jac64(const mdl_t *m, const double *u, double t, double *J) {
   int n = m->n;
   (void)t;
   switch (m->model) {
+    case ORACLE_MODEL_USER:
+      g_user.jac(J, u, p);
+      break;
     case ORACLE_MODEL_EXP:
       J[0] = p[0];
       break;
@@ -497,6 +524,8 @@ static void tgrad64(const mdl_t *m, const double *u, double t, double *dT) {
 /* number of parameters each model reads */
 static int model_np(int model, int n) {
   switch (model) {
+    case ORACLE_MODEL_USER:
+      return g_user.np;
     case ORACLE_MODEL_LORENZ:
     case ORACLE_MODEL_ROBERTSON:
     case ORACLE_MODEL_BRUSSELATOR:

@@ -50,8 +50,18 @@ enum {
   ORACLE_MODEL_VDP = 3,         /* classic x'=y, y'=mu(1-x^2)y-x                   */
   ORACLE_MODEL_BRUSSELATOR = 4, /* 1-D periodic, p = (A, B, alpha), n = 2*Ngrid    */
   ORACLE_MODEL_LINEAR = 5,      /* u' = M u, M row-major in p (np = n*n)           */
-  ORACLE_MODEL_PROTHERO = 6     /* y' = lambda (y - sin t) + cos t (non-autonomous)*/
+  ORACLE_MODEL_PROTHERO = 6,    /* y' = lambda (y - sin t) + cos t (non-autonomous)*/
+  ORACLE_MODEL_USER = 7         /* callbacks set by oracle_set_user_model            */
 };
+
+/* A run-time compiled RHS on the CPU: the tests compile the same statements
+ * the device compiles (f for double and float, J row-major n*n) and hand the
+ * functions over.  One user model at a time (process-wide). */
+typedef void (*oracle_user_f64)(double *du, const double *u, const double *p);
+typedef void (*oracle_user_f32)(float *du, const float *u, const float *p);
+typedef void (*oracle_user_jac)(double *J, const double *u, const double *p);
+int oracle_set_user_model(int32_t n, int32_t np, oracle_user_f64 f64, oracle_user_f32 f32,
+                          oracle_user_jac jac);
 
 enum {
   ORACLE_ALG_TSIT5 = 0,

@@ -10,7 +10,7 @@ import numpy as np
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB = os.path.join(REPO, "oracle", "_build", "libmideq_oracle.so")
 
-MODEL = dict(exp=0, lorenz=1, robertson=2, vdp=3, brusselator=4, linear=5, prothero=6)
+MODEL = dict(exp=0, lorenz=1, robertson=2, vdp=3, brusselator=4, linear=5, prothero=6, user=7)
 ALG = dict(tsit5=0, rodas5=1, auto=2, rodas5p=3, auto5p=4)
 PREC = dict(fp64=0, mixed=1)
 RC = dict(default=0, success=1, maxiters=2, dtlessthanmin=3, unstable=4, dtnan=5,

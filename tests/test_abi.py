@@ -26,7 +26,8 @@ def test_struct_layouts_match_header(tmp_path):
     if not shutil.which("gcc"):
         pytest.skip("gcc not available")
     structs = {"mideq_problem": _abi.Problem, "mideq_options": _abi.Options,
-               "mideq_outputs": _abi.Outputs, "mideq_stats": _abi.Stats}
+               "mideq_outputs": _abi.Outputs, "mideq_stats": _abi.Stats,
+               "mideq_user_model": _abi.UserModelSpec}
     lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "mideq.h"', "int main(){"]
     for cname, cls in structs.items():
         lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
@@ -70,3 +71,11 @@ def test_saveat_scalar_expansion_matches_core_jl():
     from mideq.scimlbase import _saveat_vector
     v = _saveat_vector(0.1, (0.0, 1.0))
     assert len(v) == 11 and v[0] == 0.0 and v[-1] == 1.0
+
+
+def test_user_model_limits_match_header():
+    import re
+    txt = open(_abi.HEADER).read()
+    get = lambda k: int(re.search(r"#define %s (\d+)" % k, txt).group(1))
+    assert (get("MIDEQ_USER_MODEL_BASE"), get("MIDEQ_USER_NMAX"), get("MIDEQ_USER_NPMAX")) == (
+        _abi.USER_MODEL_BASE, _abi.USER_NMAX, _abi.USER_NPMAX)

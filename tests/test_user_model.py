@@ -1,0 +1,105 @@
+"""Run-time compiled right-hand sides on the CPU: the device compile check
+(hipRTC runs without a GPU), and the oracle's user-model path -- the dual-
+number Jacobian against analytic and finite-difference Jacobians, and whole
+solves of a user model against the same model built in."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import user_model_host as umh
+
+LORENZ = ("du[0] = p[0] * (u[1] - u[0]);\n"
+          "du[1] = u[0] * (p[1] - u[2]) - u[1];\n"
+          "du[2] = u[0] * u[1] - p[2] * u[2];")
+ROBERTSON = ("T y1 = u[0], y2 = u[1], y3 = u[2];\n"
+             "T k1 = p[0], k2 = p[1], k3 = p[2];\n"
+             "du[0] = -k1 * y1 + k3 * y2 * y3;\n"
+             "du[1] = k1 * y1 - k2 * (y2 * y2) - k3 * y2 * y3;\n"
+             "du[2] = k2 * (y2 * y2);")
+PENDULUM = ("du[0] = u[1];\n"
+            "du[1] = -p[0] * sin(u[0]) - p[1] * u[1] + 0.1 * exp(-u[0] * u[0]) * tanh(u[1])"
+            " + pow(fabs(u[0]) + 1.0, 1.5) / sqrt(2.0 + u[1] * u[1]) - log(3.0 + cos(u[0]));")
+
+
+def _mideq():
+    import mideq
+    return mideq
+
+
+def test_device_compile_check_and_errors():
+    mideq = _mideq()
+    assert mideq.UserModel(3, 3, LORENZ).check() == ""
+    mideq.UserModel(2, 2, PENDULUM).check()
+    with pytest.raises(mideq.MideqError, match="rhs:1:13"):
+        mideq.UserModel(2, 1, "du[0] = u[1]\ndu[1] = -u[0];").check()
+    with pytest.raises(mideq.MideqError, match="jac"):
+        mideq.UserModel(1, 1, "du[0] = -u[0];", jac="J[0][0] = -1.0 +;").check()
+    with pytest.raises(ValueError):
+        mideq.UserModel(9, 1, "du[0] = 0.0;")
+
+
+def _jac(lib, n, u, p):
+    J = np.zeros(n * n)
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    p = np.ascontiguousarray(p, dtype=np.float64)
+    dp = C.POINTER(C.c_double)
+    lib.user_jac(J.ctypes.data_as(dp), u.ctypes.data_as(dp), p.ctypes.data_as(dp))
+    return J.reshape(n, n)
+
+
+def _f(lib, u, p):
+    du = np.zeros(len(u))
+    dp = C.POINTER(C.c_double)
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    p = np.ascontiguousarray(p, dtype=np.float64)
+    lib.user_f64(du.ctypes.data_as(dp), u.ctypes.data_as(dp), p.ctypes.data_as(dp))
+    return du
+
+
+def test_dual_jacobian_matches_analytic(oracle):
+    """Forward-mode AD of polynomial right-hand sides gives the hand-written
+    Jacobians of the built-in models exactly."""
+    rng = np.random.default_rng(0)
+    for name, rhs in (("lorenz", LORENZ), ("robertson", ROBERTSON)):
+        lib = umh.build(3, 3, rhs)
+        for _ in range(20):
+            u = rng.uniform(-2, 2, 3)
+            p = rng.uniform(0.1, 30, 3)
+            np.testing.assert_array_equal(_f(lib, u, p), oracle.rhs(name, u, p))
+            np.testing.assert_array_equal(_jac(lib, 3, u, p), oracle.jac(name, u, p))
+
+
+def test_dual_jacobian_transcendental_vs_finite_differences():
+    lib = umh.build(2, 2, PENDULUM)
+    rng = np.random.default_rng(1)
+    for _ in range(20):
+        u = rng.uniform(-1.5, 1.5, 2)
+        p = rng.uniform(0.5, 5.0, 2)
+        J = _jac(lib, 2, u, p)
+        Jfd = np.zeros((2, 2))
+        for j in range(2):
+            h = 1e-6 * max(1.0, abs(u[j]))
+            e = np.zeros(2)
+            e[j] = h
+            Jfd[:, j] = (_f(lib, u + e, p) - _f(lib, u - e, p)) / (2 * h)
+        np.testing.assert_allclose(J, Jfd, rtol=1e-7, atol=1e-8)
+
+
+@pytest.mark.parametrize("alg,precision,controller", [("tsit5", "fp64", 0), ("tsit5", "mixed", 1),
+                                                      ("rodas5", "fp64", 0), ("auto", "fp64", 1)])
+def test_oracle_user_model_equals_builtin(oracle, alg, precision, controller):
+    """The oracle's user-model path (host-compiled statements, dual-number J)
+    reproduces the built-in Robertson model bit for bit."""
+    umh.install(oracle, 3, 3, ROBERTSON)
+    N = 16
+    rng = np.random.default_rng(2)
+    u0 = np.zeros((3, N))
+    u0[0] = 1.0
+    p = np.array([0.04, 3e7, 1e4])[:, None] * 10.0 ** rng.uniform(-0.5, 0.5, (3, N))
+    kw = dict(alg=alg, precision=precision, controller=controller, abstol=1e-8, reltol=1e-6,
+              saveat=np.logspace(-4, 4, 9))
+    a = oracle.ensemble("user", u0, p, 0.0, 1e4, **kw)
+    b = oracle.ensemble("robertson", u0, p, 0.0, 1e4, **kw)
+    for k in ("u", "t", "retcode", "naccept", "nreject", "nf", "saveat"):
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)

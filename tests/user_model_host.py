@@ -1,0 +1,79 @@
+"""Host build of a run-time compiled user model, for the oracle.
+
+Test infrastructure.  The device compiles the user's statements into
+`struct ModelUser` (f templated over double, float and the dual numbers of
+differentialequations.jl_amd/csrc/user_dual.hpp; jac given or
+differentiated); this module compiles the same text with g++ (no
+contraction, IEEE float and double on x86-64 SSE) into a shared object and
+hands its functions to the oracle (oracle_set_user_model)."""
+import ctypes as C
+import hashlib
+import os
+import subprocess
+import tempfile
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DUAL = os.path.join(ROOT, "differentialequations.jl_amd", "csrc", "user_dual.hpp")
+_CACHE = {}
+
+
+def model_source(n, np_, rhs, jac=None):
+    np_ = max(int(np_), 1)
+    s = ["#include <math.h>", "#define MIDEQ_HD inline", f'#include "{DUAL}"',
+         "namespace mideq {", "struct ModelUser {",
+         f"  static constexpr int N = {n}, NP = {np_};",
+         "  template <class T>", "  static void f(T* du, const T* u, const T* p) {",
+         rhs, "  }", "  static void jac(double (*J)[N], const double* u, const double* p) {"]
+    if jac is not None:
+        s.append(jac)
+    else:
+        s += ["    Dual<N> ud[N], pd[NP], fd[N];",
+              "    for (int i = 0; i < N; i++) { ud[i].v = u[i]; ud[i].d[i] = 1.0; }",
+              "    for (int j = 0; j < NP; j++) pd[j].v = p[j];",
+              "    f(fd, ud, pd);",
+              "    for (int i = 0; i < N; i++)",
+              "      for (int j = 0; j < N; j++) J[i][j] = fd[i].d[j];"]
+    s += ["  }", "};", "}  // namespace mideq",
+          'extern "C" void user_f64(double* du, const double* u, const double* p) '
+          "{ mideq::ModelUser::f<double>(du, u, p); }",
+          'extern "C" void user_f32(float* du, const float* u, const float* p) '
+          "{ mideq::ModelUser::f<float>(du, u, p); }",
+          'extern "C" void user_jac(double* J, const double* u, const double* p) '
+          "{ mideq::ModelUser::jac((double (*)[mideq::ModelUser::N])J, u, p); }"]
+    return "\n".join(s) + "\n"
+
+
+def build(n, np_, rhs, jac=None):
+    """ctypes handle of the host-compiled model (cached by source)."""
+    src = model_source(n, np_, rhs, jac)
+    key = hashlib.sha1(src.encode()).hexdigest()[:16]
+    if key in _CACHE:
+        return _CACHE[key]
+    d = tempfile.mkdtemp(prefix="mideq_user_")
+    cpp = os.path.join(d, "user.cpp")
+    so = os.path.join(d, "user.so")
+    with open(cpp, "w") as f:
+        f.write(src)
+    subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
+                    "-fno-fast-math", cpp, "-o", so], check=True, capture_output=True)
+    lib = C.CDLL(so)
+    dp = C.POINTER(C.c_double)
+    fp = C.POINTER(C.c_float)
+    lib.user_f64.argtypes = [dp, dp, dp]
+    lib.user_f32.argtypes = [fp, fp, fp]
+    lib.user_jac.argtypes = [dp, dp, dp]
+    _CACHE[key] = lib
+    return lib
+
+
+def install(oracle, n, np_, rhs, jac=None):
+    """Make the oracle's "user" model this one."""
+    lib = build(n, np_, rhs, jac)
+    addr = lambda fn: C.cast(fn, C.c_void_p).value
+    L = oracle.lib
+    L.oracle_set_user_model.argtypes = [C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
+                                        C.c_void_p]
+    rc = L.oracle_set_user_model(n, max(int(np_), 1), addr(lib.user_f64), addr(lib.user_f32),
+                                 addr(lib.user_jac))
+    assert rc == 0
+    return lib
