@@ -159,7 +159,7 @@ constexpr double kStiffTol = 0.9, kDtFac = 2.0;
 constexpr double kStiffTolS = kStiffTol * kStabSize;  // fast form of the stiffness test
 
 enum : int8_t { RC_DEFAULT = 0, RC_SUCCESS = 1, RC_MAXITERS = 2, RC_DTLESSTHANMIN = 3,
-                RC_UNSTABLE = 4, RC_DTNAN = 5, RC_INITIALFAILURE = 6,
+                RC_UNSTABLE = 4, RC_DTNAN = 5, RC_INITIALFAILURE = 6, RC_TERMINATED = 7,
                 // internal: the first pass of a two-phase AutoSwitch solve hands
                 // the trajectory to the full AutoSwitch pass (never returned)
                 RC_HANDOFF = 100 };

@@ -51,6 +51,8 @@ struct LaneArgs {
   int32_t* nf;
   int32_t* nstiff;
   double* dt_first;
+  int32_t* nevent;            // callbacks: affects applied per trajectory
+  double* t_event;            // callbacks: time of the last affect
   unsigned long long* queue;  // persistent schedule: next position in the order
   const int64_t* order;       // launch order (NULL: natural order)
   const unsigned long long* order_len;  // device length of `order` (NULL: ntraj)
@@ -232,9 +234,25 @@ __device__ __forceinline__ void stiff_estimate(const double* dk, const double* d
   }
 }
 
+// Callbacks of a model (run-time compiled user models may carry them):
+// M::NCB, and per callback k the constexpr kind (0 continuous, 1 discrete),
+// direction (continuous: 0 both, 1 up-crossings, -1 down-crossings),
+// terminate flag and interpolation points, the condition M::cb_g(k, u, p, t)
+// (continuous: a function whose sign changes; discrete: fires when nonzero)
+// and the affect M::cb_affect(k, u, p, t) on the state.
+template <class M, class = void>
+struct cb_count {
+  static constexpr int value = 0;
+};
+template <class M>
+struct cb_count<M, decltype((void)M::NCB, void())> {
+  static constexpr int value = M::NCB;
+};
+
 template <class M, bool MIX, int ALG, bool FAST>
 struct Lane {
   static constexpr int N = M::N;
+  static constexpr int kNcb = cb_count<M>::value;
   static constexpr int NP = M::NP;
   static constexpr int kAlg = ALG & (ALG_DENSE - 1);
   static constexpr bool kDense = (ALG & ALG_DENSE) != 0;
@@ -261,6 +279,11 @@ struct Lane {
   int32_t isave, count, nanrej;
   int8_t rc, mode;
   double tnext;  // saveat[isave], +inf once every point is saved
+  // callbacks: condition values at the last step end, affects applied, and
+  // the time of the last one
+  double gprev[kNcb > 0 ? kNcb : 1];
+  int32_t nevent;
+  double tevent;
 
   // ------------------------------------------------------------ save times
   __device__ __forceinline__ double save_time(const LaneArgs& a, int k) const {
@@ -381,6 +404,12 @@ struct Lane {
     }
     set_tnext(a);
     fsal_init();
+    nevent = 0;
+    tevent = __builtin_nan("");
+    if constexpr (kNcb > 0) {
+#pragma unroll
+      for (int k = 0; k < kNcb; k++) gprev[k] = M::cb_g(k, u, p, t0);
+    }
     if (a.dt_fixed > 0)
       dt = a.dt_fixed;
     else if (a.dt0)
@@ -847,6 +876,132 @@ struct Lane {
       }
       if (fabs(ttmp - tf) < tsnap) ttmp = tf;
       t = ttmp;
+      // dense output of this step at th (time tprev + th dt): Tsit5's
+      // interpolant or the Rosenbrock one (rd_H, FP64 in both precisions)
+      auto interp = [&](double th, double* val) {
+        if (tsit_step) {
+          double bt[7];
+          tsit5_bthetas(th, bt);
+          if constexpr (kHasTsit5 && !MIX) {
+#pragma unroll
+            for (int c = 0; c < N; c++) {
+              double acc = bt[6] * k64[6][c];
+#pragma unroll
+              for (int j = 5; j >= 0; j--) acc = fma(bt[j], k64[j][c], acc);
+              val[c] = fma(dt, acc, u[c]);
+            }
+          } else if constexpr (kHasTsit5 && MIX) {
+            float bf[7];
+#pragma unroll
+            for (int j = 0; j < 7; j++) bf[j] = (float)bt[j];
+#pragma unroll
+            for (int c = 0; c < N; c++) {
+              float acc = bf[6] * k32[6][c];
+#pragma unroll
+              for (int j = 5; j >= 0; j--) acc = fmaf(bf[j], k32[j][c], acc);
+              val[c] = fma(dt, (double)acc, u[c]);
+            }
+          }
+        } else if constexpr (kHasRodas && !kDense) {
+          // not reached: saveat and callbacks run the kDense build
+#pragma unroll
+          for (int c = 0; c < N; c++) val[c] = __builtin_nan("");
+        } else if constexpr (kHasRodas) {
+          double th1 = 1.0 - th;
+#pragma unroll
+          for (int c = 0; c < N; c++) {
+            double K[3];
+#pragma unroll
+            for (int m = 0; m < 3; m++) {
+              double acc = rd_H<kP>(m, 0) * kd[0][c];
+#pragma unroll
+              for (int j = 1; j < 7; j++) acc = fma(rd_H<kP>(m, j), kd[j][c], acc);
+              K[m] = acc;
+            }
+            double inner = fma(th, K[2], K[1]);
+            inner = fma(th, inner, K[0]);
+            double tt = fma(th1, inner, u1[c]);
+            val[c] = fma(th, tt, th1 * u[c]);
+          }
+        }
+      };
+      // the state the step ends in: u1, or the dense output at an event
+      double uend[N];
+#pragma unroll
+      for (int c = 0; c < N; c++) uend[c] = u1[c];
+      int ev = -1;
+      if constexpr (kNcb > 0) {
+        static_assert(!kHasRodas || kDense, "callbacks need the dense output");
+        // continuous callbacks: the earliest sign change of any condition
+        // over interp_points sub-intervals of the step, located by the
+        // Illinois (modified regula falsi) bracketing iteration on th; the
+        // event is the bracket end past the crossing
+        auto G = [&](int k, double th) {
+          double tmp[N];
+          interp(th, tmp);
+          return M::cb_g(k, tmp, p, tprev + th * dt);
+        };
+        double th_ev = 2.0;
+        for (int k = 0; k < kNcb; k++) {
+          if (M::cb_kind(k) != 0) continue;
+          const int ip = M::cb_interp(k);
+          const int dir = M::cb_dir(k);
+          double ga = gprev[k], tha = 0.0;
+          for (int q = 1; q <= ip; q++) {
+            double thb, gb;
+            if (q == ip) {
+              thb = 1.0;
+              gb = M::cb_g(k, u1, p, t);
+            } else {
+              thb = (double)q / (double)ip;
+              gb = G(k, thb);
+            }
+            bool up = ga < 0.0 && gb >= 0.0;
+            bool down = ga > 0.0 && gb <= 0.0;
+            if ((up && dir >= 0) || (down && dir <= 0)) {
+              double th = thb;
+              if (gb != 0.0) {
+                double lo = tha, hi = thb, flo = ga, fhi = gb;
+                int side = 0;
+                for (int it = 0; it < 100; it++) {
+                  if (hi - lo <= 4.0 * 2.220446049250313e-16) break;
+                  double c = (lo * fhi - hi * flo) / (fhi - flo);
+                  if (!(c > lo && c < hi)) c = 0.5 * (lo + hi);
+                  double fc = G(k, c);
+                  if (fc == 0.0) {
+                    hi = c;
+                    break;
+                  }
+                  if ((fc < 0.0) == (fhi < 0.0)) {
+                    hi = c;
+                    fhi = fc;
+                    if (side == -1) flo = 0.5 * flo;
+                    side = -1;
+                  } else {
+                    lo = c;
+                    flo = fc;
+                    if (side == 1) fhi = 0.5 * fhi;
+                    side = 1;
+                  }
+                }
+                th = hi;
+              }
+              if (th < th_ev) {
+                th_ev = th;
+                ev = k;
+              }
+              break;
+            }
+            ga = gb;
+            tha = thb;
+          }
+        }
+        if (ev >= 0 && th_ev < 1.0) {
+          t = tprev + th_ev * dt;
+          tcomp = 0.0;
+          interp(th_ev, uend);
+        }
+      }
       // savevalues!
       {
         while (tnext <= t) {
@@ -855,54 +1010,9 @@ struct Lane {
             double sa = tnext;
             if (sa == t) {
 #pragma unroll
-              for (int c = 0; c < N; c++) val[c] = u1[c];
-            } else if (tsit_step) {
-              double th = (sa - tprev) / dt;
-              double b[7];
-              tsit5_bthetas(th, b);
-              if constexpr (kHasTsit5 && !MIX) {
-#pragma unroll
-                for (int c = 0; c < N; c++) {
-                  double acc = b[6] * k64[6][c];
-#pragma unroll
-                  for (int j = 5; j >= 0; j--) acc = fma(b[j], k64[j][c], acc);
-                  val[c] = fma(dt, acc, u[c]);
-                }
-              } else if constexpr (kHasTsit5 && MIX) {
-                float bf[7];
-#pragma unroll
-                for (int j = 0; j < 7; j++) bf[j] = (float)b[j];
-#pragma unroll
-                for (int c = 0; c < N; c++) {
-                  float acc = bf[6] * k32[6][c];
-#pragma unroll
-                  for (int j = 5; j >= 0; j--) acc = fmaf(bf[j], k32[j][c], acc);
-                  val[c] = fma(dt, (double)acc, u[c]);
-                }
-              }
-            } else if constexpr (kHasRodas && !kDense) {
-              // not reached: saveat runs the kDense build
-#pragma unroll
-              for (int c = 0; c < N; c++) val[c] = __builtin_nan("");
-            } else if constexpr (kHasRodas) {
-              // Rosenbrock dense output (rd_H), FP64 in both precisions
-              double th = (sa - tprev) / dt;
-              double th1 = 1.0 - th;
-#pragma unroll
-              for (int c = 0; c < N; c++) {
-                double K[3];
-#pragma unroll
-                for (int m = 0; m < 3; m++) {
-                  double acc = rd_H<kP>(m, 0) * kd[0][c];
-#pragma unroll
-                  for (int j = 1; j < 7; j++) acc = fma(rd_H<kP>(m, j), kd[j][c], acc);
-                  K[m] = acc;
-                }
-                double inner = fma(th, K[2], K[1]);
-                inner = fma(th, inner, K[0]);
-                double tt = fma(th1, inner, u1[c]);
-                val[c] = fma(th, tt, th1 * u[c]);
-              }
+              for (int c = 0; c < N; c++) val[c] = uend[c];
+            } else {
+              interp((sa - tprev) / dt, val);
             }
             store_save(a, isave, val);
           }
@@ -911,7 +1021,7 @@ struct Lane {
         }
       }
 #pragma unroll
-      for (int c = 0; c < N; c++) u[c] = u1[c];
+      for (int c = 0; c < N; c++) u[c] = uend[c];
       if (tsit_step) {
         if constexpr (kHasTsit5 && !MIX) {
 #pragma unroll
@@ -919,6 +1029,37 @@ struct Lane {
         } else if constexpr (kHasTsit5 && MIX) {
 #pragma unroll
           for (int c = 0; c < N; c++) k1f[c] = k32[6][c];
+        }
+      }
+      if constexpr (kNcb > 0) {
+        // apply_callback!: the continuous event's affect at its time, then
+        // every discrete callback whose condition holds at the step end;
+        // any affect re-evaluates the FSAL stage (the state may have moved)
+        bool fired = false, term = false;
+        if (ev >= 0) {
+          M::cb_affect(ev, u, p, t);
+          fired = true;
+          term = term || M::cb_term(ev);
+          nevent++;
+        }
+        for (int k = 0; k < kNcb; k++) {
+          if (M::cb_kind(k) == 1 && M::cb_g(k, u, p, t) != 0.0) {
+            M::cb_affect(k, u, p, t);
+            fired = true;
+            term = term || M::cb_term(k);
+            nevent++;
+          }
+        }
+        if (fired) {
+          tevent = t;
+          fsal_init();
+          nf += 1;
+        }
+        for (int k = 0; k < kNcb; k++)
+          if (M::cb_kind(k) == 0) gprev[k] = M::cb_g(k, u, p, t);
+        if (term) {
+          rc = RC_TERMINATED;
+          return;
         }
       }
       bool bad = false;
@@ -962,6 +1103,8 @@ struct Lane {
     if (a.nf) a.nf[i] = nf;
     if (a.nstiff) a.nstiff[i] = nstiff;
     if (a.dt_first) a.dt_first[i] = dtfirst;
+    if (a.nevent) a.nevent[i] = nevent;
+    if (a.t_event) a.t_event[i] = tevent;
   }
 };
 

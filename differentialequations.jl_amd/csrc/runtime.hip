@@ -34,7 +34,7 @@ struct DevBuf {
 enum {
   BUF_U0 = 0, BUF_P, BUF_T0, BUF_TF, BUF_NVEC, BUF_SAVEAT, BUF_DT0, BUF_HINT, BUF_US,
   BUF_UF, BUF_TFIN, BUF_RC, BUF_NA, BUF_NR, BUF_NF, BUF_NS, BUF_SCRATCH, BUF_ORDER,
-  BUF_SAMPLE, BUF_DTF, BUF_HANDOFF, BUF_COUNT
+  BUF_SAMPLE, BUF_DTF, BUF_HANDOFF, BUF_NEV, BUF_TEV, BUF_COUNT
 };
 
 // A run-time compiled right-hand side: its generated model source and the
@@ -46,6 +46,7 @@ struct UserKernels {
 };
 struct UserModel {
   int n = 0, np = 0;
+  int ncb = 0;  // callbacks compiled into the model
   std::string model_src;
   std::map<int, UserKernels> kernels;
 };
@@ -214,6 +215,22 @@ static int check_user_spec(const mideq_user_model* m, std::string* why) {
     *why = b;
     return MIDEQ_E_ARG;
   }
+  if (m->ncallbacks < 0 || m->ncallbacks > MIDEQ_USER_NCBMAX ||
+      (m->ncallbacks > 0 && !m->callbacks)) {
+    *why = "bad callback count or null callbacks";
+    return MIDEQ_E_ARG;
+  }
+  for (int k = 0; k < m->ncallbacks; k++) {
+    const mideq_callback& cb = m->callbacks[k];
+    if (!cb.condition || cb.kind < 0 || cb.kind > 1 || cb.direction < -1 || cb.direction > 1 ||
+        cb.interp_points < 0 || cb.interp_points > 1000 || cb.terminate < 0 ||
+        cb.terminate > 1) {
+      char b[96];
+      snprintf(b, sizeof b, "callback %d: bad kind, direction, interp_points or condition", k);
+      *why = b;
+      return MIDEQ_E_ARG;
+    }
+  }
   return MIDEQ_OK;
 }
 
@@ -239,7 +256,47 @@ static std::string user_model_source(const mideq_user_model* m) {
          "    for (int i = 0; i < N; i++)\n"
          "      for (int j = 0; j < N; j++) J[i][j] = fd[i].d[j];\n";
   }
-  s += "  }\n};\n}  // namespace mideq\n";
+  s += "  }\n";
+  // callbacks: kind / direction / terminate / interp points as switches
+  const int ncb = m->ncallbacks;
+  if (ncb > 0) {
+    s += "  static constexpr int NCB = " + std::to_string(ncb) + ";\n";
+    auto table = [&](const char* name, auto field) {
+      std::string t = "  __device__ static constexpr int " + std::string(name) +
+                      "(int k) {\n    switch (k) {\n";
+      for (int k = 0; k < ncb; k++)
+        t += "      case " + std::to_string(k) + ": return " +
+             std::to_string(field(m->callbacks[k])) + ";\n";
+      t += "    }\n    return 0;\n  }\n";
+      return t;
+    };
+    s += table("cb_kind", [](const mideq_callback& c) { return (int)c.kind; });
+    s += table("cb_dir", [](const mideq_callback& c) { return (int)c.direction; });
+    s += table("cb_term", [](const mideq_callback& c) { return (int)c.terminate; });
+    s += table("cb_interp", [](const mideq_callback& c) {
+      return c.interp_points > 0 ? (int)c.interp_points : 10;
+    });
+    s += "  __device__ static double cb_g(int k, const double* u, const double* p, double t) {\n"
+         "    switch (k) {\n";
+    for (int k = 0; k < ncb; k++) {
+      s += "      case " + std::to_string(k) + ":\n#line 1 \"condition" + std::to_string(k) +
+           "\"\n        return (double)(" + std::string(m->callbacks[k].condition) + ");\n";
+    }
+    s += "    }\n    return 0.0;\n  }\n";
+    s += "  __device__ static void cb_affect(int k, double* u, const double* p, double t) {\n"
+         "    switch (k) {\n";
+    for (int k = 0; k < ncb; k++) {
+      s += "      case " + std::to_string(k) + ": {\n";
+      if (m->callbacks[k].affect) {
+        s += "#line 1 \"affect" + std::to_string(k) + "\"\n";
+        s += m->callbacks[k].affect;
+        s += "\n";
+      }
+      s += "      } break;\n";
+    }
+    s += "    }\n  }\n";
+  }
+  s += "};\n}  // namespace mideq\n";
   return s;
 }
 
@@ -318,8 +375,9 @@ static int user_kernels(mideq_ctx* c, UserModel* um, int alg, int mix, int fast,
 
 // Template ALG of a lane launch: Rosenbrock methods with saveat use the
 // build with the dense output.
-static int lane_alg_code(const LaneArgs& a, int alg) {
-  return (a.ns > 0 && a.us && alg != ALG_TSIT5 && alg != ALG_AUTO_T) ? (alg | ALG_DENSE) : alg;
+static int lane_alg_code(const LaneArgs& a, int alg, bool callbacks = false) {
+  const bool dense = (a.ns > 0 && a.us) || callbacks;
+  return (dense && alg != ALG_TSIT5 && alg != ALG_AUTO_T) ? (alg | ALG_DENSE) : alg;
 }
 
 // The lane-kernel launch of launch_lane_solver for a user model whose
@@ -355,7 +413,8 @@ extern "C" int mideq_model_check(const mideq_user_model* m, char* log, size_t lo
     std::vector<char> code;
     // AutoTsit5 with FP32 stages instantiates f for double, float and (for
     // the Jacobian) dual numbers
-    rc = rtc_compile(user_model_source(m), "gfx950", ALG_AUTO, 1, 0, &code, &why);
+    rc = rtc_compile(user_model_source(m), "gfx950",
+                     ALG_AUTO | (m->ncallbacks > 0 ? ALG_DENSE : 0), 1, 0, &code, &why);
   }
   if (log && log_size) {
     snprintf(log, log_size, "%s", why.c_str());
@@ -373,13 +432,14 @@ extern "C" int mideq_model_register(mideq_ctx* c, const mideq_user_model* m,
   UserModel um;
   um.n = m->n;
   um.np = m->np > 0 ? m->np : 1;  // the kernel reads at least one parameter row
+  um.ncb = m->ncallbacks;
   um.model_src = user_model_source(m);
   c->user_models.push_back(um);
   UserModel* p = &c->user_models.back();
   // compile one configuration now, so that errors surface here: AutoTsit5
   // with FP32 stages instantiates f for double, float and dual numbers
   UserKernels* uk = nullptr;
-  rc = user_kernels(c, p, ALG_AUTO, 1, 0, &uk);
+  rc = user_kernels(c, p, ALG_AUTO | (p->ncb > 0 ? ALG_DENSE : 0), 1, 0, &uk);
   if (rc) {
     for (auto& kv : p->kernels)
       if (kv.second.mod) (void)hipModuleUnload(kv.second.mod);
@@ -667,6 +727,8 @@ static int run_solver(mideq_ctx* c, const mideq_problem* P, const mideq_options*
   a.nf = out->nf;
   a.nstiff = out->nstiff;
   a.dt_first = out->dt_first;
+  a.nevent = out->nevent;
+  a.t_event = out->t_event;
   a.queue = c->d_queue;
   a.order = d_order;
   int sched = O->schedule == MIDEQ_SCHED_AUTO ? SCHED_LANE : O->schedule;
@@ -706,8 +768,8 @@ static int run_solver(mideq_ctx* c, const mideq_problem* P, const mideq_options*
   UserKernels* uk_main = nullptr;
   UserKernels* uk_first = nullptr;
   if (um) {
-    int rc0 = user_kernels(c, um, lane_alg_code(a, O->alg), O->precision, O->controller,
-                           &uk_main);
+    int rc0 = user_kernels(c, um, lane_alg_code(a, O->alg, um->ncb > 0), O->precision,
+                           O->controller, &uk_main);
     if (rc0 == MIDEQ_OK && two_pass)
       rc0 = user_kernels(c, um, ALG_AUTO_T, O->precision, O->controller, &uk_first);
     if (rc0) return rc0;
@@ -879,6 +941,8 @@ extern "C" int mideq_ensemble_solve(mideq_ctx* c, const mideq_problem* P,
   if (out->nf && (rc = ensure_buf(c, BUF_NF, 4 * N, (void**)&dOut.nf))) return rc;
   if (out->nstiff && (rc = ensure_buf(c, BUF_NS, 4 * N, (void**)&dOut.nstiff))) return rc;
   if (out->dt_first && (rc = ensure_buf(c, BUF_DTF, 8 * N, (void**)&dOut.dt_first))) return rc;
+  if (out->nevent && (rc = ensure_buf(c, BUF_NEV, 4 * N, (void**)&dOut.nevent))) return rc;
+  if (out->t_event && (rc = ensure_buf(c, BUF_TEV, 8 * N, (void**)&dOut.t_event))) return rc;
 
   rc = run_solver(c, &dP, &dO, &dOut, d_order, s);
   if (rc) return rc;
@@ -894,6 +958,8 @@ extern "C" int mideq_ensemble_solve(mideq_ctx* c, const mideq_problem* P,
   if (out->nf) HIP_TRY(c, hipMemcpyAsync(out->nf, dOut.nf, 4 * N, hipMemcpyDeviceToHost, s));
   if (out->nstiff) HIP_TRY(c, hipMemcpyAsync(out->nstiff, dOut.nstiff, 4 * N, hipMemcpyDeviceToHost, s));
   if (out->dt_first) HIP_TRY(c, hipMemcpyAsync(out->dt_first, dOut.dt_first, 8 * N, hipMemcpyDeviceToHost, s));
+  if (out->nevent) HIP_TRY(c, hipMemcpyAsync(out->nevent, dOut.nevent, 4 * N, hipMemcpyDeviceToHost, s));
+  if (out->t_event) HIP_TRY(c, hipMemcpyAsync(out->t_event, dOut.t_event, 8 * N, hipMemcpyDeviceToHost, s));
   HIP_TRY(c, hipEventRecord(t1e, s));
   HIP_TRY(c, hipStreamSynchronize(s));
   float ms = 0.f;

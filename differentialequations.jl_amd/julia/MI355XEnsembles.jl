@@ -32,7 +32,7 @@ import SciMLBase: __solve, EnsembleSolution, ReturnCode
 # method types re-exported by DifferentialEquations (test/qa/qa.jl:15,97,119)
 using OrdinaryDiffEq: Tsit5, Rodas5, Rodas5P, CompositeAlgorithm
 
-export EnsembleMI355X, register_model!
+export EnsembleMI355X, register_model!, DeviceCallback
 
 const LIB = Ref{String}(get(ENV, "MIDEQ_LIB",
     joinpath(@__DIR__, "..", "_build", "libmideq.so")))
@@ -69,6 +69,7 @@ struct MideqOutputs
     us::Ptr{Cvoid}; u_final::Ptr{Float64}; t_final::Ptr{Float64}; retcode::Ptr{Int8}
     naccept::Ptr{Int32}; nreject::Ptr{Int32}; nf::Ptr{Int32}; nstiff::Ptr{Int32}
     dt_first::Ptr{Float64}
+    nevent::Ptr{Int32}; t_event::Ptr{Float64}
 end
 
 # ---- registered right-hand sides ------------------------------------------
@@ -79,9 +80,35 @@ const MODEL_IDS = Dict(:exp => 0, :lorenz => 1, :robertson => 2, :vdp => 3,
 const MODEL_FOR = IdDict{Any, Int32}()
 register_model!(f, name::Symbol) = (MODEL_FOR[f] = MODEL_IDS[name]; f)
 
+struct MideqCallback
+    kind::Int32; direction::Int32; terminate::Int32; interp_points::Int32
+    condition::Ptr{UInt8}; affect::Ptr{UInt8}
+end
+
 struct MideqUserModel
     n::Int32; np::Int32; rhs::Ptr{UInt8}; jac::Ptr{UInt8}
+    ncallbacks::Int32; reserved::Int32; callbacks::Ptr{MideqCallback}
 end
+
+"""
+    DeviceCallback(kind, condition; affect = nothing, direction = 0,
+                   terminate = false, interp_points = 10)
+
+The device form of `ContinuousCallback(condition, affect!)` (`kind = :continuous`)
+or `DiscreteCallback(condition, affect!)` (`kind = :discrete`) (test/qa/qa.jl:26,36):
+`condition` a C++ expression of `u[]`, `p[]`, `t`; `affect` statements changing
+`u[]`; compiled into a model by `register_model!(...; callbacks)`.
+"""
+Base.@kwdef struct DeviceCallback
+    kind::Symbol
+    condition::String
+    affect::Union{Nothing, String} = nothing
+    direction::Int = 0
+    terminate::Bool = false
+    interp_points::Int = 10
+end
+DeviceCallback(kind::Symbol, condition::AbstractString; kw...) =
+    DeviceCallback(; kind, condition = String(condition), kw...)
 
 """
     register_model!(f!, n, np, rhs; jac = nothing, device = 0)
@@ -99,15 +126,26 @@ The model id belongs to `device`'s context.  Example (test/core.jl:22-27):
         du[2] = u[0] * u[1] - p[2] * u[2];""")
 """
 function register_model!(f, n::Integer, np::Integer, rhs::AbstractString;
-                         jac::Union{Nothing, AbstractString} = nothing, device::Int = 0)
+                         jac::Union{Nothing, AbstractString} = nothing,
+                         callbacks::AbstractVector{DeviceCallback} = DeviceCallback[],
+                         device::Int = 0)
     ctx = context(device)
     id = Ref{Int32}(0)
     r = String(rhs)
     j = jac === nothing ? nothing : String(jac)
-    rc = GC.@preserve r j begin
-        m = MideqUserModel(n, np, pointer(r), j === nothing ? C_NULL : pointer(j))
-        ccall((:mideq_model_register, LIB[]), Cint,
-              (Ptr{Cvoid}, Ref{MideqUserModel}, Ref{Int32}), ctx.h, m, id)
+    conds = [cb.condition for cb in callbacks]
+    affs = [cb.affect === nothing ? "" : cb.affect for cb in callbacks]
+    rc = GC.@preserve r j conds affs begin
+        cbs = [MideqCallback(cb.kind === :discrete ? 1 : 0, cb.direction, cb.terminate,
+                             cb.interp_points, pointer(conds[k]),
+                             cb.affect === nothing ? C_NULL : pointer(affs[k]))
+               for (k, cb) in enumerate(callbacks)]
+        GC.@preserve cbs begin
+            m = MideqUserModel(n, np, pointer(r), j === nothing ? C_NULL : pointer(j),
+                               length(cbs), 0, isempty(cbs) ? C_NULL : pointer(cbs))
+            ccall((:mideq_model_register, LIB[]), Cint,
+                  (Ptr{Cvoid}, Ref{MideqUserModel}, Ref{Int32}), ctx.h, m, id)
+        end
     end
     check(ctx, rc, "mideq_model_register")
     MODEL_FOR[f] = id[]
@@ -125,7 +163,7 @@ alg_code(alg) = error("EnsembleMI355X supports Tsit5, Rodas5, Rodas5P and AutoTs
 
 const RETCODES = (ReturnCode.Default, ReturnCode.Success, ReturnCode.MaxIters,
                   ReturnCode.DtLessThanMin, ReturnCode.Unstable, ReturnCode.DtNaN,
-                  ReturnCode.InitialFailure)
+                  ReturnCode.InitialFailure, ReturnCode.Terminated)
 
 mutable struct Ctx
     h::Ptr{Cvoid}
@@ -205,7 +243,7 @@ function __solve(ens::SciMLBase.AbstractEnsembleProblem, alg, ealg::EnsembleMI35
                                  0, 0, 0, 0, 0, 0, C_NULL)
             out_c = MideqOutputs(ns > 0 ? pointer(US) : C_NULL, pointer(UF), pointer(TFo),
                                  pointer(RC), pointer(NA), pointer(NR), pointer(NF), pointer(NS),
-                                 C_NULL)
+                                 C_NULL, C_NULL, C_NULL)
             rc = ccall((:mideq_ensemble_solve, LIB[]), Cint,
                        (Ptr{Cvoid}, Ref{MideqProblem}, Ref{MideqOptions}, Ref{MideqOutputs}),
                        ctx.h, prob_c, opt_c, out_c)

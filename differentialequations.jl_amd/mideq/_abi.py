@@ -33,12 +33,19 @@ class Problem(C.Structure):
                 ("tspan0", C.c_double), ("tspan1", C.c_double), ("nvec", C.c_void_p)]
 
 
+class CallbackSpec(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("direction", C.c_int32), ("terminate", C.c_int32),
+                ("interp_points", C.c_int32), ("condition", C.c_char_p), ("affect", C.c_char_p)]
+
+
 class UserModelSpec(C.Structure):
-    _fields_ = [("n", C.c_int32), ("np", C.c_int32), ("rhs", C.c_char_p), ("jac", C.c_char_p)]
+    _fields_ = [("n", C.c_int32), ("np", C.c_int32), ("rhs", C.c_char_p), ("jac", C.c_char_p),
+                ("ncallbacks", C.c_int32), ("reserved", C.c_int32),
+                ("callbacks", C.POINTER(CallbackSpec))]
 
 
 USER_MODEL_BASE = 1000
-USER_NMAX, USER_NPMAX = 8, 16
+USER_NMAX, USER_NPMAX, USER_NCBMAX = 8, 16, 8
 
 
 class Options(C.Structure):
@@ -55,7 +62,8 @@ class Options(C.Structure):
 class Outputs(C.Structure):
     _fields_ = [("us", C.c_void_p), ("u_final", C.c_void_p), ("t_final", C.c_void_p),
                 ("retcode", C.c_void_p), ("naccept", C.c_void_p), ("nreject", C.c_void_p),
-                ("nf", C.c_void_p), ("nstiff", C.c_void_p), ("dt_first", C.c_void_p)]
+                ("nf", C.c_void_p), ("nstiff", C.c_void_p), ("dt_first", C.c_void_p),
+                ("nevent", C.c_void_p), ("t_event", C.c_void_p)]
 
 
 class Stats(C.Structure):

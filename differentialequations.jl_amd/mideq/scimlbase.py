@@ -38,6 +38,7 @@ class ReturnCode(enum.IntEnum):
     Unstable = 4
     DtNaN = 5
     InitialFailure = 6
+    Terminated = 7
 
 
 def successful_retcode(rc):
@@ -91,6 +92,48 @@ class ODEFunction:
         raise ValueError(self.name)
 
 
+@dataclass(frozen=True)
+class ContinuousCallback:
+    """ContinuousCallback(condition, affect!) (qa.jl:26) for the GPU:
+    `condition` is a double expression of u[], p[], t whose sign change
+    inside a step is the event; `affect` (statements changing u[]) runs at
+    the located event time.  direction: 0 both crossings, 1 upcrossings,
+    -1 downcrossings; terminate: end the trajectory (ReturnCode.Terminated)
+    after the affect; interp_points: sign checks per step."""
+    condition: str
+    affect: Optional[str] = None
+    direction: int = 0
+    terminate: bool = False
+    interp_points: int = 10
+
+    def spec(self):
+        return _abi.CallbackSpec(0, int(self.direction), int(bool(self.terminate)),
+                                 int(self.interp_points), self.condition.encode(),
+                                 None if self.affect is None else self.affect.encode())
+
+
+@dataclass(frozen=True)
+class DiscreteCallback:
+    """DiscreteCallback(condition, affect!) (qa.jl:36) for the GPU: the
+    affect runs at every step end where `condition` (an expression of u[],
+    p[], t) is nonzero."""
+    condition: str
+    affect: Optional[str] = None
+    terminate: bool = False
+
+    def spec(self):
+        return _abi.CallbackSpec(1, 0, int(bool(self.terminate)), 0, self.condition.encode(),
+                                 None if self.affect is None else self.affect.encode())
+
+
+def CallbackSet(*cbs):
+    """CallbackSet (qa.jl:24): the callbacks as one tuple."""
+    out = []
+    for c in cbs:
+        out += list(c) if isinstance(c, (list, tuple)) else [c]
+    return tuple(out)
+
+
 class UserModel:
     """A right-hand side given as C++ statements, compiled for the GPU at run
     time (hipRTC) -- the counterpart of passing an in-place f!(du, u, p, t)
@@ -102,7 +145,7 @@ class UserModel:
     Rosenbrock methods differentiate rhs in forward mode.  A model with no
     parameters still takes one (unused) parameter row."""
 
-    def __init__(self, n, np_, rhs, jac=None, name="user"):
+    def __init__(self, n, np_, rhs, jac=None, name="user", callbacks=()):
         if not (1 <= n <= _abi.USER_NMAX) or not (0 <= np_ <= _abi.USER_NPMAX):
             raise ValueError(f"user model n={n} np={np_} outside 1..{_abi.USER_NMAX} / "
                              f"0..{_abi.USER_NPMAX}")
@@ -112,11 +155,28 @@ class UserModel:
         self.jac_src = None if jac is None else str(jac)
         self.jac = True
         self.name = name
+        self.callbacks = CallbackSet(callbacks) if callbacks else ()
+        if len(self.callbacks) > _abi.USER_NCBMAX:
+            raise ValueError(f"at most {_abi.USER_NCBMAX} callbacks")
         self._ids = {}
+        self._with = {}
 
     def spec(self):
-        return _abi.UserModelSpec(self.n, self.np, self.rhs.encode(),
-                                  None if self.jac_src is None else self.jac_src.encode())
+        cbs = [c.spec() for c in self.callbacks]
+        arr = (_abi.CallbackSpec * max(1, len(cbs)))(*cbs)
+        sp = _abi.UserModelSpec(self.n, self.np, self.rhs.encode(),
+                                None if self.jac_src is None else self.jac_src.encode(),
+                                len(cbs), 0, arr if cbs else None)
+        sp._keep = arr
+        return sp
+
+    def with_callbacks(self, callbacks):
+        """This model with `callbacks` compiled in (cached)."""
+        cbs = CallbackSet(callbacks)
+        if cbs not in self._with:
+            self._with[cbs] = UserModel(self.n, self.np, self.rhs, self.jac_src,
+                                        name=self.name, callbacks=cbs)
+        return self._with[cbs]
 
     def check(self):
         """Compile without a GPU; returns the compiler log (raises on error)."""
@@ -137,6 +197,43 @@ class UserModel:
 
     def __call__(self, u, p, t=0.0):
         raise NotImplementedError("a user model runs on the GPU only")
+
+
+# Device source of the built-in one-lane models, as user-model statements
+# (the same arithmetic as their compiled form): callbacks compile into them.
+BUILTIN_SOURCE = {
+    "exp": (1, 1, "du[0] = p[0] * u[0];", "J[0][0] = p[0];"),
+    "lorenz": (3, 3, "du[0] = p[0] * (u[1] - u[0]);\ndu[1] = u[0] * (p[1] - u[2]) - u[1];\n"
+                     "du[2] = u[0] * u[1] - p[2] * u[2];",
+               "J[0][0] = -p[0]; J[0][1] = p[0]; J[0][2] = 0.0;\n"
+               "J[1][0] = p[1] - u[2]; J[1][1] = -1.0; J[1][2] = -u[0];\n"
+               "J[2][0] = u[1]; J[2][1] = u[0]; J[2][2] = -p[2];"),
+    "robertson": (3, 3, "T y1 = u[0], y2 = u[1], y3 = u[2];\nT k1 = p[0], k2 = p[1], k3 = p[2];\n"
+                        "du[0] = -k1 * y1 + k3 * y2 * y3;\n"
+                        "du[1] = k1 * y1 - k2 * (y2 * y2) - k3 * y2 * y3;\n"
+                        "du[2] = k2 * (y2 * y2);",
+                  "double y2 = u[1], y3 = u[2], k1 = p[0], k2 = p[1], k3 = p[2];\n"
+                  "J[0][0] = -k1; J[0][1] = k3 * y3; J[0][2] = k3 * y2;\n"
+                  "J[1][0] = k1; J[1][1] = -2.0 * k2 * y2 - k3 * y3; J[1][2] = -k3 * y2;\n"
+                  "J[2][0] = 0.0; J[2][1] = 2.0 * k2 * y2; J[2][2] = 0.0;"),
+    "vdp": (2, 1, "T x = u[0], y = u[1], mu = p[0];\ndu[0] = y;\n"
+                  "du[1] = mu * (T(1) - x * x) * y - x;",
+            "double x = u[0], y = u[1], mu = p[0];\nJ[0][0] = 0.0; J[0][1] = 1.0;\n"
+            "J[1][0] = -2.0 * mu * x * y - 1.0; J[1][1] = mu * (1.0 - x * x);"),
+}
+_BUILTIN_USER = {}
+
+
+def user_model_of(f):
+    """f as a UserModel (built-in one-lane models through their source)."""
+    if isinstance(f, UserModel):
+        return f
+    if f.name not in BUILTIN_SOURCE:
+        raise ValueError(f"callbacks are not available for the {f.name} model")
+    if f.name not in _BUILTIN_USER:
+        n, np_, rhs, jac = BUILTIN_SOURCE[f.name]
+        _BUILTIN_USER[f.name] = UserModel(n, np_, rhs, jac=jac, name=f.name)
+    return _BUILTIN_USER[f.name]
 
 
 exp_growth = ODEFunction("exp", 1, 1)
@@ -298,6 +395,8 @@ class EnsembleArrays:
     total_ms: float = 0.0
     lanes: int = 0
     graph: bool = False
+    nevent: Optional[np.ndarray] = None   # [N] callback affects applied
+    t_event: Optional[np.ndarray] = None  # [N] time of the last one (NaN if none)
 
     def solution(self, i, prob=None):
         """ODESolution of trajectory i (0-based)."""
@@ -385,8 +484,12 @@ def _ptr(a):
 
 def device_solve(f, u0, p, t0, tf, nvec, alg, ealg, saveat=None, abstol=1e-6, reltol=1e-3,
                  dt=0.0, dtmin=0.0, dtmax=0.0, maxiters=0, dt0=None, alg_hint=None,
-                 order=None, block=0, blocks_per_cu=0):
-    """One call of mideq_ensemble_solve on host arrays -> EnsembleArrays."""
+                 order=None, block=0, blocks_per_cu=0, callback=None):
+    """One call of mideq_ensemble_solve on host arrays -> EnsembleArrays.
+    callback: ContinuousCallback / DiscreteCallback / CallbackSet, compiled
+    into the right-hand side (a UserModel, or a built-in model's source)."""
+    if callback:
+        f = user_model_of(f).with_callbacks(callback)
     ctx = _abi.context(ealg.device)
     n, N = u0.shape
     P = _abi.Problem()
@@ -446,6 +549,9 @@ def device_solve(f, u0, p, t0, tf, nvec, alg, ealg, saveat=None, abstol=1e-6, re
     res = dict(u_final=np.zeros((n, N)), t_final=np.empty(N), retcode=np.empty(N, np.int8),
                naccept=np.empty(N, np.int32), nreject=np.empty(N, np.int32),
                nf=np.empty(N, np.int32), nstiff=np.empty(N, np.int32), dt_first=np.empty(N))
+    if isinstance(f, UserModel) and f.callbacks:
+        res["nevent"] = np.zeros(N, np.int32)
+        res["t_event"] = np.empty(N)
     out = _abi.Outputs()
     out.us = _ptr(us)
     for k, v in res.items():

@@ -79,7 +79,8 @@ typedef enum {
   MIDEQ_RC_DTLESSTHANMIN = 3,
   MIDEQ_RC_UNSTABLE = 4,
   MIDEQ_RC_DTNAN = 5,
-  MIDEQ_RC_INITIALFAILURE = 6  /* invalid per-trajectory input (e.g. ragged size out of range) */
+  MIDEQ_RC_INITIALFAILURE = 6, /* invalid per-trajectory input (e.g. ragged size out of range) */
+  MIDEQ_RC_TERMINATED = 7      /* a callback terminated the trajectory (terminate!) */
 } mideq_retcode;
 
 typedef enum {
@@ -156,6 +157,8 @@ typedef struct {
   int32_t *nf;          /* [ntraj] RHS evaluations */
   int32_t *nstiff;      /* [ntraj] accepted Rodas5 steps */
   double *dt_first;     /* [ntraj] size of the first accepted step */
+  int32_t *nevent;      /* [ntraj] callback affects applied (user models with callbacks) */
+  double *t_event;      /* [ntraj] time of the last callback affect (NaN if none) */
 } mideq_outputs;
 
 /* Timing and launch record of the last solve on a context. */
@@ -197,6 +200,28 @@ int mideq_ensemble_solve_device(mideq_ctx *ctx, const mideq_problem *prob,
 #define MIDEQ_USER_MODEL_BASE 1000 /* first id of a registered user model */
 #define MIDEQ_USER_NMAX 8          /* states (one lane per trajectory)     */
 #define MIDEQ_USER_NPMAX 16        /* parameters                           */
+/* Callbacks of a user model.  Replaces ContinuousCallback(condition, affect!)
+ * and DiscreteCallback(condition, affect!) passed as solve(...; callback)
+ * (test/qa/qa.jl:26,36; test/core.jl:60-79).  A continuous callback fires
+ * where its condition changes sign inside an accepted step: the step's dense
+ * output is checked at interp_points sub-intervals, the crossing located by
+ * Illinois bracketing, the integrator moved back to it, and the affect
+ * applied there (the earliest event of all continuous callbacks).  A discrete
+ * callback fires at a step end where its condition is nonzero.  Every affect
+ * re-evaluates the FSAL stage.  Saved values are the saveat grid's (no extra
+ * points at events). */
+#define MIDEQ_USER_NCBMAX 8
+typedef enum { MIDEQ_CB_CONTINUOUS = 0, MIDEQ_CB_DISCRETE = 1 } mideq_callback_kind;
+typedef struct {
+  int32_t kind;          /* mideq_callback_kind */
+  int32_t direction;     /* continuous: 0 both crossings, 1 upcrossings (affect!),
+                            -1 downcrossings (affect_neg!) */
+  int32_t terminate;     /* 1: terminate! after the affect (retcode MIDEQ_RC_TERMINATED) */
+  int32_t interp_points; /* continuous: sign checks per step, 0 = 10 (the upstream default) */
+  const char *condition; /* double expression of u[], p[], t */
+  const char *affect;    /* statements changing u[] (double) from u, p, t; NULL: none */
+} mideq_callback;
+
 typedef struct {
   int32_t n;        /* states, 1..MIDEQ_USER_NMAX */
   int32_t np;       /* parameters, 0..MIDEQ_USER_NPMAX */
@@ -207,6 +232,9 @@ typedef struct {
   const char *jac;  /* statements assigning double J[i][j] = df_i/du_j from u[], p[];
                        NULL: forward-mode automatic differentiation of rhs (the default
                        of OrdinaryDiffEq's Rosenbrock methods without a Jacobian) */
+  int32_t ncallbacks;                /* 0..MIDEQ_USER_NCBMAX */
+  int32_t reserved;
+  const mideq_callback *callbacks;   /* [ncallbacks] or NULL */
 } mideq_user_model;
 
 /* Registers and compiles a user model on the context's device; *model_id

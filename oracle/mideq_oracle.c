@@ -321,7 +321,27 @@ static struct {
   oracle_user_f64 f64;
   oracle_user_f32 f32;
   oracle_user_jac jac;
+  int ncb;
+  int kind[8], dir[8], term[8], ip[8];
+  oracle_user_cond cond;
+  oracle_user_affect affect;
 } g_user;
+
+int oracle_set_user_callbacks(int32_t ncb, const int32_t *kind, const int32_t *direction,
+                              const int32_t *terminate, const int32_t *interp_points,
+                              oracle_user_cond cond, oracle_user_affect affect) {
+  if (ncb < 0 || ncb > 8 || (ncb > 0 && (!cond || !affect))) return -1;
+  g_user.ncb = ncb;
+  for (int k = 0; k < ncb; k++) {
+    g_user.kind[k] = kind[k];
+    g_user.dir[k] = direction[k];
+    g_user.term[k] = terminate[k];
+    g_user.ip[k] = interp_points[k] > 0 ? interp_points[k] : 10;
+  }
+  g_user.cond = cond;
+  g_user.affect = affect;
+  return 0;
+}
 
 int oracle_set_user_model(int32_t n, int32_t np, oracle_user_f64 f64, oracle_user_f32 f32,
                           oracle_user_jac jac) {
@@ -1105,6 +1125,21 @@ int oracle_rodas5_step(int32_t model, int32_t n, const double *u, const double *
   return oracle_rosenbrock_step(0, model, n, u, p, t, dt, abstol, reltol, u1, eest);
 }
 
+/* Dense output of the last step at theta: Tsit5's interpolant (FP64 or    */
+/* mixed) in Tsit5 mode, the Rosenbrock one in Rosenbrock mode.             */
+static void step_interp(int n, int mode, int mixed, const rdtab_t *rtab, const double *uprev,
+                        const double *u1, double k64[7][NMAX], float k32[7][NMAX],
+                        double kd[8][NMAX], double dt, double th, double *out) {
+  if (mode == 0) {
+    if (!mixed)
+      tsit5_interp64(n, uprev, k64, dt, th, out);
+    else
+      tsit5_interp_mix(n, uprev, k32, dt, th, out);
+  } else {
+    rosenbrock_interp(n, rtab, uprev, u1, kd, th, out);
+  }
+}
+
 /* ------------------------------------------------------------------------ */
 /* The integrator loop (solve! with loopheader!/perform_step!/loopfooter!). */
 /* ------------------------------------------------------------------------ */
@@ -1132,6 +1167,9 @@ static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
 
   memset(res, 0, sizeof(*res));
   res->first_stiff_iter = -1;
+  res->t_event = NAN;
+  const int ncb = (m->model == ORACLE_MODEL_USER) ? g_user.ncb : 0;
+  double gprev[8];
   for (int c = 0; c < n; c++) uprev[c] = u0[c];
   double t = t0, tcomp = 0.0;
   int isave = 0;
@@ -1160,6 +1198,7 @@ static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
   }
   for (int c = 0; c < n; c++) k64[0][c] = f0[c];
 
+  for (int k = 0; k < ncb; k++) gprev[k] = g_user.cond(k, uprev, m->p, t0);
   double dt = o->dt > 0 ? o->dt
                : o->dt0_in ? o->dt0_in[idx]
                            : initdt_impl(m, uprev, f0, t0, dtmax, o, 5, &res->nf);
@@ -1301,32 +1340,134 @@ static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
       }
       if (fabs(ttmp - tf) < 100.0 * jl_eps(fmax(t, tf))) ttmp = tf;
       t = ttmp;
+      /* the state the step ends in: u1, or the dense output at an event */
+      double uend[NMAX];
+      for (int c = 0; c < n; c++) uend[c] = u1[c];
+      int ev = -1;
+      if (ncb > 0) {
+        /* continuous callbacks: the earliest sign change of any condition
+         * over interp_points sub-intervals, Illinois (modified regula falsi)
+         * bracketing on theta; the event is the bracket end past it */
+        double th_ev = 2.0;
+        for (int k = 0; k < ncb; k++) {
+          if (g_user.kind[k] != 0) continue;
+          int ip = g_user.ip[k], dir = g_user.dir[k];
+          double ga = gprev[k], tha = 0.0;
+          for (int q = 1; q <= ip; q++) {
+            double thb, gb;
+            if (q == ip) {
+              thb = 1.0;
+              gb = g_user.cond(k, u1, m->p, t);
+            } else {
+              double tmp[NMAX];
+              thb = (double)q / (double)ip;
+              step_interp(n, mode, mixed, rtab, uprev, u1, k64, k32, kdsave, dt, thb, tmp);
+              gb = g_user.cond(k, tmp, m->p, tprev + thb * dt);
+            }
+            int up = ga < 0.0 && gb >= 0.0;
+            int down = ga > 0.0 && gb <= 0.0;
+            if ((up && dir >= 0) || (down && dir <= 0)) {
+              double th = thb;
+              if (gb != 0.0) {
+                double lo = tha, hi = thb, flo = ga, fhi = gb;
+                int side = 0;
+                for (int it = 0; it < 100; it++) {
+                  if (hi - lo <= 4.0 * 2.220446049250313e-16) break;
+                  double cc = (lo * fhi - hi * flo) / (fhi - flo);
+                  if (!(cc > lo && cc < hi)) cc = 0.5 * (lo + hi);
+                  double tmp[NMAX];
+                  step_interp(n, mode, mixed, rtab, uprev, u1, k64, k32, kdsave, dt, cc, tmp);
+                  double fc = g_user.cond(k, tmp, m->p, tprev + cc * dt);
+                  if (fc == 0.0) {
+                    hi = cc;
+                    break;
+                  }
+                  if ((fc < 0.0) == (fhi < 0.0)) {
+                    hi = cc;
+                    fhi = fc;
+                    if (side == -1) flo = 0.5 * flo;
+                    side = -1;
+                  } else {
+                    lo = cc;
+                    flo = fc;
+                    if (side == 1) fhi = 0.5 * fhi;
+                    side = 1;
+                  }
+                }
+                th = hi;
+              }
+              if (th < th_ev) {
+                th_ev = th;
+                ev = k;
+              }
+              break;
+            }
+            ga = gb;
+            tha = thb;
+          }
+        }
+        if (ev >= 0 && th_ev < 1.0) {
+          t = tprev + th_ev * dt;
+          tcomp = 0.0;
+          step_interp(n, mode, mixed, rtab, uprev, u1, k64, k32, kdsave, dt, th_ev, uend);
+        }
+      }
       /* savevalues!: every saveat point <= t */
       while (isave < ns && sa[isave] <= t) {
         if (save) {
           double val[NMAX];
           if (sa[isave] == t) {
-            for (int c = 0; c < n; c++) val[c] = u1[c];
-          } else if (mode == 0) {
-            double th = (sa[isave] - tprev) / dt;
-            if (!mixed)
-              tsit5_interp64(n, uprev, k64, dt, th, val);
-            else
-              tsit5_interp_mix(n, uprev, k32, dt, th, val);
+            for (int c = 0; c < n; c++) val[c] = uend[c];
           } else {
             double th = (sa[isave] - tprev) / dt;
-            rosenbrock_interp(n, rtab, uprev, u1, kdsave, th, val);
+            step_interp(n, mode, mixed, rtab, uprev, u1, k64, k32, kdsave, dt, th, val);
           }
           for (int c = 0; c < n; c++) save[((long)isave * n + c) * save_stride] = val[c];
         }
         isave++;
       }
-      for (int c = 0; c < n; c++) uprev[c] = u1[c];
+      for (int c = 0; c < n; c++) uprev[c] = uend[c];
       if (mode == 0) {
         if (!mixed)
           for (int c = 0; c < n; c++) k64[0][c] = k64[6][c];
         else
           for (int c = 0; c < n; c++) k32[0][c] = k32[6][c];
+      }
+      if (ncb > 0) {
+        /* apply_callback!: the continuous event's affect, then every
+         * discrete callback whose condition holds; any affect re-evaluates
+         * the FSAL stage */
+        int fired = 0, term = 0;
+        if (ev >= 0) {
+          g_user.affect(ev, uprev, m->p, t);
+          fired = 1;
+          term |= g_user.term[ev];
+          res->nevent++;
+        }
+        for (int k = 0; k < ncb; k++) {
+          if (g_user.kind[k] == 1 && g_user.cond(k, uprev, m->p, t) != 0.0) {
+            g_user.affect(k, uprev, m->p, t);
+            fired = 1;
+            term |= g_user.term[k];
+            res->nevent++;
+          }
+        }
+        if (fired) {
+          res->t_event = t;
+          rhs64(m, uprev, t, k64[0]);
+          if (mixed) {
+            float uf[NMAX];
+            for (int c = 0; c < n; c++) uf[c] = (float)uprev[c];
+            rhs32(m, uf, t, k32[0]);
+          }
+          res->nf += 1;
+        }
+        for (int k = 0; k < ncb; k++)
+          if (g_user.kind[k] == 0) gprev[k] = g_user.cond(k, uprev, m->p, t);
+        if (term) {
+          rc = ORACLE_RC_TERMINATED;
+          break;
+        }
       }
       int bad = 0;
       for (int c = 0; c < n; c++) bad |= (u1[c] != u1[c]);
@@ -1382,6 +1523,8 @@ typedef struct {
   int8_t *retcode;
   int32_t *naccept, *nreject, *nf, *nstiff;
   double *dt_first;
+  int32_t *nevent;
+  double *t_event;
   atomic_llong next;
 } ens_job;
 
@@ -1422,6 +1565,8 @@ static void *ens_worker(void *arg) {
       if (J->dt0) J->dt0[i] = res.dt0;
       if (J->nstiff) J->nstiff[i] = res.nstiff_steps;
       if (J->dt_first) J->dt_first[i] = res.dt_first_accept;
+      if (J->nevent) J->nevent[i] = res.nevent;
+      if (J->t_event) J->t_event[i] = res.t_event;
     }
   }
   free(pl);
@@ -1435,7 +1580,8 @@ int oracle_ensemble_solve(int32_t model, int32_t n, int32_t np, int64_t ntraj,
                           const oracle_options *opt, int32_t nthreads,
                           double *out_save, double *u_final, double *t_final,
                           int8_t *retcode, int32_t *naccept, int32_t *nreject,
-                          int32_t *nf, double *dt0, int32_t *nstiff, double *dt_first) {
+                          int32_t *nf, double *dt0, int32_t *nstiff, double *dt_first,
+                          int32_t *nevent, double *t_event) {
   if (n < 1 || n > NMAX || ntraj < 0) return -1;
   ens_job J;
   memset(&J, 0, sizeof(J));
@@ -1459,6 +1605,8 @@ int oracle_ensemble_solve(int32_t model, int32_t n, int32_t np, int64_t ntraj,
   J.dt0 = dt0;
   J.nstiff = nstiff;
   J.dt_first = dt_first;
+  J.nevent = nevent;
+  J.t_event = t_event;
   atomic_init(&J.next, 0);
   if (nthreads < 1) nthreads = 1;
   if (nthreads == 1 || ntraj <= 1) {

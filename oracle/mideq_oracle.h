@@ -62,6 +62,15 @@ typedef void (*oracle_user_f32)(float *du, const float *u, const float *p);
 typedef void (*oracle_user_jac)(double *J, const double *u, const double *p);
 int oracle_set_user_model(int32_t n, int32_t np, oracle_user_f64 f64, oracle_user_f32 f32,
                           oracle_user_jac jac);
+/* Callbacks of the user model (ncb = 0 clears them): per callback the kind
+ * (0 continuous, 1 discrete), direction (0 both, 1 up, -1 down), terminate
+ * flag and interpolation points (0 = 10); cond(k, u, p, t) and
+ * affect(k, u, p, t) as the device compiles them. */
+typedef double (*oracle_user_cond)(int32_t k, const double *u, const double *p, double t);
+typedef void (*oracle_user_affect)(int32_t k, double *u, const double *p, double t);
+int oracle_set_user_callbacks(int32_t ncb, const int32_t *kind, const int32_t *direction,
+                              const int32_t *terminate, const int32_t *interp_points,
+                              oracle_user_cond cond, oracle_user_affect affect);
 
 enum {
   ORACLE_ALG_TSIT5 = 0,
@@ -80,7 +89,8 @@ enum {
   ORACLE_RC_DTLESSTHANMIN = 3,
   ORACLE_RC_UNSTABLE = 4,
   ORACLE_RC_DTNAN = 5,
-  ORACLE_RC_INITIALFAILURE = 6
+  ORACLE_RC_INITIALFAILURE = 6,
+  ORACLE_RC_TERMINATED = 7
 };
 
 typedef struct {
@@ -113,6 +123,8 @@ typedef struct {
   int32_t nstiff_steps;   /* accepted steps taken by Rodas5 (AUTO / RODAS5) */
   int32_t nswitch;        /* algorithm switches (AUTO) */
   int32_t first_stiff_iter; /* iteration of the first switch to Rodas5, -1 if none */
+  int32_t nevent;         /* callback affects applied (user model callbacks) */
+  double t_event;         /* time of the last one, NaN if none */
 } oracle_result;
 
 /* Single trajectory.  u0[n], p[np].  out_save (may be NULL) is [ns][n]. */
@@ -131,7 +143,8 @@ int oracle_ensemble_solve(int32_t model, int32_t n, int32_t np, int64_t ntraj,
                           const oracle_options *opt, int32_t nthreads,
                           double *out_save, double *u_final, double *t_final,
                           int8_t *retcode, int32_t *naccept, int32_t *nreject,
-                          int32_t *nf, double *dt0, int32_t *nstiff, double *dt_first);
+                          int32_t *nf, double *dt0, int32_t *nstiff, double *dt_first,
+                          int32_t *nevent, double *t_event);
 
 /* ---- hooks that expose internals to the tests ---- */
 int oracle_tsit5_tableau(double *c7, double *a49, double *btilde7, double *r28);

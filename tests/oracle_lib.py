@@ -14,7 +14,7 @@ MODEL = dict(exp=0, lorenz=1, robertson=2, vdp=3, brusselator=4, linear=5, proth
 ALG = dict(tsit5=0, rodas5=1, auto=2, rodas5p=3, auto5p=4)
 PREC = dict(fp64=0, mixed=1)
 RC = dict(default=0, success=1, maxiters=2, dtlessthanmin=3, unstable=4, dtnan=5,
-          initialfailure=6)
+          initialfailure=6, terminated=7)
 NMAX = 128
 
 _dp = C.POINTER(C.c_double)
@@ -55,6 +55,8 @@ class Result(C.Structure):
         ("nstiff_steps", C.c_int32),
         ("nswitch", C.c_int32),
         ("first_stiff_iter", C.c_int32),
+        ("nevent", C.c_int32),
+        ("t_event", C.c_double),
     ]
 
 
@@ -74,7 +76,7 @@ class Oracle:
         L.oracle_ensemble_solve.argtypes = [
             C.c_int32, C.c_int32, C.c_int32, C.c_int64, _dp, _dp, _dp, _dp, _ip,
             C.POINTER(Options), C.c_int32, _dp, _dp, _dp, C.POINTER(C.c_int8), _ip, _ip, _ip,
-            _dp, _ip, _dp]
+            _dp, _ip, _dp, _ip, _dp]
         L.oracle_tsit5_tableau.argtypes = [_dp, _dp, _dp, _dp]
         L.oracle_rodas5_tableau.argtypes = [_dp, _dp, _dp, _dp, _dp]
         L.oracle_rosenbrock_tableau.argtypes = [C.c_int32, _dp, _dp, _dp, _dp, _dp]
@@ -177,15 +179,18 @@ class Oracle:
         dt0 = np.zeros(N)
         nst = np.zeros(N, dtype=np.int32)
         dtf = np.zeros(N)
+        nev = np.zeros(N, dtype=np.int32)
+        tev = np.zeros(N)
         nv = None if nvec is None else np.ascontiguousarray(nvec, dtype=np.int32)
         r = self.lib.oracle_ensemble_solve(
             MODEL[model], n, p.shape[0], N, _ptr(u0), _ptr(p), _ptr(t0), _ptr(tf),
             _ptr(nv, C.c_int32), C.byref(o), nthreads, _ptr(save), _ptr(uf), _ptr(tfin),
             rc.ctypes.data_as(C.POINTER(C.c_int8)), _ptr(na, C.c_int32), _ptr(nr, C.c_int32),
-            _ptr(nf, C.c_int32), _ptr(dt0), _ptr(nst, C.c_int32), _ptr(dtf))
+            _ptr(nf, C.c_int32), _ptr(dt0), _ptr(nst, C.c_int32), _ptr(dtf),
+            _ptr(nev, C.c_int32), _ptr(tev))
         assert r == 0
         out = dict(u=uf, t=tfin, retcode=rc, naccept=na, nreject=nr, nf=nf, dt0=dt0,
-                   nstiff=nst, dt_first=dtf)
+                   nstiff=nst, dt_first=dtf, nevent=nev, t_event=tev)
         if save is not None:
             out["saveat"] = save
         return out

@@ -1,0 +1,123 @@
+"""In-kernel callbacks (ContinuousCallback / DiscreteCallback, qa.jl:26,36;
+test/core.jl:60-79) compiled into the right-hand side: event location on
+the dense output (Tsit5 and Rosenbrock), affects, terminate!, saveat around
+events -- bit-exact against the oracle running the same statements, and
+against closed forms."""
+import numpy as np
+import pytest
+
+import user_model_host as umh
+from gpu_util import ALGS, assert_identical, cpu, say
+from mideq import (CallbackSet, ContinuousCallback, DiscreteCallback, EnsembleMI355X,
+                   UserModel, device_solve, lorenz, robertson, workloads)
+from mideq.scimlbase import BUILTIN_SOURCE
+from test_user_model import BALL, LORENZ, _bounce_times
+
+pytestmark = pytest.mark.gpu
+
+
+def run(f, u0, p, t0, tf, alg="tsit5", precision="fp64", schedule="lane", controller="reference",
+        saveat=None, callback=None, **kw):
+    N = u0.shape[1]
+    t0 = np.broadcast_to(np.asarray(t0, dtype=np.float64), (N,)).copy()
+    tf = np.broadcast_to(np.asarray(tf, dtype=np.float64), (N,)).copy()
+    ealg = EnsembleMI355X(precision=precision, schedule=schedule, controller=controller)
+    return device_solve(f, np.ascontiguousarray(u0), np.ascontiguousarray(p), t0, tf, None,
+                        ALGS[alg], ealg, saveat=saveat, callback=callback, **kw)
+
+
+def _check(g, c):
+    assert_identical(g, c)
+    np.testing.assert_array_equal(g.nevent, c["nevent"])
+    np.testing.assert_array_equal(g.t_event, c["t_event"])
+
+
+def test_core_jl_callbacks_on_lorenz(oracle):
+    """test/core.jl:60-79 on a 2,000-trajectory Lorenz ensemble (the built-in
+    model with the callbacks compiled in)."""
+    N = 2000
+    say(f"core.jl callbacks lorenz N={N}")
+    w = workloads.lorenz(N, seed=111)
+    kw = dict(abstol=1e-6, reltol=1e-3)
+    cb = ContinuousCallback("t - 0.5")
+    g = run(lorenz, w["u0"], w["p"], 0.0, 1.0, callback=cb, **kw)
+    assert (g.retcode == 1).all() and (g.nevent == 1).all()
+    assert np.abs(g.t_event - 0.5).max() < 1e-12
+    umh.install(oracle, 3, 3, BUILTIN_SOURCE["lorenz"][2], callbacks=(cb,))
+    _check(g, cpu(oracle, "user", w["u0"], w["p"], 0.0, 1.0, **kw))
+    dcb = DiscreteCallback("t >= 0.5")
+    g2 = run(lorenz, w["u0"], w["p"], 0.0, 1.0, callback=dcb, **kw)
+    assert (g2.retcode == 1).all() and (g2.nevent >= 1).all() and (g2.t_event == 1.0).all()
+    umh.install(oracle, 3, 3, BUILTIN_SOURCE["lorenz"][2], callbacks=(dcb,))
+    _check(g2, cpu(oracle, "user", w["u0"], w["p"], 0.0, 1.0, **kw))
+
+
+@pytest.mark.parametrize("alg,precision,schedule", [("tsit5", "fp64", "lane"),
+                                                    ("tsit5", "mixed", "persistent"),
+                                                    ("rodas5", "fp64", "lane"),
+                                                    ("auto5p", "fp64", "lane")])
+def test_bouncing_ball(oracle, alg, precision, schedule):
+    """Downcrossings of the height with a restitution affect: bit-exact vs
+    the oracle, bounce count and times vs the closed form, saveat values on
+    both sides of the bounces."""
+    N = 1024
+    say(f"bouncing ball {alg} {precision} {schedule} N={N}")
+    cb = ContinuousCallback("u[0]", affect="u[1] = -p[1] * u[1];", direction=-1)
+    model = UserModel(2, 2, BALL, callbacks=(cb,))
+    rng = np.random.default_rng(112)
+    # no Zeno point before tf and no bounce shorter than a tenth of the
+    # longest step (dtmax = tf)
+    h0 = rng.uniform(5.0, 20.0, N)
+    u0 = np.vstack([h0, np.zeros(N)])
+    p = np.vstack([np.full(N, 9.81), rng.uniform(0.8, 0.95, N)])
+    sa = np.linspace(0.0, 6.0, 25)
+    kw = dict(abstol=1e-10, reltol=1e-10, saveat=sa)
+    g = run(model, u0, p, 0.0, 6.0, alg=alg, precision=precision, schedule=schedule, **kw)
+    umh.install(oracle, 2, 2, BALL, callbacks=(cb,))
+    c = cpu(oracle, "user", u0, p, 0.0, 6.0, alg=alg, precision=precision, **kw)
+    assert (c["retcode"] == 1).all()
+    _check(g, c)
+    if precision == "fp64":
+        for i in range(0, N, 97):
+            times = _bounce_times(h0[i], e=p[1, i])
+            assert g.nevent[i] == len(times)
+            assert abs(g.t_event[i] - times[-1]) < 1e-7 * max(1.0, times[-1])
+        assert (g.us[:, 0, :] > -1e-6).all()  # the ball stays above the floor
+
+
+def test_terminate_and_callback_set(oracle):
+    """terminate! at u = 1/2 of u' = -k u (t = ln 2 / k) with a discrete
+    callback in the same set; saves after termination stay NaN."""
+    N = 512
+    say(f"terminate + CallbackSet N={N}")
+    cbs = CallbackSet(ContinuousCallback("u[0] - 0.5", terminate=True),
+                      DiscreteCallback("t > 0.25 && t < 0.3", affect="u[0] = u[0] * 1.0;"))
+    model = UserModel(1, 1, "du[0] = -p[0] * u[0];", callbacks=cbs)
+    rng = np.random.default_rng(113)
+    k = rng.uniform(0.5, 3.0, N)
+    sa = np.linspace(0.0, 2.0, 9)
+    kw = dict(abstol=1e-12, reltol=1e-12, saveat=sa)
+    g = run(model, np.ones((1, N)), k[None, :], 0.0, 4.0, **kw)
+    assert (g.retcode == 7).all()
+    np.testing.assert_allclose(g.t_final, np.log(2.0) / k, rtol=1e-9)
+    umh.install(oracle, 1, 1, "du[0] = -p[0] * u[0];", callbacks=cbs)
+    _check(g, cpu(oracle, "user", np.ones((1, N)), k[None, :], 0.0, 4.0, **kw))
+    late = sa[None, :] > g.t_final[:, None]
+    assert np.isnan(g.us[:, 0, :].T[late]).all()
+
+
+def test_rosenbrock_event_on_robertson(oracle):
+    """A continuous event located on the Rosenbrock dense output (Robertson,
+    Rodas5): where y3 reaches 1e-3, y2 is halved."""
+    N = 800
+    say(f"robertson event rodas5 N={N}")
+    n, np_, rhs, jac = BUILTIN_SOURCE["robertson"]
+    cb = ContinuousCallback("u[2] - 1e-3", affect="u[0] = u[0] + 0.5 * u[1]; u[1] = 0.5 * u[1];")
+    w = workloads.robertson(N, seed=114)
+    kw = dict(abstol=1e-8, reltol=1e-6, saveat=np.logspace(-3, 4, 15))
+    g = run(robertson, w["u0"], w["p"], 0.0, 1e4, alg="rodas5", callback=cb, **kw)
+    umh.install(oracle, n, np_, rhs, jac, callbacks=(cb,))
+    c = cpu(oracle, "user", w["u0"], w["p"], 0.0, 1e4, alg="rodas5", **kw)
+    assert (c["retcode"] == 1).all() and (c["nevent"] == 1).all()
+    _check(g, c)
+    np.testing.assert_allclose(g.u_final.sum(axis=0), 1.0, atol=1e-10)  # sum y kept

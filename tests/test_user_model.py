@@ -103,3 +103,74 @@ def test_oracle_user_model_equals_builtin(oracle, alg, precision, controller):
     b = oracle.ensemble("robertson", u0, p, 0.0, 1e4, **kw)
     for k in ("u", "t", "retcode", "naccept", "nreject", "nf", "saveat"):
         np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+
+
+BALL = ("du[0] = u[1];\ndu[1] = -p[0];")
+
+
+def _ball(oracle, alg="tsit5", N=4, tf=6.0):
+    from mideq import ContinuousCallback
+    cb = ContinuousCallback("u[0]", affect="u[1] = -p[1] * u[1];", direction=-1)
+    umh.install(oracle, 2, 2, BALL, callbacks=(cb,))
+    h0 = np.array([10.0, 5.0, 2.0, 20.0])[:N]
+    u0 = np.vstack([h0, np.zeros(N)])
+    p = np.vstack([np.full(N, 9.81), np.full(N, 0.9)])
+    return oracle.ensemble("user", u0, p, 0.0, tf, alg=alg, abstol=1e-12, reltol=1e-12), h0
+
+
+def _bounce_times(h0, g=9.81, e=0.9, tf=6.0):
+    t, v, out = np.sqrt(2 * h0 / g), e * np.sqrt(2 * g * h0), []
+    while t <= tf:
+        out.append(t)
+        t += 2 * v / g
+        v *= e
+    return out
+
+
+@pytest.mark.parametrize("alg", ["tsit5", "rodas5", "auto"])
+def test_oracle_bouncing_ball_event_times(oracle, alg):
+    """Downcrossings of the height located on the dense output: the number of
+    bounces and the last bounce time follow the closed form."""
+    r, h0 = _ball(oracle, alg)
+    assert (r["retcode"] == 1).all()
+    for i in range(len(h0)):
+        times = _bounce_times(h0[i])
+        assert r["nevent"][i] == len(times)
+        assert abs(r["t_event"][i] - times[-1]) < 1e-8 * max(1.0, times[-1])
+
+
+def test_oracle_terminate_at_half(oracle):
+    """u' = -u from 1, terminate where u = 1/2: t = ln 2, retcode Terminated,
+    save points after the event unfilled."""
+    from mideq import ContinuousCallback
+    umh.install(oracle, 1, 1, "du[0] = -p[0] * u[0];",
+                callbacks=(ContinuousCallback("u[0] - 0.5", terminate=True),))
+    sa = np.array([0.5, 1.0])
+    r = oracle.ensemble("user", np.ones((1, 3)), np.ones((1, 3)), 0.0, 2.0, abstol=1e-12,
+                        reltol=1e-12, saveat=sa)
+    assert (r["retcode"] == 7).all() and (r["nevent"] == 1).all()
+    np.testing.assert_allclose(r["t"], np.log(2.0), rtol=1e-10)
+    np.testing.assert_allclose(r["u"][0], 0.5, rtol=1e-10)
+    assert np.isnan(r["saveat"][1]).all() and not np.isnan(r["saveat"][0]).any()
+
+
+def test_oracle_core_jl_callbacks(oracle):
+    """test/core.jl:60-79: ContinuousCallback(t - 0.5) and DiscreteCallback(t >= 0.5)
+    with no affect on Lorenz both finish with Success; the continuous one
+    stops a step at t = 0.5."""
+    from mideq import ContinuousCallback, DiscreteCallback
+    u0 = np.array([[1.0], [0.0], [0.0]])
+    p = np.array([[10.0], [28.0], [8.0 / 3.0]])
+    umh.install(oracle, 3, 3, LORENZ, callbacks=(ContinuousCallback("t - 0.5"),))
+    r = oracle.ensemble("user", u0, p, 0.0, 1.0, abstol=1e-6, reltol=1e-3)
+    assert r["retcode"][0] == 1 and r["nevent"][0] == 1
+    assert abs(r["t_event"][0] - 0.5) < 1e-12
+    umh.install(oracle, 3, 3, LORENZ, callbacks=(DiscreteCallback("t >= 0.5"),))
+    r2 = oracle.ensemble("user", u0, p, 0.0, 1.0, abstol=1e-6, reltol=1e-3)
+    assert r2["retcode"][0] == 1 and r2["nevent"][0] >= 1 and r2["t_event"][0] == 1.0
+    # without callbacks: the same steps as the built-in model
+    umh.install(oracle, 3, 3, LORENZ)
+    r3 = oracle.ensemble("user", u0, p, 0.0, 1.0, abstol=1e-6, reltol=1e-3)
+    r4 = oracle.ensemble("lorenz", u0, p, 0.0, 1.0, abstol=1e-6, reltol=1e-3)
+    np.testing.assert_array_equal(r3["u"], r4["u"])
+    assert r3["nevent"][0] == 0

@@ -17,7 +17,7 @@ DUAL = os.path.join(ROOT, "differentialequations.jl_amd", "csrc", "user_dual.hpp
 _CACHE = {}
 
 
-def model_source(n, np_, rhs, jac=None):
+def model_source(n, np_, rhs, jac=None, callbacks=()):
     np_ = max(int(np_), 1)
     s = ["#include <math.h>", "#define MIDEQ_HD inline", f'#include "{DUAL}"',
          "namespace mideq {", "struct ModelUser {",
@@ -40,12 +40,23 @@ def model_source(n, np_, rhs, jac=None):
           "{ mideq::ModelUser::f<float>(du, u, p); }",
           'extern "C" void user_jac(double* J, const double* u, const double* p) '
           "{ mideq::ModelUser::jac((double (*)[mideq::ModelUser::N])J, u, p); }"]
+    # callbacks, as the device compiles them (double)
+    s.append('extern "C" double user_cb_g(int k, const double* u, const double* p, double t) {')
+    s.append("  switch (k) {")
+    for k, cb in enumerate(callbacks):
+        s.append(f"    case {k}: return (double)({cb.condition});")
+    s += ["  }", "  return 0.0;", "}"]
+    s.append('extern "C" void user_cb_affect(int k, double* u, const double* p, double t) {')
+    s.append("  switch (k) {")
+    for k, cb in enumerate(callbacks):
+        s.append(f"    case {k}: {{ {cb.affect or ''} }} break;")
+    s += ["  }", "}"]
     return "\n".join(s) + "\n"
 
 
-def build(n, np_, rhs, jac=None):
+def build(n, np_, rhs, jac=None, callbacks=()):
     """ctypes handle of the host-compiled model (cached by source)."""
-    src = model_source(n, np_, rhs, jac)
+    src = model_source(n, np_, rhs, jac, callbacks)
     key = hashlib.sha1(src.encode()).hexdigest()[:16]
     if key in _CACHE:
         return _CACHE[key]
@@ -62,18 +73,32 @@ def build(n, np_, rhs, jac=None):
     lib.user_f64.argtypes = [dp, dp, dp]
     lib.user_f32.argtypes = [fp, fp, fp]
     lib.user_jac.argtypes = [dp, dp, dp]
+    lib.user_cb_g.argtypes = [C.c_int32, dp, dp, C.c_double]
+    lib.user_cb_g.restype = C.c_double
     _CACHE[key] = lib
     return lib
 
 
-def install(oracle, n, np_, rhs, jac=None):
-    """Make the oracle's "user" model this one."""
-    lib = build(n, np_, rhs, jac)
+def install(oracle, n, np_, rhs, jac=None, callbacks=()):
+    """Make the oracle's "user" model this one (callbacks: the mideq
+    ContinuousCallback / DiscreteCallback objects)."""
+    lib = build(n, np_, rhs, jac, callbacks)
     addr = lambda fn: C.cast(fn, C.c_void_p).value
     L = oracle.lib
     L.oracle_set_user_model.argtypes = [C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
                                         C.c_void_p]
     rc = L.oracle_set_user_model(n, max(int(np_), 1), addr(lib.user_f64), addr(lib.user_f32),
                                  addr(lib.user_jac))
+    assert rc == 0
+    k = len(callbacks)
+    arr = lambda vals: (C.c_int32 * max(k, 1))(*vals)
+    kind = [0 if type(cb).__name__ == "ContinuousCallback" else 1 for cb in callbacks]
+    direction = [getattr(cb, "direction", 0) for cb in callbacks]
+    term = [int(bool(cb.terminate)) for cb in callbacks]
+    ip = [getattr(cb, "interp_points", 0) for cb in callbacks]
+    L.oracle_set_user_callbacks.argtypes = [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                            C.c_void_p, C.c_void_p, C.c_void_p]
+    rc = L.oracle_set_user_callbacks(k, arr(kind), arr(direction), arr(term), arr(ip),
+                                     addr(lib.user_cb_g), addr(lib.user_cb_affect))
     assert rc == 0
     return lib
