@@ -87,6 +87,52 @@ static hipError_t launch_model(const LaneArgs& a, int alg, int mix, int fast, in
              : launch_alg<M, false, false>(a, alg, sched, block, grid, s);
 }
 
+// Deterministic fold of x[row][0..N) per row: blocks of 256 by a halving
+// tree (identity past the end), then the block results in block order.
+__device__ __forceinline__ double fold_identity(int op) {
+  return op == 3 ? __builtin_inf() : op == 4 ? -__builtin_inf() : 0.0;
+}
+__device__ __forceinline__ double fold2(int op, double a, double b) {
+  return op == 3 ? fmin(a, b) : op == 4 ? fmax(a, b) : a + b;
+}
+
+__global__ __launch_bounds__(256) void k_fold_blocks(const double* x, int64_t N, int op,
+                                                     int64_t nb, double* partial) {
+  __shared__ double s[256];
+  const int64_t b = blockIdx.x, row = blockIdx.y;
+  const int tl = threadIdx.x;
+  const int64_t i = b * 256 + tl;
+  s[tl] = i < N ? x[row * N + i] : fold_identity(op);
+  __syncthreads();
+  for (int w = 128; w >= 1; w >>= 1) {
+    if (tl < w) s[tl] = fold2(op, s[tl], s[tl + w]);
+    __syncthreads();
+  }
+  if (tl == 0) partial[row * nb + b] = s[0];
+}
+
+__global__ void k_fold_final(const double* partial, int64_t nb, int op, int64_t N, int nrow,
+                             double* result) {
+  const int row = threadIdx.x;
+  if (row >= nrow) return;
+  double acc = partial[(int64_t)row * nb];
+  for (int64_t b = 1; b < nb; b++) acc = fold2(op, acc, partial[(int64_t)row * nb + b]);
+  if (op == 2) acc = acc / (double)N;
+  result[row] = acc;
+}
+
+hipError_t launch_fold(const double* x, int64_t N, int nrow, int op, double* partial,
+                       double* result, hipStream_t s) {
+  if (N <= 0 || nrow <= 0 || nrow > 1024 || op < 1 || op > 4) return hipErrorInvalidValue;
+  const int64_t nb = (N + 255) / 256;
+  if (nb > 0x7fffffff) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_fold_blocks, dim3((unsigned)nb, (unsigned)nrow), dim3(256), 0, s, x, N,
+                     op, nb, partial);
+  hipLaunchKernelGGL(k_fold_final, dim3(1), dim3(((nrow + 63) / 64) * 64), 0, s, partial, nb,
+                     op, N, nrow, result);
+  return hipGetLastError();
+}
+
 hipError_t launch_set_u64(unsigned long long* p, unsigned long long v, hipStream_t s) {
   hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, s, p, v);
   return hipGetLastError();

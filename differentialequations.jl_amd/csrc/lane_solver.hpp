@@ -53,6 +53,7 @@ struct LaneArgs {
   double* dt_first;
   int32_t* nevent;            // callbacks: affects applied per trajectory
   double* t_event;            // callbacks: time of the last affect
+  double* out;                // output_func values [nout][ntraj]
   unsigned long long* queue;  // persistent schedule: next position in the order
   const int64_t* order;       // launch order (NULL: natural order)
   const unsigned long long* order_len;  // device length of `order` (NULL: ntraj)
@@ -249,10 +250,22 @@ struct cb_count<M, decltype((void)M::NCB, void())> {
   static constexpr int value = M::NCB;
 };
 
+// On-device output_func of a model: M::NOUT values out[j] of the final
+// state, M::output(out, u, p, t, rc).
+template <class M, class = void>
+struct out_count {
+  static constexpr int value = 0;
+};
+template <class M>
+struct out_count<M, decltype((void)M::NOUT, void())> {
+  static constexpr int value = M::NOUT;
+};
+
 template <class M, bool MIX, int ALG, bool FAST>
 struct Lane {
   static constexpr int N = M::N;
   static constexpr int kNcb = cb_count<M>::value;
+  static constexpr int kNout = out_count<M>::value;
   static constexpr int NP = M::NP;
   static constexpr int kAlg = ALG & (ALG_DENSE - 1);
   static constexpr bool kDense = (ALG & ALG_DENSE) != 0;
@@ -1105,6 +1118,14 @@ struct Lane {
     if (a.dt_first) a.dt_first[i] = dtfirst;
     if (a.nevent) a.nevent[i] = nevent;
     if (a.t_event) a.t_event[i] = tevent;
+    if constexpr (kNout > 0) {
+      if (a.out) {
+        double o[kNout];
+        M::output(o, u, p, t, (int)rc);
+#pragma unroll
+        for (int j = 0; j < kNout; j++) a.out[(int64_t)j * NT + i] = o[j];
+      }
+    }
   }
 };
 

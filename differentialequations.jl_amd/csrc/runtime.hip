@@ -34,7 +34,7 @@ struct DevBuf {
 enum {
   BUF_U0 = 0, BUF_P, BUF_T0, BUF_TF, BUF_NVEC, BUF_SAVEAT, BUF_DT0, BUF_HINT, BUF_US,
   BUF_UF, BUF_TFIN, BUF_RC, BUF_NA, BUF_NR, BUF_NF, BUF_NS, BUF_SCRATCH, BUF_ORDER,
-  BUF_SAMPLE, BUF_DTF, BUF_HANDOFF, BUF_NEV, BUF_TEV, BUF_COUNT
+  BUF_SAMPLE, BUF_DTF, BUF_HANDOFF, BUF_NEV, BUF_TEV, BUF_OUT, BUF_RED, BUF_FOLD, BUF_COUNT
 };
 
 // A run-time compiled right-hand side: its generated model source and the
@@ -46,7 +46,8 @@ struct UserKernels {
 };
 struct UserModel {
   int n = 0, np = 0;
-  int ncb = 0;  // callbacks compiled into the model
+  int ncb = 0;   // callbacks compiled into the model
+  int nout = 0;  // output_func values per trajectory
   std::string model_src;
   std::map<int, UserKernels> kernels;
 };
@@ -215,6 +216,10 @@ static int check_user_spec(const mideq_user_model* m, std::string* why) {
     *why = b;
     return MIDEQ_E_ARG;
   }
+  if (m->nout < 0 || m->nout > MIDEQ_USER_NOUTMAX || (m->nout > 0 && !m->output)) {
+    *why = "bad nout or null output";
+    return MIDEQ_E_ARG;
+  }
   if (m->ncallbacks < 0 || m->ncallbacks > MIDEQ_USER_NCBMAX ||
       (m->ncallbacks > 0 && !m->callbacks)) {
     *why = "bad callback count or null callbacks";
@@ -295,6 +300,13 @@ static std::string user_model_source(const mideq_user_model* m) {
       s += "      } break;\n";
     }
     s += "    }\n  }\n";
+  }
+  if (m->nout > 0) {
+    s += "  static constexpr int NOUT = " + std::to_string(m->nout) + ";\n";
+    s += "  __device__ static void output(double* out, const double* u, const double* p, "
+         "double t, int rc) {\n#line 1 \"output\"\n";
+    s += m->output;
+    s += "\n  }\n";
   }
   s += "};\n}  // namespace mideq\n";
   return s;
@@ -433,6 +445,7 @@ extern "C" int mideq_model_register(mideq_ctx* c, const mideq_user_model* m,
   um.n = m->n;
   um.np = m->np > 0 ? m->np : 1;  // the kernel reads at least one parameter row
   um.ncb = m->ncallbacks;
+  um.nout = m->nout;
   um.model_src = user_model_source(m);
   c->user_models.push_back(um);
   UserModel* p = &c->user_models.back();
@@ -495,6 +508,12 @@ static int validate(mideq_ctx* c, const mideq_problem* P, const mideq_options* O
   if (O->schedule < 0 || O->schedule > 2) return set_err(c, MIDEQ_E_ARG, "bad schedule");
   if (O->controller < 0 || O->controller > 1) return set_err(c, MIDEQ_E_ARG, "bad controller");
   if (O->auto_passes < 0 || O->auto_passes > 2) return set_err(c, MIDEQ_E_ARG, "bad auto_passes");
+  if (O->reduction < 0 || O->reduction > 4) return set_err(c, MIDEQ_E_ARG, "bad reduction");
+  if (O->reduction != 0) {
+    UserModel* um = user_model(c, P->model);
+    if (!um || um->nout == 0)
+      return set_err(c, MIDEQ_E_ARG, "a reduction needs a user model with an output");
+  }
   if (O->block != 0 && (O->block < 64 || O->block > 256 || (O->block & 63)))
     return set_err(c, MIDEQ_E_ARG, "block must be 64..256, multiple of 64");
   return MIDEQ_OK;
@@ -729,6 +748,7 @@ static int run_solver(mideq_ctx* c, const mideq_problem* P, const mideq_options*
   a.dt_first = out->dt_first;
   a.nevent = out->nevent;
   a.t_event = out->t_event;
+  a.out = out->out;
   a.queue = c->d_queue;
   a.order = d_order;
   int sched = O->schedule == MIDEQ_SCHED_AUTO ? SCHED_LANE : O->schedule;
@@ -774,6 +794,19 @@ static int run_solver(mideq_ctx* c, const mideq_problem* P, const mideq_options*
       rc0 = user_kernels(c, um, ALG_AUTO_T, O->precision, O->controller, &uk_first);
     if (rc0) return rc0;
   }
+  // on-device reduction of the output values: block partials in scratch
+  double* fold_partial = nullptr;
+  const bool fold = O->reduction != 0 && um && um->nout > 0;
+  if (fold) {
+    if (!out->out || !out->reduced)
+      return set_err(c, MIDEQ_E_ARG, "a reduction needs the out and reduced buffers");
+    void* d;
+    int rc0 = ensure_buf(c, BUF_FOLD,
+                         sizeof(double) * (size_t)um->nout * (size_t)((P->ntraj + 255) / 256),
+                         &d);
+    if (rc0) return rc0;
+    fold_partial = (double*)d;
+  }
   auto launch = [&](const LaneArgs& la, int alg, hipStream_t st) -> hipError_t {
     if (um)
       return launch_user(alg == ALG_AUTO_T ? uk_first : uk_main, la, sched, block, grid, c->ncu,
@@ -781,12 +814,17 @@ static int run_solver(mideq_ctx* c, const mideq_problem* P, const mideq_options*
     return launch_lane_solver(la, P->model, alg, O->precision, O->controller, sched, block, grid,
                               st);
   };
+  auto finish_fold = [&](hipError_t e, hipStream_t st) -> hipError_t {
+    if (e != hipSuccess || !fold) return e;
+    return launch_fold(out->out, P->ntraj, um->nout, O->reduction, fold_partial, out->reduced,
+                       st);
+  };
   int rc = enqueue(c, O->graph != 0, &key, sizeof key, s, [&](hipStream_t st) {
     if (!two_pass) {
       LaneArgs a1 = a;
       a1.handoff = nullptr;
       a1.handoff_count = nullptr;
-      return launch(a1, O->alg, st);
+      return finish_fold(launch(a1, O->alg, st), st);
     }
     hipError_t e = launch_set_u64(a.handoff_count, 0ull, st);
     if (e != hipSuccess) return e;
@@ -797,7 +835,7 @@ static int run_solver(mideq_ctx* c, const mideq_problem* P, const mideq_options*
     a2.order_len = a.handoff_count;
     a2.handoff = nullptr;
     a2.handoff_count = nullptr;
-    return launch(a2, O->alg, st);
+    return finish_fold(launch(a2, O->alg, st), st);
   });
   if (rc) return rc;
   HIP_TRY(c, hipEventRecord(c->ev1, s));
@@ -943,6 +981,16 @@ extern "C" int mideq_ensemble_solve(mideq_ctx* c, const mideq_problem* P,
   if (out->dt_first && (rc = ensure_buf(c, BUF_DTF, 8 * N, (void**)&dOut.dt_first))) return rc;
   if (out->nevent && (rc = ensure_buf(c, BUF_NEV, 4 * N, (void**)&dOut.nevent))) return rc;
   if (out->t_event && (rc = ensure_buf(c, BUF_TEV, 8 * N, (void**)&dOut.t_event))) return rc;
+  // output_func values: kept on the device when only their reduction is asked for
+  UserModel* um_out = user_model(c, P->model);
+  const int nout = um_out ? um_out->nout : 0;
+  if (nout > 0 && (out->out || O->reduction != 0)) {
+    if ((rc = ensure_buf(c, BUF_OUT, sizeof(double) * (size_t)nout * N, (void**)&dOut.out)))
+      return rc;
+  }
+  if (nout > 0 && O->reduction != 0 && out->reduced) {
+    if ((rc = ensure_buf(c, BUF_RED, sizeof(double) * nout, (void**)&dOut.reduced))) return rc;
+  }
 
   rc = run_solver(c, &dP, &dO, &dOut, d_order, s);
   if (rc) return rc;
@@ -960,6 +1008,12 @@ extern "C" int mideq_ensemble_solve(mideq_ctx* c, const mideq_problem* P,
   if (out->dt_first) HIP_TRY(c, hipMemcpyAsync(out->dt_first, dOut.dt_first, 8 * N, hipMemcpyDeviceToHost, s));
   if (out->nevent) HIP_TRY(c, hipMemcpyAsync(out->nevent, dOut.nevent, 4 * N, hipMemcpyDeviceToHost, s));
   if (out->t_event) HIP_TRY(c, hipMemcpyAsync(out->t_event, dOut.t_event, 8 * N, hipMemcpyDeviceToHost, s));
+  if (out->out && dOut.out)
+    HIP_TRY(c, hipMemcpyAsync(out->out, dOut.out, sizeof(double) * (size_t)nout * N,
+                              hipMemcpyDeviceToHost, s));
+  if (out->reduced && dOut.reduced)
+    HIP_TRY(c, hipMemcpyAsync(out->reduced, dOut.reduced, sizeof(double) * nout,
+                              hipMemcpyDeviceToHost, s));
   HIP_TRY(c, hipEventRecord(t1e, s));
   HIP_TRY(c, hipStreamSynchronize(s));
   float ms = 0.f;

@@ -63,6 +63,7 @@ struct MideqOptions
     qsteady_default::Int32; schedule::Int32; block::Int32; blocks_per_cu::Int32
     lanes::Int32; graph::Int32; controller::Int32; auto_passes::Int32
     order::Ptr{Int64}
+    reduction::Int32; reserved2::Int32
 end
 
 struct MideqOutputs
@@ -70,6 +71,7 @@ struct MideqOutputs
     naccept::Ptr{Int32}; nreject::Ptr{Int32}; nf::Ptr{Int32}; nstiff::Ptr{Int32}
     dt_first::Ptr{Float64}
     nevent::Ptr{Int32}; t_event::Ptr{Float64}
+    out::Ptr{Float64}; reduced::Ptr{Float64}
 end
 
 # ---- registered right-hand sides ------------------------------------------
@@ -88,6 +90,7 @@ end
 struct MideqUserModel
     n::Int32; np::Int32; rhs::Ptr{UInt8}; jac::Ptr{UInt8}
     ncallbacks::Int32; reserved::Int32; callbacks::Ptr{MideqCallback}
+    nout::Int32; reserved2::Int32; output::Ptr{UInt8}
 end
 
 """
@@ -128,21 +131,24 @@ The model id belongs to `device`'s context.  Example (test/core.jl:22-27):
 function register_model!(f, n::Integer, np::Integer, rhs::AbstractString;
                          jac::Union{Nothing, AbstractString} = nothing,
                          callbacks::AbstractVector{DeviceCallback} = DeviceCallback[],
+                         output::Union{Nothing, AbstractString} = nothing, nout::Integer = 0,
                          device::Int = 0)
     ctx = context(device)
     id = Ref{Int32}(0)
     r = String(rhs)
     j = jac === nothing ? nothing : String(jac)
+    o = output === nothing ? nothing : String(output)
     conds = [cb.condition for cb in callbacks]
     affs = [cb.affect === nothing ? "" : cb.affect for cb in callbacks]
-    rc = GC.@preserve r j conds affs begin
+    rc = GC.@preserve r j o conds affs begin
         cbs = [MideqCallback(cb.kind === :discrete ? 1 : 0, cb.direction, cb.terminate,
                              cb.interp_points, pointer(conds[k]),
                              cb.affect === nothing ? C_NULL : pointer(affs[k]))
                for (k, cb) in enumerate(callbacks)]
         GC.@preserve cbs begin
             m = MideqUserModel(n, np, pointer(r), j === nothing ? C_NULL : pointer(j),
-                               length(cbs), 0, isempty(cbs) ? C_NULL : pointer(cbs))
+                               length(cbs), 0, isempty(cbs) ? C_NULL : pointer(cbs),
+                               o === nothing ? 0 : nout, 0, o === nothing ? C_NULL : pointer(o))
             ccall((:mideq_model_register, LIB[]), Cint,
                   (Ptr{Cvoid}, Ref{MideqUserModel}, Ref{Int32}), ctx.h, m, id)
         end
@@ -240,10 +246,10 @@ function __solve(ens::SciMLBase.AbstractEnsembleProblem, alg, ealg::EnsembleMI35
                                  ns > 0 ? pointer(sa) : C_NULL, ns,
                                  ealg.save_dtype === Float32 ? 1 : 0, C_NULL, C_NULL, 1,
                                  ealg.schedule === :lane ? 1 : ealg.schedule === :persistent ? 2 : 0,
-                                 0, 0, 0, 0, 0, 0, C_NULL)
+                                 0, 0, 0, 0, 0, 0, C_NULL, 0, 0)
             out_c = MideqOutputs(ns > 0 ? pointer(US) : C_NULL, pointer(UF), pointer(TFo),
                                  pointer(RC), pointer(NA), pointer(NR), pointer(NF), pointer(NS),
-                                 C_NULL, C_NULL, C_NULL)
+                                 C_NULL, C_NULL, C_NULL, C_NULL, C_NULL)
             rc = ccall((:mideq_ensemble_solve, LIB[]), Cint,
                        (Ptr{Cvoid}, Ref{MideqProblem}, Ref{MideqOptions}, Ref{MideqOutputs}),
                        ctx.h, prob_c, opt_c, out_c)

@@ -14,15 +14,15 @@ include/mideq.h); there is no CPU fallback.
 """
 from . import workloads
 from ._abi import MideqError, header_functions
-from .scimlbase import (AutoTsit5, CallbackSet, ContinuousCallback, DiscreteCallback,
-                        EnsembleArrays, EnsembleMI355X, EnsembleProblem,
+from .scimlbase import (AutoTsit5, CallbackSet, ContinuousCallback, DeviceOutputFunc,
+                        DeviceReduction, DiscreteCallback, EnsembleArrays, EnsembleMI355X, EnsembleProblem,
                         EnsembleSolution, ODEFunction, ODEProblem, ODESolution, ReturnCode,
                         Rodas5, Rodas5P, Stats, Tsit5, UserModel, brusselator, device_solve,
                         exp_growth, linear3, lorenz, remake, robertson, solve,
                         successful_retcode, vanderpol)
 
-__all__ = ["AutoTsit5", "CallbackSet", "ContinuousCallback", "DiscreteCallback",
-           "EnsembleArrays", "EnsembleMI355X", "EnsembleProblem",
+__all__ = ["AutoTsit5", "CallbackSet", "ContinuousCallback", "DeviceOutputFunc",
+           "DeviceReduction", "DiscreteCallback", "EnsembleArrays", "EnsembleMI355X", "EnsembleProblem",
            "EnsembleSolution", "MideqError", "ODEFunction", "ODEProblem", "ODESolution",
            "ReturnCode", "Rodas5", "Rodas5P", "Stats", "Tsit5", "UserModel", "brusselator",
            "device_solve",

@@ -41,11 +41,13 @@ class CallbackSpec(C.Structure):
 class UserModelSpec(C.Structure):
     _fields_ = [("n", C.c_int32), ("np", C.c_int32), ("rhs", C.c_char_p), ("jac", C.c_char_p),
                 ("ncallbacks", C.c_int32), ("reserved", C.c_int32),
-                ("callbacks", C.POINTER(CallbackSpec))]
+                ("callbacks", C.POINTER(CallbackSpec)), ("nout", C.c_int32),
+                ("reserved2", C.c_int32), ("output", C.c_char_p)]
 
 
 USER_MODEL_BASE = 1000
-USER_NMAX, USER_NPMAX, USER_NCBMAX = 8, 16, 8
+USER_NMAX, USER_NPMAX, USER_NCBMAX, USER_NOUTMAX = 8, 16, 8, 16
+REDUCTION = dict(none=0, sum=1, mean=2, min=3, max=4)
 
 
 class Options(C.Structure):
@@ -56,14 +58,16 @@ class Options(C.Structure):
                 ("alg_hint", C.c_void_p), ("qsteady_default", C.c_int32),
                 ("schedule", C.c_int32), ("block", C.c_int32), ("blocks_per_cu", C.c_int32),
                 ("lanes", C.c_int32), ("graph", C.c_int32), ("controller", C.c_int32),
-                ("auto_passes", C.c_int32), ("order", C.c_void_p)]
+                ("auto_passes", C.c_int32), ("order", C.c_void_p),
+                ("reduction", C.c_int32), ("reserved2", C.c_int32)]
 
 
 class Outputs(C.Structure):
     _fields_ = [("us", C.c_void_p), ("u_final", C.c_void_p), ("t_final", C.c_void_p),
                 ("retcode", C.c_void_p), ("naccept", C.c_void_p), ("nreject", C.c_void_p),
                 ("nf", C.c_void_p), ("nstiff", C.c_void_p), ("dt_first", C.c_void_p),
-                ("nevent", C.c_void_p), ("t_event", C.c_void_p)]
+                ("nevent", C.c_void_p), ("t_event", C.c_void_p), ("out", C.c_void_p),
+                ("reduced", C.c_void_p)]
 
 
 class Stats(C.Structure):

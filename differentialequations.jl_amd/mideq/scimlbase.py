@@ -126,6 +126,25 @@ class DiscreteCallback:
                                  None if self.affect is None else self.affect.encode())
 
 
+@dataclass(frozen=True)
+class DeviceOutputFunc:
+    """output_func(sol, i) -> (out, false) of EnsembleProblem (qa.jl:50)
+    evaluated on the GPU: statements assigning out[0..nout-1] from the final
+    state u[], p[], the final time t and the retcode rc (compiled into the
+    right-hand side).  Only the nout values per trajectory come back."""
+    output: str
+    nout: int = 1
+
+
+@dataclass(frozen=True)
+class DeviceReduction:
+    """reduction(u, batch, I) of EnsembleProblem for the common folds,
+    computed on the GPU over the DeviceOutputFunc values: "sum", "mean",
+    "min" or "max" (fixed order: blocks of 256 by a halving tree, then block
+    by block).  Only the nout reduced values come back."""
+    op: str
+
+
 def CallbackSet(*cbs):
     """CallbackSet (qa.jl:24): the callbacks as one tuple."""
     out = []
@@ -145,7 +164,7 @@ class UserModel:
     Rosenbrock methods differentiate rhs in forward mode.  A model with no
     parameters still takes one (unused) parameter row."""
 
-    def __init__(self, n, np_, rhs, jac=None, name="user", callbacks=()):
+    def __init__(self, n, np_, rhs, jac=None, name="user", callbacks=(), output=None):
         if not (1 <= n <= _abi.USER_NMAX) or not (0 <= np_ <= _abi.USER_NPMAX):
             raise ValueError(f"user model n={n} np={np_} outside 1..{_abi.USER_NMAX} / "
                              f"0..{_abi.USER_NPMAX}")
@@ -158,25 +177,33 @@ class UserModel:
         self.callbacks = CallbackSet(callbacks) if callbacks else ()
         if len(self.callbacks) > _abi.USER_NCBMAX:
             raise ValueError(f"at most {_abi.USER_NCBMAX} callbacks")
+        self.output = output
+        if output is not None and not (1 <= output.nout <= _abi.USER_NOUTMAX):
+            raise ValueError(f"nout must be 1..{_abi.USER_NOUTMAX}")
         self._ids = {}
         self._with = {}
 
     def spec(self):
         cbs = [c.spec() for c in self.callbacks]
         arr = (_abi.CallbackSpec * max(1, len(cbs)))(*cbs)
+        out = self.output
         sp = _abi.UserModelSpec(self.n, self.np, self.rhs.encode(),
                                 None if self.jac_src is None else self.jac_src.encode(),
-                                len(cbs), 0, arr if cbs else None)
+                                len(cbs), 0, arr if cbs else None,
+                                0 if out is None else int(out.nout), 0,
+                                None if out is None else out.output.encode())
         sp._keep = arr
         return sp
 
-    def with_callbacks(self, callbacks):
-        """This model with `callbacks` compiled in (cached)."""
-        cbs = CallbackSet(callbacks)
-        if cbs not in self._with:
-            self._with[cbs] = UserModel(self.n, self.np, self.rhs, self.jac_src,
-                                        name=self.name, callbacks=cbs)
-        return self._with[cbs]
+    def with_callbacks(self, callbacks=(), output=None):
+        """This model with `callbacks` (and an on-device output) compiled in
+        (cached)."""
+        cbs = CallbackSet(callbacks) if callbacks else ()
+        key = (cbs, output)
+        if key not in self._with:
+            self._with[key] = UserModel(self.n, self.np, self.rhs, self.jac_src,
+                                        name=self.name, callbacks=cbs, output=output)
+        return self._with[key]
 
     def check(self):
         """Compile without a GPU; returns the compiler log (raises on error)."""
@@ -397,6 +424,8 @@ class EnsembleArrays:
     graph: bool = False
     nevent: Optional[np.ndarray] = None   # [N] callback affects applied
     t_event: Optional[np.ndarray] = None  # [N] time of the last one (NaN if none)
+    out: Optional[np.ndarray] = None      # [nout, N] on-device output_func values
+    reduced: Optional[np.ndarray] = None  # [nout] their on-device reduction
 
     def solution(self, i, prob=None):
         """ODESolution of trajectory i (0-based)."""
@@ -484,12 +513,18 @@ def _ptr(a):
 
 def device_solve(f, u0, p, t0, tf, nvec, alg, ealg, saveat=None, abstol=1e-6, reltol=1e-3,
                  dt=0.0, dtmin=0.0, dtmax=0.0, maxiters=0, dt0=None, alg_hint=None,
-                 order=None, block=0, blocks_per_cu=0, callback=None):
+                 order=None, block=0, blocks_per_cu=0, callback=None, output=None,
+                 reduction=None, keep_states=True):
     """One call of mideq_ensemble_solve on host arrays -> EnsembleArrays.
     callback: ContinuousCallback / DiscreteCallback / CallbackSet, compiled
-    into the right-hand side (a UserModel, or a built-in model's source)."""
-    if callback:
-        f = user_model_of(f).with_callbacks(callback)
+    into the right-hand side (a UserModel, or a built-in model's source).
+    output: DeviceOutputFunc (values per trajectory in .out); reduction:
+    DeviceReduction or its op name (folded values in .reduced, and with
+    keep_states=False nothing per trajectory comes back)."""
+    if callback or output is not None:
+        f = user_model_of(f).with_callbacks(callback or (), output)
+    if isinstance(reduction, DeviceReduction):
+        reduction = reduction.op
     ctx = _abi.context(ealg.device)
     n, N = u0.shape
     P = _abi.Problem()
@@ -546,12 +581,24 @@ def device_solve(f, u0, p, t0, tf, nvec, alg, ealg, saveat=None, abstol=1e-6, re
         if order.shape != (N,):
             raise ValueError(f"order has shape {order.shape}, expected ({N},)")
         O.order = _ptr(order)
-    res = dict(u_final=np.zeros((n, N)), t_final=np.empty(N), retcode=np.empty(N, np.int8),
-               naccept=np.empty(N, np.int32), nreject=np.empty(N, np.int32),
-               nf=np.empty(N, np.int32), nstiff=np.empty(N, np.int32), dt_first=np.empty(N))
-    if isinstance(f, UserModel) and f.callbacks:
+    if reduction is not None:
+        O.reduction = _abi.REDUCTION[reduction]
+    if keep_states:
+        res = dict(u_final=np.zeros((n, N)), t_final=np.empty(N), retcode=np.empty(N, np.int8),
+                   naccept=np.empty(N, np.int32), nreject=np.empty(N, np.int32),
+                   nf=np.empty(N, np.int32), nstiff=np.empty(N, np.int32),
+                   dt_first=np.empty(N))
+    else:
+        res = dict(u_final=None, t_final=None, retcode=None, naccept=None, nreject=None,
+                   nf=None, nstiff=None, dt_first=None)
+    if isinstance(f, UserModel) and f.callbacks and keep_states:
         res["nevent"] = np.zeros(N, np.int32)
         res["t_event"] = np.empty(N)
+    nout = f.output.nout if isinstance(f, UserModel) and f.output is not None else 0
+    if nout and keep_states:
+        res["out"] = np.empty((nout, N))
+    if nout and reduction is not None:
+        res["reduced"] = np.empty(nout)
     out = _abi.Outputs()
     out.us = _ptr(us)
     for k, v in res.items():
@@ -600,6 +647,17 @@ def solve(prob, alg, ensemblealg=None, *, trajectories=None, saveat=None, abstol
     N = int(trajectories)
     idx = np.arange(N)
     sa = _saveat_vector(saveat, prob.prob.tspan)
+    if isinstance(prob.output_func, DeviceOutputFunc):
+        # output_func (and reduction) on the device: one pass, no ODESolutions
+        red = prob.reduction if isinstance(prob.reduction, DeviceReduction) else None
+        f, u0, p, t0, tf, nvec, probs = _pack(prob, idx, 1)
+        arr = device_solve(f, u0, p, t0, tf, nvec, alg, ensemblealg, saveat=None,
+                           abstol=abstol, reltol=reltol, dt=dt, dtmin=dtmin, dtmax=dtmax,
+                           maxiters=maxiters, output=prob.output_func, reduction=red,
+                           keep_states=red is None, **kw)
+        u = arr.reduced if red is not None else arr.out.T
+        return EnsembleSolution(u=u, elapsedTime=time.perf_counter() - tstart,
+                                converged=True, arrays=arr)
     outputs = [None] * N
     repeat = 1
     pending = idx

@@ -144,6 +144,8 @@ typedef struct {
                            at natural positions.  The host entry rejects a non-permutation;
                            on the device entry, entries outside [0, ntraj) are skipped.
                            Overrides the built-in cost order of the Brusselator queue. */
+  int32_t reduction;    /* mideq_reduction over the trajectories' output_func values */
+  int32_t reserved2;
 } mideq_options;
 
 /* Results; any pointer may be NULL. */
@@ -159,6 +161,8 @@ typedef struct {
   double *dt_first;     /* [ntraj] size of the first accepted step */
   int32_t *nevent;      /* [ntraj] callback affects applied (user models with callbacks) */
   double *t_event;      /* [ntraj] time of the last callback affect (NaN if none) */
+  double *out;          /* [nout][ntraj] output_func values (user models with an output) */
+  double *reduced;      /* [nout] the reduction of `out` over the trajectories */
 } mideq_outputs;
 
 /* Timing and launch record of the last solve on a context. */
@@ -200,6 +204,17 @@ int mideq_ensemble_solve_device(mideq_ctx *ctx, const mideq_problem *prob,
 #define MIDEQ_USER_MODEL_BASE 1000 /* first id of a registered user model */
 #define MIDEQ_USER_NMAX 8          /* states (one lane per trajectory)     */
 #define MIDEQ_USER_NPMAX 16        /* parameters                           */
+/* On-device reduction of the output_func values over the trajectories
+ * (replaces reduction(u, batch, I) of EnsembleProblem, test/qa/qa.jl:50, for
+ * the common folds).  Order: blocks of 256 consecutive trajectories folded
+ * by a halving tree (element i with i + w, w = 128, 64, ..., 1; past the
+ * end the fold's identity), then the block results in block order; MEAN
+ * divides the sum by ntraj.  MIN / MAX ignore NaN (fmin / fmax). */
+typedef enum {
+  MIDEQ_REDUCE_NONE = 0, MIDEQ_REDUCE_SUM = 1, MIDEQ_REDUCE_MEAN = 2,
+  MIDEQ_REDUCE_MIN = 3, MIDEQ_REDUCE_MAX = 4
+} mideq_reduction;
+
 /* Callbacks of a user model.  Replaces ContinuousCallback(condition, affect!)
  * and DiscreteCallback(condition, affect!) passed as solve(...; callback)
  * (test/qa/qa.jl:26,36; test/core.jl:60-79).  A continuous callback fires
@@ -211,6 +226,7 @@ int mideq_ensemble_solve_device(mideq_ctx *ctx, const mideq_problem *prob,
  * re-evaluates the FSAL stage.  Saved values are the saveat grid's (no extra
  * points at events). */
 #define MIDEQ_USER_NCBMAX 8
+#define MIDEQ_USER_NOUTMAX 16
 typedef enum { MIDEQ_CB_CONTINUOUS = 0, MIDEQ_CB_DISCRETE = 1 } mideq_callback_kind;
 typedef struct {
   int32_t kind;          /* mideq_callback_kind */
@@ -235,6 +251,12 @@ typedef struct {
   int32_t ncallbacks;                /* 0..MIDEQ_USER_NCBMAX */
   int32_t reserved;
   const mideq_callback *callbacks;   /* [ncallbacks] or NULL */
+  int32_t nout;       /* on-device output_func: values per trajectory, 0..MIDEQ_USER_NOUTMAX */
+  int32_t reserved2;
+  const char *output; /* statements assigning double out[0..nout-1] from the final state
+                         u[], p[], the final time t and the retcode rc; replaces
+                         output_func(sol, i) -> (out, false) of EnsembleProblem
+                         (test/qa/qa.jl:50) for values of the final state */
 } mideq_user_model;
 
 /* Registers and compiles a user model on the context's device; *model_id
