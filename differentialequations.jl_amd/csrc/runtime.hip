@@ -279,7 +279,7 @@ static std::string user_model_source(const mideq_user_model* m) {
     s += table("cb_dir", [](const mideq_callback& c) { return (int)c.direction; });
     s += table("cb_term", [](const mideq_callback& c) { return (int)c.terminate; });
     s += table("cb_interp", [](const mideq_callback& c) {
-      return c.interp_points > 0 ? (int)c.interp_points : 10;
+      return c.interp_points > 0 ? (int)c.interp_points : 1;  // 0: the step end only
     });
     s += "  __device__ static double cb_g(int k, const double* u, const double* p, double t) {\n"
          "    switch (k) {\n";

@@ -336,7 +336,7 @@ int oracle_set_user_callbacks(int32_t ncb, const int32_t *kind, const int32_t *d
     g_user.kind[k] = kind[k];
     g_user.dir[k] = direction[k];
     g_user.term[k] = terminate[k];
-    g_user.ip[k] = interp_points[k] > 0 ? interp_points[k] : 10;
+    g_user.ip[k] = interp_points[k] > 0 ? interp_points[k] : 1; /* 0: the step end only */
   }
   g_user.cond = cond;
   g_user.affect = affect;

@@ -64,7 +64,7 @@ int oracle_set_user_model(int32_t n, int32_t np, oracle_user_f64 f64, oracle_use
                           oracle_user_jac jac);
 /* Callbacks of the user model (ncb = 0 clears them): per callback the kind
  * (0 continuous, 1 discrete), direction (0 both, 1 up, -1 down), terminate
- * flag and interpolation points (0 = 10); cond(k, u, p, t) and
+ * flag and interpolation points (0: the step end only); cond(k, u, p, t) and
  * affect(k, u, p, t) as the device compiles them. */
 typedef double (*oracle_user_cond)(int32_t k, const double *u, const double *p, double t);
 typedef void (*oracle_user_affect)(int32_t k, double *u, const double *p, double t);

@@ -85,12 +85,15 @@ def test_bouncing_ball(oracle, alg, precision, schedule):
         assert (g.us[:, 0, :] > -1e-6).all()  # the ball stays above the floor
 
 
-def test_terminate_and_callback_set(oracle):
+@pytest.mark.parametrize("interp_points", [10, 0])
+def test_terminate_and_callback_set(oracle, interp_points):
     """terminate! at u = 1/2 of u' = -k u (t = ln 2 / k) with a discrete
-    callback in the same set; saves after termination stay NaN."""
+    callback in the same set; saves after termination stay NaN.  With
+    interp_points = 0 only step ends are checked (a monotone condition)."""
     N = 512
-    say(f"terminate + CallbackSet N={N}")
-    cbs = CallbackSet(ContinuousCallback("u[0] - 0.5", terminate=True),
+    say(f"terminate + CallbackSet interp_points={interp_points} N={N}")
+    cbs = CallbackSet(ContinuousCallback("u[0] - 0.5", terminate=True,
+                                         interp_points=interp_points),
                       DiscreteCallback("t > 0.25 && t < 0.3", affect="u[0] = u[0] * 1.0;"))
     model = UserModel(1, 1, "du[0] = -p[0] * u[0];", callbacks=cbs)
     rng = np.random.default_rng(113)
