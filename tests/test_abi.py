@@ -79,3 +79,21 @@ def test_user_model_limits_match_header():
     get = lambda k: int(re.search(r"#define %s (\d+)" % k, txt).group(1))
     assert (get("MIDEQ_USER_MODEL_BASE"), get("MIDEQ_USER_NMAX"), get("MIDEQ_USER_NPMAX")) == (
         _abi.USER_MODEL_BASE, _abi.USER_NMAX, _abi.USER_NPMAX)
+
+
+def test_c_abi_example_builds_and_loads():
+    """The C example (examples/abi_example.c) is built with the library and
+    links libmideq.so; without a GPU it stops at mideq_ctx_create."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                       "differentialequations.jl_amd", "_build", "abi_example")
+    assert os.path.exists(exe)
+    try:
+        import torch
+        if torch.cuda.device_count() > 0:
+            pytest.skip("a GPU is present: tests/test_gpu_api.py runs it")
+    except ImportError:
+        pass
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 3 and "no GPU" in r.stdout

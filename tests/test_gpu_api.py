@@ -83,3 +83,17 @@ def test_vectorized_prob_func_large():
     a = es.arrays
     assert (a.retcode == 1).all()
     assert 500 < a.naccept.mean() < 1100
+
+
+def test_c_abi_example_program():
+    """differentialequations.jl_amd/examples/abi_example.c, built with the
+    library: the C ABI from plain C (built-in model, run-time compiled model
+    with a terminating callback and an on-device mean, a compile error)."""
+    import os
+    import subprocess
+    say("C ABI example program")
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                       "differentialequations.jl_amd", "_build", "abi_example")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, (r.stdout, r.stderr)
+    assert r.stdout.startswith("ok"), r.stdout
