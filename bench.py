@@ -320,7 +320,10 @@ def other_mix(torch, n_total=1_000_000, n_train=1_000_000):
     model with the predicted dt0, the predicted class as the initial method
     and a launch order that groups trajectories by predicted class and cost
     (longest first); learned = ordered with the division-free log2 PI
-    controller; learned_mixed = learned with FP32 stages.  The net is
+    controller; learned_mixed = learned with FP32 stages; learned_trust = the
+    predicted class picks the method (plain Tsit5 for the non-stiff class,
+    capped at 4x the predicted cost, AutoTsit5 for the stiff class and the
+    capped-out trajectories; auto_passes = 3).  The net is
     trained on the plain solver's logs of a separately seeded mix."""
     import mideq
     from mideq import EnsembleMI355X, device_solve
@@ -329,20 +332,21 @@ def other_mix(torch, n_total=1_000_000, n_train=1_000_000):
     F = {"lorenz": mideq.lorenz, "robertson": mideq.robertson, "vdp": mideq.vanderpol}
 
     def run(m, w, prec="fp64", dt0=None, hint=None, order=None, reltol=None, abstol=None,
-            ctl="reference"):
+            ctl="reference", passes=0, cap=None):
         at, rt = MIX_TOL[m]
         at = abstol or at
         rt = reltol or rt
         u0, p, tf = w["u0"], w["p"], w["tf"]
         N = u0.shape[1]
         return device_solve(F[m], u0, p, np.zeros(N), tf, None, AutoTsit5(),
-                            EnsembleMI355X(precision=prec, controller=ctl), abstol=at,
+                            EnsembleMI355X(precision=prec, controller=ctl, auto_passes=passes,
+                                           iter_cap=cap), abstol=at,
                             reltol=rt, dt0=dt0, alg_hint=hint, order=order,
                             maxiters=1_000_000)
 
     t_logs = time.perf_counter()
     train = _mix_sets(n_train // 3, seed=777)
-    Xs, Ts, Ls, Cs = [], [], [], []
+    Xs, Ts, Ls, Cs, Es = [], [], [], [], []
     for m, w in train.items():
         r = run(m, w)
         t, l, c = training_targets(r.dt_first, r.naccept, r.nstiff, r.nreject)
@@ -352,6 +356,7 @@ def other_mix(torch, n_total=1_000_000, n_train=1_000_000):
         Ts.append(t[ok])
         Ls.append(l[ok])
         Cs.append(c[ok])
+        Es.append((r.nstiff > 0.02 * r.naccept)[ok])  # over 2 % of its steps in Rosenbrock
     X = np.concatenate(Xs)
     T = np.concatenate(Ts)
     Lb = np.concatenate(Ls)
@@ -365,10 +370,19 @@ def other_mix(torch, n_total=1_000_000, n_train=1_000_000):
     losses = net.fit(Xd, Td, Ld, Cd, epochs=4, batch=8192, lr=3e-3)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t_train
+    # trust threshold, calibrated on the training logs: of the trajectories
+    # whose stiffness logit is below it, at most 1 % took over 2 % of their
+    # steps in Rosenbrock, so plain Tsit5 is trusted for them
+    lt = net.infer(Xd).cpu().numpy()[:, 1]
+    ever = np.concatenate(Es)
+    srt = np.argsort(lt, kind="stable")
+    frac = np.cumsum(ever[srt]) / np.arange(1, len(srt) + 1)
+    okk = np.where(frac <= 0.01)[0]
+    trust_thr = float(lt[srt][okk[-1]]) if len(okk) else -np.inf
 
     evals = _mix_sets(n_total // 3, seed=1234)
     forms = (("ordered", "fp64", "reference"), ("learned", "fp64", "fast"),
-             ("learned_mixed", "mixed", "fast"))
+             ("learned_mixed", "mixed", "fast"), ("learned_trust", "fp64", "fast"))
     res = {"plain": dict(kernel_ms=0.0, per_model_ms={})}
     for form, _, _ in forms:
         res[form] = dict(kernel_ms=0.0, infer_ms=0.0, per_model_ms={})
@@ -398,7 +412,23 @@ def other_mix(torch, n_total=1_000_000, n_train=1_000_000):
             order = od.cpu().numpy()
             dt0 = 10.0 ** Y[:, 0].astype(np.float64)
             stiff = Y[:, 1] > 0
-            r = run(m, w, prec=prec, dt0=dt0, hint=stiff.astype(np.int8), order=order, ctl=ctl)
+            if form == "learned_trust":
+                # the predictor picks the method: plain Tsit5 (no stiffness test)
+                # below the calibrated logit, capped at 4x the predicted cost;
+                # AutoTsit5 from t0 for the rest and for the capped-out
+                cap = np.minimum(4.0 * np.exp2(np.minimum(Y[:, 2], 30.0)) + 64.0,
+                                 1e9).astype(np.int32)
+                auto = Y[:, 1] > trust_thr
+                # hinted stiff: straight to AutoSwitch (starting in Rodas5); cap 0:
+                # to AutoSwitch before the first step (starting in Tsit5)
+                h = np.where(auto, stiff, 0).astype(np.int8)
+                cap = np.where(auto, 0, cap).astype(np.int32)
+                r = run(m, w, prec=prec, dt0=dt0, hint=h, order=order, ctl=ctl, passes=3,
+                        cap=cap)
+                res[form]["trusted"] = res[form].get("trusted", 0) + int((auto == 0).sum())
+            else:
+                r = run(m, w, prec=prec, dt0=dt0, hint=stiff.astype(np.int8), order=order,
+                        ctl=ctl)
             res[form]["kernel_ms"] += r.kernel_ms
             res[form]["per_model_ms"][m] = r.kernel_ms
             u = r.u_final
@@ -425,6 +455,7 @@ def other_mix(torch, n_total=1_000_000, n_train=1_000_000):
                          weighted_err_p99=float(np.quantile(e, 0.99)))
     for form, _, _ in forms:
         res[form]["speedup_vs_plain"] = res["plain"]["total_ms"] / res[form]["total_ms"]
+    res["learned_trust"]["trust_logit"] = trust_thr
     return dict(workload="1e6 mix: 1/3 Lorenz (1e-8), 1/3 Robertson (1e-8/1e-6), 1/3 classic "
                          "Van der Pol mu log-uniform 1..1e4 tspan (0, 2mu) (1e-8/1e-6)",
                 training=dict(samples=int(len(X)), log_solve_s=t_logs, train_s=t_train,

@@ -71,6 +71,7 @@ static hipError_t launch_alg(const LaneArgs& a, int alg, int sched, int block, i
       return dense ? launch_t<M, MIX, ALG_AUTO5P | ALG_DENSE, FAST>(a, sched, block, grid, s)
                    : launch_t<M, MIX, ALG_AUTO5P, FAST>(a, sched, block, grid, s);
     case ALG_AUTO_T: return launch_t<M, MIX, ALG_AUTO_T, FAST>(a, sched, block, grid, s);
+    case ALG_TRUST: return launch_t<M, MIX, ALG_TRUST, FAST>(a, sched, block, grid, s);
     default:
       return dense ? launch_t<M, MIX, ALG_AUTO | ALG_DENSE, FAST>(a, sched, block, grid, s)
                    : launch_t<M, MIX, ALG_AUTO, FAST>(a, sched, block, grid, s);

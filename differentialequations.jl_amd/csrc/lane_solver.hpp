@@ -18,8 +18,12 @@ namespace mideq {
 // test, handing every trajectory that would switch (or starts stiff) to a
 // second, full AutoSwitch pass; a trajectory that never switches takes the
 // same steps either way, so results equal the single-pass solve.
+// TRUST (internal): first pass of the predictor-trusting AutoSwitch solve --
+// plain Tsit5 (no stiffness test) for the trajectories hinted non-stiff,
+// handing to the full AutoSwitch pass every trajectory hinted stiff and every
+// one that exceeds its iteration cap (the predictor was wrong about it).
 enum { ALG_TSIT5 = 0, ALG_RODAS5 = 1, ALG_AUTO = 2, ALG_RODAS5P = 3, ALG_AUTO5P = 4,
-       ALG_AUTO_T = 5 };
+       ALG_AUTO_T = 5, ALG_TRUST = 6 };
 // Template flag on a Rosenbrock-capable ALG: build the Rosenbrock dense
 // output for saveat.  It keeps the 8 stage increments live past the step,
 // so kernels without saveat are built without it.
@@ -59,6 +63,7 @@ struct LaneArgs {
   const unsigned long long* order_len;  // device length of `order` (NULL: ntraj)
   int64_t* handoff;           // two-phase AutoSwitch: trajectories for the second pass
   unsigned long long* handoff_count;
+  const int32_t* iter_cap;    // TRUST pass: per-trajectory iteration cap (NULL: none)
 };
 
 // Trajectory scheduled at position k of the launch order; -1 for an entry
@@ -271,8 +276,9 @@ struct Lane {
   static constexpr bool kDense = (ALG & ALG_DENSE) != 0;
   static constexpr bool kAuto = (kAlg == ALG_AUTO || kAlg == ALG_AUTO5P || kAlg == ALG_AUTO_T);
   static constexpr bool kHasTsit5 = (kAlg != ALG_RODAS5 && kAlg != ALG_RODAS5P);
-  static constexpr bool kHasRodas = (kAlg != ALG_TSIT5 && kAlg != ALG_AUTO_T);
-  static constexpr bool kHandoff = (kAlg == ALG_AUTO_T);
+  static constexpr bool kHasRodas = (kAlg != ALG_TSIT5 && kAlg != ALG_AUTO_T && kAlg != ALG_TRUST);
+  static constexpr bool kTrust = (kAlg == ALG_TRUST);
+  static constexpr bool kHandoff = (kAlg == ALG_AUTO_T || kTrust);
   static constexpr bool kP = (kAlg == ALG_RODAS5P || kAlg == ALG_AUTO5P);  // Rodas5P tableau
   // the division-free stiffness test of the fast controller (n <= 3)
   static constexpr bool kFastStiff = FAST && N <= 3;
@@ -297,6 +303,7 @@ struct Lane {
   double gprev[kNcb > 0 ? kNcb : 1];
   int32_t nevent;
   double tevent;
+  int32_t cap;  // TRUST pass: iteration cap of this trajectory
 
   // ------------------------------------------------------------ save times
   __device__ __forceinline__ double save_time(const LaneArgs& a, int k) const {
@@ -402,6 +409,13 @@ struct Lane {
     if (kHandoff && mode == 1) {
       rc = RC_HANDOFF;  // starts stiff: the full AutoSwitch pass solves it
       return;
+    }
+    if constexpr (kTrust) {
+      if (a.alg_hint && a.alg_hint[i]) {
+        rc = RC_HANDOFF;  // predicted stiff: the AutoSwitch pass solves it
+        return;
+      }
+      cap = a.iter_cap ? a.iter_cap[i] : 0x7fffffff;
     }
     qsteady_max = (a.qsteady_default && mode == 1) ? 1.2 : 1.0;
     // saveat points before t0 are unreachable; a point at t0 is u0 itself
@@ -798,6 +812,12 @@ struct Lane {
     }
     iter++;
     int64_t maxit = a.maxiters > 0 ? a.maxiters : 100000;
+    if constexpr (kTrust) {
+      if (iter > cap && iter <= maxit) {
+        rc = RC_HANDOFF;  // more steps than predicted: hand it to AutoSwitch
+        return;
+      }
+    }
     if (iter > maxit) {
       rc = RC_MAXITERS;
       return;

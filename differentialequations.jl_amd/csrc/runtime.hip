@@ -34,7 +34,7 @@ struct DevBuf {
 enum {
   BUF_U0 = 0, BUF_P, BUF_T0, BUF_TF, BUF_NVEC, BUF_SAVEAT, BUF_DT0, BUF_HINT, BUF_US,
   BUF_UF, BUF_TFIN, BUF_RC, BUF_NA, BUF_NR, BUF_NF, BUF_NS, BUF_SCRATCH, BUF_ORDER,
-  BUF_SAMPLE, BUF_DTF, BUF_HANDOFF, BUF_NEV, BUF_TEV, BUF_OUT, BUF_RED, BUF_FOLD, BUF_COUNT
+  BUF_SAMPLE, BUF_DTF, BUF_HANDOFF, BUF_NEV, BUF_TEV, BUF_OUT, BUF_RED, BUF_FOLD, BUF_CAP, BUF_COUNT
 };
 
 // A run-time compiled right-hand side: its generated model source and the
@@ -507,7 +507,7 @@ static int validate(mideq_ctx* c, const mideq_problem* P, const mideq_options* O
   if (O->save_dtype < 0 || O->save_dtype > 1) return set_err(c, MIDEQ_E_ARG, "bad save_dtype");
   if (O->schedule < 0 || O->schedule > 2) return set_err(c, MIDEQ_E_ARG, "bad schedule");
   if (O->controller < 0 || O->controller > 1) return set_err(c, MIDEQ_E_ARG, "bad controller");
-  if (O->auto_passes < 0 || O->auto_passes > 2) return set_err(c, MIDEQ_E_ARG, "bad auto_passes");
+  if (O->auto_passes < 0 || O->auto_passes > 3) return set_err(c, MIDEQ_E_ARG, "bad auto_passes");
   if (O->reduction < 0 || O->reduction > 4) return set_err(c, MIDEQ_E_ARG, "bad reduction");
   if (O->reduction != 0) {
     UserModel* um = user_model(c, P->model);
@@ -770,7 +770,11 @@ static int run_solver(mideq_ctx* c, const mideq_problem* P, const mideq_options*
   // Opt-in: the Tsit5 pass re-solves the non-stiff start of every switching
   // trajectory, so it pays only when few trajectories switch.
   const bool two_pass = (O->alg == MIDEQ_ALG_AUTO || O->alg == MIDEQ_ALG_AUTO_RODAS5P) &&
-                        O->auto_passes == 2;
+                        O->auto_passes >= 2;
+  // the first pass: Tsit5 with the stiffness test (2), or plain Tsit5 for the
+  // trajectories hinted non-stiff, capped per trajectory (3)
+  const int first_alg = O->auto_passes == 3 ? ALG_TRUST : ALG_AUTO_T;
+  a.iter_cap = O->auto_passes == 3 ? O->iter_cap : nullptr;
   if (two_pass) {
     void* d;
     int rc0 = ensure_buf(c, BUF_HANDOFF, sizeof(int64_t) * (size_t)P->ntraj, &d);
@@ -781,8 +785,9 @@ static int run_solver(mideq_ctx* c, const mideq_problem* P, const mideq_options*
   HIP_TRY(c, hipEventRecord(c->ev0, s));
   struct Key {
     LaneArgs a;
-    int model, alg, prec, ctl, sched, block, grid, two_pass;
-  } key{a, P->model, O->alg, O->precision, O->controller, sched, block, grid, two_pass};
+    int model, alg, prec, ctl, sched, block, grid, two_pass, first_alg;
+  } key{a, P->model, O->alg, O->precision, O->controller, sched, block, grid, two_pass,
+        first_alg};
   // user model: build its kernels for this configuration before any capture
   UserModel* um = user_model(c, P->model);
   UserKernels* uk_main = nullptr;
@@ -791,7 +796,7 @@ static int run_solver(mideq_ctx* c, const mideq_problem* P, const mideq_options*
     int rc0 = user_kernels(c, um, lane_alg_code(a, O->alg, um->ncb > 0), O->precision,
                            O->controller, &uk_main);
     if (rc0 == MIDEQ_OK && two_pass)
-      rc0 = user_kernels(c, um, ALG_AUTO_T, O->precision, O->controller, &uk_first);
+      rc0 = user_kernels(c, um, first_alg, O->precision, O->controller, &uk_first);
     if (rc0) return rc0;
   }
   // on-device reduction of the output values: block partials in scratch
@@ -809,7 +814,7 @@ static int run_solver(mideq_ctx* c, const mideq_problem* P, const mideq_options*
   }
   auto launch = [&](const LaneArgs& la, int alg, hipStream_t st) -> hipError_t {
     if (um)
-      return launch_user(alg == ALG_AUTO_T ? uk_first : uk_main, la, sched, block, grid, c->ncu,
+      return launch_user(alg == first_alg ? uk_first : uk_main, la, sched, block, grid, c->ncu,
                          st);
     return launch_lane_solver(la, P->model, alg, O->precision, O->controller, sched, block, grid,
                               st);
@@ -828,7 +833,7 @@ static int run_solver(mideq_ctx* c, const mideq_problem* P, const mideq_options*
     }
     hipError_t e = launch_set_u64(a.handoff_count, 0ull, st);
     if (e != hipSuccess) return e;
-    e = launch(a, ALG_AUTO_T, st);
+    e = launch(a, first_alg, st);
     if (e != hipSuccess) return e;
     LaneArgs a2 = a;
     a2.order = a.handoff;
@@ -935,6 +940,11 @@ extern "C" int mideq_ensemble_solve(mideq_ctx* c, const mideq_problem* P,
     if ((rc = ensure_buf(c, BUF_HINT, N, &d))) return rc;
     HIP_TRY(c, hipMemcpyAsync(d, O->alg_hint, N, hipMemcpyHostToDevice, s));
     dO.alg_hint = (const int8_t*)d;
+  }
+  if (O->iter_cap && O->auto_passes == 3) {
+    if ((rc = ensure_buf(c, BUF_CAP, sizeof(int32_t) * N, &d))) return rc;
+    HIP_TRY(c, hipMemcpyAsync(d, O->iter_cap, sizeof(int32_t) * N, hipMemcpyHostToDevice, s));
+    dO.iter_cap = (const int32_t*)d;
   }
   // launch order: the caller's, else for the large-system kernel most
   // expensive first (LPT), cost ~ n * Ngrid^2 * (tf - t0), the Tsit5 work

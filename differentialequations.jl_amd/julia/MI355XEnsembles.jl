@@ -64,6 +64,7 @@ struct MideqOptions
     lanes::Int32; graph::Int32; controller::Int32; auto_passes::Int32
     order::Ptr{Int64}
     reduction::Int32; reserved2::Int32
+    iter_cap::Ptr{Int32}
 end
 
 struct MideqOutputs
@@ -246,7 +247,7 @@ function __solve(ens::SciMLBase.AbstractEnsembleProblem, alg, ealg::EnsembleMI35
                                  ns > 0 ? pointer(sa) : C_NULL, ns,
                                  ealg.save_dtype === Float32 ? 1 : 0, C_NULL, C_NULL, 1,
                                  ealg.schedule === :lane ? 1 : ealg.schedule === :persistent ? 2 : 0,
-                                 0, 0, 0, 0, 0, 0, C_NULL, 0, 0)
+                                 0, 0, 0, 0, 0, 0, C_NULL, 0, 0, C_NULL)
             out_c = MideqOutputs(ns > 0 ? pointer(US) : C_NULL, pointer(UF), pointer(TFo),
                                  pointer(RC), pointer(NA), pointer(NR), pointer(NF), pointer(NS),
                                  C_NULL, C_NULL, C_NULL, C_NULL, C_NULL)

@@ -59,7 +59,7 @@ class Options(C.Structure):
                 ("schedule", C.c_int32), ("block", C.c_int32), ("blocks_per_cu", C.c_int32),
                 ("lanes", C.c_int32), ("graph", C.c_int32), ("controller", C.c_int32),
                 ("auto_passes", C.c_int32), ("order", C.c_void_p),
-                ("reduction", C.c_int32), ("reserved2", C.c_int32)]
+                ("reduction", C.c_int32), ("reserved2", C.c_int32), ("iter_cap", C.c_void_p)]
 
 
 class Outputs(C.Structure):

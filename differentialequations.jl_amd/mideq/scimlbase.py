@@ -368,7 +368,10 @@ class EnsembleMI355X:
     divergence; results keep their natural positions.  device: GPU ordinal.
     auto_passes (AutoTsit5 on n <= 3 models): 0/1 = one AutoSwitch pass;
     2 = a Tsit5 pass, then the full AutoSwitch kernel over the trajectories
-    that switch (faster when few switch).  Both give identical results."""
+    that switch (faster when few switch).  Both give identical results.
+    3 = trust alg_hint: plain Tsit5 for the trajectories hinted non-stiff,
+    each capped at iter_cap[i] attempted steps, then AutoSwitch from t0 for
+    the capped-out and the hinted-stiff ones."""
     precision: str = "fp64"
     schedule: str = "auto"
     device: int = 0
@@ -380,6 +383,7 @@ class EnsembleMI355X:
     graph: bool = False
     controller: str = "reference"   # or "fast" (n <= 3 models)
     auto_passes: int = 0
+    iter_cap: Any = None
 
 
 # ---------------------------------------------------------------- solutions
@@ -560,6 +564,12 @@ def device_solve(f, u0, p, t0, tf, nvec, alg, ealg, saveat=None, abstol=1e-6, re
     O.graph = 1 if ealg.graph else 0
     O.controller = dict(reference=0, fast=1)[ealg.controller]
     O.auto_passes = int(ealg.auto_passes)
+    iter_cap = None
+    if ealg.iter_cap is not None:
+        iter_cap = np.ascontiguousarray(ealg.iter_cap, dtype=np.int32)
+        if iter_cap.shape != (N,):
+            raise ValueError(f"iter_cap has shape {iter_cap.shape}, expected ({N},)")
+        O.iter_cap = _ptr(iter_cap)
     f32 = ealg.save_dtype == "float32"
     O.save_dtype = 1 if f32 else 0
     us = None
@@ -617,6 +627,8 @@ def _subset_ealg(ealg, pending):
         kw["dt0"] = np.asarray(ealg.dt0)[pending]
     if ealg.alg_hint is not None:
         kw["alg_hint"] = np.asarray(ealg.alg_hint)[pending]
+    if ealg.iter_cap is not None:
+        kw["iter_cap"] = np.asarray(ealg.iter_cap)[pending]
     if ealg.order is not None:
         rank = np.empty(len(ealg.order), dtype=np.int64)
         rank[np.asarray(ealg.order, dtype=np.int64)] = np.arange(len(ealg.order))

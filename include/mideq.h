@@ -136,7 +136,13 @@ typedef struct {
                            Results are identical.  Two passes pay when a minority of
                            the batch switches and the schedule is lane (Van der Pol,
                            mu in [1, 1e4], 78% switching: 5.7 -> 5.0 ms); they cost
-                           when all switch.  Ignored elsewhere. */
+                           when all switch.  3 = trust the predictor: trajectories with
+                           alg_hint 0 run plain Tsit5 (no stiffness test) until done or
+                           past iter_cap[i] attempted steps; those, and the ones hinted
+                           stiff, then run the full AutoSwitch kernel from t0.  A
+                           trajectory finished in the first pass has Tsit5's result, which
+                           equals AutoTsit5's when it never switches.  Ignored
+                           elsewhere. */
   const int64_t *order; /* [ntraj] launch order or NULL (natural order).  A permutation
                            of 0..ntraj-1: entry k is the k-th trajectory scheduled, so
                            consecutive entries share a wavefront -- grouping trajectories
@@ -146,6 +152,10 @@ typedef struct {
                            Overrides the built-in cost order of the Brusselator queue. */
   int32_t reduction;    /* mideq_reduction over the trajectories' output_func values */
   int32_t reserved2;
+  const int32_t *iter_cap; /* [ntraj] auto_passes = 3: iteration cap of the plain Tsit5 pass
+                              per trajectory (e.g. a multiple of the predicted cost; 0
+                              sends it to AutoSwitch before its first step, starting in
+                              Tsit5), or NULL for none */
 } mideq_options;
 
 /* Results; any pointer may be NULL. */

@@ -99,3 +99,35 @@ def test_graph_replays_two_pass():
     for rep in range(3):
         g = gpu("vdp", w["u0"], w["p"], 0.0, tf, graph=True, auto_passes=2, **kw)
         _same(ref, g)
+
+
+@pytest.mark.parametrize("alg", ["auto", "auto5p"])
+def test_trust_pass(alg):
+    """auto_passes = 3 (trust the hints): a trajectory hinted non-stiff that
+    finishes within its cap has plain Tsit5's result; every other one has
+    the single AutoSwitch pass's result for the same hints.  Some stiff
+    trajectories are hinted non-stiff on purpose, so the cap hands them on."""
+    N = 4099
+    say(f"vdp {alg} trust pass N={N}")
+    w, tf = _vdp_mixed(N, 86)
+    kw = dict(abstol=1e-8, reltol=1e-6, maxiters=1_000_000)
+    plain = gpu("vdp", w["u0"], w["p"], 0.0, tf, alg=alg, **kw)
+    tsit = gpu("vdp", w["u0"], w["p"], 0.0, tf, alg="tsit5", **kw)
+    hint = (plain.nstiff > 0).astype(np.int8)
+    rng = np.random.default_rng(3)
+    wrong = rng.choice(np.where(hint == 1)[0], 200, replace=False)
+    hint[wrong] = 0
+    cap = (3 * (plain.naccept + plain.nreject) + 16).astype(np.int32)
+    auto_h = gpu("vdp", w["u0"], w["p"], 0.0, tf, alg=alg, alg_hint=hint, **kw)
+    trust = gpu("vdp", w["u0"], w["p"], 0.0, tf, alg=alg, alg_hint=hint, iter_cap=cap,
+                auto_passes=3, **kw)
+    first = (hint == 0) & (tsit.naccept + tsit.nreject <= cap)
+    assert first.sum() > 100 and (~first).sum() > 100
+    capped = (hint == 0) & ~first
+    assert capped.sum() > 0  # the cap caught mispredicted trajectories
+    for k in KEYS:
+        t, a, b = getattr(trust, k), getattr(tsit, k), getattr(auto_h, k)
+        sel = (slice(None), first) if t.ndim == 2 else first
+        np.testing.assert_array_equal(t[sel], a[sel], err_msg=k)
+        sel = (slice(None), ~first) if t.ndim == 2 else ~first
+        np.testing.assert_array_equal(t[sel], b[sel], err_msg=k)
