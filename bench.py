@@ -47,6 +47,9 @@ FORMS = {
     "mixed": ("mixed", 1e-6, "lane", "float32", 0),
     "tuned": ("fp64", 1e-8, "lane", "float64", 1),
     "tuned_persistent": ("fp64", 1e-8, "persistent", "float64", 1),
+    # the loosest tolerance whose FP64 solve still meets the error gate
+    # (median 4.7e-7, 94 % of trajectories within 1e-6 vs the 1e-12 solve)
+    "tuned_tol": ("fp64", 1e-7, "lane", "float64", 1),
 }
 
 
@@ -56,11 +59,11 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--ntraj", type=int, default=1 << 20)
-    ap.add_argument("--forms", default="plain,mixed,tuned,tuned_persistent")
+    ap.add_argument("--forms", default="plain,mixed,tuned,tuned_persistent,tuned_tol")
     ap.add_argument("--other", default="robertson,brusselator,mix",
                     help="other configurations to time (comma list, '' for none)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=20000)
+    ap.add_argument("--cpu-sample", type=int, default=131072)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (HBM counter calibration)")
     return ap.parse_args()
@@ -489,7 +492,8 @@ def cpu_baseline(sample, ns=100):
     return dict(value=sample / dt, unit="trajectories/s", cores=threads, kind="port",
                 sample=f"first {sample} of the 1,048,576 seeded Lorenz trajectories, Tsit5 FP64 "
                        f"abstol=reltol=1e-8, saveat 100 points, oracle EnsembleThreads "
-                       f"restatement on {threads} host threads, {dt:.2f} s")
+                       f"restatement on {threads} host threads, {dt:.2f} s wall "
+                       f"(~{dt * threads:.0f} thread-seconds)")
 
 
 def main():
@@ -521,7 +525,8 @@ def main():
         r["traj_per_s"] = a.ntraj * a.steps / r["wall_s"]
         flops = r["attempted_steps"] * FLOPS_PER_STEP + a.ntraj * 100 * FLOPS_PER_SAVE
         r["tflops"] = flops / (r["kernel_ms_mean"] * 1e-3) / 1e12
-        r["gate_ok"] = err["median"] <= 1e-6 and r["all_success"]
+        r["gate_ok"] = (err["median"] <= 1e-6 and err["frac_le_1e6"] >= 0.9
+                        and r["all_success"])
         detail[f] = r
     ok = [f for f in forms if detail[f]["gate_ok"]]
     head = max(ok, key=lambda f: detail[f]["traj_per_s"]) if ok else forms[0]
@@ -557,7 +562,9 @@ def main():
                                "seeded, u0=[1,0,0], tspan (0,10), Tsit5, saveat 100 uniform "
                                "points, SoA output 100x3xN",
                    "ntraj": a.ntraj, "headline_form": head,
-                   "gate": "median final-state rel err vs FP64 tol-1e-12 solve <= 1e-6"},
+                   "headline_tol": h["tol"],
+                   "gate": "final-state rel err vs FP64 tol-1e-12 solve: median <= 1e-6 and "
+                           ">= 90 % of trajectories <= 1e-6"},
         "roofline": roof,
         "cpu_baseline": cpu,
         "forms": {f: {k: v for k, v in d.items() if k != "kernel_ms"} for f, d in detail.items()},

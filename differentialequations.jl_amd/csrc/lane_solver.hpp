@@ -191,17 +191,25 @@ __device__ __forceinline__ double rms(const double* x) {
   return sqrt(rms_sum_small<N>(x) / (double)N);
 }
 
-// Tsit5 dense-output weights b_i(theta).
+// Tsit5 dense-output weights b_i(theta), in the power basis: b_1 = theta +
+// r12 theta^2 + r13 theta^3 + r14 theta^4 (r11 = 1), b_i = r_i2 theta^2 +
+// r_i3 theta^3 + r_i4 theta^4, lowest power first.  Every operation has at
+// most one method constant (an SGPR operand), so a save costs 3 + 21 FP64
+// instructions for the weights; Horner's innermost fma(theta, r_i4, r_i3)
+// has two and costs two extra VGPR moves per constant.
 __device__ __forceinline__ void tsit5_bthetas(double th, double* b) {
   using namespace ts;
+  static_assert(r11 == 1.0, "b_1 starts at theta");
   double th2 = th * th;
-  b[0] = th * fma(th, fma(th, fma(th, r14, r13), r12), r11);
-  b[1] = th2 * fma(th, fma(th, r24, r23), r22);
-  b[2] = th2 * fma(th, fma(th, r34, r33), r32);
-  b[3] = th2 * fma(th, fma(th, r44, r43), r42);
-  b[4] = th2 * fma(th, fma(th, r54, r53), r52);
-  b[5] = th2 * fma(th, fma(th, r64, r63), r62);
-  b[6] = th2 * fma(th, fma(th, r74, r73), r72);
+  double th3 = th2 * th;
+  double th4 = th2 * th2;
+  b[0] = fma(r14, th4, fma(r13, th3, fma(r12, th2, th)));
+  b[1] = fma(r24, th4, fma(r23, th3, r22 * th2));
+  b[2] = fma(r34, th4, fma(r33, th3, r32 * th2));
+  b[3] = fma(r44, th4, fma(r43, th3, r42 * th2));
+  b[4] = fma(r54, th4, fma(r53, th3, r52 * th2));
+  b[5] = fma(r64, th4, fma(r63, th3, r62 * th2));
+  b[6] = fma(r74, th4, fma(r73, th3, r72 * th2));
 }
 
 // AutoSwitch stiffness estimate max_c |dk_c / dg_c| over the components.
@@ -316,8 +324,10 @@ struct Lane {
 
   // ------------------------------------------------------------ helpers
   __device__ __forceinline__ void store_save(const LaneArgs& a, int k, const double* v) {
+    // element (k, 0, i) of the [ns][N][ntraj] output; N * ntraj < 2^32 (checked
+    // by the host), so the offset is one 32 x 32 -> 64-bit multiply-add
     const int64_t NT = a.ntraj;
-    const int64_t o = (int64_t)k * (N * NT) + i;
+    const int64_t o = (int64_t)((uint64_t)(uint32_t)k * (uint32_t)(N * NT) + (uint64_t)i);
     if (a.save_f32) {
       float* out = (float*)a.us + o;
 #pragma unroll

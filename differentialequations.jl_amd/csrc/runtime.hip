@@ -504,6 +504,9 @@ static int validate(mideq_ctx* c, const mideq_problem* P, const mideq_options* O
   if (!(O->abstol > 0) || !(O->reltol > 0))
     return set_err(c, MIDEQ_E_ARG, "abstol and reltol must be > 0");
   if (O->ns < 0 || (O->ns > 0 && !O->saveat)) return set_err(c, MIDEQ_E_ARG, "bad saveat");
+  if (O->ns > 0 && (int64_t)P->n * P->ntraj >= ((int64_t)1 << 32))
+    return set_err(c, MIDEQ_E_UNSUPPORTED, "saveat needs n * ntraj < 2^32 (got %lld); split the "
+                   "ensemble", (long long)((int64_t)P->n * P->ntraj));
   if (O->save_dtype < 0 || O->save_dtype > 1) return set_err(c, MIDEQ_E_ARG, "bad save_dtype");
   if (O->schedule < 0 || O->schedule > 2) return set_err(c, MIDEQ_E_ARG, "bad schedule");
   if (O->controller < 0 || O->controller > 1) return set_err(c, MIDEQ_E_ARG, "bad controller");

@@ -115,7 +115,8 @@ typedef struct {
   double dtmin;         /* > 0: DtLessThanMin below it; 0: off */
   double dtmax;         /* > 0: cap; 0: |tf - t0| */
   int64_t maxiters;     /* 0: 100000 */
-  const double *saveat; /* [ns] ascending absolute times or NULL (endpoint only) */
+  const double *saveat; /* [ns] ascending absolute times or NULL (endpoint only); with
+                           saveat, n * ntraj must be < 2^32 (MIDEQ_E_UNSUPPORTED) */
   int32_t ns;
   int32_t save_dtype;   /* 0: double, 1: float for `us` */
   const double *dt0;    /* [ntraj] per-trajectory initial dt (learned) or NULL */

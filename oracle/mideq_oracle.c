@@ -910,12 +910,18 @@ void oracle_tsit5_step(int32_t model, int32_t n, const double *u, const double *
 
 /* Tsit5 free interpolant: u(theta) = uprev + dt * sum_i b_i(theta) k_i,     */
 /* b_1 = theta*evalpoly(theta, r11..r14), b_i = theta^2*evalpoly(theta,      */
-/* r_i2..r_i4) for i >= 2, Horner with fma, sum with @muladd order.          */
+/* r_i2..r_i4) for i >= 2 (the upstream Tsit5 interpolant's polynomials),    */
+/* evaluated in the power basis lowest power first with fma (r11 = 1):       */
+/* b_1 = fma(r14, th^4, fma(r13, th^3, fma(r12, th^2, th))), b_i =           */
+/* fma(ri4, th^4, fma(ri3, th^3, ri2 th^2)), th^3 = th^2 th, th^4 = th^2 th^2; */
+/* the sum over stages in @muladd order.                                     */
 static void tsit5_bthetas(double th, double b[7]) {
   double th2 = th * th;
-  b[0] = th * fma(th, fma(th, fma(th, TS_R[0][3], TS_R[0][2]), TS_R[0][1]), TS_R[0][0]);
+  double th3 = th2 * th;
+  double th4 = th2 * th2;
+  b[0] = fma(TS_R[0][3], th4, fma(TS_R[0][2], th3, fma(TS_R[0][1], th2, th)));
   for (int i = 1; i < 7; i++)
-    b[i] = th2 * fma(th, fma(th, TS_R[i][3], TS_R[i][2]), TS_R[i][1]);
+    b[i] = fma(TS_R[i][3], th4, fma(TS_R[i][2], th3, TS_R[i][1] * th2));
 }
 
 static void tsit5_interp64(int n, const double *uprev, double k[7][NMAX],
