@@ -36,9 +36,9 @@ PEAK_FP64 = 78.6   # TFLOP/s, MI355X FP64 vector
 PEAK_HBM = 8000.0  # GB/s
 # HBM bytes per launch of the solver kernel, from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE passes scaled by a 1 GiB copy kernel (factors 2.0 and 1.0);
-# profiles/tsit5_lorenz_1M_saveat100_hbm_counters.md.  Re-measured when
-# the kernel changes; None where no counter pass exists for that form.
-MEASURED_TRAFFIC = {"plain": 11549978186, "tuned": 12420613856}
+# profiles/tsit5_lorenz_1M_saveat100_fastctl_hbm_counters.md.  Re-measured
+# when the kernel changes; None where no counter pass exists for that form.
+MEASURED_TRAFFIC = {"tuned_tol": 12370973536}
 ALGORITHMIC_BYTES_PER_TRAJ = 48 + 100 * 3 * 8 + 24 + 8 + 1 + 12
 
 FORMS = {
