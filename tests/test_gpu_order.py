@@ -122,3 +122,32 @@ def test_device_entry_skips_out_of_range_entries():
     keep = np.arange(N) != missing
     np.testing.assert_array_equal(rch[keep], ref.retcode[keep])
     np.testing.assert_array_equal(uf.cpu().numpy()[:, keep], ref.u_final[:, keep])
+
+
+def test_saveat_size_limit_rejected_before_any_access():
+    """n * ntraj >= 2^32 with saveat is refused by validation, which runs
+    before any input is read (the buffers here hold one trajectory)."""
+    say("saveat size limit: n*ntraj >= 2^32 rejected")
+    ctx = _abi.context(0)
+    u0 = np.zeros(3)
+    p = np.ones(3)
+    sa = np.array([0.5])
+    P = _abi.Problem()
+    P.model = _abi.MODEL["lorenz"]
+    P.n = 3
+    P.np = 3
+    P.ntraj = 1 << 31
+    P.u0 = u0.ctypes.data
+    P.p = p.ctypes.data
+    P.tspan0 = 0.0
+    P.tspan1 = 1.0
+    O = _abi.Options()
+    O.alg = _abi.ALG["tsit5"]
+    O.abstol = O.reltol = 1e-6
+    O.saveat = sa.ctypes.data
+    O.ns = 1
+    out = _abi.Outputs()
+    us = np.zeros(3)
+    out.us = us.ctypes.data
+    with pytest.raises(_abi.MideqError, match="2\\^32"):
+        ctx.solve(P, O, out)
