@@ -581,14 +581,36 @@ struct BTeam {
         ldsr[k] = rr;
         ldsi[k] = pk;
       }
+      double l[C];
 #pragma unroll
       for (int m = 0; m < C; m++) {
         int r = rowc(m);
+        l[m] = 0.0;
         if (r > k && r < n) {
-          double l = W[(int64_t)k * n + r] * rr;
-          W[(int64_t)k * n + r] = l;
-          for (int c = k + 1; c < n; c++)
-            W[(int64_t)c * n + r] = fma(-l, W[(int64_t)c * n + k], W[(int64_t)c * n + r]);
+          l[m] = W[(int64_t)k * n + r] * rr;
+          W[(int64_t)k * n + r] = l[m];
+        }
+      }
+      // trailing update a_rc -= l_r a_kc, kUpd columns at a time: all loads
+      // of a chunk issue before its stores, so a chunk waits on one memory
+      // round trip instead of one per column (row k is not written here)
+      constexpr int kUpd = 8;
+      for (int c0 = k + 1; c0 < n; c0 += kUpd) {
+        double wk[kUpd];
+#pragma unroll
+        for (int j = 0; j < kUpd; j++) wk[j] = (c0 + j < n) ? W[(int64_t)(c0 + j) * n + k] : 0.0;
+#pragma unroll
+        for (int m = 0; m < C; m++) {
+          int r = rowc(m);
+          if (r > k && r < n) {
+            double wr[kUpd];
+#pragma unroll
+            for (int j = 0; j < kUpd; j++)
+              wr[j] = (c0 + j < n) ? W[(int64_t)(c0 + j) * n + r] : 0.0;
+#pragma unroll
+            for (int j = 0; j < kUpd; j++)
+              if (c0 + j < n) W[(int64_t)(c0 + j) * n + r] = fma(-l[m], wk[j], wr[j]);
+          }
         }
       }
       team_sync();
@@ -610,12 +632,15 @@ struct BTeam {
 #pragma unroll
     for (int m = 0; m < C; m++) y[m] = (rowc(m) < n) ? ldsd[sigma[m]] : 0.0;
     team_sync();
+    // x_j lives in lane j % L, slot j / L: broadcast it with a shuffle, so
+    // no memory fence stands between consecutive columns and the W loads
+    // of later columns can issue early
     for (int j = 0; j < n; j++) {
+      const int mj = j / L;
+      double own = y[0];
 #pragma unroll
-      for (int m = 0; m < C; m++)
-        if (rowc(m) == j) ldsd[j] = y[m];
-      team_sync();
-      double xj = ldsd[j];
+      for (int m = 1; m < C; m++) own = (m == mj) ? y[m] : own;
+      double xj = __shfl(own, j % L, L);
 #pragma unroll
       for (int m = 0; m < C; m++) {
         int r = rowc(m);
@@ -623,15 +648,14 @@ struct BTeam {
       }
     }
     for (int j = n - 1; j >= 0; j--) {
+      const int mj = j / L;
 #pragma unroll
-      for (int m = 0; m < C; m++) {
-        if (rowc(m) == j) {
-          y[m] = y[m] * ldsr[j];
-          ldsd[j] = y[m];
-        }
-      }
-      team_sync();
-      double xj = ldsd[j];
+      for (int m = 0; m < C; m++)
+        if (rowc(m) == j) y[m] = y[m] * ldsr[j];
+      double own = y[0];
+#pragma unroll
+      for (int m = 1; m < C; m++) own = (m == mj) ? y[m] : own;
+      double xj = __shfl(own, j % L, L);
 #pragma unroll
       for (int m = 0; m < C; m++) {
         int r = rowc(m);
