@@ -316,6 +316,13 @@ def _mix_sets(n_each, seed):
 MIX_TOL = {"lorenz": (1e-8, 1e-8), "robertson": (1e-8, 1e-6), "vdp": (1e-8, 1e-6)}
 
 
+# learned_trust: step cap = TRUST_CAP[0] x predicted cost + TRUST_CAP[1]; the
+# trusted set is the predicted non-stiff class below the calibrated logit
+# (TRUST_CALIBRATED) or the whole predicted non-stiff class
+TRUST_CAP = (4.0, 64.0)
+TRUST_CALIBRATED = True
+
+
 def other_mix(torch, n_total=1_000_000, n_train=1_000_000):
     """Learned dt0 / method class / cost on the 1e6 Lorenz + Robertson + Van
     der Pol mix: plain = AutoTsit5(Rodas5()) FP64 with the initdt heuristic,
@@ -419,9 +426,9 @@ def other_mix(torch, n_total=1_000_000, n_train=1_000_000):
                 # the predictor picks the method: plain Tsit5 (no stiffness test)
                 # below the calibrated logit, capped at 4x the predicted cost;
                 # AutoTsit5 from t0 for the rest and for the capped-out
-                cap = np.minimum(4.0 * np.exp2(np.minimum(Y[:, 2], 30.0)) + 64.0,
+                cap = np.minimum(TRUST_CAP[0] * np.exp2(np.minimum(Y[:, 2], 30.0)) + TRUST_CAP[1],
                                  1e9).astype(np.int32)
-                auto = Y[:, 1] > trust_thr
+                auto = Y[:, 1] > (trust_thr if TRUST_CALIBRATED else 0.0)
                 # hinted stiff: straight to AutoSwitch (starting in Rodas5); cap 0:
                 # to AutoSwitch before the first step (starting in Tsit5)
                 h = np.where(auto, stiff, 0).astype(np.int8)
