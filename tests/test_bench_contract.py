@@ -1,0 +1,33 @@
+"""bench.py's command-line contract, checked without a GPU: more than one
+GPU is refused with a JSON message, and the form table names the fields the
+measurement reads."""
+import json
+import os
+import subprocess
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    line = json.loads(r.stdout.strip().splitlines()[-1])
+    assert line["n_gpus"] == 2 and "one GPU" in line["error"]
+
+
+def test_forms_and_gate():
+    sys.path.insert(0, ROOT)
+    import bench
+    for name, (precision, tol, schedule, save_dtype, controller) in bench.FORMS.items():
+        assert precision in ("fp64", "mixed")
+        assert 0 < tol <= 1e-6
+        assert schedule in ("lane", "persistent")
+        assert save_dtype in ("float64", "float32")
+        assert controller in (0, 1)
+    # the plain form is the configuration's FP64 Tsit5 at 1e-8 with the
+    # reference controller; every measured traffic entry names a form
+    assert bench.FORMS["plain"] == ("fp64", 1e-8, "lane", "float64", 0)
+    assert set(bench.MEASURED_TRAFFIC) <= set(bench.FORMS)
+    assert bench.FLOPS_PER_STEP == 262 and bench.FLOPS_PER_SAVE == 98
