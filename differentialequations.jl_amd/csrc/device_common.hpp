@@ -77,6 +77,13 @@ constexpr double C[8][7] = {
     {42.57076742291101, -13.80770672017997, 93.98938432427124,
      18.77919633714503, -31.58359187223370, -6.685968952921985,
      -5.810979938412932}};
+// stage times c_i (row sums of the A coefficients of the stage inputs) and
+// the time-derivative weights d_i (row sums of Gamma), used by
+// non-autonomous right-hand sides
+constexpr double c[8] = {0.0, 0.38, 0.3878509998321533, 0.4839718937873840,
+                         0.4570477008819580, 1.0, 1.0, 1.0};
+constexpr double d[8] = {0.19, -0.1823079225333714636, -0.319231832186874912,
+                         0.3449828624712811, -0.377417564392089818, 0.0, 0.0, 0.0};
 // Continuous extension (this library's derivation; see rd_H below).
 constexpr double H[3][7] = {
     {46.335061333323615, -6.136914790748748, 108.40712618588684, -15.432459674306823,
@@ -113,6 +120,10 @@ constexpr double C[8][7] = {
      -19.097331116725623, -28.087943162872662, 0},
     {37.80277123390563, -3.2571969029072276, 112.26918849496327, 66.9347231244047,
      -40.06618937091002, -54.66780262877968, -9.48861652309627}};
+constexpr double c[8] = {0.0, 0.6358126895828704, 0.4095798393397535, 0.9769306725060716,
+                         0.4288403609558664, 1.0, 1.0, 1.0};
+constexpr double d[8] = {0.21193756319429014, -0.42387512638858027, -0.3384627126235924,
+                         1.8046452872882734, 2.325825639765069, 0.0, 0.0, 0.0};
 constexpr double H[3][7] = {
     {18.16834780539935, -2.4039764856508183, -28.49033949791586, -39.2353239248467,
      24.203612973337464, 31.138884328789313, 5.813992390809251},
@@ -133,6 +144,10 @@ template <bool P>
 __device__ __forceinline__ constexpr double rd_C(int s, int j) {
   return P ? rd5p::C[s][j] : rd::C[s][j];
 }
+template <bool P>
+__device__ __forceinline__ constexpr double rd_c(int s) { return P ? rd5p::c[s] : rd::c[s]; }
+template <bool P>
+__device__ __forceinline__ constexpr double rd_d(int s) { return P ? rd5p::d[s] : rd::d[s]; }
 // Dense output of a Rosenbrock step (saveat inside a Rodas5/Rodas5P step):
 //   u(th) = (1-th) uprev + th (u1 + (1-th)(K1 + th (K2 + th K3))),
 //   K_m = sum_{j<7} H[m][j] k_j

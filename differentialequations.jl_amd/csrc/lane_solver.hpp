@@ -274,9 +274,22 @@ struct out_count<M, decltype((void)M::NOUT, void())> {
   static constexpr int value = M::NOUT;
 };
 
+// Non-autonomous right-hand side u' = f(u, p, t): a model with TDEP = true
+// takes the time as a last argument of f and jac and gives tgrad(dT, u, p, t)
+// = df/dt; the built-in models are autonomous.
+template <class M, class = void>
+struct time_dep {
+  static constexpr bool value = false;
+};
+template <class M>
+struct time_dep<M, decltype((void)M::TDEP, void())> {
+  static constexpr bool value = M::TDEP;
+};
+
 template <class M, bool MIX, int ALG, bool FAST>
 struct Lane {
   static constexpr int N = M::N;
+  static constexpr bool kTdep = time_dep<M>::value;
   static constexpr int kNcb = cb_count<M>::value;
   static constexpr int kNout = out_count<M>::value;
   static constexpr int NP = M::NP;
@@ -339,13 +352,23 @@ struct Lane {
     }
   }
 
+  // f at time tt (ignored by autonomous models); T is double, float or a
+  // dual number, the time is passed in that type
+  template <class T>
+  __device__ __forceinline__ void rhs(T* du, const T* uu, const T* pp, double tt) const {
+    if constexpr (kTdep)
+      M::f(du, uu, pp, (T)tt);
+    else
+      M::f(du, uu, pp);
+  }
+
   __device__ __forceinline__ void fsal_init() {
-    M::f(k1, u, p);
+    rhs(k1, u, p, t);
     if (MIX) {
       float uf[N];
 #pragma unroll
       for (int c = 0; c < N; c++) uf[c] = (float)u[c];
-      M::f(k1f, uf, pf);
+      rhs(k1f, uf, pf, t);
     }
   }
 
@@ -369,7 +392,7 @@ struct Lane {
     double u1[N], f1[N];
 #pragma unroll
     for (int c = 0; c < N; c++) u1[c] = fma(dt0, k1[c], u[c]);
-    M::f(f1, u1, p);
+    rhs(f1, u1, p, t0 + dt0);
     nf += 1;
     bool same = true;
 #pragma unroll
@@ -474,14 +497,14 @@ struct Lane {
       k[0][c] = k1[c];
       g[c] = fma(aa, k[0][c], u[c]);
     }
-    M::f(k[1], g, p);
+    rhs(k[1], g, p, t + ts::c2 * dts);
 #pragma unroll
     for (int c = 0; c < N; c++) {
       double acc = a32 * k[1][c];
       acc = fma(a31, k[0][c], acc);
       g[c] = fma(dts, acc, u[c]);
     }
-    M::f(k[2], g, p);
+    rhs(k[2], g, p, t + ts::c3 * dts);
 #pragma unroll
     for (int c = 0; c < N; c++) {
       double acc = a43 * k[2][c];
@@ -489,7 +512,7 @@ struct Lane {
       acc = fma(a41, k[0][c], acc);
       g[c] = fma(dts, acc, u[c]);
     }
-    M::f(k[3], g, p);
+    rhs(k[3], g, p, t + ts::c4 * dts);
 #pragma unroll
     for (int c = 0; c < N; c++) {
       double acc = a54 * k[3][c];
@@ -498,7 +521,7 @@ struct Lane {
       acc = fma(a51, k[0][c], acc);
       g[c] = fma(dts, acc, u[c]);
     }
-    M::f(k[4], g, p);
+    rhs(k[4], g, p, t + ts::c5 * dts);
 #pragma unroll
     for (int c = 0; c < N; c++) {
       double acc = a65 * k[4][c];
@@ -508,7 +531,7 @@ struct Lane {
       acc = fma(a61, k[0][c], acc);
       g6[c] = fma(dts, acc, u[c]);
     }
-    M::f(k[5], g6, p);
+    rhs(k[5], g6, p, t + dts);
 #pragma unroll
     for (int c = 0; c < N; c++) {
       double acc = a76 * k[5][c];
@@ -519,7 +542,7 @@ struct Lane {
       acc = fma(a71, k[0][c], acc);
       u1[c] = fma(dts, acc, u[c]);
     }
-    M::f(k[6], u1, p);
+    rhs(k[6], u1, p, t + dts);
     if (kAuto) {
       double dk[N], dg[N];
 #pragma unroll
@@ -557,14 +580,14 @@ struct Lane {
     float aa = d32 * (float)a21;
 #pragma unroll
     for (int c = 0; c < N; c++) g[c] = fmaf(aa, k[0][c], u32[c]);
-    M::f(k[1], g, pf);
+    rhs(k[1], g, pf, t + ts::c2 * dts);
 #pragma unroll
     for (int c = 0; c < N; c++) {
       float acc = (float)a32 * k[1][c];
       acc = fmaf((float)a31, k[0][c], acc);
       g[c] = fmaf(d32, acc, u32[c]);
     }
-    M::f(k[2], g, pf);
+    rhs(k[2], g, pf, t + ts::c3 * dts);
 #pragma unroll
     for (int c = 0; c < N; c++) {
       float acc = (float)a43 * k[2][c];
@@ -572,7 +595,7 @@ struct Lane {
       acc = fmaf((float)a41, k[0][c], acc);
       g[c] = fmaf(d32, acc, u32[c]);
     }
-    M::f(k[3], g, pf);
+    rhs(k[3], g, pf, t + ts::c4 * dts);
 #pragma unroll
     for (int c = 0; c < N; c++) {
       float acc = (float)a54 * k[3][c];
@@ -581,7 +604,7 @@ struct Lane {
       acc = fmaf((float)a51, k[0][c], acc);
       g[c] = fmaf(d32, acc, u32[c]);
     }
-    M::f(k[4], g, pf);
+    rhs(k[4], g, pf, t + ts::c5 * dts);
 #pragma unroll
     for (int c = 0; c < N; c++) {
       float acc = (float)a65 * k[4][c];
@@ -591,7 +614,7 @@ struct Lane {
       acc = fmaf((float)a61, k[0][c], acc);
       g6[c] = fmaf(d32, acc, u32[c]);
     }
-    M::f(k[5], g6, pf);
+    rhs(k[5], g6, pf, t + dts);
 #pragma unroll
     for (int c = 0; c < N; c++) {
       float acc = (float)a76 * k[5][c];
@@ -603,7 +626,7 @@ struct Lane {
       u1[c] = fma(dts, (double)acc, u[c]);
       u1f[c] = (float)u1[c];
     }
-    M::f(k[6], u1f, pf);
+    rhs(k[6], u1f, pf, t + dts);
     if (kAuto) {
       double dk[N], dg[N];
 #pragma unroll
@@ -633,7 +656,10 @@ struct Lane {
   __device__ bool rodas5_step(double dts, double* u1, double& EEst, double abstol,
                               double reltol, double (&kd)[8][N]) {
     double W[N][N];
-    M::jac(W, u, p);
+    if constexpr (kTdep)
+      M::jac(W, u, p, t);
+    else
+      M::jac(W, u, p);
     if (kAuto) {
       double best = 0.0;
 #pragma unroll
@@ -693,6 +719,10 @@ struct Lane {
       EEst = 2.0;
       return false;
     }
+    double dT[N];  // df/dt at (uprev, t); zero for autonomous models
+#pragma unroll
+    for (int c = 0; c < N; c++) dT[c] = 0.0;
+    if constexpr (kTdep) M::tgrad(dT, u, p, t);
     double U[N], du[N], x[N];
     float Uf[N], duf[N];
 #pragma unroll
@@ -713,7 +743,7 @@ struct Lane {
 #pragma unroll
           for (int c = 0; c < N; c++) U[c] = U[c] + kd[s - 1][c];
         }
-        M::f(du, U, p);
+        rhs(du, U, p, t + rd_c<kP>(s) * dts);
       } else {
         if (s == 0) {
 #pragma unroll
@@ -730,26 +760,28 @@ struct Lane {
 #pragma unroll
           for (int c = 0; c < N; c++) Uf[c] = Uf[c] + (float)kd[s - 1][c];
         }
-        M::f(duf, Uf, pf);
+        rhs(duf, Uf, pf, t + rd_c<kP>(s) * dts);
 #pragma unroll
         for (int c = 0; c < N; c++) du[c] = (double)duf[c];
       }
 #pragma unroll
       for (int c = 0; c < N; c++) {
+        double base = du[c];
+        if constexpr (kTdep) base = base + (dts * rd_d<kP>(s)) * dT[c];
         double r;
         if (s == 0) {
-          r = du[c];
+          r = base;
         } else if (!MIX) {
           double acc = (rd_C<kP>(s, 0) * inv_dt) * kd[0][c];
 #pragma unroll
           for (int j = 1; j < s; j++) acc = fma(rd_C<kP>(s, j) * inv_dt, kd[j][c], acc);
-          r = du[c] + acc;
+          r = base + acc;
         } else {
           float idt = (float)inv_dt;
           float acc = ((float)rd_C<kP>(s, 0) * idt) * (float)kd[0][c];
 #pragma unroll
           for (int j = 1; j < s; j++) acc = fmaf((float)rd_C<kP>(s, j) * idt, (float)kd[j][c], acc);
-          r = (double)((float)du[c] + acc);
+          r = (double)((float)base + acc);
         }
         x[c] = -r;
       }

@@ -7,6 +7,7 @@
 // fan-out of test/qa/qa.jl:50,56 -- for problems packed into SoA buffers.
 #include <hip/hip_runtime.h>
 #include <hip/hiprtc.h>
+#include <cctype>
 
 #include <algorithm>
 #include <cmath>
@@ -241,14 +242,28 @@ static int check_user_spec(const mideq_user_model* m, std::string* why) {
 
 // struct ModelUser: the user's statements as the templated RHS (T = double,
 // float, or Dual<N> for the Jacobian when none is given).
+// Does the statement text use the identifier t (the time)?  Then the model
+// is non-autonomous: f and jac take t, and df/dt comes from dual numbers.
+static bool uses_time(const char* text) {
+  if (!text) return false;
+  auto ident = [](char ch) { return std::isalnum((unsigned char)ch) || ch == '_'; };
+  for (size_t k = 0; text[k]; k++)
+    if (text[k] == 't' && (k == 0 || !ident(text[k - 1])) && !ident(text[k + 1])) return true;
+  return false;
+}
+
 static std::string user_model_source(const mideq_user_model* m) {
   const int np = m->np > 0 ? m->np : 1;
+  const bool tdep = uses_time(m->rhs) || uses_time(m->jac);
   std::string s = "namespace mideq {\nstruct ModelUser {\n  static constexpr int N = " +
                   std::to_string(m->n) + ", NP = " + std::to_string(np) + ";\n";
-  s += "  template <class T>\n  __device__ static void f(T* du, const T* u, const T* p) {\n";
+  s += std::string("  static constexpr bool TDEP = ") + (tdep ? "true" : "false") + ";\n";
+  s += std::string("  template <class T>\n  __device__ static void f(T* du, const T* u, const T* p") +
+       (tdep ? ", T t" : "") + ") {\n";
   s += "#line 1 \"rhs\"\n";
   s += m->rhs;
-  s += "\n  }\n  __device__ static void jac(double (*J)[N], const double* u, const double* p) {\n";
+  s += std::string("\n  }\n  __device__ static void jac(double (*J)[N], const double* u, "
+                   "const double* p") + (tdep ? ", double t" : "") + ") {\n";
   if (m->jac) {
     s += "#line 1 \"jac\"\n";
     s += m->jac;
@@ -256,12 +271,22 @@ static std::string user_model_source(const mideq_user_model* m) {
   } else {
     s += "    Dual<N> ud[N], pd[NP], fd[N];\n"
          "    for (int i = 0; i < N; i++) { ud[i].v = u[i]; ud[i].d[i] = 1.0; }\n"
-         "    for (int j = 0; j < NP; j++) pd[j].v = p[j];\n"
-         "    f(fd, ud, pd);\n"
-         "    for (int i = 0; i < N; i++)\n"
+         "    for (int j = 0; j < NP; j++) pd[j].v = p[j];\n";
+    s += tdep ? "    f(fd, ud, pd, Dual<N>(t));\n" : "    f(fd, ud, pd);\n";
+    s += "    for (int i = 0; i < N; i++)\n"
          "      for (int j = 0; j < N; j++) J[i][j] = fd[i].d[j];\n";
   }
   s += "  }\n";
+  if (tdep)  // df/dt with the time as the one dual direction
+    s += "  __device__ static void tgrad(double* dT, const double* u, const double* p, "
+         "double t) {\n"
+         "    Dual<1> ud[N], pd[NP], fd[N], td(t);\n"
+         "    td.d[0] = 1.0;\n"
+         "    for (int i = 0; i < N; i++) ud[i].v = u[i];\n"
+         "    for (int j = 0; j < NP; j++) pd[j].v = p[j];\n"
+         "    f(fd, ud, pd, td);\n"
+         "    for (int i = 0; i < N; i++) dT[i] = fd[i].d[0];\n"
+         "  }\n";
   // callbacks: kind / direction / terminate / interp points as switches
   const int ncb = m->ncallbacks;
   if (ncb > 0) {

@@ -118,8 +118,8 @@ DeviceCallback(kind::Symbol, condition::AbstractString; kw...) =
     register_model!(f!, n, np, rhs; jac = nothing, device = 0)
 
 Registers `f!` with a device right-hand side written as C++ statements for
-an autonomous u' = f(u, p): assign `du[0..n-1]` from `u[i]`, `p[j]`
-(0-based); the number type is `T`, constants `2.0` or `T(2)`, functions
+u' = f(u, p, t): assign `du[0..n-1]` from `u[i]`, `p[j]` (0-based) and the
+time `t` (naming `t` makes the model non-autonomous); the number type is `T`, constants `2.0` or `T(2)`, functions
 `sqrt exp log sin cos tanh fabs pow(x, c)`.  `jac` assigns `J[i][j]`;
 without it the Rosenbrock methods differentiate `rhs` in forward mode.
 The model id belongs to `device`'s context.  Example (test/core.jl:22-27):

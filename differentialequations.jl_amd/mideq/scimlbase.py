@@ -156,8 +156,10 @@ def CallbackSet(*cbs):
 class UserModel:
     """A right-hand side given as C++ statements, compiled for the GPU at run
     time (hipRTC) -- the counterpart of passing an in-place f!(du, u, p, t)
-    to ODEProblem (test/core.jl:22-27).  `rhs` assigns du[0..n-1] from u[i]
-    and p[j] for an autonomous system; its number type is T (double, float
+    to ODEProblem (test/core.jl:22-27).  `rhs` assigns du[0..n-1] from u[i],
+    p[j] and the time t (statements that name t make the model non-autonomous:
+    stage times, and Rosenbrock's d_i dt df/dt terms with df/dt by dual
+    numbers); its number type is T (double, float
     for FP32 stages, or a dual number when differentiated), so constants are
     written 2.0 or T(2), and sqrt exp log sin cos tanh fabs pow(x, c) are
     available.  `jac` assigns J[i][j] = df_i/du_j (double); without it the

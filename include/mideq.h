@@ -255,11 +255,14 @@ typedef struct {
 typedef struct {
   int32_t n;        /* states, 1..MIDEQ_USER_NMAX */
   int32_t np;       /* parameters, 0..MIDEQ_USER_NPMAX */
-  const char *rhs;  /* statements of an autonomous u' = f(u, p): assign du[0..n-1] from
-                       u[i] and p[j].  The number type is T (double; float for FP32
-                       stages; a dual number when differentiated): write T(2) or 2.0 for
-                       constants and use sqrt exp log sin cos tanh fabs pow(x, c). */
-  const char *jac;  /* statements assigning double J[i][j] = df_i/du_j from u[], p[];
+  const char *rhs;  /* statements of u' = f(u, p, t): assign du[0..n-1] from u[i], p[j]
+                       and the time t (a model whose statements name t is non-autonomous:
+                       t is the stage time, and Rosenbrock steps add d_i dt df/dt with
+                       df/dt by forward-mode differentiation).  The number type is T
+                       (double; float for FP32 stages; a dual number when
+                       differentiated), t included: write T(2) or 2.0 for constants and
+                       use sqrt exp log sin cos tanh fabs pow(x, c). */
+  const char *jac;  /* statements assigning double J[i][j] = df_i/du_j from u[], p[], t;
                        NULL: forward-mode automatic differentiation of rhs (the default
                        of OrdinaryDiffEq's Rosenbrock methods without a Jacobian) */
   int32_t ncallbacks;                /* 0..MIDEQ_USER_NCBMAX */

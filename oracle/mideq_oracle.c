@@ -321,6 +321,8 @@ static struct {
   oracle_user_f64 f64;
   oracle_user_f32 f32;
   oracle_user_jac jac;
+  oracle_user_tgrad tgrad;
+  int tdep;
   int ncb;
   int kind[8], dir[8], term[8], ip[8];
   oracle_user_cond cond;
@@ -344,13 +346,15 @@ int oracle_set_user_callbacks(int32_t ncb, const int32_t *kind, const int32_t *d
 }
 
 int oracle_set_user_model(int32_t n, int32_t np, oracle_user_f64 f64, oracle_user_f32 f32,
-                          oracle_user_jac jac) {
-  if (n < 1 || n > NMAX || np < 0 || np > 16 || !f64) return -1;
+                          oracle_user_jac jac, oracle_user_tgrad tgrad, int32_t tdep) {
+  if (n < 1 || n > NMAX || np < 0 || np > 16 || !f64 || (tdep && !tgrad)) return -1;
   g_user.n = n;
   g_user.np = np;
   g_user.f64 = f64;
   g_user.f32 = f32;
   g_user.jac = jac;
+  g_user.tgrad = tgrad;
+  g_user.tdep = tdep != 0;
   return 0;
 }
 
@@ -358,7 +362,7 @@ static void rhs64(const mdl_t *m, const double *u, double t, double *du) {
   const double *p = m->p;
   switch (m->model) {
     case ORACLE_MODEL_USER:
-      g_user.f64(du, u, p);
+      g_user.f64(du, u, p, t);
       break;
     case ORACLE_MODEL_EXP:
       du[0] = p[0] * u[0];
@@ -416,7 +420,7 @@ static void rhs32(const mdl_t *m, const float *u, double t, float *du) {
   const float *p = m->pf;
   switch (m->model) {
     case ORACLE_MODEL_USER:
-      g_user.f32(du, u, p);
+      g_user.f32(du, u, p, (float)t);
       break;
     case ORACLE_MODEL_EXP:
       du[0] = p[0] * u[0];
@@ -477,7 +481,7 @@ static void jac64(const mdl_t *m, const double *u, double t, double *J) {
   (void)t;
   switch (m->model) {
     case ORACLE_MODEL_USER:
-      g_user.jac(J, u, p);
+      g_user.jac(J, u, p, t);
       break;
     case ORACLE_MODEL_EXP:
       J[0] = p[0];
@@ -530,12 +534,16 @@ static void jac64(const mdl_t *m, const double *u, double t, double *J) {
   }
 }
 
-static int model_autonomous(int model) { return model != ORACLE_MODEL_PROTHERO; }
+static int model_autonomous(int model) {
+  if (model == ORACLE_MODEL_USER) return !g_user.tdep;
+  return model != ORACLE_MODEL_PROTHERO;
+}
 
 static void tgrad64(const mdl_t *m, const double *u, double t, double *dT) {
-  (void)u;
   if (m->model == ORACLE_MODEL_PROTHERO) {
     dT[0] = -m->p[0] * cos(t) - sin(t);
+  } else if (m->model == ORACLE_MODEL_USER && g_user.tdep) {
+    g_user.tgrad(dT, u, m->p, t);
   } else {
     for (int i = 0; i < m->n; i++) dT[i] = 0.0;
   }

@@ -55,13 +55,16 @@ enum {
 };
 
 /* A run-time compiled RHS on the CPU: the tests compile the same statements
- * the device compiles (f for double and float, J row-major n*n) and hand the
- * functions over.  One user model at a time (process-wide). */
-typedef void (*oracle_user_f64)(double *du, const double *u, const double *p);
-typedef void (*oracle_user_f32)(float *du, const float *u, const float *p);
-typedef void (*oracle_user_jac)(double *J, const double *u, const double *p);
+ * the device compiles (f for double and float, J row-major n*n, df/dt) and
+ * hand the functions over.  The time argument is the stage time (float for
+ * the FP32 form); tdep != 0 marks a non-autonomous model (Rosenbrock steps
+ * then add the d_i dt df/dt terms).  One user model at a time (process-wide). */
+typedef void (*oracle_user_f64)(double *du, const double *u, const double *p, double t);
+typedef void (*oracle_user_f32)(float *du, const float *u, const float *p, float t);
+typedef void (*oracle_user_jac)(double *J, const double *u, const double *p, double t);
+typedef void (*oracle_user_tgrad)(double *dT, const double *u, const double *p, double t);
 int oracle_set_user_model(int32_t n, int32_t np, oracle_user_f64 f64, oracle_user_f32 f32,
-                          oracle_user_jac jac);
+                          oracle_user_jac jac, oracle_user_tgrad tgrad, int32_t tdep);
 /* Callbacks of the user model (ncb = 0 clears them): per callback the kind
  * (0 continuous, 1 discrete), direction (0 both, 1 up, -1 down), terminate
  * flag and interpolation points (0: the step end only); cond(k, u, p, t) and

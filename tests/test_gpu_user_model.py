@@ -143,3 +143,53 @@ def test_user_model_graph_replays_and_errors():
     decay = UserModel(1, 0, "du[0] = -T(2) * u[0];")
     r = run(decay, np.ones((1, 8)), np.zeros((1, 8)), 0.0, 1.0, abstol=1e-12, reltol=1e-12)
     np.testing.assert_allclose(r.u_final[0], np.exp(-2.0), rtol=1e-9)
+
+
+# non-autonomous user models: polynomial forcing, so that device and host
+# arithmetic agree bit for bit (no libm calls)
+PR_POLY = "du[0] = p[0] * (u[0] - (t * t - t)) + (2.0 * t - 1.0);"
+FORCED = "du[0] = u[1];\ndu[1] = -p[0] * p[0] * u[0] + t;"
+
+
+@pytest.mark.parametrize("alg,precision,controller", [
+    ("rodas5", "fp64", "reference"), ("rodas5p", "fp64", "fast"), ("rodas5", "mixed", "reference"),
+    ("auto", "fp64", "reference"), ("tsit5", "fp64", "fast")])
+def test_user_time_dependent_prothero_vs_oracle(oracle, alg, precision, controller):
+    """y' = lambda (y - phi(t)) + phi'(t), phi = t^2 - t, y(0) = phi(0): the
+    solution is phi.  Stage times, the d_i dt df/dt terms and the dual-number
+    df/dt bit-exact against the oracle, saveat included."""
+    N = 1024
+    say(f"user non-autonomous Prothero-Robinson {alg} {precision} {controller} N={N}")
+    umh.install(oracle, 1, 1, PR_POLY)
+    model = UserModel(1, 1, PR_POLY, name="prothero_poly")
+    rng = np.random.default_rng(107)
+    lam = -(10.0 ** rng.uniform(0.0, 4.0 if alg != "tsit5" else 1.0, N))
+    sa = np.linspace(0.0, 2.0, 9)
+    kw = dict(saveat=sa, abstol=1e-8, reltol=1e-6)
+    g = run(model, np.zeros((1, N)), lam[None, :], 0.0, 2.0, alg=alg, precision=precision,
+            controller=controller, **kw)
+    c = cpu(oracle, "user", np.zeros((1, N)), lam[None, :], 0.0, 2.0, alg=alg,
+            precision=precision, controller=1 if controller == "fast" else 0, **kw)
+    assert (c["retcode"] == 1).all()
+    assert_identical(g, c)
+    if precision == "fp64":
+        np.testing.assert_allclose(g.u_final[0], 2.0, atol=1e-5)
+
+
+@pytest.mark.parametrize("alg", ["tsit5", "auto5p"])
+def test_user_time_dependent_forced_oscillator(oracle, alg):
+    """x'' = -w^2 x + t from rest, saveat on [0, 6]: bit-exact against the
+    oracle and within tolerance of x = t / w^2 - sin(w t) / w^3."""
+    N = 2048
+    say(f"user forced oscillator {alg} N={N}")
+    umh.install(oracle, 2, 1, FORCED)
+    model = UserModel(2, 1, FORCED, name="forced_oscillator")
+    w = np.linspace(0.5, 3.0, N)
+    sa = np.linspace(0.0, 6.0, 13)
+    kw = dict(saveat=sa, abstol=1e-10, reltol=1e-10)
+    g = run(model, np.zeros((2, N)), w[None, :], 0.0, 6.0, alg=alg, **kw)
+    c = cpu(oracle, "user", np.zeros((2, N)), w[None, :], 0.0, 6.0, alg=alg, **kw)
+    assert (c["retcode"] == 1).all()
+    assert_identical(g, c)
+    x = sa[:, None] / w[None, :] ** 2 - np.sin(sa[:, None] * w[None, :]) / w[None, :] ** 3
+    np.testing.assert_allclose(g.us[:, 0, :], x, rtol=1e-6, atol=1e-8)

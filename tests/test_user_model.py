@@ -44,7 +44,7 @@ def _jac(lib, n, u, p):
     u = np.ascontiguousarray(u, dtype=np.float64)
     p = np.ascontiguousarray(p, dtype=np.float64)
     dp = C.POINTER(C.c_double)
-    lib.user_jac(J.ctypes.data_as(dp), u.ctypes.data_as(dp), p.ctypes.data_as(dp))
+    lib.user_jac(J.ctypes.data_as(dp), u.ctypes.data_as(dp), p.ctypes.data_as(dp), 0.0)
     return J.reshape(n, n)
 
 
@@ -53,7 +53,7 @@ def _f(lib, u, p):
     dp = C.POINTER(C.c_double)
     u = np.ascontiguousarray(u, dtype=np.float64)
     p = np.ascontiguousarray(p, dtype=np.float64)
-    lib.user_f64(du.ctypes.data_as(dp), u.ctypes.data_as(dp), p.ctypes.data_as(dp))
+    lib.user_f64(du.ctypes.data_as(dp), u.ctypes.data_as(dp), p.ctypes.data_as(dp), 0.0)
     return du
 
 
@@ -174,3 +174,51 @@ def test_oracle_core_jl_callbacks(oracle):
     r4 = oracle.ensemble("lorenz", u0, p, 0.0, 1.0, abstol=1e-6, reltol=1e-3)
     np.testing.assert_array_equal(r3["u"], r4["u"])
     assert r3["nevent"][0] == 0
+
+
+# non-autonomous right-hand sides (ODEProblem f(du, u, p, t)): the time is
+# the stage time, Rosenbrock steps add d_i dt df/dt with df/dt by dual numbers
+PROTHERO = "du[0] = p[0] * (u[0] - sin(t)) + cos(t);"
+FORCED = "du[0] = u[1];\ndu[1] = -p[0] * p[0] * u[0] + t;"
+
+
+def test_time_dependence_detected():
+    mideq = _mideq()
+    assert mideq.UserModel(1, 1, PROTHERO).check() == ""  # f(u, p, t), jac and tgrad compile
+    assert mideq.UserModel(2, 1, FORCED, jac="J[0][0] = 0.0; J[0][1] = 1.0;\n"
+                           "J[1][0] = -p[0] * p[0] + 0.0 * t; J[1][1] = 0.0;").check() == ""
+    assert umh.uses_time(PROTHERO) and umh.uses_time(FORCED)
+    assert not umh.uses_time(LORENZ) and not umh.uses_time(ROBERTSON)
+    assert not umh.uses_time("T tt = u[0]; du[0] = tt * sqrt(u[0]);")
+    assert umh.uses_time("du[0] = (t);")
+
+
+@pytest.mark.parametrize("alg", ["rodas5", "rodas5p", "tsit5"])
+def test_oracle_user_prothero_equals_builtin(oracle, alg):
+    """The user form of y' = lambda (y - sin t) + cos t equals the oracle's
+    built-in one bit for bit: same RHS expression, J = lambda exactly from
+    the dual numbers, and df/dt from Dual<1> rounding like the analytic
+    -lambda cos t - sin t; its solution is sin t (y(0) = 0)."""
+    umh.install(oracle, 1, 1, PROTHERO)
+    N = 64
+    lam = -np.logspace(1, 5, N)
+    kw = dict(alg=alg, abstol=1e-9, reltol=1e-7)
+    a = oracle.ensemble("user", np.zeros((1, N)), lam[None, :], 0.0, 3.0, **kw)
+    b = oracle.ensemble("prothero", np.zeros((1, N)), lam[None, :], 0.0, 3.0, **kw)
+    assert (a["retcode"] == 1).all()
+    for k in ("u", "t", "naccept", "nreject", "nf"):
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+    np.testing.assert_allclose(a["u"][0], np.sin(3.0), atol=1e-6)
+
+
+@pytest.mark.parametrize("alg", ["tsit5", "rodas5"])
+def test_oracle_user_forced_oscillator_closed_form(oracle, alg):
+    """x'' = -w^2 x + t from rest: x = t / w^2 - sin(w t) / w^3."""
+    umh.install(oracle, 2, 1, FORCED)
+    N = 32
+    w = np.linspace(0.5, 3.0, N)
+    r = oracle.ensemble("user", np.zeros((2, N)), w[None, :], 0.0, 4.0, alg=alg,
+                        abstol=1e-11, reltol=1e-11)
+    assert (r["retcode"] == 1).all()
+    x = 4.0 / w**2 - np.sin(4.0 * w) / w**3
+    np.testing.assert_allclose(r["u"][0], x, rtol=1e-7, atol=1e-9)

@@ -9,6 +9,7 @@ hands its functions to the oracle (oracle_set_user_model)."""
 import ctypes as C
 import hashlib
 import os
+import re
 import subprocess
 import tempfile
 
@@ -17,29 +18,51 @@ DUAL = os.path.join(ROOT, "differentialequations.jl_amd", "csrc", "user_dual.hpp
 _CACHE = {}
 
 
+def uses_time(text):
+    """The device's rule: the statements name the identifier t."""
+    return bool(text) and re.search(r"(?<![A-Za-z0-9_])t(?![A-Za-z0-9_])", text) is not None
+
+
 def model_source(n, np_, rhs, jac=None, callbacks=()):
     np_ = max(int(np_), 1)
+    tdep = uses_time(rhs) or uses_time(jac)
+    ta, tj = (", T t", ", double t") if tdep else ("", "")
     s = ["#include <math.h>", "#define MIDEQ_HD inline", f'#include "{DUAL}"',
          "namespace mideq {", "struct ModelUser {",
          f"  static constexpr int N = {n}, NP = {np_};",
-         "  template <class T>", "  static void f(T* du, const T* u, const T* p) {",
-         rhs, "  }", "  static void jac(double (*J)[N], const double* u, const double* p) {"]
+         "  template <class T>", f"  static void f(T* du, const T* u, const T* p{ta}) {{",
+         rhs, "  }", f"  static void jac(double (*J)[N], const double* u, const double* p{tj}) {{"]
     if jac is not None:
         s.append(jac)
     else:
         s += ["    Dual<N> ud[N], pd[NP], fd[N];",
               "    for (int i = 0; i < N; i++) { ud[i].v = u[i]; ud[i].d[i] = 1.0; }",
               "    for (int j = 0; j < NP; j++) pd[j].v = p[j];",
-              "    f(fd, ud, pd);",
+              "    f(fd, ud, pd, Dual<N>(t));" if tdep else "    f(fd, ud, pd);",
               "    for (int i = 0; i < N; i++)",
               "      for (int j = 0; j < N; j++) J[i][j] = fd[i].d[j];"]
-    s += ["  }", "};", "}  // namespace mideq",
-          'extern "C" void user_f64(double* du, const double* u, const double* p) '
-          "{ mideq::ModelUser::f<double>(du, u, p); }",
-          'extern "C" void user_f32(float* du, const float* u, const float* p) '
-          "{ mideq::ModelUser::f<float>(du, u, p); }",
-          'extern "C" void user_jac(double* J, const double* u, const double* p) '
-          "{ mideq::ModelUser::jac((double (*)[mideq::ModelUser::N])J, u, p); }"]
+    s.append("  }")
+    if tdep:
+        s += ["  static void tgrad(double* dT, const double* u, const double* p, double t) {",
+              "    Dual<1> ud[N], pd[NP], fd[N], td(t);",
+              "    td.d[0] = 1.0;",
+              "    for (int i = 0; i < N; i++) ud[i].v = u[i];",
+              "    for (int j = 0; j < NP; j++) pd[j].v = p[j];",
+              "    f(fd, ud, pd, td);",
+              "    for (int i = 0; i < N; i++) dT[i] = fd[i].d[0];",
+              "  }"]
+    targ = ", t" if tdep else ""
+    s += ["};", "}  // namespace mideq",
+          'extern "C" void user_f64(double* du, const double* u, const double* p, double t) '
+          f"{{ (void)t; mideq::ModelUser::f<double>(du, u, p{targ}); }}",
+          'extern "C" void user_f32(float* du, const float* u, const float* p, float t) '
+          f"{{ (void)t; mideq::ModelUser::f<float>(du, u, p{targ}); }}",
+          'extern "C" void user_jac(double* J, const double* u, const double* p, double t) '
+          f"{{ (void)t; mideq::ModelUser::jac((double (*)[mideq::ModelUser::N])J, u, p{targ}); }}",
+          'extern "C" void user_tgrad(double* dT, const double* u, const double* p, double t) '
+          + ("{ mideq::ModelUser::tgrad(dT, u, p, t); }" if tdep else
+             "{ (void)u; (void)p; (void)t; for (int i = 0; i < mideq::ModelUser::N; i++) dT[i] = 0.0; }"),
+          f'extern "C" int user_tdep(void) {{ return {1 if tdep else 0}; }}']
     # callbacks, as the device compiles them (double)
     s.append('extern "C" double user_cb_g(int k, const double* u, const double* p, double t) {')
     s.append("  switch (k) {")
@@ -70,9 +93,10 @@ def build(n, np_, rhs, jac=None, callbacks=()):
     lib = C.CDLL(so)
     dp = C.POINTER(C.c_double)
     fp = C.POINTER(C.c_float)
-    lib.user_f64.argtypes = [dp, dp, dp]
-    lib.user_f32.argtypes = [fp, fp, fp]
-    lib.user_jac.argtypes = [dp, dp, dp]
+    lib.user_f64.argtypes = [dp, dp, dp, C.c_double]
+    lib.user_f32.argtypes = [fp, fp, fp, C.c_float]
+    lib.user_jac.argtypes = [dp, dp, dp, C.c_double]
+    lib.user_tgrad.argtypes = [dp, dp, dp, C.c_double]
     lib.user_cb_g.argtypes = [C.c_int32, dp, dp, C.c_double]
     lib.user_cb_g.restype = C.c_double
     _CACHE[key] = lib
@@ -86,9 +110,9 @@ def install(oracle, n, np_, rhs, jac=None, callbacks=()):
     addr = lambda fn: C.cast(fn, C.c_void_p).value
     L = oracle.lib
     L.oracle_set_user_model.argtypes = [C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
-                                        C.c_void_p]
+                                        C.c_void_p, C.c_void_p, C.c_int32]
     rc = L.oracle_set_user_model(n, max(int(np_), 1), addr(lib.user_f64), addr(lib.user_f32),
-                                 addr(lib.user_jac))
+                                 addr(lib.user_jac), addr(lib.user_tgrad), lib.user_tdep())
     assert rc == 0
     k = len(callbacks)
     arr = lambda vals: (C.c_int32 * max(k, 1))(*vals)
