@@ -205,18 +205,41 @@ function _saveat_vector(saveat, tspan)
 end
 
 function __solve(ens::SciMLBase.AbstractEnsembleProblem, alg, ealg::EnsembleMI355X;
-                 trajectories, saveat = nothing, abstol = 1e-6, reltol = 1e-3,
-                 dt = 0.0, dtmin = 0.0, dtmax = 0.0, maxiters = 100_000, kwargs...)
+                 trajectories, batch_size = trajectories, saveat = nothing, abstol = 1e-6,
+                 reltol = 1e-3, dt = 0.0, dtmin = 0.0, dtmax = 0.0, maxiters = 100_000,
+                 kwargs...)
     f = ens.prob.f.f
     if !haskey(MODEL_FOR, f)
         @warn "EnsembleMI355X: unregistered right-hand side, falling back to EnsembleThreads"
-        return __solve(ens, alg, EnsembleThreads(); trajectories, saveat, abstol,
+        return __solve(ens, alg, EnsembleThreads(); trajectories, batch_size, saveat, abstol,
                        reltol, kwargs...)
     end
     t_start = time()
     N = trajectories
-    outputs = Vector{Any}(undef, N)
-    pending = collect(1:N)
+    trajectories ÷ batch_size < 1 && error("trajectories ÷ batch_size cannot be less than 1")
+    # batches of batch_size trajectories, the reduction after each, stopping
+    # once it reports convergence
+    u = ens.u_init === nothing ? [] : ens.u_init
+    converged = false
+    for first in 1:batch_size:N
+        II = first:min(first + batch_size - 1, N)
+        batch = _solve_batch(ens, alg, ealg, f, II; saveat, abstol, reltol, dt, dtmin,
+                             dtmax, maxiters)
+        u, converged = ens.reduction(u, batch, II)
+        converged && break
+    end
+    # one batch with the default reduction: the ensemble is complete
+    length(1:batch_size:N) == 1 && ens.reduction === SciMLBase.DEFAULT_REDUCTION &&
+        (converged = true)
+    return EnsembleSolution(u, time() - t_start, converged)
+end
+
+# The trajectories II of one batch: prob_func, the device solve, output_func
+# with its rerun loop; returns their outputs in index order.
+function _solve_batch(ens, alg, ealg::EnsembleMI355X, f, II; saveat, abstol, reltol, dt,
+                      dtmin, dtmax, maxiters)
+    outputs = Dict{Int,Any}()
+    pending = collect(II)
     repeat = 1
     sa = _saveat_vector(saveat, ens.prob.tspan)
     while !isempty(pending)
@@ -275,8 +298,7 @@ function __solve(ens::SciMLBase.AbstractEnsembleProblem, alg, ealg::EnsembleMI35
         pending = rerun
         repeat += 1
     end
-    u, converged = ens.reduction(ens.u_init === nothing ? [] : ens.u_init, outputs, 1:N)
-    return EnsembleSolution(u, time() - t_start, converged)
+    return [outputs[i] for i in II]
 end
 
 end # module

@@ -659,6 +659,9 @@ def solve(prob, alg, ensemblealg=None, *, trajectories=None, saveat=None, abstol
         raise TypeError("trajectories is required")
     tstart = time.perf_counter()
     N = int(trajectories)
+    batch_size = int(kw.pop("batch_size", N))
+    if batch_size < 1 or N // batch_size < 1:
+        raise ValueError("trajectories // batch_size cannot be less than 1")
     idx = np.arange(N)
     sa = _saveat_vector(saveat, prob.prob.tspan)
     if isinstance(prob.output_func, DeviceOutputFunc):
@@ -672,29 +675,41 @@ def solve(prob, alg, ensemblealg=None, *, trajectories=None, saveat=None, abstol
         u = arr.reduced if red is not None else arr.out.T
         return EnsembleSolution(u=u, elapsedTime=time.perf_counter() - tstart,
                                 converged=True, arrays=arr)
-    outputs = [None] * N
-    repeat = 1
-    pending = idx
-    last_arrays = None
-    while len(pending):
-        f, u0, p, t0, tf, nvec, probs = _pack(prob, pending, repeat)
-        ealg = ensemblealg if repeat == 1 else _subset_ealg(ensemblealg, pending)
-        arr = device_solve(f, u0, p, t0, tf, nvec, alg, ealg, saveat=sa, abstol=abstol,
-                           reltol=reltol, dt=dt, dtmin=dtmin, dtmax=dtmax,
-                           maxiters=maxiters, **kw)
-        last_arrays = arr if last_arrays is None else last_arrays
-        if repeat == 1:
-            last_arrays = arr
-        rerun = []
-        for k, i in enumerate(pending):
-            sol = arr.solution(k, probs[k] if probs else None)
-            out, again = prob.output_func(sol, int(i) + 1)
-            outputs[i] = out
-            if again:
-                rerun.append(i)
-        pending = np.array(rerun, dtype=np.int64)
-        repeat += 1
+    # batches of batch_size trajectories, the reduction after each (1-based
+    # indices I), stopping once it reports convergence
     u = prob.u_init if prob.u_init is not None else []
-    u, converged = prob.reduction(u, outputs, range(1, N + 1))
+    converged = False
+    first_arrays = None
+    starts = range(0, N, batch_size)
+    for b0 in starts:
+        batch = idx[b0:b0 + batch_size]
+        outputs = {}
+        repeat = 1
+        pending = batch
+        while len(pending):
+            f, u0, p, t0, tf, nvec, probs = _pack(prob, pending, repeat)
+            ealg = ensemblealg if (repeat == 1 and len(batch) == N) else \
+                _subset_ealg(ensemblealg, pending)
+            arr = device_solve(f, u0, p, t0, tf, nvec, alg, ealg, saveat=sa, abstol=abstol,
+                               reltol=reltol, dt=dt, dtmin=dtmin, dtmax=dtmax,
+                               maxiters=maxiters, **kw)
+            if first_arrays is None:
+                first_arrays = arr
+            rerun = []
+            for k, i in enumerate(pending):
+                sol = arr.solution(k, probs[k] if probs else None)
+                out, again = prob.output_func(sol, int(i) + 1)
+                outputs[int(i)] = out
+                if again:
+                    rerun.append(i)
+            pending = np.array(rerun, dtype=np.int64)
+            repeat += 1
+        I = range(int(batch[0]) + 1, int(batch[-1]) + 2)
+        u, converged = prob.reduction(u, [outputs[int(i)] for i in batch], I)
+        converged = bool(converged)
+        if converged:
+            break
+    if len(starts) == 1 and prob.reduction is _default_reduction:
+        converged = True  # one batch with the default reduction: complete
     return EnsembleSolution(u=u, elapsedTime=time.perf_counter() - tstart,
-                            converged=bool(converged) or True, arrays=last_arrays)
+                            converged=converged, arrays=first_arrays)

@@ -97,3 +97,37 @@ def test_c_abi_example_program():
     r = subprocess.run([exe], capture_output=True, text=True, timeout=240)
     assert r.returncode == 0, (r.stdout, r.stderr)
     assert r.stdout.startswith("ok"), r.stdout
+
+
+def test_ensemble_batch_size_and_converged():
+    """solve(...; batch_size): trajectories in batches of batch_size, the
+    reduction after each with its 1-based indices, stopping at the first
+    batch it reports converged; the default reduction concatenates."""
+    say("ensemble batch_size / converged")
+    base = ODEProblem(mideq.lorenz, [1.0, 0.0, 0.0], (0.0, 1.0), [10.0, 28.0, 8 / 3])
+    called = []
+
+    def prob_func(prob, i, repeat):
+        called.append(i)
+        return remake(prob, u0=[1.0 + 0.01 * i, 0.0, 0.0])
+
+    seen_I = []
+
+    def reduction(u, batch, I):
+        seen_I.append(list(I))
+        u = u + [float(b.u[-1][0]) for b in batch]
+        return u, len(u) >= 32
+
+    ens = EnsembleProblem(base, prob_func=prob_func, reduction=reduction, u_init=[])
+    es = solve(ens, Tsit5(), EnsembleMI355X(), trajectories=40, batch_size=16, abstol=1e-9,
+               reltol=1e-9)
+    assert seen_I == [list(range(1, 17)), list(range(17, 33))]
+    assert sorted(called) == list(range(1, 33)) and es.converged is True and len(es.u) == 32
+    one = solve(EnsembleProblem(base, prob_func=prob_func), Tsit5(), EnsembleMI355X(),
+                trajectories=40, abstol=1e-9, reltol=1e-9)
+    batched = solve(EnsembleProblem(base, prob_func=prob_func), Tsit5(), EnsembleMI355X(),
+                    trajectories=40, batch_size=16, abstol=1e-9, reltol=1e-9)
+    assert one.converged is True and batched.converged is False
+    assert [s.u[-1][0] for s in one.u] == [s.u[-1][0] for s in batched.u]
+    with pytest.raises(ValueError):
+        solve(ens, Tsit5(), EnsembleMI355X(), trajectories=4, batch_size=8)
