@@ -311,6 +311,7 @@ struct Lane {
   double k1[N];   // FSAL stage (FP64 form)
   float k1f[N];   // FSAL stage (FP32 form)
   double t, tcomp, tf, dt, dtmax;
+  bool back;  // tf < t0: integrated backward with steps of -dt (dt > 0 is the magnitude)
   double tsnap;  // 100 eps(tf): a step ending this close to tf ends at tf
   double qold, qoldpow, qsteady_max, eig, dtfirst, lqold;
   double eigd;  // fast form: the stiffness estimate is eig / eigd
@@ -332,7 +333,12 @@ struct Lane {
     return ((const __attribute__((address_space(1))) double*)a.saveat)[k];
   }
   __device__ __forceinline__ void set_tnext(const LaneArgs& a) {
-    tnext = isave < a.ns ? save_time(a, isave) : __builtin_inf();
+    // save points in ascending order, visited from the last one down when
+    // integrating backward; +-inf once every point is saved
+    if (back)
+      tnext = isave >= 0 ? save_time(a, isave) : -__builtin_inf();
+    else
+      tnext = isave < a.ns ? save_time(a, isave) : __builtin_inf();
   }
 
   // ------------------------------------------------------------ helpers
@@ -390,9 +396,10 @@ struct Lane {
     dt0 = fmin(dt0, dtmax);
     if (dt0 < 10.0 * 2.220446049250313e-16) return smalldt;
     double u1[N], f1[N];
+    double sdt0 = (back ? -1.0 : 1.0) * dt0;  // the trial step in the direction of integration
 #pragma unroll
-    for (int c = 0; c < N; c++) u1[c] = fma(dt0, k1[c], u[c]);
-    rhs(f1, u1, p, t0 + dt0);
+    for (int c = 0; c < N; c++) u1[c] = fma(sdt0, k1[c], u[c]);
+    rhs(f1, u1, p, t0 + sdt0);
     nf += 1;
     bool same = true;
 #pragma unroll
@@ -423,6 +430,7 @@ struct Lane {
     for (int c = 0; c < N; c++) u[c] = a.u0[(int64_t)c * NT + i];
     double t0 = a.t0 ? a.t0[i] : a.tspan0;
     tf = a.tf ? a.tf[i] : a.tspan1;
+    back = tf < t0;
     t = t0;
     tcomp = 0.0;
     dtmax = a.dtmax > 0 ? a.dtmax : fabs(tf - t0);
@@ -457,10 +465,21 @@ struct Lane {
       double nanv[N];
 #pragma unroll
       for (int c = 0; c < N; c++) nanv[c] = __builtin_nan("");
-      while (isave < a.ns && save_time(a, isave) < t0) store_save(a, isave++, nanv);
-      while (isave < a.ns && save_time(a, isave) == t0) store_save(a, isave++, u);
+      if (!back) {
+        while (isave < a.ns && save_time(a, isave) < t0) store_save(a, isave++, nanv);
+        while (isave < a.ns && save_time(a, isave) == t0) store_save(a, isave++, u);
+      } else {
+        isave = a.ns - 1;
+        while (isave >= 0 && save_time(a, isave) > t0) store_save(a, isave--, nanv);
+        while (isave >= 0 && save_time(a, isave) == t0) store_save(a, isave--, u);
+      }
     } else {
-      while (isave < a.ns && save_time(a, isave) <= t0) isave++;
+      if (!back) {
+        while (isave < a.ns && save_time(a, isave) <= t0) isave++;
+      } else {
+        isave = a.ns - 1;
+        while (isave >= 0 && save_time(a, isave) >= t0) isave--;
+      }
     }
     set_tnext(a);
     fsal_init();
@@ -479,10 +498,10 @@ struct Lane {
     qold = kQoldInit;
     qoldpow = ctl_pow(kQoldInit, kBeta2);
     lqold = kL2QoldInit;
-    // a zero-length span is done; a reversed (or NaN) span is not integrated
+    // a zero-length span is done; a NaN end is not integrated
     if (tf == t0)
       rc = RC_SUCCESS;
-    else if (!(tf > t0))
+    else if (!(tf > t0) && !(tf < t0))
       rc = RC_INITIALFAILURE;
   }
 
@@ -869,7 +888,8 @@ struct Lane {
       rc = RC_DTLESSTHANMIN;
       return;
     }
-    dt = fmin(dt, tf - t);
+    dt = fmin(dt, back ? t - tf : tf - t);
+    const double sdt = back ? -dt : dt;  // the signed step
 
     double u1[N];
     double EEst = 0.0;
@@ -880,14 +900,14 @@ struct Lane {
     if (tsit_step) {
       if constexpr (kHasTsit5) {
         if constexpr (!MIX)
-          EEst = tsit5_step64(dt, k64, u1, a.abstol, a.reltol);
+          EEst = tsit5_step64(sdt, k64, u1, a.abstol, a.reltol);
         else
-          EEst = tsit5_step_mix(dt, k32, u1, a.abstol, a.reltol);
+          EEst = tsit5_step_mix(sdt, k32, u1, a.abstol, a.reltol);
       }
       nf += 6;
     } else {
       if constexpr (kHasRodas) {
-        if (!rodas5_step(dt, u1, EEst, a.abstol, a.reltol, kd)) EEst = 2.0;
+        if (!rodas5_step(sdt, u1, EEst, a.abstol, a.reltol, kd)) EEst = 2.0;
       }
       nf += 8;
     }
@@ -943,9 +963,9 @@ struct Lane {
       double tprev = t;
       double ttmp;
       if (!MIX) {
-        ttmp = t + dt;
+        ttmp = t + sdt;
       } else {
-        double y = dt - tcomp;
+        double y = sdt - tcomp;
         ttmp = t + y;
         tcomp = (ttmp - t) - y;
       }
@@ -963,7 +983,7 @@ struct Lane {
               double acc = bt[6] * k64[6][c];
 #pragma unroll
               for (int j = 5; j >= 0; j--) acc = fma(bt[j], k64[j][c], acc);
-              val[c] = fma(dt, acc, u[c]);
+              val[c] = fma(sdt, acc, u[c]);
             }
           } else if constexpr (kHasTsit5 && MIX) {
             float bf[7];
@@ -974,7 +994,7 @@ struct Lane {
               float acc = bf[6] * k32[6][c];
 #pragma unroll
               for (int j = 5; j >= 0; j--) acc = fmaf(bf[j], k32[j][c], acc);
-              val[c] = fma(dt, (double)acc, u[c]);
+              val[c] = fma(sdt, (double)acc, u[c]);
             }
           }
         } else if constexpr (kHasRodas && !kDense) {
@@ -1014,7 +1034,7 @@ struct Lane {
         auto G = [&](int k, double th) {
           double tmp[N];
           interp(th, tmp);
-          return M::cb_g(k, tmp, p, tprev + th * dt);
+          return M::cb_g(k, tmp, p, tprev + th * sdt);
         };
         double th_ev = 2.0;
         for (int k = 0; k < kNcb; k++) {
@@ -1072,14 +1092,14 @@ struct Lane {
           }
         }
         if (ev >= 0 && th_ev < 1.0) {
-          t = tprev + th_ev * dt;
+          t = tprev + th_ev * sdt;
           tcomp = 0.0;
           interp(th_ev, uend);
         }
       }
       // savevalues!
       {
-        while (tnext <= t) {
+        while (back ? tnext >= t : tnext <= t) {
           if (a.us) {
             double val[N];
             double sa = tnext;
@@ -1087,11 +1107,11 @@ struct Lane {
 #pragma unroll
               for (int c = 0; c < N; c++) val[c] = uend[c];
             } else {
-              interp((sa - tprev) / dt, val);
+              interp((sa - tprev) / sdt, val);
             }
             store_save(a, isave, val);
           }
-          isave++;
+          isave += back ? -1 : 1;
           set_tnext(a);
         }
       }
@@ -1152,7 +1172,7 @@ struct Lane {
       else
         dt = dt * fast_exp2(-fmin(kL2QminInv, l11 - kL2Gamma));
     }
-    if (!(t < tf)) rc = RC_SUCCESS;
+    if (back ? !(t > tf) : !(t < tf)) rc = RC_SUCCESS;
   }
 
   // ------------------------------------------------------------ finish
@@ -1166,7 +1186,10 @@ struct Lane {
       double nanv[N];
 #pragma unroll
       for (int c = 0; c < N; c++) nanv[c] = __builtin_nan("");
-      for (int k = isave; k < a.ns; k++) store_save(a, k, nanv);
+      if (!back)
+        for (int k = isave; k < a.ns; k++) store_save(a, k, nanv);
+      else
+        for (int k = isave; k >= 0; k--) store_save(a, k, nanv);
     }
     if (a.u_final)
 #pragma unroll

@@ -670,7 +670,7 @@ static double jl_eps(double x) {
 /* ode_determine_initdt (Hairer-Norsett-Wanner II.4 with upstream edits).    */
 /* ------------------------------------------------------------------------ */
 static double initdt_impl(const mdl_t *m, const double *u0, const double *f0,
-                          double t0, double dtmax, const oracle_options *o,
+                          double t0, double tdir, double dtmax, const oracle_options *o,
                           int order, int *nf) {
   int n = m->n;
   double dtmin = nextafter(fmax(o->dtmin, jl_eps(t0)), INFINITY);
@@ -686,8 +686,9 @@ static double initdt_impl(const mdl_t *m, const double *u0, const double *f0,
   dt0 = fmin(dt0, dtmax);
   if (dt0 < 10.0 * DBL_EPSILON) return smalldt;
   double u1[NMAX], f1[NMAX];
-  for (int i = 0; i < n; i++) u1[i] = fma(dt0, f0[i], u0[i]);
-  rhs64(m, u1, t0 + dt0, f1);
+  double sdt0 = tdir * dt0; /* the trial step in the direction of integration */
+  for (int i = 0; i < n; i++) u1[i] = fma(sdt0, f0[i], u0[i]);
+  rhs64(m, u1, t0 + sdt0, f1);
   *nf += 1;
   int same = 1;
   for (int i = 0; i < n; i++) same &= (f0[i] == f1[i]);
@@ -713,7 +714,7 @@ double oracle_initdt(int32_t model, int32_t n, const double *u0, const double *p
   rhs64(&m, u0, t0, f0);
   int nf = 0;
   double dtmax = opt->dtmax > 0 ? opt->dtmax : fabs(tf - t0);
-  return initdt_impl(&m, u0, f0, t0, dtmax, opt, order, &nf);
+  return initdt_impl(&m, u0, f0, t0, tf < t0 ? -1.0 : 1.0, dtmax, opt, order, &nf);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1186,19 +1187,26 @@ static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
   double gprev[8];
   for (int c = 0; c < n; c++) uprev[c] = u0[c];
   double t = t0, tcomp = 0.0;
-  int isave = 0;
+  /* direction of integration: tf < t0 integrates backward with steps of
+   * -dt (dt is the step's magnitude everywhere else); the save points, in
+   * ascending order, are then visited from the last one down */
+  const double tdir = tf < t0 ? -1.0 : 1.0;
+  const int back = tf < t0;
   const double *sa = o->saveat;
   int ns = sa ? o->ns : 0;
+  int isave = back ? ns - 1 : 0;
+#define SAVE_LEFT (back ? isave >= 0 : isave < ns)
+#define SAVE_NEXT (back ? isave-- : isave++)
   /* save points before t0 are not reachable; the point at t0 is u0 itself */
-  while (isave < ns && sa[isave] < t0) {
+  while (SAVE_LEFT && (back ? sa[isave] > t0 : sa[isave] < t0)) {
     if (save)
       for (int c = 0; c < n; c++) save[((long)isave * n + c) * save_stride] = NAN;
-    isave++;
+    SAVE_NEXT;
   }
-  while (isave < ns && sa[isave] == t0) {
+  while (SAVE_LEFT && sa[isave] == t0) {
     if (save)
       for (int c = 0; c < n; c++) save[((long)isave * n + c) * save_stride] = u0[c];
-    isave++;
+    SAVE_NEXT;
   }
 
   /* FSAL initialisation: k1 = f(u0) */
@@ -1215,7 +1223,7 @@ static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
   for (int k = 0; k < ncb; k++) gprev[k] = g_user.cond(k, uprev, m->p, t0);
   double dt = o->dt > 0 ? o->dt
                : o->dt0_in ? o->dt0_in[idx]
-                           : initdt_impl(m, uprev, f0, t0, dtmax, o, 5, &res->nf);
+                           : initdt_impl(m, uprev, f0, t0, tdir, dtmax, o, 5, &res->nf);
   res->dt0 = dt;
   double qold = CTL_QOLDINIT;
   double q11 = 1.0;
@@ -1227,12 +1235,12 @@ static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
   double lqold = L2_QOLDINIT;
   double lqs_cur = 0.0; /* log2 of qsteady_max: 0 or log2(1.2) */
 
-  /* a zero-length span is done; a reversed (or NaN) span is not integrated */
+  /* a zero-length span is done; a NaN end is not integrated */
   if (tf == t0)
     rc = ORACLE_RC_SUCCESS;
-  else if (!(tf > t0))
+  else if (!(tf > t0) && !(tf < t0))
     rc = ORACLE_RC_INITIALFAILURE;
-  while (rc == ORACLE_RC_DEFAULT && t < tf) {
+  while (rc == ORACLE_RC_DEFAULT && (back ? t > tf : t < tf)) {
     /* loopheader!: choose algorithm, fix dt at bounds, clamp to tstop */
     if (autosw && iter > 0) {
       int stiff;
@@ -1275,21 +1283,22 @@ static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
       rc = ORACLE_RC_DTLESSTHANMIN;
       break;
     }
-    dt = fmin(dt, tf - t);
+    dt = fmin(dt, back ? t - tf : tf - t);
+    const double sdt = back ? -dt : dt; /* the signed step */
 
     /* perform_step! */
     double EEst;
     if (mode == 0) {
       if (!mixed)
-        EEst = tsit5_step64(m, uprev, t, dt, abstol, reltol, k64, u1, &eig, &eigd,
+        EEst = tsit5_step64(m, uprev, t, sdt, abstol, reltol, k64, u1, &eig, &eigd,
                             o->norm_lanes);
       else
-        EEst = tsit5_step_mix(m, uprev, t, dt, abstol, reltol, k32, u1, &eig, &eigd,
+        EEst = tsit5_step_mix(m, uprev, t, sdt, abstol, reltol, k32, u1, &eig, &eigd,
                               o->norm_lanes);
       res->nf += 6;
     } else {
       double nJ;
-      if (rodas5_step(m, rtab, mixed, uprev, t, dt, abstol, reltol, u1, &EEst, ws, &nJ,
+      if (rodas5_step(m, rtab, mixed, uprev, t, sdt, abstol, reltol, u1, &EEst, ws, &nJ,
                       o->norm_lanes, kdsave) != 0)
         EEst = 2.0;
       res->nf += 8;
@@ -1345,10 +1354,10 @@ static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
       double tprev = t;
       double ttmp;
       if (!mixed) {
-        ttmp = t + dt;
+        ttmp = t + sdt;
       } else {
         /* compensated (Kahan) time accumulation */
-        double y = dt - tcomp;
+        double y = sdt - tcomp;
         ttmp = t + y;
         tcomp = (ttmp - t) - y;
       }
@@ -1375,8 +1384,8 @@ static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
             } else {
               double tmp[NMAX];
               thb = (double)q / (double)ip;
-              step_interp(n, mode, mixed, rtab, uprev, u1, k64, k32, kdsave, dt, thb, tmp);
-              gb = g_user.cond(k, tmp, m->p, tprev + thb * dt);
+              step_interp(n, mode, mixed, rtab, uprev, u1, k64, k32, kdsave, sdt, thb, tmp);
+              gb = g_user.cond(k, tmp, m->p, tprev + thb * sdt);
             }
             int up = ga < 0.0 && gb >= 0.0;
             int down = ga > 0.0 && gb <= 0.0;
@@ -1390,8 +1399,8 @@ static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
                   double cc = (lo * fhi - hi * flo) / (fhi - flo);
                   if (!(cc > lo && cc < hi)) cc = 0.5 * (lo + hi);
                   double tmp[NMAX];
-                  step_interp(n, mode, mixed, rtab, uprev, u1, k64, k32, kdsave, dt, cc, tmp);
-                  double fc = g_user.cond(k, tmp, m->p, tprev + cc * dt);
+                  step_interp(n, mode, mixed, rtab, uprev, u1, k64, k32, kdsave, sdt, cc, tmp);
+                  double fc = g_user.cond(k, tmp, m->p, tprev + cc * sdt);
                   if (fc == 0.0) {
                     hi = cc;
                     break;
@@ -1421,24 +1430,24 @@ static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
           }
         }
         if (ev >= 0 && th_ev < 1.0) {
-          t = tprev + th_ev * dt;
+          t = tprev + th_ev * sdt;
           tcomp = 0.0;
-          step_interp(n, mode, mixed, rtab, uprev, u1, k64, k32, kdsave, dt, th_ev, uend);
+          step_interp(n, mode, mixed, rtab, uprev, u1, k64, k32, kdsave, sdt, th_ev, uend);
         }
       }
       /* savevalues!: every saveat point <= t */
-      while (isave < ns && sa[isave] <= t) {
+      while (SAVE_LEFT && (back ? sa[isave] >= t : sa[isave] <= t)) {
         if (save) {
           double val[NMAX];
           if (sa[isave] == t) {
             for (int c = 0; c < n; c++) val[c] = uend[c];
           } else {
-            double th = (sa[isave] - tprev) / dt;
-            step_interp(n, mode, mixed, rtab, uprev, u1, k64, k32, kdsave, dt, th, val);
+            double th = (sa[isave] - tprev) / sdt;
+            step_interp(n, mode, mixed, rtab, uprev, u1, k64, k32, kdsave, sdt, th, val);
           }
           for (int c = 0; c < n; c++) save[((long)isave * n + c) * save_stride] = val[c];
         }
-        isave++;
+        SAVE_NEXT;
       }
       for (int c = 0; c < n; c++) uprev[c] = uend[c];
       if (mode == 0) {
@@ -1500,9 +1509,11 @@ static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
     }
   }
   if (rc == ORACLE_RC_DEFAULT) rc = ORACLE_RC_SUCCESS;
-  for (; isave < ns; isave++)
+  for (; SAVE_LEFT; SAVE_NEXT)
     if (save)
       for (int c = 0; c < n; c++) save[((long)isave * n + c) * save_stride] = NAN;
+#undef SAVE_LEFT
+#undef SAVE_NEXT
   for (int c = 0; c < n; c++) res->u[c] = uprev[c];
   res->t = t;
   res->retcode = rc;

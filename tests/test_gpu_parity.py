@@ -126,14 +126,51 @@ def test_retcodes_forced_from_host(oracle):
     c = cpu(oracle, "lorenz", w["u0"], p, 0.0, 1.0)
     assert g.retcode[3] in (4, 5) and (np.delete(g.retcode, 3) == 1).all()
     np.testing.assert_array_equal(g.retcode, c["retcode"])
-    # per-trajectory spans: zero length -> Success at u0; reversed -> InitialFailure
+    # per-trajectory spans: zero length -> Success at u0; reversed -> integrated
+    # backward to tf; a NaN end -> InitialFailure at u0
     tf = np.full(N, 1.0)
-    tf[5], tf[9] = 0.0, -1.0
+    tf[5], tf[9], tf[11] = 0.0, -1.0, np.nan
     g = gpu("lorenz", w["u0"], w["p"], 0.0, tf)
     c = cpu(oracle, "lorenz", w["u0"], w["p"], 0.0, tf)
-    assert g.retcode[5] == 1 and g.retcode[9] == 6
-    np.testing.assert_array_equal(g.u_final[:, [5, 9]], w["u0"][:, [5, 9]])
+    assert g.retcode[5] == 1 and g.retcode[9] == 1 and g.retcode[11] == 6
+    assert g.t_final[9] == -1.0 and g.naccept[9] > 0
+    np.testing.assert_array_equal(g.u_final[:, [5, 11]], w["u0"][:, [5, 11]])
     assert_identical(g, c)
+
+
+@pytest.mark.parametrize("alg,precision,controller,schedule", [
+    ("tsit5", "fp64", "reference", "lane"), ("tsit5", "mixed", "fast", "persistent"),
+    ("rodas5", "fp64", "fast", "lane"), ("auto", "fp64", "reference", "lane"),
+    ("rodas5p", "mixed", "reference", "lane")])
+def test_backward_and_forward_spans_with_saveat(oracle, alg, precision, controller, schedule):
+    """Trajectories integrated forward and backward in one ensemble against
+    one ascending saveat: backward ones fill the points in (tf, t0] from the
+    last one down; bit-exact against the oracle."""
+    N = 1000
+    say(f"backward spans {alg} {precision} {controller} {schedule} N={N}")
+    rng = np.random.default_rng(45)
+    # decay forward, growth backward: both directions stay well posed (the
+    # Lorenz flow run backward from the attractor's neighbourhood is not)
+    model, w = "exp", dict(u0=np.ones((1, N)), p=-rng.uniform(0.5, 2.0, (1, N)))
+    back = rng.random(N) < 0.5
+    t0 = np.where(back, 1.0, 0.0)
+    tf = np.where(back, 0.0, 1.0) + rng.uniform(-0.3, 0.3, N)
+    sa = np.linspace(-0.5, 1.5, 11)
+    kw = dict(alg=alg, precision=precision, controller=controller, schedule=schedule, saveat=sa,
+              abstol=1e-8, reltol=1e-6)
+    g = gpu(model, w["u0"], w["p"], t0, tf, **kw)
+    c = cpu(oracle, model, w["u0"], w["p"], t0, tf, alg=alg, precision=precision,
+            controller=1 if controller == "fast" else 0, saveat=sa, abstol=1e-8, reltol=1e-6)
+    assert (c["retcode"] == 1).all()
+    assert_identical(g, c, f32=False)
+    np.testing.assert_array_equal(g.t_final, tf)
+    # saved points strictly outside [min(t0, tf), max(t0, tf)] stay NaN
+    lo, hi = np.minimum(t0, tf), np.maximum(t0, tf)
+    outside = (sa[:, None] < lo[None, :]) | (sa[:, None] > hi[None, :])
+    assert np.isnan(g.us[:, 0, :][outside]).all() and not np.isnan(g.us[:, 0, :][~outside]).any()
+    inside = g.us[:, 0, :][~outside]
+    exact = np.exp(w["p"][0][None, :] * (sa[:, None] - t0[None, :]))[~outside]
+    np.testing.assert_allclose(inside, exact, rtol=1e-4)
 
 
 def test_per_trajectory_tspans_and_fixed_dt(oracle):

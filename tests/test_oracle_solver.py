@@ -404,9 +404,35 @@ def test_retcodes(oracle):
     assert r["retcode"] in (4, 5)
     r = oracle.solve("exp", [1.0], [1.0], (0.0, 0.0))
     assert r["retcode"] == 1 and r["naccept"] == 0
-    # a reversed span is not integrated (InitialFailure), never reported as Success
-    r = oracle.solve("exp", [1.0], [1.0], (1.0, 0.0))
+    # a NaN end is not integrated (InitialFailure), never reported as Success
+    r = oracle.solve("exp", [1.0], [1.0], (1.0, float("nan")))
     assert r["retcode"] == 6 and r["naccept"] == 0 and r["u"][0] == 1.0
+
+
+@pytest.mark.parametrize("alg", ["tsit5", "rodas5", "rodas5p", "auto"])
+def test_backward_integration_closed_form(oracle, alg):
+    """tspan (1, 0): integrated backward with negative steps, as upstream
+    does for tf < t0.  u' = 1.01 u from u(1) = e^1.01 gives u(0) = 1
+    (test/core.jl:10-14 run in reverse); save points in ascending order are
+    filled from the last one down, points beyond t0 stay NaN."""
+    sa = np.array([-0.5, 0.0, 0.25, 0.5, 1.0, 1.5])
+    r = oracle.solve("exp", [np.exp(1.01)], [1.01], (1.0, 0.0), alg=alg, abstol=1e-12,
+                     reltol=1e-12, saveat=sa)
+    assert r["retcode"] == 1 and r["t"] == 0.0 and r["naccept"] > 0
+    assert abs(r["u"][0] - 1.0) < 1e-9
+    s = r["saveat"][:, 0]
+    assert np.isnan(s[0]) and np.isnan(s[5])  # before tf / beyond t0
+    np.testing.assert_allclose(s[1:4], np.exp(1.01 * sa[1:4]), rtol=1e-9)
+    assert s[4] == np.exp(1.01)  # the point at t0 is u0 itself
+
+
+def test_backward_then_forward_lorenz(oracle):
+    """Lorenz over (0, 1) forward, then back from the end state over (1, 0):
+    the start comes back (to 1e-6: the reversed flow amplifies the error)."""
+    f = oracle.solve("lorenz", [1, 0, 0], LORENZ_P, (0.0, 1.0), abstol=1e-12, reltol=1e-12)
+    b = oracle.solve("lorenz", f["u"], LORENZ_P, (1.0, 0.0), abstol=1e-12, reltol=1e-12)
+    assert b["retcode"] == 1 and b["t"] == 0.0
+    np.testing.assert_allclose(b["u"], [1.0, 0.0, 0.0], atol=1e-6)
 
 
 # ----------------------------------------------------------- ensemble
