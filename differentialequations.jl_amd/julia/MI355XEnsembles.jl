@@ -197,9 +197,11 @@ end
 function _saveat_vector(saveat, tspan)
     saveat === nothing && return Float64[]
     if saveat isa Number
-        pts = collect(tspan[1]:saveat:tspan[2])
-        last(pts) < tspan[2] && push!(pts, tspan[2])
-        return Float64.(pts)
+        # in the direction of integration, handed over ascending
+        dir = tspan[2] >= tspan[1] ? 1 : -1
+        pts = collect(tspan[1]:(dir * saveat):tspan[2])
+        (last(pts) - tspan[2]) * dir < 0 && push!(pts, tspan[2])
+        return sort(Float64.(pts))
     end
     return sort(Float64.(collect(saveat)))
 end

@@ -503,13 +503,16 @@ def _saveat_vector(saveat, tspan):
     if saveat is None:
         return None
     if np.isscalar(saveat):
+        # t0, t0 + dir d, ... and tf, in the direction of integration; handed
+        # over in ascending order (a backward solve fills them from the end)
         t0, t1 = tspan
+        sgn = 1.0 if t1 >= t0 else -1.0
         d = float(saveat)
-        n = int(np.floor((t1 - t0) / d + 1e-9)) + 1
-        pts = t0 + d * np.arange(n)
-        if pts[-1] < t1:
+        n = int(np.floor(abs(t1 - t0) / d + 1e-9)) + 1
+        pts = t0 + sgn * d * np.arange(n)
+        if (pts[-1] - t1) * sgn < 0:
             pts = np.append(pts, t1)
-        return np.ascontiguousarray(pts, dtype=np.float64)
+        return np.ascontiguousarray(np.sort(pts), dtype=np.float64)
     return np.ascontiguousarray(np.sort(np.asarray(saveat, dtype=np.float64)))
 
 

@@ -2,6 +2,7 @@
 argument validation works without a GPU (no compute calls here)."""
 import ctypes
 
+import numpy as np
 import pytest
 
 import mideq
@@ -71,6 +72,12 @@ def test_saveat_scalar_expansion_matches_core_jl():
     from mideq.scimlbase import _saveat_vector
     v = _saveat_vector(0.1, (0.0, 1.0))
     assert len(v) == 11 and v[0] == 0.0 and v[-1] == 1.0
+    # backward: t0, t0 - 0.3, ... and tf, handed over ascending
+    b = _saveat_vector(0.3, (1.0, 0.0))
+    np.testing.assert_allclose(b, [0.0, 0.1, 0.4, 0.7, 1.0], atol=1e-15)
+    assert b[0] == 0.0 and b[-1] == 1.0
+    f = _saveat_vector(0.3, (0.0, 1.0))
+    np.testing.assert_allclose(f, [0.0, 0.3, 0.6, 0.9, 1.0], atol=1e-15)
 
 
 def test_user_model_limits_match_header():

@@ -124,3 +124,25 @@ def test_rosenbrock_event_on_robertson(oracle):
     assert (c["retcode"] == 1).all() and (c["nevent"] == 1).all()
     _check(g, c)
     np.testing.assert_allclose(g.u_final.sum(axis=0), 1.0, atol=1e-10)  # sum y kept
+
+
+def test_terminate_integrating_backward(oracle):
+    """A backward solve (tspan (2, 0)) of u' = -k u from u(2) = 1 grows as
+    e^{k (2 - t)}; terminate! where u reaches 2, at t = 2 - ln 2 / k:
+    event location on the dense output of negative steps, bit-exact vs the
+    oracle."""
+    N = 512
+    say(f"terminate, backward span N={N}")
+    cb = ContinuousCallback("u[0] - 2.0", terminate=True)
+    model = UserModel(1, 1, "du[0] = -p[0] * u[0];", callbacks=(cb,))
+    rng = np.random.default_rng(115)
+    k = rng.uniform(0.5, 3.0, N)
+    sa = np.linspace(0.0, 2.0, 9)
+    kw = dict(abstol=1e-12, reltol=1e-12, saveat=sa)
+    g = run(model, np.ones((1, N)), k[None, :], 2.0, 0.0, **kw)
+    assert (g.retcode == 7).all()
+    np.testing.assert_allclose(g.t_final, 2.0 - np.log(2.0) / k, rtol=1e-9)
+    umh.install(oracle, 1, 1, "du[0] = -p[0] * u[0];", callbacks=(cb,))
+    _check(g, cpu(oracle, "user", np.ones((1, N)), k[None, :], 2.0, 0.0, **kw))
+    early = sa[None, :] < g.t_final[:, None]  # not reached going backward
+    assert np.isnan(g.us[:, 0, :].T[early]).all()
