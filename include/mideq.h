@@ -101,7 +101,9 @@ typedef struct {
   const double *u0;     /* [n][ntraj] */
   const double *p;      /* [np][ntraj] */
   const double *t0;     /* [ntraj] or NULL: tspan0 for all */
-  const double *tf;     /* [ntraj] or NULL: tspan1 for all */
+  const double *tf;     /* [ntraj] or NULL: tspan1 for all.  tf < t0 integrates backward
+                           (one-lane models; the Brusselator gives InitialFailure); tf == t0
+                           is Success at u0; a NaN end is InitialFailure */
   double tspan0, tspan1;
   const int32_t *nvec;  /* [ntraj] ragged state sizes or NULL */
 } mideq_problem;
@@ -119,7 +121,7 @@ typedef struct {
                            saveat, n * ntraj must be < 2^32 (MIDEQ_E_UNSUPPORTED) */
   int32_t ns;
   int32_t save_dtype;   /* 0: double, 1: float for `us` */
-  const double *dt0;    /* [ntraj] per-trajectory initial dt (learned) or NULL */
+  const double *dt0;    /* [ntraj] per-trajectory initial |dt| (learned) or NULL */
   const int8_t *alg_hint; /* [ntraj] initial method for ALG_AUTO (0 Tsit5, 1 Rodas5) or NULL */
   int32_t qsteady_default; /* 1: method-default qsteady band; 0: [1,1] */
   int32_t schedule;     /* mideq_schedule */
