@@ -193,3 +193,29 @@ def test_user_time_dependent_forced_oscillator(oracle, alg):
     assert_identical(g, c)
     x = sa[:, None] / w[None, :] ** 2 - np.sin(sa[:, None] * w[None, :]) / w[None, :] ** 3
     np.testing.assert_allclose(g.us[:, 0, :], x, rtol=1e-6, atol=1e-8)
+
+
+CHAIN8 = ("for (int i = 0; i < 8; i++) {\n"
+          "  T left = i > 0 ? u[i - 1] : T(1);\n"
+          "  du[i] = p[0] * (left - u[i]) - p[1] * u[i] * u[i] + p[2] * u[(i + 3) % 8];\n"
+          "}")
+
+
+@pytest.mark.parametrize("alg,precision", [("tsit5", "fp64"), ("rodas5", "fp64"),
+                                           ("auto", "mixed")])
+def test_user_eight_states_at_the_limit(oracle, alg, precision):
+    """n = 8, the largest one-lane model (8 x 8 register LU, dual numbers with
+    eight directions), with saveat: bit-exact against the oracle."""
+    N = 700
+    say(f"user 8-state chain {alg} {precision} N={N}")
+    umh.install(oracle, 8, 3, CHAIN8)
+    model = UserModel(8, 3, CHAIN8, name="chain8")
+    rng = np.random.default_rng(108)
+    u0 = rng.uniform(0.0, 1.0, (8, N))
+    p = np.vstack([10.0 ** rng.uniform(-1, 2, N), rng.uniform(0.1, 2.0, N),
+                   rng.uniform(0.0, 0.5, N)])
+    kw = dict(saveat=np.linspace(0.0, 3.0, 7), abstol=1e-8, reltol=1e-6)
+    g = run(model, u0, p, 0.0, 3.0, alg=alg, precision=precision, **kw)
+    c = cpu(oracle, "user", u0, p, 0.0, 3.0, alg=alg, precision=precision, **kw)
+    assert (c["retcode"] == 1).all()
+    assert_identical(g, c)
