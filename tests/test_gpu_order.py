@@ -151,3 +151,32 @@ def test_saveat_size_limit_rejected_before_any_access():
     out.us = us.ctypes.data
     with pytest.raises(_abi.MideqError, match="2\\^32"):
         ctx.solve(P, O, out)
+
+
+def test_empty_ensemble_both_entries():
+    """ntraj = 0 is a successful no-op through both entry points (no
+    launch, no output written), with and without saveat."""
+    import torch
+    say("empty ensemble, host and device entry")
+    ctx = _abi.context(0)
+    for ns in (0, 3):
+        sa = np.array([0.0, 0.5, 1.0])
+        P = _abi.Problem()
+        P.model = _abi.MODEL["lorenz"]
+        P.n = 3
+        P.np = 3
+        P.ntraj = 0
+        P.tspan0 = 0.0
+        P.tspan1 = 1.0
+        O = _abi.Options()
+        O.alg = _abi.ALG["tsit5"]
+        O.abstol = O.reltol = 1e-6
+        if ns:
+            O.saveat = sa.ctypes.data
+            O.ns = ns
+        out = _abi.Outputs()
+        ctx.solve(P, O, out)
+        if ns:
+            O.saveat = torch.from_numpy(sa).cuda().data_ptr()
+        ctx.solve_device(P, O, out, None)
+        torch.cuda.synchronize()
