@@ -440,7 +440,10 @@ class EnsembleArrays:
         if self.us is not None:
             vals = self.us[:, :, i]
             ok = ~np.isnan(vals).any(axis=1)
-            return ODESolution(self.saveat[ok].copy(), [v.copy() for v in vals[ok]], rc, st, prob)
+            ts, us = self.saveat[ok], vals[ok]
+            if self.t_final is not None and self.t_final[i] < self.t0[i]:
+                ts, us = ts[::-1], us[::-1]  # a backward solve: in the order of integration
+            return ODESolution(ts.copy(), [v.copy() for v in us], rc, st, prob)
         return ODESolution(np.array([self.t0[i], self.t_final[i]]),
                            [self.u0[:, i].copy(), self.u_final[:, i].copy()], rc, st, prob)
 

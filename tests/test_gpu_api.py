@@ -131,3 +131,22 @@ def test_ensemble_batch_size_and_converged():
     assert [s.u[-1][0] for s in one.u] == [s.u[-1][0] for s in batched.u]
     with pytest.raises(ValueError):
         solve(ens, Tsit5(), EnsembleMI355X(), trajectories=4, batch_size=8)
+
+
+def test_single_problem_non_autonomous_backward(oracle):
+    """The INTEGRATION.md example: one ODEProblem of a run-time compiled,
+    non-autonomous model over tspan (10, 0) with saveat = 1.0: sol.t runs
+    10, 9, ..., 0 (the order of integration), values bit-exact against the
+    oracle."""
+    import user_model_host as umh
+    say("single problem, non-autonomous, backward, scalar saveat")
+    rhs = "du[0] = u[1];\ndu[1] = -p[0] * u[0] - p[1] * u[1] + t * t;"
+    forced = mideq.UserModel(2, 2, rhs)
+    prob = ODEProblem(forced, [0.0, 0.0], (10.0, 0.0), [4.0, 0.1])
+    sol = solve(prob, Tsit5(), saveat=1.0, abstol=1e-9, reltol=1e-9)
+    assert sol.retcode == ReturnCode.Success
+    np.testing.assert_array_equal(sol.t, np.arange(10.0, -1.0, -1.0))
+    umh.install(oracle, 2, 2, rhs)
+    c = oracle.solve("user", [0.0, 0.0], [4.0, 0.1], (10.0, 0.0), abstol=1e-9, reltol=1e-9,
+                     saveat=np.arange(0.0, 11.0))
+    np.testing.assert_array_equal(np.array(sol.u), c["saveat"][::-1])
