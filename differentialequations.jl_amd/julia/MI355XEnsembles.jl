@@ -287,6 +287,7 @@ function _solve_batch(ens, alg, ealg::EnsembleMI355X, f, II; saveat, abstol, rel
             nk = length(pr.u0)
             if ns > 0
                 keep = [j for j in 1:ns if !isnan(US[k, 1, j])]
+                TFo[k] < pr.tspan[1] && reverse!(keep)   # backward: order of integration
                 ts = sa[keep]; us = [Vector{Float64}(US[k, 1:nk, j]) for j in keep]
             else
                 ts = [pr.tspan[1], TFo[k]]; us = [copy(pr.u0), UF[k, 1:nk]]
