@@ -320,7 +320,7 @@ struct Lane {
   int32_t naccept, nreject, nf, nstiff;
   int32_t isave, count, nanrej;
   int8_t rc, mode;
-  double tnext;  // saveat[isave], +inf once every point is saved
+  double tnext;  // saveat[isave]; +inf (-inf backward) once every point is saved
   // callbacks: condition values at the last step end, affects applied, and
   // the time of the last one
   double gprev[kNcb > 0 ? kNcb : 1];
@@ -436,7 +436,8 @@ struct Lane {
     t = t0;
     tcomp = 0.0;
     dtmax = a.dtmax > 0 ? a.dtmax : fabs(tf - t0);
-    // the step loop runs while t < tf, so max(t, tf) there is tf
+    // a step ending within 100 eps(tf) of tf ends at tf (forward, t < tf in
+    // the loop, so tf is the larger time; the same window is used backward)
     tsnap = 100.0 * jl_eps(tf);
     naccept = nreject = nstiff = 0;
     nf = 1;
