@@ -38,7 +38,7 @@ PEAK_HBM = 8000.0  # GB/s
 # WRITE_SIZE passes scaled by a 1 GiB copy kernel (factors 2.0 and 1.0);
 # profiles/tsit5_lorenz_1M_saveat100_fastctl_hbm_counters.md.  Re-measured
 # when the kernel changes; None where no counter pass exists for that form.
-MEASURED_TRAFFIC = {"tuned_tol": 12371242240}
+MEASURED_TRAFFIC = {}
 ALGORITHMIC_BYTES_PER_TRAJ = 48 + 100 * 3 * 8 + 24 + 8 + 1 + 12
 
 FORMS = {
@@ -47,10 +47,13 @@ FORMS = {
     "mixed": ("mixed", 1e-6, "lane", "float32", 0),
     "tuned": ("fp64", 1e-8, "lane", "float64", 1),
     "tuned_persistent": ("fp64", 1e-8, "persistent", "float64", 1),
-    # the loosest tolerance whose FP64 solve still meets the error gate
-    # (median 4.7e-7, 94 % of trajectories within 1e-6 vs the 1e-12 solve)
+    # a looser tolerance than the configuration states (reported beside the
+    # headline, never chosen as it)
     "tuned_tol": ("fp64", 1e-7, "lane", "float64", 1),
 }
+# The configuration's tolerance per precision: FP64 at 1e-8, FP32 stages at
+# 1e-6.  Only forms at their precision's stated tolerance can be the headline.
+CONFIG_TOL = {"fp64": 1e-8, "mixed": 1e-6}
 
 
 def parse():
@@ -64,6 +67,8 @@ def parse():
                     help="other configurations to time (comma list, '' for none)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=131072)
+    ap.add_argument("--no-e2e", action="store_true",
+                    help="skip the host-entry (copies included) timing")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (HBM counter calibration)")
     return ap.parse_args()
@@ -232,7 +237,7 @@ def other_robertson(torch, ctx, steps, warmup, N=262144):
                                                               na.data_ptr(), nr.data_ptr())
         return P, O, out
 
-    P, O, out = structs("fp64", 0, reltol=1e-10, abstol=1e-14)
+    P, O, out = structs("fp64", 0, reltol=1e-12, abstol=1e-14)
     ctx.solve_device(P, O, out, sp)
     torch.cuda.synchronize()
     ref = uf.cpu().numpy().copy()
@@ -252,10 +257,11 @@ def other_robertson(torch, ctx, steps, warmup, N=262144):
                          steps_mean=attempted / N,
                          weighted_err_median=float(np.median(werr)),
                          weighted_err_max=float(werr.max()),
+                         rel_err=errors(u, ref),
                          tflops=attempted * ROBERTSON_FLOPS_PER_STEP / (kms * 1e-3) / 1e12)
     return dict(workload="Robertson 262,144 trajectories, k log-uniform +-0.5 decades, "
                          "tspan (0,1e5), Rodas5 analytic J, abstol 1e-8 reltol 1e-6, endpoint",
-                error_reference="Rodas5 FP64 reltol 1e-10 abstol 1e-14 (device)",
+                error_reference="Rodas5 FP64 reltol 1e-12 abstol 1e-14 (device)",
                 flops_per_attempted_step=ROBERTSON_FLOPS_PER_STEP, forms=res)
 
 
@@ -393,19 +399,36 @@ def other_mix(torch, n_total=1_000_000, n_train=1_000_000):
     evals = _mix_sets(n_total // 3, seed=1234)
     forms = (("ordered", "fp64", "reference"), ("learned", "fp64", "fast"),
              ("learned_mixed", "mixed", "fast"), ("learned_trust", "fp64", "fast"))
-    res = {"plain": dict(kernel_ms=0.0, per_model_ms={})}
+    res = {"plain": dict(kernel_ms=0.0, per_model_ms={}),
+           "plain_fast": dict(kernel_ms=0.0, per_model_ms={})}
     for form, _, _ in forms:
         res[form] = dict(kernel_ms=0.0, infer_ms=0.0, per_model_ms={})
     agree = 0
     cnt = dict(plain_rej=0, learned_rej=0, plain_steps=0, learned_steps=0)
     errs = {k: [] for k in res}
+    rels = {k: [] for k in res}
+
+    def relerr(u, ref):
+        return np.linalg.norm(u - ref, axis=0) / np.linalg.norm(ref, axis=0)
+
     for m, w in evals.items():
         N = w["u0"].shape[1]
-        ref = run(m, w, reltol=1e-10, abstol=1e-12)
+        ref = run(m, w, reltol=1e-12, abstol=1e-14)
         plain = run(m, w)
         res["plain"]["kernel_ms"] += plain.kernel_ms
         res["plain"]["per_model_ms"][m] = plain.kernel_ms
+        pf = run(m, w, ctl="fast")
+        res["plain_fast"]["kernel_ms"] += pf.kernel_ms
+        res["plain_fast"]["per_model_ms"][m] = pf.kernel_ms
+        res["plain_fast"].setdefault("all_success", True)
+        res["plain_fast"]["all_success"] &= bool((pf.retcode == 1).all())
+        res["plain"].setdefault("all_success", True)
+        res["plain"]["all_success"] &= bool((plain.retcode == 1).all())
         at, rt = MIX_TOL[m]
+        errs["plain_fast"].append((np.abs(pf.u_final - ref.u_final) /
+                                   (at + rt * np.abs(ref.u_final))).max(axis=0))
+        rels["plain_fast"].append(relerr(pf.u_final, ref.u_final))
+        rels["plain"].append(relerr(plain.u_final, ref.u_final))
         Xe = torch.from_numpy(features(m, w["p"], w["u0"], 0.0, w["tf"], at, rt)).cuda()
         net.launch_order(net.infer(Xe))  # warm-up (first-call set-up untimed)
         torch.cuda.synchronize()
@@ -453,18 +476,32 @@ def other_mix(torch, n_total=1_000_000, n_train=1_000_000):
                 cnt["learned_steps"] += steps
             errs[form].append((np.abs(u - ref.u_final) /
                                (at + rt * np.abs(ref.u_final))).max(axis=0))
+            rels[form].append(relerr(u, ref.u_final))
         cnt["plain_rej"] += int(plain.nreject.sum())
         cnt["plain_steps"] += int((plain.naccept + plain.nreject).sum())
         errs["plain"].append((np.abs(plain.u_final - ref.u_final) /
                               (at + rt * np.abs(ref.u_final))).max(axis=0))
     for form in res:
         e = np.concatenate(errs[form])
+        rel = np.concatenate(rels[form])
         tot = res[form]["kernel_ms"] + res[form].get("infer_ms", 0.0)
         res[form].update(total_ms=tot, traj_per_s=n_total / (tot * 1e-3),
                          weighted_err_median=float(np.median(e)),
-                         weighted_err_p99=float(np.quantile(e, 0.99)))
-    for form, _, _ in forms:
-        res[form]["speedup_vs_plain"] = res["plain"]["total_ms"] / res[form]["total_ms"]
+                         weighted_err_p99=float(np.quantile(e, 0.99)),
+                         rel_err=dict(median=float(np.median(rel)),
+                                      p99=float(np.quantile(rel, 0.99)),
+                                      max=float(rel.max()),
+                                      frac_le_1e6=float((rel <= 1e-6).mean())))
+        res[form]["gate_ok"] = bool(res[form]["rel_err"]["median"] <= 1e-6
+                                    and res[form]["rel_err"]["frac_le_1e6"] >= 0.9
+                                    and res[form].get("all_success", False))
+    for form in list(res):
+        if form != "plain":
+            res[form]["speedup_vs_plain"] = res["plain"]["total_ms"] / res[form]["total_ms"]
+        if form not in ("plain", "plain_fast"):
+            # what the network itself buys: same controller, no predictions
+            res[form]["speedup_vs_plain_fast"] = (res["plain_fast"]["total_ms"] /
+                                                  res[form]["total_ms"])
     res["learned_trust"]["trust_logit"] = trust_thr
     return dict(workload="1e6 mix: 1/3 Lorenz (1e-8), 1/3 Robertson (1e-8/1e-6), 1/3 classic "
                          "Van der Pol mu log-uniform 1..1e4 tspan (0, 2mu) (1e-8/1e-6)",
@@ -472,8 +509,63 @@ def other_mix(torch, n_total=1_000_000, n_train=1_000_000):
                               epochs=4, batch=8192, final_loss=losses[-1] if losses else None),
                 class_agreement_with_autoswitch=agree / n_total,
                 counts=cnt,
-                error_metric="max_c |u - u*| / (abstol + reltol |u*|), u* = FP64 reltol 1e-10",
+                error_metric="weighted: max_c |u - u*| / (abstol + reltol |u*|); rel_err: "
+                             "|u - u*|_2 / |u*|_2 (gate: median <= 1e-6 and >= 90 % <= 1e-6); "
+                             "u* = FP64 AutoTsit5 reltol 1e-12 abstol 1e-14",
                 forms=res)
+
+
+def end_to_end(torch, ctx, N, ns=100, runs=3):
+    """The host entry point (mideq_ensemble_solve) on host buffers: inputs
+    copied in, solve, saveat values and final states copied out, timed by
+    the wall clock around each call.  Pageable numpy buffers (what a Julia
+    or numpy caller hands over) and page-locked ones."""
+    from mideq import _abi, workloads
+    w = workloads.lorenz(N, seed=1234)
+    sa = workloads.saveat_uniform(0.0, 10.0, ns)
+    res = {}
+    for kind in ("pageable", "pinned"):
+        if kind == "pageable":
+            u0 = np.ascontiguousarray(w["u0"])
+            p = np.ascontiguousarray(w["p"])
+            us = np.empty((ns, 3, N))
+            uf = np.empty((3, N))
+            ptr = lambda a: a.ctypes.data
+        else:
+            u0 = torch.from_numpy(w["u0"]).pin_memory()
+            p = torch.from_numpy(w["p"]).pin_memory()
+            us = torch.empty((ns, 3, N), dtype=torch.float64, pin_memory=True)
+            uf = torch.empty((3, N), dtype=torch.float64, pin_memory=True)
+            ptr = lambda a: a.data_ptr()
+        rc = np.empty(N, dtype=np.int8)
+        P = _abi.Problem()
+        P.model, P.n, P.np, P.ntraj = _abi.MODEL["lorenz"], 3, 3, N
+        P.u0, P.p, P.tspan0, P.tspan1 = ptr(u0), ptr(p), 0.0, 10.0
+        O = _abi.Options()
+        O.alg, O.precision, O.abstol, O.reltol = _abi.ALG["tsit5"], 0, 1e-8, 1e-8
+        O.qsteady_default, O.controller, O.schedule = 1, 1, _abi.SCHEDULE["lane"]
+        sa_h = np.ascontiguousarray(sa)
+        O.saveat, O.ns, O.save_dtype = sa_h.ctypes.data, ns, 0
+        out = _abi.Outputs()
+        out.us, out.u_final, out.retcode = ptr(us), ptr(uf), rc.ctypes.data
+        ctx.solve(P, O, out)  # warm-up (device buffers allocated here)
+        walls, totals, kernels = [], [], []
+        for _ in range(runs):
+            t0 = time.perf_counter()
+            ctx.solve(P, O, out)
+            walls.append(time.perf_counter() - t0)
+            st = ctx.stats()
+            totals.append(st.total_ms)
+            kernels.append(st.kernel_ms)
+        assert (rc == 1).all()
+        wall = statistics.median(walls)
+        res[kind] = dict(traj_per_s=N / wall, wall_ms=wall * 1e3,
+                         total_ms_events=statistics.median(totals),
+                         kernel_ms=statistics.median(kernels))
+    bytes_in = N * (3 + 3) * 8 + ns * 8
+    bytes_out = N * (ns * 3 * 8 + 3 * 8 + 1)
+    return dict(form="tuned (FP64, 1e-8, division-free controller), host entry point",
+                bytes_h2d=bytes_in, bytes_d2h=bytes_out, runs=runs, **res)
 
 
 def errors(u, ref):
@@ -482,25 +574,36 @@ def errors(u, ref):
                 max=float(e.max()), frac_le_1e6=float((e <= 1e-6).mean()))
 
 
-def cpu_baseline(sample, ns=100):
+def cpu_baseline(sample, ns=100, single_sample=16384):
     """The oracle's EnsembleThreads restatement (C, pthreads) on this host's
-    cores, on the first `sample` trajectories of the same workload."""
+    cores, on the first `sample` trajectories of the same workload; and on
+    one thread over the first `single_sample`."""
     sys.path.insert(0, os.path.join(REPO, "tests"))
     import oracle_lib
     from mideq import workloads
     O = oracle_lib.load()
     w = workloads.lorenz(sample, seed=1234)
     sa = workloads.saveat_uniform(0.0, 10.0, ns)
-    threads = min(os.cpu_count() or 1, 64)
+    # the host's share on a one-GPU box (16 threads), at most its core count
+    threads = min(os.cpu_count() or 1, 16)
     t0 = time.perf_counter()
     O.ensemble("lorenz", w["u0"], w["p"], 0.0, 10.0, nthreads=threads, saveat=sa,
                abstol=1e-8, reltol=1e-8)
     dt = time.perf_counter() - t0
+    m = min(single_sample, sample)
+    t1 = time.perf_counter()
+    O.ensemble("lorenz", np.ascontiguousarray(w["u0"][:, :m]),
+               np.ascontiguousarray(w["p"][:, :m]), 0.0, 10.0, nthreads=1, saveat=sa,
+               abstol=1e-8, reltol=1e-8)
+    d1 = time.perf_counter() - t1
     return dict(value=sample / dt, unit="trajectories/s", cores=threads, kind="port",
                 sample=f"first {sample} of the 1,048,576 seeded Lorenz trajectories, Tsit5 FP64 "
                        f"abstol=reltol=1e-8, saveat 100 points, oracle EnsembleThreads "
                        f"restatement on {threads} host threads, {dt:.2f} s wall "
-                       f"(~{dt * threads:.0f} thread-seconds)")
+                       f"(~{dt * threads:.0f} thread-seconds)",
+                single_thread_value=m / d1,
+                single_thread_sample=f"first {m} trajectories, same form, 1 thread, "
+                                     f"{d1:.2f} s wall")
 
 
 def main():
@@ -534,8 +637,9 @@ def main():
         r["tflops"] = flops / (r["kernel_ms_mean"] * 1e-3) / 1e12
         r["gate_ok"] = (err["median"] <= 1e-6 and err["frac_le_1e6"] >= 0.9
                         and r["all_success"])
+        r["config_tol"] = r["tol"] == CONFIG_TOL[r["precision"]]
         detail[f] = r
-    ok = [f for f in forms if detail[f]["gate_ok"]]
+    ok = [f for f in forms if detail[f]["gate_ok"] and detail[f]["config_tol"]]
     head = max(ok, key=lambda f: detail[f]["traj_per_s"]) if ok else forms[0]
     h = detail[head]
     ms = h["wall_s"] / a.steps * 1e3
@@ -550,6 +654,10 @@ def main():
                 attempted_steps=h["attempted_steps"], kernel_ms=h["kernel_ms_mean"],
                 output_gbps=out_bytes / (h["kernel_ms_mean"] * 1e-3) / 1e9)
     cpu = None if a.no_cpu else cpu_baseline(a.cpu_sample)
+    e2e = None
+    if not a.no_e2e:
+        print("[bench] end-to-end (host entry point)", file=sys.stderr, flush=True)
+        e2e = end_to_end(torch, ctx, a.ntraj)
     other = {}
     for name in [x for x in a.other.split(",") if x]:
         print(f"[bench] other config {name}", file=sys.stderr, flush=True)
@@ -571,15 +679,25 @@ def main():
                    "ntraj": a.ntraj, "headline_form": head,
                    "headline_tol": h["tol"],
                    "gate": "final-state rel err vs FP64 tol-1e-12 solve: median <= 1e-6 and "
-                           ">= 90 % of trajectories <= 1e-6"},
+                           ">= 90 % of trajectories <= 1e-6; headline only from forms at "
+                           "their precision's stated tolerance (FP64 1e-8, FP32 stages 1e-6)"},
         "roofline": roof,
         "cpu_baseline": cpu,
         "forms": {f: {k: v for k, v in d.items() if k != "kernel_ms"} for f, d in detail.items()},
     }
     if cpu:
+        # the baseline runs the same tolerance (1e-8) as the headline form
         line["gpu_vs_cpu"] = h["traj_per_s"] / cpu["value"]
+        line["gpu_vs_cpu_single_thread"] = h["traj_per_s"] / cpu["single_thread_value"]
     if "plain" in detail and head != "plain":
         line["tuned_vs_plain"] = h["traj_per_s"] / detail["plain"]["traj_per_s"]
+    if "tuned_tol" in detail and "plain" in detail:
+        d = detail["tuned_tol"]
+        line["looser_tol_form"] = dict(form="tuned_tol", tol=d["tol"], traj_per_s=d["traj_per_s"],
+                                       vs_plain_at_1e8=d["traj_per_s"] / detail["plain"]["traj_per_s"],
+                                       error=d["error"], gate_ok=d["gate_ok"])
+    if e2e:
+        line["end_to_end"] = e2e
     if other:
         line["other_configs"] = other
     print(json.dumps(line))

@@ -40,15 +40,7 @@ struct BrussArgs {
   unsigned long long* queue;  // next position in `order`
   const int64_t* order;       // work order (NULL: natural order)
   int64_t nqueue;             // queue length (<= ntraj; a sample when autotuning)
-  double* scratch;            // Rodas5 workspace, scratch_stride doubles per team
-  int64_t scratch_stride;
 };
-
-// Per-team Rodas5 workspace: W (nmax*nmax, column-major), reciprocal pivots
-// (nmax) and pivot rows (nmax, stored as doubles).
-inline int64_t bruss_scratch_per_team(int nmax) {
-  return (int64_t)nmax * nmax + 2 * (int64_t)nmax;
-}
 
 // Resident teams of the persistent grid for (team shape, alg, precision,
 // controller): grid blocks and teams per block.  Returns hipSuccess or an

@@ -7,11 +7,10 @@
 //     team's LDS slot (periodic indices).
 //   * Norms: per-lane partial sums of squares, then a butterfly tree across
 //     the team (shfl_xor), the order the CPU restatement reproduces.
-//   * Rodas5: W = J - I/(gamma dt) stored column-major in the team's
-//     scratch in HBM/L2, LU with partial pivoting parallel over rows (each
-//     matrix element sees the same fma sequence as a sequential LU),
-//     triangular solves column by column with the pivot element broadcast
-//     through LDS.
+//   * Rodas5: W = J - I/(gamma dt) in the team's LDS in band + border form
+//     (16 NK doubles), LU with partial pivoting over the structurally
+//     nonzero entries only (each of them sees the same fma sequence as in a
+//     dense sequential LU), triangular solves as register chains.
 // Stage arithmetic is FP64 or, in the mixed form, FP32 (RHS and stage
 // sums); state update, norm, time, J, W and LU are FP64.
 #include <type_traits>
@@ -54,6 +53,13 @@ __device__ __forceinline__ int team_or(int v) {
 }
 
 template <int L>
+__device__ __forceinline__ int team_min_int(int v) {
+#pragma unroll
+  for (int w = L / 2; w >= 1; w >>= 1) v = min(v, __shfl_xor(v, w, L));
+  return v;
+}
+
+template <int L>
 __device__ __forceinline__ int team_and(int v) {
 #pragma unroll
   for (int w = L / 2; w >= 1; w >>= 1) v &= __shfl_xor(v, w, L);
@@ -85,15 +91,18 @@ struct BTeam {
   double* ldsr;  // team LDS: reciprocal pivots
   float* ldsf;   // team LDS: FP32 exchange vector
   int* ldsi;     // team LDS: pivot rows
-  double* W;     // team scratch: W column-major
+  int nb;        // n - kT: positions of W stored in band form
+  double* Wb;    // team LDS (Rosenbrock kernels): W band rows
+  double* Wr;    //   their entries in the last kT columns
+  double* Wd;    //   the last kT rows, dense
 
   // Lane tl owns grid points j = tl + L*g (g < C/2), each as the pair of
   // components (u_j, v_j) = (2j, 2j + 1) in slots (2g, 2g + 1), so both rows
   // of a grid point's RHS are evaluated by one lane with no exchange.
   static_assert(C % 2 == 0, "slots hold (u, v) pairs");
   __device__ __forceinline__ int comp(int m) const { return 2 * (tl + L * (m >> 1)) + (m & 1); }
-  // Rows of W (and of the triangular solves) are owned lane-strided instead,
-  // so that a lane's row updates of a column-major W are coalesced.
+  // Rows of W (its construction and the border pass of the back
+  // substitution) are owned lane-strided.
   __device__ __forceinline__ int rowc(int m) const { return tl + L * m; }
 
   // ---------------------------------------------------------------- RHS
@@ -225,6 +234,7 @@ struct BTeam {
     const int64_t NT = a.ntraj;
     n = a.nvec ? a.nvec[i] : a.nmax;
     N = n / 2;
+    nb = n - kT;
     A = a.p[i];
     B = a.p[NT + i];
     double al = a.p[2 * NT + i];
@@ -477,221 +487,326 @@ struct BTeam {
   }
 
   // ---------------------------------------------------------------- Rodas5
-  // Column-major W in scratch: element (r, c) at W[c*n + r].
-  __device__ bool factor_W(double dts) {
-    // J at u: owned rows r; zero the row, then its four nonzeros
+  // W = J - I/(gamma dt) lives in the team's LDS in band + border form.  The
+  // Brusselator's J (interleaved u, v; periodic) has its nonzeros in
+  // columns r-2..r+2 of row r plus the wrap entries of rows 0, 1, n-2, n-1.
+  // During LU with partial pivoting every row still to be factored at step k
+  // keeps its nonzeros in columns [k, k+4] and the last kT columns, and the
+  // pivot comes from positions k, k+1, k+2 or the last kT rows.  Storage by
+  // position p (nb = n - kT):
+  //   p <  nb: Wb[p*8 + (c - p + 2)] for c in [p-2, p+4], c < nb (the two
+  //            entries left of the diagonal end up holding L multipliers)
+  //            Wr[p*kT + (c - nb)]  for c >= nb
+  //   p >= nb: Wd[(p - nb)*NK + c]  every column (dense rows)
+  // The factorization and the solves perform exactly the operations of the
+  // dense LU that involve a stored entry, in the same order per entry; the
+  // others multiply an exact zero and leave values unchanged, unless a
+  // non-finite value meets that zero (then the step is marked failed,
+  // EEst = NaN, as the dense factorization's NaNs would make it).  Row
+  // interchanges move columns >= k only (LINPACK form): the forward
+  // elimination applies each interchange when it reaches it, which per entry
+  // is the same sequence of operations as permuting first and then
+  // eliminating with the fully interchanged L.
+  static constexpr int kT = 4;
+  static constexpr int NK = L * C;
+
+  __device__ __forceinline__ double* wp(int p, int c) const {
+    if (p >= nb) return Wd + (p - nb) * NK + c;
+    if (c >= nb) return Wr + p * kT + (c - nb);
+    return Wb + p * 8 + (c - p + 2);
+  }
+  // pivot candidates at step k: k, then k+1, k+2 while < nb, then the
+  // dense positions past k, ascending (nbc band candidates past k)
+  __device__ __forceinline__ int nband_after(int k) const { return max(0, min(2, nb - 1 - k)); }
+  __device__ __forceinline__ int cand(int k, int nbc, int i) const {
+    return i == 0 ? k : (i <= nbc ? k + i : max(nb, k + 1) + (i - 1 - nbc));
+  }
+  // columns of the pivot row's pattern: k, then k+1.. while < nb and <= k+4
+  // (nwc of them), then the border columns past k
+  __device__ __forceinline__ int nwin_after(int k) const { return max(0, min(4, nb - 1 - k)); }
+  __device__ __forceinline__ int patcol(int k, int nwc, int j) const {
+    return j == 0 ? k : (j <= nwc ? k + j : max(nb, k + 1) + (j - 1 - nwc));
+  }
+
+  // Builds W and factors it.  0: factored; 1: a zero pivot came first
+  // (EEst = 2, as the dense LU's early return); 2: a non-finite value met a
+  // skipped zero first (EEst = NaN).
+  __device__ int factor_band(double dts) {
 #pragma unroll
     for (int m = 0; m < C; m++) {
       int c = comp(m);
       if (c < n) ldsd[c] = u[m];
     }
+    for (int e = tl; e < 8 * nb; e += L) Wb[e] = 0.0;
+    for (int e = tl; e < kT * nb; e += L) Wr[e] = 0.0;
+    for (int e = tl; e < kT * NK; e += L) Wd[e] = 0.0;
     team_sync();
+    const double dtgamma = dts * rd_gamma<kP>();
+    const double w = 1.0 / dtgamma;
+    double best = 0.0;
 #pragma unroll
     for (int m = 0; m < C; m++) {
       int r = rowc(m);
       if (r < n) {
-        for (int c = 0; c < n; c++) W[(int64_t)c * n + r] = 0.0;
         int j = r >> 1;
         int jm = (j == 0) ? N - 1 : j - 1;
         int jp = (j == N - 1) ? 0 : j + 1;
         double uu = ldsd[2 * j], vv = ldsd[2 * j + 1];
         double uv2 = 2.0 * uu * vv, u2 = uu * uu;
-        int ru = 2 * j, rv = 2 * j + 1;
+        // the row's four J entries (0.0 + value, as J accumulates into a
+        // zeroed matrix) in ascending column order; e_d the diagonal one
+        int c0, c1, c2, c3, id;
+        double e0, e1, e2, e3;
+        const double dd = 0.0 + D;
         if ((r & 1) == 0) {
-          W[(int64_t)ru * n + ru] = 0.0 + (uv2 - B1 - 2.0 * D);
-          W[(int64_t)rv * n + ru] = 0.0 + u2;
-          W[(int64_t)(2 * jm) * n + ru] = 0.0 + D;
-          W[(int64_t)(2 * jp) * n + ru] = 0.0 + D;
+          const double a = 0.0 + (uv2 - B1 - 2.0 * D), b = 0.0 + u2;
+          if (j == 0) {  // columns 0, 1, 2, n-2
+            c0 = r; e0 = a; c1 = r + 1; e1 = b; c2 = 2 * jp; e2 = dd; c3 = 2 * jm; e3 = dd; id = 0;
+          } else if (j == N - 1) {  // columns 0, n-4, n-2, n-1
+            c0 = 2 * jp; e0 = dd; c1 = 2 * jm; e1 = dd; c2 = r; e2 = a; c3 = r + 1; e3 = b; id = 2;
+          } else {
+            c0 = 2 * jm; e0 = dd; c1 = r; e1 = a; c2 = r + 1; e2 = b; c3 = 2 * jp; e3 = dd; id = 1;
+          }
         } else {
-          W[(int64_t)ru * n + rv] = 0.0 + (B - uv2);
-          W[(int64_t)rv * n + rv] = 0.0 + (-u2 - 2.0 * D);
-          W[(int64_t)(2 * jm + 1) * n + rv] = 0.0 + D;
-          W[(int64_t)(2 * jp + 1) * n + rv] = 0.0 + D;
+          const double a = 0.0 + (B - uv2), b = 0.0 + (-u2 - 2.0 * D);
+          if (j == 0) {  // columns 0, 1, 3, n-1
+            c0 = r - 1; e0 = a; c1 = r; e1 = b; c2 = 2 * jp + 1; e2 = dd; c3 = 2 * jm + 1; e3 = dd; id = 1;
+          } else if (j == N - 1) {  // columns 1, n-3, n-2, n-1
+            c0 = 2 * jp + 1; e0 = dd; c1 = 2 * jm + 1; e1 = dd; c2 = r - 1; e2 = a; c3 = r; e3 = b; id = 3;
+          } else {
+            c0 = 2 * jm + 1; e0 = dd; c1 = r - 1; e1 = a; c2 = r; e2 = b; c3 = 2 * jp + 1; e3 = dd; id = 2;
+          }
         }
+        if (kAuto) {
+          double sr = 0.0 + fabs(e0);
+          sr = sr + fabs(e1);
+          sr = sr + fabs(e2);
+          sr = sr + fabs(e3);
+          best = (sr > best) ? sr : best;
+        }
+        *wp(r, c0) = (id == 0) ? e0 - w : e0;
+        *wp(r, c1) = (id == 1) ? e1 - w : e1;
+        *wp(r, c2) = (id == 2) ? e2 - w : e2;
+        *wp(r, c3) = (id == 3) ? e3 - w : e3;
       }
     }
+    if (kAuto) eig = team_max<L>(best);
     team_sync();
-    if (kAuto) {
-      double best = 0.0;
-#pragma unroll
-      for (int m = 0; m < C; m++) {
-        int r = rowc(m);
-        if (r < n) {
-          double s = 0.0;
-          for (int c = 0; c < n; c++) s += fabs(W[(int64_t)c * n + r]);
-          best = (s > best) ? s : best;
-        }
-      }
-      eig = team_max<L>(best);
-    }
-    double dtgamma = dts * rd_gamma<kP>();
-    double w = 1.0 / dtgamma;
-#pragma unroll
-    for (int m = 0; m < C; m++) {
-      int r = rowc(m);
-      if (r < n) W[(int64_t)r * n + r] = W[(int64_t)r * n + r] - w;
-    }
-    team_sync();
-    // LU with partial pivoting (first maximal |pivot|), rows in parallel
-    int singular = 0;
+    int sing_k = n, pois_k = n;
     for (int k = 0; k < n; k++) {
-      double bv = -1.0;
-      int bi = n;
-#pragma unroll
-      for (int m = 0; m < C; m++) {
-        int r = rowc(m);
-        if (r >= k && r < n) {
-          double v = fabs(W[(int64_t)k * n + r]);
-          if (v > bv) {
-            bv = v;
-            bi = r;
-          }
+      const int tail0 = max(nb, k + 1);
+      const int nbc = nband_after(k);
+      const int nc = 1 + nbc + (n - tail0);
+      const int nwc = nwin_after(k);
+      const int np = 1 + nwc + (n - tail0);
+      // pivot: the first maximal |W(q, k)| over the candidates (every other
+      // row of column k is an exact zero)
+      double d = *wp(k, k);
+      double bv = fabs(d);
+      int pk = k;
+      for (int i = 1; i < nc; i++) {
+        const int q = cand(k, nbc, i);
+        const double v = *wp(q, k);
+        if (fabs(v) > bv) {
+          bv = fabs(v);
+          pk = q;
+          d = v;
         }
       }
-#pragma unroll
-      for (int s = L / 2; s >= 1; s >>= 1) {
-        double ov = __shfl_xor(bv, s, L);
-        int oi = __shfl_xor(bi, s, L);
-        bool take = (ov > bv) || (ov == bv && oi < bi);
-        bv = take ? ov : bv;
-        bi = take ? oi : bi;
-      }
-      // no comparable pivot (NaN column): keep row k, flag singular
-      if (bi >= n) {
-        bi = k;
-        singular = 1;
-      }
-      int pk = bi;
       if (pk != k) {
-#pragma unroll
-        for (int m = 0; m < C; m++) {
-          int c = rowc(m);
-          if (c < n) {
-            double x = W[(int64_t)c * n + k], y = W[(int64_t)c * n + pk];
-            W[(int64_t)c * n + k] = y;
-            W[(int64_t)c * n + pk] = x;
-          }
+        for (int jc = tl; jc < np; jc += L) {
+          const int c = patcol(k, nwc, jc);
+          double* a = wp(k, c);
+          double* b = wp(pk, c);
+          const double x = *a, y = *b;
+          *a = y;
+          *b = x;
         }
         team_sync();
       }
-      double d = W[(int64_t)k * n + k];
-      singular |= (d == 0.0) ? 1 : 0;
-      double rr = 1.0 / d;
+      if (d == 0.0 && sing_k == n) sing_k = k;
+      const double rr = 1.0 / d;
       if (tl == 0) {
         ldsr[k] = rr;
         ldsi[k] = pk;
       }
-      double l[C];
-#pragma unroll
-      for (int m = 0; m < C; m++) {
-        int r = rowc(m);
-        l[m] = 0.0;
-        if (r > k && r < n) {
-          l[m] = W[(int64_t)k * n + r] * rr;
-          W[(int64_t)k * n + r] = l[m];
+      // rows past k outside the candidates / columns outside the pattern:
+      // the dense LU multiplies their zeros (skipped here)
+      const bool skip_rows = (n - 1 - k) > (nc - 1);
+      const bool skip_cols = (n - 1 - k) > (np - 1);
+      bool bad = skip_rows && !(fabs(rr) <= __DBL_MAX__);
+      // trailing update a_qc -= l_q a_kc over candidate rows x pattern columns
+      for (int pi = tl; pi < (nc - 1) * 8; pi += L) {
+        const int cj = 1 + (pi & 7);
+        if (cj < np) {
+          const int q = cand(k, nbc, 1 + (pi >> 3));
+          const int c = patcol(k, nwc, cj);
+          const double l = *wp(q, k) * rr;
+          const double wk = *wp(k, c);
+          double* a = wp(q, c);
+          *a = fma(-l, wk, *a);
+          bad = bad || (skip_rows && !(fabs(wk) <= __DBL_MAX__));
         }
       }
-      // trailing update a_rc -= l_r a_kc, kUpd columns at a time: all loads
-      // of a chunk issue before its stores, so a chunk waits on one memory
-      // round trip instead of one per column (row k is not written here)
-      constexpr int kUpd = 8;
-      for (int c0 = k + 1; c0 < n; c0 += kUpd) {
-        double wk[kUpd];
-#pragma unroll
-        for (int j = 0; j < kUpd; j++) wk[j] = (c0 + j < n) ? W[(int64_t)(c0 + j) * n + k] : 0.0;
-#pragma unroll
-        for (int m = 0; m < C; m++) {
-          int r = rowc(m);
-          if (r > k && r < n) {
-            double wr[kUpd];
-#pragma unroll
-            for (int j = 0; j < kUpd; j++)
-              wr[j] = (c0 + j < n) ? W[(int64_t)(c0 + j) * n + r] : 0.0;
-#pragma unroll
-            for (int j = 0; j < kUpd; j++)
-              if (c0 + j < n) W[(int64_t)(c0 + j) * n + r] = fma(-l[m], wk[j], wr[j]);
-          }
-        }
+      // multipliers, stored after every read of column k above
+      for (int qi = tl; qi < nc - 1; qi += L) {
+        double* a = wp(cand(k, nbc, 1 + qi), k);
+        const double l = *a * rr;
+        *a = l;
+        bad = bad || (skip_cols && !(fabs(l) <= __DBL_MAX__));
       }
+      if (bad && pois_k == n) pois_k = k;
       team_sync();
     }
-    return singular == 0;
+    pois_k = team_min_int<L>(pois_k);
+    if (sing_k < n && sing_k <= pois_k) return 1;
+    return pois_k < n ? 2 : 0;
   }
 
-  // Solve W x = b for owned components (x in/out, component layout); rows
-  // are processed in row ownership, pivots applied as the composite
-  // permutation sigma (value at row position r comes from sigma[r]).
-  __device__ void lu_solve(double* x, const int* sigma) {
+  // Solve W x = b with the factors of factor_band; x: owned components in
+  // and out.  Forward elimination (interchanges as they come) and back
+  // substitution walk the band as chains in registers, every lane of the
+  // team computing the same values (lane 0 stores them).  Returns false when
+  // a non-finite value met a skipped zero (the dense solve's NaNs).
+  __device__ bool band_solve(double* x) {
 #pragma unroll
     for (int m = 0; m < C; m++) {
       int c = comp(m);
       if (c < n) ldsd[c] = x[m];
     }
     team_sync();
-    double y[C];
+    bool bad = false;
+    double bd0 = ldsd[nb], bd1 = ldsd[nb + 1], bd2 = ldsd[nb + 2], bd3 = ldsd[nb + 3];
+    double b0 = ldsd[0], b1 = ldsd[1], b2 = ldsd[2];
+    // forward, positions j with j + 2 < nb: candidates j+1, j+2 and the
+    // dense rows (7); rows past j+2 outside them while j < n - 7
+    for (int j = 0; j <= nb - 3; j++) {
+      const int pj = ldsi[j];
+      const double l1 = Wb[(j + 1) * 8 + 1], l2 = Wb[(j + 2) * 8];
+      const double d0 = Wd[j], d1 = Wd[NK + j], d2 = Wd[2 * NK + j], d3 = Wd[3 * NK + j];
+      const double bn = (j + 3 < nb) ? ldsd[j + 3] : 0.0;
+      if (pj != j) {
+        double t = b0;
+        if (pj == j + 1) {
+          b0 = b1;
+          b1 = t;
+        } else if (pj == j + 2) {
+          b0 = b2;
+          b2 = t;
+        } else if (pj == nb) {
+          b0 = bd0;
+          bd0 = t;
+        } else if (pj == nb + 1) {
+          b0 = bd1;
+          bd1 = t;
+        } else if (pj == nb + 2) {
+          b0 = bd2;
+          bd2 = t;
+        } else {
+          b0 = bd3;
+          bd3 = t;
+        }
+      }
+      const double xj = b0;
+      if (tl == 0) ldsd[j] = xj;
+      b1 = fma(-l1, xj, b1);
+      b2 = fma(-l2, xj, b2);
+      bd0 = fma(-d0, xj, bd0);
+      bd1 = fma(-d1, xj, bd1);
+      bd2 = fma(-d2, xj, bd2);
+      bd3 = fma(-d3, xj, bd3);
+      bad = bad || (j < n - 7 && !(fabs(xj) <= __DBL_MAX__));
+      b0 = b1;
+      b1 = b2;
+      b2 = bn;
+    }
+    // the last two band positions and the dense ones: every later position
+    // is a candidate
+    double v[6] = {b0, b1, bd0, bd1, bd2, bd3};
 #pragma unroll
-    for (int m = 0; m < C; m++) y[m] = (rowc(m) < n) ? ldsd[sigma[m]] : 0.0;
+    for (int jj = 0; jj < 6; jj++) {
+      const int j = nb - 2 + jj;
+      const int pj = ldsi[j] - (nb - 2);
+#pragma unroll
+      for (int ii = jj + 1; ii < 6; ii++) {
+        if (pj == ii) {
+          const double t = v[jj];
+          v[jj] = v[ii];
+          v[ii] = t;
+        }
+      }
+      const double xj = v[jj];
+      if (tl == 0) ldsd[j] = xj;
+#pragma unroll
+      for (int ii = jj + 1; ii < 6; ii++) v[ii] = fma(-*wp(nb - 2 + ii, j), xj, v[ii]);
+    }
+    // back substitution, dense block (every row above holds its entry)
+#pragma unroll
+    for (int jj = 5; jj >= 2; jj--) {
+      const int j = nb - 2 + jj;
+      v[jj] = v[jj] * ldsr[j];
+#pragma unroll
+      for (int ii = 2; ii < jj; ii++) v[ii] = fma(-Wd[(ii - 2) * NK + j], v[jj], v[ii]);
+    }
+    if (tl == 0) {
+#pragma unroll
+      for (int t = 0; t < kT; t++) ldsd[nb + t] = v[2 + t];
+    }
     team_sync();
-    // x_j lives in lane j % L, slot j / L: broadcast it with a shuffle, so
-    // no memory fence stands between consecutive columns and the W loads
-    // of later columns can issue early
-    for (int j = 0; j < n; j++) {
-      const int mj = j / L;
-      double own = y[0];
+    // band rows: the border columns (j = n-1 down to nb), lanes in parallel
 #pragma unroll
-      for (int m = 1; m < C; m++) own = (m == mj) ? y[m] : own;
-      double xj = __shfl(own, j % L, L);
-#pragma unroll
-      for (int m = 0; m < C; m++) {
-        int r = rowc(m);
-        if (r > j && r < n) y[m] = fma(-W[(int64_t)j * n + r], xj, y[m]);
+    for (int m = 0; m < C; m++) {
+      const int r = rowc(m);
+      if (r < nb) {
+        double y = ldsd[r];
+        const double* R = Wr + r * kT;
+        y = fma(-R[3], v[5], y);
+        y = fma(-R[2], v[4], y);
+        y = fma(-R[1], v[3], y);
+        y = fma(-R[0], v[2], y);
+        ldsd[r] = y;
       }
     }
-    for (int j = n - 1; j >= 0; j--) {
-      const int mj = j / L;
-#pragma unroll
-      for (int m = 0; m < C; m++)
-        if (rowc(m) == j) y[m] = y[m] * ldsr[j];
-      double own = y[0];
-#pragma unroll
-      for (int m = 1; m < C; m++) own = (m == mj) ? y[m] : own;
-      double xj = __shfl(own, j % L, L);
-#pragma unroll
-      for (int m = 0; m < C; m++) {
-        int r = rowc(m);
-        if (r < j) y[m] = fma(-W[(int64_t)j * n + r], xj, y[m]);
-      }
-    }
     team_sync();
-    // back from row ownership to the (u, v) component layout
-#pragma unroll
-    for (int m = 0; m < C; m++)
-      if (rowc(m) < n) ldsd[rowc(m)] = y[m];
+    // band rows: the window columns i+4 .. i+1 (those < nb), a chain
+    double x1 = 0.0, x2 = 0.0, x3 = 0.0, x4 = 0.0;
+    for (int i = nb - 1; i >= 0; i--) {
+      const double* U = Wb + i * 8;
+      const double u6 = U[6], u5 = U[5], u4 = U[4], u3 = U[3];
+      const double ri = ldsr[i];
+      double y = ldsd[i];
+      if (i + 4 < nb) y = fma(-u6, x4, y);
+      if (i + 3 < nb) y = fma(-u5, x3, y);
+      if (i + 2 < nb) y = fma(-u4, x2, y);
+      if (i + 1 < nb) y = fma(-u3, x1, y);
+      const double xi = y * ri;
+      if (tl == 0) ldsd[i] = xi;
+      // column i: rows above i-4 hold exact zeros
+      bad = bad || (i >= 5 && !(fabs(xi) <= __DBL_MAX__));
+      x4 = x3;
+      x3 = x2;
+      x2 = x1;
+      x1 = xi;
+    }
     team_sync();
 #pragma unroll
     for (int m = 0; m < C; m++) x[m] = (comp(m) < n) ? ldsd[comp(m)] : 0.0;
     team_sync();
+    return !team_or<L>(bad ? 1 : 0);
   }
 
   __device__ bool rodas5_step(double dts, double* u1, double& EEst, double abstol,
                               double reltol) {
-    if (!factor_W(dts)) {
+    const int fac = factor_band(dts);
+    if (fac == 1) {
       EEst = 2.0;
       return false;
     }
-    int sigma[C];
-#pragma unroll
-    for (int m = 0; m < C; m++) {
-      int pos = rowc(m);
-      for (int k = n - 1; k >= 0; k--) {
-        int pk = ldsi[k];
-        if (pos == k)
-          pos = pk;
-        else if (pos == pk)
-          pos = k;
-      }
-      sigma[m] = pos;
+    if (fac == 2) {
+      EEst = __builtin_nan("");
+      return true;
     }
+    bool solved = true;
     double inv_dt = 1.0 / dts;
     double kd[8][C];
     double U[C], du[C], x[C];
@@ -754,7 +869,7 @@ struct BTeam {
         }
         x[m] = -r;
       }
-      lu_solve(x, sigma);
+      solved = band_solve(x) && solved;
 #pragma unroll
       for (int m = 0; m < C; m++) kd[s][m] = MIX ? (double)(float)x[m] : x[m];
     }
@@ -767,7 +882,7 @@ struct BTeam {
       acc = acc + kd[6][m];
       u1[m] = acc + kd[7][m];
     }
-    EEst = err_norm(kd[7], u1, abstol, reltol);
+    EEst = solved ? err_norm(kd[7], u1, abstol, reltol) : __builtin_nan("");
     return true;
   }
 
@@ -934,19 +1049,27 @@ template <int L, int NK, bool MIX, int ALG, bool FAST>
 __device__ __forceinline__ void bruss_run(const BrussArgs& a) {
   constexpr int C = NK / L;
   constexpr int TPB = 256 / L;
+  using Team = BTeam<L, C, MIX, ALG, FAST>;
+  // W in band + border form: 8 NK band entries, kT NK border columns, kT
+  // dense rows of NK (Rosenbrock-capable kernels only)
+  constexpr int WSZ = Team::kHasRodas ? (8 + 2 * Team::kT) * NK : 1;
   __shared__ double s_d[TPB][NK];
   __shared__ double s_r[TPB][NK];
   __shared__ float s_f[TPB][NK];
   __shared__ int s_i[TPB][NK];
+  __shared__ double s_w[Team::kHasRodas ? TPB : 1][WSZ];
   const int team = threadIdx.x / L;
-  BTeam<L, C, MIX, ALG, FAST> T;
+  Team T;
   T.tl = threadIdx.x % L;
   T.ldsd = s_d[team];
   T.ldsr = s_r[team];
   T.ldsf = s_f[team];
   T.ldsi = s_i[team];
-  const int64_t gteam = (int64_t)blockIdx.x * TPB + team;
-  T.W = a.scratch ? a.scratch + gteam * a.scratch_stride : nullptr;
+  if constexpr (Team::kHasRodas) {
+    T.Wb = s_w[team];
+    T.Wr = T.Wb + 8 * NK;
+    T.Wd = T.Wr + Team::kT * NK;
+  }
   bool active = false, drained = false;
   while (true) {
     if (!active && !drained) {

@@ -313,7 +313,7 @@ struct Lane {
   double t, tcomp, tf, dt, dtmax;
   bool back;    // tf < t0: integrated backward with steps of -dt (dt > 0 is the magnitude)
   double tdir;  // +1 or -1 (signs by multiplication: one instruction, no selects)
-  double tsnap;  // 100 eps(tf): a step ending this close to tf ends at tf
+  double tsnap;  // 100 eps(tf): a forward step ending this close to tf ends at tf
   double qold, qoldpow, qsteady_max, eig, dtfirst, lqold;
   double eigd;  // fast form: the stiffness estimate is eig / eigd
   int64_t iter;
@@ -436,8 +436,9 @@ struct Lane {
     t = t0;
     tcomp = 0.0;
     dtmax = a.dtmax > 0 ? a.dtmax : fabs(tf - t0);
-    // a step ending within 100 eps(tf) of tf ends at tf (forward, t < tf in
-    // the loop, so tf is the larger time; the same window is used backward)
+    // a step ending within 100 eps(max(t, tf)) of tf ends at tf, t the
+    // step's start: forward t < tf in the loop, so the window is fixed;
+    // backward max(t, tf) = t and the window is taken per step
     tsnap = 100.0 * jl_eps(tf);
     naccept = nreject = nstiff = 0;
     nf = 1;
@@ -972,7 +973,8 @@ struct Lane {
         ttmp = t + y;
         tcomp = (ttmp - t) - y;
       }
-      if (fabs(ttmp - tf) < tsnap) ttmp = tf;
+      const double snapw = back ? 100.0 * jl_eps(fmax(t, tf)) : tsnap;
+      if (fabs(ttmp - tf) < snapw) ttmp = tf;
       t = ttmp;
       // dense output of this step at th (time tprev + th dt): Tsit5's
       // interpolant or the Rosenbrock one (rd_H, FP64 in both precisions)

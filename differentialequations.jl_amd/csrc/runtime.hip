@@ -244,13 +244,35 @@ static int check_user_spec(const mideq_user_model* m, std::string* why) {
 // float, or Dual<N> for the Jacobian when none is given).
 // Does the statement text use the identifier t (the time)?  Then the model
 // is non-autonomous: f and jac take t, and df/dt comes from dual numbers.
-static bool uses_time(const char* text) {
-  if (!text) return false;
+// Comments (// and /* */) and character/string literals are skipped.  The
+// statements sit in a block of their own, so a model that declares a local
+// t shadows the time parameter (it is then merely treated as
+// non-autonomous, with df/dt = 0).
+extern "C" int mideq_uses_time(const char* text) {
+  if (!text) return 0;
   auto ident = [](char ch) { return std::isalnum((unsigned char)ch) || ch == '_'; };
-  for (size_t k = 0; text[k]; k++)
-    if (text[k] == 't' && (k == 0 || !ident(text[k - 1])) && !ident(text[k + 1])) return true;
-  return false;
+  for (size_t k = 0; text[k]; k++) {
+    const char ch = text[k];
+    if (ch == '/' && text[k + 1] == '/') {
+      while (text[k + 1] && text[k + 1] != '\n') k++;
+    } else if (ch == '/' && text[k + 1] == '*') {
+      k += 2;
+      while (text[k] && !(text[k] == '*' && text[k + 1] == '/')) k++;
+      if (!text[k]) return 0;
+      k++;
+    } else if (ch == '"' || ch == '\'') {
+      for (k++; text[k] && text[k] != ch; k++)
+        if (text[k] == '\\' && text[k + 1]) k++;
+      if (!text[k]) return 0;
+    } else if (ident(ch)) {
+      size_t b = k;
+      while (ident(text[k + 1])) k++;
+      if (k == b && ch == 't' && (b == 0 || text[b - 1] != '.')) return 1;
+    }
+  }
+  return 0;
 }
+static bool uses_time(const char* text) { return mideq_uses_time(text) != 0; }
 
 static std::string user_model_source(const mideq_user_model* m) {
   const int np = m->np > 0 ? m->np : 1;
@@ -260,14 +282,15 @@ static std::string user_model_source(const mideq_user_model* m) {
   s += std::string("  static constexpr bool TDEP = ") + (tdep ? "true" : "false") + ";\n";
   s += std::string("  template <class T>\n  __device__ static void f(T* du, const T* u, const T* p") +
        (tdep ? ", T t" : "") + ") {\n";
-  s += "#line 1 \"rhs\"\n";
+  // the statements in a block of their own: a local t shadows the time
+  s += "  {\n#line 1 \"rhs\"\n";
   s += m->rhs;
-  s += std::string("\n  }\n  __device__ static void jac(double (*J)[N], const double* u, "
+  s += std::string("\n  }\n  }\n  __device__ static void jac(double (*J)[N], const double* u, "
                    "const double* p") + (tdep ? ", double t" : "") + ") {\n";
   if (m->jac) {
-    s += "#line 1 \"jac\"\n";
+    s += "  {\n#line 1 \"jac\"\n";
     s += m->jac;
-    s += "\n";
+    s += "\n  }\n";
   } else {
     s += "    Dual<N> ud[N], pd[NP], fd[N];\n"
          "    for (int i = 0; i < N; i++) { ud[i].v = u[i]; ud[i].d[i] = 1.0; }\n"
@@ -329,9 +352,9 @@ static std::string user_model_source(const mideq_user_model* m) {
   if (m->nout > 0) {
     s += "  static constexpr int NOUT = " + std::to_string(m->nout) + ";\n";
     s += "  __device__ static void output(double* out, const double* u, const double* p, "
-         "double t, int rc) {\n#line 1 \"output\"\n";
+         "double t, int rc) {\n  {\n#line 1 \"output\"\n";
     s += m->output;
-    s += "\n  }\n";
+    s += "\n  }\n  }\n";
   }
   s += "};\n}  // namespace mideq\n";
   return s;
@@ -604,17 +627,6 @@ static int bruss_launch_with(mideq_ctx* c, BrussArgs b, int lanes, int alg, int 
   const int nk = bruss_nk(lanes, b.nmax);
   hipError_t e = bruss_plan(lanes, nk, alg, mix, fast, c->ncu, b.nqueue, &grid, &tpb);
   if (e != hipSuccess) return set_err(c, MIDEQ_E_HIP, "bruss_plan: %s", hipGetErrorString(e));
-  if (alg != MIDEQ_ALG_TSIT5) {
-    b.scratch_stride = bruss_scratch_per_team(b.nmax);
-    size_t bytes = sizeof(double) * (size_t)b.scratch_stride * (size_t)grid * (size_t)tpb;
-    void* d;
-    int rc = ensure_buf(c, BUF_SCRATCH, bytes, &d);
-    if (rc) return rc;
-    b.scratch = (double*)d;
-  } else {
-    b.scratch = nullptr;
-    b.scratch_stride = 0;
-  }
   *grid_used = grid;
   struct Key {
     BrussArgs b;

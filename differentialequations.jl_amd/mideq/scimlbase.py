@@ -17,6 +17,7 @@ output_func; output_func returns (out, rerun); reduction(u, batch, I)
 returns (u, converged).
 """
 import enum
+import math
 import time
 from dataclasses import dataclass, field, replace
 from typing import Any, Callable, Optional
@@ -502,18 +503,86 @@ def _pack(ens, idx, repeat):
     return f, u0, p, t0, tf, nvec, probs
 
 
+def _rat(x):
+    """Continued-fraction rational approximation of a float, as Julia's
+    Base.rat does for Float64 ranges (coefficients bounded by
+    maxintfloat(Float32) = 2^24): returns (num, den), den == 0 if none."""
+    m = 16777216.0
+    y = x
+    a, d = 1, 1
+    b, c = 0, 0
+    while abs(y) <= m:
+        f = int(y)  # trunc
+        y -= f
+        a, c = f * a + c, a
+        b, d = f * b + d, b
+        if max(abs(a), abs(b)) > m:
+            return c, d
+        if b != 0 and float(a) / float(b) == x:
+            break
+        if y == 0.0:
+            break
+        y = 1.0 / y
+    return a, b
+
+
+def julia_range(start, step, stop):
+    """The elements of Julia's `start:step:stop` for Float64 (base/
+    twiceprecision.jl semantics): when start, step and stop are exact
+    rationals p/q (found by `_rat`), element k is the correctly rounded
+    value of (start_n + k step_n) / den, so 0:0.1:0.3 ends at 0.3 exactly;
+    otherwise element k is start + k step correctly rounded, and the length
+    is round((stop - start) / step) + 1, minus one on overshoot."""
+    from fractions import Fraction
+    start, step, stop = float(start), float(step), float(stop)
+    if step == 0.0:
+        raise ValueError("range step cannot be zero")
+
+    def between(a, x, b):
+        return (a <= x <= b) or (b <= x <= a)
+
+    def tdiv(p, q):  # Julia's div: truncation toward zero
+        r = abs(p) // abs(q)
+        return r if (p >= 0) == (q > 0) else -r
+
+    sn, sd = _rat(step)
+    if sd != 0 and sn / sd == step:
+        an, ad = _rat(start)
+        zn, zd = _rat(stop)
+        if ad != 0 and zd != 0 and an / ad == start and zn / zd == stop:
+            den = (ad // math.gcd(ad, sd)) * sd  # Julia's lcm_unchecked (sign kept)
+            m = 2.0 ** 53
+            if abs(start * den) <= m and abs(step * den) <= m:
+                a_n = int(round(start * den))
+                s_n = int(round(step * den))
+                ln = max(0, tdiv(den * zn - zd * a_n + s_n * zd, s_n * zd))
+                if (between(start, start + (ln - 1) * step, stop + step / 2)
+                        and not between(start, start + ln * step, stop)):
+                    return np.array([float(Fraction(a_n + k * s_n, den)) for k in range(ln)])
+    lf = (stop - start) / step
+    if lf < 0:
+        ln = 0
+    elif lf == 0:
+        ln = 1
+    else:
+        ln = int(round(lf)) + 1
+        stop2 = start + (ln - 1) * step
+        ln -= int(start < stop < stop2) + int(start > stop > stop2)
+    fs, fd = Fraction(start), Fraction(step)
+    return np.array([float(fs + k * fd) for k in range(ln)])
+
+
 def _saveat_vector(saveat, tspan):
     if saveat is None:
         return None
     if np.isscalar(saveat):
-        # t0, t0 + dir d, ... and tf, in the direction of integration; handed
-        # over in ascending order (a backward solve fills them from the end)
-        t0, t1 = tspan
+        # the range t0:dir*d:tf, then tf itself when the range stops short of
+        # it (save_end), in the direction of integration; handed over in
+        # ascending order (a backward solve fills them from the end)
+        t0, t1 = float(tspan[0]), float(tspan[1])
         sgn = 1.0 if t1 >= t0 else -1.0
-        d = float(saveat)
-        n = int(np.floor(abs(t1 - t0) / d + 1e-9)) + 1
-        pts = t0 + sgn * d * np.arange(n)
-        if (pts[-1] - t1) * sgn < 0:
+        pts = julia_range(t0, sgn * abs(float(saveat)), t1)
+        if len(pts) == 0 or pts[-1] != t1:
             pts = np.append(pts, t1)
         return np.ascontiguousarray(np.sort(pts), dtype=np.float64)
     return np.ascontiguousarray(np.sort(np.asarray(saveat, dtype=np.float64)))

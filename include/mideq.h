@@ -290,6 +290,14 @@ int mideq_model_register(mideq_ctx *ctx, const mideq_user_model *m, int32_t *mod
  * NULL) receives the compiler output.  Returns 0 or MIDEQ_E_ARG. */
 int mideq_model_check(const mideq_user_model *m, char *log, size_t log_size);
 
+/* 1 if statement text names the time t -- the rule by which
+ * mideq_model_register makes a model non-autonomous -- else 0.  t is
+ * reserved for the time: comments and character/string literals are not
+ * read, and a model that declares its own local t shadows it (the model is
+ * then treated as non-autonomous with df/dt = 0).  No counterpart in the
+ * reference; bindings use it to report how a model will be compiled. */
+int mideq_uses_time(const char *text);
+
 /* ---- learned dt0 / method-class / cost predictor (no counterpart in the
  * reference: it feeds dt0, alg_hint and order above; the initdt heuristic
  * and AutoTsit5 switching remain the fallback).  16 -> 64 -> 64 -> 64 -> 3

@@ -28,8 +28,13 @@
  *        derivation).
  *   PI controller, tstop snap, saveat .... pinned: hand-computed single
  *        steps, 11-point saveat count and exact u0 copy (core.jl:34,95).
- *   AutoSwitch thresholds ................ parity unpinned (upstream
- *        constants restated, not observable without Julia).
+ *   AutoSwitch rule ...................... pinned: on u' = lambda u and
+ *        on a lambda(t) whose stiffness drops, the traced method / dt
+ *        sequence equals a replay of the rule (stability size 3.5068, 9/10,
+ *        10 / 3 consecutive tests, dtfac 2) with the exact estimate
+ *        |lambda|; the estimates themselves equal |lambda| (Tsit5 to
+ *        rounding at the step end, |J|_inf exactly at the step start).
+ *        switch_max: not restated (see the AutoSwitch constants below).
  *   accept/reject sequences vs upstream .. parity unpinned (the upstream
  *        FastPower approximation and Julia's muladd association cannot be
  *        observed without Julia; the restatement fixes its own).
@@ -179,6 +184,11 @@ static const rdtab_t RD5P = {
 #define AS_TOL 0.9
 #define AS_TOL_S (AS_TOL * AS_STAB_SIZE) /* fast form of the stiffness test */
 #define AS_DTFAC 2.0
+/* Upstream's AutoSwitch also takes switch_max = 5.  Its role (a bound on
+ * consecutive non-stiff test results that decides whether a step's error
+ * test may be skipped) cannot be checked against the source here; this
+ * restatement runs every step's error test and never caps the number of
+ * switches.  Counted per trajectory in oracle_result.nswitch. */
 
 /* ------------------------------------------------------------------------ */
 /* Controller power: x^y = exp2(y log2 x) with own series (atanh series for  */
@@ -1158,6 +1168,10 @@ static void step_interp(int n, int mode, int mixed, const rdtab_t *rtab, const d
 /* ------------------------------------------------------------------------ */
 /* The integrator loop (solve! with loopheader!/perform_step!/loopfooter!). */
 /* ------------------------------------------------------------------------ */
+/* Trace of the attempted steps of one solve (oracle_solve_trace only). */
+static __thread oracle_trace_rec *g_trace = NULL;
+static __thread int32_t g_trace_cap = 0, g_trace_len = 0;
+
 static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
                      const oracle_options *o, int64_t idx, double *save, long save_stride,
                      oracle_result *res, rodas_ws *ws) {
@@ -1242,6 +1256,7 @@ static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
     rc = ORACLE_RC_INITIALFAILURE;
   while (rc == ORACLE_RC_DEFAULT && (back ? t > tf : t < tf)) {
     /* loopheader!: choose algorithm, fix dt at bounds, clamp to tstop */
+    const double dt_hdr = dt;
     if (autosw && iter > 0) {
       int stiff;
       if (!g_fast_ctl || n > 3) {
@@ -1333,6 +1348,18 @@ static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
       lq = fmax(L2_QMAXINV, fmin(L2_QMININV, lq));
     }
 
+    if (g_trace) {
+      if (g_trace_len < g_trace_cap) {
+        oracle_trace_rec *r = &g_trace[g_trace_len];
+        r->t = t;
+        r->dt_hdr = dt_hdr;
+        r->dt = dt;
+        r->eig = eig / eigd;
+        r->mode = mode;
+        r->accepted = EEst <= 1.0;
+      }
+      g_trace_len++;
+    }
     if (EEst <= 1.0) {
       /* step_accept_controller! */
       res->naccept++;
@@ -1532,6 +1559,19 @@ int oracle_solve(int32_t model, int32_t n, const double *u0, const double *p,
   solve_one(&m, u0, t0, tf, opt, 0, out_save, 1, res, ws);
   free(ws);
   return 0;
+}
+
+int oracle_solve_trace(int32_t model, int32_t n, const double *u0, const double *p,
+                       int32_t np, double t0, double tf, const oracle_options *opt,
+                       double *out_save, oracle_result *res, oracle_trace_rec *trace,
+                       int32_t cap) {
+  g_trace = trace;
+  g_trace_cap = trace ? cap : 0;
+  g_trace_len = 0;
+  int rc = oracle_solve(model, n, u0, p, np, t0, tf, opt, out_save, res);
+  g_trace = NULL;
+  g_trace_cap = 0;
+  return rc != 0 ? rc : g_trace_len;
 }
 
 /* ------------------------------------------------------------------------ */

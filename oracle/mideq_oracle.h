@@ -135,6 +135,23 @@ int oracle_solve(int32_t model, int32_t n, const double *u0, const double *p,
                  int32_t np, double t0, double tf, const oracle_options *opt,
                  double *out_save, oracle_result *res);
 
+/* One attempted step of a traced solve: the step's start time, dt as the
+ * loop header found it (set by the previous controller decision), the size
+ * attempted (after the AutoSwitch factor and the dtmax / tf clamps), the
+ * method used (0 Tsit5, 1 Rosenbrock), whether it was accepted, and the
+ * stiffness estimate left by the attempt (eig / eigd). */
+typedef struct {
+  double t, dt_hdr, dt, eig;
+  int32_t mode, accepted;
+} oracle_trace_rec;
+
+/* oracle_solve recording up to `cap` attempted steps into trace; returns
+ * the number of attempts made (may exceed cap: only cap are stored). */
+int oracle_solve_trace(int32_t model, int32_t n, const double *u0, const double *p,
+                       int32_t np, double t0, double tf, const oracle_options *opt,
+                       double *out_save, oracle_result *res, oracle_trace_rec *trace,
+                       int32_t cap);
+
 /* EnsembleThreads restatement: ntraj independent solves on nthreads host
  * threads with dynamic chunking, results in index order.
  * SoA layouts: u0[c*ntraj + i], p[j*ntraj + i], out_save[(k*n + c)*ntraj + i],

@@ -18,6 +18,8 @@ RC = dict(default=0, success=1, maxiters=2, dtlessthanmin=3, unstable=4, dtnan=5
 NMAX = 128
 
 _dp = C.POINTER(C.c_double)
+TRACE_DTYPE = np.dtype([("t", "<f8"), ("dt_hdr", "<f8"), ("dt", "<f8"), ("eig", "<f8"),
+                        ("mode", "<i4"), ("accepted", "<i4")])
 _ip = C.POINTER(C.c_int32)
 
 
@@ -159,6 +161,27 @@ class Oracle:
         if save is not None:
             out["saveat"] = save
         return out
+
+    def solve_trace(self, model, u0, p, tspan, cap=200000, **kw):
+        """oracle_solve with the record of every attempted step: (result
+        dict, structured array of t, dt_hdr, dt, eig, mode, accepted)."""
+        u0 = np.ascontiguousarray(u0, dtype=np.float64).reshape(-1)
+        p = np.ascontiguousarray(p, dtype=np.float64).reshape(-1)
+        o = self.options(**kw)
+        n = len(u0)
+        rec = np.zeros(cap, dtype=TRACE_DTYPE)
+        r = Result()
+        L = self.lib
+        L.oracle_solve_trace.argtypes = [C.c_int32, C.c_int32, _dp, _dp, C.c_int32, C.c_double,
+                                         C.c_double, C.POINTER(Options), _dp, C.POINTER(Result),
+                                         C.c_void_p, C.c_int32]
+        m = L.oracle_solve_trace(MODEL[model], n, _ptr(u0), _ptr(p), len(p), tspan[0], tspan[1],
+                                 C.byref(o), None, C.byref(r), rec.ctypes.data, cap)
+        assert 0 <= m <= cap, m
+        out = dict(u=np.array(r.u[:n]), t=r.t, retcode=r.retcode, naccept=r.naccept,
+                   nreject=r.nreject, nf=r.nf, nstiff=r.nstiff_steps, nswitch=r.nswitch,
+                   first_stiff_iter=r.first_stiff_iter, dt0=r.dt0)
+        return out, rec[:m]
 
     # ------------------------------------------------------ ensemble (SoA)
     def ensemble(self, model, u0, p, t0, tf, nthreads=1, nvec=None, **kw):

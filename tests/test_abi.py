@@ -80,6 +80,30 @@ def test_saveat_scalar_expansion_matches_core_jl():
     np.testing.assert_allclose(f, [0.0, 0.3, 0.6, 0.9, 1.0], atol=1e-15)
 
 
+def test_saveat_scalar_range_rounds_as_julia():
+    """A scalar saveat expands as Julia's Float64 range t0:d:tf: with
+    rational t0, d, tf the k-th point is k d rounded once (0:0.1:0.3 ends at
+    0.3, not 0.30000000000000004), so tf itself is always among the points.
+    Expected values are the correctly rounded quotients k / 10 (one IEEE
+    division of exact integers)."""
+    from mideq.scimlbase import _saveat_vector, julia_range
+    for stop, npts in ((0.3, 4), (0.7, 8), (1.0, 11)):
+        v = _saveat_vector(0.1, (0.0, stop))
+        assert len(v) == npts
+        assert list(v) == [k / 10 for k in range(npts)]
+        assert v[-1] == stop
+    b = _saveat_vector(0.1, (0.3, 0.0))  # backward span, handed over ascending
+    assert list(b) == [0.0, 0.1, 0.2, 0.3]
+    # a step that does not divide the span: the range stops short, tf appended
+    v = _saveat_vector(0.1, (0.0, 0.25))
+    assert list(v) == [0.0, 0.1, 0.2, 0.25]
+    # non-rational inputs: t0 + k d rounded once, length round((tf - t0)/d) + 1
+    r = julia_range(0.0, 1.0 / 3.0, 1.0)
+    assert len(r) == 4 and r[-1] == 1.0
+    assert list(julia_range(1.0, -0.25, 0.0)) == [1.0, 0.75, 0.5, 0.25, 0.0]
+    assert len(julia_range(0.0, 0.1, -1.0)) == 0
+
+
 def test_user_model_limits_match_header():
     import re
     txt = open(_abi.HEADER).read()

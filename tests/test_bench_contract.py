@@ -31,3 +31,15 @@ def test_forms_and_gate():
     assert bench.FORMS["plain"] == ("fp64", 1e-8, "lane", "float64", 0)
     assert set(bench.MEASURED_TRAFFIC) <= set(bench.FORMS)
     assert bench.FLOPS_PER_STEP == 262 and bench.FLOPS_PER_SAVE == 98
+
+
+def test_headline_only_at_the_stated_tolerance():
+    """The headline is chosen among forms at their precision's stated
+    tolerance (FP64 1e-8, FP32 stages 1e-6); a looser form is reported
+    beside it, never as it."""
+    sys.path.insert(0, ROOT)
+    import bench
+    assert bench.CONFIG_TOL == {"fp64": 1e-8, "mixed": 1e-6}
+    on = [f for f, v in bench.FORMS.items() if v[1] == bench.CONFIG_TOL[v[0]]]
+    assert {"plain", "tuned", "mixed"} <= set(on)
+    assert "tuned_tol" not in on

@@ -127,3 +127,79 @@ def test_bruss_narrow_batch_32_lane_shape(oracle, alg, precision, controller):
     assert (c["retcode"] == 1).all()
     assert_identical(g, c, nvec=w["nvec"])
     np.testing.assert_array_equal(g.nstiff, c["nstiff"])
+
+
+def batch_range(ntraj, seed, lo, hi, tf_max):
+    """Grid sizes drawn from lo..hi (the first two at hi, so n = 2 hi is
+    always present), initial states rebuilt for them."""
+    w = workloads.brusselator(ntraj, seed=seed)
+    ng = lo + (w["nvec"] // 2 - 8) % (hi - lo + 1)
+    ng[:2] = hi
+    u0 = np.zeros((2 * hi, ntraj))
+    for k, g in enumerate(ng):
+        x = np.arange(g) / g
+        u0[0:2 * g:2, k] = 1 + np.sin(2 * np.pi * x)
+        u0[1:2 * g:2, k] = 3.0
+    return dict(u0=u0, p=w["p"], tf=np.minimum(w["tf"], tf_max),
+                nvec=(2 * ng).astype(np.int32))
+
+
+@pytest.mark.parametrize("alg", ["rodas5", "rodas5p", "auto", "auto5p"])
+@pytest.mark.parametrize("lanes", [64, 32])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_bruss_rosenbrock_team_full_size_range(oracle, alg, lanes, precision):
+    """The Rosenbrock team path over the whole size range of the uneven
+    batch: Ngrid 21..64 (n = 42..128, so every row of a 128 x 128 W and every
+    lane-strided row slot of both team shapes is factored and solved),
+    against the oracle: identical retcodes and step, f-eval and stiff-step
+    counts, equal final states."""
+    N = 64
+    say(f"brusselator {alg} {precision} lanes={lanes} Ngrid 21..64 N={N}")
+    w = batch_range(N, seed=70, lo=21, hi=64, tf_max=1.5)
+    tol = dict(abstol=1e-8, reltol=1e-6)
+    g = gpu("brusselator", w["u0"], w["p"], 0.0, w["tf"], nvec=w["nvec"], lanes=lanes,
+            alg=alg, precision=precision, schedule="persistent", **tol)
+    c = cpu(oracle, "brusselator", w["u0"], w["p"], 0.0, w["tf"], nvec=w["nvec"],
+            norm_lanes=lanes, alg=alg, precision=precision, **tol)
+    assert (c["retcode"] == 1).all()
+    if alg.startswith("auto"):
+        assert (c["nstiff"] > 0).any() and (c["nstiff"] < c["naccept"]).any()
+    assert g.lanes == lanes
+    assert_identical(g, c, nvec=w["nvec"])
+    np.testing.assert_array_equal(g.nstiff, c["nstiff"])
+
+
+@pytest.mark.parametrize("alg,lanes,precision", [("rodas5", 64, "fp64"), ("auto", 32, "fp64"),
+                                                 ("auto5p", 64, "mixed"),
+                                                 ("rodas5p", 32, "mixed")])
+def test_bruss_rosenbrock_team_fast_controller_full_size(oracle, alg, lanes, precision):
+    """Same size range with the division-free controller."""
+    N = 48
+    say(f"brusselator fast controller {alg} {precision} lanes={lanes} Ngrid 21..64 N={N}")
+    w = batch_range(N, seed=71, lo=21, hi=64, tf_max=1.5)
+    tol = dict(abstol=1e-8, reltol=1e-6)
+    g = gpu("brusselator", w["u0"], w["p"], 0.0, w["tf"], nvec=w["nvec"], lanes=lanes,
+            alg=alg, precision=precision, schedule="persistent", controller="fast", **tol)
+    c = cpu(oracle, "brusselator", w["u0"], w["p"], 0.0, w["tf"], nvec=w["nvec"],
+            norm_lanes=lanes, alg=alg, precision=precision, controller=1, **tol)
+    assert (c["retcode"] == 1).all()
+    assert_identical(g, c, nvec=w["nvec"])
+    np.testing.assert_array_equal(g.nstiff, c["nstiff"])
+
+
+def test_bruss_tsit5_32_lane_fp64_fast_full_size(oracle):
+    """The benchmarked instantiation: Tsit5, FP64, division-free controller,
+    32-lane teams over the full size range (n up to 128), i.e. the kernel
+    built for 4 waves per SIMD."""
+    N = 256
+    say(f"brusselator tsit5 fp64 fast lanes=32 full range N={N}")
+    w = batch(N, seed=72, tf_max=4.0)
+    assert w["nvec"].max() == 128
+    tol = dict(abstol=1e-8, reltol=1e-6)
+    g = gpu("brusselator", w["u0"], w["p"], 0.0, w["tf"], nvec=w["nvec"], lanes=32,
+            schedule="persistent", controller="fast", **tol)
+    c = cpu(oracle, "brusselator", w["u0"], w["p"], 0.0, w["tf"], nvec=w["nvec"],
+            norm_lanes=32, controller=1, **tol)
+    assert (c["retcode"] == 1).all()
+    assert g.lanes == 32
+    assert_identical(g, c, nvec=w["nvec"])

@@ -173,6 +173,31 @@ def test_backward_and_forward_spans_with_saveat(oracle, alg, precision, controll
     np.testing.assert_allclose(inside, exact, rtol=1e-4)
 
 
+@pytest.mark.parametrize("alg,precision,controller", [
+    ("tsit5", "fp64", "reference"), ("tsit5", "mixed", "fast"), ("rodas5", "fp64", "fast"),
+    ("auto", "fp64", "reference")])
+def test_backward_span_ending_near_zero_snaps_to_tf(oracle, alg, precision, controller):
+    """Backward spans whose end is tiny next to their start, e.g. (1, 1e-3)
+    and (10, 1e-6): the last step's end snaps to tf within 100 eps(max(t,
+    tf)) = 100 eps(t), t the step's start (a window of 100 eps(tf) would miss
+    and overshoot).  t_final == tf and results bit-exact vs the oracle."""
+    N = 512
+    say(f"backward spans ending near 0 {alg} {precision} {controller} N={N}")
+    rng = np.random.default_rng(46)
+    u0 = np.ones((1, N))
+    p = rng.uniform(0.5, 2.0, (1, N))  # u' = p u run backward: decay
+    t0 = rng.choice([1.0, 2.0, 10.0], N) * rng.uniform(0.9, 1.1, N)
+    tf = rng.choice([1e-3, 1e-6, 1e-9, 0.0, -1e-7], N) * rng.uniform(0.5, 2.0, N)
+    kw = dict(abstol=1e-8, reltol=1e-6)
+    g = gpu("exp", u0, p, t0, tf, alg=alg, precision=precision, controller=controller, **kw)
+    c = cpu(oracle, "exp", u0, p, t0, tf, alg=alg, precision=precision,
+            controller=1 if controller == "fast" else 0, **kw)
+    assert (c["retcode"] == 1).all()
+    np.testing.assert_array_equal(c["t"], tf)
+    assert_identical(g, c)
+    np.testing.assert_array_equal(g.t_final, tf)
+
+
 def test_per_trajectory_tspans_and_fixed_dt(oracle):
     N = 500
     say(f"per-trajectory tspan N={N}")

@@ -480,3 +480,113 @@ def test_autoswitch(oracle):
     assert r["retcode"] == 1 and r["nswitch"] >= 1 and r["first_stiff_iter"] > 10
     assert r["nstiff"] > 0.5 * r["naccept"]
     assert abs(r["u"].sum() - 1) < 1e-12
+
+
+# --------------------------------------------- AutoSwitch pinned by closed forms
+# AutoTsit5's switching rule (OrdinaryDiffEq AutoSwitch): before every attempt
+# but the first, the attempt's dt times the last stiffness estimate over
+# Tsit5's real stability size 3.5068 is compared with 9/10; consecutive
+# positive results count up, negative ones down; more than 10 positives in
+# Tsit5 switch to the Rosenbrock method with dt * 2, more than 3 negatives in
+# the Rosenbrock method switch back with dt / 2.  These values are written
+# here from that definition, not read from the oracle.
+AS_STAB, AS_TOL, AS_MAXSTIFF, AS_MAXNONSTIFF, AS_DTFAC = 3.5068, 0.9, 10, 3, 2.0
+
+# u' = lambda(t) (u - g(t)) + g'(t), g = 1 / (1 + t^2) (the solution from
+# u(0) = 1), |lambda| dropping from p0 to p1 past t = p2 (lambda rational in
+# t, so host and device evaluate it identically): stiff, then not
+DROP_LAM = ("T s = t / p[2]; T s2 = s * s; T s4 = s2 * s2; T s8 = s4 * s4;\n"
+            "T lam = -(p[1] + (p[0] - p[1]) / (1.0 + s8));\n")
+DROP_RHS = DROP_LAM + "T q = 1.0 / (1.0 + t * t);\ndu[0] = lam * (u[0] - q) - 2.0 * t * q * q;"
+DROP_JAC = DROP_LAM.replace("T ", "double ") + "J[0][0] = lam;"
+
+
+DROP_PARAMS = ([1e4, 1.0, 1.0], [3e3, 0.5, 1.5], [2e4, 2.0, 0.8])
+
+
+def lam_drop(p, t):
+    s = t / p[2]
+    s2 = s * s
+    s4 = s2 * s2
+    s8 = s4 * s4
+    return -(p[1] + (p[0] - p[1]) / (1.0 + s8))
+
+
+def replay_autoswitch(tr, eig_exact, tf, dtmax, start_mode=0):
+    """Replays the switching rule on a traced solve with the exact stiffness
+    estimate of every attempt (eig_exact(record)); asserts the method and the
+    dt of every attempt.  Returns (switches to stiff, switches back) or None
+    when a test lands within 1e-6 of the threshold (undecidable here)."""
+    count, mode, up, down = 0, start_mode, 0, 0
+    for i, r in enumerate(tr):
+        dt = r["dt_hdr"]
+        if i > 0:
+            s = eig_exact(tr[i - 1]) * r["dt_hdr"] / AS_STAB
+            if abs(s / AS_TOL - 1.0) < 1e-6:
+                return None
+            if s > AS_TOL:
+                count = 1 if count < 0 else count + 1
+            else:
+                count = -1 if count > 0 else count - 1
+            if mode == 0 and count > AS_MAXSTIFF:
+                mode, dt, up = 1, dt * AS_DTFAC, up + 1
+            elif mode == 1 and count < -AS_MAXNONSTIFF:
+                mode, dt, down = 0, dt / AS_DTFAC, down + 1
+        dt = min(dt, dtmax, tf - r["t"])
+        assert r["mode"] == mode, (i, r, mode)
+        assert r["dt"] == dt, (i, r, dt)
+    return up, down
+
+
+@pytest.mark.parametrize("alg", ["auto", "auto5p"])
+@pytest.mark.parametrize("controller", [0, 1])
+def test_autoswitch_switch_to_stiff_pinned_on_linear_decay(oracle, alg, controller):
+    """u' = lambda u: Tsit5's estimate |(k7 - k6) / (u1 - g6)| is |lambda|
+    (to rounding) and the Rosenbrock one, |J|_inf, is |lambda| exactly; the
+    whole method/dt sequence follows from the rule above."""
+    decided = 0
+    for lam in (-20.0, -150.0, -1000.0, -3.3e3, -2.0e4, -1.0e5):
+        r, tr = oracle.solve_trace("exp", [1.0], [lam], (0.0, 3.0), alg=alg, abstol=1e-8,
+                                   reltol=1e-6, controller=controller)
+        assert r["retcode"] == 1 and len(tr) == r["naccept"] + r["nreject"]
+        t0 = tr["mode"] == 0
+        np.testing.assert_allclose(tr["eig"][t0], abs(lam), rtol=1e-9)
+        assert (tr["eig"][~t0] == abs(lam)).all()
+        got = replay_autoswitch(tr, lambda rec: abs(lam), 3.0, 3.0)
+        if got is None:
+            continue
+        decided += 1
+        # |lambda| = 20 stays below Tsit5's stability bound at this accuracy
+        assert got == ((1, 0) if lam <= -150.0 else (0, 0)) and r["nswitch"] == got[0]
+        assert r["nstiff"] == int((tr["accepted"][tr["mode"] == 1]).sum())
+    assert decided >= 5
+
+
+@pytest.mark.parametrize("alg", ["auto", "auto5p"])
+def test_autoswitch_switch_back_pinned_on_dropping_stiffness(oracle, alg):
+    """A stiff phase that ends: |lambda(t)| drops from ~1e4 to ~1 past
+    t = p2.  Estimates: Tsit5's at the step end (stages 6 and 7 both at t +
+    dt), the Rosenbrock one |J(t)| at the step start; the rule then fixes one
+    switch to the Rosenbrock method and one back to Tsit5 (dt / 2)."""
+    import user_model_host as umh
+    umh.install(oracle, 1, 3, DROP_RHS, jac=DROP_JAC)
+    decided = 0
+    for p in DROP_PARAMS:
+        p = np.array(p)
+        r, tr = oracle.solve_trace("user", [1.0], p, (0.0, 6.0), alg=alg, abstol=1e-8,
+                                   reltol=1e-6)
+        assert r["retcode"] == 1
+        np.testing.assert_allclose(r["u"][0], 1.0 / 37.0, rtol=1e-5)
+
+        def eig_exact(rec):
+            tt = rec["t"] + rec["dt"] if rec["mode"] == 0 else rec["t"]
+            return abs(lam_drop(p, tt))
+
+        ex = np.array([eig_exact(x) for x in tr])
+        np.testing.assert_allclose(tr["eig"], ex, rtol=1e-6)
+        got = replay_autoswitch(tr, eig_exact, 6.0, 6.0)
+        if got is None:
+            continue
+        decided += 1
+        assert got == (1, 1) and r["nswitch"] == 2
+    assert decided >= 2

@@ -222,3 +222,28 @@ def test_oracle_user_forced_oscillator_closed_form(oracle, alg):
     assert (r["retcode"] == 1).all()
     x = 4.0 / w**2 - np.sin(4.0 * w) / w**3
     np.testing.assert_allclose(r["u"][0], x, rtol=1e-7, atol=1e-9)
+
+
+def test_uses_time_rule_and_local_t():
+    """The library's rule for a non-autonomous model (mideq_uses_time) skips
+    comments and literals and equals the host restatement; a model declaring
+    its own local t compiles (the statements are a block of their own)."""
+    mideq = _mideq()
+    from mideq import _abi
+    lib = _abi.load()
+    lib.mideq_uses_time.argtypes = [C.c_char_p]
+    cases = {
+        PROTHERO: 1, FORCED: 1, LORENZ: 0, ROBERTSON: 0,
+        "du[0] = -u[0]; // decays in t": 0,
+        "du[0] = -u[0]; /* not t-dependent:\n t */": 0,
+        "du[0] = -u[0] * (t > 1.0 ? 2.0 : 1.0);": 1,
+        "const char* s = \"t\"; du[0] = -u[0]; char c = 't';": 0,
+        "T tt = u[0]; du[0] = tt;": 0,
+        "du[0] = s.t;": 0,
+        "du[0] = sin(t) /* t */;": 1,
+        "double t = u[0]; du[0] = -t;": 1,
+    }
+    for text, want in cases.items():
+        assert lib.mideq_uses_time(text.encode()) == want, text
+        assert umh.uses_time(text) == bool(want), text
+    assert mideq.UserModel(1, 1, "T t = u[0]; du[0] = -t;").check() == ""

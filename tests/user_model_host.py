@@ -19,8 +19,13 @@ _CACHE = {}
 
 
 def uses_time(text):
-    """The device's rule: the statements name the identifier t."""
-    return bool(text) and re.search(r"(?<![A-Za-z0-9_])t(?![A-Za-z0-9_])", text) is not None
+    """The device's rule: the statements name the identifier t, outside
+    comments and character/string literals (and not as a member after '.')."""
+    if not text:
+        return False
+    code = re.sub(r'//[^\n]*|/\*.*?\*/|"(?:\\.|[^"\\])*"|\'(?:\\.|[^\'\\])*\'', " ", text,
+                  flags=re.S)
+    return re.search(r"(?<![A-Za-z0-9_.])t(?![A-Za-z0-9_])", code) is not None
 
 
 def model_source(n, np_, rhs, jac=None, callbacks=()):
@@ -31,9 +36,10 @@ def model_source(n, np_, rhs, jac=None, callbacks=()):
          "namespace mideq {", "struct ModelUser {",
          f"  static constexpr int N = {n}, NP = {np_};",
          "  template <class T>", f"  static void f(T* du, const T* u, const T* p{ta}) {{",
-         rhs, "  }", f"  static void jac(double (*J)[N], const double* u, const double* p{tj}) {{"]
+         "  {", rhs, "  }", "  }",
+         f"  static void jac(double (*J)[N], const double* u, const double* p{tj}) {{"]
     if jac is not None:
-        s.append(jac)
+        s += ["  {", jac, "  }"]
     else:
         s += ["    Dual<N> ud[N], pd[NP], fd[N];",
               "    for (int i = 0; i < N; i++) { ud[i].v = u[i]; ud[i].d[i] = 1.0; }",
