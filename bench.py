@@ -65,6 +65,10 @@ def parse():
     ap.add_argument("--forms", default="plain,mixed,tuned,tuned_persistent,tuned_tol")
     ap.add_argument("--other", default="robertson,brusselator,mix",
                     help="other configurations to time (comma list, '' for none)")
+    ap.add_argument("--bruss-forms", default="plain,tuned,mixed,auto,auto_tuned,rodas5",
+                    help="forms of the Brusselator leg (comma list)")
+    ap.add_argument("--bruss-ntraj", type=int, default=131072,
+                    help="instances of the Brusselator leg (a prefix of the seeded batch)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=131072)
     ap.add_argument("--no-e2e", action="store_true",
@@ -265,46 +269,60 @@ def other_robertson(torch, ctx, steps, warmup, N=262144):
                 flops_per_attempted_step=ROBERTSON_FLOPS_PER_STEP, forms=res)
 
 
-def other_brusselator(torch, steps, warmup, N=131072):
+def other_brusselator(torch, steps, warmup, N=131072, only=None):
     """Uneven Brusselator batch (n = 16..128, tspan ends 1..20), one
-    persistent kernel, autotuned lanes per instance."""
+    persistent kernel, autotuned lanes per instance.  Tsit5 forms, and the
+    configuration's "Tsit5 or Rodas5 by stiffness" as AutoTsit5(Rodas5())
+    over the whole batch (band LU in LDS); weighted error of every form
+    against an FP64 Tsit5 solve at reltol 1e-12, abstol 1e-14."""
     from mideq import EnsembleMI355X, brusselator, device_solve, workloads
-    from mideq.scimlbase import AutoTsit5, Tsit5
+    from mideq.scimlbase import AutoTsit5, Rodas5, Tsit5
     w = workloads.brusselator(N, seed=1234)
     t0 = np.zeros(N)
+    nv = w["nvec"]
+    rows = np.arange(w["u0"].shape[0])[:, None] < nv[None, :]
+
+    def solve(alg, ealg, reltol=1e-6, abstol=1e-8):
+        return device_solve(brusselator, w["u0"], w["p"], t0, w["tf"], nv, alg, ealg,
+                            abstol=abstol, reltol=reltol)
+
+    print("[bench] brusselator reference (Tsit5 FP64 1e-12)", file=sys.stderr, flush=True)
+    ref = solve(Tsit5(), EnsembleMI355X(schedule="persistent"), reltol=1e-12, abstol=1e-14)
+    uref = ref.u_final
     res = {}
-    for name, prec, ctl in (("plain", "fp64", "reference"), ("tuned", "fp64", "fast"),
-                            ("mixed", "mixed", "reference")):
+    forms = (("plain", "fp64", "reference", Tsit5()), ("tuned", "fp64", "fast", Tsit5()),
+             ("mixed", "mixed", "reference", Tsit5()),
+             ("auto", "fp64", "reference", AutoTsit5(Rodas5())),
+             ("auto_tuned", "fp64", "fast", AutoTsit5(Rodas5())),
+             ("rodas5", "fp64", "fast", Rodas5()))
+    for name, prec, ctl, alg in forms:
+        if only is not None and name not in only:
+            continue
+        print(f"[bench] brusselator {name}", file=sys.stderr, flush=True)
         ealg = EnsembleMI355X(precision=prec, schedule="persistent", controller=ctl)
         r = None
         for _ in range(max(1, warmup)):
-            r = device_solve(brusselator, w["u0"], w["p"], t0, w["tf"], w["nvec"], Tsit5(), ealg,
-                             abstol=1e-8, reltol=1e-6)
+            r = solve(alg, ealg)
         kms = []
         for _ in range(steps):
-            r = device_solve(brusselator, w["u0"], w["p"], t0, w["tf"], w["nvec"], Tsit5(), ealg,
-                             abstol=1e-8, reltol=1e-6)
+            r = solve(alg, ealg)
             kms.append(r.kernel_ms)
-        res[name] = dict(kernel_ms=statistics.fmean(kms), instances_per_s=N / (statistics.fmean(kms) * 1e-3),
-                         lanes=r.lanes, ok=float((r.retcode == 1).mean()),
-                         steps_mean=float((r.naccept + r.nreject).mean()), u=r.u_final)
-    ref = res["plain"].pop("u")
-    for name in ("tuned", "mixed"):
-        d = np.abs(res[name].pop("u") - ref) / (1e-8 + 1e-6 * np.abs(ref))
-        res[name]["weighted_diff_vs_fp64_median"] = float(np.median(d.max(axis=0)))
-    # AutoTsit5(Rodas5()) on a sample: dense-LU Rosenbrock at n up to 128
-    S = 4096
-    ws = workloads.brusselator(S, seed=1234)
-    r = device_solve(brusselator, ws["u0"], ws["p"], np.zeros(S), ws["tf"], ws["nvec"],
-                     AutoTsit5(), EnsembleMI355X(schedule="persistent"), abstol=1e-8, reltol=1e-6)
-    r = device_solve(brusselator, ws["u0"], ws["p"], np.zeros(S), ws["tf"], ws["nvec"],
-                     AutoTsit5(), EnsembleMI355X(schedule="persistent"), abstol=1e-8, reltol=1e-6)
-    res["auto_sample"] = dict(instances=S, kernel_ms=r.kernel_ms,
-                              instances_per_s=S / (r.kernel_ms * 1e-3),
-                              stiff_step_fraction=float(r.nstiff.sum() / max(1, r.naccept.sum())))
-    return dict(workload="Brusselator uneven batch 131,072 instances, Ngrid U{8..64}, "
-                         "B U[2,4], tf U[1,20], Tsit5, abstol 1e-8 reltol 1e-6",
+        km = statistics.fmean(kms)
+        werr = np.where(rows, np.abs(r.u_final - uref) / (1e-8 + 1e-6 * np.abs(uref)), 0.0)
+        werr = werr.max(axis=0)
+        res[name] = dict(kernel_ms=km, instances_per_s=N / (km * 1e-3), lanes=r.lanes,
+                         ok=float((r.retcode == 1).mean()),
+                         steps_mean=float((r.naccept + r.nreject).mean()),
+                         stiff_step_fraction=float(r.nstiff.sum() / max(1, r.naccept.sum())),
+                         weighted_err_median=float(np.median(werr)),
+                         weighted_err_p99=float(np.quantile(werr, 0.99)),
+                         weighted_err_max=float(werr.max()))
+    return dict(workload=f"Brusselator uneven batch {N:,} instances, Ngrid U{{8..64}}, "
+                         "B U[2,4], tf U[1,20], abstol 1e-8 reltol 1e-6; Tsit5, "
+                         "AutoTsit5(Rodas5()) and Rodas5 over the whole batch",
                 timing="kernel only (inputs copied in beforehand by the host entry point)",
+                error_reference="FP64 Tsit5 reltol 1e-12 abstol 1e-14 (device); weighted error "
+                                "max_c |u_c - u*_c| / (abstol + reltol |u*_c|) per instance",
                 forms=res)
 
 
@@ -606,6 +624,18 @@ def cpu_baseline(sample, ns=100, single_sample=16384):
                                      f"{d1:.2f} s wall")
 
 
+def run_other(torch, ctx, a, name):
+    if name == "robertson":
+        return other_robertson(torch, ctx, max(2, a.steps // 2), 1)
+    if name == "brusselator":
+        only = [f for f in a.bruss_forms.split(",") if f]
+        return other_brusselator(torch, max(1, a.steps // 2), min(1, a.warmup), N=a.bruss_ntraj,
+                                 only=only)
+    if name == "mix":
+        return other_mix(torch)
+    raise SystemExit(f"unknown configuration {name}")
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -620,8 +650,16 @@ def main():
         copy_calibration(torch)
         return
     ctx = _abi.context(0)
-    ens = DeviceEnsemble(torch, a.ntraj)
     forms = [f for f in a.forms.split(",") if f]
+    if not forms:
+        # other configurations only (profiling runs): one JSON line of them
+        other = {}
+        for name in [x for x in a.other.split(",") if x]:
+            print(f"[bench] other config {name}", file=sys.stderr, flush=True)
+            other[name] = run_other(torch, ctx, a, name)
+        print(json.dumps({"other_configs": other}))
+        return
+    ens = DeviceEnsemble(torch, a.ntraj)
     results = {}
     for f in forms:
         print(f"[bench] form {f}: {FORMS[f]}", file=sys.stderr, flush=True)
@@ -661,12 +699,7 @@ def main():
     other = {}
     for name in [x for x in a.other.split(",") if x]:
         print(f"[bench] other config {name}", file=sys.stderr, flush=True)
-        if name == "robertson":
-            other[name] = other_robertson(torch, ctx, max(2, a.steps // 2), 1)
-        elif name == "brusselator":
-            other[name] = other_brusselator(torch, max(2, a.steps // 2), 1)
-        elif name == "mix":
-            other[name] = other_mix(torch)
+        other[name] = run_other(torch, ctx, a, name)
     line = {
         "metric": "trajectories/s at <=1e-6 final-state err vs FP64 oracle, plain vs tuned form",
         "value": h["traj_per_s"], "unit": "trajectories/s", "n_gpus": 1, "steps": a.steps,

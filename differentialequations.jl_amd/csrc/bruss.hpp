@@ -40,6 +40,12 @@ struct BrussArgs {
   unsigned long long* queue;  // next position in `order`
   const int64_t* order;       // work order (NULL: natural order)
   int64_t nqueue;             // queue length (<= ntraj; a sample when autotuning)
+  const double* saveat;       // [ns] ascending save times, or NULL
+  void* us;                   // saved values (double or float), or NULL
+  int32_t ns;
+  int32_t save_f32;           // 1: us holds float
+  int32_t save_layout;        // 0: [ns][nmax][ntraj]; 1: [ntraj][ns][nmax]
+  int32_t pad2;
 };
 
 // Resident teams of the persistent grid for (team shape, alg, precision,
@@ -48,7 +54,7 @@ struct BrussArgs {
 // largest state size a team holds.  fast: the division-free log2-domain PI
 // controller.
 hipError_t bruss_plan(int lanes, int nk, int alg, int mix, int fast, int ncu, int64_t ntraj,
-                      int* grid, int* teams_per_block);
+                      int* grid, int* teams_per_block, bool save = false);
 // out[k] = order[(k * ntraj) / S], k < S (natural order for a NULL order):
 // the autotuner's sample, strided over the launch order.
 hipError_t bruss_sample_order(int64_t* out, const int64_t* order, int64_t ntraj, int64_t S,
@@ -57,5 +63,10 @@ hipError_t bruss_sample_order(int64_t* out, const int64_t* order, int64_t ntraj,
 // have n <= nk (a larger one finishes with RC_INITIALFAILURE).
 hipError_t launch_bruss_solver(const BrussArgs& a, int lanes, int nk, int alg, int mix, int fast,
                                int grid, hipStream_t s);
+// The builds that write saveat points (a.us and a.ns > 0), compiled in their
+// own translation unit: occupancy and launch of one configuration.
+hipError_t bruss_occupancy_save(int lanes, int nk, int alg, int mix, int fast, int* per_cu);
+hipError_t bruss_launch_save(const BrussArgs& a, int lanes, int nk, int alg, int mix, int fast,
+                             int grid, hipStream_t s);
 
 }  // namespace mideq

@@ -34,9 +34,14 @@ struct DevBuf {
 
 enum {
   BUF_U0 = 0, BUF_P, BUF_T0, BUF_TF, BUF_NVEC, BUF_SAVEAT, BUF_DT0, BUF_HINT, BUF_US,
-  BUF_UF, BUF_TFIN, BUF_RC, BUF_NA, BUF_NR, BUF_NF, BUF_NS, BUF_SCRATCH, BUF_ORDER,
-  BUF_SAMPLE, BUF_DTF, BUF_HANDOFF, BUF_NEV, BUF_TEV, BUF_OUT, BUF_RED, BUF_FOLD, BUF_CAP, BUF_COUNT
+  BUF_UF, BUF_TFIN, BUF_RC, BUF_NA, BUF_NR, BUF_NF, BUF_NS, BUF_ORDER,
+  BUF_DTF, BUF_NEV, BUF_TEV, BUF_OUT, BUF_RED, BUF_CAP, BUF_COUNT
 };
+
+// Device memory a solve uses beyond its inputs and outputs: the work-queue
+// counters, the two-pass handoff list, the reduction's block partials and
+// the autotuner's sample order.
+enum { SCR_HANDOFF = 0, SCR_FOLD, SCR_SAMPLE, SCR_COUNT };
 
 // A run-time compiled right-hand side: its generated model source and the
 // kernels built from it so far, one code object per (algorithm, precision,
@@ -58,21 +63,44 @@ struct GraphCache {
   hipGraphExec_t exec = nullptr;
 };
 
+// The scratch of the solves enqueued on one stream: solves in flight on
+// different streams of a context each use their own queue counters, scratch
+// buffers and cached graph.
+struct StreamSlot {
+  hipStream_t stream = nullptr;
+  unsigned long long* queue = nullptr;  // [0] persistent queue, [1] handoff count
+  DevBuf bufs[SCR_COUNT];
+  GraphCache graph;
+  uint64_t last_use = 0;
+};
+constexpr int kMaxSlots = 8;
+
 struct mideq_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   std::string err;
   mideq_stats stats{};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t tune0 = nullptr, tune1 = nullptr;  // autotuner timing
   bool ev_pending = false;
   int ncu = 256;
-  unsigned long long* d_queue = nullptr;
-  DevBuf bufs[BUF_COUNT];
+  DevBuf bufs[BUF_COUNT];  // host entry point's copies of the caller's arrays
+  std::vector<StreamSlot> slots;
+  uint64_t slot_clock = 0;
   // autotuned lanes per instance, keyed by (alg, precision, controller, nmax)
   std::map<std::tuple<int, int, int, int>, int> tuned_lanes;
-  GraphCache graph;
   std::vector<UserModel> user_models;  // model id MIDEQ_USER_MODEL_BASE + index
   std::string arch;                    // device code target for hipRTC
+};
+
+// What one solve call uses for its scratch: a stream slot's buffers (grown
+// on demand), or a fixed region it may not grow (null slot).
+struct Scratch {
+  unsigned long long* queue = nullptr;
+  StreamSlot* slot = nullptr;
+  char* base = nullptr;  // fixed region past the counters
+  size_t cap = 0, used = 0;
+  GraphCache* graph = nullptr;  // null: no graph replay
 };
 
 static int set_err(mideq_ctx* c, int code, const char* fmt, ...) {
@@ -100,8 +128,9 @@ static int ensure_buf(mideq_ctx* c, int slot, size_t bytes, void** out) {
   }
   if (b.size < bytes) {
     if (b.ptr) {
-      // the previous contents may still be in use by queued work
-      hipError_t e = hipDeviceSynchronize();
+      // the previous contents may still be in use by work queued on the
+      // context's stream (the only stream these buffers are used on)
+      hipError_t e = hipStreamSynchronize(c->stream);
       e = hipFree(b.ptr);
       (void)e;
       b.ptr = nullptr;
@@ -116,13 +145,98 @@ static int ensure_buf(mideq_ctx* c, int slot, size_t bytes, void** out) {
   return 0;
 }
 
-static void drop_graph(mideq_ctx* c) {
-  if (c->graph.exec) {
-    hipError_t e = hipGraphExecDestroy(c->graph.exec);
+static void drop_graph(GraphCache* g) {
+  if (g->exec) {
+    hipError_t e = hipGraphExecDestroy(g->exec);
     (void)e;
   }
-  c->graph.exec = nullptr;
-  c->graph.key.clear();
+  g->exec = nullptr;
+  g->key.clear();
+}
+
+static void free_slot(StreamSlot& sl) {
+  hipError_t e = hipStreamSynchronize(sl.stream);
+  drop_graph(&sl.graph);
+  for (auto& b : sl.bufs)
+    if (b.ptr) e = hipFree(b.ptr);
+  if (sl.queue) e = hipFree(sl.queue);
+  (void)e;
+  sl = StreamSlot{};
+}
+
+// The slot of stream s: found, else a new one (the least recently used one
+// is recycled past kMaxSlots streams, after its stream has drained).
+static int slot_for(mideq_ctx* c, hipStream_t s, StreamSlot** out) {
+  StreamSlot* lru = nullptr;
+  for (auto& sl : c->slots) {
+    if (sl.stream == s) {
+      sl.last_use = ++c->slot_clock;
+      *out = &sl;
+      return MIDEQ_OK;
+    }
+    if (!lru || sl.last_use < lru->last_use) lru = &sl;
+  }
+  StreamSlot* sl;
+  if ((int)c->slots.size() < kMaxSlots) {
+    c->slots.emplace_back();
+    sl = &c->slots.back();
+  } else {
+    free_slot(*lru);
+    sl = lru;
+  }
+  hipError_t e = hipMalloc((void**)&sl->queue, 256);
+  if (e != hipSuccess) {
+    sl->queue = nullptr;
+    return set_err(c, MIDEQ_E_NOMEM, "hipMalloc(256) failed: %s", hipGetErrorString(e));
+  }
+  sl->stream = s;
+  sl->last_use = ++c->slot_clock;
+  *out = sl;
+  return MIDEQ_OK;
+}
+
+// A scratch buffer of `bytes` for this call: from the slot (grown after its
+// stream has drained), or carved from the fixed region (256-byte aligned).
+static int scratch_get(mideq_ctx* c, Scratch& S, int kind, size_t bytes, void** out) {
+  *out = nullptr;
+  if (bytes == 0) return MIDEQ_OK;
+  if (!S.slot) {
+    size_t at = (S.used + 255) & ~(size_t)255;
+    if (at > S.cap || bytes > S.cap - at)
+      return set_err(c, MIDEQ_E_ARG,
+                     "the solve needs %zu bytes of scratch beyond the %zu-byte fixed region",
+                     at + bytes, S.cap);
+    *out = S.base + at;
+    S.used = at + bytes;
+    return MIDEQ_OK;
+  }
+  DevBuf& b = S.slot->bufs[kind];
+  if (b.size < bytes) {
+    if (b.ptr) {
+      hipError_t e = hipStreamSynchronize(S.slot->stream);
+      e = hipFree(b.ptr);
+      (void)e;
+      b.ptr = nullptr;
+      b.size = 0;
+    }
+    hipError_t e = hipMalloc(&b.ptr, bytes);
+    if (e != hipSuccess)
+      return set_err(c, MIDEQ_E_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    b.size = bytes;
+  }
+  *out = b.ptr;
+  return MIDEQ_OK;
+}
+
+static int scratch_for_stream(mideq_ctx* c, hipStream_t s, Scratch* S) {
+  StreamSlot* sl = nullptr;
+  int rc = slot_for(c, s, &sl);
+  if (rc) return rc;
+  *S = Scratch{};
+  S->queue = sl->queue;
+  S->slot = sl;
+  S->graph = &sl->graph;
+  return MIDEQ_OK;
 }
 
 extern "C" int mideq_version(void) { return MIDEQ_VERSION; }
@@ -149,10 +263,11 @@ extern "C" int mideq_ctx_create(mideq_ctx** out, int32_t device) {
   }
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
-      hipMalloc((void**)&c->d_queue, 256) != hipSuccess) {
-    delete c;
+      hipEventCreate(&c->tune0) != hipSuccess || hipEventCreate(&c->tune1) != hipSuccess) {
+    mideq_ctx_destroy(c);
     return MIDEQ_E_HIP;
   }
+
   *out = c;
   return MIDEQ_OK;
 }
@@ -160,16 +275,19 @@ extern "C" int mideq_ctx_create(mideq_ctx** out, int32_t device) {
 extern "C" void mideq_ctx_destroy(mideq_ctx* c) {
   if (!c) return;
   hipError_t e = hipSetDevice(c->device);
-  e = hipDeviceSynchronize();
-  drop_graph(c);
+  // only the context's own work is waited for: its stream and the streams
+  // its device-entry solves were enqueued on
+  for (auto& sl : c->slots) free_slot(sl);
+  if (c->stream) e = hipStreamSynchronize(c->stream);
   for (auto& b : c->bufs)
     if (b.ptr) e = hipFree(b.ptr);
   for (auto& um : c->user_models)
     for (auto& kv : um.kernels)
       if (kv.second.mod) e = hipModuleUnload(kv.second.mod);
-  if (c->d_queue) e = hipFree(c->d_queue);
   if (c->ev0) e = hipEventDestroy(c->ev0);
   if (c->ev1) e = hipEventDestroy(c->ev1);
+  if (c->tune0) e = hipEventDestroy(c->tune0);
+  if (c->tune1) e = hipEventDestroy(c->tune1);
   if (c->stream) e = hipStreamDestroy(c->stream);
   (void)e;
   delete c;
@@ -415,10 +533,13 @@ static int rtc_compile(const std::string& model_src, const std::string& arch, in
 }
 
 static int user_kernels(mideq_ctx* c, UserModel* um, int alg, int mix, int fast,
-                        UserKernels** out) {
+                        UserKernels** out, bool may_build = true) {
   int key = (alg << 2) | (mix << 1) | fast;
   auto it = um->kernels.find(key);
   if (it == um->kernels.end()) {
+    if (!may_build)
+      return set_err(c, MIDEQ_E_ARG,
+                     "the user model's kernels for this configuration are not built yet");
     std::vector<char> code;
     std::string log;
     int rc = rtc_compile(um->model_src, c->arch, alg, mix, fast, &code, &log);
@@ -536,7 +657,6 @@ static int validate(mideq_ctx* c, const mideq_problem* P, const mideq_options* O
     if (P->n < BRUSS_NMIN || P->n > BRUSS_NMAX || (P->n & 1))
       return set_err(c, MIDEQ_E_ARG, "brusselator n=%d must be even in [%d,%d]", P->n,
                      BRUSS_NMIN, BRUSS_NMAX);
-    if (O->ns > 0) return set_err(c, MIDEQ_E_UNSUPPORTED, "saveat not provided for the brusselator");
     if (O->lanes != 0 && O->lanes != 32 && O->lanes != 64)
       return set_err(c, MIDEQ_E_ARG, "lanes must be 0, 32 or 64");
   } else if (P->n != n) {
@@ -556,6 +676,11 @@ static int validate(mideq_ctx* c, const mideq_problem* P, const mideq_options* O
     return set_err(c, MIDEQ_E_UNSUPPORTED, "saveat needs n * ntraj < 2^32 (got %lld); split the "
                    "ensemble", (long long)((int64_t)P->n * P->ntraj));
   if (O->save_dtype < 0 || O->save_dtype > 1) return set_err(c, MIDEQ_E_ARG, "bad save_dtype");
+  if (O->save_layout < 0 || O->save_layout > 1) return set_err(c, MIDEQ_E_ARG, "bad save_layout");
+  if (O->save_layout == 1 && P->model != MODEL_BRUSSELATOR)
+    return set_err(c, MIDEQ_E_UNSUPPORTED,
+                   "save_layout 1 (trajectory-major) is provided by the team kernel (brusselator) "
+                   "only");
   if (O->schedule < 0 || O->schedule > 2) return set_err(c, MIDEQ_E_ARG, "bad schedule");
   if (O->controller < 0 || O->controller > 1) return set_err(c, MIDEQ_E_ARG, "bad controller");
   if (O->auto_passes < 0 || O->auto_passes > 3) return set_err(c, MIDEQ_E_ARG, "bad auto_passes");
@@ -579,9 +704,11 @@ static int check_saveat_host(mideq_ctx* c, const double* sa, int ns) {
 // Enqueue a launch sequence, optionally through a cached hipGraph whose key
 // is the exact argument bytes of the sequence plus the stream.
 template <class F>
-static int enqueue(mideq_ctx* c, bool use_graph, const void* key, size_t keylen,
+static int enqueue(mideq_ctx* c, GraphCache* gc, bool use_graph, const void* key, size_t keylen,
                    hipStream_t s, F&& body) {
   c->stats.graph = 0;
+  if (use_graph && !gc)
+    return set_err(c, MIDEQ_E_ARG, "graph replay is not available on this entry point");
   if (!use_graph) {
     hipError_t e = body(s);
     if (e != hipSuccess) return set_err(c, MIDEQ_E_HIP, "kernel launch: %s", hipGetErrorString(e));
@@ -591,8 +718,8 @@ static int enqueue(mideq_ctx* c, bool use_graph, const void* key, size_t keylen,
   std::vector<unsigned char> k(kb, kb + keylen);
   const unsigned char* sb = (const unsigned char*)&s;
   k.insert(k.end(), sb, sb + sizeof(s));
-  if (!c->graph.exec || c->graph.key != k) {
-    drop_graph(c);
+  if (!gc->exec || gc->key != k) {
+    drop_graph(gc);
     hipGraph_t g = nullptr;
     HIP_TRY(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     hipError_t e = body(s);
@@ -606,10 +733,10 @@ static int enqueue(mideq_ctx* c, bool use_graph, const void* key, size_t keylen,
     e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (e != hipSuccess) return set_err(c, MIDEQ_E_HIP, "graph instantiate: %s", hipGetErrorString(e));
-    c->graph.exec = ex;
-    c->graph.key = k;
+    gc->exec = ex;
+    gc->key = k;
   }
-  HIP_TRY(c, hipGraphLaunch(c->graph.exec, s));
+  HIP_TRY(c, hipGraphLaunch(gc->exec, s));
   c->stats.graph = 1;
   return MIDEQ_OK;
 }
@@ -621,24 +748,26 @@ static int bruss_nk(int lanes, int nmax) {
   return (lanes == 32 && nmax <= BRUSS_NMAX / 2) ? BRUSS_NMAX / 2 : BRUSS_NMAX;
 }
 
-static int bruss_launch_with(mideq_ctx* c, BrussArgs b, int lanes, int alg, int mix, int fast,
-                             bool use_graph, hipStream_t s, int* grid_used) {
+static int bruss_launch_with(mideq_ctx* c, GraphCache* gc, BrussArgs b, int lanes, int alg,
+                             int mix, int fast, bool use_graph, hipStream_t s, int* grid_used) {
   int grid = 0, tpb = 0;
   const int nk = bruss_nk(lanes, b.nmax);
-  hipError_t e = bruss_plan(lanes, nk, alg, mix, fast, c->ncu, b.nqueue, &grid, &tpb);
+  const bool save = b.us && b.ns > 0;
+  hipError_t e = bruss_plan(lanes, nk, alg, mix, fast, c->ncu, b.nqueue, &grid, &tpb, save);
   if (e != hipSuccess) return set_err(c, MIDEQ_E_HIP, "bruss_plan: %s", hipGetErrorString(e));
   *grid_used = grid;
   struct Key {
     BrussArgs b;
     int lanes, alg, mix, fast, grid;
   } key{b, lanes, alg, mix, fast, grid};
-  return enqueue(c, use_graph, &key, sizeof key, s, [&](hipStream_t st) {
+  return enqueue(c, gc, use_graph, &key, sizeof key, s, [&](hipStream_t st) {
     return launch_bruss_solver(b, lanes, nk, alg, mix, fast, grid, st);
   });
 }
 
 static int run_bruss(mideq_ctx* c, const mideq_problem* P, const mideq_options* O,
-                     const mideq_outputs* out, const int64_t* d_order, hipStream_t s) {
+                     const mideq_outputs* out, const int64_t* d_order, hipStream_t s,
+                     Scratch& Sc) {
   BrussArgs b{};
   b.ntraj = P->ntraj;
   b.nqueue = P->ntraj;
@@ -667,8 +796,15 @@ static int run_bruss(mideq_ctx* c, const mideq_problem* P, const mideq_options* 
   b.nf = out->nf;
   b.nstiff = out->nstiff;
   b.dt_first = out->dt_first;
-  b.queue = c->d_queue;
+  b.queue = Sc.queue;
   b.order = d_order;
+  if (O->ns > 0 && out->us) {
+    b.saveat = O->saveat;
+    b.us = out->us;
+    b.ns = O->ns;
+    b.save_f32 = O->save_dtype;
+    b.save_layout = O->save_layout;
+  }
   int lanes = O->lanes;
   int grid = 0;
   if (lanes == 0) {
@@ -678,6 +814,9 @@ static int run_bruss(mideq_ctx* c, const mideq_problem* P, const mideq_options* 
     auto it = c->tuned_lanes.find(key);
     if (it != c->tuned_lanes.end()) {
       lanes = it->second;
+    } else if (!Sc.slot) {
+      // fixed scratch (no allocation, no tuning run): 64-lane teams
+      lanes = 64;
     } else {
       // The sample is strided over the launch order and, for Tsit5, holds
       // ~8 instances per resident team: with fewer, the run is its tail
@@ -690,14 +829,14 @@ static int run_bruss(mideq_ctx* c, const mideq_problem* P, const mideq_options* 
         for (int cand : {64, 32}) {
           int g = 0, tpb = 0;
           if (bruss_plan(cand, bruss_nk(cand, P->n), O->alg, O->precision, O->controller,
-                         c->ncu, INT64_MAX / 2, &g, &tpb) == hipSuccess)
+                         c->ncu, INT64_MAX / 2, &g, &tpb, b.us != nullptr) == hipSuccess)
             resident = std::max<int64_t>(resident, (int64_t)g * tpb);
         }
         S = std::max<int64_t>(8192, 8 * resident);
       }
       S = std::min<int64_t>(S, P->ntraj);
       void* d;
-      int rc = ensure_buf(c, BUF_SAMPLE, sizeof(int64_t) * S, &d);
+      int rc = scratch_get(c, Sc, SCR_SAMPLE, sizeof(int64_t) * S, &d);
       if (rc) return rc;
       {
         hipError_t e = bruss_sample_order((int64_t*)d, d_order, P->ntraj, S, s);
@@ -709,36 +848,31 @@ static int run_bruss(mideq_ctx* c, const mideq_problem* P, const mideq_options* 
       bs.nqueue = S;
       float best = 1e30f;
       int best_l = 64;
-      hipEvent_t a0, a1;
-      HIP_TRY(c, hipEventCreate(&a0));
-      HIP_TRY(c, hipEventCreate(&a1));
       for (int cand : {64, 32}) {
         int g = 0;
-        if ((rc = bruss_launch_with(c, bs, cand, O->alg, O->precision, O->controller, false, s,
-                                    &g)))
+        if ((rc = bruss_launch_with(c, nullptr, bs, cand, O->alg, O->precision, O->controller,
+                                    false, s, &g)))
           return rc;
-        HIP_TRY(c, hipEventRecord(a0, s));
-        if ((rc = bruss_launch_with(c, bs, cand, O->alg, O->precision, O->controller, false, s,
-                                    &g)))
+        HIP_TRY(c, hipEventRecord(c->tune0, s));
+        if ((rc = bruss_launch_with(c, nullptr, bs, cand, O->alg, O->precision, O->controller,
+                                    false, s, &g)))
           return rc;
-        HIP_TRY(c, hipEventRecord(a1, s));
-        HIP_TRY(c, hipEventSynchronize(a1));
+        HIP_TRY(c, hipEventRecord(c->tune1, s));
+        HIP_TRY(c, hipEventSynchronize(c->tune1));
         float ms = 0.f;
-        HIP_TRY(c, hipEventElapsedTime(&ms, a0, a1));
+        HIP_TRY(c, hipEventElapsedTime(&ms, c->tune0, c->tune1));
         if (ms < best) {
           best = ms;
           best_l = cand;
         }
       }
-      HIP_TRY(c, hipEventDestroy(a0));
-      HIP_TRY(c, hipEventDestroy(a1));
       lanes = best_l;
       c->tuned_lanes[key] = lanes;
     }
   }
   HIP_TRY(c, hipEventRecord(c->ev0, s));
-  int rc = bruss_launch_with(c, b, lanes, O->alg, O->precision, O->controller, O->graph != 0, s,
-                             &grid);
+  int rc = bruss_launch_with(c, Sc.graph, b, lanes, O->alg, O->precision, O->controller,
+                             O->graph != 0, s, &grid);
   if (rc) return rc;
   HIP_TRY(c, hipEventRecord(c->ev1, s));
   c->ev_pending = true;
@@ -752,11 +886,12 @@ static int run_bruss(mideq_ctx* c, const mideq_problem* P, const mideq_options* 
 
 // ---------------------------------------------------------------- launch
 static int run_solver(mideq_ctx* c, const mideq_problem* P, const mideq_options* O,
-                      const mideq_outputs* out, const int64_t* d_order, hipStream_t s) {
+                      const mideq_outputs* out, const int64_t* d_order, hipStream_t s,
+                      Scratch& Sc) {
   c->stats = mideq_stats{};
   c->ev_pending = false;
   if (P->ntraj == 0) return MIDEQ_OK;
-  if (P->model == MODEL_BRUSSELATOR) return run_bruss(c, P, O, out, d_order, s);
+  if (P->model == MODEL_BRUSSELATOR) return run_bruss(c, P, O, out, d_order, s, Sc);
   LaneArgs a{};
   a.ntraj = P->ntraj;
   a.u0 = P->u0;
@@ -789,7 +924,7 @@ static int run_solver(mideq_ctx* c, const mideq_problem* P, const mideq_options*
   a.nevent = out->nevent;
   a.t_event = out->t_event;
   a.out = out->out;
-  a.queue = c->d_queue;
+  a.queue = Sc.queue;
   a.order = d_order;
   int sched = O->schedule == MIDEQ_SCHED_AUTO ? SCHED_LANE : O->schedule;
   int block = O->block > 0 ? O->block : 256;
@@ -817,10 +952,10 @@ static int run_solver(mideq_ctx* c, const mideq_problem* P, const mideq_options*
   a.iter_cap = O->auto_passes == 3 ? O->iter_cap : nullptr;
   if (two_pass) {
     void* d;
-    int rc0 = ensure_buf(c, BUF_HANDOFF, sizeof(int64_t) * (size_t)P->ntraj, &d);
+    int rc0 = scratch_get(c, Sc, SCR_HANDOFF, sizeof(int64_t) * (size_t)P->ntraj, &d);
     if (rc0) return rc0;
     a.handoff = (int64_t*)d;
-    a.handoff_count = c->d_queue + 1;  // d_queue[0] is the persistent queue
+    a.handoff_count = Sc.queue + 1;  // queue[0] is the persistent queue
   }
   HIP_TRY(c, hipEventRecord(c->ev0, s));
   struct Key {
@@ -833,10 +968,11 @@ static int run_solver(mideq_ctx* c, const mideq_problem* P, const mideq_options*
   UserKernels* uk_main = nullptr;
   UserKernels* uk_first = nullptr;
   if (um) {
+    const bool may_build = Sc.slot != nullptr;
     int rc0 = user_kernels(c, um, lane_alg_code(a, O->alg, um->ncb > 0), O->precision,
-                           O->controller, &uk_main);
+                           O->controller, &uk_main, may_build);
     if (rc0 == MIDEQ_OK && two_pass)
-      rc0 = user_kernels(c, um, first_alg, O->precision, O->controller, &uk_first);
+      rc0 = user_kernels(c, um, first_alg, O->precision, O->controller, &uk_first, may_build);
     if (rc0) return rc0;
   }
   // on-device reduction of the output values: block partials in scratch
@@ -846,7 +982,7 @@ static int run_solver(mideq_ctx* c, const mideq_problem* P, const mideq_options*
     if (!out->out || !out->reduced)
       return set_err(c, MIDEQ_E_ARG, "a reduction needs the out and reduced buffers");
     void* d;
-    int rc0 = ensure_buf(c, BUF_FOLD,
+    int rc0 = scratch_get(c, Sc, SCR_FOLD,
                          sizeof(double) * (size_t)um->nout * (size_t)((P->ntraj + 255) / 256),
                          &d);
     if (rc0) return rc0;
@@ -864,7 +1000,7 @@ static int run_solver(mideq_ctx* c, const mideq_problem* P, const mideq_options*
     return launch_fold(out->out, P->ntraj, um->nout, O->reduction, fold_partial, out->reduced,
                        st);
   };
-  int rc = enqueue(c, O->graph != 0, &key, sizeof key, s, [&](hipStream_t st) {
+  int rc = enqueue(c, Sc.graph, O->graph != 0, &key, sizeof key, s, [&](hipStream_t st) {
     if (!two_pass) {
       LaneArgs a1 = a;
       a1.handoff = nullptr;
@@ -902,7 +1038,9 @@ extern "C" int mideq_ensemble_solve_device(mideq_ctx* c, const mideq_problem* P,
   if (!out) return set_err(c, MIDEQ_E_ARG, "null outputs");
   HIP_TRY(c, hipSetDevice(c->device));
   hipStream_t s = stream ? (hipStream_t)stream : c->stream;
-  return run_solver(c, P, O, out, O->order, s);
+  Scratch Sc;
+  if ((rc = scratch_for_stream(c, s, &Sc))) return rc;
+  return run_solver(c, P, O, out, O->order, s, Sc);
 }
 
 // A launch order must be a permutation of 0..N-1.
@@ -1014,6 +1152,9 @@ extern "C" int mideq_ensemble_solve(mideq_ctx* c, const mideq_problem* P,
   size_t us_bytes = (size_t)O->ns * n * N * (O->save_dtype ? 4 : 8);
   if (out->us && O->ns > 0) {
     if ((rc = ensure_buf(c, BUF_US, us_bytes, &dOut.us))) return rc;
+    // rows past a ragged instance's size are not written by the kernel
+    if (P->nvec)
+      HIP_TRY(c, hipMemcpyAsync(dOut.us, out->us, us_bytes, hipMemcpyHostToDevice, s));
   }
   if (out->u_final) {
     if ((rc = ensure_buf(c, BUF_UF, sizeof(double) * (size_t)n * N, (void**)&dOut.u_final))) return rc;
@@ -1042,7 +1183,9 @@ extern "C" int mideq_ensemble_solve(mideq_ctx* c, const mideq_problem* P,
     if ((rc = ensure_buf(c, BUF_RED, sizeof(double) * nout, (void**)&dOut.reduced))) return rc;
   }
 
-  rc = run_solver(c, &dP, &dO, &dOut, d_order, s);
+  Scratch Sc;
+  if ((rc = scratch_for_stream(c, s, &Sc))) return rc;
+  rc = run_solver(c, &dP, &dO, &dOut, d_order, s, Sc);
   if (rc) return rc;
   if (out->us && O->ns > 0)
     HIP_TRY(c, hipMemcpyAsync(out->us, dOut.us, us_bytes, hipMemcpyDeviceToHost, s));

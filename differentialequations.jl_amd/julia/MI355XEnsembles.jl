@@ -63,7 +63,7 @@ struct MideqOptions
     qsteady_default::Int32; schedule::Int32; block::Int32; blocks_per_cu::Int32
     lanes::Int32; graph::Int32; controller::Int32; auto_passes::Int32
     order::Ptr{Int64}
-    reduction::Int32; reserved2::Int32
+    reduction::Int32; save_layout::Int32
     iter_cap::Ptr{Int32}
 end
 

@@ -11,7 +11,8 @@ import re
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 PKG_ROOT = os.path.dirname(_HERE)
-LIB_PATH = os.path.join(PKG_ROOT, "_build", "libmideq.so")
+# MIDEQ_LIB selects another build of the library (the host-sanitizer build)
+LIB_PATH = os.environ.get("MIDEQ_LIB") or os.path.join(PKG_ROOT, "_build", "libmideq.so")
 HEADER = os.path.join(os.path.dirname(PKG_ROOT), "include", "mideq.h")
 
 MODEL = dict(exp=0, lorenz=1, robertson=2, vdp=3, brusselator=4, linear=5)
@@ -59,7 +60,7 @@ class Options(C.Structure):
                 ("schedule", C.c_int32), ("block", C.c_int32), ("blocks_per_cu", C.c_int32),
                 ("lanes", C.c_int32), ("graph", C.c_int32), ("controller", C.c_int32),
                 ("auto_passes", C.c_int32), ("order", C.c_void_p),
-                ("reduction", C.c_int32), ("reserved2", C.c_int32), ("iter_cap", C.c_void_p)]
+                ("reduction", C.c_int32), ("save_layout", C.c_int32), ("iter_cap", C.c_void_p)]
 
 
 class Outputs(C.Structure):

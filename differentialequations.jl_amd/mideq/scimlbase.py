@@ -374,11 +374,14 @@ class EnsembleMI355X:
     that switch (faster when few switch).  Both give identical results.
     3 = trust alg_hint: plain Tsit5 for the trajectories hinted non-stiff,
     each capped at iter_cap[i] attempted steps, then AutoSwitch from t0 for
-    the capped-out and the hinted-stiff ones."""
+    the capped-out and the hinted-stiff ones.  save_layout: "soa" (saved
+    values [ns, n, N], every model) or "trajectory" ([N, ns, n], one
+    trajectory's saved states contiguous; the Brusselator team kernel)."""
     precision: str = "fp64"
     schedule: str = "auto"
     device: int = 0
     save_dtype: str = "float64"
+    save_layout: str = "soa"
     dt0: Any = None
     alg_hint: Any = None
     order: Any = None
@@ -422,7 +425,7 @@ class EnsembleArrays:
     nstiff: np.ndarray
     dt_first: np.ndarray          # [N] first accepted step
     saveat: Optional[np.ndarray]  # [ns]
-    us: Optional[np.ndarray]      # [ns, n, N]
+    us: Optional[np.ndarray]      # [ns, n, N], or [N, ns, n] (save_layout "trajectory")
     t0: np.ndarray
     u0: np.ndarray
     kernel_ms: float = 0.0
@@ -433,20 +436,28 @@ class EnsembleArrays:
     t_event: Optional[np.ndarray] = None  # [N] time of the last one (NaN if none)
     out: Optional[np.ndarray] = None      # [nout, N] on-device output_func values
     reduced: Optional[np.ndarray] = None  # [nout] their on-device reduction
+    save_layout: str = "soa"
+    nvec: Optional[np.ndarray] = None     # [N] state sizes of a ragged batch
+
+    def saved(self, i):
+        """Saved states of trajectory i (0-based), [ns, n_i]."""
+        vals = self.us[i] if self.save_layout == "trajectory" else self.us[:, :, i]
+        return vals[:, :int(self.nvec[i])] if self.nvec is not None else vals
 
     def solution(self, i, prob=None):
         """ODESolution of trajectory i (0-based)."""
         st = Stats(int(self.naccept[i]), int(self.nreject[i]), int(self.nf[i]))
         rc = ReturnCode(int(self.retcode[i]))
         if self.us is not None:
-            vals = self.us[:, :, i]
+            vals = self.saved(i)
             ok = ~np.isnan(vals).any(axis=1)
             ts, us = self.saveat[ok], vals[ok]
             if self.t_final is not None and self.t_final[i] < self.t0[i]:
                 ts, us = ts[::-1], us[::-1]  # a backward solve: in the order of integration
             return ODESolution(ts.copy(), [v.copy() for v in us], rc, st, prob)
+        m = int(self.nvec[i]) if self.nvec is not None else self.u0.shape[0]
         return ODESolution(np.array([self.t0[i], self.t_final[i]]),
-                           [self.u0[:, i].copy(), self.u_final[:, i].copy()], rc, st, prob)
+                           [self.u0[:m, i].copy(), self.u_final[:m, i].copy()], rc, st, prob)
 
 
 @dataclass
@@ -653,7 +664,10 @@ def device_solve(f, u0, p, t0, tf, nvec, alg, ealg, saveat=None, abstol=1e-6, re
     if saveat is not None and len(saveat):
         O.saveat = _ptr(saveat)
         O.ns = len(saveat)
-        us = np.empty((len(saveat), n, N), dtype=np.float32 if f32 else np.float64)
+        O.save_layout = dict(soa=0, trajectory=1)[ealg.save_layout]
+        shape = (N, len(saveat), n) if ealg.save_layout == "trajectory" else (len(saveat), n, N)
+        # rows past a ragged instance's size are left as they are: NaN
+        us = np.full(shape, np.nan, dtype=np.float32 if f32 else np.float64)
     dt0 = dt0 if dt0 is not None else ealg.dt0
     hint = alg_hint if alg_hint is not None else ealg.alg_hint
     if dt0 is not None:
@@ -693,7 +707,8 @@ def device_solve(f, u0, p, t0, tf, nvec, alg, ealg, saveat=None, abstol=1e-6, re
     ctx.solve(P, O, out)
     st = ctx.stats()
     return EnsembleArrays(saveat=saveat, us=us, t0=t0, u0=u0, kernel_ms=st.kernel_ms,
-                          total_ms=st.total_ms, lanes=st.lanes, graph=bool(st.graph), **res)
+                          total_ms=st.total_ms, lanes=st.lanes, graph=bool(st.graph),
+                          save_layout=ealg.save_layout, nvec=nvec, **res)
 
 
 def _subset_ealg(ealg, pending):

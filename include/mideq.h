@@ -22,7 +22,11 @@
  *   u_final[c*ntraj + i].
  * Ownership: the caller owns every buffer; nothing is retained after return.
  * Threading: a context is used by one thread at a time; separate contexts
- * may run concurrently.  Errors: negative MIDEQ_E_* return values with a
+ * may run concurrently.  Device-entry solves of one context may be in flight
+ * on several streams at once: each stream gets its own work-queue counters,
+ * scratch buffers and cached graph (up to 8 streams per context; past that
+ * the least recently used stream's are reused after that stream drains).
+ * mideq_last_stats reports the last call's launch.  Errors: negative MIDEQ_E_* return values with a
  * message from mideq_last_error(); per-trajectory failures are not call
  * errors and go to retcode[i] (mapped 1:1 to SciMLBase.ReturnCode).
  */
@@ -154,7 +158,10 @@ typedef struct {
                            on the device entry, entries outside [0, ntraj) are skipped.
                            Overrides the built-in cost order of the Brusselator queue. */
   int32_t reduction;    /* mideq_reduction over the trajectories' output_func values */
-  int32_t reserved2;
+  int32_t save_layout;  /* 0: us[(k*n + c)*ntraj + i] (trajectory fastest, every model);
+                           1: us[(i*ns + k)*n + c] (component fastest: one trajectory's saved
+                           states contiguous, as Julia's per-trajectory sol.u; the team
+                           kernel's stores are then contiguous) -- brusselator only */
   const int32_t *iter_cap; /* [ntraj] auto_passes = 3: iteration cap of the plain Tsit5 pass
                               per trajectory (e.g. a multiple of the predicted cost; 0
                               sends it to AutoSwitch before its first step, starting in

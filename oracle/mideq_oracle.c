@@ -1174,7 +1174,7 @@ static __thread int32_t g_trace_cap = 0, g_trace_len = 0;
 
 static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
                      const oracle_options *o, int64_t idx, double *save, long save_stride,
-                     oracle_result *res, rodas_ws *ws) {
+                     long save_rows, oracle_result *res, rodas_ws *ws) {
   int n = m->n;
   int mixed = o->precision == ORACLE_PREC_MIXED;
   double uprev[NMAX], u1[NMAX];
@@ -1214,12 +1214,12 @@ static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
   /* save points before t0 are not reachable; the point at t0 is u0 itself */
   while (SAVE_LEFT && (back ? sa[isave] > t0 : sa[isave] < t0)) {
     if (save)
-      for (int c = 0; c < n; c++) save[((long)isave * n + c) * save_stride] = NAN;
+      for (int c = 0; c < n; c++) save[((long)isave * save_rows + c) * save_stride] = NAN;
     SAVE_NEXT;
   }
   while (SAVE_LEFT && sa[isave] == t0) {
     if (save)
-      for (int c = 0; c < n; c++) save[((long)isave * n + c) * save_stride] = u0[c];
+      for (int c = 0; c < n; c++) save[((long)isave * save_rows + c) * save_stride] = u0[c];
     SAVE_NEXT;
   }
 
@@ -1472,7 +1472,7 @@ static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
             double th = (sa[isave] - tprev) / sdt;
             step_interp(n, mode, mixed, rtab, uprev, u1, k64, k32, kdsave, sdt, th, val);
           }
-          for (int c = 0; c < n; c++) save[((long)isave * n + c) * save_stride] = val[c];
+          for (int c = 0; c < n; c++) save[((long)isave * save_rows + c) * save_stride] = val[c];
         }
         SAVE_NEXT;
       }
@@ -1538,7 +1538,7 @@ static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
   if (rc == ORACLE_RC_DEFAULT) rc = ORACLE_RC_SUCCESS;
   for (; SAVE_LEFT; SAVE_NEXT)
     if (save)
-      for (int c = 0; c < n; c++) save[((long)isave * n + c) * save_stride] = NAN;
+      for (int c = 0; c < n; c++) save[((long)isave * save_rows + c) * save_stride] = NAN;
 #undef SAVE_LEFT
 #undef SAVE_NEXT
   for (int c = 0; c < n; c++) res->u[c] = uprev[c];
@@ -1556,7 +1556,7 @@ int oracle_solve(int32_t model, int32_t n, const double *u0, const double *p,
   mdl_init(&m, model, n, p, np);
   rodas_ws *ws = (rodas_ws *)malloc(sizeof(rodas_ws));
   if (!ws) return -2;
-  solve_one(&m, u0, t0, tf, opt, 0, out_save, 1, res, ws);
+  solve_one(&m, u0, t0, tf, opt, 0, out_save, 1, n, res, ws);
   free(ws);
   return 0;
 }
@@ -1618,8 +1618,14 @@ static void *ens_worker(void *arg) {
         for (int j = 0; j < J->np && j < 16; j++) p[j] = J->p[(long long)j * N + i];
         mdl_init(&m, J->model, n, p, J->np);
       }
-      solve_one(&m, u0, J->t0[i], J->tf[i], J->opt, (int64_t)i,
-                J->out_save ? J->out_save + i : NULL, N, &res, ws);
+      /* value (k, c) of trajectory i: SoA at ((k n + c) N + i), trajectory
+       * major at ((i ns + k) n + c), n the layout's rows */
+      const int tmaj = J->opt->save_layout == 1;
+      double *sv = !J->out_save ? NULL
+                   : tmaj ? J->out_save + (long long)i * J->opt->ns * J->n
+                          : J->out_save + i;
+      solve_one(&m, u0, J->t0[i], J->tf[i], J->opt, (int64_t)i, sv, tmaj ? 1 : N, J->n, &res,
+                ws);
       if (J->u_final)
         for (int c = 0; c < n; c++) J->u_final[(long long)c * N + i] = res.u[c];
       if (J->t_final) J->t_final[i] = res.t;

@@ -114,6 +114,9 @@ typedef struct {
                              (ignored when dt > 0) */
   const int8_t *alg_hint; /* [ntraj] initial method of AUTO/AUTO5P (0 Tsit5, 1 Rosenbrock),
                              or NULL */
+  int32_t save_layout;    /* ensemble saves: 0 [ns][n][ntraj] (trajectory fastest),
+                             1 [ntraj][ns][n] (component fastest); n = the layout's rows
+                             (the largest state size of a ragged batch) */
 } oracle_options;
 
 typedef struct {

@@ -23,13 +23,13 @@ def say(msg):
 
 def gpu(model, u0, p, t0, tf, alg="tsit5", precision="fp64", schedule="lane", saveat=None,
         save_dtype="float64", nvec=None, lanes=0, graph=False, controller="reference",
-        auto_passes=0, iter_cap=None, **kw):
+        auto_passes=0, iter_cap=None, save_layout="soa", **kw):
     N = u0.shape[1]
     t0 = np.broadcast_to(np.asarray(t0, dtype=np.float64), (N,)).copy()
     tf = np.broadcast_to(np.asarray(tf, dtype=np.float64), (N,)).copy()
     ealg = EnsembleMI355X(precision=precision, schedule=schedule, save_dtype=save_dtype,
                           lanes=lanes, graph=graph, controller=controller,
-                          auto_passes=auto_passes, iter_cap=iter_cap)
+                          auto_passes=auto_passes, iter_cap=iter_cap, save_layout=save_layout)
     return device_solve(FUNCS[model], np.ascontiguousarray(u0), np.ascontiguousarray(p), t0, tf,
                         nvec, ALGS[alg], ealg, saveat=saveat, **kw)
 

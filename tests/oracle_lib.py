@@ -8,7 +8,9 @@ import os
 import numpy as np
 
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB = os.path.join(REPO, "oracle", "_build", "libmideq_oracle.so")
+# MIDEQ_ORACLE_LIB selects another build of the same source (the sanitizer build)
+LIB = os.environ.get("MIDEQ_ORACLE_LIB") or os.path.join(REPO, "oracle", "_build",
+                                                         "libmideq_oracle.so")
 
 MODEL = dict(exp=0, lorenz=1, robertson=2, vdp=3, brusselator=4, linear=5, prothero=6, user=7)
 ALG = dict(tsit5=0, rodas5=1, auto=2, rodas5p=3, auto5p=4)
@@ -40,6 +42,7 @@ class Options(C.Structure):
         ("norm_lanes", C.c_int32),
         ("dt0_in", _dp),
         ("alg_hint", C.POINTER(C.c_int8)),
+        ("save_layout", C.c_int32),
     ]
 
 
@@ -112,8 +115,9 @@ class Oracle:
     @staticmethod
     def options(alg="tsit5", precision="fp64", abstol=1e-6, reltol=1e-3, dt=0.0, dtmin=0.0,
                 dtmax=0.0, maxiters=0, saveat=None, qsteady_default=1, norm_lanes=64,
-                controller=0, dt0=None, alg_hint=None):
+                controller=0, dt0=None, alg_hint=None, save_layout=0):
         o = Options()
+        o.save_layout = save_layout
         o.alg = ALG[alg]
         o.precision = PREC[precision]
         o.abstol = abstol
@@ -192,7 +196,14 @@ class Oracle:
         t0 = np.ascontiguousarray(np.broadcast_to(t0, (N,)), dtype=np.float64)
         tf = np.ascontiguousarray(np.broadcast_to(tf, (N,)), dtype=np.float64)
         o = self.options(**kw)
-        save = np.zeros((o.ns, n, N)) if o.ns else None
+        # saves: [ns][n][N] (save_layout 0) or [N][ns][n] (1); rows of a
+        # ragged instance past its size stay NaN
+        if not o.ns:
+            save = None
+        elif o.save_layout == 1:
+            save = np.full((N, o.ns, n), np.nan)
+        else:
+            save = np.full((o.ns, n, N), np.nan)
         uf = np.zeros((n, N))
         tfin = np.zeros(N)
         rc = np.zeros(N, dtype=np.int8)

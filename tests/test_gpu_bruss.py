@@ -203,3 +203,75 @@ def test_bruss_tsit5_32_lane_fp64_fast_full_size(oracle):
     assert (c["retcode"] == 1).all()
     assert g.lanes == 32
     assert_identical(g, c, nvec=w["nvec"])
+
+
+def soa(us_traj_major):
+    """[N, ns, n] -> [ns, n, N]"""
+    return np.ascontiguousarray(np.transpose(us_traj_major, (1, 2, 0)))
+
+
+@pytest.mark.parametrize("alg,lanes,precision,layout", [
+    ("tsit5", 64, "fp64", "soa"), ("tsit5", 32, "fp64", "trajectory"),
+    ("tsit5", 32, "mixed", "soa"), ("rodas5", 64, "fp64", "trajectory"),
+    ("rodas5p", 32, "mixed", "trajectory"), ("auto", 64, "fp64", "soa"),
+    ("auto5p", 32, "fp64", "trajectory")])
+def test_bruss_saveat_bitexact(oracle, alg, lanes, precision, layout):
+    """saveat on the team kernel over the ragged batch (n = 16..128): the
+    points each accepted step passes from its dense output (Tsit5's
+    interpolant or the Rosenbrock one), u0 at t0, NaN past an instance's tf,
+    rows past an instance's size untouched; both layouts, bit-exact against
+    the oracle."""
+    rosen = alg != "tsit5"
+    N = 48 if rosen else 160
+    say(f"brusselator saveat {alg} {precision} lanes={lanes} layout={layout} N={N}")
+    w = batch_range(N, seed=80, lo=8, hi=64, tf_max=1.5 if rosen else 4.0)
+    sa = np.array([0.0, 0.05, 0.3, 0.3, 0.77, 1.0, 1.4999, 2.5, 3.9])
+    tol = dict(abstol=1e-8, reltol=1e-6)
+    g = gpu("brusselator", w["u0"], w["p"], 0.0, w["tf"], nvec=w["nvec"], lanes=lanes, alg=alg,
+            precision=precision, schedule="persistent", saveat=sa, save_layout=layout, **tol)
+    c = cpu(oracle, "brusselator", w["u0"], w["p"], 0.0, w["tf"], nvec=w["nvec"],
+            norm_lanes=lanes, alg=alg, precision=precision, saveat=sa, **tol)
+    assert (c["retcode"] == 1).all()
+    us = soa(g.us) if layout == "trajectory" else g.us
+    assert us.shape == c["saveat"].shape
+    np.testing.assert_array_equal(us, c["saveat"])
+    g.us = None
+    assert_identical(g, c, nvec=w["nvec"])
+    np.testing.assert_array_equal(g.nstiff, c["nstiff"])
+    # u0 at t0 exactly, NaN past tf, rows past n NaN (the buffer's own values)
+    n0 = int(w["nvec"][0])
+    assert np.array_equal(us[0, :n0, 0], w["u0"][:n0, 0])
+    late = sa[None, :] > w["tf"][:, None]
+    for k in range(N):
+        n = int(w["nvec"][k])
+        assert np.isnan(us[late[k], :n, k]).all() and not np.isnan(us[~late[k], :n, k]).any()
+        assert np.isnan(us[:, n:, k]).all()
+
+
+def test_bruss_saveat_f32_storage_and_autotuned(oracle):
+    """FP32 storage of the saved values (the FP64 values rounded) under the
+    autotuned team shape, with a hipGraph replay giving the same values."""
+    N = 200
+    say(f"brusselator saveat f32 storage, autotuned, graph N={N}")
+    w = batch(N, seed=81, tf_max=3.0)
+    sa = np.linspace(0.0, 3.0, 13)
+    tol = dict(abstol=1e-8, reltol=1e-6)
+    kw = dict(nvec=w["nvec"], lanes=0, schedule="persistent", saveat=sa, save_dtype="float32",
+              graph=True, **tol)
+    g1 = gpu("brusselator", w["u0"], w["p"], 0.0, w["tf"], **kw)
+    g2 = gpu("brusselator", w["u0"], w["p"], 0.0, w["tf"], **kw)
+    assert g2.graph
+    np.testing.assert_array_equal(g1.us, g2.us)
+    c = cpu(oracle, "brusselator", w["u0"], w["p"], 0.0, w["tf"], nvec=w["nvec"],
+            norm_lanes=g1.lanes, saveat=sa, **tol)
+    np.testing.assert_array_equal(g1.us, c["saveat"].astype(np.float32))
+    g1.us = None
+    assert_identical(g1, c, nvec=w["nvec"])
+
+
+def test_trajectory_layout_refused_for_lane_models():
+    say("save_layout trajectory on a lane model: refused")
+    w = workloads.lorenz(8, seed=82)
+    with pytest.raises(Exception, match="team kernel"):
+        gpu("lorenz", w["u0"], w["p"], 0.0, 1.0, saveat=np.array([0.5]),
+            save_layout="trajectory")

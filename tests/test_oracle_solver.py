@@ -590,3 +590,26 @@ def test_autoswitch_switch_back_pinned_on_dropping_stiffness(oracle, alg):
         decided += 1
         assert got == (1, 1) and r["nswitch"] == 2
     assert decided >= 2
+
+
+def test_ensemble_save_layouts(oracle):
+    """The oracle's two layouts of an ensemble's saved values hold the same
+    numbers ([ns][n][N] and [N][ns][n]); rows past a ragged instance's size
+    stay NaN; each trajectory's saves equal its single solve's."""
+    from mideq import workloads
+
+    w = workloads.brusselator(6, seed=5)
+    tf = np.minimum(w["tf"], 0.5)
+    sa = np.array([0.0, 0.1, 0.25, 0.5, 0.7])
+    kw = dict(abstol=1e-8, reltol=1e-6, saveat=sa)
+    a = oracle.ensemble("brusselator", w["u0"], w["p"], 0.0, tf, nvec=w["nvec"], **kw)
+    b = oracle.ensemble("brusselator", w["u0"], w["p"], 0.0, tf, nvec=w["nvec"], save_layout=1,
+                        **kw)
+    np.testing.assert_array_equal(np.transpose(b["saveat"], (1, 2, 0)), a["saveat"])
+    for k in range(6):
+        n = int(w["nvec"][k])
+        assert np.isnan(a["saveat"][:, n:, k]).all()
+        one = oracle.ensemble("brusselator", w["u0"][:n, k:k + 1], w["p"][:, k:k + 1], 0.0,
+                              tf[k:k + 1], **kw)
+        np.testing.assert_array_equal(a["saveat"][:, :n, k], one["saveat"][:, :, 0])
+        assert np.isnan(a["saveat"][sa > tf[k], :n, k]).all()
