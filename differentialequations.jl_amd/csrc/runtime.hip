@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <tuple>
@@ -228,6 +229,29 @@ static int scratch_get(mideq_ctx* c, Scratch& S, int kind, size_t bytes, void** 
   return MIDEQ_OK;
 }
 
+// ---------------------------------------------------------------- fixed scratch
+// The scratch of mideq_ensemble_solve_device_noalloc: a region of the
+// library's own device data, one copy per GPU, held until the process ends.
+// [0, 256): work-queue counters; then what the call needs of the two-pass
+// handoff list and the reduction's block partials.
+static __device__ __align__(256) unsigned char scratch[MIDEQ_NOALLOC_SCRATCH_BYTES];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices];       // device address of `scratch` per device
+static hipError_t g_scratch_err[kMaxDevices];  // result of looking it up
+
+// With `device` current: look up the device address of `scratch` on it (the
+// array's name is a host-side stand-in; kernels get this address).
+static void scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return;
+  void* ptr = nullptr;
+  hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  g_scratch_err[device] = e;
+  if (e == hipSuccess && ptr) g_scratch_ptr[device] = ptr;
+}
+
 static int scratch_for_stream(mideq_ctx* c, hipStream_t s, Scratch* S) {
   StreamSlot* sl = nullptr;
   int rc = slot_for(c, s, &sl);
@@ -267,7 +291,9 @@ extern "C" int mideq_ctx_create(mideq_ctx** out, int32_t device) {
     mideq_ctx_destroy(c);
     return MIDEQ_E_HIP;
   }
-
+  // the no-alloc entry point's scratch on this device (looked up once per
+  // device; a failure is reported by that entry point)
+  scratch_setup(device);
   *out = c;
   return MIDEQ_OK;
 }
@@ -1040,6 +1066,36 @@ extern "C" int mideq_ensemble_solve_device(mideq_ctx* c, const mideq_problem* P,
   hipStream_t s = stream ? (hipStream_t)stream : c->stream;
   Scratch Sc;
   if ((rc = scratch_for_stream(c, s, &Sc))) return rc;
+  return run_solver(c, P, O, out, O->order, s, Sc);
+}
+
+extern "C" int mideq_ensemble_solve_device_noalloc(mideq_ctx* c, const mideq_problem* P,
+                                                   const mideq_options* O, mideq_outputs* out,
+                                                   void* stream) {
+  if (!c) return MIDEQ_E_ARG;
+  int rc = validate(c, P, O);
+  if (rc) return rc;
+  if (!out) return set_err(c, MIDEQ_E_ARG, "null outputs");
+  if (O->graph)
+    return set_err(c, MIDEQ_E_ARG,
+                   "graph = 1 instantiates a graph, which allocates; capture this call in a "
+                   "graph of the caller's instead");
+  void* base = nullptr;
+  hipError_t se = hipErrorInvalidDevice;
+  if (c->device >= 0 && c->device < kMaxDevices) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    base = g_scratch_ptr[c->device];
+    se = g_scratch_err[c->device];
+  }
+  if (!base)
+    return set_err(c, MIDEQ_E_HIP, "the fixed scratch of device %d is unavailable: %s",
+                   c->device, se == hipSuccess ? "not looked up" : hipGetErrorString(se));
+  HIP_TRY(c, hipSetDevice(c->device));
+  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  Scratch Sc;
+  Sc.queue = (unsigned long long*)base;
+  Sc.base = (char*)base + 256;
+  Sc.cap = MIDEQ_NOALLOC_SCRATCH_BYTES - 256;
   return run_solver(c, P, O, out, O->order, s, Sc);
 }
 

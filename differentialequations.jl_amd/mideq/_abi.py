@@ -122,6 +122,10 @@ def load(path=LIB_PATH):
     L.mideq_ensemble_solve_device.argtypes = [C.c_void_p, C.POINTER(Problem),
                                               C.POINTER(Options), C.POINTER(Outputs),
                                               C.c_void_p]
+    if hasattr(L, "mideq_ensemble_solve_device_noalloc"):
+        L.mideq_ensemble_solve_device_noalloc.argtypes = [C.c_void_p, C.POINTER(Problem),
+                                                          C.POINTER(Options),
+                                                          C.POINTER(Outputs), C.c_void_p]
     _LIB = L
     return L
 
@@ -161,6 +165,11 @@ class Context:
         self.check(self.lib.mideq_ensemble_solve_device(self.h, C.byref(prob), C.byref(opt),
                                                         C.byref(out), stream),
                    "mideq_ensemble_solve_device")
+
+    def solve_device_noalloc(self, prob, opt, out, stream=None):
+        self.check(self.lib.mideq_ensemble_solve_device_noalloc(
+            self.h, C.byref(prob), C.byref(opt), C.byref(out), stream),
+            "mideq_ensemble_solve_device_noalloc")
 
     def register_model(self, spec):
         mid = C.c_int32()

@@ -217,6 +217,26 @@ int mideq_ensemble_solve_device(mideq_ctx *ctx, const mideq_problem *prob,
                                 const mideq_options *opt, mideq_outputs *out,
                                 void *stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): mideq_ensemble_solve_device with its scratch -- the
+ * work-queue counters, the two-pass AutoSwitch handoff list (8 bytes per
+ * trajectory when auto_passes >= 2) and the reduction's block partials
+ * (8 * nout * ceil(ntraj / 256) bytes) -- in a fixed region of
+ * MIDEQ_NOALLOC_SCRATCH_BYTES inside the library, one per GPU, held until
+ * the process ends.  Calls no hipMalloc or hipFree.  A solve needing more
+ * scratch than the region holds is refused (MIDEQ_E_ARG) before any launch;
+ * graph = 1 is refused (instantiating a graph allocates); a user model must
+ * have its kernels for the configuration built by an earlier call of
+ * another entry point; lanes = 0 on the Brusselator takes the context's
+ * tuned choice or 64-lane teams (no tuning run).  Every call on one GPU
+ * shares the region: calls that overlap in time on one GPU (any context,
+ * any stream) race and must be ordered by the caller. */
+#define MIDEQ_NOALLOC_SCRATCH_BYTES (8u << 20)
+int mideq_ensemble_solve_device_noalloc(mideq_ctx *ctx, const mideq_problem *prob,
+                                        const mideq_options *opt, mideq_outputs *out,
+                                        void *stream);
+
 /* ---- run-time compiled right-hand sides (hipRTC).  Replaces the user's
  * in-place f!(du, u, p, t) of ODEProblem(f!, u0, tspan, p) (test/core.jl:22-27,
  * test/qa/qa.jl:86) and ODEFunction(f!; jac) (qa.jl:84) for the lane kernels:
