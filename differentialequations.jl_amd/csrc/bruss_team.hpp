@@ -61,6 +61,29 @@ __device__ __forceinline__ int team_min_int(int v) {
   return v;
 }
 
+// A value that is the same in every lane of the team, marked uniform to the
+// compiler when the team is the whole wavefront (scalar registers, scalar
+// branches); a 32-lane team shares its wavefront with another instance, so
+// its values stay per lane.
+template <int L>
+__device__ __forceinline__ int team_uni(int v) {
+  if constexpr (L == 64)
+    return __builtin_amdgcn_readfirstlane(v);
+  else
+    return v;
+}
+
+template <int L>
+__device__ __forceinline__ unsigned long long team_uni_u64(unsigned long long v) {
+  if constexpr (L == 64) {
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+    return ((unsigned long long)hi << 32) | lo;
+  } else {
+    return v;
+  }
+}
+
 template <int L>
 __device__ __forceinline__ int team_and(int v) {
 #pragma unroll
@@ -238,7 +261,7 @@ struct BTeam {
   __device__ void init(const BrussArgs& a, int64_t idx) {
     i = idx;
     const int64_t NT = a.ntraj;
-    n = a.nvec ? a.nvec[i] : a.nmax;
+    n = team_uni<L>(a.nvec ? a.nvec[i] : a.nmax);
     N = n / 2;
     nb = n - kT;
     A = a.p[i];
@@ -583,6 +606,9 @@ struct BTeam {
   // (EEst = 2, as the dense LU's early return); 2: a non-finite value met a
   // skipped zero first (EEst = NaN).
   __device__ int factor_band(double dts) {
+    // the sizes are the team's (uniform: scalar loop control on 64-lane teams)
+    n = team_uni<L>(n);
+    nb = team_uni<L>(nb);
 #pragma unroll
     for (int m = 0; m < C; m++) {
       int c = comp(m);
@@ -644,7 +670,28 @@ struct BTeam {
     if (kAuto) eig = team_max<L>(best);
     team_sync();
     int sing_k = n, pois_k = n;
-    for (int k = 0; k < n; k++) {
+    // columns k <= nb - 5: the candidates and the pattern have their fixed
+    // shape (rows k, k+1, k+2 and the kT dense rows; columns k..k+4 and the
+    // border), so column k's seven candidate values load at once
+    const int kmain = nb - 5;
+    for (int k = 0; k <= kmain; k++) {
+      const double v0 = Wb[k * 8 + 2], v1 = Wb[(k + 1) * 8 + 1], v2 = Wb[(k + 2) * 8];
+      const double v3 = Wd[k], v4 = Wd[NK + k], v5 = Wd[2 * NK + k], v6 = Wd[3 * NK + k];
+      // the first maximal |W(q, k)| in ascending row order
+      double d = v0, bv = fabs(v0);
+      int pk = k;
+      const double vq[6] = {v1, v2, v3, v4, v5, v6};
+#pragma unroll
+      for (int t = 0; t < 6; t++) {
+        const bool take = fabs(vq[t]) > bv;
+        bv = take ? fabs(vq[t]) : bv;
+        d = take ? vq[t] : d;
+        pk = take ? (t < 2 ? k + 1 + t : nb + (t - 2)) : pk;
+      }
+      pk = team_uni<L>(pk);
+      lu_column(k, 2, 4, 7, 9, pk, d, sing_k, pois_k);
+    }
+    for (int k = kmain + 1; k < n; k++) {
       const int tail0 = max(nb, k + 1);
       const int nbc = nband_after(k);
       const int nc = 1 + nbc + (n - tail0);
@@ -664,54 +711,161 @@ struct BTeam {
           d = v;
         }
       }
-      if (pk != k) {
-        for (int jc = tl; jc < np; jc += L) {
-          const int c = patcol(k, nwc, jc);
-          double* a = wp(k, c);
-          double* b = wp(pk, c);
-          const double x = *a, y = *b;
-          *a = y;
-          *b = x;
-        }
-        team_sync();
-      }
-      if (d == 0.0 && sing_k == n) sing_k = k;
-      const double rr = 1.0 / d;
-      if (tl == 0) {
-        ldsr[k] = rr;
-        ldsi[k] = pk;
-      }
-      // rows past k outside the candidates / columns outside the pattern:
-      // the dense LU multiplies their zeros (skipped here)
-      const bool skip_rows = (n - 1 - k) > (nc - 1);
-      const bool skip_cols = (n - 1 - k) > (np - 1);
-      bool bad = skip_rows && !(fabs(rr) <= __DBL_MAX__);
-      // trailing update a_qc -= l_q a_kc over candidate rows x pattern columns
-      for (int pi = tl; pi < (nc - 1) * 8; pi += L) {
-        const int cj = 1 + (pi & 7);
-        if (cj < np) {
-          const int q = cand(k, nbc, 1 + (pi >> 3));
-          const int c = patcol(k, nwc, cj);
-          const double l = *wp(q, k) * rr;
-          const double wk = *wp(k, c);
-          double* a = wp(q, c);
-          *a = fma(-l, wk, *a);
-          bad = bad || (skip_rows && !(fabs(wk) <= __DBL_MAX__));
-        }
-      }
-      // multipliers, stored after every read of column k above
-      for (int qi = tl; qi < nc - 1; qi += L) {
-        double* a = wp(cand(k, nbc, 1 + qi), k);
-        const double l = *a * rr;
-        *a = l;
-        bad = bad || (skip_cols && !(fabs(l) <= __DBL_MAX__));
-      }
-      if (bad && pois_k == n) pois_k = k;
-      team_sync();
+      lu_column(k, nbc, nwc, nc, np, team_uni<L>(pk), d, sing_k, pois_k);
     }
     pois_k = team_min_int<L>(pois_k);
     if (sing_k < n && sing_k <= pois_k) return 1;
     return pois_k < n ? 2 : 0;
+  }
+
+  // Column k of the factorization once its pivot row pk (pivot value d) is
+  // chosen: interchange rows k and pk over the pattern, store 1/d and pk,
+  // update the trailing candidate rows x pattern columns, store the
+  // multipliers.
+  __device__ __forceinline__ void lu_column(int k, int nbc, int nwc, int nc, int np, int pk,
+                                            double d, int& sing_k, int& pois_k) {
+    if (pk != k) {
+      for (int jc = tl; jc < np; jc += L) {
+        const int c = patcol(k, nwc, jc);
+        double* a = wp(k, c);
+        double* b = wp(pk, c);
+        const double x = *a, y = *b;
+        *a = y;
+        *b = x;
+      }
+      team_sync();
+    }
+    if (d == 0.0 && sing_k == n) sing_k = k;
+    const double rr = 1.0 / d;
+    if (tl == 0) {
+      ldsr[k] = rr;
+      ldsi[k] = pk;
+    }
+    // rows past k outside the candidates / columns outside the pattern:
+    // the dense LU multiplies their zeros (skipped here)
+    const bool skip_rows = (n - 1 - k) > (nc - 1);
+    const bool skip_cols = (n - 1 - k) > (np - 1);
+    bool bad = skip_rows && !(fabs(rr) <= __DBL_MAX__);
+    // trailing update a_qc -= l_q a_kc over candidate rows x pattern columns
+    for (int pi = tl; pi < (nc - 1) * 8; pi += L) {
+      const int cj = 1 + (pi & 7);
+      if (cj < np) {
+        const int q = cand(k, nbc, 1 + (pi >> 3));
+        const int c = patcol(k, nwc, cj);
+        const double l = *wp(q, k) * rr;
+        const double wk = *wp(k, c);
+        double* a = wp(q, c);
+        *a = fma(-l, wk, *a);
+        bad = bad || (skip_rows && !(fabs(wk) <= __DBL_MAX__));
+      }
+    }
+    // multipliers, stored after every read of column k above
+    for (int qi = tl; qi < nc - 1; qi += L) {
+      double* a = wp(cand(k, nbc, 1 + qi), k);
+      const double l = *a * rr;
+      *a = l;
+      bad = bad || (skip_cols && !(fabs(l) <= __DBL_MAX__));
+    }
+    if (bad && pois_k == n) pois_k = k;
+    team_sync();
+  }
+
+  // The substitution chains of band_solve: the factors one position needs,
+  // loaded a position ahead, and the chain's running values.
+  struct FwdFactors {
+    int pj;
+    double l1, l2, d0, d1, d2, d3, t3;
+  };
+  struct FwdChain {
+    double b0, b1, b2, bd0, bd1, bd2, bd3, nonfin;
+  };
+  struct BackFactors {
+    double u6, u5, u4, u3, ri, y;
+  };
+  struct BackChain {
+    double x1, x2, x3, x4, nonfin;
+  };
+
+  // position j's pivot row, multipliers of rows j+1, j+2 and the dense rows,
+  // and the right-hand side entry j + 3
+  __device__ __forceinline__ void fwd_load(FwdFactors& f, int j) const {
+    f.pj = ldsi[j];
+    f.l1 = Wb[(j + 1) * 8 + 1];
+    f.l2 = Wb[(j + 2) * 8];
+    f.d0 = Wd[j];
+    f.d1 = Wd[NK + j];
+    f.d2 = Wd[2 * NK + j];
+    f.d3 = Wd[3 * NK + j];
+    f.t3 = ldsd[j + 3];
+  }
+
+  __device__ __forceinline__ void fwd_step(int j, const FwdFactors& f, FwdFactors& next,
+                                           FwdChain& c) {
+    const int pj = team_uni<L>(f.pj);
+    const double bn = (j + 3 < nb) ? f.t3 : 0.0;
+    fwd_load(next, j + 1);
+    double b0 = c.b0, b1 = c.b1, b2 = c.b2, bd0 = c.bd0, bd1 = c.bd1, bd2 = c.bd2, bd3 = c.bd3;
+    if (pj != j) {
+      double t = b0;
+      if (pj == j + 1) {
+        b0 = b1;
+        b1 = t;
+      } else if (pj == j + 2) {
+        b0 = b2;
+        b2 = t;
+      } else if (pj == nb) {
+        b0 = bd0;
+        bd0 = t;
+      } else if (pj == nb + 1) {
+        b0 = bd1;
+        bd1 = t;
+      } else if (pj == nb + 2) {
+        b0 = bd2;
+        bd2 = t;
+      } else {
+        b0 = bd3;
+        bd3 = t;
+      }
+    }
+    const double xj = b0;
+    if (tl == 0) ldsd[j] = xj;
+    c.b0 = fma(-f.l1, xj, b1);
+    c.b1 = fma(-f.l2, xj, b2);
+    c.b2 = bn;
+    c.bd0 = fma(-f.d0, xj, bd0);
+    c.bd1 = fma(-f.d1, xj, bd1);
+    c.bd2 = fma(-f.d2, xj, bd2);
+    c.bd3 = fma(-f.d3, xj, bd3);
+    if (j < n - 7) c.nonfin = fma(xj, 0.0, c.nonfin);
+  }
+
+  // row i's window entries, reciprocal pivot and right-hand side
+  __device__ __forceinline__ void back_load(BackFactors& f, int i) const {
+    f.u6 = Wb[i * 8 + 6];
+    f.u5 = Wb[i * 8 + 5];
+    f.u4 = Wb[i * 8 + 4];
+    f.u3 = Wb[i * 8 + 3];
+    f.ri = ldsr[i];
+    f.y = ldsd[i];
+  }
+
+  // row i <= nb - 5: all four window columns are band columns
+  __device__ __forceinline__ void back_step(int i, const BackFactors& f, BackFactors& next,
+                                            BackChain& c) {
+    back_load(next, i > 0 ? i - 1 : 0);
+    double y = f.y;
+    y = fma(-f.u6, c.x4, y);
+    y = fma(-f.u5, c.x3, y);
+    y = fma(-f.u4, c.x2, y);
+    y = fma(-f.u3, c.x1, y);
+    const double xi = y * f.ri;
+    if (tl == 0) ldsd[i] = xi;
+    // column i: rows above i-4 hold exact zeros
+    if (i >= 5) c.nonfin = fma(xi, 0.0, c.nonfin);
+    c.x4 = c.x3;
+    c.x3 = c.x2;
+    c.x2 = c.x1;
+    c.x1 = xi;
   }
 
   // Solve W x = b with the factors of factor_band; x: owned components in
@@ -720,6 +874,8 @@ struct BTeam {
   // team computing the same values (lane 0 stores them).  Returns false when
   // a non-finite value met a skipped zero (the dense solve's NaNs).
   __device__ bool band_solve(double* x) {
+    n = team_uni<L>(n);
+    nb = team_uni<L>(nb);
 #pragma unroll
     for (int m = 0; m < C; m++) {
       int c = comp(m);
@@ -730,47 +886,37 @@ struct BTeam {
     double bd0 = ldsd[nb], bd1 = ldsd[nb + 1], bd2 = ldsd[nb + 2], bd3 = ldsd[nb + 3];
     double b0 = ldsd[0], b1 = ldsd[1], b2 = ldsd[2];
     // forward, positions j with j + 2 < nb: candidates j+1, j+2 and the
-    // dense rows (7); rows past j+2 outside them while j < n - 7
-    for (int j = 0; j <= nb - 3; j++) {
-      const int pj = ldsi[j];
-      const double l1 = Wb[(j + 1) * 8 + 1], l2 = Wb[(j + 2) * 8];
-      const double d0 = Wd[j], d1 = Wd[NK + j], d2 = Wd[2 * NK + j], d3 = Wd[3 * NK + j];
-      const double bn = (j + 3 < nb) ? ldsd[j + 3] : 0.0;
-      if (pj != j) {
-        double t = b0;
-        if (pj == j + 1) {
-          b0 = b1;
-          b1 = t;
-        } else if (pj == j + 2) {
-          b0 = b2;
-          b2 = t;
-        } else if (pj == nb) {
-          b0 = bd0;
-          bd0 = t;
-        } else if (pj == nb + 1) {
-          b0 = bd1;
-          bd1 = t;
-        } else if (pj == nb + 2) {
-          b0 = bd2;
-          bd2 = t;
-        } else {
-          b0 = bd3;
-          bd3 = t;
-        }
+    // dense rows (7); rows past j+2 outside them while j < n - 7.  Two
+    // register sets of factors alternate: position j + 1's load while
+    // position j computes (indices stay inside the team's arrays: j + 4 <=
+    // nb + 1 < NK).  A non-finite x_j (j < n - 7) turns nonfin into NaN.
+    FwdChain st;
+    st.b0 = b0;
+    st.b1 = b1;
+    st.b2 = b2;
+    st.bd0 = bd0;
+    st.bd1 = bd1;
+    st.bd2 = bd2;
+    st.bd3 = bd3;
+    st.nonfin = 0.0;
+    {
+      FwdFactors fa, fb;
+      fwd_load(fa, 0);
+      const int jl = nb - 3;
+      int j = 0;
+      for (; j + 1 <= jl; j += 2) {
+        fwd_step(j, fa, fb, st);
+        fwd_step(j + 1, fb, fa, st);
       }
-      const double xj = b0;
-      if (tl == 0) ldsd[j] = xj;
-      b1 = fma(-l1, xj, b1);
-      b2 = fma(-l2, xj, b2);
-      bd0 = fma(-d0, xj, bd0);
-      bd1 = fma(-d1, xj, bd1);
-      bd2 = fma(-d2, xj, bd2);
-      bd3 = fma(-d3, xj, bd3);
-      bad = bad || (j < n - 7 && !(fabs(xj) <= __DBL_MAX__));
-      b0 = b1;
-      b1 = b2;
-      b2 = bn;
+      if (j <= jl) fwd_step(j, fa, fb, st);
     }
+    b0 = st.b0;
+    b1 = st.b1;
+    bd0 = st.bd0;
+    bd1 = st.bd1;
+    bd2 = st.bd2;
+    bd3 = st.bd3;
+    bad = bad || (st.nonfin != st.nonfin);
     // the last two band positions and the dense ones: every later position
     // is a candidate
     double v[6] = {b0, b1, bd0, bd1, bd2, bd3};
@@ -819,25 +965,43 @@ struct BTeam {
       }
     }
     team_sync();
-    // band rows: the window columns i+4 .. i+1 (those < nb), a chain
+    // band rows: the window columns i+4 .. i+1 (those < nb), a chain; the
+    // last four rows (nb >= 12) hold fewer window columns
     double x1 = 0.0, x2 = 0.0, x3 = 0.0, x4 = 0.0;
-    for (int i = nb - 1; i >= 0; i--) {
+#pragma unroll
+    for (int ii = 1; ii <= 4; ii++) {
+      const int i = nb - ii;
       const double* U = Wb + i * 8;
-      const double u6 = U[6], u5 = U[5], u4 = U[4], u3 = U[3];
-      const double ri = ldsr[i];
       double y = ldsd[i];
-      if (i + 4 < nb) y = fma(-u6, x4, y);
-      if (i + 3 < nb) y = fma(-u5, x3, y);
-      if (i + 2 < nb) y = fma(-u4, x2, y);
-      if (i + 1 < nb) y = fma(-u3, x1, y);
-      const double xi = y * ri;
+      if (ii >= 4) y = fma(-U[5], x3, y);
+      if (ii >= 3) y = fma(-U[4], x2, y);
+      if (ii >= 2) y = fma(-U[3], x1, y);
+      const double xi = y * ldsr[i];
       if (tl == 0) ldsd[i] = xi;
-      // column i: rows above i-4 hold exact zeros
       bad = bad || (i >= 5 && !(fabs(xi) <= __DBL_MAX__));
       x4 = x3;
       x3 = x2;
       x2 = x1;
       x1 = xi;
+    }
+    // the other rows, two register sets alternating (row i - 1's factors
+    // load while row i computes); rows i >= 5 feed the non-finite check
+    {
+      BackChain bs;
+      bs.x1 = x1;
+      bs.x2 = x2;
+      bs.x3 = x3;
+      bs.x4 = x4;
+      bs.nonfin = 0.0;
+      BackFactors ba, bb;
+      int i = nb - 5;
+      back_load(ba, i);
+      for (; i - 1 >= 0; i -= 2) {
+        back_step(i, ba, bb, bs);
+        back_step(i - 1, bb, ba, bs);
+      }
+      if (i >= 0) back_step(i, ba, bb, bs);
+      bad = bad || (bs.nonfin != bs.nonfin);
     }
     team_sync();
 #pragma unroll
@@ -1201,11 +1365,12 @@ __device__ __forceinline__ void bruss_run(const BrussArgs& a) {
     if (!active && !drained) {
       unsigned long long q = 0;
       if (T.tl == 0) q = atomicAdd(a.queue, 1ull);
-      q = __shfl(q, 0, L);
+      q = team_uni_u64<L>(__shfl(q, 0, L));
       if (q >= (unsigned long long)a.nqueue) {
         drained = true;
       } else {
-        int64_t idx = a.order ? a.order[q] : (int64_t)q;
+        int64_t idx = (int64_t)team_uni_u64<L>(
+            (unsigned long long)(a.order ? a.order[q] : (int64_t)q));
         // an order entry outside [0, ntraj) is skipped
         if ((uint64_t)idx < (uint64_t)a.ntraj) {
           T.init(a, idx);
