@@ -674,6 +674,22 @@ struct BTeam {
     // shape (rows k, k+1, k+2 and the kT dense rows; columns k..k+4 and the
     // border), so column k's seven candidate values load at once
     const int kmain = nb - 5;
+    // this lane's entries of the trailing update (6 candidate rows x 8
+    // pattern columns) and of the multiplier column, as offsets from Wb
+    // affine in k (the band, border and dense parts are contiguous)
+    constexpr int kE = (48 + L - 1) / L;
+    int a_off[kE], a_st[kE], l_off[kE], l_st[kE], w_off[kE], w_st[kE];
+#pragma unroll
+    for (int e = 0; e < kE; e++) {
+      const int pi = tl + e * L;
+      const int qi = (pi >> 3) < 6 ? (pi >> 3) : 5, cj = 1 + (pi & 7);
+      lu_pattern_offsets(qi, cj, a_off[e], a_st[e], l_off[e], l_st[e], w_off[e], w_st[e]);
+    }
+    int m_off = 0, m_st = 0;
+    {
+      int t0, t1, t2, t3;
+      lu_pattern_offsets(tl < 6 ? tl : 5, 1, t0, t1, m_off, m_st, t2, t3);
+    }
     for (int k = 0; k <= kmain; k++) {
       const double v0 = Wb[k * 8 + 2], v1 = Wb[(k + 1) * 8 + 1], v2 = Wb[(k + 2) * 8];
       const double v3 = Wd[k], v4 = Wd[NK + k], v5 = Wd[2 * NK + k], v6 = Wd[3 * NK + k];
@@ -689,7 +705,38 @@ struct BTeam {
         pk = take ? (t < 2 ? k + 1 + t : nb + (t - 2)) : pk;
       }
       pk = team_uni<L>(pk);
-      lu_column(k, 2, 4, 7, 9, pk, d, sing_k, pois_k);
+      if (pk != k) {
+        lu_column(k, 2, 4, 7, 9, pk, d, sing_k, pois_k);
+        continue;
+      }
+      // no interchange (the usual case): the pattern's fixed addresses
+      if (d == 0.0 && sing_k == n) sing_k = k;
+      const double rr = 1.0 / d;
+      if (tl == 0) {
+        ldsr[k] = rr;
+        ldsi[k] = k;
+      }
+      const bool skip_rows = k < n - 7;
+      const bool skip_cols = k < n - 9;
+      bool bad = skip_rows && !(fabs(rr) <= __DBL_MAX__);
+#pragma unroll
+      for (int e = 0; e < kE; e++) {
+        if (tl + e * L < 48) {
+          const double l = Wb[l_off[e] + k * l_st[e]] * rr;
+          const double wk = Wb[w_off[e] + k * w_st[e]];
+          double* a = Wb + (a_off[e] + k * a_st[e]);
+          *a = fma(-l, wk, *a);
+          bad = bad || (skip_rows && !(fabs(wk) <= __DBL_MAX__));
+        }
+      }
+      if (tl < 6) {
+        double* a = Wb + (m_off + k * m_st);
+        const double l = *a * rr;
+        *a = l;
+        bad = bad || (skip_cols && !(fabs(l) <= __DBL_MAX__));
+      }
+      if (bad && pois_k == n) pois_k = k;
+      team_sync();
     }
     for (int k = kmain + 1; k < n; k++) {
       const int tail0 = max(nb, k + 1);
@@ -716,6 +763,49 @@ struct BTeam {
     pois_k = team_min_int<L>(pois_k);
     if (sing_k < n && sing_k <= pois_k) return 1;
     return pois_k < n ? 2 : 0;
+  }
+
+  // Offsets from Wb (affine in the column k: offset + k * stride) of the
+  // fixed-pattern LU entries for candidate row index qi (0, 1: rows k+1,
+  // k+2; 2..5: dense rows nb..nb+3) and pattern column index cj (1..4:
+  // columns k+1..k+4; 5..8: border columns nb..nb+3): the updated entry
+  // W(q, c), its multiplier source W(q, k) and the pivot row's W(k, c).
+  __device__ __forceinline__ void lu_pattern_offsets(int qi, int cj, int& a_off, int& a_st,
+                                                     int& l_off, int& l_st, int& w_off,
+                                                     int& w_st) const {
+    const bool qband = qi < 2, cband = cj <= 4;
+    const int dq = qi + 1, r = qi - 2, dc = cj, cb = cj - 5;
+    constexpr int oR = 8 * NK, oD = 8 * NK + kT * NK;
+    // W(q, c)
+    if (qband && cband) {
+      a_off = 7 * dq + dc + 2;
+      a_st = 8;
+    } else if (qband) {
+      a_off = oR + kT * dq + cb;
+      a_st = kT;
+    } else if (cband) {
+      a_off = oD + r * NK + dc;
+      a_st = 1;
+    } else {
+      a_off = oD + r * NK + nb + cb;
+      a_st = 0;
+    }
+    // W(q, k)
+    if (qband) {
+      l_off = 7 * dq + 2;
+      l_st = 8;
+    } else {
+      l_off = oD + r * NK;
+      l_st = 1;
+    }
+    // W(k, c)
+    if (cband) {
+      w_off = dc + 2;
+      w_st = 8;
+    } else {
+      w_off = oR + cb;
+      w_st = kT;
+    }
   }
 
   // Column k of the factorization once its pivot row pk (pivot value d) is
