@@ -92,7 +92,9 @@ __device__ __forceinline__ int team_and(int v) {
 }
 
 // SAVE: builds with saveat (the step's stages stay live until the accepted
-// step has written its saved points).
+// step has written its saved points).  Every member function is inlined by
+// force: an out-of-line call passes the team through scratch memory, where
+// its LDS pointers turn into generic ones (flat loads instead of LDS).
 template <int L, int C, bool MIX, int ALG, bool FAST, bool SAVE>
 struct BTeam {
   static constexpr bool kAuto = (ALG == ALG_AUTO || ALG == ALG_AUTO5P);
@@ -135,7 +137,7 @@ struct BTeam {
   __device__ __forceinline__ int rowc(int m) const { return tl + L * m; }
 
   // ---------------------------------------------------------------- RHS
-  __device__ void rhs64(double* out, const double* in) {
+  __device__ __forceinline__ void rhs64(double* out, const double* in) {
 #pragma unroll
     for (int m = 0; m < C; m++) {
       int c = comp(m);
@@ -163,7 +165,7 @@ struct BTeam {
     team_sync();
   }
 
-  __device__ void rhs32(float* out, const float* in) {
+  __device__ __forceinline__ void rhs32(float* out, const float* in) {
 #pragma unroll
     for (int m = 0; m < C; m++) {
       int c = comp(m);
@@ -191,7 +193,7 @@ struct BTeam {
     team_sync();
   }
 
-  __device__ void fsal_init() {
+  __device__ __forceinline__ void fsal_init() {
     rhs64(k1, u);
     if (MIX) {
       float uf[C];
@@ -207,7 +209,7 @@ struct BTeam {
   }
 
   // ---------------------------------------------------------------- initdt
-  __device__ double initdt(const BrussArgs& a, double t0) {
+  __device__ __forceinline__ double initdt(const BrussArgs& a, double t0) {
     double dtmin_e = nextafter(fmax(a.dtmin, jl_eps(t0)), __builtin_inf());
     double smalldt = fmax(dtmin_e, 1e-6);
     double sk[C];
@@ -258,7 +260,7 @@ struct BTeam {
   }
 
   // ---------------------------------------------------------------- init
-  __device__ void init(const BrussArgs& a, int64_t idx) {
+  __device__ __forceinline__ void init(const BrussArgs& a, int64_t idx) {
     i = idx;
     const int64_t NT = a.ntraj;
     n = team_uni<L>(a.nvec ? a.nvec[i] : a.nmax);
@@ -356,7 +358,7 @@ struct BTeam {
   // ---------------------------------------------------------------- Tsit5
   // EEst (reference controller) or EEst^2 = mean(r^2) without the square
   // root (fast controller, which works on log2 EEst^2).
-  __device__ double err_norm(const double* ut, const double* u1, double abstol,
+  __device__ __forceinline__ double err_norm(const double* ut, const double* u1, double abstol,
                              double reltol) {
     double s = 0.0;
 #pragma unroll
@@ -371,7 +373,7 @@ struct BTeam {
     return rms_from_partial(s);
   }
 
-  __device__ void eig_estimate(const double* k7, const double* k6, const double* u1,
+  __device__ __forceinline__ void eig_estimate(const double* k7, const double* k6, const double* u1,
                                const double* g6) {
     double e = 0.0;
     int bad = 0;
@@ -387,7 +389,7 @@ struct BTeam {
     eig = team_or<L>(bad) ? __builtin_nan("") : e;
   }
 
-  __device__ double tsit5_step64(double dts, double* u1, double abstol, double reltol,
+  __device__ __forceinline__ double tsit5_step64(double dts, double* u1, double abstol, double reltol,
                                 double (*kout)[C]) {
     using namespace ts;
     double k[7][C];
@@ -467,7 +469,7 @@ struct BTeam {
     return err_norm(ut, u1, abstol, reltol);
   }
 
-  __device__ double tsit5_step_mix(double dts, double* u1, float* k7out, double abstol,
+  __device__ __forceinline__ double tsit5_step_mix(double dts, double* u1, float* k7out, double abstol,
                                    double reltol, float (*kout)[C]) {
     using namespace ts;
     float k[7][C];
@@ -605,7 +607,7 @@ struct BTeam {
   // Builds W and factors it.  0: factored; 1: a zero pivot came first
   // (EEst = 2, as the dense LU's early return); 2: a non-finite value met a
   // skipped zero first (EEst = NaN).
-  __device__ int factor_band(double dts) {
+  __device__ __forceinline__ int factor_band(double dts) {
     // the sizes are the team's (uniform: scalar loop control on 64-lane teams)
     n = team_uni<L>(n);
     nb = team_uni<L>(nb);
@@ -963,7 +965,7 @@ struct BTeam {
   // substitution walk the band as chains in registers, every lane of the
   // team computing the same values (lane 0 stores them).  Returns false when
   // a non-finite value met a skipped zero (the dense solve's NaNs).
-  __device__ bool band_solve(double* x) {
+  __device__ __forceinline__ bool band_solve(double* x) {
     n = team_uni<L>(n);
     nb = team_uni<L>(nb);
 #pragma unroll
@@ -1102,7 +1104,7 @@ struct BTeam {
 
   // SAVE builds: Kout receives the dense-output combinations K_m = sum_j
   // H_mj k_j (m < 3) of the step.
-  __device__ bool rodas5_step(double dts, double* u1, double& EEst, double abstol,
+  __device__ __forceinline__ bool rodas5_step(double dts, double* u1, double& EEst, double abstol,
                               double reltol, double (*Kout)[C]) {
     const int fac = factor_band(dts);
     if (fac == 1) {
@@ -1206,7 +1208,7 @@ struct BTeam {
   }
 
   // ---------------------------------------------------------------- step
-  __device__ void step(const BrussArgs& a) {
+  __device__ __forceinline__ void step(const BrussArgs& a) {
     if (kAuto && iter > 0) {
       double stiffness = fabs(eig * dt / kStabSize);
       bool stiff = stiffness > kStiffTol;
@@ -1396,7 +1398,7 @@ struct BTeam {
     if (!(t < tf)) rc = RC_SUCCESS;
   }
 
-  __device__ void finish(const BrussArgs& a) {
+  __device__ __forceinline__ void finish(const BrussArgs& a) {
     const int64_t NT = a.ntraj;
     if constexpr (SAVE) {
       // points past the end of an unfinished solve

@@ -1,4 +1,4 @@
-// mlp.hip -- the dt0 / method-class / cost predictor: a 16 -> 64 -> 64 -> 64 -> 3
+// mlp.hip -- the dt0 / method-class / cost / never-stiff predictor: a 16 -> 64 -> 64 -> 64 -> 4
 // GELU MLP with BF16 weights and activations and FP32 accumulation on the
 // CDNA4 matrix cores (v_mfma_f32_16x16x16_bf16 / v_mfma_f32_16x16x32_bf16),
 // trained on the GPU by the kernels below (forward + backward per tile,
@@ -27,7 +27,7 @@
 namespace mideq {
 namespace mlp {
 
-constexpr int NIN = 16, NH = 64, NOUT = 3;
+constexpr int NIN = 16, NH = 64, NOUT = 4;
 constexpr int OFF_W1 = 0;
 constexpr int OFF_B1 = OFF_W1 + NIN * NH;
 constexpr int OFF_W2 = OFF_B1 + NH;
@@ -157,7 +157,7 @@ struct Tile {
   v4s x;          // normalised input, bf16 (features 4g..4g+3)
   v4f z[3][4];    // pre-activations of the hidden layers
   v4s h[3][4];    // hidden activations, bf16
-  v4f y;          // outputs (rows 0..2 meaningful on lanes with g == 0)
+  v4f y;          // outputs (rows 0..3 on lanes with g == 0)
 };
 
 __device__ void forward_tile(const Frags& F, const float* X, const float* mean,
@@ -224,6 +224,7 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const short* wb, const float*
       Y[s * NOUT + 0] = T.y[0];
       Y[s * NOUT + 1] = T.y[1];
       Y[s * NOUT + 2] = T.y[2];
+      Y[s * NOUT + 3] = T.y[3];
     }
   }
 }
@@ -241,9 +242,10 @@ struct TrainArgs {
   const float* X;
   const float* target;
   const float* label;
-  const float* cost;  // may be null: no cost term
+  const float* cost;   // may be null: no cost term
+  const float* never;  // may be null: no never-stiff term
   int64_t B, Bp;
-  float w_dt, w_cls, w_cost;
+  float w_dt, w_cls, w_cost, w_never;
   short* hs0;   // [NIN][Bp]
   short* hs;    // 3 x [NH][Bp]: inputs of layers 2, 3, 4
   short* dzs;   // 3 x [NH][Bp]: dZ of layers 1, 2, 3
@@ -283,8 +285,8 @@ __global__ __launch_bounds__(64) void k_mlp_fwdbwd(TrainArgs a) {
       for (int r = 0; r < 4; r++)
         a.hs[(int64_t)l * NH * Bp + (int64_t)(16 * ob + 4 * g + r) * Bp + s] =
             valid ? T.h[l][ob][r] : (short)0;
-  // output gradient (rows 0..2 on lanes g == 0)
-  float dy0 = 0.f, dy1 = 0.f, dy2 = 0.f, lsum = 0.f;
+  // output gradient (rows 0..3 on lanes g == 0)
+  float dy0 = 0.f, dy1 = 0.f, dy2 = 0.f, dy3 = 0.f, lsum = 0.f;
   if (valid && g == 0) {
     float t = a.target[s], c = a.label[s];
     float e = T.y[0] - t;
@@ -295,25 +297,35 @@ __global__ __launch_bounds__(64) void k_mlp_fwdbwd(TrainArgs a) {
     float bce = fmaxf(y1, 0.f) - y1 * c + log1pf(expf(-fabsf(y1)));
     float ec = a.cost ? T.y[2] - a.cost[s] : 0.0f;
     dy2 = 2.0f * ec * a.w_cost / (float)a.B;
-    lsum = (a.w_dt * e * e + a.w_cls * bce + a.w_cost * ec * ec) / (float)a.B;
+    float bn = 0.0f;
+    if (a.never) {
+      const float y3 = T.y[3], cn = a.never[s];
+      const float sig3 = 1.0f / (1.0f + expf(-y3));
+      dy3 = (sig3 - cn) * a.w_never / (float)a.B;
+      bn = fmaxf(y3, 0.f) - y3 * cn + log1pf(expf(-fabsf(y3)));
+    }
+    lsum = (a.w_dt * e * e + a.w_cls * bce + a.w_cost * ec * ec + a.w_never * bn) / (float)a.B;
   }
   v4s d4 = {0, 0, 0, 0};
   if (g == 0) {
     d4[0] = f2bf(dy0);
     d4[1] = f2bf(dy1);
     d4[2] = f2bf(dy2);
+    d4[3] = f2bf(dy3);
   }
   if (g == 0) {
     a.dz4[0 * Bp + s] = d4[0];
     a.dz4[1 * Bp + s] = d4[1];
     a.dz4[2 * Bp + s] = d4[2];
+    a.dz4[3 * Bp + s] = d4[3];
   }
   {
-    float b0 = sum16(dy0), b1 = sum16(dy1), b2 = sum16(dy2), ls = sum16(lsum);
+    float b0 = sum16(dy0), b1 = sum16(dy1), b2 = sum16(dy2), b3 = sum16(dy3), ls = sum16(lsum);
     if (lane == 0) {
       atomicAdd(&a.grad[OFF_B4 + 0], b0);
       atomicAdd(&a.grad[OFF_B4 + 1], b1);
       atomicAdd(&a.grad[OFF_B4 + 2], b2);
+      atomicAdd(&a.grad[OFF_B4 + 3], b3);
       atomicAdd(a.loss, ls);
     }
   }
@@ -561,6 +573,7 @@ struct mideq_mlp {
   float* loss = nullptr;
   short* scratch = nullptr;
   size_t scratch_bytes = 0;
+  hipStream_t scratch_stream = nullptr;  // stream of the last call that used scratch
   int* ord = nullptr;  // launch-order histogram and offsets [2][ORD_BINS]
   int ncu = 256;
   int64_t step = 0;
@@ -694,8 +707,9 @@ extern "C" int mideq_mlp_infer(mideq_mlp* m, const float* X, int64_t B, float* Y
 }
 
 extern "C" int mideq_mlp_grad(mideq_mlp* m, const float* X, const float* target,
-                              const float* label, const float* cost, int64_t B, float w_dt,
-                              float w_cls, float w_cost, float* loss_host, void* stream) {
+                              const float* label, const float* cost, const float* never,
+                              int64_t B, float w_dt, float w_cls, float w_cost, float w_never,
+                              float* loss_host, void* stream) {
   if (!m || B <= 0 || !X || !target || !label) return MIDEQ_E_ARG;
   hipError_t e = hipSetDevice(m->device);
   if (e != hipSuccess) return mlp_err(m, MIDEQ_E_HIP, "grad", e);
@@ -703,8 +717,12 @@ extern "C" int mideq_mlp_grad(mideq_mlp* m, const float* X, const float* target,
   int64_t Bp = (B + 31) / 32 * 32;
   size_t need = sizeof(short) * (size_t)Bp * (NIN + 3 * NH + 3 * NH + NOUT);
   if (m->scratch_bytes < need) {
-    e = hipDeviceSynchronize();
-    if (m->scratch) e = hipFree(m->scratch);
+    // the previous scratch may still be read by the last call's kernels
+    if (m->scratch) {
+      e = hipStreamSynchronize(m->scratch_stream);
+      e = hipStreamSynchronize(s);
+      e = hipFree(m->scratch);
+    }
     m->scratch = nullptr;
     m->scratch_bytes = 0;
     e = hipMalloc((void**)&m->scratch, need);
@@ -720,11 +738,14 @@ extern "C" int mideq_mlp_grad(mideq_mlp* m, const float* X, const float* target,
   a.target = target;
   a.label = label;
   a.cost = cost;
+  a.never = never;
   a.B = B;
   a.Bp = Bp;
   a.w_dt = w_dt;
   a.w_cls = w_cls;
   a.w_cost = cost ? w_cost : 0.0f;
+  a.w_never = never ? w_never : 0.0f;
+  m->scratch_stream = s;
   a.hs0 = m->scratch;
   a.hs = a.hs0 + (size_t)NIN * Bp;
   a.dzs = a.hs + (size_t)3 * NH * Bp;

@@ -1,6 +1,6 @@
 """Learned initial step, method class and cost for ensemble solves.
 
-A 16 -> 3x64 (GELU) -> 3 MLP, run by the BF16 matrix-core kernels of
+A 16 -> 3x64 (GELU) -> 4 MLP, run by the BF16 matrix-core kernels of
 libmideq.so (`mideq_mlp_*`), predicts for every trajectory
   * log10 of a step size the controller accepts first time (the plain
     solver's first accepted step), used as `dt0`;
@@ -9,7 +9,10 @@ libmideq.so (`mideq_mlp_*`), predicts for every trajectory
   * log2 of the solve's cost (attempted steps, a Rodas5 step counted twice),
     which with the class sets the launch `order`: trajectories of one class
     and similar cost share a wavefront, so lanes neither wait for a much
-    longer neighbour nor mix Tsit5 and Rodas5 steps.
+    longer neighbour nor mix Tsit5 and Rodas5 steps;
+  * a never-stiff logit: the solve takes no Rosenbrock step at all, which
+    is when plain Tsit5 (no stiffness test) gives AutoTsit5's result and
+    can be trusted (`auto_passes = 3`, `iter_cap`).
 The plain initdt heuristic and AutoTsit5 switching stay in place: the
 prediction only sets where each trajectory starts and when it is scheduled,
 and the solver's own controller and stiffness test still decide every step.
@@ -25,8 +28,8 @@ import numpy as np
 
 from . import _abi
 
-NIN, NOUT = 16, 3
-NPARAM = 9603
+NIN, NOUT = 16, 4
+NPARAM = 9668
 RODAS_STEP_COST = 2.0  # a Rodas5 step in Tsit5 steps, for the cost target
 MODEL_SLOT = {"lorenz": 0, "robertson": 1, "vdp": 2}
 
@@ -44,8 +47,8 @@ def _lib():
         L.mideq_mlp_get_grads.argtypes = [vp, vp]
         L.mideq_mlp_set_norm.argtypes = [vp, vp, vp]
         L.mideq_mlp_infer.argtypes = [vp, vp, C.c_int64, vp, vp]
-        L.mideq_mlp_grad.argtypes = [vp, vp, vp, vp, vp, C.c_int64, C.c_float, C.c_float,
-                                     C.c_float, vp, vp]
+        L.mideq_mlp_grad.argtypes = [vp, vp, vp, vp, vp, vp, C.c_int64, C.c_float, C.c_float,
+                                     C.c_float, C.c_float, vp, vp]
         L.mideq_mlp_adam_step.argtypes = [vp, C.c_float, C.c_float, C.c_float, C.c_float, vp]
         L.mideq_mlp_launch_order.argtypes = [vp, vp, C.c_int64, vp, vp]
         L._mlp_declared = True
@@ -134,7 +137,7 @@ class MLP:
         return Y
 
     def launch_order(self, Y, order=None, stream=None):
-        """Launch order (device int64 [B]) from infer()'s output Y [B, 3]: the
+        """Launch order (device int64 [B]) from infer()'s output Y [B, 4]: the
         library's counting sort by (predicted class, decreasing predicted
         cost in 1/64-octave buckets); see order_keys()."""
         import torch
@@ -147,12 +150,13 @@ class MLP:
         return order
 
     def grad(self, X, target, label, cost=None, w_dt=1.0, w_cls=1.0, w_cost=1.0,
-             want_loss=True, stream=None):
+             want_loss=True, stream=None, never=None, w_never=1.0):
         loss = C.c_float(0.0)
         s = None if stream is None else stream.cuda_stream
         cp = None if cost is None else cost.data_ptr()
+        npt = None if never is None else never.data_ptr()
         self._check(self.L.mideq_mlp_grad(self.h, X.data_ptr(), target.data_ptr(), label.data_ptr(),
-                                          cp, X.shape[0], w_dt, w_cls, w_cost,
+                                          cp, npt, X.shape[0], w_dt, w_cls, w_cost, w_never,
                                           C.byref(loss) if want_loss else None, s), "grad")
         return loss.value if want_loss else None
 
@@ -161,20 +165,24 @@ class MLP:
         self._check(self.L.mideq_mlp_adam_step(self.h, lr, beta1, beta2, eps, s), "adam")
 
     def fit(self, X, target, label, cost=None, epochs=4, batch=8192, lr=2e-3, seed=0,
-            w_cls=1.0, w_cost=0.25, init_bias=True, log=None):
+            w_cls=1.0, w_cost=0.25, init_bias=True, log=None, never=None, w_never=1.0):
         """Minibatch Adam over device tensors X [N,16], target [N], label [N]
-        and (optional) cost [N].  init_bias: start the output biases at the
-        target means and the label log-odds, so that no head's offset
-        dominates the loss at the start (pass False to continue training)."""
+        and (optional) cost [N] and never-stiff label [N].  init_bias: start
+        the output biases at the target means and the labels' log-odds, so
+        that no head's offset dominates the loss at the start (pass False to
+        continue training)."""
         import torch
         N = X.shape[0]
         if init_bias:
             theta = self.get_params()
             pl = float(label.float().mean().clamp(1e-3, 1 - 1e-3))
-            theta[NPARAM - 3] = float(target.float().mean())
-            theta[NPARAM - 2] = float(np.log(pl / (1.0 - pl)))
+            theta[NPARAM - 4] = float(target.float().mean())
+            theta[NPARAM - 3] = float(np.log(pl / (1.0 - pl)))
             if cost is not None:
-                theta[NPARAM - 1] = float(cost.float().mean())
+                theta[NPARAM - 2] = float(cost.float().mean())
+            if never is not None:
+                pn = float(never.float().mean().clamp(1e-3, 1 - 1e-3))
+                theta[NPARAM - 1] = float(np.log(pn / (1.0 - pn)))
             self.set_params(theta)
         g = torch.Generator(device="cpu").manual_seed(seed)
         losses = []
@@ -187,8 +195,10 @@ class MLP:
                 tb = target.index_select(0, idx).contiguous()
                 lb = label.index_select(0, idx).contiguous()
                 cb = None if cost is None else cost.index_select(0, idx).contiguous()
+                nvb = None if never is None else never.index_select(0, idx).contiguous()
                 want = (nb % 16 == 0)
-                lv = self.grad(xb, tb, lb, cb, 1.0, w_cls, w_cost, want_loss=want)
+                lv = self.grad(xb, tb, lb, cb, 1.0, w_cls, w_cost, want_loss=want, never=nvb,
+                               w_never=w_never)
                 self.adam_step(lr * (0.5 ** ep))
                 if want:
                     tot += lv
@@ -220,6 +230,12 @@ def training_targets(dt_first, naccept, nstiff, nreject=None):
     rej = 0 if nreject is None else nreject
     cost = naccept + rej + (RODAS_STEP_COST - 1.0) * nstiff
     return tgt, lab, np.log2(np.maximum(cost, 1.0)).astype(np.float32)
+
+
+def never_stiff_label(nstiff):
+    """1 where the plain AutoTsit5 solve took no Rosenbrock step: plain
+    Tsit5 then gives the same result."""
+    return (np.asarray(nstiff) == 0).astype(np.float32)
 
 
 ORDER_BUCKETS = 2048  # log2-cost buckets per class, 1/64 octave each over [0, 32)

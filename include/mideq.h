@@ -326,16 +326,16 @@ int mideq_model_check(const mideq_user_model *m, char *log, size_t log_size);
 int mideq_uses_time(const char *text);
 
 /* ---- learned dt0 / method-class / cost predictor (no counterpart in the
- * reference: it feeds dt0, alg_hint and order above; the initdt heuristic
- * and AutoTsit5 switching remain the fallback).  16 -> 64 -> 64 -> 64 -> 3
- * GELU MLP, BF16 weights/activations, FP32 accumulation on the matrix cores.
- * Parameters (float32, W[in][out] row-major): W1[16x64] b1[64] W2[64x64]
- * b2[64] W3[64x64] b3[64] W4[64x3] b4[3].  Outputs per sample: (log10 dt0,
- * stiffness logit, log2 cost).  X, Y, target, label, cost are device
- * pointers. */
+ * reference: it feeds dt0, alg_hint, iter_cap and order above; the initdt
+ * heuristic and AutoTsit5 switching remain the fallback).  16 -> 64 -> 64 ->
+ * 64 -> 4 GELU MLP, BF16 weights/activations, FP32 accumulation on the
+ * matrix cores.  Parameters (float32, W[in][out] row-major): W1[16x64]
+ * b1[64] W2[64x64] b2[64] W3[64x64] b3[64] W4[64x4] b4[4].  Outputs per
+ * sample: (log10 dt0, stiffness logit, log2 cost, never-stiff logit).  X, Y,
+ * target, label, cost, never are device pointers. */
 #define MIDEQ_MLP_NIN 16
-#define MIDEQ_MLP_NOUT 3
-#define MIDEQ_MLP_NPARAM 9603
+#define MIDEQ_MLP_NOUT 4
+#define MIDEQ_MLP_NPARAM 9668
 typedef struct mideq_mlp mideq_mlp;
 int mideq_mlp_create(mideq_mlp **out, int32_t device);
 void mideq_mlp_destroy(mideq_mlp *m);
@@ -343,17 +343,17 @@ const char *mideq_mlp_last_error(const mideq_mlp *m);
 int mideq_mlp_set_params(mideq_mlp *m, const float *params);   /* host [NPARAM] */
 int mideq_mlp_get_params(mideq_mlp *m, float *params);         /* host [NPARAM] */
 int mideq_mlp_set_norm(mideq_mlp *m, const float *mean, const float *std); /* host [16] */
-/* Y[B][3] = net(X[B][16]) */
+/* Y[B][4] = net(X[B][16]) */
 int mideq_mlp_infer(mideq_mlp *m, const float *X, int64_t B, float *Y, void *stream);
 /* gradient of w_dt*mean((y0-target)^2) + w_cls*mean(BCE(y1,label))
- * + w_cost*mean((y2-cost)^2) into the internal buffer (cost may be NULL:
- * that term is dropped); loss_host (may be NULL) receives the loss
- * (synchronizes) */
+ * + w_cost*mean((y2-cost)^2) + w_never*mean(BCE(y3,never)) into the
+ * internal buffer (cost / never may be NULL: that term is dropped);
+ * loss_host (may be NULL) receives the loss (synchronizes) */
 int mideq_mlp_grad(mideq_mlp *m, const float *X, const float *target, const float *label,
-                   const float *cost, int64_t B, float w_dt, float w_cls, float w_cost,
-                   float *loss_host, void *stream);
+                   const float *cost, const float *never, int64_t B, float w_dt, float w_cls,
+                   float w_cost, float w_never, float *loss_host, void *stream);
 int mideq_mlp_get_grads(mideq_mlp *m, float *grads);           /* host [NPARAM] */
-/* order[B] (device) = a launch order for mideq_options.order from Y[B][3]
+/* order[B] (device) = a launch order for mideq_options.order from Y[B][4]
  * (device, the output of mideq_mlp_infer): predicted non-stiff trajectories
  * first, each class by decreasing predicted cost in 1/64-octave buckets
  * (order inside a bucket arbitrary).  Enqueued on `stream`. */

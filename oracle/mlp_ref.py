@@ -7,16 +7,17 @@ the same arithmetic in float64 numpy, with the BF16 rounding applied at the
 points where the kernels round (normalised inputs, weights, hidden
 activations, and the pre-activation gradients that feed matrix cores).
 
-Network: 16 inputs -> 64 -> 64 -> 64 -> 3, GELU (tanh form) after each
+Network: 16 inputs -> 64 -> 64 -> 64 -> 4, GELU (tanh form) after each
 hidden layer.  Parameter vector layout (float32): W1[16x64], b1[64],
-W2[64x64], b2[64], W3[64x64], b3[64], W4[64x3], b4[3]; W[in][out] row-major.
-Outputs: log10 dt0, stiffness logit, log2 cost.
+W2[64x64], b2[64], W3[64x64], b3[64], W4[64x4], b4[4]; W[in][out] row-major.
+Outputs: log10 dt0, stiffness logit, log2 cost, never-stiff logit.
 Loss: w_dt * mean((y0 - target)^2) + w_cls * mean(BCE-with-logits(y1, label))
-      + w_cost * mean((y2 - cost)^2)   (cost term dropped when cost is None).
+      + w_cost * mean((y2 - cost)^2) + w_never * mean(BCE-with-logits(y3, never))
+      (the cost and never terms dropped when cost / never is None).
 """
 import numpy as np
 
-NIN, NHID, NOUT = 16, 64, 3
+NIN, NHID, NOUT = 16, 64, 4
 SHAPES = [("W1", (NIN, NHID)), ("b1", (NHID,)), ("W2", (NHID, NHID)), ("b2", (NHID,)),
           ("W3", (NHID, NHID)), ("b3", (NHID,)), ("W4", (NHID, NOUT)), ("b4", (NOUT,))]
 NPARAM = sum(int(np.prod(s)) for _, s in SHAPES)
@@ -81,28 +82,41 @@ def forward(theta, X, mean, std, exact=False):
     return y, cache, W
 
 
+def _bce(y, c):
+    return np.maximum(y, 0) - y * c + np.log1p(np.exp(-np.abs(y)))
+
+
 def loss(theta, X, target, label, mean, std, w_dt=1.0, w_cls=1.0, exact=False, cost=None,
-         w_cost=1.0):
+         w_cost=1.0, never=None, w_never=1.0):
     y, _, _ = forward(theta, X, mean, std, exact)
     e = y[:, 0] - target
-    bce = np.maximum(y[:, 1], 0) - y[:, 1] * label + np.log1p(np.exp(-np.abs(y[:, 1])))
+    bce = _bce(y[:, 1], label)
     ec = np.zeros(len(X)) if cost is None else y[:, 2] - cost
     wc = 0.0 if cost is None else w_cost
-    return w_dt * np.mean(e * e) + w_cls * np.mean(bce) + wc * np.mean(ec * ec)
+    bn = np.zeros(len(X)) if never is None else _bce(y[:, 3], never)
+    wn = 0.0 if never is None else w_never
+    return (w_dt * np.mean(e * e) + w_cls * np.mean(bce) + wc * np.mean(ec * ec)
+            + wn * np.mean(bn))
 
 
 def loss_and_grad(theta, X, target, label, mean, std, w_dt=1.0, w_cls=1.0, exact=False,
-                  cost=None, w_cost=1.0):
+                  cost=None, w_cost=1.0, never=None, w_never=1.0):
     y, c, W = forward(theta, X, mean, std, exact)
     bf16_ = c["rnd"]
     B = len(X)
     e = y[:, 0] - target
     sig = 1.0 / (1.0 + np.exp(-y[:, 1]))
-    bce = np.maximum(y[:, 1], 0) - y[:, 1] * label + np.log1p(np.exp(-np.abs(y[:, 1])))
+    bce = _bce(y[:, 1], label)
     ec = np.zeros(B) if cost is None else y[:, 2] - cost
     wc = 0.0 if cost is None else w_cost
-    loss = w_dt * np.mean(e * e) + w_cls * np.mean(bce) + wc * np.mean(ec * ec)
-    dz4 = np.stack([2.0 * e * w_dt / B, (sig - label) * w_cls / B, 2.0 * ec * wc / B], axis=1)
+    nv = np.zeros(B) if never is None else np.asarray(never, dtype=np.float64)
+    wn = 0.0 if never is None else w_never
+    sig3 = 1.0 / (1.0 + np.exp(-y[:, 3]))
+    bn = _bce(y[:, 3], nv)
+    loss = (w_dt * np.mean(e * e) + w_cls * np.mean(bce) + wc * np.mean(ec * ec)
+            + wn * np.mean(bn))
+    dz4 = np.stack([2.0 * e * w_dt / B, (sig - label) * w_cls / B, 2.0 * ec * wc / B,
+                    (sig3 - nv) * wn / B], axis=1)
     g = {"W4": c["H3"].T @ bf16_(dz4), "b4": dz4.sum(0)}
     dz_b = bf16_(dz4)
     for l in (3, 2, 1):

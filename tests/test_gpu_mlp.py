@@ -53,26 +53,34 @@ def test_mlp_gradients_match_reference():
     target = rng.standard_normal(B).astype(np.float32)
     label = (rng.uniform(size=B) > 0.4).astype(np.float32)
     cost = (8.0 + 2.0 * rng.standard_normal(B)).astype(np.float32)
-    for with_cost in (True, False):
+    never = (rng.uniform(size=B) > 0.7).astype(np.float32)
+    for with_cost, with_never in ((True, True), (False, True), (True, False)):
         cd = torch.from_numpy(cost).cuda() if with_cost else None
+        nd = torch.from_numpy(never).cuda() if with_never else None
         loss = net.grad(Xd, torch.from_numpy(target).cuda(), torch.from_numpy(label).cuda(), cd,
-                        1.0, 0.5, 0.3)
+                        1.0, 0.5, 0.3, never=nd, w_never=0.8)
         g = net.get_grads()
         Lr, gr, _ = mlp_ref.loss_and_grad(theta, X, target.astype(np.float64),
                                           label.astype(np.float64), mean, std, 1.0, 0.5,
                                           cost=cost.astype(np.float64) if with_cost else None,
-                                          w_cost=0.3)
-        assert abs(loss - Lr) < 1e-4 * abs(Lr) + 1e-6, (with_cost, loss, Lr)
+                                          w_cost=0.3,
+                                          never=never.astype(np.float64) if with_never else None,
+                                          w_never=0.8)
+        case = (with_cost, with_never)
+        assert abs(loss - Lr) < 1e-4 * abs(Lr) + 1e-6, (case, loss, Lr)
         k = 0
         for name, shp in mlp_ref.SHAPES:
             n = int(np.prod(shp))
             a, b = g[k:k + n], gr[k:k + n]
             rel = np.linalg.norm(a - b) / (np.linalg.norm(b) + 1e-30)
-            assert rel < 1e-2, (with_cost, name, rel)
+            assert rel < 1e-2, (case, name, rel)
             k += n
+        # a missing target: that head's column of W4 and its bias get no gradient
+        W4 = g[-4 - 4 * 64:-4].reshape(64, 4)
         if not with_cost:
-            # no cost target: the cost column of W4 and its bias get no gradient
-            assert g[-1] == 0.0 and np.all(g[-3 - 3 * 64 + 2:-3:3] == 0.0)
+            assert g[-2] == 0.0 and np.all(W4[:, 2] == 0.0)
+        if not with_never:
+            assert g[-1] == 0.0 and np.all(W4[:, 3] == 0.0)
 
 
 def test_mlp_training_lowers_loss():
@@ -85,20 +93,23 @@ def test_mlp_training_lowers_loss():
     target = (np.sin(X[:, 0]) + 0.5 * X[:, 1] * X[:, 2]).astype(np.float32)
     label = (X[:, 3] + X[:, 4] > 0).astype(np.float32)
     cost = (6.0 + X[:, 5] - 0.5 * X[:, 6] ** 2).astype(np.float32)
+    never = (X[:, 7] - X[:, 8] > 0.5).astype(np.float32)
     net = MLP(seed=1)
     net.set_norm(np.zeros(16), np.ones(16))
     Xd = torch.from_numpy(X).cuda()
     td = torch.from_numpy(target).cuda()
     ld = torch.from_numpy(label).cuda()
     cd = torch.from_numpy(cost).cuda()
-    l0 = net.grad(Xd, td, ld, cd)
-    losses = net.fit(Xd, td, ld, cd, epochs=6, batch=4096, lr=3e-3)
-    l1 = net.grad(Xd, td, ld, cd)
+    nd = torch.from_numpy(never).cuda()
+    l0 = net.grad(Xd, td, ld, cd, never=nd)
+    losses = net.fit(Xd, td, ld, cd, epochs=6, batch=4096, lr=3e-3, never=nd)
+    l1 = net.grad(Xd, td, ld, cd, never=nd)
     assert l1 < 0.2 * l0, (l0, l1, losses[-5:])
-    # classification accuracy on the training data
+    # classification accuracy of both logit heads on the training data
     Y = net.infer(Xd).cpu().numpy()
     acc = np.mean((Y[:, 1] > 0) == (label > 0.5))
     assert acc > 0.95
+    assert np.mean((Y[:, 3] > 0) == (never > 0.5)) > 0.95
     # the cost head ranks trajectories: rank correlation with the target (the
     # same training emulated with oracle/mlp_ref.py reaches 0.961 / 0.90)
     rk = lambda v: np.argsort(np.argsort(v))
@@ -114,7 +125,7 @@ def test_launch_order_kernel_is_sorted_permutation(B):
     from mideq.learned import MLP, order_keys
     say(f"launch order B={B}")
     rng = np.random.default_rng(B)
-    Y = np.zeros((B, 3), np.float32)
+    Y = np.zeros((B, 4), np.float32)
     Y[:, 1] = rng.standard_normal(B)
     Y[:, 2] = rng.uniform(-2.0, 40.0, B)
     Y[::97, 2] = np.nan

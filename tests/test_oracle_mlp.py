@@ -38,14 +38,17 @@ def test_analytic_gradient_matches_finite_differences():
     target = rng.standard_normal(B)
     label = (rng.uniform(size=B) > 0.5).astype(np.float64)
     cost = 8.0 + rng.standard_normal(B)
+    never = (rng.uniform(size=B) > 0.6).astype(np.float64)
     theta = mlp_ref.init_params(1).astype(np.float64)
     theta += 0.01 * rng.standard_normal(theta.size)  # nonzero biases
-    kw = dict(exact=True, cost=cost, w_cost=0.4)
+    kw = dict(exact=True, cost=cost, w_cost=0.4, never=never, w_never=0.6)
     L0, g, _ = mlp_ref.loss_and_grad(theta, X, target, label, mean, std, 1.0, 0.7, **kw)
     idx = rng.choice(theta.size, 60, replace=False)
-    # the three output biases, the last output column of W4, and W1/b1 entries
+    # the four output biases, entries of the last two output columns of W4,
+    # and W1/b1 entries
     P = mlp_ref.NPARAM
-    idx = np.concatenate([idx, [P - 1, P - 2, P - 3, P - 4, P - 7, 1024, 1100]])
+    idx = np.concatenate([idx, [P - 1, P - 2, P - 3, P - 4, P - 5, P - 6, P - 9, P - 10,
+                                1024, 1100]])
     h = 1e-6
     for i in idx:
         tp = theta.copy()
@@ -56,6 +59,7 @@ def test_analytic_gradient_matches_finite_differences():
               - mlp_ref.loss(tm, X, target, label, mean, std, 1.0, 0.7, **kw)) / (2 * h)
         assert abs(fd - g[i]) <= 1e-6 + 1e-5 * abs(fd), (i, fd, g[i])
     assert np.isfinite(L0)
-    # without a cost target the cost head gets no gradient
+    # without cost and never targets those heads get no gradient (bias and
+    # W4 column: the last entry of each W4 row for the never head)
     _, g0, _ = mlp_ref.loss_and_grad(theta, X, target, label, mean, std, 1.0, 0.7, exact=True)
-    assert g0[P - 1] == 0.0 and g0[P - 4] == 0.0
+    assert g0[P - 1] == 0.0 and g0[P - 2] == 0.0 and g0[P - 5] == 0.0 and g0[P - 6] == 0.0
