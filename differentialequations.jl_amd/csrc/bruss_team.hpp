@@ -126,6 +126,7 @@ struct BTeam {
   double* Wb;    // team LDS (Rosenbrock kernels): W band rows
   double* Wr;    //   their entries in the last kT columns
   double* Wd;    //   the last kT rows, dense
+  bool lu_swapped;  // the last factorization interchanged rows
 
   // Lane tl owns grid points j = tl + L*g (g < C/2), each as the pair of
   // components (u_j, v_j) = (2j, 2j + 1) in slots (2g, 2g + 1), so both rows
@@ -672,6 +673,7 @@ struct BTeam {
     if (kAuto) eig = team_max<L>(best);
     team_sync();
     int sing_k = n, pois_k = n;
+    lu_swapped = false;
     // columns k <= nb - 5: the candidates and the pattern have their fixed
     // shape (rows k, k+1, k+2 and the kT dense rows; columns k..k+4 and the
     // border), so column k's seven candidate values load at once
@@ -708,6 +710,7 @@ struct BTeam {
       }
       pk = team_uni<L>(pk);
       if (pk != k) {
+        lu_swapped = true;
         lu_column(k, 2, 4, 7, 9, pk, d, sing_k, pois_k);
         continue;
       }
@@ -760,7 +763,9 @@ struct BTeam {
           d = v;
         }
       }
-      lu_column(k, nbc, nwc, nc, np, team_uni<L>(pk), d, sing_k, pois_k);
+      pk = team_uni<L>(pk);
+      lu_swapped = lu_swapped || pk != k;
+      lu_column(k, nbc, nwc, nc, np, pk, d, sing_k, pois_k);
     }
     pois_k = team_min_int<L>(pois_k);
     if (sing_k < n && sing_k <= pois_k) return 1;
@@ -880,8 +885,9 @@ struct BTeam {
 
   // position j's pivot row, multipliers of rows j+1, j+2 and the dense rows,
   // and the right-hand side entry j + 3
+  template <bool PIV>
   __device__ __forceinline__ void fwd_load(FwdFactors& f, int j) const {
-    f.pj = ldsi[j];
+    if constexpr (PIV) f.pj = ldsi[j];
     f.l1 = Wb[(j + 1) * 8 + 1];
     f.l2 = Wb[(j + 2) * 8];
     f.d0 = Wd[j];
@@ -891,13 +897,17 @@ struct BTeam {
     f.t3 = ldsd[j + 3];
   }
 
+  // PIV: the factorization interchanged rows (else every pj == j).  LAST:
+  // position j = nb - 3 = n - 7 (no right-hand side entry j + 3 < nb, no
+  // non-finite check); every earlier position has both.
+  template <bool PIV, bool LAST>
   __device__ __forceinline__ void fwd_step(int j, const FwdFactors& f, FwdFactors& next,
                                            FwdChain& c) {
-    const int pj = team_uni<L>(f.pj);
-    const double bn = (j + 3 < nb) ? f.t3 : 0.0;
-    fwd_load(next, j + 1);
+    const int pj = PIV ? team_uni<L>(f.pj) : j;
+    const double bn = LAST ? 0.0 : f.t3;
+    fwd_load<PIV>(next, j + 1);
     double b0 = c.b0, b1 = c.b1, b2 = c.b2, bd0 = c.bd0, bd1 = c.bd1, bd2 = c.bd2, bd3 = c.bd3;
-    if (pj != j) {
+    if (PIV && pj != j) {
       double t = b0;
       if (pj == j + 1) {
         b0 = b1;
@@ -928,7 +938,26 @@ struct BTeam {
     c.bd1 = fma(-f.d1, xj, bd1);
     c.bd2 = fma(-f.d2, xj, bd2);
     c.bd3 = fma(-f.d3, xj, bd3);
-    if (j < n - 7) c.nonfin = fma(xj, 0.0, c.nonfin);
+    if (!LAST) c.nonfin = fma(xj, 0.0, c.nonfin);
+  }
+
+  // the forward chain over positions 0 .. nb - 3
+  template <bool PIV>
+  __device__ __forceinline__ void fwd_chain(FwdChain& st) {
+    FwdFactors fa, fb;
+    fwd_load<PIV>(fa, 0);
+    const int jl = nb - 3;
+    int j = 0;
+    for (; j + 1 < jl; j += 2) {
+      fwd_step<PIV, false>(j, fa, fb, st);
+      fwd_step<PIV, false>(j + 1, fb, fa, st);
+    }
+    if (j < jl) {
+      fwd_step<PIV, false>(j, fa, fb, st);
+      fwd_step<PIV, true>(jl, fb, fa, st);
+    } else {
+      fwd_step<PIV, true>(jl, fa, fb, st);
+    }
   }
 
   // row i's window entries, reciprocal pivot and right-hand side
@@ -941,7 +970,9 @@ struct BTeam {
     f.y = ldsd[i];
   }
 
-  // row i <= nb - 5: all four window columns are band columns
+  // row i <= nb - 5: all four window columns are band columns.  CHK: i >= 5
+  // (feeds the non-finite check); row 0 loads nothing further.
+  template <bool CHK>
   __device__ __forceinline__ void back_step(int i, const BackFactors& f, BackFactors& next,
                                             BackChain& c) {
     back_load(next, i > 0 ? i - 1 : 0);
@@ -953,7 +984,7 @@ struct BTeam {
     const double xi = y * f.ri;
     if (tl == 0) ldsd[i] = xi;
     // column i: rows above i-4 hold exact zeros
-    if (i >= 5) c.nonfin = fma(xi, 0.0, c.nonfin);
+    if (CHK) c.nonfin = fma(xi, 0.0, c.nonfin);
     c.x4 = c.x3;
     c.x3 = c.x2;
     c.x2 = c.x1;
@@ -991,17 +1022,12 @@ struct BTeam {
     st.bd2 = bd2;
     st.bd3 = bd3;
     st.nonfin = 0.0;
-    {
-      FwdFactors fa, fb;
-      fwd_load(fa, 0);
-      const int jl = nb - 3;
-      int j = 0;
-      for (; j + 1 <= jl; j += 2) {
-        fwd_step(j, fa, fb, st);
-        fwd_step(j + 1, fb, fa, st);
-      }
-      if (j <= jl) fwd_step(j, fa, fb, st);
-    }
+    // without an interchange (the usual case: W = J - I/(gamma dt) is
+    // diagonally dominant on this problem) the pivot rows are not read
+    if (lu_swapped)
+      fwd_chain<true>(st);
+    else
+      fwd_chain<false>(st);
     b0 = st.b0;
     b1 = st.b1;
     bd0 = st.bd0;
@@ -1086,13 +1112,26 @@ struct BTeam {
       bs.x4 = x4;
       bs.nonfin = 0.0;
       BackFactors ba, bb;
-      int i = nb - 5;
+      int i = nb - 5;  // >= 7
       back_load(ba, i);
-      for (; i - 1 >= 0; i -= 2) {
-        back_step(i, ba, bb, bs);
-        back_step(i - 1, bb, ba, bs);
+      for (; i - 1 >= 5; i -= 2) {
+        back_step<true>(i, ba, bb, bs);
+        back_step<true>(i - 1, bb, ba, bs);
       }
-      if (i >= 0) back_step(i, ba, bb, bs);
+      if (i >= 5) {  // i == 5
+        back_step<true>(5, ba, bb, bs);
+        back_step<false>(4, bb, ba, bs);
+        back_step<false>(3, ba, bb, bs);
+        back_step<false>(2, bb, ba, bs);
+        back_step<false>(1, ba, bb, bs);
+        back_step<false>(0, bb, ba, bs);
+      } else {  // i == 4
+        back_step<false>(4, ba, bb, bs);
+        back_step<false>(3, bb, ba, bs);
+        back_step<false>(2, ba, bb, bs);
+        back_step<false>(1, bb, ba, bs);
+        back_step<false>(0, ba, bb, bs);
+      }
       bad = bad || (bs.nonfin != bs.nonfin);
     }
     team_sync();

@@ -93,7 +93,7 @@ def test_mlp_training_lowers_loss():
     target = (np.sin(X[:, 0]) + 0.5 * X[:, 1] * X[:, 2]).astype(np.float32)
     label = (X[:, 3] + X[:, 4] > 0).astype(np.float32)
     cost = (6.0 + X[:, 5] - 0.5 * X[:, 6] ** 2).astype(np.float32)
-    never = (X[:, 7] - X[:, 8] > 0.5).astype(np.float32)
+    never = (X[:, 7] - X[:, 8] > 0).astype(np.float32)
     net = MLP(seed=1)
     net.set_norm(np.zeros(16), np.ones(16))
     Xd = torch.from_numpy(X).cuda()
