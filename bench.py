@@ -50,10 +50,14 @@ FORMS = {
     "mixed": ("mixed", 1e-6, "lane", "float32", 0),
     "tuned": ("fp64", 1e-8, "lane", "float64", 1),
     "tuned_persistent": ("fp64", 1e-8, "persistent", "float64", 1),
+    "tuned_traj": ("fp64", 1e-8, "lane", "float64", 1),
     # a looser tolerance than the configuration states (reported beside the
     # headline, never chosen as it)
     "tuned_tol": ("fp64", 1e-7, "lane", "float64", 1),
 }
+# Layout of the saved values: [100][3][N] (0, the default) or trajectory-major
+# [N][100][3] (1: a saved point's three values are one contiguous store).
+SAVE_LAYOUT = {"tuned_traj": 1}
 # The configuration's tolerance per precision: FP64 at 1e-8, FP32 stages at
 # 1e-6.  Only forms at their precision's stated tolerance can be the headline.
 CONFIG_TOL = {"fp64": 1e-8, "mixed": 1e-6}
@@ -65,7 +69,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--ntraj", type=int, default=1 << 20)
-    ap.add_argument("--forms", default="plain,mixed,tuned,tuned_persistent,tuned_tol")
+    ap.add_argument("--forms", default="plain,mixed,tuned,tuned_traj,tuned_persistent,tuned_tol")
     ap.add_argument("--other", default="robertson,brusselator,mix",
                     help="other configurations to time (comma list, '' for none)")
     ap.add_argument("--bruss-forms", default="plain,tuned,mixed,auto,auto_tuned,rodas5",
@@ -122,7 +126,8 @@ class DeviceEnsemble:
             self.us[dtype] = self.torch.empty((self.ns, 3, self.N), dtype=tdt, device="cuda:0")
         return self.us[dtype]
 
-    def structs(self, precision, tol, schedule, save_dtype, with_save=True, controller=0):
+    def structs(self, precision, tol, schedule, save_dtype, with_save=True, controller=0,
+                save_layout=0):
         from mideq import _abi
         P = _abi.Problem()
         P.model = _abi.MODEL["lorenz"]
@@ -145,6 +150,7 @@ class DeviceEnsemble:
             O.saveat = self.saveat.data_ptr()
             O.ns = self.ns
             O.save_dtype = 0 if save_dtype == "float64" else 1
+            O.save_layout = save_layout
         out = _abi.Outputs()
         out.us = self.us_buf(save_dtype).data_ptr() if with_save else None
         out.u_final = self.u_final.data_ptr()
@@ -158,7 +164,8 @@ class DeviceEnsemble:
 
 def run_form(torch, ctx, ens, form, steps, warmup):
     precision, tol, schedule, save_dtype, controller = FORMS[form]
-    P, O, out = ens.structs(precision, tol, schedule, save_dtype, controller=controller)
+    P, O, out = ens.structs(precision, tol, schedule, save_dtype, controller=controller,
+                            save_layout=SAVE_LAYOUT.get(form, 0))
     # a dedicated stream: the kernels and the timing events share it
     stream = ens.stream
     sp = C.c_void_p(stream.cuda_stream)
@@ -183,6 +190,7 @@ def run_form(torch, ctx, ens, form, steps, warmup):
     attempted = int(na.sum()) + int(nr.sum())
     return dict(form=form, precision=precision, tol=tol, schedule=schedule,
                 save_dtype=save_dtype, controller=["reference", "fast"][controller],
+                save_layout=["[100][3][N]", "[N][100][3]"][SAVE_LAYOUT.get(form, 0)],
                 wall_s=wall, kernel_ms=kms,
                 kernel_ms_mean=statistics.fmean(kms), attempted_steps=attempted,
                 naccept_mean=float(na.mean()), nreject_mean=float(nr.mean()),
@@ -716,7 +724,7 @@ def main():
         "data": "synthetic (seeded counter-based generator)",
         "config": {"workload": "Lorenz ensemble, 1,048,576 trajectories, sigma/rho/beta +-10% "
                                "seeded, u0=[1,0,0], tspan (0,10), Tsit5, saveat 100 uniform "
-                               "points, SoA output 100x3xN",
+                               "points, FP64 output " + h["save_layout"],
                    "ntraj": a.ntraj, "headline_form": head,
                    "headline_tol": h["tol"],
                    "gate": "final-state rel err vs FP64 tol-1e-12 solve: median <= 1e-6 and "

@@ -112,11 +112,12 @@ struct BTeam {
   float k1f[C];
   double t, tcomp, tf, dt, dtmax, qold, qoldpow, qsteady_max, eig, dtfirst;
   double lqold;  // fast controller: log2 of the previous accepted EEst (>= log2 1e-4)
-  double tsnap;  // 100 eps(tf): a step ending this close to tf ends at tf
+  double tsnap;  // 100 eps(tf): a forward step ending this close to tf ends at tf
+  double tdir;   // +1, or -1 for tf < t0: integrated backward with steps of -dt (dt > 0)
   int64_t iter;
   int32_t naccept, nreject, nf, nstiff, count, nanrej;
   int32_t isave;  // next saveat point (SAVE builds)
-  double tnext;   // its time, +inf once every point is saved
+  double tnext;   // its time, +inf (-inf backward) once every point is saved
   int8_t rc, mode;
   double* ldsd;  // team LDS: exchange vector (BRUSS_NMAX doubles)
   double* ldsr;  // team LDS: reciprocal pivots
@@ -233,8 +234,9 @@ struct BTeam {
     dt0 = fmin(dt0, dtmax);
     if (dt0 < 10.0 * 2.220446049250313e-16) return smalldt;
     double u1[C], f1[C];
+    const double sdt0 = tdir * dt0;  // the trial step in the direction of integration
 #pragma unroll
-    for (int m = 0; m < C; m++) u1[m] = fma(dt0, k1[m], u[m]);
+    for (int m = 0; m < C; m++) u1[m] = fma(sdt0, k1[m], u[m]);
     rhs64(f1, u1);
     nf += 1;
     int same = 1;
@@ -279,6 +281,7 @@ struct BTeam {
     B1f = Bf + 1.0f;
     double t0 = a.t0 ? a.t0[i] : a.tspan0;
     tf = a.tf ? a.tf[i] : a.tspan1;
+    tdir = tf < t0 ? -1.0 : 1.0;
     t = t0;
     tcomp = 0.0;
     dtmax = a.dtmax > 0 ? a.dtmax : fabs(tf - t0);
@@ -317,24 +320,39 @@ struct BTeam {
     qold = kQoldInit;
     qoldpow = ctl_pow(kQoldInit, kBeta2);
     lqold = kL2QoldInit;
-    // the step loop runs while t < tf, so max(t, tf) there is tf
+    // a step ending within 100 eps(max(t, tf)) of tf ends at tf, t the
+    // step's start: forward t < tf in the loop, so the window is fixed;
+    // backward max(t, tf) = t and the window is taken per step
     tsnap = 100.0 * jl_eps(tf);
-    // a zero-length span is done; a reversed (or NaN) span is not integrated
+    // a zero-length span is done; a NaN end is not integrated
     if (tf == t0)
       rc = RC_SUCCESS;
-    else if (!(tf > t0))
+    else if (!(tf > t0) && !(tf < t0))
       rc = RC_INITIALFAILURE;
     if constexpr (SAVE) {
-      // saveat points before t0 are unreachable; a point at t0 is u0 itself
-      if (tf >= t0) {
-        double nanv[C];
+      // saveat points (ascending) before t0 in the direction of integration
+      // are unreachable; a point at t0 is u0 itself.  Backward, the points
+      // are visited from the last one down.
+      double nanv[C];
 #pragma unroll
-        for (int m = 0; m < C; m++) nanv[m] = __builtin_nan("");
+      for (int m = 0; m < C; m++) nanv[m] = __builtin_nan("");
+      if (tdir > 0.0) {
         while (isave < a.ns && a.saveat[isave] < t0) store_save(a, isave++, nanv);
         while (isave < a.ns && a.saveat[isave] == t0) store_save(a, isave++, u);
+      } else {
+        isave = a.ns - 1;
+        while (isave >= 0 && a.saveat[isave] > t0) store_save(a, isave--, nanv);
+        while (isave >= 0 && a.saveat[isave] == t0) store_save(a, isave--, u);
       }
-      tnext = isave < a.ns ? a.saveat[isave] : __builtin_inf();
+      set_tnext(a);
     }
+  }
+
+  __device__ __forceinline__ void set_tnext(const BrussArgs& a) {
+    if (tdir > 0.0)
+      tnext = isave < a.ns ? a.saveat[isave] : __builtin_inf();
+    else
+      tnext = isave >= 0 ? a.saveat[isave] : -__builtin_inf();
   }
 
   // ---------------------------------------------------------------- saveat
@@ -1275,7 +1293,8 @@ struct BTeam {
       rc = RC_DTLESSTHANMIN;
       return;
     }
-    dt = fmin(dt, tf - t);
+    dt = fmin(dt, tdir * (tf - t));
+    const double sdt = tdir * dt;  // the signed step
 
     double u1[C];
     double EEst = 0.0;
@@ -1291,14 +1310,14 @@ struct BTeam {
 #pragma unroll
         for (int m = 0; m < C; m++) k1save[m] = k1[m];
         if constexpr (!MIX)
-          EEst = tsit5_step64(dt, u1, a.abstol, a.reltol, ks);
+          EEst = tsit5_step64(sdt, u1, a.abstol, a.reltol, ks);
         else
-          EEst = tsit5_step_mix(dt, u1, k7f, a.abstol, a.reltol, ksf);
+          EEst = tsit5_step_mix(sdt, u1, k7f, a.abstol, a.reltol, ksf);
       }
       nf += 6;
     } else {
       if constexpr (kHasRodas) {
-        if (!rodas5_step(dt, u1, EEst, a.abstol, a.reltol, Kr)) EEst = 2.0;
+        if (!rodas5_step(sdt, u1, EEst, a.abstol, a.reltol, Kr)) EEst = 2.0;
       }
       nf += 8;
     }
@@ -1350,25 +1369,27 @@ struct BTeam {
       }
       double ttmp;
       if (!MIX) {
-        ttmp = t + dt;
+        ttmp = t + sdt;
       } else {
-        double y = dt - tcomp;
+        double y = sdt - tcomp;
         ttmp = t + y;
         tcomp = (ttmp - t) - y;
       }
-      if (fabs(ttmp - tf) < tsnap) ttmp = tf;
+      const double snapw = tdir > 0.0 ? tsnap : 100.0 * jl_eps(fmax(t, tf));
+      if (fabs(ttmp - tf) < snapw) ttmp = tf;
       const double tprev = t;
       t = ttmp;
       if constexpr (SAVE) {
-        // savevalues!: every saveat point in (tprev, t], from the step's
-        // dense output (Tsit5's interpolant or the Rosenbrock one)
-        while (t >= tnext) {
+        // savevalues!: every saveat point in (tprev, t] (backward [t, tprev)),
+        // from the step's dense output (Tsit5's interpolant or the Rosenbrock
+        // one); tnext = +-inf once every point is saved
+        while (tdir * (t - tnext) >= 0.0) {
           double val[C];
           if (tnext == t) {
 #pragma unroll
             for (int m = 0; m < C; m++) val[m] = u1[m];
           } else {
-            const double th = (tnext - tprev) / dt;
+            const double th = (tnext - tprev) / sdt;
             if (tsit_step) {
               double bt[7];
               tsit5_bthetas(th, bt);
@@ -1378,7 +1399,7 @@ struct BTeam {
                   double acc = bt[6] * ks[6][m];
 #pragma unroll
                   for (int j = 5; j >= 0; j--) acc = fma(bt[j], ks[j][m], acc);
-                  val[m] = fma(dt, acc, u[m]);
+                  val[m] = fma(sdt, acc, u[m]);
                 }
               } else {
                 float bf[7];
@@ -1389,7 +1410,7 @@ struct BTeam {
                   float acc = bf[6] * ksf[6][m];
 #pragma unroll
                   for (int j = 5; j >= 0; j--) acc = fmaf(bf[j], ksf[j][m], acc);
-                  val[m] = fma(dt, (double)acc, u[m]);
+                  val[m] = fma(sdt, (double)acc, u[m]);
                 }
               }
             } else {
@@ -1404,8 +1425,8 @@ struct BTeam {
             }
           }
           store_save(a, isave, val);
-          isave++;
-          tnext = isave < a.ns ? a.saveat[isave] : __builtin_inf();
+          isave += tdir > 0.0 ? 1 : -1;
+          set_tnext(a);
         }
       }
 #pragma unroll
@@ -1434,7 +1455,7 @@ struct BTeam {
       else
         dt = dt * fast_exp2(-fmin(kL2QminInv, l11 - kL2Gamma));
     }
-    if (!(t < tf)) rc = RC_SUCCESS;
+    if (!(tdir * (tf - t) > 0.0)) rc = RC_SUCCESS;
   }
 
   __device__ __forceinline__ void finish(const BrussArgs& a) {
@@ -1444,7 +1465,10 @@ struct BTeam {
       double nanv[C];
 #pragma unroll
       for (int m = 0; m < C; m++) nanv[m] = __builtin_nan("");
-      for (int k = isave; k < a.ns; k++) store_save(a, k, nanv);
+      if (tdir > 0.0)
+        for (int k = isave; k < a.ns; k++) store_save(a, k, nanv);
+      else
+        for (int k = isave; k >= 0; k--) store_save(a, k, nanv);
     }
     if (a.u_final)
 #pragma unroll

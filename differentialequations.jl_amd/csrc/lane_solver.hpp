@@ -45,7 +45,7 @@ struct LaneArgs {
   const double* dt0;
   const int8_t* alg_hint;
   int32_t qsteady_default;
-  int32_t pad0;
+  int32_t save_layout;  // saved values [ns][N][ntraj] (0) or [ntraj][ns][N] (1)
   void* us;
   double* u_final;
   double* t_final;
@@ -344,6 +344,23 @@ struct Lane {
 
   // ------------------------------------------------------------ helpers
   __device__ __forceinline__ void store_save(const LaneArgs& a, int k, const double* v) {
+    if (a.save_layout) {
+      // element (i, k, 0) of the [ntraj][ns][N] output: the point's N values
+      // are contiguous, one partial line per lane and point; ns * N < 2^32
+      // and ntraj < 2^32 (checked by the host)
+      const int64_t o =
+          (int64_t)((uint64_t)(uint32_t)i * (uint32_t)(a.ns * N) + (uint64_t)(uint32_t)(k * N));
+      if (a.save_f32) {
+        float* out = (float*)a.us + o;
+#pragma unroll
+        for (int c = 0; c < N; c++) out[c] = (float)v[c];
+      } else {
+        double* out = (double*)a.us + o;
+#pragma unroll
+        for (int c = 0; c < N; c++) out[c] = v[c];
+      }
+      return;
+    }
     // element (k, 0, i) of the [ns][N][ntraj] output; N * ntraj < 2^32 (checked
     // by the host), so the offset is one 32 x 32 -> 64-bit multiply-add
     const int64_t NT = a.ntraj;

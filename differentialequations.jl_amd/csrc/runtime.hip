@@ -703,10 +703,8 @@ static int validate(mideq_ctx* c, const mideq_problem* P, const mideq_options* O
                    "ensemble", (long long)((int64_t)P->n * P->ntraj));
   if (O->save_dtype < 0 || O->save_dtype > 1) return set_err(c, MIDEQ_E_ARG, "bad save_dtype");
   if (O->save_layout < 0 || O->save_layout > 1) return set_err(c, MIDEQ_E_ARG, "bad save_layout");
-  if (O->save_layout == 1 && P->model != MODEL_BRUSSELATOR)
-    return set_err(c, MIDEQ_E_UNSUPPORTED,
-                   "save_layout 1 (trajectory-major) is provided by the team kernel (brusselator) "
-                   "only");
+  if (O->save_layout == 1 && O->ns > 0 && (int64_t)O->ns * P->n >= ((int64_t)1 << 32))
+    return set_err(c, MIDEQ_E_UNSUPPORTED, "save_layout 1 needs ns * n < 2^32");
   if (O->schedule < 0 || O->schedule > 2) return set_err(c, MIDEQ_E_ARG, "bad schedule");
   if (O->controller < 0 || O->controller > 1) return set_err(c, MIDEQ_E_ARG, "bad controller");
   if (O->auto_passes < 0 || O->auto_passes > 3) return set_err(c, MIDEQ_E_ARG, "bad auto_passes");
@@ -938,6 +936,7 @@ static int run_solver(mideq_ctx* c, const mideq_problem* P, const mideq_options*
   a.dt0 = O->dt0;
   a.alg_hint = O->alg_hint;
   a.qsteady_default = O->qsteady_default;
+  a.save_layout = O->save_layout;
   a.us = out->us;
   a.u_final = out->u_final;
   a.t_final = out->t_final;
@@ -1197,7 +1196,8 @@ extern "C" int mideq_ensemble_solve(mideq_ctx* c, const mideq_problem* P,
     for (int64_t i = 0; i < N; i++) {
       double ni = P->nvec ? P->nvec[i] : n;
       double span = (P->tf ? P->tf[i] : P->tspan1) - (P->t0 ? P->t0[i] : P->tspan0);
-      cost[i] = ni * (ni * 0.5) * (ni * 0.5) * std::fabs(span);
+      const double ci = ni * (ni * 0.5) * (ni * 0.5) * std::fabs(span);
+      cost[i] = ci == ci ? ci : 0.0;  // a NaN span is not integrated (and would break the sort)
     }
     std::stable_sort(order.begin(), order.end(),
                      [&](int64_t x, int64_t y) { return cost[x] > cost[y]; });

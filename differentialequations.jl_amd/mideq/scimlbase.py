@@ -375,8 +375,8 @@ class EnsembleMI355X:
     3 = trust alg_hint: plain Tsit5 for the trajectories hinted non-stiff,
     each capped at iter_cap[i] attempted steps, then AutoSwitch from t0 for
     the capped-out and the hinted-stiff ones.  save_layout: "soa" (saved
-    values [ns, n, N], every model) or "trajectory" ([N, ns, n], one
-    trajectory's saved states contiguous; the Brusselator team kernel)."""
+    values [ns, n, N]) or "trajectory" ([N, ns, n], one trajectory's saved
+    states contiguous, as upstream's per-trajectory sol.u)."""
     precision: str = "fp64"
     schedule: str = "auto"
     device: int = 0

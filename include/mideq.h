@@ -158,10 +158,11 @@ typedef struct {
                            on the device entry, entries outside [0, ntraj) are skipped.
                            Overrides the built-in cost order of the Brusselator queue. */
   int32_t reduction;    /* mideq_reduction over the trajectories' output_func values */
-  int32_t save_layout;  /* 0: us[(k*n + c)*ntraj + i] (trajectory fastest, every model);
+  int32_t save_layout;  /* 0: us[(k*n + c)*ntraj + i] (trajectory fastest);
                            1: us[(i*ns + k)*n + c] (component fastest: one trajectory's saved
-                           states contiguous, as Julia's per-trajectory sol.u; the team
-                           kernel's stores are then contiguous) -- brusselator only */
+                           states contiguous, as Julia's per-trajectory sol.u; a point's
+                           values are then one contiguous store per lane or team; needs
+                           ns*n < 2^32) -- every model */
   const int32_t *iter_cap; /* [ntraj] auto_passes = 3: iteration cap of the plain Tsit5 pass
                               per trajectory (e.g. a multiple of the predicted cost; 0
                               sends it to AutoSwitch before its first step, starting in

@@ -30,6 +30,8 @@ def test_forms_and_gate():
     # reference controller; every measured traffic entry names a form
     assert bench.FORMS["plain"] == ("fp64", 1e-8, "lane", "float64", 0)
     assert set(bench.MEASURED_TRAFFIC) <= set(bench.FORMS)
+    assert set(bench.SAVE_LAYOUT) <= set(bench.FORMS)
+    assert set(bench.SAVE_LAYOUT.values()) <= {0, 1}
     assert bench.FLOPS_PER_STEP == 262 and bench.FLOPS_PER_SAVE == 98
 
 

@@ -269,9 +269,46 @@ def test_bruss_saveat_f32_storage_and_autotuned(oracle):
     assert_identical(g1, c, nvec=w["nvec"])
 
 
-def test_trajectory_layout_refused_for_lane_models():
-    say("save_layout trajectory on a lane model: refused")
-    w = workloads.lorenz(8, seed=82)
-    with pytest.raises(Exception, match="team kernel"):
-        gpu("lorenz", w["u0"], w["p"], 0.0, 1.0, saveat=np.array([0.5]),
-            save_layout="trajectory")
+
+
+@pytest.mark.parametrize("alg,lanes,precision,layout", [
+    ("tsit5", 64, "fp64", None), ("tsit5", 32, "mixed", "soa"), ("rodas5", 64, "fp64", "soa"),
+    ("rodas5", 32, "fp64", None), ("auto", 64, "fp64", "trajectory"),
+    ("rodas5p", 32, "mixed", "trajectory")])
+def test_bruss_backward_spans_bitexact(oracle, alg, lanes, precision, layout):
+    """Instances integrated forward (0 -> tf) and backward (tf -> 0.3 tf, a
+    span short enough for the diffusion run in reverse to stay bounded) in
+    one ragged batch on the team kernel, plus a zero-length span (Success at
+    u0) and a NaN end (InitialFailure at u0); with saveat, backward instances
+    fill the points in [tf, t0] from the last one down.  Bit-exact against
+    the oracle."""
+    N = 48
+    say(f"brusselator backward spans {alg} {precision} lanes={lanes} saveat={layout} N={N}")
+    w = batch_range(N, seed=83, lo=8, hi=24, tf_max=0.3)
+    back = np.arange(N) % 2 == 1
+    t0 = np.where(back, w["tf"], 0.0)
+    tf = np.where(back, 0.3 * w["tf"], w["tf"])
+    tf[4], tf[6] = t0[4], np.nan
+    sa = np.linspace(0.0, 0.3, 7) if layout else None
+    tol = dict(abstol=1e-8, reltol=1e-6)
+    g = gpu("brusselator", w["u0"], w["p"], t0, tf, nvec=w["nvec"], lanes=lanes, alg=alg,
+            precision=precision, schedule="persistent", saveat=sa,
+            save_layout=layout or "soa", **tol)
+    c = cpu(oracle, "brusselator", w["u0"], w["p"], t0, tf, nvec=w["nvec"], norm_lanes=lanes,
+            alg=alg, precision=precision, saveat=sa, **tol)
+    assert c["retcode"][6] == 6 and (np.delete(c["retcode"], 6) == 1).all()
+    assert (c["naccept"][back] > 0).all() and c["naccept"][4] == 0
+    if layout:
+        us = soa(g.us) if layout == "trajectory" else g.us
+        np.testing.assert_array_equal(us, c["saveat"])
+        g.us = None
+        # backward: points in [0.3 t0, t0] saved, those below NaN
+        k = 1
+        n = int(w["nvec"][k])
+        reached = (sa >= tf[k]) & (sa <= t0[k])
+        assert not np.isnan(us[reached, :n, k]).any() and np.isnan(us[~reached, :n, k]).all()
+    assert_identical(g, c, nvec=w["nvec"])
+    for k in (4, 6):
+        n = int(w["nvec"][k])
+        np.testing.assert_array_equal(g.u_final[:n, k], w["u0"][:n, k])
+    assert (g.t_final[back] == tf[back]).all()

@@ -105,13 +105,15 @@ def test_mlp_training_lowers_loss():
     losses = net.fit(Xd, td, ld, cd, epochs=6, batch=4096, lr=3e-3, never=nd)
     l1 = net.grad(Xd, td, ld, cd, never=nd)
     assert l1 < 0.2 * l0, (l0, l1, losses[-5:])
-    # classification accuracy of both logit heads on the training data
+    # classification accuracy of both logit heads on the training data, and
+    # the cost head's rank correlation with its target: the same training
+    # emulated in float64 with oracle/mlp_ref.py's gradients (same data,
+    # permutation, initial weights and Adam steps) reaches 0.962, 0.949 and
+    # 0.888
     Y = net.infer(Xd).cpu().numpy()
     acc = np.mean((Y[:, 1] > 0) == (label > 0.5))
     assert acc > 0.95
-    assert np.mean((Y[:, 3] > 0) == (never > 0.5)) > 0.95
-    # the cost head ranks trajectories: rank correlation with the target (the
-    # same training emulated with oracle/mlp_ref.py reaches 0.961 / 0.90)
+    assert np.mean((Y[:, 3] > 0) == (never > 0.5)) > 0.93
     rk = lambda v: np.argsort(np.argsort(v))
     assert np.corrcoef(rk(Y[:, 2]), rk(cost))[0, 1] > 0.85
 

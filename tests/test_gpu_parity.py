@@ -289,3 +289,43 @@ def test_saveat_outside_span_and_duplicates(oracle):
     np.testing.assert_array_equal(g.us[1], w["u0"])
     np.testing.assert_array_equal(g.us[2], g.us[3])
     np.testing.assert_array_equal(g.us[5], g.u_final)
+
+
+@pytest.mark.parametrize("model,alg,precision,schedule,save_dtype", [
+    ("lorenz", "tsit5", "fp64", "lane", "float64"), ("lorenz", "tsit5", "mixed", "persistent",
+                                                     "float32"),
+    ("robertson", "rodas5", "fp64", "lane", "float64"), ("vdp", "auto", "fp64", "lane", "float64"),
+    ("exp", "rodas5p", "fp64", "lane", "float32")])
+def test_trajectory_major_saves_bitexact(oracle, model, alg, precision, schedule, save_dtype):
+    """save_layout "trajectory" ([N, ns, n]: one trajectory's saved states
+    contiguous) on the one-lane kernels: the same values as the [ns, n, N]
+    layout and as the oracle in its trajectory-major layout, NaN at points
+    outside a span; the exp model runs half its trajectories backward."""
+    N = 700
+    say(f"trajectory-major saves {model} {alg} {precision} {schedule} {save_dtype} N={N}")
+    rng = np.random.default_rng(46)
+    if model == "exp":
+        w = dict(u0=np.ones((1, N)), p=-rng.uniform(0.5, 2.0, (1, N)))
+        back = rng.random(N) < 0.5
+        t0, tf = np.where(back, 1.0, 0.0), np.where(back, 0.0, 1.0) + rng.uniform(-0.3, 0.3, N)
+        sa = np.linspace(-0.5, 1.5, 11)
+    else:
+        w = dict(lorenz=workloads.lorenz, robertson=workloads.robertson,
+                 vdp=workloads.vdp)[model](N, seed=47)
+        t0 = 0.0
+        tf = dict(lorenz=2.0, robertson=10.0, vdp=3.0)[model]
+        sa = np.linspace(0.0, tf, 17)[1:]
+    tol = dict(abstol=1e-8, reltol=1e-6)
+    kw = dict(alg=alg, precision=precision, schedule=schedule, saveat=sa, save_dtype=save_dtype,
+              **tol)
+    g = gpu(model, w["u0"], w["p"], t0, tf, save_layout="trajectory", **kw)
+    s = gpu(model, w["u0"], w["p"], t0, tf, **kw)
+    n = w["u0"].shape[0]
+    assert g.us.shape == (N, len(sa), n)
+    np.testing.assert_array_equal(np.transpose(g.us, (1, 2, 0)), s.us)
+    c = cpu(oracle, model, w["u0"], w["p"], t0, tf, alg=alg, precision=precision, saveat=sa,
+            save_layout=1, **tol)
+    ref = c["saveat"].astype(np.float32) if save_dtype == "float32" else c["saveat"]
+    np.testing.assert_array_equal(g.us, ref)
+    g.us = None
+    assert_identical(g, c)
