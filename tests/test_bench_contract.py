@@ -45,3 +45,13 @@ def test_headline_only_at_the_stated_tolerance():
     on = [f for f, v in bench.FORMS.items() if v[1] == bench.CONFIG_TOL[v[0]]]
     assert {"plain", "tuned", "mixed"} <= set(on)
     assert "tuned_tol" not in on
+
+
+def test_headline_only_with_soa_output():
+    """The configuration's output is structure-of-arrays [100][3][N]; a form
+    with another layout is reported beside the headline, never as it."""
+    sys.path.insert(0, ROOT)
+    import bench
+    src = open(os.path.join(ROOT, "bench.py")).read()
+    assert 'SAVE_LAYOUT.get(f, 0) == 0]' in src
+    assert bench.SAVE_LAYOUT.get("tuned", 0) == 0 and bench.SAVE_LAYOUT.get("plain", 0) == 0
