@@ -209,7 +209,7 @@ end
 function __solve(ens::SciMLBase.AbstractEnsembleProblem, alg, ealg::EnsembleMI355X;
                  trajectories, batch_size = trajectories, saveat = nothing, abstol = 1e-6,
                  reltol = 1e-3, dt = 0.0, dtmin = 0.0, dtmax = 0.0, maxiters = 100_000,
-                 kwargs...)
+                 dense = saveat === nothing, kwargs...)
     f = ens.prob.f.f
     if !haskey(MODEL_FOR, f)
         @warn "EnsembleMI355X: unregistered right-hand side, falling back to EnsembleThreads"
@@ -226,7 +226,7 @@ function __solve(ens::SciMLBase.AbstractEnsembleProblem, alg, ealg::EnsembleMI35
     for first in 1:batch_size:N
         II = first:min(first + batch_size - 1, N)
         batch = _solve_batch(ens, alg, ealg, f, II; saveat, abstol, reltol, dt, dtmin,
-                             dtmax, maxiters)
+                             dtmax, maxiters, dense)
         u, converged = ens.reduction(u, batch, II)
         converged && break
     end
@@ -236,64 +236,128 @@ function __solve(ens::SciMLBase.AbstractEnsembleProblem, alg, ealg::EnsembleMI35
     return EnsembleSolution(u, time() - t_start, converged)
 end
 
+# One mideq_ensemble_solve call over the problems `probs` (one per trajectory,
+# all of one registered f) with the ascending saveat points `sa`; `lanes` = 0
+# lets the library pick the team shape.  Returns the output arrays and the
+# team shape used.
+struct MideqStats
+    kernel_ms::Float64; total_ms::Float64
+    schedule::Int32; block::Int32; grid::Int32; launches::Int32; lanes::Int32; graph::Int32
+end
+
+function _device_solve(probs, alg, ealg::EnsembleMI355X, f, sa; abstol, reltol, dt, dtmin,
+                       dtmax, maxiters, lanes = 0)
+    M = length(probs)
+    n = maximum(length(pr.u0) for pr in probs)
+    np = max(length(probs[1].p), 1)   # a user model without parameters reads one row
+    # column-major (M, n): trajectory index fastest, the SoA layout of the ABI
+    U0 = zeros(Float64, M, n); P = zeros(Float64, M, np)
+    T0 = Float64[pr.tspan[1] for pr in probs]; TF = Float64[pr.tspan[2] for pr in probs]
+    nvec = Int32[length(pr.u0) for pr in probs]
+    for (k, pr) in enumerate(probs)
+        U0[k, 1:length(pr.u0)] .= pr.u0
+        P[k, 1:length(pr.p)] .= collect(pr.p)
+    end
+    ns = length(sa)
+    US = ealg.save_dtype === Float32 ? zeros(Float32, M, n, ns) : zeros(Float64, M, n, ns)
+    UF = zeros(Float64, M, n); TFo = zeros(Float64, M)
+    RC = zeros(Int8, M); NA = zeros(Int32, M); NR = zeros(Int32, M)
+    NF = zeros(Int32, M); NS = zeros(Int32, M)
+    st = Ref(MideqStats(0, 0, 0, 0, 0, 0, 0, 0))
+    ctx = context(ealg.device)
+    GC.@preserve U0 P T0 TF nvec sa US UF TFo RC NA NR NF NS begin
+        prob_c = MideqProblem(MODEL_FOR[f], n, np, 0, M, pointer(U0), pointer(P),
+                              pointer(T0), pointer(TF), T0[1], TF[1],
+                              MODEL_FOR[f] == 4 ? pointer(nvec) : C_NULL)
+        opt_c = MideqOptions(alg_code(alg), ealg.precision === :mixed ? 1 : 0,
+                             abstol, reltol, dt, dtmin, dtmax, maxiters,
+                             ns > 0 ? pointer(sa) : C_NULL, ns,
+                             ealg.save_dtype === Float32 ? 1 : 0, C_NULL, C_NULL, 1,
+                             ealg.schedule === :lane ? 1 : ealg.schedule === :persistent ? 2 : 0,
+                             0, 0, lanes, 0, 0, 0, C_NULL, 0, 0, C_NULL)
+        out_c = MideqOutputs(ns > 0 ? pointer(US) : C_NULL, pointer(UF), pointer(TFo),
+                             pointer(RC), pointer(NA), pointer(NR), pointer(NF), pointer(NS),
+                             C_NULL, C_NULL, C_NULL, C_NULL, C_NULL)
+        rc = ccall((:mideq_ensemble_solve, LIB[]), Cint,
+                   (Ptr{Cvoid}, Ref{MideqProblem}, Ref{MideqOptions}, Ref{MideqOutputs}),
+                   ctx.h, prob_c, opt_c, out_c)
+        check(ctx, rc, "mideq_ensemble_solve")
+        rc = ccall((:mideq_last_stats, LIB[]), Cint, (Ptr{Cvoid}, Ref{MideqStats}), ctx.h, st)
+        check(ctx, rc, "mideq_last_stats")
+    end
+    return (; US, UF, TFo, RC, NA, NR, NF, NS, lanes = st[].lanes)
+end
+
+# Dense output sol(t) (test/core.jl:51-58): the trajectory solved again with
+# the requested times as saveat points.  Saved points do not change the steps
+# taken, so the values are the interpolants of the steps of the solve that
+# made the solution; NaN outside the integrated span.  Values only.
+struct MideqDense{F} <: SciMLBase.AbstractDiffEqInterpolation
+    resolve::F   # ascending Vector{Float64} of times -> states at them
+end
+
+function _dense_values(id::MideqDense, tvals)
+    ts = collect(Float64, tvals)
+    perm = sortperm(ts)
+    vals = id.resolve(ts[perm])
+    out = similar(vals)
+    out[perm] = vals
+    return out
+end
+
+function (id::MideqDense)(tvals, idxs, deriv, p, continuity::Symbol = :left)
+    deriv === Val{0} || error("EnsembleMI355X dense output gives values only")
+    vals = _dense_values(id, tvals)
+    vals = idxs === nothing ? vals : [v[idxs] for v in vals]
+    return SciMLBase.DiffEqArray(vals, collect(tvals))
+end
+
+function (id::MideqDense)(t::Number, idxs, deriv, p, continuity::Symbol = :left)
+    deriv === Val{0} || error("EnsembleMI355X dense output gives values only")
+    v = _dense_values(id, (t,))[1]
+    return idxs === nothing ? v : v[idxs]
+end
+
 # The trajectories II of one batch: prob_func, the device solve, output_func
-# with its rerun loop; returns their outputs in index order.
+# with its rerun loop; returns their outputs in index order.  Without saveat
+# every solution has dense output (sol(t)), as upstream's default.
 function _solve_batch(ens, alg, ealg::EnsembleMI355X, f, II; saveat, abstol, reltol, dt,
-                      dtmin, dtmax, maxiters)
+                      dtmin, dtmax, maxiters, dense = saveat === nothing)
     outputs = Dict{Int,Any}()
     pending = collect(II)
     repeat = 1
     sa = _saveat_vector(saveat, ens.prob.tspan)
+    ns = length(sa)
+    tol = (; abstol, reltol, dt, dtmin, dtmax, maxiters)
     while !isempty(pending)
         probs = [ens.prob_func(deepcopy(ens.prob), i, repeat) for i in pending]
-        M = length(probs)
-        n = maximum(length(pr.u0) for pr in probs)
-        np = max(length(probs[1].p), 1)   # a user model without parameters reads one row
-        # column-major (M, n): trajectory index fastest, the SoA layout of the ABI
-        U0 = zeros(Float64, M, n); P = zeros(Float64, M, np)
-        T0 = Float64[pr.tspan[1] for pr in probs]; TF = Float64[pr.tspan[2] for pr in probs]
-        nvec = Int32[length(pr.u0) for pr in probs]
-        for (k, pr) in enumerate(probs)
-            U0[k, 1:length(pr.u0)] .= pr.u0
-            P[k, 1:length(pr.p)] .= collect(pr.p)
-        end
-        ns = length(sa)
-        US = ealg.save_dtype === Float32 ? zeros(Float32, M, n, ns) : zeros(Float64, M, n, ns)
-        UF = zeros(Float64, M, n); TFo = zeros(Float64, M)
-        RC = zeros(Int8, M); NA = zeros(Int32, M); NR = zeros(Int32, M)
-        NF = zeros(Int32, M); NS = zeros(Int32, M)
-        ctx = context(ealg.device)
-        GC.@preserve U0 P T0 TF nvec sa US UF TFo RC NA NR NF NS begin
-            prob_c = MideqProblem(MODEL_FOR[f], n, np, 0, M, pointer(U0), pointer(P),
-                                  pointer(T0), pointer(TF), T0[1], TF[1],
-                                  MODEL_FOR[f] == 4 ? pointer(nvec) : C_NULL)
-            opt_c = MideqOptions(alg_code(alg), ealg.precision === :mixed ? 1 : 0,
-                                 abstol, reltol, dt, dtmin, dtmax, maxiters,
-                                 ns > 0 ? pointer(sa) : C_NULL, ns,
-                                 ealg.save_dtype === Float32 ? 1 : 0, C_NULL, C_NULL, 1,
-                                 ealg.schedule === :lane ? 1 : ealg.schedule === :persistent ? 2 : 0,
-                                 0, 0, 0, 0, 0, 0, C_NULL, 0, 0, C_NULL)
-            out_c = MideqOutputs(ns > 0 ? pointer(US) : C_NULL, pointer(UF), pointer(TFo),
-                                 pointer(RC), pointer(NA), pointer(NR), pointer(NF), pointer(NS),
-                                 C_NULL, C_NULL, C_NULL, C_NULL, C_NULL)
-            rc = ccall((:mideq_ensemble_solve, LIB[]), Cint,
-                       (Ptr{Cvoid}, Ref{MideqProblem}, Ref{MideqOptions}, Ref{MideqOutputs}),
-                       ctx.h, prob_c, opt_c, out_c)
-            check(ctx, rc, "mideq_ensemble_solve")
-        end
+        r = _device_solve(probs, alg, ealg, f, sa; tol...)
         rerun = Int[]
         for (k, i) in enumerate(pending)
             pr = probs[k]
             nk = length(pr.u0)
+            # a scalar problem (u0 isa Number) gets scalar states, as upstream
+            val(row) = pr.u0 isa Number ? row[1] : Vector{Float64}(row)
             if ns > 0
-                keep = [j for j in 1:ns if !isnan(US[k, 1, j])]
-                TFo[k] < pr.tspan[1] && reverse!(keep)   # backward: order of integration
-                ts = sa[keep]; us = [Vector{Float64}(US[k, 1:nk, j]) for j in keep]
+                keep = [j for j in 1:ns if !isnan(r.US[k, 1, j])]
+                r.TFo[k] < pr.tspan[1] && reverse!(keep)   # backward: order of integration
+                ts = sa[keep]; us = [val(r.US[k, 1:nk, j]) for j in keep]
             else
-                ts = [pr.tspan[1], TFo[k]]; us = [copy(pr.u0), UF[k, 1:nk]]
+                ts = [pr.tspan[1], r.TFo[k]]; us = [val([pr.u0;]), val(r.UF[k, 1:nk])]
             end
-            sol = SciMLBase.build_solution(pr, alg, ts, us;
-                                           retcode = RETCODES[RC[k] + 1])
+            if dense
+                lanes = r.lanes
+                resolve = function (tq)
+                    d = _device_solve([pr], alg, ealg, f, tq; tol..., lanes)
+                    [val(d.US[1, 1:nk, j]) for j in eachindex(tq)]
+                end
+                sol = SciMLBase.build_solution(pr, alg, ts, us; dense = true,
+                                               interp = MideqDense(resolve),
+                                               retcode = RETCODES[r.RC[k] + 1])
+            else
+                sol = SciMLBase.build_solution(pr, alg, ts, us;
+                                               retcode = RETCODES[r.RC[k] + 1])
+            end
             out, again = ens.output_func(sol, i)
             outputs[i] = out
             again && push!(rerun, i)

@@ -408,9 +408,20 @@ class ODESolution:
     retcode: ReturnCode
     stats: Stats
     prob: Optional[ODEProblem] = None
+    interp: Optional[Callable] = field(default=None, repr=False, compare=False)
 
     def __getitem__(self, k):
         return self.u[k]
+
+    def __call__(self, t):
+        """Dense output sol(t) (core.jl:51-58): the state at time t (a
+        scalar, or an array of times -> [len(t), n]) from the interpolant of
+        the step that covers t; NaN outside the integrated span."""
+        if self.interp is None:
+            raise ValueError("this solution has no dense output: solve with dense=True "
+                             "(the default when no saveat is given)")
+        vals = self.interp(np.atleast_1d(np.asarray(t, dtype=np.float64)))
+        return vals[0] if np.ndim(t) == 0 else vals
 
 
 @dataclass
@@ -711,6 +722,40 @@ def device_solve(f, u0, p, t0, tf, nvec, alg, ealg, saveat=None, abstol=1e-6, re
                           save_layout=ealg.save_layout, nvec=nvec, **res)
 
 
+def _dense_interp(packed, k, i, alg, ealg, lanes, opts, kw):
+    """sol(t) for trajectory i (0-based), column k of the packed inputs
+    (f, u0, p, t0, tf, nvec): the same solve again with the requested times
+    as its saveat points.  Saved points do not change the steps taken (there
+    are no tstops at them), so the values are the interpolants of the steps
+    of the solve that made the solution, bit for bit.  lanes: the team shape
+    that solve used (the Brusselator team's reduction order depends on it).
+    Nothing is copied until the solution is called."""
+
+    def interp(ts):
+        f, u0, p, t0, tf, nvec = packed
+        one = np.array([i])
+        ealg1 = replace(_subset_ealg(ealg, one), lanes=lanes, graph=False, save_layout="soa",
+                        save_dtype="float64", order=None)
+        extra = {key: np.asarray(kw[key])[[k]] for key in ("dt0", "alg_hint")
+                 if kw.get(key) is not None}
+        if "callback" in kw:
+            extra["callback"] = kw["callback"]
+        ts = np.ascontiguousarray(ts, dtype=np.float64).ravel()
+        order = np.argsort(ts, kind="stable")
+        sa = np.ascontiguousarray(ts[order])
+        arr = device_solve(f, np.ascontiguousarray(u0[:, k:k + 1]),
+                           np.ascontiguousarray(p[:, k:k + 1]), np.array(t0[k:k + 1]),
+                           np.array(tf[k:k + 1]),
+                           None if nvec is None else np.asarray(nvec)[k:k + 1].copy(),
+                           alg, ealg1, saveat=sa, **opts, **extra)
+        vals = arr.saved(0)
+        out = np.empty_like(vals)
+        out[order] = vals
+        return out
+
+    return interp
+
+
 def _subset_ealg(ealg, pending):
     """The per-trajectory settings of `ealg` restricted to the trajectories
     `pending` (a rerun pass), the launch order kept as their relative order."""
@@ -754,6 +799,9 @@ def solve(prob, alg, ensemblealg=None, *, trajectories=None, saveat=None, abstol
         raise ValueError("trajectories // batch_size cannot be less than 1")
     idx = np.arange(N)
     sa = _saveat_vector(saveat, prob.prob.tspan)
+    # dense output (sol(t)): on by default without saveat, as upstream
+    dense = kw.pop("dense", None)
+    dense = saveat is None if dense is None else bool(dense)
     if isinstance(prob.output_func, DeviceOutputFunc):
         # output_func (and reduction) on the device: one pass, no ODESolutions
         red = prob.reduction if isinstance(prob.reduction, DeviceReduction) else None
@@ -786,8 +834,14 @@ def solve(prob, alg, ensemblealg=None, *, trajectories=None, saveat=None, abstol
             if first_arrays is None:
                 first_arrays = arr
             rerun = []
+            opts = dict(abstol=abstol, reltol=reltol, dt=dt, dtmin=dtmin, dtmax=dtmax,
+                        maxiters=maxiters)
+            packed = (f, u0, p, t0, tf, nvec)
             for k, i in enumerate(pending):
                 sol = arr.solution(k, probs[k] if probs else None)
+                if dense:
+                    sol.interp = _dense_interp(packed, k, int(i), alg, ensemblealg, arr.lanes,
+                                               opts, kw)
                 out, again = prob.output_func(sol, int(i) + 1)
                 outputs[int(i)] = out
                 if again:

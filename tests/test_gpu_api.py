@@ -150,3 +150,72 @@ def test_single_problem_non_autonomous_backward(oracle):
     c = oracle.solve("user", [0.0, 0.0], [4.0, 0.1], (10.0, 0.0), abstol=1e-9, reltol=1e-9,
                      saveat=np.arange(0.0, 11.0))
     np.testing.assert_array_equal(np.array(sol.u), c["saveat"][::-1])
+
+
+def test_core_dense_output_interpolation(oracle):
+    """core.jl:51-58: solve(prob, Tsit5(), dense = true); sol(0.5) > 0.5.
+    sol(t) evaluates the interpolant of the step covering t: bit-exact
+    against the same solve with t as saveat (on the device and in the
+    oracle), for scalars, unsorted arrays with duplicates, NaN outside the
+    span; solutions made with saveat have no dense output."""
+    say("core.jl dense output sol(t)")
+    prob = ODEProblem(mideq.exp_growth, 0.5, (0.0, 1.0), [1.01])
+    sol = solve(prob, Tsit5(), dense=True)
+    assert sol.retcode == ReturnCode.Success
+    v = sol(0.5)
+    assert v.shape == (1,) and v[0] > 0.5
+    assert abs(v[0] - 0.5 * math.exp(0.505)) < 1e-3 * 0.5 * math.exp(0.505) + 1e-6
+    ts = np.array([0.9, 0.1, 0.5, 0.5, 0.0, 1.0, 0.33])
+    vals = sol(ts)
+    assert vals.shape == (len(ts), 1)
+    srt = np.sort(ts)
+    ref = solve(prob, Tsit5(), saveat=srt)
+    np.testing.assert_array_equal(vals[np.argsort(ts, kind="stable"), 0], np.array(ref.u)[:, 0])
+    c = oracle.solve("exp", [0.5], [1.01], (0.0, 1.0), saveat=srt)
+    np.testing.assert_array_equal(np.sort(vals[:, 0]), c["saveat"][:, 0])
+    assert np.isnan(sol(np.array([-0.1, 1.5]))).all()
+    # dense by default without saveat; off with saveat
+    assert solve(prob, Tsit5())(0.25)[0] == solve(prob, Tsit5(), saveat=[0.25]).u[0][0]
+    with pytest.raises(ValueError):
+        solve(prob, Tsit5(), saveat=0.1)(0.5)
+
+
+@pytest.mark.parametrize("model,alg", [("lorenz", Tsit5()), ("robertson", Rodas5()),
+                                       ("brusselator", Rodas5())])
+def test_ensemble_dense_output_matches_saveat(model, alg):
+    """sim.u[i](t) of an ensemble equals the ensemble solved with t as
+    saveat, trajectory by trajectory (lane kernels and the Brusselator team
+    kernel, whose team shape the re-solve keeps)."""
+    from mideq import workloads
+    say(f"ensemble dense output {model}")
+    N = 16
+    if model == "brusselator":
+        w = workloads.brusselator(N, seed=91)
+        w["tf"] = np.minimum(w["tf"], 1.0)
+        fn = mideq.brusselator
+        base = ODEProblem(fn, w["u0"][:, 0][:int(w["nvec"][0])], (0.0, float(w["tf"][0])),
+                          w["p"][:, 0])
+
+        def pf(prob, i, repeat):
+            n = int(w["nvec"][i - 1])
+            return remake(prob, u0=w["u0"][:n, i - 1], p=w["p"][:, i - 1],
+                          tspan=(0.0, float(w["tf"][i - 1])))
+        ts = np.array([0.7, 0.05, 0.5])
+    else:
+        w = dict(lorenz=workloads.lorenz, robertson=workloads.robertson)[model](N, seed=92)
+        fn = dict(lorenz=mideq.lorenz, robertson=mideq.robertson)[model]
+        tf = 2.0 if model == "lorenz" else 10.0
+        base = ODEProblem(fn, w["u0"][:, 0], (0.0, tf), w["p"][:, 0])
+
+        def pf(prob, i, repeat):
+            return remake(prob, u0=w["u0"][:, i - 1], p=w["p"][:, i - 1])
+        ts = np.array([0.3, 1.7, 0.9]) * (1.0 if model == "lorenz" else 5.0)
+    ens = EnsembleProblem(base, prob_func=pf)
+    tol = dict(abstol=1e-8, reltol=1e-6)
+    sim = solve(ens, alg, EnsembleMI355X(), trajectories=N, **tol)
+    ref = solve(ens, alg, EnsembleMI355X(lanes=sim.arrays.lanes), trajectories=N,
+                saveat=np.sort(ts), **tol)
+    order = np.argsort(ts)
+    for i in (0, 5, N - 1):
+        vals = sim.u[i](ts)
+        np.testing.assert_array_equal(vals[order], np.array(ref.u[i].u))
