@@ -886,19 +886,23 @@ struct BTeam {
   }
 
   // The substitution chains of band_solve: the factors one position needs,
-  // loaded a position ahead, and the chain's running values.
+  // loaded a position ahead, and the chain's running values.  Without an
+  // interchange the kT dense rows are updated lane-parallel: lane tl keeps
+  // dense row tl & 3 (bdl, its multiplier dl), so their four updates per
+  // position are one fma; with interchanges (a position's pivot can be a
+  // dense row) every lane keeps all four (bd0..bd3, d0..d3).
   struct FwdFactors {
     int pj;
-    double l1, l2, d0, d1, d2, d3, t3;
+    double l1, l2, d0, d1, d2, d3, dl, t3;
   };
   struct FwdChain {
-    double b0, b1, b2, bd0, bd1, bd2, bd3, nonfin;
+    double b0, b1, b2, bd0, bd1, bd2, bd3, bdl;
   };
   struct BackFactors {
     double u6, u5, u4, u3, ri, y;
   };
   struct BackChain {
-    double x1, x2, x3, x4, nonfin;
+    double x1, x2, x3, x4;
   };
 
   // position j's pivot row, multipliers of rows j+1, j+2 and the dense rows,
@@ -908,22 +912,35 @@ struct BTeam {
     if constexpr (PIV) f.pj = ldsi[j];
     f.l1 = Wb[(j + 1) * 8 + 1];
     f.l2 = Wb[(j + 2) * 8];
-    f.d0 = Wd[j];
-    f.d1 = Wd[NK + j];
-    f.d2 = Wd[2 * NK + j];
-    f.d3 = Wd[3 * NK + j];
+    if constexpr (PIV) {
+      f.d0 = Wd[j];
+      f.d1 = Wd[NK + j];
+      f.d2 = Wd[2 * NK + j];
+      f.d3 = Wd[3 * NK + j];
+    } else {
+      f.dl = Wd[(tl & 3) * NK + j];
+    }
     f.t3 = ldsd[j + 3];
   }
 
   // PIV: the factorization interchanged rows (else every pj == j).  LAST:
-  // position j = nb - 3 = n - 7 (no right-hand side entry j + 3 < nb, no
-  // non-finite check); every earlier position has both.
+  // position j = nb - 3 = n - 7 (no right-hand side entry j + 3 < nb); every
+  // earlier position has one.
   template <bool PIV, bool LAST>
   __device__ __forceinline__ void fwd_step(int j, const FwdFactors& f, FwdFactors& next,
                                            FwdChain& c) {
     const int pj = PIV ? team_uni<L>(f.pj) : j;
     const double bn = LAST ? 0.0 : f.t3;
     fwd_load<PIV>(next, j + 1);
+    if constexpr (!PIV) {
+      const double xj = c.b0;
+      if (tl == 0) ldsd[j] = xj;
+      c.b0 = fma(-f.l1, xj, c.b1);
+      c.b1 = fma(-f.l2, xj, c.b2);
+      c.b2 = bn;
+      c.bdl = fma(-f.dl, xj, c.bdl);
+      return;
+    }
     double b0 = c.b0, b1 = c.b1, b2 = c.b2, bd0 = c.bd0, bd1 = c.bd1, bd2 = c.bd2, bd3 = c.bd3;
     if (PIV && pj != j) {
       double t = b0;
@@ -956,7 +973,6 @@ struct BTeam {
     c.bd1 = fma(-f.d1, xj, bd1);
     c.bd2 = fma(-f.d2, xj, bd2);
     c.bd3 = fma(-f.d3, xj, bd3);
-    if (!LAST) c.nonfin = fma(xj, 0.0, c.nonfin);
   }
 
   // the forward chain over positions 0 .. nb - 3
@@ -988,9 +1004,8 @@ struct BTeam {
     f.y = ldsd[i];
   }
 
-  // row i <= nb - 5: all four window columns are band columns.  CHK: i >= 5
-  // (feeds the non-finite check); row 0 loads nothing further.
-  template <bool CHK>
+  // row i <= nb - 5: all four window columns are band columns; row 0 loads
+  // nothing further
   __device__ __forceinline__ void back_step(int i, const BackFactors& f, BackFactors& next,
                                             BackChain& c) {
     back_load(next, i > 0 ? i - 1 : 0);
@@ -1001,8 +1016,6 @@ struct BTeam {
     y = fma(-f.u3, c.x1, y);
     const double xi = y * f.ri;
     if (tl == 0) ldsd[i] = xi;
-    // column i: rows above i-4 hold exact zeros
-    if (CHK) c.nonfin = fma(xi, 0.0, c.nonfin);
     c.x4 = c.x3;
     c.x3 = c.x2;
     c.x2 = c.x1;
@@ -1013,7 +1026,11 @@ struct BTeam {
   // and out.  Forward elimination (interchanges as they come) and back
   // substitution walk the band as chains in registers, every lane of the
   // team computing the same values (lane 0 stores them).  Returns false when
-  // a non-finite value met a skipped zero (the dense solve's NaNs).
+  // a non-finite value met a skipped zero (the dense solve's NaNs): a
+  // forward value x_j with j <= nb - 4 (the dense LU multiplies it into the
+  // exact zeros of rows past j + 2 that precede the dense ones), or a final
+  // x_i with 5 <= i < nb (rows above i - 4 hold exact zeros in column i);
+  // both are checked lane-parallel once the chain has stored them.
   __device__ __forceinline__ bool band_solve(double* x) {
     n = team_uni<L>(n);
     nb = team_uni<L>(nb);
@@ -1030,7 +1047,7 @@ struct BTeam {
     // dense rows (7); rows past j+2 outside them while j < n - 7.  Two
     // register sets of factors alternate: position j + 1's load while
     // position j computes (indices stay inside the team's arrays: j + 4 <=
-    // nb + 1 < NK).  A non-finite x_j (j < n - 7) turns nonfin into NaN.
+    // nb + 1 < NK).
     FwdChain st;
     st.b0 = b0;
     st.b1 = b1;
@@ -1039,20 +1056,30 @@ struct BTeam {
     st.bd1 = bd1;
     st.bd2 = bd2;
     st.bd3 = bd3;
-    st.nonfin = 0.0;
+    st.bdl = ldsd[nb + (tl & 3)];
     // without an interchange (the usual case: W = J - I/(gamma dt) is
-    // diagonally dominant on this problem) the pivot rows are not read
-    if (lu_swapped)
+    // diagonally dominant on this problem) the pivot rows are not read and
+    // the dense rows are updated lane-parallel
+    if (lu_swapped) {
       fwd_chain<true>(st);
-    else
+      bd0 = st.bd0;
+      bd1 = st.bd1;
+      bd2 = st.bd2;
+      bd3 = st.bd3;
+    } else {
       fwd_chain<false>(st);
+      bd0 = __shfl(st.bdl, 0, L);
+      bd1 = __shfl(st.bdl, 1, L);
+      bd2 = __shfl(st.bdl, 2, L);
+      bd3 = __shfl(st.bdl, 3, L);
+    }
     b0 = st.b0;
     b1 = st.b1;
-    bd0 = st.bd0;
-    bd1 = st.bd1;
-    bd2 = st.bd2;
-    bd3 = st.bd3;
-    bad = bad || (st.nonfin != st.nonfin);
+#pragma unroll
+    for (int m = 0; m < C; m++) {
+      const int c = comp(m);
+      if (c <= nb - 4) bad = bad || !(fabs(ldsd[c]) <= __DBL_MAX__);
+    }
     // the last two band positions and the dense ones: every later position
     // is a candidate
     double v[6] = {b0, b1, bd0, bd1, bd2, bd3};
@@ -1114,47 +1141,48 @@ struct BTeam {
       if (ii >= 2) y = fma(-U[3], x1, y);
       const double xi = y * ldsr[i];
       if (tl == 0) ldsd[i] = xi;
-      bad = bad || (i >= 5 && !(fabs(xi) <= __DBL_MAX__));
       x4 = x3;
       x3 = x2;
       x2 = x1;
       x1 = xi;
     }
     // the other rows, two register sets alternating (row i - 1's factors
-    // load while row i computes); rows i >= 5 feed the non-finite check
+    // load while row i computes)
     {
       BackChain bs;
       bs.x1 = x1;
       bs.x2 = x2;
       bs.x3 = x3;
       bs.x4 = x4;
-      bs.nonfin = 0.0;
       BackFactors ba, bb;
       int i = nb - 5;  // >= 7
       back_load(ba, i);
       for (; i - 1 >= 5; i -= 2) {
-        back_step<true>(i, ba, bb, bs);
-        back_step<true>(i - 1, bb, ba, bs);
+        back_step(i, ba, bb, bs);
+        back_step(i - 1, bb, ba, bs);
       }
       if (i >= 5) {  // i == 5
-        back_step<true>(5, ba, bb, bs);
-        back_step<false>(4, bb, ba, bs);
-        back_step<false>(3, ba, bb, bs);
-        back_step<false>(2, bb, ba, bs);
-        back_step<false>(1, ba, bb, bs);
-        back_step<false>(0, bb, ba, bs);
+        back_step(5, ba, bb, bs);
+        back_step(4, bb, ba, bs);
+        back_step(3, ba, bb, bs);
+        back_step(2, bb, ba, bs);
+        back_step(1, ba, bb, bs);
+        back_step(0, bb, ba, bs);
       } else {  // i == 4
-        back_step<false>(4, ba, bb, bs);
-        back_step<false>(3, bb, ba, bs);
-        back_step<false>(2, ba, bb, bs);
-        back_step<false>(1, bb, ba, bs);
-        back_step<false>(0, ba, bb, bs);
+        back_step(4, ba, bb, bs);
+        back_step(3, bb, ba, bs);
+        back_step(2, ba, bb, bs);
+        back_step(1, bb, ba, bs);
+        back_step(0, ba, bb, bs);
       }
-      bad = bad || (bs.nonfin != bs.nonfin);
     }
     team_sync();
 #pragma unroll
-    for (int m = 0; m < C; m++) x[m] = (comp(m) < n) ? ldsd[comp(m)] : 0.0;
+    for (int m = 0; m < C; m++) {
+      const int c = comp(m);
+      x[m] = (c < n) ? ldsd[c] : 0.0;
+      if (c >= 5 && c < nb) bad = bad || !(fabs(x[m]) <= __DBL_MAX__);
+    }
     team_sync();
     return !team_or<L>(bad ? 1 : 0);
   }
@@ -1495,23 +1523,28 @@ __device__ __forceinline__ void bruss_run(const BrussArgs& a) {
   constexpr int C = NK / L;
   constexpr int TPB = 256 / L;
   using Team = BTeam<L, C, MIX, ALG, FAST, SAVE>;
-  // W in band + border form: 8 NK band entries, kT NK border columns, kT
-  // dense rows of NK (Rosenbrock-capable kernels only)
-  constexpr int WSZ = Team::kHasRodas ? (8 + 2 * Team::kT) * NK : 1;
-  __shared__ double s_d[TPB][NK];
-  __shared__ double s_r[TPB][NK];
-  __shared__ float s_f[TPB][NK];
-  __shared__ int s_i[TPB][NK];
-  __shared__ double s_w[Team::kHasRodas ? TPB : 1][WSZ];
+  // One block of LDS per team, every offset inside it small enough for a DS
+  // instruction's immediate: the exchange vector and the reciprocal pivots
+  // side by side (position i's pair loads as one ds_read2), then W in band +
+  // border form (8 NK band entries, kT NK border columns, kT dense rows of
+  // NK; Rosenbrock-capable kernels), the FP32 exchange vector (mixed
+  // precision) and the pivot rows (Rosenbrock-capable kernels).
+  constexpr int kR = Team::kHasRodas ? NK : 0;
+  constexpr int kW = Team::kHasRodas ? (8 + 2 * Team::kT) * NK : 0;
+  constexpr int kF = MIX ? NK / 2 : 0;
+  constexpr int kI = Team::kHasRodas ? NK / 2 : 0;
+  constexpr int PER = NK + kR + kW + kF + kI;
+  __shared__ double s_team[TPB][PER];
   const int team = threadIdx.x / L;
   Team T;
   T.tl = threadIdx.x % L;
-  T.ldsd = s_d[team];
-  T.ldsr = s_r[team];
-  T.ldsf = s_f[team];
-  T.ldsi = s_i[team];
+  double* const base = s_team[team];
+  T.ldsd = base;
+  T.ldsr = base + NK;
+  T.ldsf = (float*)(base + NK + kR + kW);
+  T.ldsi = (int*)(base + NK + kR + kW + kF);
   if constexpr (Team::kHasRodas) {
-    T.Wb = s_w[team];
+    T.Wb = base + NK + kR;
     T.Wr = T.Wb + 8 * NK;
     T.Wd = T.Wr + Team::kT * NK;
   }
