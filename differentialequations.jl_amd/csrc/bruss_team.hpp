@@ -638,6 +638,8 @@ struct BTeam {
     for (int e = tl; e < 8 * nb; e += L) Wb[e] = 0.0;
     for (int e = tl; e < kT * nb; e += L) Wr[e] = 0.0;
     for (int e = tl; e < kT * NK; e += L) Wd[e] = 0.0;
+    // pivot rows: a column without an interchange keeps this entry
+    for (int e = tl; e < n; e += L) ldsi[e] = e;
     team_sync();
     const double dtgamma = dts * rd_gamma<kP>();
     const double w = 1.0 / dtgamma;
@@ -690,7 +692,6 @@ struct BTeam {
     }
     if (kAuto) eig = team_max<L>(best);
     team_sync();
-    int sing_k = n, pois_k = n;
     lu_swapped = false;
     // columns k <= nb - 5: the candidates and the pattern have their fixed
     // shape (rows k, k+1, k+2 and the kT dense rows; columns k..k+4 and the
@@ -716,32 +717,24 @@ struct BTeam {
       const double v0 = Wb[k * 8 + 2], v1 = Wb[(k + 1) * 8 + 1], v2 = Wb[(k + 2) * 8];
       const double v3 = Wd[k], v4 = Wd[NK + k], v5 = Wd[2 * NK + k], v6 = Wd[3 * NK + k];
       // the first maximal |W(q, k)| in ascending row order
-      double d = v0, bv = fabs(v0);
+      double d = v0;
       int pk = k;
       const double vq[6] = {v1, v2, v3, v4, v5, v6};
 #pragma unroll
       for (int t = 0; t < 6; t++) {
-        const bool take = fabs(vq[t]) > bv;
-        bv = take ? fabs(vq[t]) : bv;
+        const bool take = fabs(vq[t]) > fabs(d);
         d = take ? vq[t] : d;
         pk = take ? (t < 2 ? k + 1 + t : nb + (t - 2)) : pk;
       }
       pk = team_uni<L>(pk);
       if (pk != k) {
         lu_swapped = true;
-        lu_column(k, 2, 4, 7, 9, pk, d, sing_k, pois_k);
+        lu_column(k, 2, 4, 7, 9, pk, d);
         continue;
       }
       // no interchange (the usual case): the pattern's fixed addresses
-      if (d == 0.0 && sing_k == n) sing_k = k;
       const double rr = 1.0 / d;
-      if (tl == 0) {
-        ldsr[k] = rr;
-        ldsi[k] = k;
-      }
-      const bool skip_rows = k < n - 7;
-      const bool skip_cols = k < n - 9;
-      bool bad = skip_rows && !(fabs(rr) <= __DBL_MAX__);
+      if (tl == 0) ldsr[k] = rr;
 #pragma unroll
       for (int e = 0; e < kE; e++) {
         if (tl + e * L < 48) {
@@ -749,16 +742,12 @@ struct BTeam {
           const double wk = Wb[w_off[e] + k * w_st[e]];
           double* a = Wb + (a_off[e] + k * a_st[e]);
           *a = fma(-l, wk, *a);
-          bad = bad || (skip_rows && !(fabs(wk) <= __DBL_MAX__));
         }
       }
       if (tl < 6) {
         double* a = Wb + (m_off + k * m_st);
-        const double l = *a * rr;
-        *a = l;
-        bad = bad || (skip_cols && !(fabs(l) <= __DBL_MAX__));
+        *a = *a * rr;
       }
-      if (bad && pois_k == n) pois_k = k;
       team_sync();
     }
     for (int k = kmain + 1; k < n; k++) {
@@ -783,8 +772,33 @@ struct BTeam {
       }
       pk = team_uni<L>(pk);
       lu_swapped = lu_swapped || pk != k;
-      lu_column(k, nbc, nwc, nc, np, pk, d, sing_k, pois_k);
+      lu_column(k, nbc, nwc, nc, np, pk, d);
     }
+    // The outcomes of the dense LU, from the final factors, lane-parallel
+    // over the columns: its first zero pivot, and the first column where a
+    // non-finite value meets one of the zeros skipped here (the dense LU
+    // multiplies every row past k by 1/d and every column past k by the
+    // pivot row).  Column k's inputs -- its pivot W(k, k), 1/d, the pivot
+    // row over the pattern and the multipliers -- are final once column k is
+    // done: later columns update and interchange rows and columns past k only.
+    int sing_k = n, pois_k = n;
+    for (int k = tl; k < n; k += L) {
+      const int tail0 = max(nb, k + 1);
+      const int nbc = nband_after(k);
+      const int nc = 1 + nbc + (n - tail0);
+      const int nwc = nwin_after(k);
+      const int np = 1 + nwc + (n - tail0);
+      if (*wp(k, k) == 0.0 && sing_k == n) sing_k = k;
+      bool bad = false;
+      if ((n - 1 - k) > (nc - 1)) {  // rows past k outside the candidates
+        bad = !(fabs(ldsr[k]) <= __DBL_MAX__);
+        for (int j = 1; j < np; j++) bad = bad || !(fabs(*wp(k, patcol(k, nwc, j))) <= __DBL_MAX__);
+      }
+      if ((n - 1 - k) > (np - 1))  // columns past k outside the pattern
+        for (int i = 1; i < nc; i++) bad = bad || !(fabs(*wp(cand(k, nbc, i), k)) <= __DBL_MAX__);
+      if (bad && pois_k == n) pois_k = k;
+    }
+    sing_k = team_min_int<L>(sing_k);
     pois_k = team_min_int<L>(pois_k);
     if (sing_k < n && sing_k <= pois_k) return 1;
     return pois_k < n ? 2 : 0;
@@ -838,7 +852,7 @@ struct BTeam {
   // update the trailing candidate rows x pattern columns, store the
   // multipliers.
   __device__ __forceinline__ void lu_column(int k, int nbc, int nwc, int nc, int np, int pk,
-                                            double d, int& sing_k, int& pois_k) {
+                                            double d) {
     if (pk != k) {
       for (int jc = tl; jc < np; jc += L) {
         const int c = patcol(k, nwc, jc);
@@ -850,18 +864,14 @@ struct BTeam {
       }
       team_sync();
     }
-    if (d == 0.0 && sing_k == n) sing_k = k;
     const double rr = 1.0 / d;
     if (tl == 0) {
       ldsr[k] = rr;
       ldsi[k] = pk;
     }
-    // rows past k outside the candidates / columns outside the pattern:
-    // the dense LU multiplies their zeros (skipped here)
-    const bool skip_rows = (n - 1 - k) > (nc - 1);
-    const bool skip_cols = (n - 1 - k) > (np - 1);
-    bool bad = skip_rows && !(fabs(rr) <= __DBL_MAX__);
     // trailing update a_qc -= l_q a_kc over candidate rows x pattern columns
+    // (rows past k outside the candidates and columns outside the pattern
+    // hold exact zeros: the dense LU's operations on them are skipped)
     for (int pi = tl; pi < (nc - 1) * 8; pi += L) {
       const int cj = 1 + (pi & 7);
       if (cj < np) {
@@ -871,17 +881,13 @@ struct BTeam {
         const double wk = *wp(k, c);
         double* a = wp(q, c);
         *a = fma(-l, wk, *a);
-        bad = bad || (skip_rows && !(fabs(wk) <= __DBL_MAX__));
       }
     }
     // multipliers, stored after every read of column k above
     for (int qi = tl; qi < nc - 1; qi += L) {
       double* a = wp(cand(k, nbc, 1 + qi), k);
-      const double l = *a * rr;
-      *a = l;
-      bad = bad || (skip_cols && !(fabs(l) <= __DBL_MAX__));
+      *a = *a * rr;
     }
-    if (bad && pois_k == n) pois_k = k;
     team_sync();
   }
 

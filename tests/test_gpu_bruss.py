@@ -312,3 +312,33 @@ def test_bruss_backward_spans_bitexact(oracle, alg, lanes, precision, layout):
         n = int(w["nvec"][k])
         np.testing.assert_array_equal(g.u_final[:n, k], w["u0"][:n, k])
     assert (g.t_final[back] == tf[back]).all()
+
+
+@pytest.mark.parametrize("alg,lanes,precision", [("rodas5", 64, "fp64"), ("auto", 32, "fp64"),
+                                                 ("rodas5p", 64, "mixed"), ("rodas5", 32, "fp64")])
+def test_bruss_rosenbrock_nonfinite_inputs_bitexact(oracle, alg, lanes, precision):
+    """Instances whose W or right-hand side turn non-finite (NaN and infinite
+    parameters, a NaN state, states and a parameter large enough to
+    overflow, a denormal state) beside ordinary ones: the band LU's outcomes
+    (zero pivot first, or a non-finite value meeting a skipped zero of the
+    dense LU) and the solves' non-finite checks give the oracle's dense-LU
+    retcodes and step counts, bit for bit."""
+    N = 12
+    say(f"brusselator non-finite inputs {alg} {precision} lanes={lanes} N={N}")
+    w = batch_range(N, seed=84, lo=8, hi=14, tf_max=1.0)
+    p, u0 = w["p"].copy(), w["u0"].copy()
+    p[1, 2] = np.nan
+    p[0, 3] = np.inf
+    p[2, 4] = np.nan
+    u0[3, 5] = np.nan
+    u0[:, 6] *= 1e150
+    p[1, 7] = 1e200
+    u0[0, 8] = 1e-320
+    tol = dict(abstol=1e-8, reltol=1e-6)
+    g = gpu("brusselator", u0, p, 0.0, w["tf"], nvec=w["nvec"], lanes=lanes, alg=alg,
+            precision=precision, schedule="persistent", **tol)
+    c = cpu(oracle, "brusselator", u0, p, 0.0, w["tf"], nvec=w["nvec"], norm_lanes=lanes,
+            alg=alg, precision=precision, **tol)
+    assert (c["retcode"][2:7] == 5).all() and c["retcode"][0] == 1
+    assert_identical(g, c, nvec=w["nvec"])
+    np.testing.assert_array_equal(g.nstiff, c["nstiff"])
