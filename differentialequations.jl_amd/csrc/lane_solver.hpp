@@ -64,6 +64,11 @@ struct LaneArgs {
   int64_t* handoff;           // two-phase AutoSwitch: trajectories for the second pass
   unsigned long long* handoff_count;
   const int32_t* iter_cap;    // TRUST pass: per-trajectory iteration cap (NULL: none)
+  double* ev_t;               // callbacks: points saved at events, [ntraj][ev_cap] times,
+  double* ev_u;               //   [ntraj][ev_cap][N] states (NULL: none)
+  int32_t* ev_n;              //   [ntraj] points produced (may exceed ev_cap)
+  int32_t ev_cap;
+  int32_t pad1;
 };
 
 // Trajectory scheduled at position k of the launch order; -1 for an entry
@@ -326,7 +331,20 @@ struct Lane {
   double gprev[kNcb > 0 ? kNcb : 1];
   int32_t nevent;
   double tevent;
+  int32_t nevsave;  // points saved at events so far
   int32_t cap;  // TRUST pass: iteration cap of this trajectory
+
+  // point k of this trajectory's event points: (t, v), stored while k < ev_cap
+  __device__ __forceinline__ void save_event(const LaneArgs& a, const double* v) {
+    if (!a.ev_t) return;
+    if (nevsave < a.ev_cap) {
+      const int64_t at = (int64_t)i * a.ev_cap + nevsave;
+      a.ev_t[at] = t;
+#pragma unroll
+      for (int c = 0; c < N; c++) a.ev_u[at * N + c] = v[c];
+    }
+    nevsave++;
+  }
 
   // ------------------------------------------------------------ save times
   __device__ __forceinline__ double save_time(const LaneArgs& a, int k) const {
@@ -506,6 +524,7 @@ struct Lane {
     fsal_init();
     nevent = 0;
     tevent = __builtin_nan("");
+    nevsave = 0;
     if constexpr (kNcb > 0) {
 #pragma unroll
       for (int k = 0; k < kNcb; k++) gprev[k] = M::cb_g(k, u, p, t0);
@@ -1152,16 +1171,21 @@ struct Lane {
         // apply_callback!: the continuous event's affect at its time, then
         // every discrete callback whose condition holds at the step end;
         // any affect re-evaluates the FSAL stage (the state may have moved)
+        // save_positions: (t, u) just before and / or just after an affect
         bool fired = false, term = false;
         if (ev >= 0) {
+          if (M::cb_save(ev) & 1) save_event(a, u);
           M::cb_affect(ev, u, p, t);
+          if (M::cb_save(ev) & 2) save_event(a, u);
           fired = true;
           term = term || M::cb_term(ev);
           nevent++;
         }
         for (int k = 0; k < kNcb; k++) {
           if (M::cb_kind(k) == 1 && M::cb_g(k, u, p, t) != 0.0) {
+            if (M::cb_save(k) & 1) save_event(a, u);
             M::cb_affect(k, u, p, t);
+            if (M::cb_save(k) & 2) save_event(a, u);
             fired = true;
             term = term || M::cb_term(k);
             nevent++;
@@ -1225,6 +1249,7 @@ struct Lane {
     if (a.dt_first) a.dt_first[i] = dtfirst;
     if (a.nevent) a.nevent[i] = nevent;
     if (a.t_event) a.t_event[i] = tevent;
+    if (a.ev_n) a.ev_n[i] = nevsave;
     if constexpr (kNout > 0) {
       if (a.out) {
         double o[kNout];

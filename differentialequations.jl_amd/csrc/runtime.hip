@@ -36,7 +36,7 @@ struct DevBuf {
 enum {
   BUF_U0 = 0, BUF_P, BUF_T0, BUF_TF, BUF_NVEC, BUF_SAVEAT, BUF_DT0, BUF_HINT, BUF_US,
   BUF_UF, BUF_TFIN, BUF_RC, BUF_NA, BUF_NR, BUF_NF, BUF_NS, BUF_ORDER,
-  BUF_DTF, BUF_NEV, BUF_TEV, BUF_OUT, BUF_RED, BUF_CAP, BUF_COUNT
+  BUF_DTF, BUF_NEV, BUF_TEV, BUF_OUT, BUF_RED, BUF_CAP, BUF_EVT, BUF_EVU, BUF_EVN, BUF_COUNT
 };
 
 // Device memory a solve uses beyond its inputs and outputs: the work-queue
@@ -374,9 +374,10 @@ static int check_user_spec(const mideq_user_model* m, std::string* why) {
     const mideq_callback& cb = m->callbacks[k];
     if (!cb.condition || cb.kind < 0 || cb.kind > 1 || cb.direction < -1 || cb.direction > 1 ||
         cb.interp_points < 0 || cb.interp_points > 1000 || cb.terminate < 0 ||
-        cb.terminate > 1) {
-      char b[96];
-      snprintf(b, sizeof b, "callback %d: bad kind, direction, interp_points or condition", k);
+        cb.terminate > 1 || cb.save_positions < 0 || cb.save_positions > 3) {
+      char b[112];
+      snprintf(b, sizeof b,
+               "callback %d: bad kind, direction, interp_points, save_positions or condition", k);
       *why = b;
       return MIDEQ_E_ARG;
     }
@@ -470,6 +471,7 @@ static std::string user_model_source(const mideq_user_model* m) {
     s += table("cb_kind", [](const mideq_callback& c) { return (int)c.kind; });
     s += table("cb_dir", [](const mideq_callback& c) { return (int)c.direction; });
     s += table("cb_term", [](const mideq_callback& c) { return (int)c.terminate; });
+    s += table("cb_save", [](const mideq_callback& c) { return (int)c.save_positions; });
     s += table("cb_interp", [](const mideq_callback& c) {
       return c.interp_points > 0 ? (int)c.interp_points : 1;  // 0: the step end only
     });
@@ -948,6 +950,10 @@ static int run_solver(mideq_ctx* c, const mideq_problem* P, const mideq_options*
   a.dt_first = out->dt_first;
   a.nevent = out->nevent;
   a.t_event = out->t_event;
+  a.ev_t = out->ev_t;
+  a.ev_u = out->ev_u;
+  a.ev_n = out->ev_n;
+  a.ev_cap = out->ev_cap;
   a.out = out->out;
   a.queue = Sc.queue;
   a.order = d_order;
@@ -1054,13 +1060,28 @@ static int run_solver(mideq_ctx* c, const mideq_problem* P, const mideq_options*
   return MIDEQ_OK;
 }
 
+// Event points (mideq_outputs.ev_*): user models with callbacks only (their
+// kernels are the ones that apply affects), both arrays or neither.
+static int validate_outputs(mideq_ctx* c, const mideq_problem* P, const mideq_outputs* out) {
+  if (!out) return set_err(c, MIDEQ_E_ARG, "null outputs");
+  if (!out->ev_t && !out->ev_u) return MIDEQ_OK;
+  if (!out->ev_t || !out->ev_u)
+    return set_err(c, MIDEQ_E_ARG, "ev_t and ev_u are given together or not at all");
+  if (out->ev_cap < 1) return set_err(c, MIDEQ_E_ARG, "ev_cap must be >= 1 with ev_t");
+  const UserModel* um = user_model(c, P->model);
+  if (!um || um->ncb == 0)
+    return set_err(c, MIDEQ_E_UNSUPPORTED,
+                   "event points (ev_t) need a user model with callbacks");
+  return MIDEQ_OK;
+}
+
 extern "C" int mideq_ensemble_solve_device(mideq_ctx* c, const mideq_problem* P,
                                            const mideq_options* O, mideq_outputs* out,
                                            void* stream) {
   if (!c) return MIDEQ_E_ARG;
   int rc = validate(c, P, O);
   if (rc) return rc;
-  if (!out) return set_err(c, MIDEQ_E_ARG, "null outputs");
+  if ((rc = validate_outputs(c, P, out))) return rc;
   HIP_TRY(c, hipSetDevice(c->device));
   hipStream_t s = stream ? (hipStream_t)stream : c->stream;
   Scratch Sc;
@@ -1074,7 +1095,7 @@ extern "C" int mideq_ensemble_solve_device_noalloc(mideq_ctx* c, const mideq_pro
   if (!c) return MIDEQ_E_ARG;
   int rc = validate(c, P, O);
   if (rc) return rc;
-  if (!out) return set_err(c, MIDEQ_E_ARG, "null outputs");
+  if ((rc = validate_outputs(c, P, out))) return rc;
   if (O->graph)
     return set_err(c, MIDEQ_E_ARG,
                    "graph = 1 instantiates a graph, which allocates; capture this call in a "
@@ -1116,7 +1137,7 @@ extern "C" int mideq_ensemble_solve(mideq_ctx* c, const mideq_problem* P,
   if (!c) return MIDEQ_E_ARG;
   int rc = validate(c, P, O);
   if (rc) return rc;
-  if (!out) return set_err(c, MIDEQ_E_ARG, "null outputs");
+  if ((rc = validate_outputs(c, P, out))) return rc;
   if (O->ns > 0 && (rc = check_saveat_host(c, O->saveat, O->ns))) return rc;
   HIP_TRY(c, hipSetDevice(c->device));
   hipStream_t s = c->stream;
@@ -1228,6 +1249,13 @@ extern "C" int mideq_ensemble_solve(mideq_ctx* c, const mideq_problem* P,
   if (out->dt_first && (rc = ensure_buf(c, BUF_DTF, 8 * N, (void**)&dOut.dt_first))) return rc;
   if (out->nevent && (rc = ensure_buf(c, BUF_NEV, 4 * N, (void**)&dOut.nevent))) return rc;
   if (out->t_event && (rc = ensure_buf(c, BUF_TEV, 8 * N, (void**)&dOut.t_event))) return rc;
+  const size_t nev = out->ev_t ? (size_t)out->ev_cap * (size_t)N : 0;
+  if (out->ev_t) {
+    dOut.ev_cap = out->ev_cap;
+    if ((rc = ensure_buf(c, BUF_EVT, 8 * nev, (void**)&dOut.ev_t))) return rc;
+    if ((rc = ensure_buf(c, BUF_EVU, 8 * nev * (size_t)n, (void**)&dOut.ev_u))) return rc;
+    if (out->ev_n && (rc = ensure_buf(c, BUF_EVN, 4 * N, (void**)&dOut.ev_n))) return rc;
+  }
   // output_func values: kept on the device when only their reduction is asked for
   UserModel* um_out = user_model(c, P->model);
   const int nout = um_out ? um_out->nout : 0;
@@ -1257,6 +1285,12 @@ extern "C" int mideq_ensemble_solve(mideq_ctx* c, const mideq_problem* P,
   if (out->dt_first) HIP_TRY(c, hipMemcpyAsync(out->dt_first, dOut.dt_first, 8 * N, hipMemcpyDeviceToHost, s));
   if (out->nevent) HIP_TRY(c, hipMemcpyAsync(out->nevent, dOut.nevent, 4 * N, hipMemcpyDeviceToHost, s));
   if (out->t_event) HIP_TRY(c, hipMemcpyAsync(out->t_event, dOut.t_event, 8 * N, hipMemcpyDeviceToHost, s));
+  if (out->ev_t) {
+    HIP_TRY(c, hipMemcpyAsync(out->ev_t, dOut.ev_t, 8 * nev, hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipMemcpyAsync(out->ev_u, dOut.ev_u, 8 * nev * (size_t)n, hipMemcpyDeviceToHost, s));
+    if (out->ev_n)
+      HIP_TRY(c, hipMemcpyAsync(out->ev_n, dOut.ev_n, 4 * N, hipMemcpyDeviceToHost, s));
+  }
   if (out->out && dOut.out)
     HIP_TRY(c, hipMemcpyAsync(out->out, dOut.out, sizeof(double) * (size_t)nout * N,
                               hipMemcpyDeviceToHost, s));

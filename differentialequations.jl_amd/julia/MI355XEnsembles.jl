@@ -73,6 +73,7 @@ struct MideqOutputs
     dt_first::Ptr{Float64}
     nevent::Ptr{Int32}; t_event::Ptr{Float64}
     out::Ptr{Float64}; reduced::Ptr{Float64}
+    ev_t::Ptr{Float64}; ev_u::Ptr{Float64}; ev_n::Ptr{Int32}; ev_cap::Int32; reserved::Int32
 end
 
 # ---- registered right-hand sides ------------------------------------------
@@ -86,7 +87,11 @@ register_model!(f, name::Symbol) = (MODEL_FOR[f] = MODEL_IDS[name]; f)
 struct MideqCallback
     kind::Int32; direction::Int32; terminate::Int32; interp_points::Int32
     condition::Ptr{UInt8}; affect::Ptr{UInt8}
+    save_positions::Int32; reserved::Int32
 end
+
+# registered models whose callbacks save points at events (save_positions)
+const EVENT_SAVES = IdDict{Any, Bool}()
 
 struct MideqUserModel
     n::Int32; np::Int32; rhs::Ptr{UInt8}; jac::Ptr{UInt8}
@@ -96,12 +101,13 @@ end
 
 """
     DeviceCallback(kind, condition; affect = nothing, direction = 0,
-                   terminate = false, interp_points = 10)
+                   terminate = false, interp_points = 10, save_positions = (true, true))
 
 The device form of `ContinuousCallback(condition, affect!)` (`kind = :continuous`)
 or `DiscreteCallback(condition, affect!)` (`kind = :discrete`) (test/qa/qa.jl:26,36):
 `condition` a C++ expression of `u[]`, `p[]`, `t`; `affect` statements changing
-`u[]`; compiled into a model by `register_model!(...; callbacks)`.
+`u[]`; compiled into a model by `register_model!(...; callbacks)`.  `save_positions`
+as upstream: the state just before / just after each affect joins the solution.
 """
 Base.@kwdef struct DeviceCallback
     kind::Symbol
@@ -110,7 +116,9 @@ Base.@kwdef struct DeviceCallback
     direction::Int = 0
     terminate::Bool = false
     interp_points::Int = 10
+    save_positions::Tuple{Bool, Bool} = (true, true)
 end
+save_bits(cb::DeviceCallback) = Int32(cb.save_positions[1]) | (Int32(cb.save_positions[2]) << 1)
 DeviceCallback(kind::Symbol, condition::AbstractString; kw...) =
     DeviceCallback(; kind, condition = String(condition), kw...)
 
@@ -144,7 +152,8 @@ function register_model!(f, n::Integer, np::Integer, rhs::AbstractString;
     rc = GC.@preserve r j o conds affs begin
         cbs = [MideqCallback(cb.kind === :discrete ? 1 : 0, cb.direction, cb.terminate,
                              cb.interp_points, pointer(conds[k]),
-                             cb.affect === nothing ? C_NULL : pointer(affs[k]))
+                             cb.affect === nothing ? C_NULL : pointer(affs[k]),
+                             save_bits(cb), 0)
                for (k, cb) in enumerate(callbacks)]
         GC.@preserve cbs begin
             m = MideqUserModel(n, np, pointer(r), j === nothing ? C_NULL : pointer(j),
@@ -156,6 +165,7 @@ function register_model!(f, n::Integer, np::Integer, rhs::AbstractString;
     end
     check(ctx, rc, "mideq_model_register")
     MODEL_FOR[f] = id[]
+    EVENT_SAVES[f] = any(cb -> save_bits(cb) != 0, callbacks)
     return f
 end
 
@@ -246,7 +256,7 @@ struct MideqStats
 end
 
 function _device_solve(probs, alg, ealg::EnsembleMI355X, f, sa; abstol, reltol, dt, dtmin,
-                       dtmax, maxiters, lanes = 0)
+                       dtmax, maxiters, lanes = 0, event_cap = 16)
     M = length(probs)
     n = maximum(length(pr.u0) for pr in probs)
     np = max(length(probs[1].p), 1)   # a user model without parameters reads one row
@@ -263,9 +273,15 @@ function _device_solve(probs, alg, ealg::EnsembleMI355X, f, sa; abstol, reltol, 
     UF = zeros(Float64, M, n); TFo = zeros(Float64, M)
     RC = zeros(Int8, M); NA = zeros(Int32, M); NR = zeros(Int32, M)
     NF = zeros(Int32, M); NS = zeros(Int32, M)
+    # points saved at callback events: the ABI's [ntraj][cap] times and
+    # [ntraj][cap][n] states are (cap, M) and (n, cap, M) column-major
+    ev = get(EVENT_SAVES, f, false)
+    EVT = ev ? fill(NaN, event_cap, M) : zeros(Float64, 0)
+    EVU = ev ? fill(NaN, n, event_cap, M) : zeros(Float64, 0)
+    EVN = ev ? zeros(Int32, M) : zeros(Int32, 0)
     st = Ref(MideqStats(0, 0, 0, 0, 0, 0, 0, 0))
     ctx = context(ealg.device)
-    GC.@preserve U0 P T0 TF nvec sa US UF TFo RC NA NR NF NS begin
+    GC.@preserve U0 P T0 TF nvec sa US UF TFo RC NA NR NF NS EVT EVU EVN begin
         prob_c = MideqProblem(MODEL_FOR[f], n, np, 0, M, pointer(U0), pointer(P),
                               pointer(T0), pointer(TF), T0[1], TF[1],
                               MODEL_FOR[f] == 4 ? pointer(nvec) : C_NULL)
@@ -277,7 +293,9 @@ function _device_solve(probs, alg, ealg::EnsembleMI355X, f, sa; abstol, reltol, 
                              0, 0, lanes, 0, 0, 0, C_NULL, 0, 0, C_NULL)
         out_c = MideqOutputs(ns > 0 ? pointer(US) : C_NULL, pointer(UF), pointer(TFo),
                              pointer(RC), pointer(NA), pointer(NR), pointer(NF), pointer(NS),
-                             C_NULL, C_NULL, C_NULL, C_NULL, C_NULL)
+                             C_NULL, C_NULL, C_NULL, C_NULL, C_NULL,
+                             ev ? pointer(EVT) : C_NULL, ev ? pointer(EVU) : C_NULL,
+                             ev ? pointer(EVN) : C_NULL, ev ? event_cap : 0, 0)
         rc = ccall((:mideq_ensemble_solve, LIB[]), Cint,
                    (Ptr{Cvoid}, Ref{MideqProblem}, Ref{MideqOptions}, Ref{MideqOutputs}),
                    ctx.h, prob_c, opt_c, out_c)
@@ -285,7 +303,7 @@ function _device_solve(probs, alg, ealg::EnsembleMI355X, f, sa; abstol, reltol, 
         rc = ccall((:mideq_last_stats, LIB[]), Cint, (Ptr{Cvoid}, Ref{MideqStats}), ctx.h, st)
         check(ctx, rc, "mideq_last_stats")
     end
-    return (; US, UF, TFo, RC, NA, NR, NF, NS, lanes = st[].lanes)
+    return (; US, UF, TFo, RC, NA, NR, NF, NS, EVT, EVU, EVN, ev, lanes = st[].lanes)
 end
 
 # Dense output sol(t) (test/core.jl:51-58): the trajectory solved again with
@@ -338,12 +356,23 @@ function _solve_batch(ens, alg, ealg::EnsembleMI355X, f, II; saveat, abstol, rel
             nk = length(pr.u0)
             # a scalar problem (u0 isa Number) gets scalar states, as upstream
             val(row) = pr.u0 isa Number ? row[1] : Vector{Float64}(row)
+            back = r.TFo[k] < pr.tspan[1]
+            # points saved at events, in the order of integration
+            m = r.ev ? min(Int(r.EVN[k]), size(r.EVT, 1)) : 0
+            et = r.ev ? r.EVT[1:m, k] : Float64[]
+            eu = [val(r.EVU[1:nk, j, k]) for j in 1:m]
             if ns > 0
                 keep = [j for j in 1:ns if !isnan(r.US[k, 1, j])]
-                r.TFo[k] < pr.tspan[1] && reverse!(keep)   # backward: order of integration
+                back && reverse!(keep)   # backward: order of integration
                 ts = sa[keep]; us = [val(r.US[k, 1:nk, j]) for j in keep]
+                if m > 0   # merged; a saveat point first at an equal time
+                    allt = vcat(ts, et); allu = vcat(us, eu)
+                    perm = sortperm(back ? -allt : allt; alg = Base.Sort.DEFAULT_STABLE)
+                    ts = allt[perm]; us = allu[perm]
+                end
             else
-                ts = [pr.tspan[1], r.TFo[k]]; us = [val([pr.u0;]), val(r.UF[k, 1:nk])]
+                ts = vcat([pr.tspan[1]], et, [r.TFo[k]])
+                us = vcat([val([pr.u0;])], eu, [val(r.UF[k, 1:nk])])
             end
             if dense
                 lanes = r.lanes

@@ -36,7 +36,8 @@ class Problem(C.Structure):
 
 class CallbackSpec(C.Structure):
     _fields_ = [("kind", C.c_int32), ("direction", C.c_int32), ("terminate", C.c_int32),
-                ("interp_points", C.c_int32), ("condition", C.c_char_p), ("affect", C.c_char_p)]
+                ("interp_points", C.c_int32), ("condition", C.c_char_p), ("affect", C.c_char_p),
+                ("save_positions", C.c_int32), ("reserved", C.c_int32)]
 
 
 class UserModelSpec(C.Structure):
@@ -68,7 +69,8 @@ class Outputs(C.Structure):
                 ("retcode", C.c_void_p), ("naccept", C.c_void_p), ("nreject", C.c_void_p),
                 ("nf", C.c_void_p), ("nstiff", C.c_void_p), ("dt_first", C.c_void_p),
                 ("nevent", C.c_void_p), ("t_event", C.c_void_p), ("out", C.c_void_p),
-                ("reduced", C.c_void_p)]
+                ("reduced", C.c_void_p), ("ev_t", C.c_void_p), ("ev_u", C.c_void_p),
+                ("ev_n", C.c_void_p), ("ev_cap", C.c_int32), ("reserved", C.c_int32)]
 
 
 class Stats(C.Structure):

@@ -100,31 +100,42 @@ class ContinuousCallback:
     inside a step is the event; `affect` (statements changing u[]) runs at
     the located event time.  direction: 0 both crossings, 1 upcrossings,
     -1 downcrossings; terminate: end the trajectory (ReturnCode.Terminated)
-    after the affect; interp_points: sign checks per step."""
+    after the affect; interp_points: sign checks per step; save_positions:
+    save (t, u) just before / just after the affect (upstream's default
+    (True, True)), listed in the solution beside the saveat points."""
     condition: str
     affect: Optional[str] = None
     direction: int = 0
     terminate: bool = False
     interp_points: int = 10
+    save_positions: tuple = (True, True)
 
     def spec(self):
         return _abi.CallbackSpec(0, int(self.direction), int(bool(self.terminate)),
                                  int(self.interp_points), self.condition.encode(),
-                                 None if self.affect is None else self.affect.encode())
+                                 None if self.affect is None else self.affect.encode(),
+                                 _save_bits(self.save_positions), 0)
 
 
 @dataclass(frozen=True)
 class DiscreteCallback:
     """DiscreteCallback(condition, affect!) (qa.jl:36) for the GPU: the
     affect runs at every step end where `condition` (an expression of u[],
-    p[], t) is nonzero."""
+    p[], t) is nonzero.  save_positions as for ContinuousCallback."""
     condition: str
     affect: Optional[str] = None
     terminate: bool = False
+    save_positions: tuple = (True, True)
 
     def spec(self):
         return _abi.CallbackSpec(1, 0, int(bool(self.terminate)), 0, self.condition.encode(),
-                                 None if self.affect is None else self.affect.encode())
+                                 None if self.affect is None else self.affect.encode(),
+                                 _save_bits(self.save_positions), 0)
+
+
+def _save_bits(save_positions):
+    before, after = save_positions
+    return (1 if before else 0) | (2 if after else 0)
 
 
 @dataclass(frozen=True)
@@ -449,26 +460,50 @@ class EnsembleArrays:
     reduced: Optional[np.ndarray] = None  # [nout] their on-device reduction
     save_layout: str = "soa"
     nvec: Optional[np.ndarray] = None     # [N] state sizes of a ragged batch
+    ev_t: Optional[np.ndarray] = None     # [N, cap] times of the points saved at events
+    ev_u: Optional[np.ndarray] = None     # [N, cap, n] their states
+    ev_n: Optional[np.ndarray] = None     # [N] points produced (may exceed cap)
 
     def saved(self, i):
         """Saved states of trajectory i (0-based), [ns, n_i]."""
         vals = self.us[i] if self.save_layout == "trajectory" else self.us[:, :, i]
         return vals[:, :int(self.nvec[i])] if self.nvec is not None else vals
 
+    def events(self, i):
+        """(times, states) of the points trajectory i saved at callback events,
+        in the order of integration (at most the arrays' capacity)."""
+        if self.ev_t is None:
+            return np.empty(0), np.empty((0, self.u0.shape[0]))
+        m = min(int(self.ev_n[i]), self.ev_t.shape[1])
+        return self.ev_t[i, :m].copy(), self.ev_u[i, :m].copy()
+
     def solution(self, i, prob=None):
-        """ODESolution of trajectory i (0-based)."""
+        """ODESolution of trajectory i (0-based).  Points saved at callback
+        events are listed with the saveat points in the order of integration
+        (a saveat point first at an equal time, as upstream saves it before
+        applying the callback); without saveat, between u0 and the final
+        state."""
         st = Stats(int(self.naccept[i]), int(self.nreject[i]), int(self.nf[i]))
         rc = ReturnCode(int(self.retcode[i]))
+        back = self.t_final is not None and self.t_final[i] < self.t0[i]
+        et, eu = self.events(i)
         if self.us is not None:
             vals = self.saved(i)
             ok = ~np.isnan(vals).any(axis=1)
             ts, us = self.saveat[ok], vals[ok]
-            if self.t_final is not None and self.t_final[i] < self.t0[i]:
+            if back:
                 ts, us = ts[::-1], us[::-1]  # a backward solve: in the order of integration
-            return ODESolution(ts.copy(), [v.copy() for v in us], rc, st, prob)
+            if len(et):
+                allt = np.concatenate([ts, et])
+                allu = list(us) + list(eu)
+                order = np.argsort(-allt if back else allt, kind="stable")
+                ts, us = allt[order], [allu[k] for k in order]
+            return ODESolution(np.array(ts, dtype=np.float64), [np.array(v) for v in us], rc,
+                               st, prob)
         m = int(self.nvec[i]) if self.nvec is not None else self.u0.shape[0]
-        return ODESolution(np.array([self.t0[i], self.t_final[i]]),
-                           [self.u0[:m, i].copy(), self.u_final[:m, i].copy()], rc, st, prob)
+        ts = np.concatenate([[self.t0[i]], et, [self.t_final[i]]])
+        us = [self.u0[:m, i].copy()] + [v[:m] for v in eu] + [self.u_final[:m, i].copy()]
+        return ODESolution(ts, us, rc, st, prob)
 
 
 @dataclass
@@ -617,7 +652,7 @@ def _ptr(a):
 def device_solve(f, u0, p, t0, tf, nvec, alg, ealg, saveat=None, abstol=1e-6, reltol=1e-3,
                  dt=0.0, dtmin=0.0, dtmax=0.0, maxiters=0, dt0=None, alg_hint=None,
                  order=None, block=0, blocks_per_cu=0, callback=None, output=None,
-                 reduction=None, keep_states=True):
+                 reduction=None, keep_states=True, event_cap=16):
     """One call of mideq_ensemble_solve on host arrays -> EnsembleArrays.
     callback: ContinuousCallback / DiscreteCallback / CallbackSet, compiled
     into the right-hand side (a UserModel, or a built-in model's source).
@@ -706,6 +741,12 @@ def device_solve(f, u0, p, t0, tf, nvec, alg, ealg, saveat=None, abstol=1e-6, re
     if isinstance(f, UserModel) and f.callbacks and keep_states:
         res["nevent"] = np.zeros(N, np.int32)
         res["t_event"] = np.empty(N)
+        if any(_save_bits(getattr(cb, "save_positions", (False, False)))
+               for cb in f.callbacks):
+            # points saved at events (save_positions): event_cap per trajectory
+            res["ev_t"] = np.full((N, int(event_cap)), np.nan)
+            res["ev_u"] = np.full((N, int(event_cap), n), np.nan)
+            res["ev_n"] = np.zeros(N, np.int32)
     nout = f.output.nout if isinstance(f, UserModel) and f.output is not None else 0
     if nout and keep_states:
         res["out"] = np.empty((nout, N))
@@ -715,6 +756,8 @@ def device_solve(f, u0, p, t0, tf, nvec, alg, ealg, saveat=None, abstol=1e-6, re
     out.us = _ptr(us)
     for k, v in res.items():
         setattr(out, k, _ptr(v))
+    if "ev_t" in res:
+        out.ev_cap = int(event_cap)
     ctx.solve(P, O, out)
     st = ctx.stats()
     return EnsembleArrays(saveat=saveat, us=us, t0=t0, u0=u0, kernel_ms=st.kernel_ms,

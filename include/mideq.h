@@ -184,6 +184,14 @@ typedef struct {
   double *t_event;      /* [ntraj] time of the last callback affect (NaN if none) */
   double *out;          /* [nout][ntraj] output_func values (user models with an output) */
   double *reduced;      /* [nout] the reduction of `out` over the trajectories */
+  /* Points saved at callback events (mideq_callback.save_positions), in the
+   * order of integration; user models with callbacks, one-lane kernels.
+   * ev_t / ev_u may be NULL together (no event points). */
+  double *ev_t;         /* [ntraj][ev_cap] times */
+  double *ev_u;         /* [ntraj][ev_cap][n] states */
+  int32_t *ev_n;        /* [ntraj] points produced; only the first ev_cap are stored */
+  int32_t ev_cap;       /* points per trajectory the arrays hold (>= 1 with ev_t) */
+  int32_t reserved;
 } mideq_outputs;
 
 /* Timing and launch record of the last solve on a context. */
@@ -264,8 +272,10 @@ typedef enum {
  * Illinois bracketing, the integrator moved back to it, and the affect
  * applied there (the earliest event of all continuous callbacks).  A discrete
  * callback fires at a step end where its condition is nonzero.  Every affect
- * re-evaluates the FSAL stage.  Saved values are the saveat grid's (no extra
- * points at events). */
+ * re-evaluates the FSAL stage.  save_positions = (before, after), upstream's
+ * default (true, true): each applied affect saves (t, u) just before and/or
+ * just after it into mideq_outputs.ev_t / ev_u, beside the saveat grid (the
+ * points a solution lists at its events). */
 #define MIDEQ_USER_NCBMAX 8
 #define MIDEQ_USER_NOUTMAX 16
 typedef enum { MIDEQ_CB_CONTINUOUS = 0, MIDEQ_CB_DISCRETE = 1 } mideq_callback_kind;
@@ -280,6 +290,9 @@ typedef struct {
                             as expensive */
   const char *condition; /* double expression of u[], p[], t */
   const char *affect;    /* statements changing u[] (double) from u, p, t; NULL: none */
+  int32_t save_positions; /* bit 0: save (t, u) before the affect, bit 1: after it
+                             (upstream's (true, true) is 3); needs mideq_outputs.ev_t */
+  int32_t reserved;
 } mideq_callback;
 
 typedef struct {

@@ -335,6 +335,7 @@ static struct {
   int tdep;
   int ncb;
   int kind[8], dir[8], term[8], ip[8];
+  int save[8]; /* save_positions bits: 1 before the affect, 2 after */
   oracle_user_cond cond;
   oracle_user_affect affect;
 } g_user;
@@ -352,7 +353,30 @@ int oracle_set_user_callbacks(int32_t ncb, const int32_t *kind, const int32_t *d
   }
   g_user.cond = cond;
   g_user.affect = affect;
+  for (int k = 0; k < 8; k++) g_user.save[k] = 0;
   return 0;
+}
+
+int oracle_set_user_callback_saves(int32_t ncb, const int32_t *save_positions) {
+  if (ncb != g_user.ncb) return -1;
+  for (int k = 0; k < ncb; k++) {
+    if (save_positions[k] < 0 || save_positions[k] > 3) return -1;
+    g_user.save[k] = save_positions[k];
+  }
+  return 0;
+}
+
+/* A point saved at a callback event (save_positions): (t, u) as point k of
+ * trajectory idx while k < ev_cap; the count goes on past the capacity. */
+static void save_event(const oracle_options *o, int64_t idx, int n, double t, const double *u,
+                       oracle_result *res) {
+  if (!o->ev_t) return;
+  if (res->nevsave < o->ev_cap) {
+    const long long at = (long long)idx * o->ev_cap + res->nevsave;
+    o->ev_t[at] = t;
+    for (int c = 0; c < n; c++) o->ev_u[at * n + c] = u[c];
+  }
+  res->nevsave++;
 }
 
 int oracle_set_user_model(int32_t n, int32_t np, oracle_user_f64 f64, oracle_user_f32 f32,
@@ -1489,14 +1513,18 @@ static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
          * the FSAL stage */
         int fired = 0, term = 0;
         if (ev >= 0) {
+          if (g_user.save[ev] & 1) save_event(o, idx, n, t, uprev, res);
           g_user.affect(ev, uprev, m->p, t);
+          if (g_user.save[ev] & 2) save_event(o, idx, n, t, uprev, res);
           fired = 1;
           term |= g_user.term[ev];
           res->nevent++;
         }
         for (int k = 0; k < ncb; k++) {
           if (g_user.kind[k] == 1 && g_user.cond(k, uprev, m->p, t) != 0.0) {
+            if (g_user.save[k] & 1) save_event(o, idx, n, t, uprev, res);
             g_user.affect(k, uprev, m->p, t);
+            if (g_user.save[k] & 2) save_event(o, idx, n, t, uprev, res);
             fired = 1;
             term |= g_user.term[k];
             res->nevent++;
@@ -1638,6 +1666,7 @@ static void *ens_worker(void *arg) {
       if (J->dt_first) J->dt_first[i] = res.dt_first_accept;
       if (J->nevent) J->nevent[i] = res.nevent;
       if (J->t_event) J->t_event[i] = res.t_event;
+      if (J->opt->ev_n) J->opt->ev_n[i] = res.nevsave;
     }
   }
   free(pl);

@@ -74,6 +74,9 @@ typedef void (*oracle_user_affect)(int32_t k, double *u, const double *p, double
 int oracle_set_user_callbacks(int32_t ncb, const int32_t *kind, const int32_t *direction,
                               const int32_t *terminate, const int32_t *interp_points,
                               oracle_user_cond cond, oracle_user_affect affect);
+/* save_positions of those callbacks (bit 0 before the affect, bit 1 after;
+ * oracle_set_user_callbacks resets them to 0) */
+int oracle_set_user_callback_saves(int32_t ncb, const int32_t *save_positions);
 
 enum {
   ORACLE_ALG_TSIT5 = 0,
@@ -117,6 +120,14 @@ typedef struct {
   int32_t save_layout;    /* ensemble saves: 0 [ns][n][ntraj] (trajectory fastest),
                              1 [ntraj][ns][n] (component fastest); n = the layout's rows
                              (the largest state size of a ragged batch) */
+  /* points saved at callback events (save_positions), or NULL: trajectory
+   * i's k-th at ev_t[i*ev_cap + k], ev_u[(i*ev_cap + k)*n + c] while
+   * k < ev_cap; ev_n[i] (ensembles) the number produced */
+  double *ev_t;
+  double *ev_u;
+  int32_t *ev_n;
+  int32_t ev_cap;
+  int32_t reserved;
 } oracle_options;
 
 typedef struct {
@@ -131,6 +142,8 @@ typedef struct {
   int32_t first_stiff_iter; /* iteration of the first switch to Rodas5, -1 if none */
   int32_t nevent;         /* callback affects applied (user model callbacks) */
   double t_event;         /* time of the last one, NaN if none */
+  int32_t nevsave;        /* points saved at events (save_positions) */
+  int32_t reserved;
 } oracle_result;
 
 /* Single trajectory.  u0[n], p[np].  out_save (may be NULL) is [ns][n]. */

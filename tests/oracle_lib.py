@@ -43,6 +43,11 @@ class Options(C.Structure):
         ("dt0_in", _dp),
         ("alg_hint", C.POINTER(C.c_int8)),
         ("save_layout", C.c_int32),
+        ("ev_t", _dp),
+        ("ev_u", _dp),
+        ("ev_n", C.POINTER(C.c_int32)),
+        ("ev_cap", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 
@@ -62,6 +67,8 @@ class Result(C.Structure):
         ("first_stiff_iter", C.c_int32),
         ("nevent", C.c_int32),
         ("t_event", C.c_double),
+        ("nevsave", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 
@@ -188,8 +195,10 @@ class Oracle:
         return out, rec[:m]
 
     # ------------------------------------------------------ ensemble (SoA)
-    def ensemble(self, model, u0, p, t0, tf, nthreads=1, nvec=None, **kw):
-        """u0: [n, N], p: [np, N], t0/tf: [N] or scalars.  Returns dict of SoA arrays."""
+    def ensemble(self, model, u0, p, t0, tf, nthreads=1, nvec=None, event_cap=0, **kw):
+        """u0: [n, N], p: [np, N], t0/tf: [N] or scalars.  Returns dict of SoA arrays.
+        event_cap > 0: the points saved at callback events too (ev_t [N, cap],
+        ev_u [N, cap, n], ev_n [N], NaN past the stored ones)."""
         u0 = np.ascontiguousarray(u0, dtype=np.float64)
         p = np.ascontiguousarray(p, dtype=np.float64)
         n, N = u0.shape
@@ -216,6 +225,12 @@ class Oracle:
         nev = np.zeros(N, dtype=np.int32)
         tev = np.zeros(N)
         nv = None if nvec is None else np.ascontiguousarray(nvec, dtype=np.int32)
+        if event_cap > 0:
+            ev_t = np.full((N, event_cap), np.nan)
+            ev_u = np.full((N, event_cap, n), np.nan)
+            ev_n = np.zeros(N, dtype=np.int32)
+            o.ev_t, o.ev_u = _ptr(ev_t), _ptr(ev_u)
+            o.ev_n, o.ev_cap = _ptr(ev_n, C.c_int32), event_cap
         r = self.lib.oracle_ensemble_solve(
             MODEL[model], n, p.shape[0], N, _ptr(u0), _ptr(p), _ptr(t0), _ptr(tf),
             _ptr(nv, C.c_int32), C.byref(o), nthreads, _ptr(save), _ptr(uf), _ptr(tfin),
@@ -227,6 +242,8 @@ class Oracle:
                    nstiff=nst, dt_first=dtf, nevent=nev, t_event=tev)
         if save is not None:
             out["saveat"] = save
+        if event_cap > 0:
+            out.update(ev_t=ev_t, ev_u=ev_u, ev_n=ev_n)
         return out
 
     # ------------------------------------------------------------ internals

@@ -146,3 +146,42 @@ def test_terminate_integrating_backward(oracle):
     _check(g, cpu(oracle, "user", np.ones((1, N)), k[None, :], 2.0, 0.0, **kw))
     early = sa[None, :] < g.t_final[:, None]  # not reached going backward
     assert np.isnan(g.us[:, 0, :].T[early]).all()
+
+
+@pytest.mark.parametrize("alg,precision,saves", [("tsit5", "fp64", (True, True)),
+                                                 ("rodas5", "fp64", (False, True)),
+                                                 ("tsit5", "mixed", (True, False))])
+def test_event_points_bitexact(oracle, alg, precision, saves):
+    """save_positions: the points saved just before / after each bounce's
+    affect, bit-exact against the oracle (times, states, counts; a capacity
+    below the count keeps the first points), listed in each solution with
+    the saveat points in time order."""
+    N = 256
+    say(f"event points {alg} {precision} save_positions={saves} N={N}")
+    cb = ContinuousCallback("u[0]", affect="u[1] = -p[1] * u[1];", direction=-1,
+                            save_positions=saves)
+    model = UserModel(2, 2, BALL, callbacks=(cb,))
+    rng = np.random.default_rng(114)
+    h0 = rng.uniform(5.0, 20.0, N)
+    u0 = np.vstack([h0, np.zeros(N)])
+    p = np.vstack([np.full(N, 9.81), rng.uniform(0.8, 0.95, N)])
+    sa = np.linspace(0.0, 6.0, 7)
+    kw = dict(abstol=1e-10, reltol=1e-10, saveat=sa)
+    umh.install(oracle, 2, 2, BALL, callbacks=(cb,))
+    for cap in (16, 3):
+        g = run(model, u0, p, 0.0, 6.0, alg=alg, precision=precision, event_cap=cap, **kw)
+        c = cpu(oracle, "user", u0, p, 0.0, 6.0, alg=alg, precision=precision, event_cap=cap,
+                **kw)
+        _check(g, c)
+        per = 2 if saves == (True, True) else 1
+        np.testing.assert_array_equal(g.ev_n, per * g.nevent)
+        np.testing.assert_array_equal(g.ev_n, c["ev_n"])
+        stored = np.arange(cap)[None, :] < np.minimum(g.ev_n, cap)[:, None]
+        np.testing.assert_array_equal(np.where(stored, g.ev_t, 0.0), np.where(stored, c["ev_t"], 0.0))
+        np.testing.assert_array_equal(np.where(stored[:, :, None], g.ev_u, 0.0),
+                                      np.where(stored[:, :, None], c["ev_u"], 0.0))
+    # solutions: saveat and event points in time order, saveat first at a tie
+    for i in (0, 7, 100):
+        sol = g.solution(i)
+        assert len(sol.t) == np.count_nonzero(~np.isnan(g.us[:, 0, i])) + min(int(g.ev_n[i]), 3)
+        assert (np.diff(sol.t) >= 0).all()

@@ -247,3 +247,42 @@ def test_uses_time_rule_and_local_t():
         assert lib.mideq_uses_time(text.encode()) == want, text
         assert umh.uses_time(text) == bool(want), text
     assert mideq.UserModel(1, 1, "T t = u[0]; du[0] = -t;").check() == ""
+
+
+@pytest.mark.parametrize("saves,per_event", [((True, True), 2), ((False, True), 1),
+                                             ((True, False), 1)])
+def test_oracle_event_points_bouncing_ball(oracle, saves, per_event):
+    """save_positions at the bounces of a ball: per bounce the state just
+    before the affect (height ~0, falling at v) and / or just after it
+    (the same height, rising at e v), at the closed-form bounce times; a
+    capacity smaller than the count keeps the first points and reports
+    them all."""
+    from mideq import ContinuousCallback
+    e, g = 0.9, 9.81
+    cb = ContinuousCallback("u[0]", affect="u[1] = -p[1] * u[1];", direction=-1,
+                            save_positions=saves)
+    umh.install(oracle, 2, 2, BALL, callbacks=(cb,))
+    h0 = np.array([10.0, 5.0, 2.0, 20.0])
+    u0 = np.vstack([h0, np.zeros(4)])
+    p = np.vstack([np.full(4, g), np.full(4, e)])
+    r = oracle.ensemble("user", u0, p, 0.0, 6.0, abstol=1e-12, reltol=1e-12, event_cap=64)
+    np.testing.assert_array_equal(r["ev_n"], per_event * r["nevent"])
+    for i in range(4):
+        times = _bounce_times(h0[i], g=g, e=e)
+        m = int(r["ev_n"][i])
+        et, eu = r["ev_t"][i, :m], r["ev_u"][i, :m]
+        assert np.isnan(r["ev_t"][i, m:]).all()
+        np.testing.assert_allclose(et[::per_event], times, rtol=1e-8)
+        assert np.abs(eu[:, 0]).max() < 1e-6  # on the floor
+        if saves == (True, True):
+            np.testing.assert_array_equal(et[0::2], et[1::2])
+            np.testing.assert_array_equal(eu[1::2, 1], -e * eu[0::2, 1])
+            assert (eu[0::2, 1] < 0).all() and (eu[1::2, 1] > 0).all()
+        elif saves == (False, True):
+            assert (eu[:, 1] > 0).all()
+        else:
+            assert (eu[:, 1] < 0).all()
+    small = oracle.ensemble("user", u0, p, 0.0, 6.0, abstol=1e-12, reltol=1e-12, event_cap=2)
+    np.testing.assert_array_equal(small["ev_n"], r["ev_n"])
+    np.testing.assert_array_equal(small["ev_t"], r["ev_t"][:, :2])
+    np.testing.assert_array_equal(small["ev_u"], r["ev_u"][:, :2])

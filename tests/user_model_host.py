@@ -131,4 +131,9 @@ def install(oracle, n, np_, rhs, jac=None, callbacks=()):
     rc = L.oracle_set_user_callbacks(k, arr(kind), arr(direction), arr(term), arr(ip),
                                      addr(lib.user_cb_g), addr(lib.user_cb_affect))
     assert rc == 0
+    saves = [(1 if cb.save_positions[0] else 0) | (2 if cb.save_positions[1] else 0)
+             for cb in callbacks]
+    L.oracle_set_user_callback_saves.argtypes = [C.c_int32, C.c_void_p]
+    rc = L.oracle_set_user_callback_saves(k, arr(saves))
+    assert rc == 0
     return lib
