@@ -120,3 +120,42 @@ def test_two_streams_brusselator_and_lane():
         torch.cuda.synchronize()
         check(ka, ra, nvec=wa["nvec"])
         check(kb, rb)
+
+
+@pytest.mark.parametrize("save_dtype", ["float64", "float32"])
+def test_host_entry_large_output_into_pageable_memory(save_dtype):
+    """Saved values larger than the staging threshold reach pageable caller
+    memory through the pinned staging workers of the host entry point: the
+    same bytes as a copy into a pinned destination (torch's pinned
+    allocation) of the same solve."""
+    import torch
+    N, ns = 131072, 64
+    say(f"host entry, {save_dtype} saveat output of {N}x{ns}x3 into pageable memory")
+    w = workloads.lorenz(N, seed=121)
+    sa = np.linspace(0.0, 2.0, ns)
+    ctx = _abi.context(0)
+
+    def solve(us):
+        P = _abi.Problem()
+        P.model, P.n, P.np, P.ntraj = _abi.MODEL["lorenz"], 3, 3, N
+        u0 = np.ascontiguousarray(w["u0"])
+        p = np.ascontiguousarray(w["p"])
+        P.u0, P.p, P.tspan0, P.tspan1 = u0.ctypes.data, p.ctypes.data, 0.0, 2.0
+        O = _abi.Options()
+        O.alg, O.precision, O.abstol, O.reltol = _abi.ALG["tsit5"], 0, 1e-8, 1e-8
+        O.qsteady_default = 1
+        O.saveat, O.ns = sa.ctypes.data, ns
+        O.save_dtype = 1 if save_dtype == "float32" else 0
+        out = _abi.Outputs()
+        out.us = us.ctypes.data if isinstance(us, np.ndarray) else us.data_ptr()
+        ctx.solve(P, O, out)
+
+    dt = np.float32 if save_dtype == "float32" else np.float64
+    pageable = np.full((ns, 3, N), np.nan, dtype=dt)
+    assert pageable.nbytes > (64 << 20)
+    solve(pageable)
+    pinned = torch.empty((ns, 3, N), dtype=torch.float32 if dt == np.float32 else torch.float64,
+                         pin_memory=True)
+    solve(pinned)
+    np.testing.assert_array_equal(pageable, pinned.numpy())
+    assert not np.isnan(pageable).any()
