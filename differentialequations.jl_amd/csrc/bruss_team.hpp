@@ -716,6 +716,20 @@ struct BTeam {
     for (int k = 0; k <= kmain; k++) {
       const double v0 = Wb[k * 8 + 2], v1 = Wb[(k + 1) * 8 + 1], v2 = Wb[(k + 2) * 8];
       const double v3 = Wd[k], v4 = Wd[NK + k], v5 = Wd[2 * NK + k], v6 = Wd[3 * NK + k];
+      // this lane's operands of the trailing update and of the multiplier
+      // column without an interchange, loaded with the candidates: their
+      // LDS latency overlaps the pivot choice and 1/d (nothing writes W
+      // before the update below; an interchange reloads them)
+      double lq[kE], wq[kE], aq[kE];
+#pragma unroll
+      for (int e = 0; e < kE; e++) {
+        if (tl + e * L < 48) {
+          lq[e] = Wb[l_off[e] + k * l_st[e]];
+          wq[e] = Wb[w_off[e] + k * w_st[e]];
+          aq[e] = Wb[a_off[e] + k * a_st[e]];
+        }
+      }
+      const double mq = tl < 6 ? Wb[m_off + k * m_st] : 0.0;
       // the first maximal |W(q, k)| in ascending row order
       double d = v0;
       int pk = k;
@@ -738,16 +752,11 @@ struct BTeam {
 #pragma unroll
       for (int e = 0; e < kE; e++) {
         if (tl + e * L < 48) {
-          const double l = Wb[l_off[e] + k * l_st[e]] * rr;
-          const double wk = Wb[w_off[e] + k * w_st[e]];
-          double* a = Wb + (a_off[e] + k * a_st[e]);
-          *a = fma(-l, wk, *a);
+          const double l = lq[e] * rr;
+          Wb[a_off[e] + k * a_st[e]] = fma(-l, wq[e], aq[e]);
         }
       }
-      if (tl < 6) {
-        double* a = Wb + (m_off + k * m_st);
-        *a = *a * rr;
-      }
+      if (tl < 6) Wb[m_off + k * m_st] = mq * rr;
       team_sync();
     }
     for (int k = kmain + 1; k < n; k++) {
