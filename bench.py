@@ -330,6 +330,7 @@ def other_brusselator(torch, steps, warmup, N=131072, only=None):
         werr = np.where(rows, np.abs(r.u_final - uref) / (1e-8 + 1e-6 * np.abs(uref)), 0.0)
         werr = werr.max(axis=0)
         res[name] = dict(kernel_ms=km, instances_per_s=N / (km * 1e-3), lanes=r.lanes,
+                         blocks=r.grid,
                          ok=float((r.retcode == 1).mean()),
                          steps_mean=float((r.naccept + r.nreject).mean()),
                          stiff_step_fraction=float(r.nstiff.sum() / max(1, r.naccept.sum())),

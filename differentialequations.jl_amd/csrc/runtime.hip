@@ -90,8 +90,9 @@ struct mideq_ctx {
   DevBuf bufs[BUF_COUNT];  // host entry point's copies of the caller's arrays
   std::vector<StreamSlot> slots;
   uint64_t slot_clock = 0;
-  // autotuned lanes per instance, keyed by (alg, precision, controller, nmax)
-  std::map<std::tuple<int, int, int, int>, int> tuned_lanes;
+  // autotuned team shape (lanes per instance) and resident blocks per CU (0:
+  // as many as fit), keyed by (alg, precision, controller, nmax)
+  std::map<std::tuple<int, int, int, int>, std::pair<int, int>> tuned_lanes;
   std::vector<UserModel> user_models;  // model id MIDEQ_USER_MODEL_BASE + index
   std::string arch;                    // device code target for hipRTC
   // host entry point: pinned staging buffers and streams of the copy
@@ -787,13 +788,27 @@ static int bruss_nk(int lanes, int nmax) {
   return (lanes == 32 && nmax <= BRUSS_NMAX / 2) ? BRUSS_NMAX / 2 : BRUSS_NMAX;
 }
 
+// Persistent blocks per CU the kernel's occupancy allows for this shape.
+static int bruss_blocks_per_cu(mideq_ctx* c, const BrussArgs& b, int lanes, int alg, int mix,
+                               int fast) {
+  int grid = 0, tpb = 0;
+  if (bruss_plan(lanes, bruss_nk(lanes, b.nmax), alg, mix, fast, c->ncu, INT64_MAX / 2, &grid,
+                 &tpb, b.us && b.ns > 0) != hipSuccess)
+    return 1;
+  return std::max(1, grid / c->ncu);
+}
+
+// bpc > 0: at most bpc resident blocks per CU (fewer teams in flight);
+// 0: as many as the occupancy allows.
 static int bruss_launch_with(mideq_ctx* c, GraphCache* gc, BrussArgs b, int lanes, int alg,
-                             int mix, int fast, bool use_graph, hipStream_t s, int* grid_used) {
+                             int mix, int fast, bool use_graph, hipStream_t s, int* grid_used,
+                             int bpc = 0) {
   int grid = 0, tpb = 0;
   const int nk = bruss_nk(lanes, b.nmax);
   const bool save = b.us && b.ns > 0;
   hipError_t e = bruss_plan(lanes, nk, alg, mix, fast, c->ncu, b.nqueue, &grid, &tpb, save);
   if (e != hipSuccess) return set_err(c, MIDEQ_E_HIP, "bruss_plan: %s", hipGetErrorString(e));
+  if (bpc > 0) grid = std::max(1, std::min(grid, bpc * c->ncu));
   *grid_used = grid;
   struct Key {
     BrussArgs b;
@@ -845,14 +860,18 @@ static int run_bruss(mideq_ctx* c, const mideq_problem* P, const mideq_options* 
     b.save_layout = O->save_layout;
   }
   int lanes = O->lanes;
+  int bpc = O->blocks_per_cu > 0 ? O->blocks_per_cu : 0;
   int grid = 0;
   if (lanes == 0) {
-    // autotuner: time each candidate on a strided sample of the batch and
-    // keep the fastest for this (alg, precision, nmax)
+    // autotuner: time each candidate team shape (64- or 32-lane teams) and
+    // residency (as many blocks per CU as fit, or one fewer: fewer teams
+    // contending for the SIMDs) on a strided sample of the batch and keep
+    // the fastest for this (alg, precision, nmax)
     auto key = std::make_tuple((int)O->alg, (int)O->precision, (int)O->controller, (int)P->n);
     auto it = c->tuned_lanes.find(key);
     if (it != c->tuned_lanes.end()) {
-      lanes = it->second;
+      lanes = it->second.first;
+      if (!bpc) bpc = it->second.second;
     } else if (!Sc.slot) {
       // fixed scratch (no allocation, no tuning run): 64-lane teams
       lanes = 64;
@@ -886,32 +905,46 @@ static int run_bruss(mideq_ctx* c, const mideq_problem* P, const mideq_options* 
       bs.order = (const int64_t*)d;
       bs.nqueue = S;
       float best = 1e30f;
-      int best_l = 64;
+      int best_l = 64, best_b = 0;
       for (int cand : {64, 32}) {
-        int g = 0;
-        if ((rc = bruss_launch_with(c, nullptr, bs, cand, O->alg, O->precision, O->controller,
-                                    false, s, &g)))
-          return rc;
-        HIP_TRY(c, hipEventRecord(c->tune0, s));
-        if ((rc = bruss_launch_with(c, nullptr, bs, cand, O->alg, O->precision, O->controller,
-                                    false, s, &g)))
-          return rc;
-        HIP_TRY(c, hipEventRecord(c->tune1, s));
-        HIP_TRY(c, hipEventSynchronize(c->tune1));
-        float ms = 0.f;
-        HIP_TRY(c, hipEventElapsedTime(&ms, c->tune0, c->tune1));
-        if (ms < best) {
-          best = ms;
-          best_l = cand;
+        const int full = bruss_blocks_per_cu(c, bs, cand, O->alg, O->precision, O->controller);
+        // residencies: all that fit (0), one block per CU fewer; the
+        // caller's blocks_per_cu, when given, is the only one.  Compared
+        // only when the sample holds >= 4 instances per resident team at
+        // full residency: with fewer, the full one idles teams in the
+        // sample's tail and loses there what it gains on the batch.
+        const int64_t teams = (int64_t)full * c->ncu * (256 / cand);
+        const int cands[2] = {0, full - 1};
+        const int ncand = (bpc > 0 || full < 2 || S < 4 * teams) ? 1 : 2;
+        for (int ci = 0; ci < ncand; ci++) {
+          const int use_b = bpc > 0 ? bpc : cands[ci];
+          int g = 0;
+          if ((rc = bruss_launch_with(c, nullptr, bs, cand, O->alg, O->precision, O->controller,
+                                      false, s, &g, use_b)))
+            return rc;
+          HIP_TRY(c, hipEventRecord(c->tune0, s));
+          if ((rc = bruss_launch_with(c, nullptr, bs, cand, O->alg, O->precision, O->controller,
+                                      false, s, &g, use_b)))
+            return rc;
+          HIP_TRY(c, hipEventRecord(c->tune1, s));
+          HIP_TRY(c, hipEventSynchronize(c->tune1));
+          float ms = 0.f;
+          HIP_TRY(c, hipEventElapsedTime(&ms, c->tune0, c->tune1));
+          if (ms < best) {
+            best = ms;
+            best_l = cand;
+            best_b = use_b;
+          }
         }
       }
       lanes = best_l;
-      c->tuned_lanes[key] = lanes;
+      if (!bpc) bpc = best_b;
+      c->tuned_lanes[key] = {lanes, best_b};
     }
   }
   HIP_TRY(c, hipEventRecord(c->ev0, s));
   int rc = bruss_launch_with(c, Sc.graph, b, lanes, O->alg, O->precision, O->controller,
-                             O->graph != 0, s, &grid);
+                             O->graph != 0, s, &grid, bpc);
   if (rc) return rc;
   HIP_TRY(c, hipEventRecord(c->ev1, s));
   c->ev_pending = true;

@@ -454,6 +454,7 @@ class EnsembleArrays:
     total_ms: float = 0.0
     lanes: int = 0
     graph: bool = False
+    grid: int = 0                 # blocks launched (persistent kernels: resident teams' blocks)
     nevent: Optional[np.ndarray] = None   # [N] callback affects applied
     t_event: Optional[np.ndarray] = None  # [N] time of the last one (NaN if none)
     out: Optional[np.ndarray] = None      # [nout, N] on-device output_func values
@@ -761,7 +762,8 @@ def device_solve(f, u0, p, t0, tf, nvec, alg, ealg, saveat=None, abstol=1e-6, re
     ctx.solve(P, O, out)
     st = ctx.stats()
     return EnsembleArrays(saveat=saveat, us=us, t0=t0, u0=u0, kernel_ms=st.kernel_ms,
-                          total_ms=st.total_ms, lanes=st.lanes, graph=bool(st.graph),
+                          total_ms=st.total_ms, lanes=st.lanes, grid=st.grid,
+                          graph=bool(st.graph),
                           save_layout=ealg.save_layout, nvec=nvec, **res)
 
 
