@@ -186,7 +186,8 @@ typedef struct {
   double *reduced;      /* [nout] the reduction of `out` over the trajectories */
   /* Points saved at callback events (mideq_callback.save_positions), in the
    * order of integration; user models with callbacks, one-lane kernels.
-   * ev_t / ev_u may be NULL together (no event points). */
+   * ev_t / ev_u may be NULL together (no event points); slots past a
+   * trajectory's points are NaN. */
   double *ev_t;         /* [ntraj][ev_cap] times */
   double *ev_u;         /* [ntraj][ev_cap][n] states */
   int32_t *ev_n;        /* [ntraj] points produced; only the first ev_cap are stored */

@@ -1569,6 +1569,13 @@ static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
       for (int c = 0; c < n; c++) save[((long)isave * save_rows + c) * save_stride] = NAN;
 #undef SAVE_LEFT
 #undef SAVE_NEXT
+  /* event-point slots past this trajectory's points hold NaN */
+  if (o->ev_t)
+    for (long long k = res->nevsave; k < o->ev_cap; k++) {
+      const long long at = (long long)idx * o->ev_cap + k;
+      o->ev_t[at] = NAN;
+      for (int c = 0; c < n; c++) o->ev_u[at * n + c] = NAN;
+    }
   for (int c = 0; c < n; c++) res->u[c] = uprev[c];
   res->t = t;
   res->retcode = rc;

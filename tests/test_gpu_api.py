@@ -219,3 +219,27 @@ def test_ensemble_dense_output_matches_saveat(model, alg):
     for i in (0, 5, N - 1):
         vals = sim.u[i](ts)
         np.testing.assert_array_equal(vals[order], np.array(ref.u[i].u))
+
+
+def test_dense_output_backward_span_and_terminate():
+    """sol(t) of a solve integrated backward, and of one ended by a
+    terminating callback: the values equal saveat solves at those times
+    (bit for bit), NaN past the end of the solve."""
+    from mideq import ContinuousCallback
+    say("dense output, backward span and terminate")
+    prob = ODEProblem(mideq.exp_growth, 2.0, (1.0, 0.0), [-1.5])
+    sol = solve(prob, Tsit5(), abstol=1e-9, reltol=1e-9)
+    ts = np.array([0.9, 0.1, 0.5, 1.0])
+    ref = solve(prob, Tsit5(), saveat=np.sort(ts), abstol=1e-9, reltol=1e-9)
+    # the backward solution lists its saved points in the order of integration
+    np.testing.assert_array_equal(sol(np.sort(ts))[::-1, 0], np.array(ref.u)[:, 0])
+    np.testing.assert_allclose(sol(0.5)[0], 2.0 * math.exp(-1.5 * (0.5 - 1.0)), rtol=1e-7)
+    cb = ContinuousCallback("u[0] - 0.5", terminate=True)
+    p2 = ODEProblem(mideq.exp_growth, 1.0, (0.0, 5.0), [-1.0])
+    s2 = solve(p2, Tsit5(), callback=cb, abstol=1e-10, reltol=1e-10)
+    assert s2.retcode == ReturnCode.Terminated
+    tend = s2.t[-1]
+    assert abs(tend - math.log(2.0)) < 1e-8
+    v = s2(np.array([0.3, tend * 0.99, 1.0, 3.0]))
+    np.testing.assert_allclose(v[:2, 0], np.exp(-np.array([0.3, tend * 0.99])), rtol=1e-8)
+    assert np.isnan(v[2:]).all()

@@ -185,3 +185,32 @@ def test_event_points_bitexact(oracle, alg, precision, saves):
         sol = g.solution(i)
         assert len(sol.t) == np.count_nonzero(~np.isnan(g.us[:, 0, i])) + min(int(g.ev_n[i]), 3)
         assert (np.diff(sol.t) >= 0).all()
+
+
+def test_event_points_discrete_and_terminate(oracle):
+    """A CallbackSet of a discrete callback that kicks the state at step ends
+    inside a window and a terminating continuous callback: the points saved
+    before / after each affect (terminate's included) are bit-exact against
+    the oracle, and the last one is the final state."""
+    N = 512
+    say(f"event points, discrete + terminate N={N}")
+    cbs = CallbackSet(ContinuousCallback("u[0] - 0.25", terminate=True),
+                      DiscreteCallback("t > 0.2 && t < 0.4", affect="u[0] = u[0] * 0.999;"))
+    rhs = "du[0] = -p[0] * u[0];"
+    model = UserModel(1, 1, rhs, callbacks=cbs)
+    rng = np.random.default_rng(115)
+    k = rng.uniform(0.5, 3.0, N)
+    kw = dict(abstol=1e-10, reltol=1e-10)
+    g = run(model, np.ones((1, N)), k[None, :], 0.0, 4.0, event_cap=64, **kw)
+    umh.install(oracle, 1, 1, rhs, callbacks=cbs)
+    c = cpu(oracle, "user", np.ones((1, N)), k[None, :], 0.0, 4.0, event_cap=64, **kw)
+    _check(g, c)
+    assert (g.retcode == 7).all() and (g.ev_n == 2 * g.nevent).all()
+    np.testing.assert_array_equal(g.ev_n, c["ev_n"])
+    np.testing.assert_array_equal(g.ev_t, c["ev_t"])
+    np.testing.assert_array_equal(g.ev_u, c["ev_u"])
+    for i in (0, 100, N - 1):
+        m = int(g.ev_n[i])
+        assert g.ev_t[i, m - 1] == g.t_final[i] and g.ev_u[i, m - 1, 0] == g.u_final[0, i]
+        sol = g.solution(i)
+        assert len(sol.t) == m + 2 and sol.t[0] == 0.0 and sol.t[-1] == g.t_final[i]
