@@ -1249,14 +1249,16 @@ struct Lane {
     if (a.dt_first) a.dt_first[i] = dtfirst;
     if (a.nevent) a.nevent[i] = nevent;
     if (a.t_event) a.t_event[i] = tevent;
-    if (a.ev_n) a.ev_n[i] = nevsave;
-    if (a.ev_t) {
-      // slots past this trajectory's event points hold NaN
-      for (int k = nevsave; k < a.ev_cap; k++) {
-        const int64_t at = (int64_t)i * a.ev_cap + k;
-        a.ev_t[at] = __builtin_nan("");
+    if constexpr (kNcb > 0) {  // event points: models with callbacks only
+      if (a.ev_n) a.ev_n[i] = nevsave;
+      if (a.ev_t) {
+        // slots past this trajectory's event points hold NaN
+        for (int k = nevsave; k < a.ev_cap; k++) {
+          const int64_t at = (int64_t)i * a.ev_cap + k;
+          a.ev_t[at] = __builtin_nan("");
 #pragma unroll
-        for (int c = 0; c < N; c++) a.ev_u[at * N + c] = __builtin_nan("");
+          for (int c = 0; c < N; c++) a.ev_u[at * N + c] = __builtin_nan("");
+        }
       }
     }
     if constexpr (kNout > 0) {
