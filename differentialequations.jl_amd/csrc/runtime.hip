@@ -450,13 +450,28 @@ static std::string user_model_source(const mideq_user_model* m) {
     s += "  {\n#line 1 \"jac\"\n";
     s += m->jac;
     s += "\n  }\n";
-  } else {
+  } else if (m->n <= 8) {
     s += "    Dual<N> ud[N], pd[NP], fd[N];\n"
          "    for (int i = 0; i < N; i++) { ud[i].v = u[i]; ud[i].d[i] = 1.0; }\n"
          "    for (int j = 0; j < NP; j++) pd[j].v = p[j];\n";
     s += tdep ? "    f(fd, ud, pd, Dual<N>(t));\n" : "    f(fd, ud, pd);\n";
     s += "    for (int i = 0; i < N; i++)\n"
          "      for (int j = 0; j < N; j++) J[i][j] = fd[i].d[j];\n";
+  } else {
+    // n > 8: one direction per pass, column by column, in a loop that is
+    // not unrolled.  Every partial follows the same operations as the n
+    // directions of one Dual<N> pass (the rules act on each partial
+    // alone), so J is the same bit for bit; the kernel holds 2n + 2 live
+    // dual values instead of n (n + 1), and its Rodas5 kernel compiles in
+    // ~30 s instead of ~150 s at n = 16.
+    s += "#pragma unroll 1\n"
+         "    for (int c = 0; c < N; c++) {\n"
+         "      Dual<1> ud[N], pd[NP], fd[N];\n"
+         "      for (int i = 0; i < N; i++) { ud[i].v = u[i]; ud[i].d[0] = i == c ? 1.0 : 0.0; }\n"
+         "      for (int j = 0; j < NP; j++) pd[j].v = p[j];\n";
+    s += tdep ? "      f(fd, ud, pd, Dual<1>(t));\n" : "      f(fd, ud, pd);\n";
+    s += "      for (int i = 0; i < N; i++) J[i][c] = fd[i].d[0];\n"
+         "    }\n";
   }
   s += "  }\n";
   if (tdep)  // df/dt with the time as the one dual direction

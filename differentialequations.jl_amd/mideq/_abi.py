@@ -48,7 +48,7 @@ class UserModelSpec(C.Structure):
 
 
 USER_MODEL_BASE = 1000
-USER_NMAX, USER_NPMAX, USER_NCBMAX, USER_NOUTMAX = 8, 16, 8, 16
+USER_NMAX, USER_NPMAX, USER_NCBMAX, USER_NOUTMAX = 16, 16, 8, 16
 REDUCTION = dict(none=0, sum=1, mean=2, min=3, max=4)
 
 

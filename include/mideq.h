@@ -252,7 +252,7 @@ int mideq_ensemble_solve_device_noalloc(mideq_ctx *ctx, const mideq_problem *pro
  * test/qa/qa.jl:86) and ODEFunction(f!; jac) (qa.jl:84) for the lane kernels:
  * the shim passes f's body as C++ statements instead of a registered id. */
 #define MIDEQ_USER_MODEL_BASE 1000 /* first id of a registered user model */
-#define MIDEQ_USER_NMAX 8          /* states (one lane per trajectory)     */
+#define MIDEQ_USER_NMAX 16         /* states (one lane per trajectory)     */
 #define MIDEQ_USER_NPMAX 16        /* parameters                           */
 /* On-device reduction of the output_func values over the trajectories
  * (replaces reduction(u, batch, I) of EnsembleProblem, test/qa/qa.jl:50, for

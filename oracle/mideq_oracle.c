@@ -615,14 +615,15 @@ static void mdl_init(mdl_t *m, int model, int n, const double *p, int np) {
 /* Norms                                                                     */
 /* ------------------------------------------------------------------------ */
 /* ODE_DEFAULT_NORM for arrays: sqrt(sum(abs2, x) / length(x)); summed left  */
-/* to right for n <= 8; for larger n (the interleaved (u, v) Brusselator) as */
+/* to right for n <= 8 or L = 1 (a model solved in one GPU lane per          */
+/* trajectory); for larger n (the interleaved (u, v) Brusselator) as         */
 /* L partial sums combined by a pairwise tree (part[l] += part[l + w] for    */
 /* w = L/2, ..., 1): the order in which a team of L GPU lanes reduces.       */
 /* Partial l adds, in order, the components its lane owns: the (u, v) pairs  */
 /* of grid points l, l + L, ... , i.e. x_{2l}, x_{2l+1}, x_{2(l+L)}, ...     */
 static double sum_sq(const double *x, int n, int L) {
   double s;
-  if (n <= 8) {
+  if (n <= 8 || L == 1) {
     s = x[0] * x[0];
     for (int i = 1; i < n; i++) s = s + x[i] * x[i];
   } else {
@@ -645,6 +646,13 @@ static double sum_sq(const double *x, int n, int L) {
 }
 
 static double rms_norm(const double *x, int n, int L) { return sqrt(sum_sq(x, n, L) / (double)n); }
+
+/* Lanes that reduce a norm on the GPU: user models run one trajectory per    */
+/* lane (a sequential sum for every n); the Brusselator's team reduces over   */
+/* the options' norm_lanes.                                                   */
+static int norm_lanes(const mdl_t *m, const oracle_options *o) {
+  return m->model == ORACLE_MODEL_USER ? 1 : o->norm_lanes;
+}
 
 /* Set per solve: 1 selects the fast controller, which works on EEst^2: from
  * one division over a common denominator for n <= 3, else mean(r^2) without
@@ -712,9 +720,9 @@ static double initdt_impl(const mdl_t *m, const double *u0, const double *f0,
   double sk[NMAX], tmp[NMAX];
   for (int i = 0; i < n; i++) sk[i] = fma(fabs(u0[i]), o->reltol, o->abstol);
   for (int i = 0; i < n; i++) tmp[i] = u0[i] / sk[i];
-  double d0 = rms_norm(tmp, n, o->norm_lanes);
+  double d0 = rms_norm(tmp, n, norm_lanes(m, o));
   for (int i = 0; i < n; i++) tmp[i] = f0[i] / sk[i];
-  double d1 = rms_norm(tmp, n, o->norm_lanes);
+  double d1 = rms_norm(tmp, n, norm_lanes(m, o));
   if (d1 != d1) return dtmin;
   double dt0 = (d0 < 1e-5 || d1 < 1e-5) ? smalldt : (d0 / d1) / 100.0;
   dt0 = fmin(dt0, dtmax);
@@ -728,7 +736,7 @@ static double initdt_impl(const mdl_t *m, const double *u0, const double *f0,
   for (int i = 0; i < n; i++) same &= (f0[i] == f1[i]);
   if (same) return fmax(dtmin, 100.0 * dt0);
   for (int i = 0; i < n; i++) tmp[i] = (f1[i] - f0[i]) / sk[i];
-  double d2 = rms_norm(tmp, n, o->norm_lanes) / dt0;
+  double d2 = rms_norm(tmp, n, norm_lanes(m, o)) / dt0;
   double mx = fmax(d1, d2);
   double dt1;
   if (mx <= 1e-15) {
@@ -1330,15 +1338,15 @@ static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
     if (mode == 0) {
       if (!mixed)
         EEst = tsit5_step64(m, uprev, t, sdt, abstol, reltol, k64, u1, &eig, &eigd,
-                            o->norm_lanes);
+                            norm_lanes(m, o));
       else
         EEst = tsit5_step_mix(m, uprev, t, sdt, abstol, reltol, k32, u1, &eig, &eigd,
-                              o->norm_lanes);
+                              norm_lanes(m, o));
       res->nf += 6;
     } else {
       double nJ;
       if (rodas5_step(m, rtab, mixed, uprev, t, sdt, abstol, reltol, u1, &EEst, ws, &nJ,
-                      o->norm_lanes, kdsave) != 0)
+                      norm_lanes(m, o), kdsave) != 0)
         EEst = 2.0;
       res->nf += 8;
       res->njac += 1;

@@ -203,9 +203,10 @@ CHAIN8 = ("for (int i = 0; i < 8; i++) {\n"
 
 @pytest.mark.parametrize("alg,precision", [("tsit5", "fp64"), ("rodas5", "fp64"),
                                            ("auto", "mixed")])
-def test_user_eight_states_at_the_limit(oracle, alg, precision):
-    """n = 8, the largest one-lane model (8 x 8 register LU, dual numbers with
-    eight directions), with saveat: bit-exact against the oracle."""
+def test_user_eight_states(oracle, alg, precision):
+    """n = 8, the largest model whose dual Jacobian takes all directions in
+    one pass (8 x 8 register LU, dual numbers with eight directions), with
+    saveat: bit-exact against the oracle."""
     N = 700
     say(f"user 8-state chain {alg} {precision} N={N}")
     umh.install(oracle, 8, 3, CHAIN8)
@@ -217,5 +218,32 @@ def test_user_eight_states_at_the_limit(oracle, alg, precision):
     kw = dict(saveat=np.linspace(0.0, 3.0, 7), abstol=1e-8, reltol=1e-6)
     g = run(model, u0, p, 0.0, 3.0, alg=alg, precision=precision, **kw)
     c = cpu(oracle, "user", u0, p, 0.0, 3.0, alg=alg, precision=precision, **kw)
+    assert (c["retcode"] == 1).all()
+    assert_identical(g, c)
+
+
+L96 = ("for (int i = 0; i < 16; i++)\n"
+       "  du[i] = (u[(i + 1) % 16] - u[(i + 14) % 16]) * u[(i + 15) % 16]"
+       " - p[1 + i % 2] * u[i] + p[0];")
+
+
+@pytest.mark.parametrize("alg,precision", [("tsit5", "fp64"), ("rodas5", "fp64"),
+                                           ("auto", "mixed")])
+def test_user_sixteen_states_at_the_limit(oracle, alg, precision):
+    """n = 16, the largest one-lane model: a Lorenz-96 ring whose odd sites
+    are damped up to 2000 (stiff), with saveat.  Its Jacobian comes from
+    dual numbers one direction at a time (n > 8); the oracle's host build
+    differentiates with all 16 directions at once.  Bit-exact."""
+    N = 500
+    say(f"user 16-state Lorenz-96 {alg} {precision} N={N}")
+    umh.install(oracle, 16, 3, L96)
+    model = UserModel(16, 3, L96, name="lorenz96_16")
+    rng = np.random.default_rng(116)
+    u0 = rng.uniform(-1.0, 1.0, (16, N))
+    p = np.vstack([rng.uniform(2.0, 8.0, N), rng.uniform(0.5, 1.5, N),
+                   10.0 ** rng.uniform(0, 3.3, N)])
+    kw = dict(saveat=np.linspace(0.0, 2.0, 5), abstol=1e-8, reltol=1e-6)
+    g = run(model, u0, p, 0.0, 2.0, alg=alg, precision=precision, **kw)
+    c = cpu(oracle, "user", u0, p, 0.0, 2.0, alg=alg, precision=precision, **kw)
     assert (c["retcode"] == 1).all()
     assert_identical(g, c)

@@ -36,7 +36,7 @@ def test_device_compile_check_and_errors():
     with pytest.raises(mideq.MideqError, match="jac"):
         mideq.UserModel(1, 1, "du[0] = -u[0];", jac="J[0][0] = -1.0 +;").check()
     with pytest.raises(ValueError):
-        mideq.UserModel(9, 1, "du[0] = 0.0;")
+        mideq.UserModel(17, 1, "du[0] = 0.0;")
 
 
 def _jac(lib, n, u, p):
