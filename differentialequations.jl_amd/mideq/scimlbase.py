@@ -437,7 +437,10 @@ class ODESolution:
 
 @dataclass
 class EnsembleArrays:
-    """Structure-of-arrays result of one device solve."""
+    """Structure-of-arrays result of one device solve: numpy arrays, or for
+    a device-resident solve (resident = True) torch tensors on the GPU whose
+    per-trajectory views saved(i), events(i) and solution(i) copy no state
+    (the zero-copy EnsembleSolution view of qa.jl:52)."""
     u_final: np.ndarray           # [n, N]
     t_final: np.ndarray           # [N]
     retcode: np.ndarray           # [N] int8
@@ -464,6 +467,8 @@ class EnsembleArrays:
     ev_t: Optional[np.ndarray] = None     # [N, cap] times of the points saved at events
     ev_u: Optional[np.ndarray] = None     # [N, cap, n] their states
     ev_n: Optional[np.ndarray] = None     # [N] points produced (may exceed cap)
+    resident: bool = False                # the arrays are torch tensors on the GPU
+    keep: Optional[list] = field(default=None, repr=False, compare=False)  # buffers in use
 
     def saved(self, i):
         """Saved states of trajectory i (0-based), [ns, n_i]."""
@@ -472,10 +477,14 @@ class EnsembleArrays:
 
     def events(self, i):
         """(times, states) of the points trajectory i saved at callback events,
-        in the order of integration (at most the arrays' capacity)."""
+        in the order of integration (at most the arrays' capacity).  For a
+        device-resident result the times are read back and the states are a
+        view on the GPU."""
         if self.ev_t is None:
             return np.empty(0), np.empty((0, self.u0.shape[0]))
         m = min(int(self.ev_n[i]), self.ev_t.shape[1])
+        if self.resident:
+            return self.ev_t[i, :m].cpu().numpy(), self.ev_u[i, :m]
         return self.ev_t[i, :m].copy(), self.ev_u[i, :m].copy()
 
     def solution(self, i, prob=None):
@@ -486,24 +495,29 @@ class EnsembleArrays:
         state."""
         st = Stats(int(self.naccept[i]), int(self.nreject[i]), int(self.nf[i]))
         rc = ReturnCode(int(self.retcode[i]))
-        back = self.t_final is not None and self.t_final[i] < self.t0[i]
+        back = self.t_final is not None and float(self.t_final[i]) < float(self.t0[i])
         et, eu = self.events(i)
+        # host results hand out copies, device-resident ones views
+        own = (lambda v: v) if self.resident else np.array
         if self.us is not None:
             vals = self.saved(i)
-            ok = ~np.isnan(vals).any(axis=1)
-            ts, us = self.saveat[ok], vals[ok]
+            nan = vals.isnan().any(dim=1).cpu().numpy() if self.resident else \
+                np.isnan(vals).any(axis=1)
+            rows = np.nonzero(~nan)[0]
+            ts, us = self.saveat[rows], [vals[k] for k in rows]
             if back:
                 ts, us = ts[::-1], us[::-1]  # a backward solve: in the order of integration
             if len(et):
                 allt = np.concatenate([ts, et])
-                allu = list(us) + list(eu)
+                allu = list(us) + [eu[k] for k in range(len(et))]
                 order = np.argsort(-allt if back else allt, kind="stable")
                 ts, us = allt[order], [allu[k] for k in order]
-            return ODESolution(np.array(ts, dtype=np.float64), [np.array(v) for v in us], rc,
-                               st, prob)
+            return ODESolution(np.array(ts, dtype=np.float64), [own(v) for v in us], rc, st,
+                               prob)
         m = int(self.nvec[i]) if self.nvec is not None else self.u0.shape[0]
-        ts = np.concatenate([[self.t0[i]], et, [self.t_final[i]]])
-        us = [self.u0[:m, i].copy()] + [v[:m] for v in eu] + [self.u_final[:m, i].copy()]
+        ts = np.concatenate([[float(self.t0[i])], et, [float(self.t_final[i])]])
+        us = ([own(self.u0[:m, i])] + [eu[k, :m] for k in range(len(et))] +
+              [own(self.u_final[:m, i])])
         return ODESolution(ts, us, rc, st, prob)
 
 
@@ -647,25 +661,106 @@ def _saveat_vector(saveat, tspan):
 
 
 def _ptr(a):
-    return None if a is None else a.ctypes.data
+    if a is None:
+        return None
+    return a.data_ptr() if hasattr(a, "data_ptr") else a.ctypes.data
+
+
+class _HostMem:
+    """Buffers of a host-entry solve: numpy arrays."""
+    resident = False
+
+    def arr(self, a, dtype):
+        if hasattr(a, "detach"):  # a torch tensor in host memory
+            a = a.detach().numpy()
+        return np.ascontiguousarray(a, dtype=dtype)
+
+    def full(self, shape, value, dtype):
+        return np.full(shape, value, dtype=dtype)
+
+    def empty(self, shape, dtype):
+        return np.empty(shape, dtype=dtype)
+
+
+class _DeviceMem:
+    """Buffers of a device-resident solve: torch tensors on one GPU.  Inputs
+    already there (contiguous, of the stated dtype) are used in place, others
+    are copied there once; outputs are allocated there and stay there.
+    Every buffer is kept (and recorded on the solve's stream) until the
+    result that holds it is dropped, so the asynchronous solve never sees
+    memory handed to another allocation."""
+    resident = True
+
+    def __init__(self, torch, device):
+        self.torch, self.device, self.keep = torch, device, []
+        self.dtypes = {np.dtype(np.float64): torch.float64, np.dtype(np.float32): torch.float32,
+                       np.dtype(np.int64): torch.int64, np.dtype(np.int32): torch.int32,
+                       np.dtype(np.int8): torch.int8}
+
+    def _k(self, t):
+        self.keep.append(t)
+        return t
+
+    def arr(self, a, dtype):
+        T = self.torch
+        if not T.is_tensor(a):
+            a = T.from_numpy(np.ascontiguousarray(a, dtype=dtype))
+        return self._k(a.to(device=self.device, dtype=self.dtypes[np.dtype(dtype)]).contiguous())
+
+    def full(self, shape, value, dtype):
+        return self._k(self.torch.full(tuple(np.atleast_1d(shape).tolist()), value,
+                                       dtype=self.dtypes[np.dtype(dtype)], device=self.device))
+
+    def empty(self, shape, dtype):
+        return self._k(self.torch.empty(tuple(np.atleast_1d(shape).tolist()),
+                                        dtype=self.dtypes[np.dtype(dtype)], device=self.device))
+
+
+def _resident_device(u0):
+    """(torch, device) when u0 is a torch tensor in GPU memory, else None."""
+    if not (hasattr(u0, "is_cuda") and u0.is_cuda):
+        return None
+    import torch
+    return torch, u0.device
 
 
 def device_solve(f, u0, p, t0, tf, nvec, alg, ealg, saveat=None, abstol=1e-6, reltol=1e-3,
                  dt=0.0, dtmin=0.0, dtmax=0.0, maxiters=0, dt0=None, alg_hint=None,
                  order=None, block=0, blocks_per_cu=0, callback=None, output=None,
-                 reduction=None, keep_states=True, event_cap=16):
+                 reduction=None, keep_states=True, event_cap=16, stream=None):
     """One call of mideq_ensemble_solve on host arrays -> EnsembleArrays.
     callback: ContinuousCallback / DiscreteCallback / CallbackSet, compiled
     into the right-hand side (a UserModel, or a built-in model's source).
     output: DeviceOutputFunc (values per trajectory in .out); reduction:
     DeviceReduction or its op name (folded values in .reduced, and with
-    keep_states=False nothing per trajectory comes back)."""
+    keep_states=False nothing per trajectory comes back).
+
+    Device-resident form: when u0 is a torch tensor in GPU memory the call
+    goes to mideq_ensemble_solve_device instead.  The other inputs are used
+    in place when they are already there (else copied there once), every
+    output is a torch tensor on that GPU, nothing comes back to the host, and
+    the call returns once the solve is enqueued on `stream` (a
+    torch.cuda.Stream; default the current stream) -- synchronize before
+    reading.  The result's saved(i) and solution(i) are views into those
+    tensors (no copies)."""
     if callback or output is not None:
         f = user_model_of(f).with_callbacks(callback or (), output)
     if isinstance(reduction, DeviceReduction):
         reduction = reduction.op
+    dev = _resident_device(u0)
+    mem = _DeviceMem(*dev) if dev else _HostMem()
+    if dev is not None:
+        torch, device = dev
+        if ealg.device != device.index:
+            raise ValueError(f"u0 is on {device}, EnsembleMI355X(device={ealg.device})")
+        if stream is None:
+            stream = torch.cuda.current_stream(device)
+    elif stream is not None:
+        raise ValueError("stream is for device-resident solves (u0 a GPU tensor)")
     ctx = _abi.context(ealg.device)
     n, N = u0.shape
+    u0 = mem.arr(u0, np.float64)
+    p = mem.arr(p, np.float64)
     P = _abi.Problem()
     P.model = f.model_id_in(ctx) if isinstance(f, UserModel) else f.model_id
     P.n = n
@@ -673,14 +768,15 @@ def device_solve(f, u0, p, t0, tf, nvec, alg, ealg, saveat=None, abstol=1e-6, re
     P.ntraj = N
     P.u0 = _ptr(u0)
     P.p = _ptr(p)
-    t0 = np.ascontiguousarray(t0, dtype=np.float64)
-    tf = np.ascontiguousarray(tf, dtype=np.float64)
+    if not mem.resident:
+        P.tspan0 = float(np.asarray(t0).ravel()[0]) if N else 0.0
+        P.tspan1 = float(np.asarray(tf).ravel()[0]) if N else 0.0
+    t0 = mem.arr(t0, np.float64)
+    tf = mem.arr(tf, np.float64)
     P.t0 = _ptr(t0)
     P.tf = _ptr(tf)
-    P.tspan0 = float(t0[0]) if N else 0.0
-    P.tspan1 = float(tf[0]) if N else 0.0
     if nvec is not None:
-        nvec = np.ascontiguousarray(nvec, dtype=np.int32)
+        nvec = mem.arr(nvec, np.int32)
         P.nvec = _ptr(nvec)
     O = _abi.Options()
     O.alg = alg.code
@@ -701,7 +797,7 @@ def device_solve(f, u0, p, t0, tf, nvec, alg, ealg, saveat=None, abstol=1e-6, re
     O.auto_passes = int(ealg.auto_passes)
     iter_cap = None
     if ealg.iter_cap is not None:
-        iter_cap = np.ascontiguousarray(ealg.iter_cap, dtype=np.int32)
+        iter_cap = mem.arr(ealg.iter_cap, np.int32)
         if iter_cap.shape != (N,):
             raise ValueError(f"iter_cap has shape {iter_cap.shape}, expected ({N},)")
         O.iter_cap = _ptr(iter_cap)
@@ -709,62 +805,76 @@ def device_solve(f, u0, p, t0, tf, nvec, alg, ealg, saveat=None, abstol=1e-6, re
     O.save_dtype = 1 if f32 else 0
     us = None
     if saveat is not None and len(saveat):
-        O.saveat = _ptr(saveat)
+        # the times as given (host) for the solution; a copy where the solve reads
+        saveat_in = saveat
+        saveat = np.ascontiguousarray(saveat.cpu().numpy() if hasattr(saveat, "cpu")
+                                      else saveat, dtype=np.float64)
+        O.saveat = _ptr(mem.arr(saveat_in, np.float64) if mem.resident else saveat)
         O.ns = len(saveat)
         O.save_layout = dict(soa=0, trajectory=1)[ealg.save_layout]
         shape = (N, len(saveat), n) if ealg.save_layout == "trajectory" else (len(saveat), n, N)
         # rows past a ragged instance's size are left as they are: NaN
-        us = np.full(shape, np.nan, dtype=np.float32 if f32 else np.float64)
+        us = mem.full(shape, np.nan, np.float32 if f32 else np.float64)
     dt0 = dt0 if dt0 is not None else ealg.dt0
     hint = alg_hint if alg_hint is not None else ealg.alg_hint
     if dt0 is not None:
-        dt0 = np.ascontiguousarray(dt0, dtype=np.float64)
+        dt0 = mem.arr(dt0, np.float64)
         O.dt0 = _ptr(dt0)
     if hint is not None:
-        hint = np.ascontiguousarray(hint, dtype=np.int8)
+        hint = mem.arr(hint, np.int8)
         O.alg_hint = _ptr(hint)
     order = order if order is not None else ealg.order
     if order is not None:
-        order = np.ascontiguousarray(order, dtype=np.int64)
+        order = mem.arr(order, np.int64)
         if order.shape != (N,):
             raise ValueError(f"order has shape {order.shape}, expected ({N},)")
         O.order = _ptr(order)
     if reduction is not None:
         O.reduction = _abi.REDUCTION[reduction]
     if keep_states:
-        res = dict(u_final=np.zeros((n, N)), t_final=np.empty(N), retcode=np.empty(N, np.int8),
-                   naccept=np.empty(N, np.int32), nreject=np.empty(N, np.int32),
-                   nf=np.empty(N, np.int32), nstiff=np.empty(N, np.int32),
-                   dt_first=np.empty(N))
+        res = dict(u_final=mem.full((n, N), 0.0, np.float64), t_final=mem.empty(N, np.float64),
+                   retcode=mem.empty(N, np.int8), naccept=mem.empty(N, np.int32),
+                   nreject=mem.empty(N, np.int32), nf=mem.empty(N, np.int32),
+                   nstiff=mem.empty(N, np.int32), dt_first=mem.empty(N, np.float64))
     else:
         res = dict(u_final=None, t_final=None, retcode=None, naccept=None, nreject=None,
                    nf=None, nstiff=None, dt_first=None)
     if isinstance(f, UserModel) and f.callbacks and keep_states:
-        res["nevent"] = np.zeros(N, np.int32)
-        res["t_event"] = np.empty(N)
+        res["nevent"] = mem.full(N, 0, np.int32)
+        res["t_event"] = mem.empty(N, np.float64)
         if any(_save_bits(getattr(cb, "save_positions", (False, False)))
                for cb in f.callbacks):
             # points saved at events (save_positions): event_cap per trajectory
-            res["ev_t"] = np.full((N, int(event_cap)), np.nan)
-            res["ev_u"] = np.full((N, int(event_cap), n), np.nan)
-            res["ev_n"] = np.zeros(N, np.int32)
+            res["ev_t"] = mem.full((N, int(event_cap)), np.nan, np.float64)
+            res["ev_u"] = mem.full((N, int(event_cap), n), np.nan, np.float64)
+            res["ev_n"] = mem.full(N, 0, np.int32)
     nout = f.output.nout if isinstance(f, UserModel) and f.output is not None else 0
     if nout and keep_states:
-        res["out"] = np.empty((nout, N))
+        res["out"] = mem.empty((nout, N), np.float64)
     if nout and reduction is not None:
-        res["reduced"] = np.empty(nout)
+        res["reduced"] = mem.empty(nout, np.float64)
     out = _abi.Outputs()
     out.us = _ptr(us)
     for k, v in res.items():
         setattr(out, k, _ptr(v))
     if "ev_t" in res:
         out.ev_cap = int(event_cap)
-    ctx.solve(P, O, out)
+    if mem.resident:
+        cur = torch.cuda.current_stream(device)
+        if stream != cur:  # the buffers above were filled on the current stream
+            stream.wait_stream(cur)
+            for t in mem.keep:
+                t.record_stream(stream)
+        ctx.solve_device(P, O, out, C.c_void_p(stream.cuda_stream))
+    else:
+        ctx.solve(P, O, out)
     st = ctx.stats()
     return EnsembleArrays(saveat=saveat, us=us, t0=t0, u0=u0, kernel_ms=st.kernel_ms,
                           total_ms=st.total_ms, lanes=st.lanes, grid=st.grid,
                           graph=bool(st.graph),
-                          save_layout=ealg.save_layout, nvec=nvec, **res)
+                          save_layout=ealg.save_layout, nvec=nvec,
+                          resident=mem.resident, keep=mem.keep if mem.resident else None,
+                          **res)
 
 
 def _dense_interp(packed, k, i, alg, ealg, lanes, opts, kw):

@@ -128,3 +128,37 @@ def test_c_abi_example_builds_and_loads():
         pass
     r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
     assert r.returncode == 3 and "no GPU" in r.stdout
+
+
+def test_stream_only_for_device_resident_solves():
+    """A stream names where a device-resident solve (u0 a GPU tensor) is
+    enqueued; with host arrays it is an error, raised before any GPU call."""
+    from mideq import EnsembleMI355X, device_solve, lorenz
+    from mideq.scimlbase import Tsit5
+    with pytest.raises(ValueError, match="device-resident"):
+        device_solve(lorenz, np.zeros((3, 2)), np.zeros((3, 2)), np.zeros(2), np.ones(2), None,
+                     Tsit5(), EnsembleMI355X(), stream=object())
+
+
+def test_device_buffers_dtypes_and_shapes():
+    """The buffers of a device-resident solve, made here in host memory by
+    the same code: shapes given as an int or a tuple, every dtype the C ABI
+    takes, inputs converted (or used in place when they already fit), and
+    each buffer kept for the solve's lifetime."""
+    torch = pytest.importorskip("torch")
+    from mideq.scimlbase import _DeviceMem
+    m = _DeviceMem(torch, torch.device("cpu"))
+    a = m.full(5, 0, np.int32)
+    assert a.shape == (5,) and a.dtype == torch.int32 and (a == 0).all()
+    b = m.full((2, 3, 4), np.nan, np.float64)
+    assert b.shape == (2, 3, 4) and b.dtype == torch.float64 and b.isnan().all()
+    for dt, tdt in ((np.float64, torch.float64), (np.float32, torch.float32),
+                    (np.int64, torch.int64), (np.int32, torch.int32), (np.int8, torch.int8)):
+        assert m.empty(3, dt).dtype == tdt and m.empty((2, 3), dt).shape == (2, 3)
+    x = np.arange(6.0).reshape(2, 3)
+    t = m.arr(x.T, np.float64)  # a strided view: copied contiguous
+    assert t.is_contiguous() and np.array_equal(t.numpy(), x.T)
+    y = torch.arange(4, dtype=torch.float64)
+    assert m.arr(y, np.float64).data_ptr() == y.data_ptr()  # already fits: in place
+    assert m.arr(np.array([1, 2]), np.int8).dtype == torch.int8
+    assert len(m.keep) == 15
