@@ -32,7 +32,7 @@ import SciMLBase: __solve, EnsembleSolution, ReturnCode
 # method types re-exported by DifferentialEquations (test/qa/qa.jl:15,97,119)
 using OrdinaryDiffEq: Tsit5, Rodas5, Rodas5P, CompositeAlgorithm
 
-export EnsembleMI355X, register_model!, DeviceCallback
+export EnsembleMI355X, register_model!, DeviceCallback, solve_resident!
 
 const LIB = Ref{String}(get(ENV, "MIDEQ_LIB",
     joinpath(@__DIR__, "..", "_build", "libmideq.so")))
@@ -304,6 +304,41 @@ function _device_solve(probs, alg, ealg::EnsembleMI355X, f, sa; abstol, reltol, 
         check(ctx, rc, "mideq_last_stats")
     end
     return (; US, UF, TFo, RC, NA, NR, NF, NS, EVT, EVU, EVN, ev, lanes = st[].lanes)
+end
+
+# Device-resident solve (the zero-copy EnsembleSolution view): the inputs and
+# outputs are arrays already in HBM (AMDGPU.jl ROCArrays, or any array whose
+# `pointer` is a device address) in the ABI's layout -- U0 (N, n) and P
+# (N, np) column-major, so the trajectory index runs fastest; T0, TF (N);
+# outputs UF (N, n), TFo, RC (Int8), NA, NR, NF, NS (Int32) of length N, and
+# US (N, n, ns) with saveat.  Calls mideq_ensemble_solve_device, enqueued on
+# `stream` (a hipStream_t; C_NULL = the context's stream) without
+# synchronizing: nothing is copied to or from the host, and the caller keeps
+# every array alive until the stream has passed the solve.
+function solve_resident!(f, alg, ealg::EnsembleMI355X, U0, P, T0, TF, out;
+                         saveat = nothing, nvec = nothing, abstol = 1e-6, reltol = 1e-3,
+                         dt = 0.0, dtmin = 0.0, dtmax = 0.0, maxiters = 0,
+                         stream::Ptr{Cvoid} = C_NULL)
+    haskey(MODEL_FOR, f) || error("solve_resident!: unregistered right-hand side")
+    N, n = size(U0, 1), size(U0, 2)
+    ns = saveat === nothing ? 0 : length(saveat)
+    ctx = context(ealg.device)
+    ptr(x) = x === nothing ? C_NULL : reinterpret(Ptr{Cvoid}, pointer(x))
+    prob_c = MideqProblem(MODEL_FOR[f], n, max(size(P, 2), 1), 0, N, ptr(U0), ptr(P),
+                          ptr(T0), ptr(TF), 0.0, 0.0, ptr(nvec))
+    opt_c = MideqOptions(alg_code(alg), ealg.precision === :mixed ? 1 : 0,
+                         abstol, reltol, dt, dtmin, dtmax, maxiters, ptr(saveat), ns,
+                         ealg.save_dtype === Float32 ? 1 : 0, C_NULL, C_NULL, 1,
+                         ealg.schedule === :lane ? 1 : ealg.schedule === :persistent ? 2 : 0,
+                         0, 0, 0, 0, 0, 0, C_NULL, 0, 0, C_NULL)
+    out_c = MideqOutputs(ns > 0 ? ptr(out.US) : C_NULL, ptr(out.UF), ptr(out.TFo),
+                         ptr(out.RC), ptr(out.NA), ptr(out.NR), ptr(out.NF), ptr(out.NS),
+                         C_NULL, C_NULL, C_NULL, C_NULL, C_NULL, C_NULL, C_NULL, C_NULL, 0, 0)
+    rc = ccall((:mideq_ensemble_solve_device, LIB[]), Cint,
+               (Ptr{Cvoid}, Ref{MideqProblem}, Ref{MideqOptions}, Ref{MideqOutputs},
+                Ptr{Cvoid}), ctx.h, prob_c, opt_c, out_c, stream)
+    check(ctx, rc, "mideq_ensemble_solve_device")
+    return out
 end
 
 # Dense output sol(t) (test/core.jl:51-58): the trajectory solved again with
