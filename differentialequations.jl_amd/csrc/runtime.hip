@@ -1219,11 +1219,21 @@ static int copy_to_host(mideq_ctx* c, void* dst, const void* src, size_t bytes, 
     return MIDEQ_OK;
   }
   if (c->stages.empty()) {
-    c->stages.resize(kStageWorkers);
-    for (auto& st : c->stages) {
-      HIP_TRY(c, hipHostMalloc(&st.pinned, kStageBytes, hipHostMallocDefault));
-      HIP_TRY(c, hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
+    // all workers or none: a failure part way frees what was made
+    std::vector<mideq_ctx::Stage> made(kStageWorkers);
+    hipError_t e = hipSuccess;
+    for (auto& st : made) {
+      if ((e = hipHostMalloc(&st.pinned, kStageBytes, hipHostMallocDefault)) != hipSuccess) break;
+      if ((e = hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking)) != hipSuccess) break;
     }
+    if (e != hipSuccess) {
+      for (auto& st : made) {
+        if (st.s) (void)hipStreamDestroy(st.s);
+        if (st.pinned) (void)hipHostFree(st.pinned);
+      }
+      return set_err(c, MIDEQ_E_HIP, "staging buffers: %s", hipGetErrorString(e));
+    }
+    c->stages = std::move(made);
   }
   // the workers' DMAs start once everything enqueued on s so far is done
   hipEvent_t ready;
