@@ -387,7 +387,10 @@ class EnsembleMI355X:
     each capped at iter_cap[i] attempted steps, then AutoSwitch from t0 for
     the capped-out and the hinted-stiff ones.  save_layout: "soa" (saved
     values [ns, n, N]) or "trajectory" ([N, ns, n], one trajectory's saved
-    states contiguous, as upstream's per-trajectory sol.u)."""
+    states contiguous, as upstream's per-trajectory sol.u).  resident: the
+    packed problems go to the GPU once and every output stays there as a
+    torch tensor; each ODESolution's states are views of those tensors (its
+    times and counts are read back once per solve)."""
     precision: str = "fp64"
     schedule: str = "auto"
     device: int = 0
@@ -401,6 +404,7 @@ class EnsembleMI355X:
     controller: str = "reference"   # or "fast" (n <= 3 models)
     auto_passes: int = 0
     iter_cap: Any = None
+    resident: bool = False
 
 
 # ---------------------------------------------------------------- solutions
@@ -469,11 +473,24 @@ class EnsembleArrays:
     ev_n: Optional[np.ndarray] = None     # [N] points produced (may exceed cap)
     resident: bool = False                # the arrays are torch tensors on the GPU
     keep: Optional[list] = field(default=None, repr=False, compare=False)  # buffers in use
+    _meta: Optional[dict] = field(default=None, repr=False, compare=False)
+
+    def _h(self, name):
+        """Per-trajectory scalars (counts, codes, times) as host arrays: for a
+        device-resident result read back once, on first use (the states stay
+        on the GPU)."""
+        if not self.resident:
+            return getattr(self, name)
+        if self._meta is None:
+            names = ("naccept", "nreject", "nf", "retcode", "t_final", "t0", "nvec", "ev_n")
+            self._meta = {k: (None if getattr(self, k) is None else getattr(self, k).cpu().numpy())
+                          for k in names}
+        return self._meta[name]
 
     def saved(self, i):
         """Saved states of trajectory i (0-based), [ns, n_i]."""
         vals = self.us[i] if self.save_layout == "trajectory" else self.us[:, :, i]
-        return vals[:, :int(self.nvec[i])] if self.nvec is not None else vals
+        return vals[:, :int(self._h("nvec")[i])] if self.nvec is not None else vals
 
     def events(self, i):
         """(times, states) of the points trajectory i saved at callback events,
@@ -482,7 +499,7 @@ class EnsembleArrays:
         view on the GPU."""
         if self.ev_t is None:
             return np.empty(0), np.empty((0, self.u0.shape[0]))
-        m = min(int(self.ev_n[i]), self.ev_t.shape[1])
+        m = min(int(self._h("ev_n")[i]), self.ev_t.shape[1])
         if self.resident:
             return self.ev_t[i, :m].cpu().numpy(), self.ev_u[i, :m]
         return self.ev_t[i, :m].copy(), self.ev_u[i, :m].copy()
@@ -493,9 +510,10 @@ class EnsembleArrays:
         (a saveat point first at an equal time, as upstream saves it before
         applying the callback); without saveat, between u0 and the final
         state."""
-        st = Stats(int(self.naccept[i]), int(self.nreject[i]), int(self.nf[i]))
-        rc = ReturnCode(int(self.retcode[i]))
-        back = self.t_final is not None and float(self.t_final[i]) < float(self.t0[i])
+        h = self._h
+        st = Stats(int(h("naccept")[i]), int(h("nreject")[i]), int(h("nf")[i]))
+        rc = ReturnCode(int(h("retcode")[i]))
+        back = self.t_final is not None and float(h("t_final")[i]) < float(h("t0")[i])
         et, eu = self.events(i)
         # host results hand out copies, device-resident ones views
         own = (lambda v: v) if self.resident else np.array
@@ -514,8 +532,8 @@ class EnsembleArrays:
                 ts, us = allt[order], [allu[k] for k in order]
             return ODESolution(np.array(ts, dtype=np.float64), [own(v) for v in us], rc, st,
                                prob)
-        m = int(self.nvec[i]) if self.nvec is not None else self.u0.shape[0]
-        ts = np.concatenate([[float(self.t0[i])], et, [float(self.t_final[i])]])
+        m = int(h("nvec")[i]) if self.nvec is not None else self.u0.shape[0]
+        ts = np.concatenate([[float(h("t0")[i])], et, [float(h("t_final")[i])]])
         us = ([own(self.u0[:m, i])] + [eu[k, :m] for k in range(len(et))] +
               [own(self.u_final[:m, i])])
         return ODESolution(ts, us, rc, st, prob)
@@ -849,7 +867,9 @@ def device_solve(f, u0, p, t0, tf, nvec, alg, ealg, saveat=None, abstol=1e-6, re
             res["ev_u"] = mem.full((N, int(event_cap), n), np.nan, np.float64)
             res["ev_n"] = mem.full(N, 0, np.int32)
     nout = f.output.nout if isinstance(f, UserModel) and f.output is not None else 0
-    if nout and keep_states:
+    if nout and (keep_states or (mem.resident and reduction is not None)):
+        # the device entry point folds the per-trajectory values from a
+        # buffer of the caller's (the host entry point makes its own)
         res["out"] = mem.empty((nout, N), np.float64)
     if nout and reduction is not None:
         res["reduced"] = mem.empty(nout, np.float64)
@@ -911,6 +931,20 @@ def _dense_interp(packed, k, i, alg, ealg, lanes, opts, kw):
     return interp
 
 
+def _resident_inputs(ealg, u0, p, t0, tf, nvec):
+    """The packed inputs on the GPU (torch tensors) for a resident solve."""
+    if not ealg.resident:
+        return u0, p, t0, tf, nvec
+    import torch
+    dev = torch.device("cuda", ealg.device)
+
+    def put(a, dt):
+        return None if a is None else torch.from_numpy(np.ascontiguousarray(a, dtype=dt)).to(dev)
+
+    return (put(u0, np.float64), put(p, np.float64), put(t0, np.float64), put(tf, np.float64),
+            put(nvec, np.int32))
+
+
 def _subset_ealg(ealg, pending):
     """The per-trajectory settings of `ealg` restricted to the trajectories
     `pending` (a rerun pass), the launch order kept as their relative order."""
@@ -961,6 +995,7 @@ def solve(prob, alg, ensemblealg=None, *, trajectories=None, saveat=None, abstol
         # output_func (and reduction) on the device: one pass, no ODESolutions
         red = prob.reduction if isinstance(prob.reduction, DeviceReduction) else None
         f, u0, p, t0, tf, nvec, probs = _pack(prob, idx, 1)
+        u0, p, t0, tf, nvec = _resident_inputs(ensemblealg, u0, p, t0, tf, nvec)
         arr = device_solve(f, u0, p, t0, tf, nvec, alg, ensemblealg, saveat=None,
                            abstol=abstol, reltol=reltol, dt=dt, dtmin=dtmin, dtmax=dtmax,
                            maxiters=maxiters, output=prob.output_func, reduction=red,
@@ -983,9 +1018,9 @@ def solve(prob, alg, ensemblealg=None, *, trajectories=None, saveat=None, abstol
             f, u0, p, t0, tf, nvec, probs = _pack(prob, pending, repeat)
             ealg = ensemblealg if (repeat == 1 and len(batch) == N) else \
                 _subset_ealg(ensemblealg, pending)
-            arr = device_solve(f, u0, p, t0, tf, nvec, alg, ealg, saveat=sa, abstol=abstol,
-                               reltol=reltol, dt=dt, dtmin=dtmin, dtmax=dtmax,
-                               maxiters=maxiters, **kw)
+            arr = device_solve(f, *_resident_inputs(ealg, u0, p, t0, tf, nvec), alg, ealg,
+                               saveat=sa, abstol=abstol, reltol=reltol, dt=dt, dtmin=dtmin,
+                               dtmax=dtmax, maxiters=maxiters, **kw)
             if first_arrays is None:
                 first_arrays = arr
             rerun = []

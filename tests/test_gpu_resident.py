@@ -7,8 +7,9 @@ import numpy as np
 import pytest
 
 from gpu_util import ALGS, FUNCS, say
-from mideq import (ContinuousCallback, DeviceOutputFunc, EnsembleMI355X, UserModel,
-                   device_solve, workloads)
+from mideq import (ContinuousCallback, DeviceOutputFunc, DeviceReduction, EnsembleMI355X,
+                   EnsembleProblem, ODEProblem, Tsit5, UserModel, device_solve, lorenz, remake,
+                   solve, workloads)
 from test_user_model import BALL
 
 pytestmark = pytest.mark.gpu
@@ -148,3 +149,33 @@ def test_resident_callbacks_event_points_and_reduction():
         np.testing.assert_array_equal(sd.t, sh.t)
         for vh, vd in zip(sh.u, sd.u):
             np.testing.assert_array_equal(vd.cpu().numpy(), vh)
+
+
+def test_resident_high_level_solve():
+    """solve(EnsembleProblem, Tsit5(), EnsembleMI355X(resident=True)): the
+    same ODESolutions as the host form (times, codes, counts; states bit-equal
+    but as GPU views), and the on-device output_func / reduction path with
+    its folded value left on the GPU."""
+    N = 64
+    say(f"resident high-level solve N={N}")
+    rng = np.random.default_rng(305)
+    P = np.vstack([rng.uniform(9, 11, N), rng.uniform(25, 30, N), rng.uniform(2.4, 2.9, N)])
+    prob = ODEProblem(lorenz, [1.0, 0.0, 0.0], (0.0, 3.0), [10.0, 28.0, 8 / 3])
+    ens = EnsembleProblem(prob, prob_func=lambda pr, i, repeat: remake(pr, p=P[:, i - 1]))
+    kw = dict(trajectories=N, saveat=0.5, abstol=1e-9, reltol=1e-9)
+    h = solve(ens, Tsit5(), EnsembleMI355X(), **kw)
+    d = solve(ens, Tsit5(), EnsembleMI355X(resident=True), **kw)
+    assert d.arrays.resident and len(d.u) == len(h.u) == N
+    for a, b in zip(h.u, d.u):
+        np.testing.assert_array_equal(b.t, a.t)
+        assert b.retcode == a.retcode and b.stats == a.stats
+        assert all(torch.is_tensor(v) and v.is_cuda for v in b.u)
+        np.testing.assert_array_equal(torch.stack(b.u).cpu().numpy(), np.stack(a.u))
+    out = DeviceOutputFunc("out[0] = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];", 1)
+    ens2 = EnsembleProblem(prob, prob_func=ens.prob_func, output_func=out,
+                           reduction=DeviceReduction("sum"))
+    h2 = solve(ens2, Tsit5(), EnsembleMI355X(), trajectories=N, abstol=1e-9, reltol=1e-9)
+    d2 = solve(ens2, Tsit5(), EnsembleMI355X(resident=True), trajectories=N, abstol=1e-9,
+               reltol=1e-9)
+    assert torch.is_tensor(d2.u) and d2.u.is_cuda
+    np.testing.assert_array_equal(d2.u.cpu().numpy(), h2.u)
