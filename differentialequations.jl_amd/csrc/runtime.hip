@@ -921,6 +921,7 @@ static int run_bruss(mideq_ctx* c, const mideq_problem* P, const mideq_options* 
       bs.nqueue = S;
       float best = 1e30f;
       int best_l = 64, best_b = 0;
+      bool warmed = false;
       for (int cand : {64, 32}) {
         const int full = bruss_blocks_per_cu(c, bs, cand, O->alg, O->precision, O->controller);
         // residencies: all that fit (0), one block per CU fewer; the
@@ -934,9 +935,15 @@ static int run_bruss(mideq_ctx* c, const mideq_problem* P, const mideq_options* 
         for (int ci = 0; ci < ncand; ci++) {
           const int use_b = bpc > 0 ? bpc : cands[ci];
           int g = 0;
-          if ((rc = bruss_launch_with(c, nullptr, bs, cand, O->alg, O->precision, O->controller,
-                                      false, s, &g, use_b)))
-            return rc;
+          // one untimed run first (clocks up, caches warm); the candidates
+          // after it are timed on their first run (repeats of a candidate
+          // differ by ~1 %, the candidates by 10-40 %)
+          if (!warmed) {
+            if ((rc = bruss_launch_with(c, nullptr, bs, cand, O->alg, O->precision,
+                                        O->controller, false, s, &g, use_b)))
+              return rc;
+            warmed = true;
+          }
           HIP_TRY(c, hipEventRecord(c->tune0, s));
           if ((rc = bruss_launch_with(c, nullptr, bs, cand, O->alg, O->precision, O->controller,
                                       false, s, &g, use_b)))
