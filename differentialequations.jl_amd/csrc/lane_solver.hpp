@@ -291,11 +291,27 @@ struct time_dep<M, decltype((void)M::TDEP, void())> {
   static constexpr bool value = M::TDEP;
 };
 
+// save_everystep of a model (M::EVERYSTEP = true): every accepted step's end
+// point (t, u) is saved, into the same per-trajectory arrays as the points
+// saved at callback events.
+template <class M, class = void>
+struct every_step {
+  static constexpr bool value = false;
+};
+template <class M>
+struct every_step<M, decltype((void)M::EVERYSTEP, void())> {
+  static constexpr bool value = M::EVERYSTEP;
+};
+
 template <class M, bool MIX, int ALG, bool FAST>
 struct Lane {
   static constexpr int N = M::N;
   static constexpr bool kTdep = time_dep<M>::value;
   static constexpr int kNcb = cb_count<M>::value;
+  static constexpr bool kEveryStep = every_step<M>::value;
+  // points saved beside saveat (at events, at every step): kept only by
+  // the kernels of models that make them
+  static constexpr bool kExtra = kNcb > 0 || kEveryStep;
   static constexpr int kNout = out_count<M>::value;
   static constexpr int NP = M::NP;
   static constexpr int kAlg = ALG & (ALG_DENSE - 1);
@@ -331,7 +347,8 @@ struct Lane {
   double gprev[kNcb > 0 ? kNcb : 1];
   int32_t nevent;
   double tevent;
-  int32_t nevsave;  // points saved at events so far
+  int32_t nevsave;  // points saved at events (and every step) so far
+  double tlast;     // time of the last saved point (saveat or extra), t0 at the start
   int32_t cap;  // TRUST pass: iteration cap of this trajectory
 
   // point k of this trajectory's event points: (t, v), stored while k < ev_cap
@@ -525,6 +542,7 @@ struct Lane {
     nevent = 0;
     tevent = __builtin_nan("");
     nevsave = 0;
+    tlast = t0;
     if constexpr (kNcb > 0) {
 #pragma unroll
       for (int k = 0; k < kNcb; k++) gprev[k] = M::cb_g(k, u, p, t0);
@@ -1138,7 +1156,8 @@ struct Lane {
           interp(th_ev, uend);
         }
       }
-      // savevalues!
+      // savevalues!: the saveat points up to t (exact: one of them is t)
+      [[maybe_unused]] bool exact = false;
       {
         while (tdir * (t - tnext) >= 0.0) {  // tnext at or behind t (+-inf: never)
           if (a.us) {
@@ -1151,6 +1170,10 @@ struct Lane {
               interp((sa - tprev) / sdt, val);
             }
             store_save(a, isave, val);
+          }
+          if constexpr (kExtra) {
+            exact = exact || tnext == t;
+            tlast = tnext;
           }
           isave += back ? -1 : 1;
           set_tnext(a);
@@ -1167,30 +1190,54 @@ struct Lane {
           for (int c = 0; c < N; c++) k1f[c] = k32[6][c];
         }
       }
+      // savevalues! of the step end (not forced): the saveat points are
+      // stored above; save_everystep adds (t, u) unless the last saved point
+      // has this time.  Returns savedexactly: this call saved a point at t
+      // (a saveat point counts in the first call only).
+      [[maybe_unused]] auto step_point = [&]() -> bool {
+        bool se = exact;
+        exact = false;
+        if constexpr (kEveryStep) {
+          if (t != tlast) {
+            save_event(a, u);
+            tlast = t;
+            se = true;
+          }
+        }
+        return se;
+      };
       if constexpr (kNcb > 0) {
         // apply_callback!: the continuous event's affect at its time, then
         // every discrete callback whose condition holds at the step end;
-        // any affect re-evaluates the FSAL stage (the state may have moved)
-        // save_positions: (t, u) just before and / or just after an affect
+        // any affect re-evaluates the FSAL stage (the state may have moved).
+        // Each applied callback first runs savevalues! of the step end, then
+        // save_positions: (t, u) just before the affect unless that call
+        // saved a point at t, and / or just after it.  A step with no
+        // callback applied runs savevalues! once after them.
         bool fired = false, term = false;
         if (ev >= 0) {
-          if (M::cb_save(ev) & 1) save_event(a, u);
+          const bool se = step_point();
+          if ((M::cb_save(ev) & 1) && !se) save_event(a, u);
           M::cb_affect(ev, u, p, t);
           if (M::cb_save(ev) & 2) save_event(a, u);
+          if (M::cb_save(ev)) tlast = t;
           fired = true;
           term = term || M::cb_term(ev);
           nevent++;
         }
         for (int k = 0; k < kNcb; k++) {
           if (M::cb_kind(k) == 1 && M::cb_g(k, u, p, t) != 0.0) {
-            if (M::cb_save(k) & 1) save_event(a, u);
+            const bool se = step_point();
+            if ((M::cb_save(k) & 1) && !se) save_event(a, u);
             M::cb_affect(k, u, p, t);
             if (M::cb_save(k) & 2) save_event(a, u);
+            if (M::cb_save(k)) tlast = t;
             fired = true;
             term = term || M::cb_term(k);
             nevent++;
           }
         }
+        if (!fired) step_point();
         if (fired) {
           tevent = t;
           fsal_init();
@@ -1202,6 +1249,8 @@ struct Lane {
           rc = RC_TERMINATED;
           return;
         }
+      } else if constexpr (kEveryStep) {
+        step_point();
       }
       bool bad = false;
 #pragma unroll
@@ -1249,7 +1298,7 @@ struct Lane {
     if (a.dt_first) a.dt_first[i] = dtfirst;
     if (a.nevent) a.nevent[i] = nevent;
     if (a.t_event) a.t_event[i] = tevent;
-    if constexpr (kNcb > 0) {  // event points: models with callbacks only
+    if constexpr (kExtra) {  // event / step points: models that make them
       if (a.ev_n) a.ev_n[i] = nevsave;
       if (a.ev_t) {
         // slots past this trajectory's event points hold NaN

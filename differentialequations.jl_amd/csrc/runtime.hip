@@ -57,6 +57,7 @@ struct UserModel {
   int n = 0, np = 0;
   int ncb = 0;   // callbacks compiled into the model
   int nout = 0;  // output_func values per trajectory
+  bool everystep = false;  // save_everystep compiled into the model
   std::string model_src;
   std::map<int, UserKernels> kernels;
 };
@@ -379,6 +380,10 @@ static int check_user_spec(const mideq_user_model* m, std::string* why) {
     *why = "bad nout or null output";
     return MIDEQ_E_ARG;
   }
+  if (m->save_everystep != 0 && m->save_everystep != 1) {
+    *why = "save_everystep must be 0 or 1";
+    return MIDEQ_E_ARG;
+  }
   if (m->ncallbacks < 0 || m->ncallbacks > MIDEQ_USER_NCBMAX ||
       (m->ncallbacks > 0 && !m->callbacks)) {
     *why = "bad callback count or null callbacks";
@@ -439,6 +444,7 @@ static std::string user_model_source(const mideq_user_model* m) {
   std::string s = "namespace mideq {\nstruct ModelUser {\n  static constexpr int N = " +
                   std::to_string(m->n) + ", NP = " + std::to_string(np) + ";\n";
   s += std::string("  static constexpr bool TDEP = ") + (tdep ? "true" : "false") + ";\n";
+  if (m->save_everystep) s += "  static constexpr bool EVERYSTEP = true;\n";
   s += std::string("  template <class T>\n  __device__ static void f(T* du, const T* u, const T* p") +
        (tdep ? ", T t" : "") + ") {\n";
   // the statements in a block of their own: a local t shadows the time
@@ -672,6 +678,7 @@ extern "C" int mideq_model_register(mideq_ctx* c, const mideq_user_model* m,
   um.np = m->np > 0 ? m->np : 1;  // the kernel reads at least one parameter row
   um.ncb = m->ncallbacks;
   um.nout = m->nout;
+  um.everystep = m->save_everystep != 0;
   um.model_src = user_model_source(m);
   c->user_models.push_back(um);
   UserModel* p = &c->user_models.back();
@@ -1137,9 +1144,9 @@ static int validate_outputs(mideq_ctx* c, const mideq_problem* P, const mideq_ou
     return set_err(c, MIDEQ_E_ARG, "ev_t and ev_u are given together or not at all");
   if (out->ev_cap < 1) return set_err(c, MIDEQ_E_ARG, "ev_cap must be >= 1 with ev_t");
   const UserModel* um = user_model(c, P->model);
-  if (!um || um->ncb == 0)
+  if (!um || (um->ncb == 0 && !um->everystep))
     return set_err(c, MIDEQ_E_UNSUPPORTED,
-                   "event points (ev_t) need a user model with callbacks");
+                   "extra points (ev_t) need a user model with callbacks or save_everystep");
   return MIDEQ_OK;
 }
 

@@ -95,7 +95,7 @@ const EVENT_SAVES = IdDict{Any, Bool}()
 
 struct MideqUserModel
     n::Int32; np::Int32; rhs::Ptr{UInt8}; jac::Ptr{UInt8}
-    ncallbacks::Int32; reserved::Int32; callbacks::Ptr{MideqCallback}
+    ncallbacks::Int32; save_everystep::Int32; callbacks::Ptr{MideqCallback}
     nout::Int32; reserved2::Int32; output::Ptr{UInt8}
 end
 

@@ -42,7 +42,7 @@ class CallbackSpec(C.Structure):
 
 class UserModelSpec(C.Structure):
     _fields_ = [("n", C.c_int32), ("np", C.c_int32), ("rhs", C.c_char_p), ("jac", C.c_char_p),
-                ("ncallbacks", C.c_int32), ("reserved", C.c_int32),
+                ("ncallbacks", C.c_int32), ("save_everystep", C.c_int32),
                 ("callbacks", C.POINTER(CallbackSpec)), ("nout", C.c_int32),
                 ("reserved2", C.c_int32), ("output", C.c_char_p)]
 

@@ -178,7 +178,8 @@ class UserModel:
     Rosenbrock methods differentiate rhs in forward mode.  A model with no
     parameters still takes one (unused) parameter row."""
 
-    def __init__(self, n, np_, rhs, jac=None, name="user", callbacks=(), output=None):
+    def __init__(self, n, np_, rhs, jac=None, name="user", callbacks=(), output=None,
+                 save_everystep=False):
         if not (1 <= n <= _abi.USER_NMAX) or not (0 <= np_ <= _abi.USER_NPMAX):
             raise ValueError(f"user model n={n} np={np_} outside 1..{_abi.USER_NMAX} / "
                              f"0..{_abi.USER_NPMAX}")
@@ -192,6 +193,7 @@ class UserModel:
         if len(self.callbacks) > _abi.USER_NCBMAX:
             raise ValueError(f"at most {_abi.USER_NCBMAX} callbacks")
         self.output = output
+        self.save_everystep = bool(save_everystep)
         if output is not None and not (1 <= output.nout <= _abi.USER_NOUTMAX):
             raise ValueError(f"nout must be 1..{_abi.USER_NOUTMAX}")
         self._ids = {}
@@ -203,20 +205,22 @@ class UserModel:
         out = self.output
         sp = _abi.UserModelSpec(self.n, self.np, self.rhs.encode(),
                                 None if self.jac_src is None else self.jac_src.encode(),
-                                len(cbs), 0, arr if cbs else None,
+                                len(cbs), int(self.save_everystep), arr if cbs else None,
                                 0 if out is None else int(out.nout), 0,
                                 None if out is None else out.output.encode())
         sp._keep = arr
         return sp
 
-    def with_callbacks(self, callbacks=(), output=None):
-        """This model with `callbacks` (and an on-device output) compiled in
-        (cached)."""
+    def with_callbacks(self, callbacks=(), output=None, save_everystep=False):
+        """This model with `callbacks` (and an on-device output, and
+        save_everystep) compiled in (cached)."""
         cbs = CallbackSet(callbacks) if callbacks else ()
-        key = (cbs, output)
+        save_everystep = bool(save_everystep) or self.save_everystep
+        key = (cbs, output, save_everystep)
         if key not in self._with:
             self._with[key] = UserModel(self.n, self.np, self.rhs, self.jac_src,
-                                        name=self.name, callbacks=cbs, output=output)
+                                        name=self.name, callbacks=cbs, output=output,
+                                        save_everystep=save_everystep)
         return self._with[key]
 
     def check(self):
@@ -270,7 +274,8 @@ def user_model_of(f):
     if isinstance(f, UserModel):
         return f
     if f.name not in BUILTIN_SOURCE:
-        raise ValueError(f"callbacks are not available for the {f.name} model")
+        raise ValueError(f"callbacks and save_everystep are not available for the {f.name} "
+                         "model")
     if f.name not in _BUILTIN_USER:
         n, np_, rhs, jac = BUILTIN_SOURCE[f.name]
         _BUILTIN_USER[f.name] = UserModel(n, np_, rhs, jac=jac, name=f.name)
@@ -472,6 +477,7 @@ class EnsembleArrays:
     ev_u: Optional[np.ndarray] = None     # [N, cap, n] their states
     ev_n: Optional[np.ndarray] = None     # [N] points produced (may exceed cap)
     resident: bool = False                # the arrays are torch tensors on the GPU
+    save_everystep: bool = False          # ev_* also hold every accepted step's end point
     keep: Optional[list] = field(default=None, repr=False, compare=False)  # buffers in use
     _meta: Optional[dict] = field(default=None, repr=False, compare=False)
 
@@ -506,10 +512,12 @@ class EnsembleArrays:
 
     def solution(self, i, prob=None):
         """ODESolution of trajectory i (0-based).  Points saved at callback
-        events are listed with the saveat points in the order of integration
-        (a saveat point first at an equal time, as upstream saves it before
-        applying the callback); without saveat, between u0 and the final
-        state."""
+        events and, with save_everystep, at every step are listed with the
+        saveat points in the order of integration (a saveat point first at an
+        equal time, as upstream's savevalues! saves it first); (t0, u0)
+        starts the solution without saveat or with save_everystep, and
+        without saveat the final state ends it unless the last point saved
+        already has its time (upstream's postamble)."""
         h = self._h
         st = Stats(int(h("naccept")[i]), int(h("nreject")[i]), int(h("nf")[i]))
         rc = ReturnCode(int(h("retcode")[i]))
@@ -530,12 +538,19 @@ class EnsembleArrays:
                 allu = list(us) + [eu[k] for k in range(len(et))]
                 order = np.argsort(-allt if back else allt, kind="stable")
                 ts, us = allt[order], [allu[k] for k in order]
+            t0 = float(h("t0")[i])
+            if self.save_everystep and (len(ts) == 0 or ts[0] != t0):
+                m = int(h("nvec")[i]) if self.nvec is not None else self.u0.shape[0]
+                ts, us = np.concatenate([[t0], ts]), [self.u0[:m, i]] + list(us)
             return ODESolution(np.array(ts, dtype=np.float64), [own(v) for v in us], rc, st,
                                prob)
         m = int(h("nvec")[i]) if self.nvec is not None else self.u0.shape[0]
-        ts = np.concatenate([[float(h("t0")[i])], et, [float(h("t_final")[i])]])
-        us = ([own(self.u0[:m, i])] + [eu[k, :m] for k in range(len(et))] +
-              [own(self.u_final[:m, i])])
+        tf = float(h("t_final")[i])
+        ts = np.concatenate([[float(h("t0")[i])], et])
+        us = [own(self.u0[:m, i])] + [eu[k, :m] for k in range(len(et))]
+        cut = self.ev_t is not None and int(self._h("ev_n")[i]) > self.ev_t.shape[1]
+        if len(et) == 0 or et[-1] != tf or cut:
+            ts, us = np.concatenate([ts, [tf]]), us + [own(self.u_final[:m, i])]
         return ODESolution(ts, us, rc, st, prob)
 
 
@@ -745,13 +760,17 @@ def _resident_device(u0):
 def device_solve(f, u0, p, t0, tf, nvec, alg, ealg, saveat=None, abstol=1e-6, reltol=1e-3,
                  dt=0.0, dtmin=0.0, dtmax=0.0, maxiters=0, dt0=None, alg_hint=None,
                  order=None, block=0, blocks_per_cu=0, callback=None, output=None,
-                 reduction=None, keep_states=True, event_cap=16, stream=None):
+                 reduction=None, keep_states=True, event_cap=16, stream=None,
+                 save_everystep=False):
     """One call of mideq_ensemble_solve on host arrays -> EnsembleArrays.
     callback: ContinuousCallback / DiscreteCallback / CallbackSet, compiled
     into the right-hand side (a UserModel, or a built-in model's source).
     output: DeviceOutputFunc (values per trajectory in .out); reduction:
     DeviceReduction or its op name (folded values in .reduced, and with
-    keep_states=False nothing per trajectory comes back).
+    keep_states=False nothing per trajectory comes back).  save_everystep:
+    every accepted step's end point is kept too (one-lane models; built-in
+    ones through their source), with the event points, event_cap points per
+    trajectory at most (the count goes on past it).
 
     Device-resident form: when u0 is a torch tensor in GPU memory the call
     goes to mideq_ensemble_solve_device instead.  The other inputs are used
@@ -761,8 +780,8 @@ def device_solve(f, u0, p, t0, tf, nvec, alg, ealg, saveat=None, abstol=1e-6, re
     torch.cuda.Stream; default the current stream) -- synchronize before
     reading.  The result's saved(i) and solution(i) are views into those
     tensors (no copies)."""
-    if callback or output is not None:
-        f = user_model_of(f).with_callbacks(callback or (), output)
+    if callback or output is not None or save_everystep:
+        f = user_model_of(f).with_callbacks(callback or (), output, save_everystep)
     if isinstance(reduction, DeviceReduction):
         reduction = reduction.op
     dev = _resident_device(u0)
@@ -857,11 +876,13 @@ def device_solve(f, u0, p, t0, tf, nvec, alg, ealg, saveat=None, abstol=1e-6, re
     else:
         res = dict(u_final=None, t_final=None, retcode=None, naccept=None, nreject=None,
                    nf=None, nstiff=None, dt_first=None)
-    if isinstance(f, UserModel) and f.callbacks and keep_states:
-        res["nevent"] = mem.full(N, 0, np.int32)
-        res["t_event"] = mem.empty(N, np.float64)
-        if any(_save_bits(getattr(cb, "save_positions", (False, False)))
-               for cb in f.callbacks):
+    everystep = isinstance(f, UserModel) and f.save_everystep
+    if isinstance(f, UserModel) and (f.callbacks or everystep) and keep_states:
+        if f.callbacks:
+            res["nevent"] = mem.full(N, 0, np.int32)
+            res["t_event"] = mem.empty(N, np.float64)
+        if everystep or any(_save_bits(getattr(cb, "save_positions", (False, False)))
+                            for cb in f.callbacks):
             # points saved at events (save_positions): event_cap per trajectory
             res["ev_t"] = mem.full((N, int(event_cap)), np.nan, np.float64)
             res["ev_u"] = mem.full((N, int(event_cap), n), np.nan, np.float64)
@@ -893,6 +914,7 @@ def device_solve(f, u0, p, t0, tf, nvec, alg, ealg, saveat=None, abstol=1e-6, re
                           total_ms=st.total_ms, lanes=st.lanes, grid=st.grid,
                           graph=bool(st.graph),
                           save_layout=ealg.save_layout, nvec=nvec,
+                          save_everystep=bool(everystep and keep_states),
                           resident=mem.resident, keep=mem.keep if mem.resident else None,
                           **res)
 
@@ -966,7 +988,14 @@ def _subset_ealg(ealg, pending):
 def solve(prob, alg, ensemblealg=None, *, trajectories=None, saveat=None, abstol=1e-6,
           reltol=1e-3, dt=0.0, dtmin=0.0, dtmax=0.0, maxiters=100000, **kw):
     """solve(EnsembleProblem, alg, EnsembleMI355X(); trajectories, ...)
-    (qa.jl:192) or solve(ODEProblem, alg) for a single trajectory."""
+    (qa.jl:192) or solve(ODEProblem, alg) for a single trajectory.
+
+    save_everystep=True keeps every accepted step's end point in each
+    solution (upstream's default without saveat; here opt-in, since it
+    compiles the model with the step saves; sol(t) interpolates without it).
+    The points per trajectory are capped by event_cap, by default what
+    ~1 GiB holds over the ensemble (at least 16, at most 65,536); past the
+    cap a solution ends with the final state after the points kept."""
     if isinstance(prob, ODEProblem):
         ens = EnsembleProblem(prob)
         es = solve(ens, alg, ensemblealg or EnsembleMI355X(), trajectories=1, saveat=saveat,
@@ -983,6 +1012,10 @@ def solve(prob, alg, ensemblealg=None, *, trajectories=None, saveat=None, abstol
         raise TypeError("trajectories is required")
     tstart = time.perf_counter()
     N = int(trajectories)
+    if kw.get("save_everystep") and "event_cap" not in kw:
+        n0 = max(1, int(np.size(prob.prob.u0)))
+        kw["event_cap"] = int(min(max(16, (1 << 27) // (max(N, 1) * (n0 + 1))),
+                                  min(int(maxiters), 65536)))
     batch_size = int(kw.pop("batch_size", N))
     if batch_size < 1 or N // batch_size < 1:
         raise ValueError("trajectories // batch_size cannot be less than 1")

@@ -184,10 +184,11 @@ typedef struct {
   double *t_event;      /* [ntraj] time of the last callback affect (NaN if none) */
   double *out;          /* [nout][ntraj] output_func values (user models with an output) */
   double *reduced;      /* [nout] the reduction of `out` over the trajectories */
-  /* Points saved at callback events (mideq_callback.save_positions), in the
-   * order of integration; user models with callbacks, one-lane kernels.
-   * ev_t / ev_u may be NULL together (no event points); slots past a
-   * trajectory's points are NaN. */
+  /* Points saved beside saveat, in the order of integration: at callback
+   * events (mideq_callback.save_positions) and at every accepted step
+   * (mideq_user_model.save_everystep); user models with callbacks or
+   * save_everystep, one-lane kernels.  ev_t / ev_u may be NULL together (no
+   * such points kept); slots past a trajectory's points are NaN. */
   double *ev_t;         /* [ntraj][ev_cap] times */
   double *ev_u;         /* [ntraj][ev_cap][n] states */
   int32_t *ev_n;        /* [ntraj] points produced; only the first ev_cap are stored */
@@ -310,7 +311,15 @@ typedef struct {
                        NULL: forward-mode automatic differentiation of rhs (the default
                        of OrdinaryDiffEq's Rosenbrock methods without a Jacobian) */
   int32_t ncallbacks;                /* 0..MIDEQ_USER_NCBMAX */
-  int32_t reserved;
+  int32_t save_everystep;            /* 1: save_everystep = true -- every accepted step's end
+                                        (t, u) is saved into mideq_outputs.ev_t / ev_u with
+                                        the callbacks' points, as OrdinaryDiffEq's
+                                        savevalues! pushes it: not when the last saved point
+                                        (saveat or extra) has the same time, and in a step
+                                        where a callback is applied, once, before the first
+                                        affect (its save_positions[1] point then is not
+                                        repeated).  (t0, u0) is the solution's first point.
+                                        0: off */
   const mideq_callback *callbacks;   /* [ncallbacks] or NULL */
   int32_t nout;       /* on-device output_func: values per trajectory, 0..MIDEQ_USER_NOUTMAX */
   int32_t reserved2;

@@ -336,6 +336,7 @@ static struct {
   int ncb;
   int kind[8], dir[8], term[8], ip[8];
   int save[8]; /* save_positions bits: 1 before the affect, 2 after */
+  int everystep; /* save_everystep: every accepted step's end point saved */
   oracle_user_cond cond;
   oracle_user_affect affect;
 } g_user;
@@ -354,6 +355,12 @@ int oracle_set_user_callbacks(int32_t ncb, const int32_t *kind, const int32_t *d
   g_user.cond = cond;
   g_user.affect = affect;
   for (int k = 0; k < 8; k++) g_user.save[k] = 0;
+  return 0;
+}
+
+int oracle_set_user_everystep(int32_t on) {
+  if (on != 0 && on != 1) return -1;
+  g_user.everystep = on;
   return 0;
 }
 
@@ -379,6 +386,23 @@ static void save_event(const oracle_options *o, int64_t idx, int n, double t, co
   res->nevsave++;
 }
 
+/* savevalues! of a step end, not forced (OrdinaryDiffEq integrator_utils),
+ * after the saveat points up to t are stored: save_everystep pushes (t, u)
+ * unless the last saved point has time t.  Returns savedexactly -- this call
+ * saved a point at t; a saveat point at t counts in the first call of the
+ * step only (the queue holds no more points <= t after it). */
+static int step_point(const oracle_options *o, int64_t idx, int n, double t, const double *u,
+                      int everystep, int *exact, double *tlast, oracle_result *res) {
+  int se = *exact;
+  *exact = 0;
+  if (everystep && t != *tlast) {
+    save_event(o, idx, n, t, u, res);
+    *tlast = t;
+    se = 1;
+  }
+  return se;
+}
+
 int oracle_set_user_model(int32_t n, int32_t np, oracle_user_f64 f64, oracle_user_f32 f32,
                           oracle_user_jac jac, oracle_user_tgrad tgrad, int32_t tdep) {
   if (n < 1 || n > NMAX || np < 0 || np > 16 || !f64 || (tdep && !tgrad)) return -1;
@@ -389,6 +413,7 @@ int oracle_set_user_model(int32_t n, int32_t np, oracle_user_f64 f64, oracle_use
   g_user.jac = jac;
   g_user.tgrad = tgrad;
   g_user.tdep = tdep != 0;
+  g_user.everystep = 0;
   return 0;
 }
 
@@ -1230,6 +1255,8 @@ static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
   res->first_stiff_iter = -1;
   res->t_event = NAN;
   const int ncb = (m->model == ORACLE_MODEL_USER) ? g_user.ncb : 0;
+  const int everystep = (m->model == ORACLE_MODEL_USER) ? g_user.everystep : 0;
+  double tlast = t0; /* time of the last saved point: (t0, u0) starts the solution */
   double gprev[8];
   for (int c = 0; c < n; c++) uprev[c] = u0[c];
   double t = t0, tcomp = 0.0;
@@ -1495,7 +1522,10 @@ static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
         }
       }
       /* savevalues!: every saveat point <= t */
+      int exact = 0;
       while (SAVE_LEFT && (back ? sa[isave] >= t : sa[isave] <= t)) {
+        exact |= sa[isave] == t;
+        tlast = sa[isave];
         if (save) {
           double val[NMAX];
           if (sa[isave] == t) {
@@ -1521,23 +1551,30 @@ static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
          * the FSAL stage */
         int fired = 0, term = 0;
         if (ev >= 0) {
-          if (g_user.save[ev] & 1) save_event(o, idx, n, t, uprev, res);
+          /* each applied callback: savevalues! of the step end, then the
+           * point before the affect unless that call saved one at t */
+          int se = step_point(o, idx, n, t, uprev, everystep, &exact, &tlast, res);
+          if ((g_user.save[ev] & 1) && !se) save_event(o, idx, n, t, uprev, res);
           g_user.affect(ev, uprev, m->p, t);
           if (g_user.save[ev] & 2) save_event(o, idx, n, t, uprev, res);
+          if (g_user.save[ev]) tlast = t;
           fired = 1;
           term |= g_user.term[ev];
           res->nevent++;
         }
         for (int k = 0; k < ncb; k++) {
           if (g_user.kind[k] == 1 && g_user.cond(k, uprev, m->p, t) != 0.0) {
-            if (g_user.save[k] & 1) save_event(o, idx, n, t, uprev, res);
+            int se = step_point(o, idx, n, t, uprev, everystep, &exact, &tlast, res);
+            if ((g_user.save[k] & 1) && !se) save_event(o, idx, n, t, uprev, res);
             g_user.affect(k, uprev, m->p, t);
             if (g_user.save[k] & 2) save_event(o, idx, n, t, uprev, res);
+            if (g_user.save[k]) tlast = t;
             fired = 1;
             term |= g_user.term[k];
             res->nevent++;
           }
         }
+        if (!fired) step_point(o, idx, n, t, uprev, everystep, &exact, &tlast, res);
         if (fired) {
           res->t_event = t;
           rhs64(m, uprev, t, k64[0]);
@@ -1554,6 +1591,8 @@ static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
           rc = ORACLE_RC_TERMINATED;
           break;
         }
+      } else if (everystep) {
+        step_point(o, idx, n, t, uprev, everystep, &exact, &tlast, res);
       }
       int bad = 0;
       for (int c = 0; c < n; c++) bad |= (u1[c] != u1[c]);

@@ -77,6 +77,10 @@ int oracle_set_user_callbacks(int32_t ncb, const int32_t *kind, const int32_t *d
 /* save_positions of those callbacks (bit 0 before the affect, bit 1 after;
  * oracle_set_user_callbacks resets them to 0) */
 int oracle_set_user_callback_saves(int32_t ncb, const int32_t *save_positions);
+/* save_everystep for the user model (0 or 1; reset by oracle_set_user_model):
+ * every accepted step's end point is saved into ev_t / ev_u beside the
+ * callbacks' points.  Returns 0, or -1 for another value. */
+int oracle_set_user_everystep(int32_t on);
 
 enum {
   ORACLE_ALG_TSIT5 = 0,

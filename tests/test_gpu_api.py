@@ -243,3 +243,23 @@ def test_dense_output_backward_span_and_terminate():
     v = s2(np.array([0.3, tend * 0.99, 1.0, 3.0]))
     np.testing.assert_allclose(v[:2, 0], np.exp(-np.array([0.3, tend * 0.99])), rtol=1e-8)
     assert np.isnan(v[2:]).all()
+
+
+def test_core_save_everystep_single_problem(oracle):
+    """solve(prob, Tsit5(); save_everystep=true) (upstream's default without
+    saveat, core.jl:9-35): sol.t holds t0 and every accepted step's end,
+    sol.u[1] == u0 and the last point is the final state, bit-exact against
+    the oracle's step points."""
+    import user_model_host as umh
+    say("save_everystep single problem")
+    prob = ODEProblem(mideq.UserModel(1, 1, "du[0] = p[0] * u[0];"), [0.5], (0.0, 1.0), [1.01])
+    sol = solve(prob, Tsit5(), save_everystep=True, abstol=1e-8, reltol=1e-8)
+    assert sol.retcode == ReturnCode.Success
+    assert sol.t[0] == 0.0 and sol.t[-1] == 1.0 and sol.u[0][0] == 0.5
+    assert len(sol.t) == sol.stats.naccept + 1 and (np.diff(sol.t) > 0).all()
+    umh.install(oracle, 1, 1, "du[0] = p[0] * u[0];", save_everystep=True)
+    r = oracle.ensemble("user", np.full((1, 1), 0.5), np.full((1, 1), 1.01), 0.0, 1.0,
+                        abstol=1e-8, reltol=1e-8, event_cap=4096)
+    m = int(r["ev_n"][0])
+    np.testing.assert_array_equal(sol.t[1:], r["ev_t"][0, :m])
+    np.testing.assert_array_equal(np.array([v[0] for v in sol.u[1:]]), r["ev_u"][0, :m, 0])

@@ -213,4 +213,72 @@ def test_event_points_discrete_and_terminate(oracle):
         m = int(g.ev_n[i])
         assert g.ev_t[i, m - 1] == g.t_final[i] and g.ev_u[i, m - 1, 0] == g.u_final[0, i]
         sol = g.solution(i)
-        assert len(sol.t) == m + 2 and sol.t[0] == 0.0 and sol.t[-1] == g.t_final[i]
+        # upstream's postamble adds no final point: the last saved one has its time
+        assert len(sol.t) == m + 1 and sol.t[0] == 0.0 and sol.t[-1] == g.t_final[i]
+        assert sol.u[-1][0] == g.u_final[0, i]
+
+
+def test_save_everystep_bitexact(oracle):
+    """save_everystep: every accepted step's end point, bit-exact against the
+    oracle; one point per accepted step, the last at tf with the final state,
+    and the solution is (t0, u0) followed by them.  With saveat holding tf the
+    last step's point is not repeated (its time was just saved)."""
+    N = 300
+    say(f"save_everystep lorenz N={N}")
+    w = workloads.lorenz(N, seed=116)
+    kw = dict(abstol=1e-8, reltol=1e-8, save_everystep=True, event_cap=512)
+    src = BUILTIN_SOURCE["lorenz"]
+    umh.install(oracle, 3, 3, src[2], jac=src[3], save_everystep=True)
+    g = run(lorenz, w["u0"], w["p"], 0.0, 2.0, **kw)
+    c = cpu(oracle, "user", w["u0"], w["p"], 0.0, 2.0, event_cap=512, abstol=1e-8,
+            reltol=1e-8)
+    assert (c["retcode"] == 1).all() and (g.ev_n <= 512).all()
+    np.testing.assert_array_equal(g.naccept, c["naccept"])
+    np.testing.assert_array_equal(g.u_final, c["u"])
+    np.testing.assert_array_equal(g.ev_n, c["ev_n"])
+    np.testing.assert_array_equal(g.ev_t, c["ev_t"])
+    np.testing.assert_array_equal(g.ev_u, c["ev_u"])
+    np.testing.assert_array_equal(g.ev_n, g.naccept)
+    for i in (0, 150, N - 1):
+        m = int(g.ev_n[i])
+        assert (np.diff(g.ev_t[i, :m]) > 0).all() and g.ev_t[i, m - 1] == 2.0
+        np.testing.assert_array_equal(g.ev_u[i, m - 1], g.u_final[:, i])
+        sol = g.solution(i)
+        np.testing.assert_array_equal(sol.t, np.concatenate([[0.0], g.ev_t[i, :m]]))
+    sa = np.array([0.7, 2.0])
+    g2 = run(lorenz, w["u0"], w["p"], 0.0, 2.0, saveat=sa, **kw)
+    c2 = cpu(oracle, "user", w["u0"], w["p"], 0.0, 2.0, saveat=sa, event_cap=512,
+             abstol=1e-8, reltol=1e-8)
+    np.testing.assert_array_equal(g2.ev_n, c2["ev_n"])
+    np.testing.assert_array_equal(g2.ev_t, c2["ev_t"])
+    np.testing.assert_array_equal(g2.us, c2["saveat"])
+    np.testing.assert_array_equal(g2.ev_n, g2.naccept - 1)
+    sol = g2.solution(3)
+    assert sol.t[0] == 0.0 and sol.t[-1] == 2.0 and len(sol.t) == int(g2.naccept[3]) + 2
+
+
+def test_save_everystep_with_callbacks_and_saveat(oracle):
+    """save_everystep with a bouncing ball's callback (save_positions true,
+    true) and saveat: a bounce step saves its end once before the affect and
+    once after it, so each trajectory keeps naccept + nevent points, less the
+    last step's (its time tf is a saveat point); all bit-exact against the
+    oracle."""
+    N = 256
+    say(f"save_everystep bouncing ball N={N}")
+    cb = ContinuousCallback("u[0]", affect="u[1] = -p[1] * u[1];", direction=-1)
+    rng = np.random.default_rng(117)
+    u0 = np.vstack([rng.uniform(5.0, 20.0, N), np.zeros(N)])
+    p = np.vstack([np.full(N, 9.81), rng.uniform(0.8, 0.95, N)])
+    sa = np.linspace(0.0, 6.0, 7)
+    kw = dict(abstol=1e-10, reltol=1e-10, saveat=sa, event_cap=1024)
+    model = UserModel(2, 2, BALL)
+    g = run(model, u0, p, 0.0, 6.0, callback=cb, save_everystep=True, **kw)
+    umh.install(oracle, 2, 2, BALL, callbacks=(cb,), save_everystep=True)
+    c = cpu(oracle, "user", u0, p, 0.0, 6.0, **kw)
+    _check(g, c)
+    np.testing.assert_array_equal(g.ev_n, c["ev_n"])
+    np.testing.assert_array_equal(g.ev_t, c["ev_t"])
+    np.testing.assert_array_equal(g.ev_u, c["ev_u"])
+    assert (g.nevent > 0).all()
+    # the last step ends at tf = 6, a saveat point: its end is not saved again
+    np.testing.assert_array_equal(g.ev_n, g.naccept + g.nevent - 1)

@@ -286,3 +286,43 @@ def test_oracle_event_points_bouncing_ball(oracle, saves, per_event):
     np.testing.assert_array_equal(small["ev_n"], r["ev_n"])
     np.testing.assert_array_equal(small["ev_t"], r["ev_t"][:, :2])
     np.testing.assert_array_equal(small["ev_u"], r["ev_u"][:, :2])
+
+
+def test_oracle_save_everystep_closed_form(oracle):
+    """save_everystep on u' = -k u: one point per accepted step, strictly
+    increasing times ending at tf, states on exp(-k t) to the tolerance; with
+    saveat holding tf the last step's end is not saved a second time, and a
+    callback step saves its end once (the point before the affect is that
+    one) plus the point after the affect."""
+    from mideq import DiscreteCallback
+    rhs = "du[0] = -p[0] * u[0];"
+    umh.install(oracle, 1, 1, rhs, save_everystep=True)
+    k = np.array([0.5, 1.0, 2.0, 4.0])
+    u0, p = np.ones((1, 4)), k[None, :]
+    r = oracle.ensemble("user", u0, p, 0.0, 2.0, abstol=1e-11, reltol=1e-11, event_cap=256)
+    assert (r["retcode"] == 1).all()
+    np.testing.assert_array_equal(r["ev_n"], r["naccept"])
+    for i in range(4):
+        m = int(r["ev_n"][i])
+        t = r["ev_t"][i, :m]
+        assert (np.diff(t) > 0).all() and t[-1] == 2.0 and np.isnan(r["ev_t"][i, m:]).all()
+        np.testing.assert_allclose(r["ev_u"][i, :m, 0], np.exp(-k[i] * t), rtol=1e-9)
+        assert r["ev_u"][i, m - 1, 0] == r["u"][0, i]
+    r2 = oracle.ensemble("user", u0, p, 0.0, 2.0, abstol=1e-11, reltol=1e-11, event_cap=256,
+                         saveat=np.array([1.0, 2.0]))
+    np.testing.assert_array_equal(r2["ev_n"], r2["naccept"] - 1)
+    np.testing.assert_array_equal(r2["ev_t"][:, :int(r2["ev_n"][0])],
+                                  r["ev_t"][:, :int(r2["ev_n"][0])])
+    # a discrete callback halving u at every step end past t = 1: each such
+    # step keeps its end before the affect and the state after it
+    cb = DiscreteCallback("t > 1.0", affect="u[0] = 0.5 * u[0];")
+    umh.install(oracle, 1, 1, rhs, callbacks=(cb,), save_everystep=True)
+    r3 = oracle.ensemble("user", u0, p, 0.0, 2.0, abstol=1e-11, reltol=1e-11, event_cap=512)
+    np.testing.assert_array_equal(r3["ev_n"], r3["naccept"] + r3["nevent"])
+    for i in range(4):
+        m = int(r3["ev_n"][i])
+        t, u = r3["ev_t"][i, :m], r3["ev_u"][i, :m, 0]
+        late = np.nonzero(t > 1.0)[0]
+        # past t = 1 the points come in pairs (before, after) at one time
+        np.testing.assert_array_equal(t[late[0::2]], t[late[1::2]])
+        np.testing.assert_array_equal(u[late[1::2]], 0.5 * u[late[0::2]])

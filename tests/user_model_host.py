@@ -109,7 +109,7 @@ def build(n, np_, rhs, jac=None, callbacks=()):
     return lib
 
 
-def install(oracle, n, np_, rhs, jac=None, callbacks=()):
+def install(oracle, n, np_, rhs, jac=None, callbacks=(), save_everystep=False):
     """Make the oracle's "user" model this one (callbacks: the mideq
     ContinuousCallback / DiscreteCallback objects)."""
     lib = build(n, np_, rhs, jac, callbacks)
@@ -136,4 +136,6 @@ def install(oracle, n, np_, rhs, jac=None, callbacks=()):
     L.oracle_set_user_callback_saves.argtypes = [C.c_int32, C.c_void_p]
     rc = L.oracle_set_user_callback_saves(k, arr(saves))
     assert rc == 0
+    L.oracle_set_user_everystep.argtypes = [C.c_int32]
+    assert L.oracle_set_user_everystep(int(bool(save_everystep))) == 0
     return lib
