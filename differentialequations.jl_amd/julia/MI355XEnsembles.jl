@@ -92,6 +92,8 @@ end
 
 # registered models whose callbacks save points at events (save_positions)
 const EVENT_SAVES = IdDict{Any, Bool}()
+# f => compiled with save_everystep (every accepted step's end point kept)
+const EVERYSTEP = IdDict{Any, Bool}()
 
 struct MideqUserModel
     n::Int32; np::Int32; rhs::Ptr{UInt8}; jac::Ptr{UInt8}
@@ -123,13 +125,17 @@ DeviceCallback(kind::Symbol, condition::AbstractString; kw...) =
     DeviceCallback(; kind, condition = String(condition), kw...)
 
 """
-    register_model!(f!, n, np, rhs; jac = nothing, device = 0)
+    register_model!(f!, n, np, rhs; jac = nothing, callbacks = [], output = nothing,
+                    nout = 0, save_everystep = false, device = 0)
 
 Registers `f!` with a device right-hand side written as C++ statements for
 u' = f(u, p, t): assign `du[0..n-1]` from `u[i]`, `p[j]` (0-based) and the
 time `t` (naming `t` makes the model non-autonomous); the number type is `T`, constants `2.0` or `T(2)`, functions
 `sqrt exp log sin cos tanh fabs pow(x, c)`.  `jac` assigns `J[i][j]`;
 without it the Rosenbrock methods differentiate `rhs` in forward mode.
+`save_everystep = true` compiles in upstream's save_everystep: each
+solution then holds every accepted step's end (up to `event_cap` per
+trajectory).
 The model id belongs to `device`'s context.  Example (test/core.jl:22-27):
 
     register_model!(lorenz!, 3, 3, """
@@ -141,7 +147,7 @@ function register_model!(f, n::Integer, np::Integer, rhs::AbstractString;
                          jac::Union{Nothing, AbstractString} = nothing,
                          callbacks::AbstractVector{DeviceCallback} = DeviceCallback[],
                          output::Union{Nothing, AbstractString} = nothing, nout::Integer = 0,
-                         device::Int = 0)
+                         save_everystep::Bool = false, device::Int = 0)
     ctx = context(device)
     id = Ref{Int32}(0)
     r = String(rhs)
@@ -157,7 +163,8 @@ function register_model!(f, n::Integer, np::Integer, rhs::AbstractString;
                for (k, cb) in enumerate(callbacks)]
         GC.@preserve cbs begin
             m = MideqUserModel(n, np, pointer(r), j === nothing ? C_NULL : pointer(j),
-                               length(cbs), 0, isempty(cbs) ? C_NULL : pointer(cbs),
+                               length(cbs), Int32(save_everystep),
+                               isempty(cbs) ? C_NULL : pointer(cbs),
                                o === nothing ? 0 : nout, 0, o === nothing ? C_NULL : pointer(o))
             ccall((:mideq_model_register, LIB[]), Cint,
                   (Ptr{Cvoid}, Ref{MideqUserModel}, Ref{Int32}), ctx.h, m, id)
@@ -165,7 +172,9 @@ function register_model!(f, n::Integer, np::Integer, rhs::AbstractString;
     end
     check(ctx, rc, "mideq_model_register")
     MODEL_FOR[f] = id[]
-    EVENT_SAVES[f] = any(cb -> save_bits(cb) != 0, callbacks)
+    # points beside saveat (events, every step) come back in the ev arrays
+    EVENT_SAVES[f] = save_everystep || any(cb -> save_bits(cb) != 0, callbacks)
+    EVERYSTEP[f] = save_everystep
     return f
 end
 
@@ -256,7 +265,7 @@ struct MideqStats
 end
 
 function _device_solve(probs, alg, ealg::EnsembleMI355X, f, sa; abstol, reltol, dt, dtmin,
-                       dtmax, maxiters, lanes = 0, event_cap = 16)
+                       dtmax, maxiters, lanes = 0, event_cap = 0)
     M = length(probs)
     n = maximum(length(pr.u0) for pr in probs)
     np = max(length(probs[1].p), 1)   # a user model without parameters reads one row
@@ -276,6 +285,10 @@ function _device_solve(probs, alg, ealg::EnsembleMI355X, f, sa; abstol, reltol, 
     # points saved at callback events: the ABI's [ntraj][cap] times and
     # [ntraj][cap][n] states are (cap, M) and (n, cap, M) column-major
     ev = get(EVENT_SAVES, f, false)
+    # points per trajectory beside saveat: 16 for events; with save_everystep
+    # what ~1 GiB holds over the batch (16..65,536); event_cap > 0 sets it
+    event_cap = event_cap > 0 ? event_cap :
+                get(EVERYSTEP, f, false) ? clamp((1 << 27) ÷ (M * (n + 1)), 16, 65536) : 16
     EVT = ev ? fill(NaN, event_cap, M) : zeros(Float64, 0)
     EVU = ev ? fill(NaN, n, event_cap, M) : zeros(Float64, 0)
     EVN = ev ? zeros(Int32, M) : zeros(Int32, 0)
@@ -405,9 +418,17 @@ function _solve_batch(ens, alg, ealg::EnsembleMI355X, f, II; saveat, abstol, rel
                     perm = sortperm(back ? -allt : allt; alg = Base.Sort.DEFAULT_STABLE)
                     ts = allt[perm]; us = allu[perm]
                 end
+                # save_everystep saves the start (save_start) when saveat lacks it
+                if get(EVERYSTEP, f, false) && (isempty(ts) || ts[1] != pr.tspan[1])
+                    ts = vcat([pr.tspan[1]], ts); us = vcat([val([pr.u0;])], us)
+                end
             else
-                ts = vcat([pr.tspan[1]], et, [r.TFo[k]])
-                us = vcat([val([pr.u0;])], eu, [val(r.UF[k, 1:nk])])
+                ts = vcat([pr.tspan[1]], et); us = vcat([val([pr.u0;])], eu)
+                # the postamble's final point, unless the last point saved has its time
+                cut = r.ev && Int(r.EVN[k]) > size(r.EVT, 1)
+                if m == 0 || et[end] != r.TFo[k] || cut
+                    ts = vcat(ts, [r.TFo[k]]); us = vcat(us, [val(r.UF[k, 1:nk])])
+                end
             end
             if dense
                 lanes = r.lanes
