@@ -282,3 +282,30 @@ def test_save_everystep_with_callbacks_and_saveat(oracle):
     assert (g.nevent > 0).all()
     # the last step ends at tf = 6, a saveat point: its end is not saved again
     np.testing.assert_array_equal(g.ev_n, g.naccept + g.nevent - 1)
+
+
+@pytest.mark.parametrize("alg,precision", [("rodas5", "fp64"), ("auto", "mixed")])
+def test_save_everystep_rosenbrock(oracle, alg, precision):
+    """save_everystep through Rosenbrock steps (Robertson, the built-in
+    model's source): step points bit-exact against the oracle, one per
+    accepted step, Σy = 1 on every one of them (to 1e-12; 1e-5 with FP32
+    stages)."""
+    N = 256
+    say(f"save_everystep robertson {alg} {precision} N={N}")
+    w = workloads.robertson(N, seed=118)
+    kw = dict(abstol=1e-8, reltol=1e-6, save_everystep=True, event_cap=1024)
+    src = BUILTIN_SOURCE["robertson"]
+    umh.install(oracle, 3, 3, src[2], jac=src[3], save_everystep=True)
+    g = run(robertson, w["u0"], w["p"], 0.0, 1e3, alg=alg, precision=precision, **kw)
+    c = cpu(oracle, "user", w["u0"], w["p"], 0.0, 1e3, alg=alg, precision=precision,
+            event_cap=1024, abstol=1e-8, reltol=1e-6)
+    assert (c["retcode"] == 1).all() and (g.ev_n <= 1024).all()
+    assert_identical(g, c, save=False)
+    np.testing.assert_array_equal(g.ev_n, c["ev_n"])
+    np.testing.assert_array_equal(g.ev_t, c["ev_t"])
+    np.testing.assert_array_equal(g.ev_u, c["ev_u"])
+    np.testing.assert_array_equal(g.ev_n, g.naccept)
+    stored = ~np.isnan(g.ev_t)
+    # FP32 stages: the sum drifts by ~2e-6 (the oracle's mixed mode too)
+    np.testing.assert_allclose(g.ev_u.sum(axis=2)[stored], 1.0,
+                               atol=1e-5 if precision == "mixed" else 1e-12)
