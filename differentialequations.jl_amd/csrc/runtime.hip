@@ -506,7 +506,13 @@ static std::string user_model_source(const mideq_user_model* m) {
     s += table("cb_kind", [](const mideq_callback& c) { return (int)c.kind; });
     s += table("cb_dir", [](const mideq_callback& c) { return (int)c.direction; });
     s += table("cb_term", [](const mideq_callback& c) { return (int)c.terminate; });
-    s += table("cb_save", [](const mideq_callback& c) { return (int)c.save_positions; });
+    // a continuous callback without an affect (and not terminating: then
+    // terminate! is upstream's affect) leaves u unmodified: upstream's
+    // apply_callback! returns before its save_positions[2] point
+    s += table("cb_save", [](const mideq_callback& c) {
+      return (int)(c.kind == 0 && !c.affect && !c.terminate ? (c.save_positions & 1)
+                                                            : c.save_positions);
+    });
     s += table("cb_interp", [](const mideq_callback& c) {
       return c.interp_points > 0 ? (int)c.interp_points : 1;  // 0: the step end only
     });

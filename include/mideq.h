@@ -291,7 +291,10 @@ typedef struct {
                             dense-output evaluation: 10 make a Lorenz Tsit5 step ~1.7x
                             as expensive */
   const char *condition; /* double expression of u[], p[], t */
-  const char *affect;    /* statements changing u[] (double) from u, p, t; NULL: none */
+  const char *affect;    /* statements changing u[] (double) from u, p, t; NULL: none (a
+                            continuous callback without one that does not terminate
+                            saves no point after it: u is unmodified, as upstream's
+                            affect! = nothing) */
   int32_t save_positions; /* bit 0: save (t, u) before the affect, bit 1: after it
                              (upstream's (true, true) is 3); needs mideq_outputs.ev_t */
   int32_t reserved;

@@ -337,6 +337,7 @@ static struct {
   int kind[8], dir[8], term[8], ip[8];
   int save[8]; /* save_positions bits: 1 before the affect, 2 after */
   int everystep; /* save_everystep: every accepted step's end point saved */
+  int has_affect[8]; /* 0: a continuous callback without an affect (u unmodified) */
   oracle_user_cond cond;
   oracle_user_affect affect;
 } g_user;
@@ -355,7 +356,24 @@ int oracle_set_user_callbacks(int32_t ncb, const int32_t *kind, const int32_t *d
   g_user.cond = cond;
   g_user.affect = affect;
   for (int k = 0; k < 8; k++) g_user.save[k] = 0;
+  for (int k = 0; k < 8; k++) g_user.has_affect[k] = 1;
   return 0;
+}
+
+int oracle_set_user_callback_affects(int32_t ncb, const int32_t *has_affect) {
+  if (ncb != g_user.ncb) return -1;
+  for (int k = 0; k < ncb; k++) g_user.has_affect[k] = has_affect[k] != 0;
+  return 0;
+}
+
+/* save_positions of callback k as applied: a continuous callback without an
+ * affect (and not terminating -- terminate! is then upstream's affect)
+ * leaves u unmodified, and DiffEqBase's apply_callback! then returns before
+ * the save_positions[2] point (discrete callbacks save it always). */
+static int cb_saves(int k) {
+  return (g_user.kind[k] == 0 && !g_user.has_affect[k] && !g_user.term[k])
+             ? (g_user.save[k] & 1)
+             : g_user.save[k];
 }
 
 int oracle_set_user_everystep(int32_t on) {
@@ -1554,10 +1572,11 @@ static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
           /* each applied callback: savevalues! of the step end, then the
            * point before the affect unless that call saved one at t */
           int se = step_point(o, idx, n, t, uprev, everystep, &exact, &tlast, res);
-          if ((g_user.save[ev] & 1) && !se) save_event(o, idx, n, t, uprev, res);
+          const int sv = cb_saves(ev);
+          if ((sv & 1) && !se) save_event(o, idx, n, t, uprev, res);
           g_user.affect(ev, uprev, m->p, t);
-          if (g_user.save[ev] & 2) save_event(o, idx, n, t, uprev, res);
-          if (g_user.save[ev]) tlast = t;
+          if (sv & 2) save_event(o, idx, n, t, uprev, res);
+          if (sv) tlast = t;
           fired = 1;
           term |= g_user.term[ev];
           res->nevent++;
@@ -1565,10 +1584,11 @@ static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
         for (int k = 0; k < ncb; k++) {
           if (g_user.kind[k] == 1 && g_user.cond(k, uprev, m->p, t) != 0.0) {
             int se = step_point(o, idx, n, t, uprev, everystep, &exact, &tlast, res);
-            if ((g_user.save[k] & 1) && !se) save_event(o, idx, n, t, uprev, res);
+            const int sv = cb_saves(k);
+            if ((sv & 1) && !se) save_event(o, idx, n, t, uprev, res);
             g_user.affect(k, uprev, m->p, t);
-            if (g_user.save[k] & 2) save_event(o, idx, n, t, uprev, res);
-            if (g_user.save[k]) tlast = t;
+            if (sv & 2) save_event(o, idx, n, t, uprev, res);
+            if (sv) tlast = t;
             fired = 1;
             term |= g_user.term[k];
             res->nevent++;

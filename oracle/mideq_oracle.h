@@ -81,6 +81,9 @@ int oracle_set_user_callback_saves(int32_t ncb, const int32_t *save_positions);
  * every accepted step's end point is saved into ev_t / ev_u beside the
  * callbacks' points.  Returns 0, or -1 for another value. */
 int oracle_set_user_everystep(int32_t on);
+/* Which callbacks have an affect (0: a continuous callback with none, which
+ * saves no point after it); all have one until set.  -1 if ncb differs. */
+int oracle_set_user_callback_affects(int32_t ncb, const int32_t *has_affect);
 
 enum {
   ORACLE_ALG_TSIT5 = 0,

@@ -34,7 +34,9 @@ def _check(g, c):
 
 def test_core_jl_callbacks_on_lorenz(oracle):
     """test/core.jl:60-79 on a 2,000-trajectory Lorenz ensemble (the built-in
-    model with the callbacks compiled in)."""
+    model with the callbacks compiled in).  The continuous callback has no
+    affect, so of its default save_positions (true, true) only the point
+    before it is saved (u is unmodified: upstream saves nothing after)."""
     N = 2000
     say(f"core.jl callbacks lorenz N={N}")
     w = workloads.lorenz(N, seed=111)
@@ -44,7 +46,11 @@ def test_core_jl_callbacks_on_lorenz(oracle):
     assert (g.retcode == 1).all() and (g.nevent == 1).all()
     assert np.abs(g.t_event - 0.5).max() < 1e-12
     umh.install(oracle, 3, 3, BUILTIN_SOURCE["lorenz"][2], callbacks=(cb,))
-    _check(g, cpu(oracle, "user", w["u0"], w["p"], 0.0, 1.0, **kw))
+    c = cpu(oracle, "user", w["u0"], w["p"], 0.0, 1.0, event_cap=16, **kw)
+    _check(g, c)
+    assert (g.ev_n == 1).all() and (g.ev_t[:, 0] == g.t_event).all()
+    np.testing.assert_array_equal(g.ev_n, c["ev_n"])
+    np.testing.assert_array_equal(g.ev_u, c["ev_u"])
     dcb = DiscreteCallback("t >= 0.5")
     g2 = run(lorenz, w["u0"], w["p"], 0.0, 1.0, callback=dcb, **kw)
     assert (g2.retcode == 1).all() and (g2.nevent >= 1).all() and (g2.t_event == 1.0).all()

@@ -326,3 +326,22 @@ def test_oracle_save_everystep_closed_form(oracle):
         # past t = 1 the points come in pairs (before, after) at one time
         np.testing.assert_array_equal(t[late[0::2]], t[late[1::2]])
         np.testing.assert_array_equal(u[late[1::2]], 0.5 * u[late[0::2]])
+
+
+def test_oracle_continuous_callback_without_affect_saves_before_only(oracle):
+    """A continuous callback with no affect (upstream's affect! = nothing)
+    leaves u unmodified, so of save_positions (true, true) only the point
+    before it is saved; a discrete one saves both (its affect always runs)."""
+    from mideq import ContinuousCallback, DiscreteCallback
+    rhs = "du[0] = -p[0] * u[0];"
+    u0, p = np.ones((1, 3)), np.array([[0.5, 1.0, 2.0]])
+    cb = ContinuousCallback("t - 0.5")
+    umh.install(oracle, 1, 1, rhs, callbacks=(cb,))
+    r = oracle.ensemble("user", u0, p, 0.0, 1.0, abstol=1e-10, reltol=1e-10, event_cap=8)
+    np.testing.assert_array_equal(r["ev_n"], [1, 1, 1])
+    np.testing.assert_allclose(r["ev_t"][:, 0], 0.5, atol=1e-12)
+    np.testing.assert_allclose(r["ev_u"][:, 0, 0], np.exp(-0.5 * p[0]), rtol=1e-8)
+    dcb = DiscreteCallback("t >= 0.9")
+    umh.install(oracle, 1, 1, rhs, callbacks=(dcb,))
+    r2 = oracle.ensemble("user", u0, p, 0.0, 1.0, abstol=1e-10, reltol=1e-10, event_cap=64)
+    np.testing.assert_array_equal(r2["ev_n"], 2 * r2["nevent"])

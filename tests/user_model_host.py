@@ -136,6 +136,9 @@ def install(oracle, n, np_, rhs, jac=None, callbacks=(), save_everystep=False):
     L.oracle_set_user_callback_saves.argtypes = [C.c_int32, C.c_void_p]
     rc = L.oracle_set_user_callback_saves(k, arr(saves))
     assert rc == 0
+    L.oracle_set_user_callback_affects.argtypes = [C.c_int32, C.c_void_p]
+    assert L.oracle_set_user_callback_affects(
+        k, arr([int(getattr(cb, "affect", None) is not None) for cb in callbacks])) == 0
     L.oracle_set_user_everystep.argtypes = [C.c_int32]
     assert L.oracle_set_user_everystep(int(bool(save_everystep))) == 0
     return lib
