@@ -478,6 +478,8 @@ class EnsembleArrays:
     ev_n: Optional[np.ndarray] = None     # [N] points produced (may exceed cap)
     resident: bool = False                # the arrays are torch tensors on the GPU
     save_everystep: bool = False          # ev_* also hold every accepted step's end point
+    drop_start: bool = False              # save_start = false: no (t0, u0) point
+    drop_end: bool = False                # save_end = false: no final point appended
     keep: Optional[list] = field(default=None, repr=False, compare=False)  # buffers in use
     _meta: Optional[dict] = field(default=None, repr=False, compare=False)
 
@@ -549,8 +551,10 @@ class EnsembleArrays:
         ts = np.concatenate([[float(h("t0")[i])], et])
         us = [own(self.u0[:m, i])] + [eu[k, :m] for k in range(len(et))]
         cut = self.ev_t is not None and int(self._h("ev_n")[i]) > self.ev_t.shape[1]
-        if len(et) == 0 or et[-1] != tf or cut:
+        if (len(et) == 0 or et[-1] != tf or cut) and not self.drop_end:
             ts, us = np.concatenate([ts, [tf]]), us + [own(self.u_final[:m, i])]
+        if self.drop_start:
+            ts, us = ts[1:], us[1:]
         return ODESolution(ts, us, rc, st, prob)
 
 
@@ -677,20 +681,31 @@ def julia_range(start, step, stop):
     return np.array([float(fs + k * fd) for k in range(ln)])
 
 
-def _saveat_vector(saveat, tspan):
+def _saveat_vector(saveat, tspan, save_start=None, save_end=None):
+    """The saveat points handed to the kernels, ascending.  save_start /
+    save_end (None: upstream's defaults -- true for a scalar saveat, else
+    whether t0 / tf is among the points) decide whether t0 and tf are saved,
+    as OrdinaryDiffEq keeps only the points strictly inside the span in its
+    queue and saves the ends by these flags."""
     if saveat is None:
         return None
+    t0, t1 = float(tspan[0]), float(tspan[1])
     if np.isscalar(saveat):
         # the range t0:dir*d:tf, then tf itself when the range stops short of
         # it (save_end), in the direction of integration; handed over in
         # ascending order (a backward solve fills them from the end)
-        t0, t1 = float(tspan[0]), float(tspan[1])
         sgn = 1.0 if t1 >= t0 else -1.0
         pts = julia_range(t0, sgn * abs(float(saveat)), t1)
         if len(pts) == 0 or pts[-1] != t1:
             pts = np.append(pts, t1)
-        return np.ascontiguousarray(np.sort(pts), dtype=np.float64)
-    return np.ascontiguousarray(np.sort(np.asarray(saveat, dtype=np.float64)))
+    else:
+        pts = np.asarray(saveat, dtype=np.float64).ravel()
+    for end, flag in ((t0, save_start), (t1, save_end)):
+        if flag is not None:
+            pts = pts[pts != end]
+            if flag:
+                pts = np.append(pts, end)
+    return np.ascontiguousarray(np.sort(pts), dtype=np.float64)
 
 
 def _ptr(a):
@@ -990,6 +1005,10 @@ def solve(prob, alg, ensemblealg=None, *, trajectories=None, saveat=None, abstol
     """solve(EnsembleProblem, alg, EnsembleMI355X(); trajectories, ...)
     (qa.jl:192) or solve(ODEProblem, alg) for a single trajectory.
 
+    save_start / save_end (upstream's flags): whether t0 and tf are saved
+    (defaults as upstream: with a scalar saveat or none, both; with a vector,
+    whether it holds them).
+
     save_everystep=True keeps every accepted step's end point in each
     solution (upstream's default without saveat; here opt-in, since it
     compiles the model with the step saves; sol(t) interpolates without it).
@@ -1020,7 +1039,8 @@ def solve(prob, alg, ensemblealg=None, *, trajectories=None, saveat=None, abstol
     if batch_size < 1 or N // batch_size < 1:
         raise ValueError("trajectories // batch_size cannot be less than 1")
     idx = np.arange(N)
-    sa = _saveat_vector(saveat, prob.prob.tspan)
+    save_start, save_end = kw.pop("save_start", None), kw.pop("save_end", None)
+    sa = _saveat_vector(saveat, prob.prob.tspan, save_start, save_end)
     # dense output (sol(t)): on by default without saveat, as upstream
     dense = kw.pop("dense", None)
     dense = saveat is None if dense is None else bool(dense)
@@ -1054,6 +1074,9 @@ def solve(prob, alg, ensemblealg=None, *, trajectories=None, saveat=None, abstol
             arr = device_solve(f, *_resident_inputs(ealg, u0, p, t0, tf, nvec), alg, ealg,
                                saveat=sa, abstol=abstol, reltol=reltol, dt=dt, dtmin=dtmin,
                                dtmax=dtmax, maxiters=maxiters, **kw)
+            if sa is None:
+                # without saveat: (t0, u0) and the final state by the flags
+                arr.drop_start, arr.drop_end = save_start is False, save_end is False
             if first_arrays is None:
                 first_arrays = arr
             rerun = []

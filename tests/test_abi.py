@@ -162,3 +162,22 @@ def test_device_buffers_dtypes_and_shapes():
     assert m.arr(y, np.float64).data_ptr() == y.data_ptr()  # already fits: in place
     assert m.arr(np.array([1, 2]), np.int8).dtype == torch.int8
     assert len(m.keep) == 15
+
+
+def test_saveat_save_start_save_end_flags():
+    """save_start / save_end as OrdinaryDiffEq applies them: only the points
+    strictly inside the span come from saveat; t0 and tf are saved by the
+    flags, whose defaults are true for a scalar saveat and membership for a
+    vector."""
+    from mideq.scimlbase import _saveat_vector
+    np.testing.assert_array_equal(_saveat_vector(0.25, (0.0, 1.0), save_start=False),
+                                  [0.25, 0.5, 0.75, 1.0])
+    np.testing.assert_array_equal(_saveat_vector(0.25, (0.0, 1.0), save_end=False),
+                                  [0.0, 0.25, 0.5, 0.75])
+    np.testing.assert_array_equal(_saveat_vector([0.5, 0.2], (0.0, 1.0)), [0.2, 0.5])
+    np.testing.assert_array_equal(_saveat_vector([0.5, 0.2], (0.0, 1.0), save_start=True,
+                                                 save_end=True), [0.0, 0.2, 0.5, 1.0])
+    np.testing.assert_array_equal(_saveat_vector([0.0, 0.5, 1.0], (0.0, 1.0), save_start=False),
+                                  [0.5, 1.0])
+    # backward span: the ends are t0 = 1 and tf = 0
+    np.testing.assert_array_equal(_saveat_vector(0.5, (1.0, 0.0), save_end=False), [0.5, 1.0])

@@ -263,3 +263,23 @@ def test_core_save_everystep_single_problem(oracle):
     m = int(r["ev_n"][0])
     np.testing.assert_array_equal(sol.t[1:], r["ev_t"][0, :m])
     np.testing.assert_array_equal(np.array([v[0] for v in sol.u[1:]]), r["ev_u"][0, :m, 0])
+
+
+def test_save_start_and_save_end():
+    """save_start / save_end (OrdinaryDiffEq's flags) on core.jl's saveat
+    problem: saveat = 0.1 without the start keeps 10 points from 0.1; without
+    saveat, save_start = false leaves the final point only and save_end =
+    false the initial one."""
+    say("save_start / save_end")
+    f = mideq.UserModel(1, 1, "du[0] = p[0] * u[0];")
+    prob = ODEProblem(f, [0.5], (0.0, 1.0), [1.01])
+    full = solve(prob, Tsit5(), saveat=0.1, abstol=1e-8, reltol=1e-8)
+    sol = solve(prob, Tsit5(), saveat=0.1, save_start=False, abstol=1e-8, reltol=1e-8)
+    assert len(sol.t) == 10 and sol.t[0] == full.t[1] and sol.t[-1] == 1.0
+    np.testing.assert_array_equal(np.array(sol.u), np.array(full.u[1:]))
+    sol = solve(prob, Tsit5(), saveat=0.1, save_end=False, abstol=1e-8, reltol=1e-8)
+    assert len(sol.t) == 10 and sol.t[-1] == full.t[-2]
+    s1 = solve(prob, Tsit5(), save_start=False, abstol=1e-8, reltol=1e-8)
+    s2 = solve(prob, Tsit5(), save_end=False, abstol=1e-8, reltol=1e-8)
+    assert list(s1.t) == [1.0] and list(s2.t) == [0.0] and s2.u[0][0] == 0.5
+    assert s1.u[0][0] == full.u[-1][0]
