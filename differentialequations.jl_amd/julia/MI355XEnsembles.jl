@@ -213,22 +213,32 @@ function check(ctx, rc, what)
     error("$what failed ($rc): $msg")
 end
 
-function _saveat_vector(saveat, tspan)
+# save_start / save_end (nothing: upstream's defaults -- true for a scalar
+# saveat, else whether t0 / tf is among the points): OrdinaryDiffEq keeps the
+# points strictly inside the span and saves the ends by these flags.
+function _saveat_vector(saveat, tspan, save_start = nothing, save_end = nothing)
     saveat === nothing && return Float64[]
     if saveat isa Number
         # in the direction of integration, handed over ascending
         dir = tspan[2] >= tspan[1] ? 1 : -1
-        pts = collect(tspan[1]:(dir * saveat):tspan[2])
+        pts = Float64.(collect(tspan[1]:(dir * saveat):tspan[2]))
         (last(pts) - tspan[2]) * dir < 0 && push!(pts, tspan[2])
-        return sort(Float64.(pts))
+    else
+        pts = Float64.(collect(saveat))
     end
-    return sort(Float64.(collect(saveat)))
+    for (e, flag) in ((Float64(tspan[1]), save_start), (Float64(tspan[2]), save_end))
+        flag === nothing && continue
+        filter!(x -> x != e, pts)
+        flag && push!(pts, e)
+    end
+    return sort(pts)
 end
 
 function __solve(ens::SciMLBase.AbstractEnsembleProblem, alg, ealg::EnsembleMI355X;
                  trajectories, batch_size = trajectories, saveat = nothing, abstol = 1e-6,
                  reltol = 1e-3, dt = 0.0, dtmin = 0.0, dtmax = 0.0, maxiters = 100_000,
-                 dense = saveat === nothing, kwargs...)
+                 dense = saveat === nothing, save_start = nothing, save_end = nothing,
+                 kwargs...)
     f = ens.prob.f.f
     if !haskey(MODEL_FOR, f)
         @warn "EnsembleMI355X: unregistered right-hand side, falling back to EnsembleThreads"
@@ -245,7 +255,7 @@ function __solve(ens::SciMLBase.AbstractEnsembleProblem, alg, ealg::EnsembleMI35
     for first in 1:batch_size:N
         II = first:min(first + batch_size - 1, N)
         batch = _solve_batch(ens, alg, ealg, f, II; saveat, abstol, reltol, dt, dtmin,
-                             dtmax, maxiters, dense)
+                             dtmax, maxiters, dense, save_start, save_end)
         u, converged = ens.reduction(u, batch, II)
         converged && break
     end
@@ -388,11 +398,12 @@ end
 # with its rerun loop; returns their outputs in index order.  Without saveat
 # every solution has dense output (sol(t)), as upstream's default.
 function _solve_batch(ens, alg, ealg::EnsembleMI355X, f, II; saveat, abstol, reltol, dt,
-                      dtmin, dtmax, maxiters, dense = saveat === nothing)
+                      dtmin, dtmax, maxiters, dense = saveat === nothing,
+                      save_start = nothing, save_end = nothing)
     outputs = Dict{Int,Any}()
     pending = collect(II)
     repeat = 1
-    sa = _saveat_vector(saveat, ens.prob.tspan)
+    sa = _saveat_vector(saveat, ens.prob.tspan, save_start, save_end)
     ns = length(sa)
     tol = (; abstol, reltol, dt, dtmin, dtmax, maxiters)
     while !isempty(pending)
@@ -426,8 +437,11 @@ function _solve_batch(ens, alg, ealg::EnsembleMI355X, f, II; saveat, abstol, rel
                 ts = vcat([pr.tspan[1]], et); us = vcat([val([pr.u0;])], eu)
                 # the postamble's final point, unless the last point saved has its time
                 cut = r.ev && Int(r.EVN[k]) > size(r.EVT, 1)
-                if m == 0 || et[end] != r.TFo[k] || cut
+                if (m == 0 || et[end] != r.TFo[k] || cut) && save_end !== false
                     ts = vcat(ts, [r.TFo[k]]); us = vcat(us, [val(r.UF[k, 1:nk])])
+                end
+                if save_start === false
+                    ts = ts[2:end]; us = us[2:end]
                 end
             end
             if dense
