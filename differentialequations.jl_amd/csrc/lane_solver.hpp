@@ -69,6 +69,9 @@ struct LaneArgs {
   int32_t* ev_n;              //   [ntraj] points produced (may exceed ev_cap)
   int32_t ev_cap;
   int32_t pad1;
+  const double* tstops;       // models compiled with tstops: [nts] ascending times every
+  int32_t nts;                //   trajectory's steps land on (NULL / 0: tf only)
+  int32_t pad2;
 };
 
 // Trajectory scheduled at position k of the launch order; -1 for an entry
@@ -303,12 +306,24 @@ struct every_step<M, decltype((void)M::EVERYSTEP, void())> {
   static constexpr bool value = M::EVERYSTEP;
 };
 
+// tstops of a model (M::TSTOPS = true): the steps land on the times of
+// LaneArgs.tstops inside the span, as on tf.
+template <class M, class = void>
+struct has_tstops {
+  static constexpr bool value = false;
+};
+template <class M>
+struct has_tstops<M, decltype((void)M::TSTOPS, void())> {
+  static constexpr bool value = M::TSTOPS;
+};
+
 template <class M, bool MIX, int ALG, bool FAST>
 struct Lane {
   static constexpr int N = M::N;
   static constexpr bool kTdep = time_dep<M>::value;
   static constexpr int kNcb = cb_count<M>::value;
   static constexpr bool kEveryStep = every_step<M>::value;
+  static constexpr bool kTstops = has_tstops<M>::value;
   // points saved beside saveat (at events, at every step): kept only by
   // the kernels of models that make them
   static constexpr bool kExtra = kNcb > 0 || kEveryStep;
@@ -349,6 +364,26 @@ struct Lane {
   double tevent;
   int32_t nevsave;  // points saved at events (and every step) so far
   double tlast;     // time of the last saved point (saveat or extra), t0 at the start
+  int32_t its;      // tstops: index of the next one ahead (-1 / nts: none)
+
+  // The first tstop ahead of t: the next of LaneArgs.tstops strictly inside
+  // (t, tf) in the direction of integration, else tf (upstream keeps tf in
+  // the same queue).
+  __device__ __forceinline__ double next_tstop(const LaneArgs& a) const {
+    if constexpr (kTstops) {
+      if (its >= 0 && its < a.nts) {
+        const double s = a.tstops[its];
+        if (tdir * (tf - s) > 0.0) return s;
+      }
+    }
+    return tf;
+  }
+  // handle_tstop!: drop the tstops at or behind t
+  __device__ __forceinline__ void pop_tstops(const LaneArgs& a) {
+    if constexpr (kTstops) {
+      while (its >= 0 && its < a.nts && tdir * (t - a.tstops[its]) >= 0.0) its += tdir > 0.0 ? 1 : -1;
+    }
+  }
   int32_t cap;  // TRUST pass: iteration cap of this trajectory
 
   // point k of this trajectory's event points: (t, v), stored while k < ev_cap
@@ -543,6 +578,11 @@ struct Lane {
     tevent = __builtin_nan("");
     nevsave = 0;
     tlast = t0;
+    if constexpr (kTstops) {
+      // the first tstop strictly after t0 (forward) or before it (backward)
+      its = tdir > 0.0 ? 0 : a.nts - 1;
+      pop_tstops(a);
+    }
     if constexpr (kNcb > 0) {
 #pragma unroll
       for (int k = 0; k < kNcb; k++) gprev[k] = M::cb_g(k, u, p, t0);
@@ -946,7 +986,12 @@ struct Lane {
       rc = RC_DTLESSTHANMIN;
       return;
     }
-    dt = fmin(dt, tdir * (tf - t));
+    pop_tstops(a);  // handle_tstop!: the ones the last step reached
+    [[maybe_unused]] const double tstop = next_tstop(a);
+    if constexpr (kTstops)
+      dt = fmin(dt, tdir * (tstop - t));  // modify_dt_for_tstops!
+    else
+      dt = fmin(dt, tdir * (tf - t));
     const double sdt = tdir * dt;  // the signed step
 
     double u1[N];
@@ -1027,8 +1072,14 @@ struct Lane {
         ttmp = t + y;
         tcomp = (ttmp - t) - y;
       }
-      const double snapw = back ? 100.0 * jl_eps(fmax(t, tf)) : tsnap;
-      if (fabs(ttmp - tf) < snapw) ttmp = tf;
+      if constexpr (kTstops) {
+        // fixed_t_for_floatingpoint_error!: onto the first tstop within
+        // 100 eps(max(t, tstop)) (tf included: then the window is the one below)
+        if (fabs(ttmp - tstop) < 100.0 * jl_eps(fmax(t, tstop))) ttmp = tstop;
+      } else {
+        const double snapw = back ? 100.0 * jl_eps(fmax(t, tf)) : tsnap;
+        if (fabs(ttmp - tf) < snapw) ttmp = tf;
+      }
       t = ttmp;
       // dense output of this step at th (time tprev + th dt): Tsit5's
       // interpolant or the Rosenbrock one (rd_H, FP64 in both precisions)

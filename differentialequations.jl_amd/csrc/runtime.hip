@@ -38,7 +38,8 @@ struct DevBuf {
 enum {
   BUF_U0 = 0, BUF_P, BUF_T0, BUF_TF, BUF_NVEC, BUF_SAVEAT, BUF_DT0, BUF_HINT, BUF_US,
   BUF_UF, BUF_TFIN, BUF_RC, BUF_NA, BUF_NR, BUF_NF, BUF_NS, BUF_ORDER,
-  BUF_DTF, BUF_NEV, BUF_TEV, BUF_OUT, BUF_RED, BUF_CAP, BUF_EVT, BUF_EVU, BUF_EVN, BUF_COUNT
+  BUF_DTF, BUF_NEV, BUF_TEV, BUF_OUT, BUF_RED, BUF_CAP, BUF_EVT, BUF_EVU, BUF_EVN, BUF_TST,
+  BUF_COUNT
 };
 
 // Device memory a solve uses beyond its inputs and outputs: the work-queue
@@ -58,6 +59,7 @@ struct UserModel {
   int ncb = 0;   // callbacks compiled into the model
   int nout = 0;  // output_func values per trajectory
   bool everystep = false;  // save_everystep compiled into the model
+  bool tstops = false;     // compiled to honour mideq_options.tstops
   std::string model_src;
   std::map<int, UserKernels> kernels;
 };
@@ -380,6 +382,10 @@ static int check_user_spec(const mideq_user_model* m, std::string* why) {
     *why = "bad nout or null output";
     return MIDEQ_E_ARG;
   }
+  if (m->with_tstops != 0 && m->with_tstops != 1) {
+    *why = "with_tstops must be 0 or 1";
+    return MIDEQ_E_ARG;
+  }
   if (m->save_everystep != 0 && m->save_everystep != 1) {
     *why = "save_everystep must be 0 or 1";
     return MIDEQ_E_ARG;
@@ -445,6 +451,7 @@ static std::string user_model_source(const mideq_user_model* m) {
                   std::to_string(m->n) + ", NP = " + std::to_string(np) + ";\n";
   s += std::string("  static constexpr bool TDEP = ") + (tdep ? "true" : "false") + ";\n";
   if (m->save_everystep) s += "  static constexpr bool EVERYSTEP = true;\n";
+  if (m->with_tstops) s += "  static constexpr bool TSTOPS = true;\n";
   s += std::string("  template <class T>\n  __device__ static void f(T* du, const T* u, const T* p") +
        (tdep ? ", T t" : "") + ") {\n";
   // the statements in a block of their own: a local t shadows the time
@@ -685,6 +692,7 @@ extern "C" int mideq_model_register(mideq_ctx* c, const mideq_user_model* m,
   um.ncb = m->ncallbacks;
   um.nout = m->nout;
   um.everystep = m->save_everystep != 0;
+  um.tstops = m->with_tstops != 0;
   um.model_src = user_model_source(m);
   c->user_models.push_back(um);
   UserModel* p = &c->user_models.back();
@@ -719,6 +727,14 @@ static int model_dims(mideq_ctx* c, int model, int* n, int* np) {
 
 static int validate(mideq_ctx* c, const mideq_problem* P, const mideq_options* O) {
   if (!P || !O) return set_err(c, MIDEQ_E_ARG, "null problem or options");
+  if (O->ntstops < 0 || (O->ntstops > 0 && !O->tstops))
+    return set_err(c, MIDEQ_E_ARG, "ntstops < 0, or tstops NULL with ntstops > 0");
+  if (O->ntstops > 0) {
+    const UserModel* um = user_model(c, P->model);
+    if (!um || !um->tstops)
+      return set_err(c, MIDEQ_E_UNSUPPORTED,
+                     "tstops need a user model registered with with_tstops = 1");
+  }
   int n, np;
   if (model_dims(c, P->model, &n, &np) != 0)
     return set_err(c, MIDEQ_E_MODEL, "unknown model id %d", P->model);
@@ -1035,6 +1051,8 @@ static int run_solver(mideq_ctx* c, const mideq_problem* P, const mideq_options*
   a.ev_u = out->ev_u;
   a.ev_n = out->ev_n;
   a.ev_cap = out->ev_cap;
+  a.tstops = O->ntstops > 0 ? O->tstops : nullptr;
+  a.nts = O->ntstops > 0 ? O->ntstops : 0;
   a.out = out->out;
   a.queue = Sc.queue;
   a.order = d_order;
@@ -1301,6 +1319,9 @@ extern "C" int mideq_ensemble_solve(mideq_ctx* c, const mideq_problem* P,
   if (rc) return rc;
   if ((rc = validate_outputs(c, P, out))) return rc;
   if (O->ns > 0 && (rc = check_saveat_host(c, O->saveat, O->ns))) return rc;
+  for (int k = 1; k < O->ntstops; k++)
+    if (!(O->tstops[k] >= O->tstops[k - 1]))
+      return set_err(c, MIDEQ_E_ARG, "tstops must be ascending");
   HIP_TRY(c, hipSetDevice(c->device));
   hipStream_t s = c->stream;
   const int64_t N = P->ntraj;
@@ -1346,6 +1367,12 @@ extern "C" int mideq_ensemble_solve(mideq_ctx* c, const mideq_problem* P,
     if ((rc = ensure_buf(c, BUF_SAVEAT, sizeof(double) * O->ns, &d))) return rc;
     HIP_TRY(c, hipMemcpyAsync(d, O->saveat, sizeof(double) * O->ns, hipMemcpyHostToDevice, s));
     dO.saveat = (const double*)d;
+  }
+  if (O->ntstops > 0) {
+    if ((rc = ensure_buf(c, BUF_TST, sizeof(double) * O->ntstops, &d))) return rc;
+    HIP_TRY(c, hipMemcpyAsync(d, O->tstops, sizeof(double) * O->ntstops,
+                              hipMemcpyHostToDevice, s));
+    dO.tstops = (const double*)d;
   }
   if (O->dt0) {
     if ((rc = ensure_buf(c, BUF_DT0, sizeof(double) * N, &d))) return rc;

@@ -65,6 +65,7 @@ struct MideqOptions
     order::Ptr{Int64}
     reduction::Int32; save_layout::Int32
     iter_cap::Ptr{Int32}
+    tstops::Ptr{Float64}; ntstops::Int32; reserved::Int32
 end
 
 struct MideqOutputs
@@ -98,7 +99,7 @@ const EVERYSTEP = IdDict{Any, Bool}()
 struct MideqUserModel
     n::Int32; np::Int32; rhs::Ptr{UInt8}; jac::Ptr{UInt8}
     ncallbacks::Int32; save_everystep::Int32; callbacks::Ptr{MideqCallback}
-    nout::Int32; reserved2::Int32; output::Ptr{UInt8}
+    nout::Int32; with_tstops::Int32; output::Ptr{UInt8}
 end
 
 """
@@ -313,7 +314,7 @@ function _device_solve(probs, alg, ealg::EnsembleMI355X, f, sa; abstol, reltol, 
                              ns > 0 ? pointer(sa) : C_NULL, ns,
                              ealg.save_dtype === Float32 ? 1 : 0, C_NULL, C_NULL, 1,
                              ealg.schedule === :lane ? 1 : ealg.schedule === :persistent ? 2 : 0,
-                             0, 0, lanes, 0, 0, 0, C_NULL, 0, 0, C_NULL)
+                             0, 0, lanes, 0, 0, 0, C_NULL, 0, 0, C_NULL, C_NULL, 0, 0)
         out_c = MideqOutputs(ns > 0 ? pointer(US) : C_NULL, pointer(UF), pointer(TFo),
                              pointer(RC), pointer(NA), pointer(NR), pointer(NF), pointer(NS),
                              C_NULL, C_NULL, C_NULL, C_NULL, C_NULL,
@@ -353,7 +354,7 @@ function solve_resident!(f, alg, ealg::EnsembleMI355X, U0, P, T0, TF, out;
                          abstol, reltol, dt, dtmin, dtmax, maxiters, ptr(saveat), ns,
                          ealg.save_dtype === Float32 ? 1 : 0, C_NULL, C_NULL, 1,
                          ealg.schedule === :lane ? 1 : ealg.schedule === :persistent ? 2 : 0,
-                         0, 0, 0, 0, 0, 0, C_NULL, 0, 0, C_NULL)
+                         0, 0, 0, 0, 0, 0, C_NULL, 0, 0, C_NULL, C_NULL, 0, 0)
     out_c = MideqOutputs(ns > 0 ? ptr(out.US) : C_NULL, ptr(out.UF), ptr(out.TFo),
                          ptr(out.RC), ptr(out.NA), ptr(out.NR), ptr(out.NF), ptr(out.NS),
                          C_NULL, C_NULL, C_NULL, C_NULL, C_NULL, C_NULL, C_NULL, C_NULL, 0, 0)

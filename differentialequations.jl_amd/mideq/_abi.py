@@ -44,7 +44,7 @@ class UserModelSpec(C.Structure):
     _fields_ = [("n", C.c_int32), ("np", C.c_int32), ("rhs", C.c_char_p), ("jac", C.c_char_p),
                 ("ncallbacks", C.c_int32), ("save_everystep", C.c_int32),
                 ("callbacks", C.POINTER(CallbackSpec)), ("nout", C.c_int32),
-                ("reserved2", C.c_int32), ("output", C.c_char_p)]
+                ("with_tstops", C.c_int32), ("output", C.c_char_p)]
 
 
 USER_MODEL_BASE = 1000
@@ -61,7 +61,8 @@ class Options(C.Structure):
                 ("schedule", C.c_int32), ("block", C.c_int32), ("blocks_per_cu", C.c_int32),
                 ("lanes", C.c_int32), ("graph", C.c_int32), ("controller", C.c_int32),
                 ("auto_passes", C.c_int32), ("order", C.c_void_p),
-                ("reduction", C.c_int32), ("save_layout", C.c_int32), ("iter_cap", C.c_void_p)]
+                ("reduction", C.c_int32), ("save_layout", C.c_int32), ("iter_cap", C.c_void_p),
+                ("tstops", C.c_void_p), ("ntstops", C.c_int32), ("reserved", C.c_int32)]
 
 
 class Outputs(C.Structure):

@@ -179,7 +179,7 @@ class UserModel:
     parameters still takes one (unused) parameter row."""
 
     def __init__(self, n, np_, rhs, jac=None, name="user", callbacks=(), output=None,
-                 save_everystep=False):
+                 save_everystep=False, with_tstops=False):
         if not (1 <= n <= _abi.USER_NMAX) or not (0 <= np_ <= _abi.USER_NPMAX):
             raise ValueError(f"user model n={n} np={np_} outside 1..{_abi.USER_NMAX} / "
                              f"0..{_abi.USER_NPMAX}")
@@ -194,6 +194,7 @@ class UserModel:
             raise ValueError(f"at most {_abi.USER_NCBMAX} callbacks")
         self.output = output
         self.save_everystep = bool(save_everystep)
+        self.with_tstops = bool(with_tstops)
         if output is not None and not (1 <= output.nout <= _abi.USER_NOUTMAX):
             raise ValueError(f"nout must be 1..{_abi.USER_NOUTMAX}")
         self._ids = {}
@@ -206,21 +207,24 @@ class UserModel:
         sp = _abi.UserModelSpec(self.n, self.np, self.rhs.encode(),
                                 None if self.jac_src is None else self.jac_src.encode(),
                                 len(cbs), int(self.save_everystep), arr if cbs else None,
-                                0 if out is None else int(out.nout), 0,
+                                0 if out is None else int(out.nout), int(self.with_tstops),
                                 None if out is None else out.output.encode())
         sp._keep = arr
         return sp
 
-    def with_callbacks(self, callbacks=(), output=None, save_everystep=False):
-        """This model with `callbacks` (and an on-device output, and
-        save_everystep) compiled in (cached)."""
+    def with_callbacks(self, callbacks=(), output=None, save_everystep=False,
+                       with_tstops=False):
+        """This model with `callbacks` (and an on-device output, save_everystep
+        and tstops) compiled in (cached)."""
         cbs = CallbackSet(callbacks) if callbacks else ()
         save_everystep = bool(save_everystep) or self.save_everystep
-        key = (cbs, output, save_everystep)
+        with_tstops = bool(with_tstops) or self.with_tstops
+        key = (cbs, output, save_everystep, with_tstops)
         if key not in self._with:
             self._with[key] = UserModel(self.n, self.np, self.rhs, self.jac_src,
                                         name=self.name, callbacks=cbs, output=output,
-                                        save_everystep=save_everystep)
+                                        save_everystep=save_everystep,
+                                        with_tstops=with_tstops)
         return self._with[key]
 
     def check(self):
@@ -776,7 +780,7 @@ def device_solve(f, u0, p, t0, tf, nvec, alg, ealg, saveat=None, abstol=1e-6, re
                  dt=0.0, dtmin=0.0, dtmax=0.0, maxiters=0, dt0=None, alg_hint=None,
                  order=None, block=0, blocks_per_cu=0, callback=None, output=None,
                  reduction=None, keep_states=True, event_cap=16, stream=None,
-                 save_everystep=False):
+                 save_everystep=False, tstops=None):
     """One call of mideq_ensemble_solve on host arrays -> EnsembleArrays.
     callback: ContinuousCallback / DiscreteCallback / CallbackSet, compiled
     into the right-hand side (a UserModel, or a built-in model's source).
@@ -785,7 +789,8 @@ def device_solve(f, u0, p, t0, tf, nvec, alg, ealg, saveat=None, abstol=1e-6, re
     keep_states=False nothing per trajectory comes back).  save_everystep:
     every accepted step's end point is kept too (one-lane models; built-in
     ones through their source), with the event points, event_cap points per
-    trajectory at most (the count goes on past it).
+    trajectory at most (the count goes on past it).  tstops: times every
+    trajectory's steps land on (upstream's kwarg; one-lane models).
 
     Device-resident form: when u0 is a torch tensor in GPU memory the call
     goes to mideq_ensemble_solve_device instead.  The other inputs are used
@@ -795,8 +800,13 @@ def device_solve(f, u0, p, t0, tf, nvec, alg, ealg, saveat=None, abstol=1e-6, re
     torch.cuda.Stream; default the current stream) -- synchronize before
     reading.  The result's saved(i) and solution(i) are views into those
     tensors (no copies)."""
-    if callback or output is not None or save_everystep:
-        f = user_model_of(f).with_callbacks(callback or (), output, save_everystep)
+    if tstops is not None:
+        tstops = np.sort(np.asarray(tstops, dtype=np.float64).ravel())
+        if len(tstops) == 0:
+            tstops = None
+    if callback or output is not None or save_everystep or tstops is not None:
+        f = user_model_of(f).with_callbacks(callback or (), output, save_everystep,
+                                            tstops is not None)
     if isinstance(reduction, DeviceReduction):
         reduction = reduction.op
     dev = _resident_device(u0)
@@ -883,6 +893,10 @@ def device_solve(f, u0, p, t0, tf, nvec, alg, ealg, saveat=None, abstol=1e-6, re
         O.order = _ptr(order)
     if reduction is not None:
         O.reduction = _abi.REDUCTION[reduction]
+    if tstops is not None:
+        tstops = mem.arr(tstops, np.float64)
+        O.tstops = _ptr(tstops)
+        O.ntstops = len(tstops)
     if keep_states:
         res = dict(u_final=mem.full((n, N), 0.0, np.float64), t_final=mem.empty(N, np.float64),
                    retcode=mem.empty(N, np.int8), naccept=mem.empty(N, np.int32),

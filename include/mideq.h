@@ -167,6 +167,12 @@ typedef struct {
                               per trajectory (e.g. a multiple of the predicted cost; 0
                               sends it to AutoSwitch before its first step, starting in
                               Tsit5), or NULL for none */
+  const double *tstops; /* [ntstops] ascending times the steps of every trajectory land on
+                           (OrdinaryDiffEq's tstops; the ones strictly inside a span are
+                           used), or NULL; needs a user model registered with
+                           with_tstops = 1 (one-lane kernels) */
+  int32_t ntstops;
+  int32_t reserved;
 } mideq_options;
 
 /* Results; any pointer may be NULL. */
@@ -325,7 +331,8 @@ typedef struct {
                                         0: off */
   const mideq_callback *callbacks;   /* [ncallbacks] or NULL */
   int32_t nout;       /* on-device output_func: values per trajectory, 0..MIDEQ_USER_NOUTMAX */
-  int32_t reserved2;
+  int32_t with_tstops; /* 1: the kernels honour mideq_options.tstops (steps land on them,
+                          as on tf); 0: tstops refused */
   const char *output; /* statements assigning double out[0..nout-1] from the final state
                          u[], p[], the final time t and the retcode rc; replaces
                          output_func(sol, i) -> (out, false) of EnsembleProblem

@@ -1283,6 +1283,14 @@ static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
    * ascending order, are then visited from the last one down */
   const double tdir = tf < t0 ? -1.0 : 1.0;
   const int back = tf < t0;
+  /* tstops: index of the next one ahead (upstream keeps tf in the same
+   * queue; tstops at or behind t0 are dropped at the start) */
+  const int nts = o->tstops ? o->ntstops : 0;
+  int its = back ? nts - 1 : 0;
+#define TSTOP_POP                                                              \
+  while (its >= 0 && its < nts && tdir * (t - o->tstops[its]) >= 0.0) its += back ? -1 : 1
+#define TSTOP_NEXT                                                             \
+  ((its >= 0 && its < nts && tdir * (tf - o->tstops[its]) > 0.0) ? o->tstops[its] : tf)
   const double *sa = o->saveat;
   int ns = sa ? o->ns : 0;
   int isave = back ? ns - 1 : 0;
@@ -1375,7 +1383,12 @@ static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
       rc = ORACLE_RC_DTLESSTHANMIN;
       break;
     }
-    dt = fmin(dt, back ? t - tf : tf - t);
+    TSTOP_POP; /* handle_tstop!: the ones the last step reached */
+    const double tstop = TSTOP_NEXT;
+    if (nts > 0)
+      dt = fmin(dt, tdir * (tstop - t)); /* modify_dt_for_tstops! */
+    else
+      dt = fmin(dt, back ? t - tf : tf - t);
     const double sdt = back ? -dt : dt; /* the signed step */
 
     /* perform_step! */
@@ -1465,7 +1478,8 @@ static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
         ttmp = t + y;
         tcomp = (ttmp - t) - y;
       }
-      if (fabs(ttmp - tf) < 100.0 * jl_eps(fmax(t, tf))) ttmp = tf;
+      /* fixed_t_for_floatingpoint_error!: onto the first tstop (tf included) */
+      if (fabs(ttmp - tstop) < 100.0 * jl_eps(fmax(t, tstop))) ttmp = tstop;
       t = ttmp;
       /* the state the step ends in: u1, or the dense output at an event */
       double uend[NMAX];
@@ -1636,6 +1650,8 @@ static int solve_one(const mdl_t *m, const double *u0, double t0, double tf,
       for (int c = 0; c < n; c++) save[((long)isave * save_rows + c) * save_stride] = NAN;
 #undef SAVE_LEFT
 #undef SAVE_NEXT
+#undef TSTOP_POP
+#undef TSTOP_NEXT
   /* event-point slots past this trajectory's points hold NaN */
   if (o->ev_t)
     for (long long k = res->nevsave; k < o->ev_cap; k++) {

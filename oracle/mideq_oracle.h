@@ -135,6 +135,11 @@ typedef struct {
   int32_t *ev_n;
   int32_t ev_cap;
   int32_t reserved;
+  /* tstops (OrdinaryDiffEq's kwarg), ascending, or NULL: the steps land on
+   * the ones strictly inside the span as on tf */
+  const double *tstops;
+  int32_t ntstops;
+  int32_t reserved2;
 } oracle_options;
 
 typedef struct {

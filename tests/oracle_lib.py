@@ -48,6 +48,9 @@ class Options(C.Structure):
         ("ev_n", C.POINTER(C.c_int32)),
         ("ev_cap", C.c_int32),
         ("reserved", C.c_int32),
+        ("tstops", _dp),
+        ("ntstops", C.c_int32),
+        ("reserved2", C.c_int32),
     ]
 
 
@@ -122,9 +125,13 @@ class Oracle:
     @staticmethod
     def options(alg="tsit5", precision="fp64", abstol=1e-6, reltol=1e-3, dt=0.0, dtmin=0.0,
                 dtmax=0.0, maxiters=0, saveat=None, qsteady_default=1, norm_lanes=64,
-                controller=0, dt0=None, alg_hint=None, save_layout=0):
+                controller=0, dt0=None, alg_hint=None, save_layout=0, tstops=None):
         o = Options()
         o.save_layout = save_layout
+        if tstops is not None:
+            o._tstops_keep = np.sort(np.ascontiguousarray(tstops, dtype=np.float64))
+            o.tstops = _ptr(o._tstops_keep)
+            o.ntstops = len(o._tstops_keep)
         o.alg = ALG[alg]
         o.precision = PREC[precision]
         o.abstol = abstol

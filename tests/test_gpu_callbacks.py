@@ -315,3 +315,34 @@ def test_save_everystep_rosenbrock(oracle, alg, precision):
     # FP32 stages: the sum drifts by ~2e-6 (the oracle's mixed mode too)
     np.testing.assert_allclose(g.ev_u.sum(axis=2)[stored], 1.0,
                                atol=1e-5 if precision == "mixed" else 1e-12)
+
+
+@pytest.mark.parametrize("alg", ["tsit5", "rodas5"])
+def test_tstops_dose_bitexact(oracle, alg):
+    """tstops with a dosing DiscreteCallback at t == 0.5 (steps land on the
+    tstop, the dose applies there): bit-exact against the oracle, the step
+    ends (save_everystep) hold the tstops, and the final states follow
+    u(1) = (exp(-k/2) + 1) exp(-k/2)."""
+    N = 512
+    say(f"tstops dose {alg} N={N}")
+    rhs = "du[0] = -p[0] * u[0];"
+    dose = DiscreteCallback("t == 0.5", affect="u[0] = u[0] + 1.0;")
+    rng = np.random.default_rng(119)
+    k = rng.uniform(0.2, 5.0, N)
+    ts = np.array([0.25, 0.5])
+    kw = dict(abstol=1e-10, reltol=1e-10, tstops=ts, save_everystep=True, event_cap=1024)
+    g = run(UserModel(1, 1, rhs), np.ones((1, N)), k[None, :], 0.0, 1.0, alg=alg,
+            callback=dose, **kw)
+    umh.install(oracle, 1, 1, rhs, callbacks=(dose,), save_everystep=True)
+    c = cpu(oracle, "user", np.ones((1, N)), k[None, :], 0.0, 1.0, alg=alg, tstops=ts,
+            event_cap=1024, abstol=1e-10, reltol=1e-10)
+    _check(g, c)
+    np.testing.assert_array_equal(g.ev_n, c["ev_n"])
+    np.testing.assert_array_equal(g.ev_t, c["ev_t"])
+    np.testing.assert_array_equal(g.ev_u, c["ev_u"])
+    assert (g.nevent == 1).all() and (g.t_event == 0.5).all()
+    for i in (0, 200, N - 1):
+        t = g.ev_t[i, :int(g.ev_n[i])]
+        assert 0.25 in t and 0.5 in t
+    np.testing.assert_allclose(g.u_final[0], (np.exp(-k / 2) + 1.0) * np.exp(-k / 2),
+                               rtol=1e-7)

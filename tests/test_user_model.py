@@ -345,3 +345,30 @@ def test_oracle_continuous_callback_without_affect_saves_before_only(oracle):
     umh.install(oracle, 1, 1, rhs, callbacks=(dcb,))
     r2 = oracle.ensemble("user", u0, p, 0.0, 1.0, abstol=1e-10, reltol=1e-10, event_cap=64)
     np.testing.assert_array_equal(r2["ev_n"], 2 * r2["nevent"])
+
+
+def test_oracle_tstops_land_and_dose(oracle):
+    """tstops: the steps land exactly on the ones inside the span (kept as
+    step ends with save_everystep; ones outside it are ignored), and a
+    discrete callback at t == 0.5 with tstops [0.5] -- a dose -- gives the
+    closed form u(1) = (exp(-k/2) + 1) exp(-k/2) of u' = -k u."""
+    from mideq import DiscreteCallback
+    rhs = "du[0] = -p[0] * u[0];"
+    k = np.array([0.5, 1.0, 3.0])
+    u0, p = np.ones((1, 3)), k[None, :]
+    umh.install(oracle, 1, 1, rhs, save_everystep=True)
+    ts = np.array([-1.0, 0.3, 0.77, 5.0])
+    r = oracle.ensemble("user", u0, p, 0.0, 1.0, abstol=1e-10, reltol=1e-10, event_cap=512,
+                        tstops=ts)
+    for i in range(3):
+        t = r["ev_t"][i, :int(r["ev_n"][i])]
+        assert 0.3 in t and 0.77 in t and t[-1] == 1.0
+    np.testing.assert_allclose(r["u"][0], np.exp(-k), rtol=1e-8)
+    dose = DiscreteCallback("t == 0.5", affect="u[0] = u[0] + 1.0;")
+    umh.install(oracle, 1, 1, rhs, callbacks=(dose,))
+    r2 = oracle.ensemble("user", u0, p, 0.0, 1.0, abstol=1e-10, reltol=1e-10,
+                         tstops=np.array([0.5]))
+    np.testing.assert_array_equal(r2["nevent"], [1, 1, 1])
+    np.testing.assert_allclose(r2["u"][0], (np.exp(-k / 2) + 1.0) * np.exp(-k / 2), rtol=1e-8)
+    r3 = oracle.ensemble("user", u0, p, 0.0, 1.0, abstol=1e-10, reltol=1e-10)
+    assert (r3["nevent"] == 0).all()  # without the tstop no step ends at 0.5
