@@ -127,13 +127,14 @@ DeviceCallback(kind::Symbol, condition::AbstractString; kw...) =
 
 """
     register_model!(f!, n, np, rhs; jac = nothing, callbacks = [], output = nothing,
-                    nout = 0, save_everystep = false, device = 0)
+                    nout = 0, save_everystep = false, with_tstops = false, device = 0)
 
 Registers `f!` with a device right-hand side written as C++ statements for
 u' = f(u, p, t): assign `du[0..n-1]` from `u[i]`, `p[j]` (0-based) and the
 time `t` (naming `t` makes the model non-autonomous); the number type is `T`, constants `2.0` or `T(2)`, functions
 `sqrt exp log sin cos tanh fabs pow(x, c)`.  `jac` assigns `J[i][j]`;
 without it the Rosenbrock methods differentiate `rhs` in forward mode.
+`with_tstops = true` lets `solve(...; tstops)` land steps on those times.
 `save_everystep = true` compiles in upstream's save_everystep: each
 solution then holds every accepted step's end (up to `event_cap` per
 trajectory).
@@ -148,7 +149,8 @@ function register_model!(f, n::Integer, np::Integer, rhs::AbstractString;
                          jac::Union{Nothing, AbstractString} = nothing,
                          callbacks::AbstractVector{DeviceCallback} = DeviceCallback[],
                          output::Union{Nothing, AbstractString} = nothing, nout::Integer = 0,
-                         save_everystep::Bool = false, device::Int = 0)
+                         save_everystep::Bool = false, with_tstops::Bool = false,
+                         device::Int = 0)
     ctx = context(device)
     id = Ref{Int32}(0)
     r = String(rhs)
@@ -166,7 +168,8 @@ function register_model!(f, n::Integer, np::Integer, rhs::AbstractString;
             m = MideqUserModel(n, np, pointer(r), j === nothing ? C_NULL : pointer(j),
                                length(cbs), Int32(save_everystep),
                                isempty(cbs) ? C_NULL : pointer(cbs),
-                               o === nothing ? 0 : nout, 0, o === nothing ? C_NULL : pointer(o))
+                               o === nothing ? 0 : nout, Int32(with_tstops),
+                               o === nothing ? C_NULL : pointer(o))
             ccall((:mideq_model_register, LIB[]), Cint,
                   (Ptr{Cvoid}, Ref{MideqUserModel}, Ref{Int32}), ctx.h, m, id)
         end
@@ -239,7 +242,7 @@ function __solve(ens::SciMLBase.AbstractEnsembleProblem, alg, ealg::EnsembleMI35
                  trajectories, batch_size = trajectories, saveat = nothing, abstol = 1e-6,
                  reltol = 1e-3, dt = 0.0, dtmin = 0.0, dtmax = 0.0, maxiters = 100_000,
                  dense = saveat === nothing, save_start = nothing, save_end = nothing,
-                 kwargs...)
+                 tstops = nothing, kwargs...)
     f = ens.prob.f.f
     if !haskey(MODEL_FOR, f)
         @warn "EnsembleMI355X: unregistered right-hand side, falling back to EnsembleThreads"
@@ -256,7 +259,7 @@ function __solve(ens::SciMLBase.AbstractEnsembleProblem, alg, ealg::EnsembleMI35
     for first in 1:batch_size:N
         II = first:min(first + batch_size - 1, N)
         batch = _solve_batch(ens, alg, ealg, f, II; saveat, abstol, reltol, dt, dtmin,
-                             dtmax, maxiters, dense, save_start, save_end)
+                             dtmax, maxiters, dense, save_start, save_end, tstops)
         u, converged = ens.reduction(u, batch, II)
         converged && break
     end
@@ -276,7 +279,7 @@ struct MideqStats
 end
 
 function _device_solve(probs, alg, ealg::EnsembleMI355X, f, sa; abstol, reltol, dt, dtmin,
-                       dtmax, maxiters, lanes = 0, event_cap = 0)
+                       dtmax, maxiters, lanes = 0, event_cap = 0, tstops = Float64[])
     M = length(probs)
     n = maximum(length(pr.u0) for pr in probs)
     np = max(length(probs[1].p), 1)   # a user model without parameters reads one row
@@ -305,7 +308,7 @@ function _device_solve(probs, alg, ealg::EnsembleMI355X, f, sa; abstol, reltol, 
     EVN = ev ? zeros(Int32, M) : zeros(Int32, 0)
     st = Ref(MideqStats(0, 0, 0, 0, 0, 0, 0, 0))
     ctx = context(ealg.device)
-    GC.@preserve U0 P T0 TF nvec sa US UF TFo RC NA NR NF NS EVT EVU EVN begin
+    GC.@preserve U0 P T0 TF nvec sa US UF TFo RC NA NR NF NS EVT EVU EVN tstops begin
         prob_c = MideqProblem(MODEL_FOR[f], n, np, 0, M, pointer(U0), pointer(P),
                               pointer(T0), pointer(TF), T0[1], TF[1],
                               MODEL_FOR[f] == 4 ? pointer(nvec) : C_NULL)
@@ -314,7 +317,8 @@ function _device_solve(probs, alg, ealg::EnsembleMI355X, f, sa; abstol, reltol, 
                              ns > 0 ? pointer(sa) : C_NULL, ns,
                              ealg.save_dtype === Float32 ? 1 : 0, C_NULL, C_NULL, 1,
                              ealg.schedule === :lane ? 1 : ealg.schedule === :persistent ? 2 : 0,
-                             0, 0, lanes, 0, 0, 0, C_NULL, 0, 0, C_NULL, C_NULL, 0, 0)
+                             0, 0, lanes, 0, 0, 0, C_NULL, 0, 0, C_NULL,
+                             isempty(tstops) ? C_NULL : pointer(tstops), length(tstops), 0)
         out_c = MideqOutputs(ns > 0 ? pointer(US) : C_NULL, pointer(UF), pointer(TFo),
                              pointer(RC), pointer(NA), pointer(NR), pointer(NF), pointer(NS),
                              C_NULL, C_NULL, C_NULL, C_NULL, C_NULL,
@@ -400,13 +404,14 @@ end
 # every solution has dense output (sol(t)), as upstream's default.
 function _solve_batch(ens, alg, ealg::EnsembleMI355X, f, II; saveat, abstol, reltol, dt,
                       dtmin, dtmax, maxiters, dense = saveat === nothing,
-                      save_start = nothing, save_end = nothing)
+                      save_start = nothing, save_end = nothing, tstops = nothing)
     outputs = Dict{Int,Any}()
     pending = collect(II)
     repeat = 1
     sa = _saveat_vector(saveat, ens.prob.tspan, save_start, save_end)
     ns = length(sa)
-    tol = (; abstol, reltol, dt, dtmin, dtmax, maxiters)
+    ts = tstops === nothing ? Float64[] : sort(Float64.(collect(tstops)))
+    tol = (; abstol, reltol, dt, dtmin, dtmax, maxiters, tstops = ts)
     while !isempty(pending)
         probs = [ens.prob_func(deepcopy(ens.prob), i, repeat) for i in pending]
         r = _device_solve(probs, alg, ealg, f, sa; tol...)

@@ -964,8 +964,9 @@ def _dense_interp(packed, k, i, alg, ealg, lanes, opts, kw):
                         save_dtype="float64", order=None)
         extra = {key: np.asarray(kw[key])[[k]] for key in ("dt0", "alg_hint")
                  if kw.get(key) is not None}
-        if "callback" in kw:
-            extra["callback"] = kw["callback"]
+        for key in ("callback", "tstops"):  # the same steps: the same events and tstops
+            if kw.get(key) is not None:
+                extra[key] = kw[key]
         ts = np.ascontiguousarray(ts, dtype=np.float64).ravel()
         order = np.argsort(ts, kind="stable")
         sa = np.ascontiguousarray(ts[order])

@@ -283,3 +283,16 @@ def test_save_start_and_save_end():
     s2 = solve(prob, Tsit5(), save_end=False, abstol=1e-8, reltol=1e-8)
     assert list(s1.t) == [1.0] and list(s2.t) == [0.0] and s2.u[0][0] == 0.5
     assert s1.u[0][0] == full.u[-1][0]
+
+
+def test_dense_output_with_tstops():
+    """sol(t) of a solve with tstops solves again with the same tstops, so
+    it matches saveat values of that solve bit for bit."""
+    say("dense output with tstops")
+    f = mideq.UserModel(1, 1, "du[0] = -p[0] * u[0];")
+    prob = ODEProblem(f, [1.0], (0.0, 1.0), [2.0])
+    ts = [0.31, 0.62]
+    sol = solve(prob, Tsit5(), tstops=ts, abstol=1e-9, reltol=1e-9)
+    ref = solve(prob, Tsit5(), tstops=ts, saveat=[0.1, 0.45, 0.9], abstol=1e-9, reltol=1e-9)
+    np.testing.assert_array_equal(np.array([sol(t)[0] for t in (0.1, 0.45, 0.9)]),
+                                  np.array([u[0] for u in ref.u]))
